@@ -1,0 +1,149 @@
+"""Benchmark of the explicit 3D heat step on one MI355X.
+
+Workload: the heterogeneous heat block, 512^3 cells of f64, per-cell conductivity lognormal(0,
+0.5) W/m/K with harmonic-mean faces, rho c = 4e6 J/m^3/K, dx = 1 mm, eight seeded Gaussian hot
+spots, stepped at 0.9 of its stability limit with the conservation audit after every step.  A
+"step" is one sweep over the 512^3 cells plus that audit; inputs are resident in HBM before the
+timed region starts.
+
+Prints one JSON line: GCUPS (cell updates per second / 1e9) over the timed steps, the roofline of
+the dominant kernel (the sweep) from HIP events around each of its launches, and the CPU oracle
+timed on this host's cores over a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+N = 512
+BYTES_PER_CELL = 24          # T read, k read, T written (f64 each)
+HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
+METRIC = "GCUPS + %HBM roofline, FP64 heat 512³; surrogate vs stencil speedup @1e-3 L2"
+
+
+def _traffic_from_profiles():
+    """HBM bytes per sweep launch, from the committed rocprofv3 counter summary, if present."""
+    p = ROOT / "profiles" / "heat_sweep_512_counters.json"
+    if not p.exists():
+        return None
+    try:
+        return json.loads(p.read_text()).get("hbm_bytes_per_launch")
+    except (ValueError, OSError):
+        return None
+
+
+def cpu_baseline(threads: int, budget_s: float = 12.0):
+    """The oracle's sweep (the reference's arithmetic, OpenMP over z-planes) on the heterogeneous
+    block at 512^3: whole steps until about `budget_s` seconds of stepping have run."""
+    import numpy as np
+    from oracle import oracle as o
+    from pantometry_amd import workloads as w
+    k, t = o.heterogeneous_fields((N, N, N))
+    b = o.OracleSolid3D("block", w.RHO, 1.0, w.CP, (N, N, N), w.DX, 300.0)
+    b.with_conductivity(k)
+    b.set_cells(t)
+    del k, t
+    dt = 0.9 * b.max_stable_dt()
+    b.step(dt, threads_=threads)  # warm-up
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        b.step(dt, threads_=threads)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or steps >= 50:
+            break
+    value = N ** 3 * steps / el / 1e9
+    return {"value": round(value, 4), "unit": "GCUPS", "cores": threads, "kind": "port",
+            "sample": f"{steps} steps of the 512^3 heterogeneous block ({el:.1f} s), oracle "
+                      f"sweep with OpenMP over z-planes, gcc -O2 -ffp-contract=off"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--tile-y", type=int, default=0)
+    ap.add_argument("--z-chunk", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: the heat block is not sharded across GPUs "
+              "(see DESIGN.md for how it would be); run with --gpus 1", file=sys.stderr)
+        return 2
+
+    import pantometry_amd as pm
+    from pantometry_amd import workloads as w
+
+    block, dt = w.heterogeneous_block(N)
+    if args.tile_y or args.z_chunk:
+        block.set_tile(args.tile_y, args.z_chunk)
+    if args.warmup:
+        block.advance(dt, args.warmup, tolerance=1e-12, want_totals=False)
+    block.set_profiling(True)
+    t0 = time.perf_counter()
+    totals, loop_ms = block.advance(dt, args.steps, tolerance=1e-12)
+    wall = time.perf_counter() - t0
+    sweep_ms, audit_ms = block.kernel_times()
+    block.set_profiling(False)
+    # The same steps without per-launch events, for the headline number.
+    t0 = time.perf_counter()
+    _, loop_ms_plain = block.advance(dt, args.steps, tolerance=1e-12, want_totals=False)
+    wall_plain = time.perf_counter() - t0
+
+    cells = N ** 3
+    ms_per_step = wall_plain * 1e3 / args.steps
+    gcups = cells * args.steps / wall_plain / 1e9
+    achieved = BYTES_PER_CELL * cells / (sweep_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(gcups, 3),
+        "unit": "GCUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded lognormal(0, 0.5) conductivity and 8 Gaussian hot spots",
+        "config": {
+            "workload": "heat 512^3 f64, heterogeneous conductivity, harmonic faces, "
+                        "audit every step (0.9 of the stability limit)",
+            "cells": cells, "dt": dt, "tile_y": args.tile_y or 8, "z_chunk": args.z_chunk or 32,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": _traffic_from_profiles(),
+            "kernel": "heat_sweep<1,0,8> (per-cell k, uniform C)",
+            "kernel_ms": round(sweep_ms, 5),
+            "audit_kernel_ms": round(audit_ms, 5),
+            "bytes_per_cell": BYTES_PER_CELL,
+        },
+        "device_loop_ms_per_step": round(loop_ms_plain / args.steps, 5),
+        "audit_drift": float(abs(totals[-1] - totals[0]) / abs(totals[0])),
+    }
+    del block
+    if not args.no_cpu:
+        threads = max(1, min(16, os.cpu_count() or 1))
+        out["cpu_baseline"] = cpu_baseline(threads)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

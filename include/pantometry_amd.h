@@ -1,0 +1,250 @@
+/*
+ * pantometry_amd — the C ABI of the MI355X (gfx950) backend for pantometry's explicit 3D
+ * grid-stencil path: Solid3D's conduction sweep, Hall's staggered pressure/velocity leapfrog,
+ * the conservation audit that totals heat or acoustic energy after each step, and a batched
+ * ensemble of uneven heat grids.
+ *
+ * Every entry point takes plain pointers and sizes.  Each declaration cites the reference
+ * interface it replaces (paths relative to the pantometry repository).  The Domain trait those
+ * interfaces implement is crates/pantometry-core/src/sim.rs:77-252; its Rust binding over this
+ * header is shown in INTEGRATION.md.
+ *
+ * Conventions
+ *   - Status: PM_OK (0); PM_VIOLATION (1) when a step is refused or an audit breaks, with the
+ *     pm_violation out-parameter filled (Result<(), Violation>, conserved.rs:128-143); a
+ *     PM_ERR_* code for device, argument or state failures, with pm_last_error() describing it.
+ *   - A refused step mutates nothing (solid.rs:1718-1738, hall.rs:409-419).
+ *   - Handles are single-threaded and not re-entrant; every handle owns its device buffers and
+ *     one HIP stream.  Host arrays are caller-owned and copied in and out.
+ *   - There is no CPU fallback: without a usable gfx950 device, create calls fail with
+ *     PM_ERR_DEVICE.
+ */
+#ifndef PANTOMETRY_AMD_H
+#define PANTOMETRY_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    PM_OK = 0,
+    PM_VIOLATION = 1,
+    PM_ERR_DEVICE = 2,
+    PM_ERR_ARGUMENT = 3,
+    PM_ERR_STATE = 4,
+};
+
+/* Violation, crates/pantometry-core/src/conserved.rs:128-143.  `step` names which step of a
+ * batched run broke (-1 for a single call). */
+typedef struct pm_violation {
+    char quantity[64];
+    char site[160];
+    double before;
+    double after;
+    double scale;
+    double tolerance;
+    int64_t step;
+} pm_violation;
+
+/* The thermal part of Substance, crates/pantometry-core/src/substance.rs:43-239: density in
+ * kg/m^3, conductivity in W/m/K, specific heat in J/kg/K.  NaN conductivity or specific heat is a
+ * substance that does not say (Substance::thermal = None). */
+typedef struct pm_substance {
+    double density;
+    double conductivity;
+    double specific_heat;
+} pm_substance;
+
+/* The three entries of Solid3D::ledger, crates/pantometry-thermal/src/solid.rs:1904-1927:
+ * Ledger::new().with(ENERGY, stored).with(ENERGY, lost).with(ENERGY, -supplied). */
+typedef struct pm_books {
+    double stored;
+    double lost;
+    double supplied;
+} pm_books;
+
+/* Message for the last PM_ERR_* status on this thread. */
+const char *pm_last_error(void);
+
+/* Name, compute-unit count and HBM size of the current device; PM_ERR_DEVICE without one. */
+int pm_device_info(char *name, size_t name_len, int *compute_units, size_t *hbm_bytes);
+
+/* ---------------------------------------------------------------------------------------- */
+/* Rng, crates/pantometry-core/src/rng.rs:37-152 (host-side: seeded inputs are generated once */
+/* on the host and uploaded, so libm's exp/ln/cos are never recomputed on the device).       */
+/* ---------------------------------------------------------------------------------------- */
+
+/* Rng::new (rng.rs:49-55) and Rng::for_index (rng.rs:66-68): both return the stream state. */
+uint64_t pm_rng_new(uint64_t seed);
+uint64_t pm_rng_for_index(uint64_t seed, uint64_t index);
+/* Rng::unit (rng.rs:91-93), Rng::range (96-98), Rng::gaussian (146-151), Rng::split (75-78). */
+double pm_rng_unit(uint64_t *state);
+double pm_rng_range(uint64_t *state, double lo, double hi);
+double pm_rng_gaussian(uint64_t *state);
+uint64_t pm_rng_split(uint64_t *state);
+
+/* Seeded workloads.  Gaussian hot spot plus uniform noise (the uniform heat block):
+ * T = base + amp*exp(-r^2/(2 sigma^2)) + range(-noise, noise), r in cells from the centre,
+ * noise from Rng::for_index(noise_seed, cell).  No reference counterpart: it writes the
+ * initial condition a caller would otherwise set with Solid3D::set_temperature (solid.rs:1346). */
+int pm_workload_hot_spot(size_t nx, size_t ny, size_t nz, double base, double amp, double sigma,
+                         double noise, uint64_t noise_seed, double *cells);
+/* Heterogeneous block: k_c = exp(0.5*gaussian) from Rng::for_index(k_seed, c); T = 300 + sum of
+ * `spots` Gaussian hot spots drawn from Rng::for_index(spot_seed, m). */
+int pm_workload_heterogeneous(size_t nx, size_t ny, size_t nz, uint64_t k_seed,
+                              uint64_t spot_seed, int spots, double *conductivity,
+                              double *cells);
+
+/* ---------------------------------------------------------------------------------------- */
+/* Solid3D, crates/pantometry-thermal/src/solid.rs                                          */
+/* ---------------------------------------------------------------------------------------- */
+
+typedef struct pm_solid3d pm_solid3d;
+
+/* Solid3D::new, solid.rs:377-421: counts forced to at least one, every cell at `initial`, the
+ * ledger measured from `initial` (the enthalpy reference). */
+int pm_solid3d_create(const char *name, const pm_substance *substance, size_t nx, size_t ny,
+                      size_t nz, double dx, double initial, pm_solid3d **out);
+void pm_solid3d_destroy(pm_solid3d *solid);
+int pm_solid3d_counts(const pm_solid3d *solid, size_t counts[3]);
+
+/* Solid3D::fill, solid.rs:519-543: give the cells where mask[c] != 0 another substance, then
+ * resolve (solid.rs:588-784).  The caller evaluates the predicate into the mask. */
+int pm_solid3d_fill(pm_solid3d *solid, const pm_substance *substance, const uint8_t *mask);
+/* Per-cell conductivity with the capacity left as it is: arithmetically what one fill per
+ * distinct conductivity produces (faces are the harmonic mean, solid.rs:706-734), without the
+ * O(N^2) material search of fill.  No reference counterpart. */
+int pm_solid3d_set_conductivity(pm_solid3d *solid, const double *conductivity);
+
+/* Temperatures in kelvin: Solid3D::set_temperature over every cell (solid.rs:1346-1361) and
+ * Solid3D::temperature_at over every cell (solid.rs:1323). */
+int pm_solid3d_set_cells(pm_solid3d *solid, const double *cells);
+int pm_solid3d_get_cells(pm_solid3d *solid, double *cells);
+int pm_solid3d_set_temperature(pm_solid3d *solid, size_t i, size_t j, size_t k, double kelvin);
+/* Solid3D::deposit, solid.rs:1368-1375: joules into one cell, counted as absorbed. */
+int pm_solid3d_deposit(pm_solid3d *solid, size_t i, size_t j, size_t k, double joules);
+
+/* Domain::max_stable_dt (solid.rs:1709-1715) and Solid3D::stability_ratio (solid.rs:1435). */
+double pm_solid3d_max_stable_dt(const pm_solid3d *solid);
+double pm_solid3d_stability_ratio(const pm_solid3d *solid, double dt);
+/* The guard at the top of Domain::step (solid.rs:1718-1738), without stepping: PM_VIOLATION
+ * with `v` filled if `dt` would be refused.  A binding calls this before taking its share of
+ * the bus, so a refused step leaves the bus untouched as the reference does. */
+int pm_solid3d_refuses(const pm_solid3d *solid, double dt, pm_violation *v);
+/* Domain::step, solid.rs:1717-1901 (plain branch): `gained` is what the binding took from the
+ * bus with Exchange::take_share(HEAT, dt) (solid.rs:1752); nonzero gains spread as a uniform
+ * rise gained/sum(C) before the sweep. */
+int pm_solid3d_step(pm_solid3d *solid, double dt, double gained, pm_violation *v);
+/* Domain::ledger, solid.rs:1904-1927. */
+int pm_solid3d_ledger(pm_solid3d *solid, pm_books *books);
+/* Device-resident loop of Simulation::advance (sim.rs:947-965) for this one domain with an
+ * empty bus: `steps` steps of `dt`, each followed by the audit of audit_with (conserved.rs:
+ * 323-363) against the previous audited ledger with relative tolerance `tolerance`.  The
+ * ledger total after every step is written to totals[0..steps] when totals is not NULL.  On a
+ * breach the run stops with the state as the breaking step left it, and PM_VIOLATION names the
+ * step.  `elapsed_ms` (optional) receives the device time of the loop, from HIP events on the
+ * handle's stream. */
+int pm_solid3d_advance(pm_solid3d *solid, double dt, uint64_t steps, double tolerance,
+                       double *totals, double *elapsed_ms, pm_violation *v);
+/* Domain::checkpoint / Domain::restore, solid.rs:1935-1950. */
+int pm_solid3d_checkpoint(pm_solid3d *solid);
+int pm_solid3d_restore(pm_solid3d *solid);
+/* Domain::readings, solid.rs:1962-2012: peak, mean and coldest in degrees Celsius, then the
+ * absorbed joules. */
+int pm_solid3d_readings(pm_solid3d *solid, double out[4]);
+/* Solid3D::heat_capacity, solid.rs:1441-1443: the left-fold sum of the cell capacities. */
+double pm_solid3d_heat_capacity(pm_solid3d *solid);
+/* Tile shape of the sweep (x, y, z-chunk); 0 keeps the current one.  No reference counterpart. */
+int pm_solid3d_set_tile(pm_solid3d *solid, int tile_y, int z_chunk);
+/* When `on`, pm_solid3d_advance brackets every sweep launch and every audit launch with HIP
+ * events on the handle's stream; pm_solid3d_kernel_times then gives the mean device duration of
+ * each over the last advance.  No reference counterpart (measurement only). */
+int pm_solid3d_set_profiling(pm_solid3d *solid, int on);
+int pm_solid3d_kernel_times(pm_solid3d *solid, double *sweep_ms, double *audit_ms);
+
+/* ---------------------------------------------------------------------------------------- */
+/* Hall, crates/pantometry-acoustic/src/hall.rs                                             */
+/* ---------------------------------------------------------------------------------------- */
+
+typedef struct pm_hall pm_hall;
+
+/* Hall::new, hall.rs:99-134: nx = max(across, 2), dx = width/(nx-1), height and depth
+ * quantised to whole cells. */
+int pm_hall_create(const char *name, double width, double height, double depth, size_t across,
+                   double speed, double density, pm_hall **out);
+void pm_hall_destroy(pm_hall *hall);
+int pm_hall_nodes(const pm_hall *hall, size_t nodes[3]);
+double pm_hall_spacing(const pm_hall *hall);
+/* Hall::released_from, hall.rs:166-184, with the profile already evaluated at every node:
+ * pressure set, pressure_prev = pressure, velocities zero, unstaggered, datum = invariant. */
+int pm_hall_release(pm_hall *hall, const double *pressure);
+/* A point source at one node: after the pressure sweep of step n the node gains
+ * increments[n] Pa, and the energy that puts into the invariant is booked as supplied.  No
+ * reference counterpart (the reference hall has no source term). */
+int pm_hall_set_source(pm_hall *hall, size_t node, const double *increments, uint64_t count);
+/* Domain::max_stable_dt (hall.rs:399-405) and Hall::courant (hall.rs:320-322). */
+double pm_hall_max_stable_dt(const pm_hall *hall);
+double pm_hall_courant(const pm_hall *hall, double dt);
+/* Domain::step, hall.rs:408-530: Courant guard, half step the first time, velocity then
+ * pressure sweeps, and the startup check of the first step. */
+int pm_hall_step(pm_hall *hall, double dt, pm_violation *v);
+/* Hall::energy (hall.rs:249-251) split into the invariant (hall.rs:334-384) and the startup
+ * offset, plus the energy a source has supplied. */
+int pm_hall_energy(pm_hall *hall, double *invariant, double *offset, double *supplied);
+/* Device-resident loop: `steps` steps, each audited (energy - supplied against the previous
+ * step's, relative tolerance `tolerance`, after the first step's startup).  The ledger total
+ * after every step goes to totals[0..steps] when not NULL. */
+int pm_hall_advance(pm_hall *hall, double dt, uint64_t steps, double tolerance, double *totals,
+                    double *elapsed_ms, pm_violation *v);
+/* All five arrays: p and pressure_prev [nx ny nz], vx [(nx-1) ny nz], vy [nx (ny-1) nz],
+ * vz [nx ny (nz-1)].  Any pointer may be NULL. */
+int pm_hall_get_fields(pm_hall *hall, double *p, double *pp, double *vx, double *vy, double *vz);
+/* Domain::checkpoint / restore, hall.rs:536-555. */
+int pm_hall_checkpoint(pm_hall *hall);
+int pm_hall_restore(pm_hall *hall);
+/* Hall::peak_pressure, hall.rs:242-244. */
+double pm_hall_peak_pressure(pm_hall *hall);
+
+/* ---------------------------------------------------------------------------------------- */
+/* Ensemble of independent heat grids, crates/pantometry-core/src/ensemble.rs:102-147       */
+/* ---------------------------------------------------------------------------------------- */
+
+/* One member: a uniform Solid3D (counts, spacing, substance, initial = ledger reference) and
+ * the step it takes. */
+typedef struct pm_member {
+    size_t nx, ny, nz;
+    double dx;
+    pm_substance substance;
+    double initial;
+    double dt;
+} pm_member;
+
+typedef struct pm_ensemble pm_ensemble;
+
+int pm_ensemble_create(const pm_member *members, size_t count, pm_ensemble **out);
+void pm_ensemble_destroy(pm_ensemble *ens);
+/* Member cells in, out (kelvin, x-fastest). */
+int pm_ensemble_set_cells(pm_ensemble *ens, size_t member, const double *cells);
+int pm_ensemble_get_cells(pm_ensemble *ens, size_t member, double *cells);
+/* `steps` steps of every member in one graph launch; each member's step is refused first if it
+ * is past that member's limit (PM_VIOLATION names the member in `step`). */
+int pm_ensemble_run(pm_ensemble *ens, uint64_t steps, double *elapsed_ms, pm_violation *v);
+/* Per-member stored heat (Solid3D::ledger's stored entry), index order. */
+int pm_ensemble_ledgers(pm_ensemble *ens, double *stored);
+/* Fix the schedule: members whose largest edge is at most `resident_edge` (and whose two copies
+ * fit in LDS) run resident, the rest streamed in z-chunks of `z_chunk` planes.  No reference
+ * counterpart. */
+int pm_ensemble_set_schedule(pm_ensemble *ens, int resident_edge, int z_chunk);
+/* Time the candidate schedules on this ensemble (a few steps each, state restored afterwards)
+ * and keep the fastest.  `report` (optional, `report_len` bytes) receives one line per
+ * candidate. */
+int pm_ensemble_tune(pm_ensemble *ens, char *report, size_t report_len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,385 @@
+"""Python face of the CPU oracle (pm_oracle.c).  TEST INFRASTRUCTURE ONLY: imported by tests/,
+the smoke check and the benchmark's CPU-baseline leg, never by the product.
+
+`OracleSolid3D` and `OracleHall` restate Solid3D and Hall (crates/pantometry-thermal/src/solid.rs,
+crates/pantometry-acoustic/src/hall.rs) on top of the C functions, with the same method names as
+the GPU domains so one test body can drive either.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import subprocess
+from pathlib import Path
+from typing import Callable, Optional, Tuple
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+
+_D = C.c_double
+_DP = C.POINTER(C.c_double)
+_S = C.c_size_t
+_U64 = C.c_uint64
+_I = C.c_int
+
+_SIG = {
+    "orc_mix64": (_U64, [_U64]),
+    "orc_rng_new": (_U64, [_U64]),
+    "orc_rng_for_index": (_U64, [_U64, _U64]),
+    "orc_rng_next_u64": (_U64, [C.POINTER(_U64)]),
+    "orc_rng_unit": (_D, [C.POINTER(_U64)]),
+    "orc_rng_range": (_D, [C.POINTER(_U64), _D, _D]),
+    "orc_rng_gaussian": (_D, [C.POINTER(_U64)]),
+    "orc_rng_split": (_U64, [C.POINTER(_U64)]),
+    "orc_rng_digest": (_U64, [_U64, _I]),
+    "orc_heat_capacity": (_D, [_D, _D, _D]),
+    "orc_series": (_D, [_D, _D]),
+    "orc_resolve": (_D, [_S, _S, _S, _D, _DP, _DP, _DP, _DP, _DP, _DP, _DP]),
+    "orc_solid_step": (None, [_S, _S, _S, _DP, _DP, _DP, _DP, _D, _DP, _DP]),
+    "orc_solid_step_omp": (None, [_S, _S, _S, _DP, _DP, _DP, _DP, _D, _DP, _DP, _I]),
+    "orc_solid_bus_heat": (None, [_S, _DP, _D, _DP]),
+    "orc_stored_heat": (_D, [_S, _DP, _DP, _D]),
+    "orc_stored_heat_compensated": (_D, [_S, _DP, _DP, _D]),
+    "orc_release_mode": (None, [_S, _S, _S, _I, _I, _I, _D, _D, _DP]),
+    "orc_mode_amplitude": (_D, [_S, _S, _S, _I, _I, _I, _DP]),
+    "orc_mode_amplification": (_D, [_S, _S, _S, _I, _I, _I, _D, _D, _D, _D, _D]),
+    "orc_hall_invariant": (_D, [_S, _S, _S, _D, _D, _D, _DP, _DP, _DP, _DP, _DP]),
+    "orc_hall_sweep": (_D, [_S, _S, _S, _D, _D, _D, _D, _D, _DP, _DP, _DP, _DP, _DP, _S, _D, _D,
+                            _I]),
+    "orc_hall_mode": (None, [_S, _S, _S, _D, _I, _I, _I, _D, _DP]),
+    "orc_workload_hot_spot": (None, [_S, _S, _S, _D, _D, _D, _D, _U64, _DP, _I]),
+    "orc_workload_heterogeneous": (None, [_S, _S, _S, _U64, _U64, _I, _DP, _DP, _I]),
+    "orc_workload_member": (None, [_U64, _U64, _D, _D, _D, _D, C.POINTER(C.c_longlong), _DP]),
+}
+
+_lib = None
+
+
+def build() -> Path:
+    """Compiles the oracle with its Makefile (gcc, -O2 -ffp-contract=off)."""
+    subprocess.run(["make", "-s", "-C", str(HERE), "-B"], check=True)
+    return LIB
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        h = C.CDLL(str(LIB))
+        for name, (res, args) in _SIG.items():
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = h
+    return _lib
+
+
+def threads() -> int:
+    n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def p(a: np.ndarray):
+    return a.ctypes.data_as(_DP)
+
+
+# ---- Rng ----------------------------------------------------------------------------------
+class OracleRng:
+    def __init__(self, seed: int):
+        self.s = C.c_uint64(lib().orc_rng_new(seed))
+
+    @staticmethod
+    def for_index(seed: int, index: int) -> "OracleRng":
+        r = OracleRng.__new__(OracleRng)
+        r.s = C.c_uint64(lib().orc_rng_for_index(seed, index))
+        return r
+
+    def unit(self) -> float:
+        return lib().orc_rng_unit(C.byref(self.s))
+
+    def range(self, lo: float, hi: float) -> float:
+        return lib().orc_rng_range(C.byref(self.s), lo, hi)
+
+    def gaussian(self) -> float:
+        return lib().orc_rng_gaussian(C.byref(self.s))
+
+    def next_u64(self) -> int:
+        return lib().orc_rng_next_u64(C.byref(self.s))
+
+
+def rng_digest(seed: int, draws: int) -> int:
+    return lib().orc_rng_digest(seed, draws)
+
+
+# ---- Ledger / audit (conserved.rs) ---------------------------------------------------------
+def ledger_total(entries) -> Tuple[float, float]:
+    """Ledger::with over `entries` (conserved.rs:78-89): (total, scale)."""
+    total, scale = 0.0, 0.0
+    for e in entries:
+        total += e
+        scale = max(scale, abs(e))
+    return total, scale
+
+
+def audit_breaks(before: Tuple[float, float], after: Tuple[float, float], tol: float) -> bool:
+    """audit_with for one quantity (conserved.rs:335-360)."""
+    b, sb = before
+    a, sa = after
+    scale = max(abs(b), abs(a), sb, sa)
+    if scale < 1e-300:
+        return False
+    return abs(a - b) / scale > tol
+
+
+# ---- Solid3D -------------------------------------------------------------------------------
+class OracleSolid3D:
+    """Solid3D's plain branch on the CPU, as the reference computes it."""
+
+    def __init__(self, name: str, density: float, conductivity: float, specific_heat: float,
+                 counts: Tuple[int, int, int], dx: float, initial: float):
+        self.name_ = name
+        self.counts_ = tuple(max(1, int(c)) for c in counts)
+        nx, ny, nz = self.counts_
+        self.n = nx * ny * nz
+        self.dx = float(dx)
+        self.reference = float(initial)
+        self.cells = np.full(self.n, float(initial))
+        self.saved = self.cells.copy()
+        self.kcell = np.full(self.n, float(conductivity))
+        cap = lib().orc_heat_capacity(density, specific_heat, self.dx)
+        self.cap = np.full(self.n, cap)
+        self.first = (density, conductivity, specific_heat)
+        self.absorbed = 0.0
+        self._old = np.empty(self.n)
+        self.resolve()
+
+    def resolve(self) -> None:
+        nx, ny, nz = self.counts_
+        self.kx = np.empty((nx + 1) * ny * nz)
+        self.ky = np.empty(nx * (ny + 1) * nz)
+        self.kz = np.empty(nx * ny * (nz + 1))
+        self.mob = np.empty(self.n)
+        self.face_sum = np.empty(self.n)
+        self.worst_rate = lib().orc_resolve(nx, ny, nz, self.dx, p(self.kcell), p(self.cap),
+                                            p(self.kx), p(self.ky), p(self.kz), p(self.mob),
+                                            p(self.face_sum))
+
+    def fill(self, density: float, conductivity: float, specific_heat: float, mask: np.ndarray):
+        m = np.asarray(mask).reshape(-1).astype(bool)
+        self.kcell[m] = conductivity
+        self.cap[m] = lib().orc_heat_capacity(density, specific_heat, self.dx)
+        self.resolve()
+        return self
+
+    def with_conductivity(self, k: np.ndarray):
+        self.kcell = np.ascontiguousarray(k, dtype=np.float64).reshape(-1).copy()
+        self.resolve()
+        return self
+
+    def set_cells(self, cells: np.ndarray) -> None:
+        self.cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1).copy()
+
+    def max_stable_dt(self, now: float = 0.0) -> float:
+        if math.isnan(self.worst_rate):
+            return math.inf
+        return math.inf if self.worst_rate == 0.0 else 1.0 / self.worst_rate
+
+    def refuses(self, dt: float) -> Optional[str]:
+        ratio = dt * self.worst_rate
+        if math.isnan(ratio):
+            return "substance has no diffusivity"
+        if ratio > 1.0 + 6e-12:
+            return "Fourier number"
+        return None
+
+    def step(self, dt: float, gained: float = 0.0, threads_: int = 1) -> None:
+        why = self.refuses(dt)
+        if why:
+            raise ValueError(why)
+        nx, ny, nz = self.counts_
+        if gained != 0.0:
+            self.absorbed += gained
+            lib().orc_solid_bus_heat(self.n, p(self.cap), gained, p(self.cells))
+        if threads_ > 1:
+            lib().orc_solid_step_omp(nx, ny, nz, p(self.kx), p(self.ky), p(self.kz), p(self.mob),
+                                     dt, p(self.cells), p(self._old), threads_)
+        else:
+            lib().orc_solid_step(nx, ny, nz, p(self.kx), p(self.ky), p(self.kz), p(self.mob), dt,
+                                 p(self.cells), p(self._old))
+
+    def stored_heat(self) -> float:
+        return lib().orc_stored_heat(self.n, p(self.cap), p(self.cells), self.reference)
+
+    def stored_heat_compensated(self) -> float:
+        return lib().orc_stored_heat_compensated(self.n, p(self.cap), p(self.cells),
+                                                 self.reference)
+
+    def ledger(self) -> Tuple[float, float]:
+        return ledger_total([self.stored_heat(), 0.0, -0.0])
+
+    def release_mode(self, mode, mean: float, amplitude: float) -> None:
+        nx, ny, nz = self.counts_
+        lib().orc_release_mode(nx, ny, nz, mode[0], mode[1], mode[2], mean, amplitude,
+                               p(self.cells))
+        self.saved = self.cells.copy()
+
+    def mode_amplitude(self, mode) -> float:
+        nx, ny, nz = self.counts_
+        return lib().orc_mode_amplitude(nx, ny, nz, mode[0], mode[1], mode[2], p(self.cells))
+
+    def mode_amplification(self, mode, dt: float) -> float:
+        nx, ny, nz = self.counts_
+        d, k, c = self.first
+        return lib().orc_mode_amplification(nx, ny, nz, mode[0], mode[1], mode[2], k, d, c,
+                                            self.dx, dt)
+
+    def deposit(self, i: int, j: int, k: int, joules: float) -> None:
+        nx, ny, nz = self.counts_
+        if i >= nx or j >= ny or k >= nz:
+            return
+        idx = i + nx * (j + ny * k)
+        self.cells[idx] += joules / self.cap[idx]
+        self.absorbed += joules
+
+    def grid(self) -> np.ndarray:
+        nx, ny, nz = self.counts_
+        return self.cells.reshape(nz, ny, nx)
+
+
+# ---- Hall ----------------------------------------------------------------------------------
+class OracleHall:
+    """Hall on the CPU, as the reference computes it (plus the point-source extension)."""
+
+    def __init__(self, width: float, height: float, depth: float, across: int, speed: float,
+                 density: float):
+        nx = max(across, 2)
+        dx = max(width / float(nx - 1), 2.2250738585072014e-308)
+
+        def quantise(length: float) -> int:
+            return int(max(_round_half_away(length / dx), 0.0)) + 1
+
+        self.nx, self.ny, self.nz = nx, quantise(height), quantise(depth)
+        self.dx, self.speed, self.density = dx, speed, density
+        n = self.nx * self.ny * self.nz
+        self.p = np.zeros(n)
+        self.pp = np.zeros(n)
+        self.vx = np.zeros(max((self.nx - 1) * self.ny * self.nz, 1))
+        self.vy = np.zeros(max(self.nx * (self.ny - 1) * self.nz, 1))
+        self.vz = np.zeros(max(self.nx * self.ny * (self.nz - 1), 1))
+        self.staggered = False
+        self.offset = 0.0
+        self.datum = 0.0
+        self.supplied = 0.0
+        self.src = None
+        self.increments = None
+        self.cursor = 0
+
+    def nodes(self):
+        return self.nx, self.ny, self.nz
+
+    def invariant(self) -> float:
+        return lib().orc_hall_invariant(self.nx, self.ny, self.nz, self.dx, self.speed,
+                                        self.density, p(self.p), p(self.pp), p(self.vx),
+                                        p(self.vy), p(self.vz))
+
+    def energy(self) -> float:
+        return self.invariant() + self.offset
+
+    def release(self, pressure: np.ndarray) -> None:
+        self.p = np.ascontiguousarray(pressure, dtype=np.float64).reshape(-1).copy()
+        self.pp = self.p.copy()
+        self.vx[:] = 0.0
+        self.vy[:] = 0.0
+        self.vz[:] = 0.0
+        self.staggered = False
+        self.offset = 0.0
+        self.supplied = 0.0
+        self.cursor = 0
+        self.datum = self.invariant()
+
+    def mode(self, mode, amplitude: float) -> np.ndarray:
+        out = np.empty(self.nx * self.ny * self.nz)
+        lib().orc_hall_mode(self.nx, self.ny, self.nz, self.dx, mode[0], mode[1], mode[2],
+                            amplitude, p(out))
+        return out
+
+    def set_source(self, node: int, increments: np.ndarray) -> None:
+        self.src = node
+        self.increments = np.asarray(increments, dtype=np.float64)
+        self.cursor = 0
+
+    def max_stable_dt(self) -> float:
+        return self.dx / (self.speed * math.sqrt(3.0))
+
+    def courant(self, dt: float) -> float:
+        return self.speed * dt * math.sqrt(3.0) / self.dx
+
+    def step(self, dt: float, threads_: int = 1) -> None:
+        if self.courant(dt) > 1.0 + 1e-12:
+            raise ValueError("Courant number")
+        if dt <= 0.0:
+            return
+        vh = dt if self.staggered else 0.5 * dt
+        starting = not self.staggered
+        self.staggered = True
+        src, inc, prev = (2**64 - 1), 0.0, 0.0
+        if self.src is not None and self.cursor <= len(self.increments):
+            src = self.src
+            if self.cursor < len(self.increments):
+                inc = float(self.increments[self.cursor])
+            if self.cursor > 0:
+                prev = float(self.increments[self.cursor - 1])
+        self.cursor += 1
+        added = lib().orc_hall_sweep(self.nx, self.ny, self.nz, self.dx, self.speed, self.density,
+                                     dt, vh, p(self.p), p(self.pp), p(self.vx), p(self.vy),
+                                     p(self.vz), src, inc, prev, threads_)
+        self.supplied += added
+        if starting:
+            inv = self.invariant() - added
+            off = self.datum - inv
+            scale = max(abs(self.datum), abs(inv))
+            if scale > 0.0 and abs(off) / scale > 0.25:
+                raise ValueError("energy (starting the leapfrog)")
+            self.offset = off
+
+    def ledger(self) -> Tuple[float, float]:
+        entries = [self.energy()]
+        if self.supplied != 0.0:
+            entries.append(-self.supplied)
+        return ledger_total(entries)
+
+
+def _round_half_away(x: float) -> float:
+    """Rust's f64::round: halves away from zero (x - floor(x) is exact below 2**52)."""
+    a = abs(x)
+    f = math.floor(a)
+    r = f + 1.0 if a - f >= 0.5 else f
+    return math.copysign(r, x)
+
+
+# ---- workloads -----------------------------------------------------------------------------
+def hot_spot_cells(counts, amp=100.0, sigma=6.0, noise=0.1, seed=0x5EED0000, base=300.0):
+    nx, ny, nz = counts
+    out = np.empty(nx * ny * nz)
+    lib().orc_workload_hot_spot(nx, ny, nz, base, amp, sigma, noise, seed, p(out), threads())
+    return out
+
+
+def heterogeneous_fields(counts, spots=8, k_seed=0x5EED0001, spot_seed=0x5EED0002):
+    nx, ny, nz = counts
+    k = np.empty(nx * ny * nz)
+    t = np.empty(nx * ny * nz)
+    lib().orc_workload_heterogeneous(nx, ny, nz, k_seed, spot_seed, spots, p(k), p(t), threads())
+    return k, t
+
+
+def member_shape(seed: int, i: int, lo_edge=16.0, hi_edge=96.0, lo_alpha=1e-5, hi_alpha=1e-3):
+    edge = C.c_longlong()
+    alpha = C.c_double()
+    lib().orc_workload_member(seed, i, lo_edge, hi_edge, lo_alpha, hi_alpha, C.byref(edge),
+                              C.byref(alpha))
+    return edge.value, alpha.value

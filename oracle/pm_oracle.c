@@ -1,0 +1,472 @@
+/*
+ * CPU oracle for the explicit 3D grid-stencil path of pantometry.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file restates, in plain C99 and IEEE binary64 with
+ * round-to-nearest, the arithmetic of the reference's Rust implementation, in the reference's
+ * own evaluation order: Rust never contracts `a*b+c` into an FMA, never reassociates, and
+ * evaluates `a * b * c` as `(a*b)*c`.  It is compiled with `-O2 -ffp-contract=off` and without
+ * `-ffast-math`.  Only tests, the smoke check and the benchmark's CPU-baseline leg load it; the
+ * product library never links or calls anything here.
+ *
+ * Every function cites the reference file:line it follows.  Which test pins which function is
+ * listed in oracle/README.md.
+ */
+#include <math.h>
+#include <stddef.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------------------------ */
+/* Rng: crates/pantometry-core/src/rng.rs:37-152                                               */
+/* ------------------------------------------------------------------------------------------ */
+
+/* SplitMix64 finaliser, rng.rs:37-42. */
+uint64_t orc_mix64(uint64_t z) {
+    z = z + 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+/* Rng::new, rng.rs:49-55: a zero seed is replaced. */
+uint64_t orc_rng_new(uint64_t seed) { return seed == 0 ? 0x9E3779B97F4A7C15ULL : seed; }
+
+/* Rng::for_index, rng.rs:66-68. */
+uint64_t orc_rng_for_index(uint64_t seed, uint64_t index) {
+    return orc_rng_new(orc_mix64(seed ^ orc_mix64(index)));
+}
+
+/* Rng::next_u64, rng.rs:81-88: xorshift64* . */
+uint64_t orc_rng_next_u64(uint64_t *s) {
+    uint64_t x = *s;
+    x ^= x >> 12;
+    x ^= x << 25;
+    x ^= x >> 27;
+    *s = x;
+    return x * 0x2545F4914F6CDD1DULL;
+}
+
+/* Rng::unit, rng.rs:91-93. */
+double orc_rng_unit(uint64_t *s) {
+    return (double)(orc_rng_next_u64(s) >> 11) / (double)(1ULL << 53);
+}
+
+/* Rng::range, rng.rs:96-98. */
+double orc_rng_range(uint64_t *s, double lo, double hi) { return lo + (hi - lo) * orc_rng_unit(s); }
+
+/* Rng::gaussian, rng.rs:146-151 (Box-Muller through libm log, sqrt and cos). */
+double orc_rng_gaussian(uint64_t *s) {
+    double u1 = orc_rng_unit(s);
+    if (!(u1 >= 2.2250738585072014e-308)) u1 = 2.2250738585072014e-308; /* f64::max(MIN_POSITIVE) */
+    double u2 = orc_rng_unit(s);
+    return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+/* Rng::split, rng.rs:75-78. */
+uint64_t orc_rng_split(uint64_t *s) { return orc_rng_new(orc_mix64(orc_rng_next_u64(s))); }
+
+/* The order-sensitive digest of rng.rs:295-305 (`the_stream_is_pinned`). */
+uint64_t orc_rng_digest(uint64_t seed, int draws) {
+    uint64_t s = orc_rng_new(seed);
+    uint64_t hash = 0;
+    for (int n = 0; n < draws; n++) {
+        uint64_t rot = (hash << 7) | (hash >> 57);
+        hash = (rot * 0x1000000001B3ULL) ^ (uint64_t)(orc_rng_unit(&s) * (double)(1ULL << 53));
+    }
+    return hash;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Solid3D: crates/pantometry-thermal/src/solid.rs                                             */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Substance::heat_capacity = mass_of(volume) * c_p = (rho*volume)*c_p,
+ * crates/pantometry-core/src/substance.rs:219-239; volume = (dx*dx)*dx, solid.rs:591. */
+double orc_heat_capacity(double density, double specific_heat, double dx) {
+    double volume = dx * dx * dx;
+    return (density * volume) * specific_heat;
+}
+
+/* The harmonic face mean `series`, solid.rs:706-713: `2.0 * a * b / sum` guarded at zero. */
+double orc_series(double a, double b) {
+    double sum = a + b;
+    if (sum > 0.0) return 2.0 * a * b / sum;
+    return 0.0;
+}
+
+/* Solid3D::resolve for the plain branch (no phase change, no void), solid.rs:704-782.
+ *   kcell[N]   per-cell conductivity, the `mixed.0` of solid.rs:686-702
+ *   cap[N]     per-cell heat capacity
+ * Out: kx[(nx+1)ny nz], ky[nx(ny+1)nz], kz[nx ny(nz+1)], mobility[N], face_sum[N];
+ * returns worst_rate (NaN when any rate is NaN, solid.rs:771-782). */
+double orc_resolve(size_t nx, size_t ny, size_t nz, double dx, const double *kcell,
+                   const double *cap, double *kx, double *ky, double *kz, double *mobility,
+                   double *face_sum) {
+    memset(kx, 0, sizeof(double) * (nx + 1) * ny * nz);
+    memset(ky, 0, sizeof(double) * nx * (ny + 1) * nz);
+    memset(kz, 0, sizeof(double) * nx * ny * (nz + 1));
+    for (size_t k = 0; k < nz; k++)
+        for (size_t j = 0; j < ny; j++)
+            for (size_t i = 0; i < nx; i++) {
+                size_t c = i + nx * (j + ny * k);
+                if (i > 0) kx[i + (nx + 1) * (j + ny * k)] = orc_series(kcell[c - 1], kcell[c]);
+                if (j > 0) ky[i + nx * (j + (ny + 1) * k)] = orc_series(kcell[c - nx], kcell[c]);
+                if (k > 0) kz[i + nx * (j + ny * k)] = orc_series(kcell[c - nx * ny], kcell[c]);
+            }
+    double worst = 0.0;
+    int nowhere = 0;
+    for (size_t k = 0; k < nz; k++)
+        for (size_t j = 0; j < ny; j++)
+            for (size_t i = 0; i < nx; i++) {
+                size_t c = i + nx * (j + ny * k);
+                mobility[c] = dx / cap[c];
+                double sum = kx[i + (nx + 1) * (j + ny * k)] + kx[i + 1 + (nx + 1) * (j + ny * k)] +
+                             ky[i + nx * (j + (ny + 1) * k)] + ky[i + nx * (j + 1 + (ny + 1) * k)] +
+                             kz[i + nx * (j + ny * k)] + kz[i + nx * (j + ny * (k + 1))];
+                face_sum[c] = sum;
+                double rate = mobility[c] * sum;
+                if (isnan(rate))
+                    nowhere = 1;
+                else
+                    worst = fmax(worst, rate);
+            }
+    return nowhere ? NAN : worst;
+}
+
+/* One cell of the plain sweep, solid.rs:1776-1826 (Appendix A.1 of the contract). */
+static inline double orc_cell(size_t nx, size_t ny, size_t nz, size_t i, size_t j, size_t k,
+                              const double *kx, const double *ky, const double *kz,
+                              const double *mobility, double dts, const double *old) {
+    size_t c = i + nx * (j + ny * k);
+    double t = old[c];
+    size_t xr = (nx + 1) * (j + ny * k), yr = nx * (j + (ny + 1) * k);
+    double flux = 0.0;
+    if (i > 0) flux += kx[i + xr] * (old[c - 1] - t);
+    if (i + 1 < nx) flux += kx[i + 1 + xr] * (old[c + 1] - t);
+    if (j > 0) flux += ky[i + yr] * (old[c - nx] - t);
+    if (j + 1 < ny) flux += ky[i + nx + yr] * (old[c + nx] - t);
+    if (k > 0) flux += kz[c] * (old[c - nx * ny] - t);
+    if (k + 1 < nz) flux += kz[c + nx * ny] * (old[c + nx * ny] - t);
+    /* add_kelvin(c, dts * mobility[c] * (flux - shed)) with shed = 0 and latent = 0,
+     * solid.rs:1818 and 1620-1635. */
+    double rise = dts * mobility[c] * (flux - 0.0);
+    return t + rise;
+}
+
+/* Solid3D::step, plain branch, single thread exactly as the reference (with the per-step clone
+ * of solid.rs:1774 into `old`, which the caller provides as scratch of N doubles). */
+void orc_solid_step(size_t nx, size_t ny, size_t nz, const double *kx, const double *ky,
+                    const double *kz, const double *mobility, double dts, double *cells,
+                    double *old) {
+    memcpy(old, cells, sizeof(double) * nx * ny * nz);
+    for (size_t k = 0; k < nz; k++)
+        for (size_t j = 0; j < ny; j++)
+            for (size_t i = 0; i < nx; i++)
+                cells[i + nx * (j + ny * k)] =
+                    orc_cell(nx, ny, nz, i, j, k, kx, ky, kz, mobility, dts, old);
+}
+
+/* The same sweep with OpenMP over z-planes.  Cells are independent, so the result is
+ * bit-identical to orc_solid_step; used for full-size parity runs and the CPU baseline. */
+void orc_solid_step_omp(size_t nx, size_t ny, size_t nz, const double *kx, const double *ky,
+                        const double *kz, const double *mobility, double dts, double *cells,
+                        double *old, int threads) {
+    size_t n = nx * ny * nz;
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (long long c = 0; c < (long long)n; c++) old[c] = cells[c];
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (long long k = 0; k < (long long)nz; k++)
+        for (size_t j = 0; j < ny; j++)
+            for (size_t i = 0; i < nx; i++)
+                cells[i + nx * (j + ny * (size_t)k)] =
+                    orc_cell(nx, ny, nz, i, j, (size_t)k, kx, ky, kz, mobility, dts, old);
+}
+
+/* Placeless bus heat, solid.rs:1752-1765: a uniform rise gained / sum(capacity), the sum a left
+ * fold in index order (`self.capacity.iter().sum::<f64>()`). */
+void orc_solid_bus_heat(size_t n, const double *cap, double gained, double *cells) {
+    double total = 0.0;
+    for (size_t c = 0; c < n; c++) total += cap[c];
+    double rise = gained / total;
+    for (size_t c = 0; c < n; c++) cells[c] += rise;
+}
+
+/* Solid3D::stored_heat, solid.rs:1557-1561 with enthalpy_joules' latent <= 0 branch
+ * (solid.rs:1583-1589): a left fold of capacity*(t - reference) - 0. */
+double orc_stored_heat(size_t n, const double *cap, const double *cells, double reference) {
+    double acc = 0.0;
+    for (size_t c = 0; c < n; c++) acc += cap[c] * (cells[c] - reference) - 0.0;
+    return acc;
+}
+
+/* Neumaier-compensated form of the same sum: the diagnostic the audit is judged beside. */
+double orc_stored_heat_compensated(size_t n, const double *cap, const double *cells,
+                                   double reference) {
+    double sum = 0.0, comp = 0.0;
+    for (size_t c = 0; c < n; c++) {
+        double x = cap[c] * (cells[c] - reference) - 0.0;
+        double t = sum + x;
+        if (fabs(sum) >= fabs(x))
+            comp += (sum - t) + x;
+        else
+            comp += (x - t) + sum;
+        sum = t;
+    }
+    return sum + comp;
+}
+
+/* Solid3D::release_mode, solid.rs:1478-1500. */
+void orc_release_mode(size_t nx, size_t ny, size_t nz, int a, int b, int cm, double mean,
+                      double amplitude, double *cells) {
+    const double PI = 3.141592653589793;
+    for (size_t k = 0; k < nz; k++)
+        for (size_t j = 0; j < ny; j++)
+            for (size_t i = 0; i < nx; i++) {
+                double px = nx <= 1 ? 1.0 : cos((double)a * PI * ((double)i + 0.5) / (double)nx);
+                double py = ny <= 1 ? 1.0 : cos((double)b * PI * ((double)j + 0.5) / (double)ny);
+                double pz = nz <= 1 ? 1.0 : cos((double)cm * PI * ((double)k + 0.5) / (double)nz);
+                cells[i + nx * (j + ny * k)] = mean + amplitude * px * py * pz;
+            }
+}
+
+/* Solid3D::mode_amplitude, solid.rs:1507-1535. */
+double orc_mode_amplitude(size_t nx, size_t ny, size_t nz, int a, int b, int cm,
+                          const double *cells) {
+    const double PI = 3.141592653589793;
+    double n0 = (nx <= 1 || a == 0) ? 1.0 : 0.5;
+    double n1 = (ny <= 1 || b == 0) ? 1.0 : 0.5;
+    double n2 = (nz <= 1 || cm == 0) ? 1.0 : 0.5;
+    double norm = 1.0 * n0 * n1 * n2;
+    double sum = 0.0;
+    for (size_t k = 0; k < nz; k++)
+        for (size_t j = 0; j < ny; j++)
+            for (size_t i = 0; i < nx; i++) {
+                double px = nx <= 1 ? 1.0 : cos((double)a * PI * ((double)i + 0.5) / (double)nx);
+                double py = ny <= 1 ? 1.0 : cos((double)b * PI * ((double)j + 0.5) / (double)ny);
+                double pz = nz <= 1 ? 1.0 : cos((double)cm * PI * ((double)k + 0.5) / (double)nz);
+                sum += cells[i + nx * (j + ny * k)] * px * py * pz;
+            }
+    return sum / ((double)(nx * ny * nz) * norm);
+}
+
+/* mode_eigenvalue_dx2 and mode_amplification, solid.rs:1455-1474; fourier_number with
+ * alpha = k/(rho*c_p), solid.rs:1423-1428 and substance.rs:211-216. */
+double orc_mode_amplification(size_t nx, size_t ny, size_t nz, int a, int b, int cm,
+                              double conductivity, double density, double specific_heat,
+                              double dx, double dt) {
+    const double PI = 3.141592653589793;
+    double alpha = conductivity / (density * specific_heat);
+    double f = alpha * dt / (dx * dx);
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+    if (nx > 1) { double s = sin((double)a * PI / (2.0 * (double)nx)); t0 = -4.0 * s * s; }
+    if (ny > 1) { double s = sin((double)b * PI / (2.0 * (double)ny)); t1 = -4.0 * s * s; }
+    if (nz > 1) { double s = sin((double)cm * PI / (2.0 * (double)nz)); t2 = -4.0 * s * s; }
+    return 1.0 + f * (t0 + t1 + t2);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Hall: crates/pantometry-acoustic/src/hall.rs                                                */
+/* ------------------------------------------------------------------------------------------ */
+
+/* wall_weight, hall.rs:675-681. */
+static inline double orc_wall(size_t k, size_t n) { return (k == 0 || k + 1 == n) ? 2.0 : 1.0; }
+
+/* Hall::invariant_si, hall.rs:334-384: four sequential folds into one total. */
+double orc_hall_invariant(size_t nx, size_t ny, size_t nz, double dx, double speed,
+                          double density, const double *p, const double *pp, const double *vx,
+                          const double *vy, const double *vz) {
+    double volume = dx * dx * dx;
+    double rc2 = density * speed * speed;
+    if (rc2 <= 0.0) return 0.0;
+    double total = 0.0;
+    for (size_t k = 0; k < nz; k++)
+        for (size_t j = 0; j < ny; j++)
+            for (size_t i = 0; i < nx; i++) {
+                double share = 1.0 / (orc_wall(i, nx) * orc_wall(j, ny) * orc_wall(k, nz));
+                size_t idx = i + nx * (j + ny * k);
+                total += share * p[idx] * pp[idx] / (2.0 * rc2) * volume;
+            }
+    double half_rho = 0.5 * density;
+    for (size_t k = 0; k < nz; k++)
+        for (size_t j = 0; j < ny; j++)
+            for (size_t i = 0; i + 1 < nx; i++) {
+                double v = vx[i + (nx - 1) * (j + ny * k)];
+                double share = 1.0 / (orc_wall(j, ny) * orc_wall(k, nz));
+                total += share * half_rho * v * v * volume;
+            }
+    for (size_t k = 0; k < nz; k++)
+        for (size_t j = 0; j + 1 < ny; j++)
+            for (size_t i = 0; i < nx; i++) {
+                double v = vy[i + nx * (j + (ny - 1) * k)];
+                double share = 1.0 / (orc_wall(i, nx) * orc_wall(k, nz));
+                total += share * half_rho * v * v * volume;
+            }
+    for (size_t k = 0; k + 1 < nz; k++)
+        for (size_t j = 0; j < ny; j++)
+            for (size_t i = 0; i < nx; i++) {
+                double v = vz[i + nx * (j + ny * k)];
+                double share = 1.0 / (orc_wall(i, nx) * orc_wall(j, ny));
+                total += share * half_rho * v * v * volume;
+            }
+    return total;
+}
+
+/* The sweeps of Hall::step, hall.rs:430-505, after the Courant guard and the stagger choice
+ * (the caller passes vh = h/2 on the first step, h afterwards).  pressure_prev receives the
+ * pre-update pressure (hall.rs:463).  A point source adds `source` Pa at node `src` after the
+ * pressure sweep (an extension: the reference hall has no source term); src = SIZE_MAX for
+ * none.  Returns the energy the source moved into the invariant this step:
+ * ((share*(source + source_prev))*pp)/(2 rc2)*volume, where source_prev is the previous step's
+ * increment.  An increment d added to p^n changes the invariant once at once (d*p^(n-1)) and
+ * once at the next step (d*p^n), because the leapfrog's energy identity for step n+1 uses
+ * p^n - p^(n-1), which the increment has moved. */
+double orc_hall_sweep(size_t nx, size_t ny, size_t nz, double dx, double speed, double density,
+                      double h, double vh, double *p, double *pp, double *vx, double *vy,
+                      double *vz, size_t src, double source, double source_prev,
+                      int threads) {
+    double rc2 = density * speed * speed;
+    double gain = vh / (density * dx);
+    (void)threads;
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (long long k = 0; k < (long long)nz; k++)
+        for (size_t j = 0; j < ny; j++)
+            for (size_t i = 0; i + 1 < nx; i++) {
+                size_t a = i + nx * (j + ny * (size_t)k);
+                double d = p[a + 1] - p[a];
+                vx[i + (nx - 1) * (j + ny * (size_t)k)] -= gain * d;
+            }
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (long long k = 0; k < (long long)nz; k++)
+        for (size_t j = 0; j + 1 < ny; j++)
+            for (size_t i = 0; i < nx; i++) {
+                size_t a = i + nx * (j + ny * (size_t)k);
+                double d = p[a + nx] - p[a];
+                vy[i + nx * (j + (ny - 1) * (size_t)k)] -= gain * d;
+            }
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (long long k = 0; k < (long long)nz - 1; k++)
+        for (size_t j = 0; j < ny; j++)
+            for (size_t i = 0; i < nx; i++) {
+                size_t a = i + nx * (j + ny * (size_t)k);
+                double d = p[a + nx * ny] - p[a];
+                vz[a] -= gain * d;
+            }
+    memcpy(pp, p, sizeof(double) * nx * ny * nz);
+    double scale = h * rc2 / dx;
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (long long kk = 0; kk < (long long)nz; kk++)
+        for (size_t j = 0; j < ny; j++)
+            for (size_t i = 0; i < nx; i++) {
+                size_t k = (size_t)kk;
+                double lo_x = i > 0 ? vx[i - 1 + (nx - 1) * (j + ny * k)] : 0.0;
+                double hi_x = i + 1 < nx ? vx[i + (nx - 1) * (j + ny * k)] : 0.0;
+                double lo_y = j > 0 ? vy[i + nx * (j - 1 + (ny - 1) * k)] : 0.0;
+                double hi_y = j + 1 < ny ? vy[i + nx * (j + (ny - 1) * k)] : 0.0;
+                double lo_z = k > 0 ? vz[i + nx * (j + ny * (k - 1))] : 0.0;
+                double hi_z = k + 1 < nz ? vz[i + nx * (j + ny * k)] : 0.0;
+                size_t idx = i + nx * (j + ny * k);
+                p[idx] -= scale * (orc_wall(i, nx) * (hi_x - lo_x) + orc_wall(j, ny) * (hi_y - lo_y) +
+                                   orc_wall(k, nz) * (hi_z - lo_z));
+            }
+    if (src == (size_t)-1) return 0.0;
+    p[src] = p[src] + source;
+    size_t si = src % nx, sj = (src / nx) % ny, sk = src / (nx * ny);
+    double share = 1.0 / (orc_wall(si, nx) * orc_wall(sj, ny) * orc_wall(sk, nz));
+    double volume = dx * dx * dx;
+    return share * (source + source_prev) * pp[src] / (2.0 * rc2) * volume;
+}
+
+/* Hall::released_in_mode's profile, hall.rs:190-210, evaluated at node positions
+ * DVec3(i, j, k) * dx (hall.rs:170-180). */
+void orc_hall_mode(size_t nx, size_t ny, size_t nz, double dx, int a, int b, int cm,
+                   double amplitude, double *p) {
+    const double PI = 3.141592653589793;
+    double lx = (double)(nx - 1) * dx, ly = (double)(ny - 1) * dx, lz = (double)(nz - 1) * dx;
+    for (size_t k = 0; k < nz; k++)
+        for (size_t j = 0; j < ny; j++)
+            for (size_t i = 0; i < nx; i++) {
+                double x = (double)i * dx, y = (double)j * dx, z = (double)k * dx;
+                double px = lx > 0.0 ? cos((double)a * PI * x / lx) : 1.0;
+                double py = ly > 0.0 ? cos((double)b * PI * y / ly) : 1.0;
+                double pz = lz > 0.0 ? cos((double)cm * PI * z / lz) : 1.0;
+                p[i + nx * (j + ny * k)] = amplitude * px * py * pz;
+            }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Seeded workloads (the benchmark configurations), restated independently of the library's  */
+/* generator so the two can be compared bit for bit.                                           */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Gaussian hot spot plus uniform noise: T = 300 + amp*exp(-r2/(2 s^2)) + range(-0.1, 0.1),
+ * r measured in cells from the block centre, noise from Rng::for_index(noise_seed, cell). */
+void orc_workload_hot_spot(size_t nx, size_t ny, size_t nz, double base, double amp,
+                           double sigma, double noise, uint64_t noise_seed, double *cells,
+                           int threads) {
+    (void)threads;
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (long long kk = 0; kk < (long long)nz; kk++)
+        for (size_t j = 0; j < ny; j++)
+            for (size_t i = 0; i < nx; i++) {
+                size_t k = (size_t)kk;
+                size_t c = i + nx * (j + ny * k);
+                double ddx = ((double)i + 0.5) - 0.5 * (double)nx;
+                double ddy = ((double)j + 0.5) - 0.5 * (double)ny;
+                double ddz = ((double)k + 0.5) - 0.5 * (double)nz;
+                double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                uint64_t s = orc_rng_for_index(noise_seed, c);
+                double t = base + amp * exp(-r2 / (2.0 * sigma * sigma));
+                cells[c] = t + orc_rng_range(&s, -noise, noise);
+            }
+}
+
+/* Heterogeneous block: k_c = exp(0 + 0.5 * gaussian) from Rng::for_index(k_seed, c); T = 300 +
+ * sum over `spots` Gaussian hot spots whose amplitude, width and centre are drawn from
+ * Rng::for_index(spot_seed, m). */
+void orc_workload_heterogeneous(size_t nx, size_t ny, size_t nz, uint64_t k_seed,
+                                uint64_t spot_seed, int spots, double *kcell, double *cells,
+                                int threads) {
+    double amp[64], sig[64], cx[64], cy[64], cz[64];
+    if (spots > 64) spots = 64;
+    for (int m = 0; m < spots; m++) {
+        uint64_t s = orc_rng_for_index(spot_seed, (uint64_t)m);
+        amp[m] = orc_rng_range(&s, 50.0, 200.0);
+        sig[m] = orc_rng_range(&s, 4.0, 16.0);
+        cx[m] = orc_rng_range(&s, 0.1 * (double)nx, 0.9 * (double)nx);
+        cy[m] = orc_rng_range(&s, 0.1 * (double)ny, 0.9 * (double)ny);
+        cz[m] = orc_rng_range(&s, 0.1 * (double)nz, 0.9 * (double)nz);
+    }
+    (void)threads;
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (long long kk = 0; kk < (long long)nz; kk++)
+        for (size_t j = 0; j < ny; j++)
+            for (size_t i = 0; i < nx; i++) {
+                size_t k = (size_t)kk;
+                size_t c = i + nx * (j + ny * k);
+                uint64_t s = orc_rng_for_index(k_seed, c);
+                kcell[c] = exp(0.0 + 0.5 * orc_rng_gaussian(&s));
+                double t = 300.0;
+                for (int m = 0; m < spots; m++) {
+                    double ddx = ((double)i + 0.5) - cx[m];
+                    double ddy = ((double)j + 0.5) - cy[m];
+                    double ddz = ((double)k + 0.5) - cz[m];
+                    double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                    t = t + amp[m] * exp(-r2 / (2.0 * sig[m] * sig[m]));
+                }
+                cells[c] = t;
+            }
+}
+
+/* One ensemble member's shape: edge = round(exp(range(ln 16, ln 96))) and
+ * alpha = exp(range(ln 1e-5, ln 1e-3)) from Rng::for_index(seed, i). */
+void orc_workload_member(uint64_t seed, uint64_t i, double lo_edge, double hi_edge,
+                         double lo_alpha, double hi_alpha, long long *edge, double *alpha) {
+    uint64_t s = orc_rng_for_index(seed, i);
+    *edge = (long long)round(exp(orc_rng_range(&s, log(lo_edge), log(hi_edge))));
+    *alpha = exp(orc_rng_range(&s, log(lo_alpha), log(hi_alpha)));
+}

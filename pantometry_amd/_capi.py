@@ -1,0 +1,146 @@
+"""ctypes binding of ``include/pantometry_amd.h``.
+
+Loads ``libpantometry_amd.so`` from this package directory and fails loudly when it is missing:
+there is no Python or CPU stand-in for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libpantometry_amd.so"
+
+PM_OK, PM_VIOLATION, PM_ERR_DEVICE, PM_ERR_ARGUMENT, PM_ERR_STATE = 0, 1, 2, 3, 4
+
+
+class pm_violation(C.Structure):
+    _fields_ = [("quantity", C.c_char * 64), ("site", C.c_char * 160), ("before", C.c_double),
+                ("after", C.c_double), ("scale", C.c_double), ("tolerance", C.c_double),
+                ("step", C.c_int64)]
+
+
+class pm_substance(C.Structure):
+    _fields_ = [("density", C.c_double), ("conductivity", C.c_double),
+                ("specific_heat", C.c_double)]
+
+
+class pm_books(C.Structure):
+    _fields_ = [("stored", C.c_double), ("lost", C.c_double), ("supplied", C.c_double)]
+
+
+class pm_member(C.Structure):
+    _fields_ = [("nx", C.c_size_t), ("ny", C.c_size_t), ("nz", C.c_size_t), ("dx", C.c_double),
+                ("substance", pm_substance), ("initial", C.c_double), ("dt", C.c_double)]
+
+
+_D = C.c_double
+_DP = C.POINTER(C.c_double)
+_VP = C.c_void_p
+_S = C.c_size_t
+_U64 = C.c_uint64
+_I = C.c_int
+_VIOL = C.POINTER(pm_violation)
+
+# name: (restype, argtypes)
+SIGNATURES = {
+    "pm_last_error": (C.c_char_p, []),
+    "pm_device_info": (_I, [C.c_char_p, _S, C.POINTER(_I), C.POINTER(_S)]),
+    "pm_rng_new": (_U64, [_U64]),
+    "pm_rng_for_index": (_U64, [_U64, _U64]),
+    "pm_rng_unit": (_D, [C.POINTER(_U64)]),
+    "pm_rng_range": (_D, [C.POINTER(_U64), _D, _D]),
+    "pm_rng_gaussian": (_D, [C.POINTER(_U64)]),
+    "pm_rng_split": (_U64, [C.POINTER(_U64)]),
+    "pm_workload_hot_spot": (_I, [_S, _S, _S, _D, _D, _D, _D, _U64, _DP]),
+    "pm_workload_heterogeneous": (_I, [_S, _S, _S, _U64, _U64, _I, _DP, _DP]),
+    "pm_solid3d_create": (_I, [C.c_char_p, C.POINTER(pm_substance), _S, _S, _S, _D, _D,
+                               C.POINTER(_VP)]),
+    "pm_solid3d_destroy": (None, [_VP]),
+    "pm_solid3d_counts": (_I, [_VP, C.POINTER(_S)]),
+    "pm_solid3d_fill": (_I, [_VP, C.POINTER(pm_substance), C.POINTER(C.c_uint8)]),
+    "pm_solid3d_set_conductivity": (_I, [_VP, _DP]),
+    "pm_solid3d_set_cells": (_I, [_VP, _DP]),
+    "pm_solid3d_get_cells": (_I, [_VP, _DP]),
+    "pm_solid3d_set_temperature": (_I, [_VP, _S, _S, _S, _D]),
+    "pm_solid3d_deposit": (_I, [_VP, _S, _S, _S, _D]),
+    "pm_solid3d_max_stable_dt": (_D, [_VP]),
+    "pm_solid3d_stability_ratio": (_D, [_VP, _D]),
+    "pm_solid3d_refuses": (_I, [_VP, _D, _VIOL]),
+    "pm_solid3d_step": (_I, [_VP, _D, _D, _VIOL]),
+    "pm_solid3d_ledger": (_I, [_VP, C.POINTER(pm_books)]),
+    "pm_solid3d_advance": (_I, [_VP, _D, _U64, _D, _DP, _DP, _VIOL]),
+    "pm_solid3d_checkpoint": (_I, [_VP]),
+    "pm_solid3d_restore": (_I, [_VP]),
+    "pm_solid3d_readings": (_I, [_VP, _DP]),
+    "pm_solid3d_heat_capacity": (_D, [_VP]),
+    "pm_solid3d_set_tile": (_I, [_VP, _I, _I]),
+    "pm_solid3d_set_profiling": (_I, [_VP, _I]),
+    "pm_solid3d_kernel_times": (_I, [_VP, _DP, _DP]),
+    "pm_hall_create": (_I, [C.c_char_p, _D, _D, _D, _S, _D, _D, C.POINTER(_VP)]),
+    "pm_hall_destroy": (None, [_VP]),
+    "pm_hall_nodes": (_I, [_VP, C.POINTER(_S)]),
+    "pm_hall_spacing": (_D, [_VP]),
+    "pm_hall_release": (_I, [_VP, _DP]),
+    "pm_hall_set_source": (_I, [_VP, _S, _DP, _U64]),
+    "pm_hall_max_stable_dt": (_D, [_VP]),
+    "pm_hall_courant": (_D, [_VP, _D]),
+    "pm_hall_step": (_I, [_VP, _D, _VIOL]),
+    "pm_hall_energy": (_I, [_VP, _DP, _DP, _DP]),
+    "pm_hall_advance": (_I, [_VP, _D, _U64, _D, _DP, _DP, _VIOL]),
+    "pm_hall_get_fields": (_I, [_VP, _DP, _DP, _DP, _DP, _DP]),
+    "pm_hall_checkpoint": (_I, [_VP]),
+    "pm_hall_restore": (_I, [_VP]),
+    "pm_hall_peak_pressure": (_D, [_VP]),
+    "pm_ensemble_create": (_I, [C.POINTER(pm_member), _S, C.POINTER(_VP)]),
+    "pm_ensemble_destroy": (None, [_VP]),
+    "pm_ensemble_set_cells": (_I, [_VP, _S, _DP]),
+    "pm_ensemble_get_cells": (_I, [_VP, _S, _DP]),
+    "pm_ensemble_run": (_I, [_VP, _U64, _DP, _VIOL]),
+    "pm_ensemble_ledgers": (_I, [_VP, _DP]),
+    "pm_ensemble_set_schedule": (_I, [_VP, _I, _I]),
+    "pm_ensemble_tune": (_I, [_VP, C.c_char_p, _S]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The loaded library, with every declared signature attached."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is not built; run `python -m pantometry_amd.build` (or "
+                "__graft_entry__.build()) first. There is no CPU fallback.")
+        handle = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+class DeviceError(RuntimeError):
+    """A PM_ERR_* status: device, argument or state failure."""
+
+
+def last_error() -> str:
+    msg = lib().pm_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(status: int, what: str, violation: pm_violation | None = None) -> None:
+    """Raises Violation for PM_VIOLATION and DeviceError for any PM_ERR_* status."""
+    if status == PM_OK:
+        return
+    if status == PM_VIOLATION and violation is not None:
+        from .core import Violation
+        raise Violation(violation.quantity.decode(), violation.site.decode(), violation.before,
+                        violation.after, violation.scale, violation.tolerance, violation.step)
+    raise DeviceError(f"{what}: status {status}: {last_error()}")
+
+
+def dptr(a) -> "C.POINTER(C.c_double)":
+    """A double* into a C-contiguous float64 numpy array."""
+    return a.ctypes.data_as(_DP)

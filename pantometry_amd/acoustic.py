@@ -1,0 +1,227 @@
+"""Hall on the GPU: the Domain of crates/pantometry-acoustic/src/hall.rs, stepped by the gfx950
+library through its C ABI.  Pressures in pascal, lengths in metres, energies in joules."""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from typing import Callable, List, Optional, Tuple, Union
+
+import numpy as np
+
+from . import _capi
+from ._capi import check, dptr, lib
+from .core import ENERGY, Exchange, Kind, Ledger
+
+
+class Hall:
+    """A rectangular room with rigid walls on a node-centred staggered grid (hall.rs:65-90)."""
+
+    def __init__(self, name: str, width: float, height: float, depth: float, across: int,
+                 speed: float, density: float):
+        self._name = name
+        self._speed = float(speed)
+        self._density = float(density)
+        handle = C.c_void_p()
+        check(lib().pm_hall_create(name.encode(), float(width), float(height), float(depth),
+                                   int(across), self._speed, self._density, C.byref(handle)),
+              "pm_hall_create")
+        self._h = handle
+        out = (C.c_size_t * 3)()
+        check(lib().pm_hall_nodes(self._h, out), "pm_hall_nodes")
+        self._nodes = (out[0], out[1], out[2])
+        self._dx = float(lib().pm_hall_spacing(self._h))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().pm_hall_destroy(h)
+            self._h = None
+
+    @staticmethod
+    def of_air(name: str, width: float, height: float, depth: float, across: int) -> "Hall":
+        """343 m/s, 1.204 kg/m^3 (hall.rs:137-153)."""
+        return Hall(name, width, height, depth, across, 343.0, 1.204)
+
+    # ---- geometry ----------------------------------------------------------------------
+    def name(self) -> str:
+        return self._name
+
+    def nodes(self) -> Tuple[int, int, int]:
+        return self._nodes
+
+    def spacing(self) -> float:
+        return self._dx
+
+    def width(self) -> float:
+        return float(self._nodes[0] - 1) * self._dx
+
+    def height(self) -> float:
+        return float(self._nodes[1] - 1) * self._dx
+
+    def depth(self) -> float:
+        return float(self._nodes[2] - 1) * self._dx
+
+    def node_index(self, i: int, j: int, k: int) -> int:
+        nx, ny, _ = self._nodes
+        return i + nx * (j + ny * k)
+
+    # ---- release -----------------------------------------------------------------------
+    def released_from(self, profile: Union[Callable[[float, float, float], float], np.ndarray]) -> "Hall":
+        """Hall::released_from, hall.rs:166-184; the profile sees node positions (i, j, k)*dx."""
+        nx, ny, nz = self._nodes
+        if callable(profile):
+            p = np.empty((nz, ny, nx), dtype=np.float64)
+            for k in range(nz):
+                for j in range(ny):
+                    for i in range(nx):
+                        p[k, j, i] = profile(float(i) * self._dx, float(j) * self._dx,
+                                             float(k) * self._dx)
+        else:
+            p = np.ascontiguousarray(profile, dtype=np.float64).reshape(nz, ny, nx)
+        p = np.ascontiguousarray(p.reshape(-1))
+        check(lib().pm_hall_release(self._h, dptr(p)), "pm_hall_release")
+        return self
+
+    def mode_profile(self, mode: Tuple[int, int, int], amplitude: float) -> np.ndarray:
+        """The pressure of hall.rs:190-210 at every node: ((a*px)*py)*pz."""
+        nx, ny, nz = self._nodes
+        lx, ly, lz = self.width(), self.height(), self.depth()
+
+        def phase(n: int, count: int, length: float) -> np.ndarray:
+            if length > 0.0:
+                return np.array([math.cos(float(n) * math.pi * (float(i) * self._dx) / length)
+                                 for i in range(count)])
+            return np.ones(count)
+
+        px, py, pz = phase(mode[0], nx, lx), phase(mode[1], ny, ly), phase(mode[2], nz, lz)
+        a = amplitude * px
+        return (a[None, None, :] * py[None, :, None]) * pz[:, None, None]
+
+    def released_in_mode(self, mode: Tuple[int, int, int], amplitude: float) -> "Hall":
+        return self.released_from(self.mode_profile(mode, amplitude))
+
+    def set_source(self, node: int, increments: Optional[np.ndarray]) -> None:
+        """A point source: increments[n] Pa added at `node` after step n's pressure sweep, its
+        energy booked as supplied (no reference counterpart)."""
+        if increments is None or len(increments) == 0:
+            check(lib().pm_hall_set_source(self._h, 0, None, 0), "pm_hall_set_source")
+            return
+        inc = np.ascontiguousarray(increments, dtype=np.float64)
+        check(lib().pm_hall_set_source(self._h, int(node), dptr(inc), inc.size),
+              "pm_hall_set_source")
+
+    # ---- closed forms ------------------------------------------------------------------
+    def mode_frequency(self, mode: Tuple[int, int, int]) -> float:
+        """hall.rs:266-279."""
+        total = 0.0
+        for n, length in zip(mode, (self.width(), self.height(), self.depth())):
+            total += (float(n) / length) ** 2 if length > 0.0 else 0.0
+        return self._speed / 2.0 * math.sqrt(total)
+
+    def modes_below(self, limit: float) -> List[Tuple[Tuple[int, int, int], float]]:
+        """hall.rs:286-316."""
+        lengths = (self.width(), self.height(), self.depth())
+
+        def bound(length: float) -> int:
+            if length <= 0.0 or self._speed <= 0.0:
+                return 0
+            return int(math.floor(2.0 * limit * length / self._speed))
+
+        mx, my, mz = (bound(length) for length in lengths)
+        found = []
+        for a in range(mx + 1):
+            for b in range(my + 1):
+                for c in range(mz + 1):
+                    if a == b == c == 0:
+                        continue
+                    f = self.mode_frequency((a, b, c))
+                    if f <= limit:
+                        found.append(((a, b, c), f))
+        found.sort(key=lambda x: x[1])
+        return found
+
+    def courant(self, dt: float) -> float:
+        return float(lib().pm_hall_courant(self._h, float(dt)))
+
+    # ---- Domain ------------------------------------------------------------------------
+    def kind(self) -> Kind:
+        return Kind.EVOLVING
+
+    def books_balance(self) -> bool:
+        return True
+
+    def supports_restore(self) -> bool:
+        return True
+
+    def residual(self) -> float:
+        return 0.0
+
+    def max_stable_dt(self, now: float = 0.0) -> float:
+        return float(lib().pm_hall_max_stable_dt(self._h))
+
+    def step(self, t: float, dt: float, bus: Exchange) -> None:
+        v = _capi.pm_violation()
+        check(lib().pm_hall_step(self._h, float(dt), C.byref(v)), "pm_hall_step", v)
+
+    def energy_parts(self) -> Tuple[float, float, float]:
+        inv, off, sup = C.c_double(), C.c_double(), C.c_double()
+        check(lib().pm_hall_energy(self._h, C.byref(inv), C.byref(off), C.byref(sup)),
+              "pm_hall_energy")
+        return inv.value, off.value, sup.value
+
+    def energy(self) -> float:
+        """Hall::energy, hall.rs:249-251: invariant + startup offset."""
+        inv, off, _ = self.energy_parts()
+        return inv + off
+
+    def startup_adjustment(self) -> float:
+        return self.energy_parts()[1]
+
+    def supplied(self) -> float:
+        return self.energy_parts()[2]
+
+    def ledger(self) -> Ledger:
+        inv, off, sup = self.energy_parts()
+        led = Ledger().with_(ENERGY, inv + off)
+        if sup != 0.0:
+            led.add(ENERGY, -sup)
+        return led
+
+    def checkpoint(self) -> None:
+        check(lib().pm_hall_checkpoint(self._h), "pm_hall_checkpoint")
+
+    def restore(self) -> None:
+        check(lib().pm_hall_restore(self._h), "pm_hall_restore")
+
+    def fields(self):
+        """(p, pressure_prev, vx, vy, vz) as flat x-fastest arrays."""
+        nx, ny, nz = self._nodes
+        p = np.empty(nx * ny * nz)
+        pp = np.empty(nx * ny * nz)
+        vx = np.empty(max((nx - 1) * ny * nz, 1))
+        vy = np.empty(max(nx * (ny - 1) * nz, 1))
+        vz = np.empty(max(nx * ny * (nz - 1), 1))
+        check(lib().pm_hall_get_fields(self._h, dptr(p), dptr(pp), dptr(vx), dptr(vy), dptr(vz)),
+              "pm_hall_get_fields")
+        return (p, pp, vx[:(nx - 1) * ny * nz], vy[:nx * (ny - 1) * nz], vz[:nx * ny * (nz - 1)])
+
+    def pressure_at(self, i: int, j: int, k: int) -> float:
+        nx, ny, nz = self._nodes
+        p = self.fields()[0]
+        return float(p[self.node_index(min(i, nx - 1), min(j, ny - 1), min(k, nz - 1))])
+
+    def peak_pressure(self) -> float:
+        return float(lib().pm_hall_peak_pressure(self._h))
+
+    def readings(self):
+        return [(self._name, "peak", self.peak_pressure(), "Pa")]
+
+    def advance(self, dt: float, steps: int, tolerance: float = 1e-9):
+        """`steps` steps with the energy audit after each, on the device.  Returns (ledger
+        totals after each step, device milliseconds)."""
+        v = _capi.pm_violation()
+        totals = np.empty(max(steps, 1))
+        ms = C.c_double(0.0)
+        check(lib().pm_hall_advance(self._h, float(dt), int(steps), float(tolerance), dptr(totals),
+                                    C.byref(ms), C.byref(v)), "pm_hall_advance", v)
+        return totals[:steps], ms.value

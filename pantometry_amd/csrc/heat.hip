@@ -1,0 +1,920 @@
+// Solid3D on gfx950: the explicit conduction sweep of crates/pantometry-thermal/src/solid.rs
+// (plain branch), with the conservation audit fused into the sweep's store.
+//
+// Layout in HBM: x-fastest `i + nx*(j + ny*k)`, f64.  Two temperature buffers ping-pong (the
+// reference's per-step `old = cells.clone()` becomes a pointer swap).  The operator is held in
+// one of three forms, chosen by what the cells hold:
+//   uniform      one conductivity, one capacity: face k and dt*dx/C are kernel scalars (16 B/cell)
+//   conductive   per-cell conductivity, one capacity: the three new faces of each cell are
+//                recomputed with the harmonic mean in the sweep, from the per-cell k (24 B/cell)
+//   general      per-cell conductivity, mobility dx/C and capacity (40 B/cell)
+// Every form is arithmetically the reference's: faces are series(k[c-1], k[c]) as resolve
+// builds them (solid.rs:716-734), and each cell's update follows solid.rs:1776-1826 term by
+// term with contraction off.
+//
+// The sweep marches in z.  A workgroup of 64 x TY threads owns a 64 x TY column tile and a
+// chunk of z-planes; each thread keeps its own column's below/current/above temperatures and
+// conductivities in registers, stages the current xy plane (with a one-cell halo) in LDS for
+// its x and y neighbours, and carries the z face it computed for the plane above into the next
+// plane.  The audit's C*(T'-T_ref) is summed per thread as the cell is stored, reduced per
+// workgroup in a fixed order, and the per-workgroup partials are summed by a one-block finalize
+// kernel in a fixed order, which also performs audit_with on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace pm {
+
+struct HeatArgs {
+    const double *__restrict__ told;
+    double *__restrict__ tnew;
+    const double *__restrict__ kcell;  // per-cell conductivity (conductive, general)
+    const double *__restrict__ mob;    // per-cell dx/C (general)
+    const double *__restrict__ cap;    // per-cell C (general)
+    double *__restrict__ partials;     // one audit partial per workgroup
+    const AuditState *audit;           // halted flag: copy instead of step
+    double kface;                      // uniform face conductivity
+    double f;                          // dt * dx/C (uniform capacity)
+    double dts;                        // dt (general)
+    double cap0;                       // uniform capacity
+    double tref;                       // ledger reference temperature
+    int nx, ny, nz, zc;
+};
+
+enum { K_UNIFORM = 0, K_PER_CELL = 1 };
+enum { C_UNIFORM = 0, C_PER_CELL = 1 };
+
+template <int KMODE, int CMODE, int TY>
+__global__ __launch_bounds__(64 * TY) void heat_sweep(HeatArgs a) {
+    constexpr int TX = 64;
+    constexpr int NT = TX * TY;
+    __shared__ double sT[2][TY + 2][TX + 2];
+    __shared__ double sK[KMODE ? 2 : 1][KMODE ? TY + 2 : 1][KMODE ? TX + 2 : 1];
+    __shared__ double fx[KMODE ? TY : 1][KMODE ? TX + 1 : 1];
+    __shared__ double fy[KMODE ? TY + 1 : 1][KMODE ? TX : 1];
+    __shared__ double red[NT / 64];
+
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TX * ty;
+    const int64_t nx = a.nx, ny = a.ny, nz = a.nz, plane = nx * ny;
+    const int64_t x0 = (int64_t)blockIdx.x * TX, y0 = (int64_t)blockIdx.y * TY;
+    const int64_t z0 = (int64_t)blockIdx.z * a.zc, z1 = std::min<int64_t>(z0 + a.zc, nz);
+    const int64_t gi = x0 + tx, gj = y0 + ty;
+    const bool inside = gi < nx && gj < ny;
+    const int64_t col = std::min(gi, nx - 1) + nx * std::min(gj, ny - 1);
+    const int64_t bid = blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
+
+    if (a.audit->halted) {
+        // A breach froze the run: carry the state over unchanged so the ping-pong stays valid.
+        if (inside)
+            for (int64_t z = z0; z < z1; z++) a.tnew[col + plane * z] = a.told[col + plane * z];
+        if (tid == 0) a.partials[bid] = 0.0;
+        return;
+    }
+
+    // Halo role: one halo cell per thread for the first 2*TX + 2*TY threads (bottom row, top row,
+    // left column, right column), read from the clamped index so every LDS value is finite.
+    int hr = -1, hc = -1;
+    int64_t hidx = 0;
+    if (tid < TX) {
+        hr = 0, hc = tid + 1;
+        hidx = std::min(x0 + tid, nx - 1) + nx * std::max<int64_t>(y0 - 1, 0);
+    } else if (tid < 2 * TX) {
+        hr = TY + 1, hc = tid - TX + 1;
+        hidx = std::min(x0 + tid - TX, nx - 1) + nx * std::min(y0 + TY, ny - 1);
+    } else if (tid < 2 * TX + TY) {
+        hr = tid - 2 * TX + 1, hc = 0;
+        hidx = std::max<int64_t>(x0 - 1, 0) + nx * std::min(y0 + hr - 1, ny - 1);
+    } else if (tid < 2 * TX + 2 * TY) {
+        hr = tid - 2 * TX - TY + 1, hc = TX + 1;
+        hidx = std::min(x0 + TX, nx - 1) + nx * std::min(y0 + hr - 1, ny - 1);
+    }
+    const bool halo = hr >= 0;
+
+    const double *__restrict__ T = a.told;
+    const double *__restrict__ K = a.kcell;
+    auto zcl = [&](int64_t z) { return std::min(std::max<int64_t>(z, 0), nz - 1); };
+
+    double t_m = T[col + plane * zcl(z0 - 1)];
+    double t_c = T[col + plane * z0];
+    double t_p = T[col + plane * zcl(z0 + 1)];
+    double h_t = halo ? T[hidx + plane * z0] : 0.0;
+    double k_c = 0.0, k_p = 0.0, h_k = 0.0, kz_lo = 0.0;
+    if (KMODE == K_PER_CELL) {
+        double k_m = K[col + plane * zcl(z0 - 1)];
+        k_c = K[col + plane * z0];
+        k_p = K[col + plane * zcl(z0 + 1)];
+        h_k = halo ? K[hidx + plane * z0] : 0.0;
+        kz_lo = z0 > 0 ? series(k_m, k_c) : 0.0;
+    }
+
+    // Uniform-form faces depend only on position.
+    const double kxl_u = (gi > 0 && gi < nx) ? a.kface : 0.0;
+    const double kxh_u = (gi + 1 < nx) ? a.kface : 0.0;
+    const double kyl_u = (gj > 0 && gj < ny) ? a.kface : 0.0;
+    const double kyh_u = (gj + 1 < ny) ? a.kface : 0.0;
+
+    double acc = 0.0;
+    for (int64_t z = z0; z < z1; z++) {
+        const int buf = (int)((z - z0) & 1);
+        // Prefetch: own column two planes ahead, halo one plane ahead.
+        const int64_t zpp = zcl(z + 2), zn = zcl(z + 1);
+        double t_pp = T[col + plane * zpp];
+        double h_t_n = halo ? T[hidx + plane * zn] : 0.0;
+        double k_pp = 0.0, h_k_n = 0.0;
+        if (KMODE == K_PER_CELL) {
+            k_pp = K[col + plane * zpp];
+            h_k_n = halo ? K[hidx + plane * zn] : 0.0;
+        }
+
+        sT[buf][ty + 1][tx + 1] = t_c;
+        if (halo) sT[buf][hr][hc] = h_t;
+        if (KMODE == K_PER_CELL) {
+            sK[buf][ty + 1][tx + 1] = k_c;
+            if (halo) sK[buf][hr][hc] = h_k;
+        }
+        __syncthreads();
+
+        double kxl, kxh, kyl, kyh, kzl, kzh;
+        if (KMODE == K_PER_CELL) {
+            // Each thread builds its own low x and y faces; wave 0 adds the tile's right-edge x
+            // faces and the last wave the top-edge y faces.  Face g along an axis of n cells
+            // lies between cells g-1 and g and is zero unless 0 < g < n.
+            fx[ty][tx] = (gi > 0 && gi < nx) ? series(sK[buf][ty + 1][tx], sK[buf][ty + 1][tx + 1]) : 0.0;
+            fy[ty][tx] = (gj > 0 && gj < ny) ? series(sK[buf][ty][tx + 1], sK[buf][ty + 1][tx + 1]) : 0.0;
+            if (tid < TY) {
+                const int r = tid;
+                const int64_t g = x0 + TX;
+                fx[r][TX] = (g < nx) ? series(sK[buf][r + 1][TX], sK[buf][r + 1][TX + 1]) : 0.0;
+            }
+            if (ty == TY - 1) {
+                const int64_t g = y0 + TY;
+                fy[TY][tx] = (g < ny) ? series(sK[buf][TY][tx + 1], sK[buf][TY + 1][tx + 1]) : 0.0;
+            }
+            kzh = (z + 1 < nz) ? series(k_c, k_p) : 0.0;
+            kzl = kz_lo;
+            __syncthreads();
+            kxl = fx[ty][tx], kxh = fx[ty][tx + 1];
+            kyl = fy[ty][tx], kyh = fy[ty + 1][tx];
+        } else {
+            kxl = kxl_u, kxh = kxh_u, kyl = kyl_u, kyh = kyh_u;
+            kzl = z > 0 ? a.kface : 0.0;
+            kzh = z + 1 < nz ? a.kface : 0.0;
+        }
+
+        const double tw = sT[buf][ty + 1][tx], te = sT[buf][ty + 1][tx + 2];
+        const double ts = sT[buf][ty][tx + 1], tn = sT[buf][ty + 2][tx + 1];
+        // solid.rs:1781-1798, in the reference's order: x-, x+, y-, y+, z-, z+.
+        double flux = 0.0;
+        flux = flux + kxl * (tw - t_c);
+        flux = flux + kxh * (te - t_c);
+        flux = flux + kyl * (ts - t_c);
+        flux = flux + kyh * (tn - t_c);
+        flux = flux + kzl * (t_m - t_c);
+        flux = flux + kzh * (t_p - t_c);
+        if (inside) {
+            const int64_t c = col + plane * z;
+            double rise;
+            if (CMODE == C_PER_CELL)
+                rise = (a.dts * a.mob[c]) * flux;
+            else
+                rise = a.f * flux;
+            const double t_new = t_c + rise;
+            a.tnew[c] = t_new;
+            const double cap = (CMODE == C_PER_CELL) ? a.cap[c] : a.cap0;
+            acc += cap * (t_new - a.tref);
+        }
+
+        t_m = t_c, t_c = t_p, t_p = t_pp, h_t = h_t_n;
+        if (KMODE == K_PER_CELL) k_c = k_p, k_p = k_pp, h_k = h_k_n, kz_lo = kzh;
+    }
+
+    const double total = block_sum<NT>(acc, red);
+    if (tid == 0) a.partials[bid] = total;
+}
+
+// Ledger total and audit of the step that just ran: stored = sum of partials; the ledger is
+// (0 + stored) + lost + (-supplied) with scale max(|stored|, |lost|, |supplied|).
+__global__ __launch_bounds__(1024) void heat_finalize(const double *partials, int64_t n,
+                                                      AuditState *st, double lost,
+                                                      double supplied, int64_t step,
+                                                      double *totals, double *stored_out) {
+    __shared__ double scratch[16];
+    if (step >= 0 && st->halted) return;
+    const double stored = fixed_order_sum_1024(partials, n, scratch);
+    if (threadIdx.x == 0) {
+        double total = 0.0;
+        total += stored;
+        total += lost;
+        total += -supplied;
+        double scale = rmax(rmax(rmax(0.0, std::fabs(stored)), std::fabs(lost)), std::fabs(supplied));
+        if (stored_out) *stored_out = stored;
+        if (step >= 0) audit_step(st, total, scale, step, totals);
+    }
+}
+
+// The per-cell rate mob*face_sum of resolve (solid.rs:757-776), with face_sum summed in the
+// reference's order; per-block maximum (Rust f64::max) and a NaN count.
+__global__ __launch_bounds__(256) void heat_rates(const double *kcell, const double *mob,
+                                                  double kface, double mob0, int kmode,
+                                                  int cmode, int nx, int ny, int nz,
+                                                  double *block_max, int *block_nan) {
+    __shared__ double smax[256];
+    __shared__ int snan[256];
+    const int64_t n = (int64_t)nx * ny * nz;
+    double worst = 0.0;
+    int nan = 0;
+    for (int64_t c = blockIdx.x * 256 + threadIdx.x; c < n; c += (int64_t)gridDim.x * 256) {
+        const int64_t i = c % nx, j = (c / nx) % ny, k = c / ((int64_t)nx * ny);
+        double f[6];
+        if (kmode == K_PER_CELL) {
+            const double kc = kcell[c];
+            f[0] = i > 0 ? series(kcell[c - 1], kc) : 0.0;
+            f[1] = i + 1 < nx ? series(kc, kcell[c + 1]) : 0.0;
+            f[2] = j > 0 ? series(kcell[c - nx], kc) : 0.0;
+            f[3] = j + 1 < ny ? series(kc, kcell[c + nx]) : 0.0;
+            f[4] = k > 0 ? series(kcell[c - (int64_t)nx * ny], kc) : 0.0;
+            f[5] = k + 1 < nz ? series(kc, kcell[c + (int64_t)nx * ny]) : 0.0;
+        } else {
+            f[0] = i > 0 ? kface : 0.0;
+            f[1] = i + 1 < nx ? kface : 0.0;
+            f[2] = j > 0 ? kface : 0.0;
+            f[3] = j + 1 < ny ? kface : 0.0;
+            f[4] = k > 0 ? kface : 0.0;
+            f[5] = k + 1 < nz ? kface : 0.0;
+        }
+        const double sum = f[0] + f[1] + f[2] + f[3] + f[4] + f[5];
+        const double m = cmode == C_PER_CELL ? mob[c] : mob0;
+        const double rate = m * sum;
+        if (std::isnan(rate))
+            nan++;
+        else
+            worst = rmax(worst, rate);
+    }
+    smax[threadIdx.x] = worst;
+    snan[threadIdx.x] = nan;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            smax[threadIdx.x] = rmax(smax[threadIdx.x], smax[threadIdx.x + s]);
+            snan[threadIdx.x] += snan[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) block_max[blockIdx.x] = smax[0], block_nan[blockIdx.x] = snan[0];
+}
+
+// fill: cells under the mask take the substance; mobility = dx/C (solid.rs:764).
+__global__ void heat_fill(const uint8_t *mask, int64_t n, double *kcell, double *cap, double *mob,
+                          double k, double c, double dx) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        if (!mask[i]) continue;
+        if (kcell) kcell[i] = k;
+        if (cap) cap[i] = c;
+        if (mob) mob[i] = dx / c;
+    }
+}
+
+__global__ void fill_value(double *p, int64_t n, double v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// A uniform rise on every cell: the placeless bus heat of solid.rs:1752-1765.
+__global__ void add_uniform(double *p, int64_t n, double rise) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] += rise;
+}
+
+__global__ void audit_reset(AuditState *st, double total, double scale, double tol) {
+    st->prev_total = total;
+    st->prev_scale = scale;
+    st->tolerance = tol;
+    st->halted = 0;
+    st->breach_step = -1;
+    st->before = st->after = st->scale = 0.0;
+}
+
+static inline int grid_for(int64_t n, int threads) {
+    int64_t g = (n + threads - 1) / threads;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(g, 65536));
+}
+
+}  // namespace pm
+
+using namespace pm;
+
+struct pm_solid3d {
+    std::string name;
+    int64_t nx = 1, ny = 1, nz = 1;
+    double dx = 0.0;
+    double reference = 0.0;  // the ledger's enthalpy reference, the constructor's temperature
+    double absorbed = 0.0;
+    double lost = 0.0, supplied = 0.0;          // zero on the plain branch, kept for the ledger
+    double saved_lost = 0.0, saved_supplied = 0.0;
+    // operator
+    int kmode = K_UNIFORM, cmode = C_UNIFORM;
+    double k0 = 0.0, cap0 = 0.0, mob0 = 0.0, kface = 0.0;
+    double worst_rate = 0.0;
+    double heat_capacity_sum = 0.0;
+    bool heat_capacity_valid = false;
+    // device state
+    hipStream_t stream = nullptr;
+    DevBuf t[2];
+    int cur = 0;
+    DevBuf kcell, cap, mob, saved;
+    bool saved_valid = false;
+    DevBuf partials, audit, scalars, stored_dev;
+    int64_t partial_count = 0;
+    int tile_y = 8, z_chunk = 32;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // Per-launch events of a profiled advance: [3*s] before the sweep, [3*s+1] between sweep and
+    // audit, [3*s+2] after the audit.
+    bool profiling = false;
+    std::vector<hipEvent_t> prof;
+    int64_t prof_used = 0;
+    double sweep_ms = 0.0, audit_ms = 0.0;
+
+    int64_t n() const { return nx * ny * nz; }
+    double *T() const { return t[cur].d(); }
+};
+
+namespace pm {
+
+static int heat_grid(const pm_solid3d *s, dim3 *grid, dim3 *block) {
+    *block = dim3(64, (unsigned)s->tile_y, 1);
+    *grid = dim3((unsigned)((s->nx + 63) / 64), (unsigned)((s->ny + s->tile_y - 1) / s->tile_y),
+                 (unsigned)((s->nz + s->z_chunk - 1) / s->z_chunk));
+    return PM_OK;
+}
+
+static int ensure_partials(pm_solid3d *s) {
+    dim3 g, b;
+    heat_grid(s, &g, &b);
+    int64_t need = (int64_t)g.x * g.y * g.z;
+    if (s->partial_count < need || !s->partials) {
+        PM_CHECK(s->partials.alloc(sizeof(double) * (size_t)need));
+        s->partial_count = need;
+    }
+    return PM_OK;
+}
+
+template <int KM, int CM>
+static void launch_sweep_t(const pm_solid3d *s, const HeatArgs &a, dim3 g) {
+    switch (s->tile_y) {
+        case 4: heat_sweep<KM, CM, 4><<<g, dim3(64, 4), 0, s->stream>>>(a); break;
+        case 16: heat_sweep<KM, CM, 16><<<g, dim3(64, 16), 0, s->stream>>>(a); break;
+        default: heat_sweep<KM, CM, 8><<<g, dim3(64, 8), 0, s->stream>>>(a); break;
+    }
+}
+
+// One sweep from t[cur] into t[cur^1] plus the finalize/audit; then swaps.
+static int sweep_once(pm_solid3d *s, double dt, int64_t step, double *totals_dev) {
+    PM_CHECK(ensure_partials(s));
+    dim3 g, b;
+    heat_grid(s, &g, &b);
+    HeatArgs a;
+    a.told = s->t[s->cur].d();
+    a.tnew = s->t[s->cur ^ 1].d();
+    a.kcell = s->kcell.d();
+    a.mob = s->mob.d();
+    a.cap = s->cap.d();
+    a.partials = s->partials.d();
+    a.audit = s->audit.as<AuditState>();
+    a.kface = s->kface;
+    a.f = dt * s->mob0;
+    a.dts = dt;
+    a.cap0 = s->cap0;
+    a.tref = s->reference;
+    a.nx = (int)s->nx, a.ny = (int)s->ny, a.nz = (int)s->nz, a.zc = s->z_chunk;
+    const bool prof = s->profiling && step >= 0 && 3 * (step + 1) <= (int64_t)s->prof.size();
+    if (prof) PM_HIP(hipEventRecord(s->prof[3 * step], s->stream));
+    if (s->kmode == K_UNIFORM && s->cmode == C_UNIFORM)
+        launch_sweep_t<K_UNIFORM, C_UNIFORM>(s, a, g);
+    else if (s->cmode == C_UNIFORM)
+        launch_sweep_t<K_PER_CELL, C_UNIFORM>(s, a, g);
+    else
+        launch_sweep_t<K_PER_CELL, C_PER_CELL>(s, a, g);
+    PM_HIP(hipGetLastError());
+    if (prof) PM_HIP(hipEventRecord(s->prof[3 * step + 1], s->stream));
+    heat_finalize<<<1, 1024, 0, s->stream>>>(s->partials.d(), (int64_t)g.x * g.y * g.z,
+                                             s->audit.as<AuditState>(), s->lost, s->supplied,
+                                             step, totals_dev, s->stored_dev.d());
+    PM_HIP(hipGetLastError());
+    if (prof) {
+        PM_HIP(hipEventRecord(s->prof[3 * step + 2], s->stream));
+        s->prof_used = std::max<int64_t>(s->prof_used, step + 1);
+    }
+    s->cur ^= 1;
+    return PM_OK;
+}
+
+// Mean sweep and audit durations over the profiled steps of the last advance.
+static int collect_profile(pm_solid3d *s) {
+    double sw = 0.0, au = 0.0;
+    for (int64_t n = 0; n < s->prof_used; n++) {
+        float a = 0.0f, b = 0.0f;
+        PM_HIP(hipEventElapsedTime(&a, s->prof[3 * n], s->prof[3 * n + 1]));
+        PM_HIP(hipEventElapsedTime(&b, s->prof[3 * n + 1], s->prof[3 * n + 2]));
+        sw += a, au += b;
+    }
+    s->sweep_ms = s->prof_used ? sw / (double)s->prof_used : 0.0;
+    s->audit_ms = s->prof_used ? au / (double)s->prof_used : 0.0;
+    return PM_OK;
+}
+
+// Rebuild worst_rate after the operator changed (resolve, solid.rs:757-782).
+static int resolve(pm_solid3d *s) {
+    s->heat_capacity_valid = false;
+    if (s->kmode == K_UNIFORM && s->cmode == C_UNIFORM) {
+        // Every cell's rate is one of the 27 boundary classes; evaluate one cell of each.
+        double worst = 0.0;
+        bool nan = false;
+        auto reps = [](int64_t n) {
+            std::vector<int64_t> r{0};
+            if (n > 2) r.push_back(1);
+            if (n > 1) r.push_back(n - 1);
+            return r;
+        };
+        for (int64_t k : reps(s->nz))
+            for (int64_t j : reps(s->ny))
+                for (int64_t i : reps(s->nx)) {
+                    double f0 = i > 0 ? s->kface : 0.0, f1 = i + 1 < s->nx ? s->kface : 0.0;
+                    double f2 = j > 0 ? s->kface : 0.0, f3 = j + 1 < s->ny ? s->kface : 0.0;
+                    double f4 = k > 0 ? s->kface : 0.0, f5 = k + 1 < s->nz ? s->kface : 0.0;
+                    double sum = f0 + f1 + f2 + f3 + f4 + f5;
+                    double rate = s->mob0 * sum;
+                    if (std::isnan(rate))
+                        nan = true;
+                    else
+                        worst = rmax(worst, rate);
+                }
+        s->worst_rate = nan ? NAN : worst;
+        return PM_OK;
+    }
+    const int blocks = 1024;
+    DevBuf bm, bn;
+    PM_CHECK(bm.alloc(sizeof(double) * blocks));
+    PM_CHECK(bn.alloc(sizeof(int) * blocks));
+    heat_rates<<<blocks, 256, 0, s->stream>>>(s->kcell.d(), s->mob.d(), s->kface, s->mob0,
+                                              s->kmode, s->cmode, (int)s->nx, (int)s->ny,
+                                              (int)s->nz, bm.d(), bn.as<int>());
+    PM_HIP(hipGetLastError());
+    std::vector<double> hm(blocks);
+    std::vector<int> hn(blocks);
+    PM_HIP(hipMemcpyAsync(hm.data(), bm.d(), sizeof(double) * blocks, hipMemcpyDeviceToHost, s->stream));
+    PM_HIP(hipMemcpyAsync(hn.data(), bn.d(), sizeof(int) * blocks, hipMemcpyDeviceToHost, s->stream));
+    PM_HIP(hipStreamSynchronize(s->stream));
+    double worst = 0.0;
+    bool nan = false;
+    for (int b = 0; b < blocks; b++) {
+        worst = rmax(worst, hm[b]);
+        nan = nan || hn[b] > 0;
+    }
+    s->worst_rate = nan ? NAN : worst;
+    return PM_OK;
+}
+
+static double capacity_of(const pm_substance *sub, double dx) {
+    // Substance::heat_capacity: (rho * volume) * c_p, volume = (dx*dx)*dx.
+    double volume = dx * dx * dx;
+    return (sub->density * volume) * sub->specific_heat;
+}
+
+static bool same_bits(double a, double b) { return std::memcmp(&a, &b, sizeof a) == 0; }
+
+static int promote_k(pm_solid3d *s) {
+    if (s->kmode == K_PER_CELL) return PM_OK;
+    PM_CHECK(s->kcell.alloc(sizeof(double) * (size_t)s->n()));
+    fill_value<<<grid_for(s->n(), 256), 256, 0, s->stream>>>(s->kcell.d(), s->n(), s->k0);
+    PM_HIP(hipGetLastError());
+    s->kmode = K_PER_CELL;
+    return PM_OK;
+}
+
+static int promote_c(pm_solid3d *s) {
+    if (s->cmode == C_PER_CELL) return PM_OK;
+    PM_CHECK(promote_k(s));
+    PM_CHECK(s->cap.alloc(sizeof(double) * (size_t)s->n()));
+    PM_CHECK(s->mob.alloc(sizeof(double) * (size_t)s->n()));
+    fill_value<<<grid_for(s->n(), 256), 256, 0, s->stream>>>(s->cap.d(), s->n(), s->cap0);
+    fill_value<<<grid_for(s->n(), 256), 256, 0, s->stream>>>(s->mob.d(), s->n(), s->mob0);
+    PM_HIP(hipGetLastError());
+    s->cmode = C_PER_CELL;
+    return PM_OK;
+}
+
+// Heat capacity of one cell (host read of the device copy in the general form).
+static int cell_capacity(pm_solid3d *s, int64_t idx, double *out) {
+    if (s->cmode == C_UNIFORM) {
+        *out = s->cap0;
+        return PM_OK;
+    }
+    PM_HIP(hipMemcpyAsync(out, s->cap.d() + idx, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    PM_HIP(hipStreamSynchronize(s->stream));
+    return PM_OK;
+}
+
+static int total_capacity(pm_solid3d *s, double *out) {
+    if (!s->heat_capacity_valid) {
+        // `self.capacity.iter().sum::<f64>()`: a left fold in index order, on the host.
+        double total = 0.0;
+        if (s->cmode == C_UNIFORM) {
+            for (int64_t c = 0; c < s->n(); c++) total += s->cap0;
+        } else {
+            std::vector<double> h((size_t)s->n());
+            PM_HIP(hipMemcpyAsync(h.data(), s->cap.d(), sizeof(double) * h.size(), hipMemcpyDeviceToHost, s->stream));
+            PM_HIP(hipStreamSynchronize(s->stream));
+            for (double v : h) total += v;
+        }
+        s->heat_capacity_sum = total;
+        s->heat_capacity_valid = true;
+    }
+    *out = s->heat_capacity_sum;
+    return PM_OK;
+}
+
+static int check_ratio(const pm_solid3d *s, double dt, pm_violation *v, int64_t step) {
+    const double ratio = dt * s->worst_rate;
+    std::string site = s->name + " (explicit 3D conduction)";
+    if (std::isnan(ratio)) {
+        violation_at(v, s->name.c_str(), "substance has no diffusivity", INFINITY, step);
+        return PM_VIOLATION;
+    }
+    if (ratio > 1.0 + 6e-12) {
+        const double F = 1.0 / 6.0;
+        fill_violation(v, "Fourier number", site.c_str(), F, ratio * F, F, 1e-12, step);
+        return PM_VIOLATION;
+    }
+    return PM_OK;
+}
+
+__global__ __launch_bounds__(256) void heat_stored_kernel(const double *T, const double *cap,
+                                                          double cap0, double tref, int64_t n,
+                                                          int64_t per_block, double *partials) {
+    __shared__ double red[4];
+    const int64_t lo = blockIdx.x * per_block, hi = std::min(n, lo + per_block);
+    double acc = 0.0;
+    for (int64_t c = lo + threadIdx.x; c < hi; c += 256) acc += (cap ? cap[c] : cap0) * (T[c] - tref);
+    double total = block_sum<256>(acc, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = total;
+}
+
+int heat_stored(pm_solid3d *s, double *stored) {
+    const int64_t n = s->n();
+    const int64_t per_block = 16384;
+    const int64_t blocks = (n + per_block - 1) / per_block;
+    DevBuf part;
+    PM_CHECK(part.alloc(sizeof(double) * (size_t)blocks));
+    heat_stored_kernel<<<(unsigned)blocks, 256, 0, s->stream>>>(
+        s->T(), s->cmode == C_PER_CELL ? s->cap.d() : nullptr, s->cap0, s->reference, n,
+        per_block, part.d());
+    PM_HIP(hipGetLastError());
+    heat_finalize<<<1, 1024, 0, s->stream>>>(part.d(), blocks, s->audit.as<AuditState>(), 0.0,
+                                             0.0, -1, nullptr, s->stored_dev.d());
+    PM_HIP(hipGetLastError());
+    PM_HIP(hipMemcpyAsync(stored, s->stored_dev.d(), sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    PM_HIP(hipStreamSynchronize(s->stream));
+    return PM_OK;
+}
+
+static int reset_audit(pm_solid3d *s, double total, double scale, double tol) {
+    audit_reset<<<1, 1, 0, s->stream>>>(s->audit.as<AuditState>(), total, scale, tol);
+    PM_HIP(hipGetLastError());
+    return PM_OK;
+}
+
+}  // namespace pm
+
+extern "C" {
+
+int pm_solid3d_create(const char *name, const pm_substance *substance, size_t nx, size_t ny,
+                      size_t nz, double dx, double initial, pm_solid3d **out) {
+    if (!out || !substance) {
+        set_error("pm_solid3d_create: null substance or output");
+        return PM_ERR_ARGUMENT;
+    }
+    *out = nullptr;
+    PM_CHECK(require_device());
+    nx = std::max<size_t>(nx, 1), ny = std::max<size_t>(ny, 1), nz = std::max<size_t>(nz, 1);
+    if (nx > (1u << 30) || ny > (1u << 30) || nz > (1u << 30) || nx * ny > ((size_t)1 << 31)) {
+        set_error("pm_solid3d_create: counts (%zu, %zu, %zu) are beyond the sweep's indexing", nx, ny, nz);
+        return PM_ERR_ARGUMENT;
+    }
+    std::unique_ptr<pm_solid3d> s(new pm_solid3d());
+    s->name = name ? name : "block";
+    s->nx = (int64_t)nx, s->ny = (int64_t)ny, s->nz = (int64_t)nz;
+    s->dx = dx;
+    s->reference = initial;
+    // resolve, solid.rs:604-626 and 704-782, for one material.
+    s->k0 = substance->conductivity;
+    s->cap0 = capacity_of(substance, dx);
+    s->mob0 = dx / s->cap0;
+    {
+        double sum = s->k0 + s->k0;
+        s->kface = sum > 0.0 ? 2.0 * s->k0 * s->k0 / sum : 0.0;
+    }
+    PM_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    PM_HIP(hipEventCreate(&s->ev0));
+    PM_HIP(hipEventCreate(&s->ev1));
+    const size_t bytes = sizeof(double) * (size_t)s->n();
+    PM_CHECK(s->t[0].alloc(bytes));
+    PM_CHECK(s->t[1].alloc(bytes));
+    PM_CHECK(s->audit.alloc(sizeof(AuditState)));
+    PM_CHECK(s->stored_dev.alloc(sizeof(double)));
+    fill_value<<<grid_for(s->n(), 256), 256, 0, s->stream>>>(s->t[0].d(), s->n(), initial);
+    PM_HIP(hipGetLastError());
+    PM_CHECK(reset_audit(s.get(), 0.0, 0.0, 1e-9));
+    PM_CHECK(resolve(s.get()));
+    PM_HIP(hipStreamSynchronize(s->stream));
+    *out = s.release();
+    return PM_OK;
+}
+
+void pm_solid3d_destroy(pm_solid3d *s) {
+    if (!s) return;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    for (hipEvent_t ev : s->prof) (void)hipEventDestroy(ev);
+    hipStream_t st = s->stream;
+    delete s;
+    if (st) (void)hipStreamDestroy(st);
+}
+
+int pm_solid3d_counts(const pm_solid3d *s, size_t counts[3]) {
+    if (!s || !counts) return PM_ERR_ARGUMENT;
+    counts[0] = (size_t)s->nx, counts[1] = (size_t)s->ny, counts[2] = (size_t)s->nz;
+    return PM_OK;
+}
+
+int pm_solid3d_fill(pm_solid3d *s, const pm_substance *sub, const uint8_t *mask) {
+    if (!s || !sub || !mask) {
+        set_error("pm_solid3d_fill: null argument");
+        return PM_ERR_ARGUMENT;
+    }
+    const double k = sub->conductivity;
+    const double c = capacity_of(sub, s->dx);
+    const int64_t n = s->n();
+    bool any = false;
+    for (int64_t i = 0; i < n && !any; i++) any = mask[i] != 0;
+    if (!any) return PM_OK;
+    bool all = true;
+    for (int64_t i = 0; i < n && all; i++) all = mask[i] != 0;
+    const bool k_same = s->kmode == K_UNIFORM && same_bits(k, s->k0);
+    const bool c_same = s->cmode == C_UNIFORM && same_bits(c, s->cap0);
+    if (all && (s->kmode == K_UNIFORM) && (s->cmode == C_UNIFORM)) {
+        // The whole block becomes the new substance: still one material.
+        s->k0 = k, s->cap0 = c, s->mob0 = s->dx / c;
+        double sum = k + k;
+        s->kface = sum > 0.0 ? 2.0 * k * k / sum : 0.0;
+        return resolve(s);
+    }
+    if (k_same && c_same) return PM_OK;
+    if (!c_same) PM_CHECK(promote_c(s));
+    if (!k_same) PM_CHECK(promote_k(s));
+    DevBuf dmask;
+    PM_CHECK(dmask.alloc((size_t)n));
+    PM_HIP(hipMemcpyAsync(dmask.d(), mask, (size_t)n, hipMemcpyHostToDevice, s->stream));
+    heat_fill<<<grid_for(n, 256), 256, 0, s->stream>>>(
+        dmask.as<uint8_t>(), n, s->kmode == K_PER_CELL ? s->kcell.d() : nullptr,
+        s->cmode == C_PER_CELL ? s->cap.d() : nullptr,
+        s->cmode == C_PER_CELL ? s->mob.d() : nullptr, k, c, s->dx);
+    PM_HIP(hipGetLastError());
+    PM_CHECK(resolve(s));
+    PM_HIP(hipStreamSynchronize(s->stream));
+    return PM_OK;
+}
+
+int pm_solid3d_set_conductivity(pm_solid3d *s, const double *k) {
+    if (!s || !k) return PM_ERR_ARGUMENT;
+    if (s->kmode != K_PER_CELL) {
+        PM_CHECK(s->kcell.alloc(sizeof(double) * (size_t)s->n()));
+        s->kmode = K_PER_CELL;
+    }
+    PM_HIP(hipMemcpyAsync(s->kcell.d(), k, sizeof(double) * (size_t)s->n(), hipMemcpyHostToDevice, s->stream));
+    PM_CHECK(resolve(s));
+    PM_HIP(hipStreamSynchronize(s->stream));
+    return PM_OK;
+}
+
+int pm_solid3d_set_cells(pm_solid3d *s, const double *cells) {
+    if (!s || !cells) return PM_ERR_ARGUMENT;
+    PM_HIP(hipMemcpyAsync(s->T(), cells, sizeof(double) * (size_t)s->n(), hipMemcpyHostToDevice, s->stream));
+    PM_HIP(hipStreamSynchronize(s->stream));
+    return PM_OK;
+}
+
+int pm_solid3d_get_cells(pm_solid3d *s, double *cells) {
+    if (!s || !cells) return PM_ERR_ARGUMENT;
+    PM_HIP(hipMemcpyAsync(cells, s->T(), sizeof(double) * (size_t)s->n(), hipMemcpyDeviceToHost, s->stream));
+    PM_HIP(hipStreamSynchronize(s->stream));
+    return PM_OK;
+}
+
+int pm_solid3d_set_temperature(pm_solid3d *s, size_t i, size_t j, size_t k, double kelvin) {
+    if (!s) return PM_ERR_ARGUMENT;
+    if ((int64_t)i >= s->nx || (int64_t)j >= s->ny || (int64_t)k >= s->nz) return PM_OK;  // ignored, solid.rs:1346
+    const int64_t idx = (int64_t)i + s->nx * ((int64_t)j + s->ny * (int64_t)k);
+    PM_HIP(hipMemcpyAsync(s->T() + idx, &kelvin, sizeof(double), hipMemcpyHostToDevice, s->stream));
+    PM_HIP(hipStreamSynchronize(s->stream));
+    return PM_OK;
+}
+
+int pm_solid3d_deposit(pm_solid3d *s, size_t i, size_t j, size_t k, double joules) {
+    if (!s) return PM_ERR_ARGUMENT;
+    if ((int64_t)i >= s->nx || (int64_t)j >= s->ny || (int64_t)k >= s->nz) return PM_OK;  // refused, solid.rs:1368
+    const int64_t idx = (int64_t)i + s->nx * ((int64_t)j + s->ny * (int64_t)k);
+    double cap = 0.0, t = 0.0;
+    PM_CHECK(cell_capacity(s, idx, &cap));
+    PM_HIP(hipMemcpyAsync(&t, s->T() + idx, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    PM_HIP(hipStreamSynchronize(s->stream));
+    const double rise = joules / cap;
+    t += rise;  // add_kelvin, latent <= 0 (solid.rs:1634)
+    PM_HIP(hipMemcpyAsync(s->T() + idx, &t, sizeof(double), hipMemcpyHostToDevice, s->stream));
+    PM_HIP(hipStreamSynchronize(s->stream));
+    s->absorbed += joules;
+    return PM_OK;
+}
+
+double pm_solid3d_max_stable_dt(const pm_solid3d *s) {
+    if (!s) return NAN;
+    if (std::isnan(s->worst_rate)) return INFINITY;
+    return 1.0 / s->worst_rate;
+}
+
+double pm_solid3d_stability_ratio(const pm_solid3d *s, double dt) {
+    if (!s) return NAN;
+    return dt * s->worst_rate;
+}
+
+int pm_solid3d_refuses(const pm_solid3d *s, double dt, pm_violation *v) {
+    if (!s) return PM_ERR_ARGUMENT;
+    return check_ratio(s, dt, v, -1);
+}
+
+int pm_solid3d_step(pm_solid3d *s, double dt, double gained, pm_violation *v) {
+    if (!s) return PM_ERR_ARGUMENT;
+    PM_CHECK(check_ratio(s, dt, v, -1));
+    if (gained != 0.0) {
+        s->absorbed += gained;
+        double total = 0.0;
+        PM_CHECK(total_capacity(s, &total));
+        const double rise = gained / total;
+        add_uniform<<<grid_for(s->n(), 256), 256, 0, s->stream>>>(s->T(), s->n(), rise);
+        PM_HIP(hipGetLastError());
+    }
+    PM_CHECK(reset_audit(s, 0.0, 0.0, INFINITY));
+    PM_CHECK(sweep_once(s, dt, -1, nullptr));
+    PM_HIP(hipStreamSynchronize(s->stream));
+    return PM_OK;
+}
+
+int pm_solid3d_ledger(pm_solid3d *s, pm_books *books) {
+    if (!s || !books) return PM_ERR_ARGUMENT;
+    double stored = 0.0;
+    PM_CHECK(heat_stored(s, &stored));
+    books->stored = stored;
+    books->lost = s->lost;
+    books->supplied = s->supplied;
+    return PM_OK;
+}
+
+int pm_solid3d_advance(pm_solid3d *s, double dt, uint64_t steps, double tolerance,
+                       double *totals, double *elapsed_ms, pm_violation *v) {
+    if (!s) return PM_ERR_ARGUMENT;
+    PM_CHECK(check_ratio(s, dt, v, 0));
+    // The opening ledger.
+    double stored0 = 0.0;
+    PM_CHECK(heat_stored(s, &stored0));
+    double total0 = 0.0;
+    total0 += stored0;
+    total0 += s->lost;
+    total0 += -s->supplied;
+    const double scale0 = rmax(rmax(rmax(0.0, std::fabs(stored0)), std::fabs(s->lost)), std::fabs(s->supplied));
+    PM_CHECK(reset_audit(s, total0, scale0, tolerance));
+    DevBuf dtot;
+    if (totals) {
+        PM_CHECK(dtot.alloc(sizeof(double) * (size_t)std::max<uint64_t>(steps, 1)));
+        PM_HIP(hipMemsetAsync(dtot.d(), 0xFF, sizeof(double) * (size_t)std::max<uint64_t>(steps, 1), s->stream));
+    }
+    if (s->profiling) {
+        while (s->prof.size() < 3 * (size_t)steps) {
+            hipEvent_t ev;
+            PM_HIP(hipEventCreate(&ev));
+            s->prof.push_back(ev);
+        }
+        s->prof_used = 0;
+    }
+    PM_HIP(hipEventRecord(s->ev0, s->stream));
+    for (uint64_t n = 0; n < steps; n++) PM_CHECK(sweep_once(s, dt, (int64_t)n, totals ? dtot.d() : nullptr));
+    PM_HIP(hipEventRecord(s->ev1, s->stream));
+    AuditState st;
+    PM_HIP(hipMemcpyAsync(&st, s->audit.d(), sizeof st, hipMemcpyDeviceToHost, s->stream));
+    if (totals && steps) PM_HIP(hipMemcpyAsync(totals, dtot.d(), sizeof(double) * steps, hipMemcpyDeviceToHost, s->stream));
+    PM_HIP(hipStreamSynchronize(s->stream));
+    if (elapsed_ms) {
+        float ms = 0.0f;
+        PM_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        *elapsed_ms = ms;
+    }
+    if (s->profiling) PM_CHECK(collect_profile(s));
+    if (st.halted) {
+        // Steps after the breach copied the state unchanged; the ping-pong parity is preserved.
+        fill_violation(v, "energy", "simulation", st.before, st.after, st.scale, tolerance, st.breach_step);
+        return PM_VIOLATION;
+    }
+    return PM_OK;
+}
+
+int pm_solid3d_checkpoint(pm_solid3d *s) {
+    if (!s) return PM_ERR_ARGUMENT;
+    if (!s->saved) PM_CHECK(s->saved.alloc(sizeof(double) * (size_t)s->n()));
+    PM_HIP(hipMemcpyAsync(s->saved.d(), s->T(), sizeof(double) * (size_t)s->n(), hipMemcpyDeviceToDevice, s->stream));
+    PM_HIP(hipStreamSynchronize(s->stream));
+    s->saved_valid = true;
+    s->saved_lost = s->lost;
+    s->saved_supplied = s->supplied;
+    return PM_OK;
+}
+
+int pm_solid3d_restore(pm_solid3d *s) {
+    if (!s) return PM_ERR_ARGUMENT;
+    if (!s->saved_valid) {
+        // `saved` starts as the constructor's cells (solid.rs:401): every cell at the reference.
+        fill_value<<<grid_for(s->n(), 256), 256, 0, s->stream>>>(s->T(), s->n(), s->reference);
+        PM_HIP(hipGetLastError());
+    } else {
+        PM_HIP(hipMemcpyAsync(s->T(), s->saved.d(), sizeof(double) * (size_t)s->n(), hipMemcpyDeviceToDevice, s->stream));
+    }
+    PM_HIP(hipStreamSynchronize(s->stream));
+    s->lost = s->saved_lost;
+    s->supplied = s->saved_supplied;
+    return PM_OK;
+}
+
+int pm_solid3d_readings(pm_solid3d *s, double out[4]) {
+    if (!s || !out) return PM_ERR_ARGUMENT;
+    std::vector<double> h((size_t)s->n());
+    PM_CHECK(pm_solid3d_get_cells(s, h.data()));
+    // peak_temperature / mean_temperature / coldest_temperature, solid.rs:1378-1405, in celsius.
+    double peak = -1.7976931348623157e308, cold = 1.7976931348623157e308, sum = 0.0;
+    for (double t : h) {
+        peak = rmax(peak, t);
+        cold = std::fmin(cold, t);
+        sum += t;
+    }
+    out[0] = peak - 273.15;
+    out[1] = sum / (double)h.size() - 273.15;
+    out[2] = cold - 273.15;
+    out[3] = s->absorbed;
+    return PM_OK;
+}
+
+double pm_solid3d_heat_capacity(pm_solid3d *s) {
+    if (!s) return NAN;
+    double total = NAN;
+    if (total_capacity(s, &total) != PM_OK) return NAN;
+    return total;
+}
+
+int pm_solid3d_set_tile(pm_solid3d *s, int tile_y, int z_chunk) {
+    if (!s) return PM_ERR_ARGUMENT;
+    if (tile_y) {
+        if (tile_y != 4 && tile_y != 8 && tile_y != 16) {
+            set_error("tile_y must be 4, 8 or 16");
+            return PM_ERR_ARGUMENT;
+        }
+        s->tile_y = tile_y;
+    }
+    if (z_chunk) {
+        if (z_chunk < 1) return PM_ERR_ARGUMENT;
+        s->z_chunk = z_chunk;
+    }
+    s->partial_count = 0;
+    return PM_OK;
+}
+
+int pm_solid3d_set_profiling(pm_solid3d *s, int on) {
+    if (!s) return PM_ERR_ARGUMENT;
+    s->profiling = on != 0;
+    return PM_OK;
+}
+
+int pm_solid3d_kernel_times(pm_solid3d *s, double *sweep_ms, double *audit_ms) {
+    if (!s) return PM_ERR_ARGUMENT;
+    if (sweep_ms) *sweep_ms = s->sweep_ms;
+    if (audit_ms) *audit_ms = s->audit_ms;
+    return PM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,318 @@
+"""Solid3D on the GPU: the Domain of crates/pantometry-thermal/src/solid.rs, stepped by the
+gfx950 library through its C ABI.
+
+Temperatures are kelvin, lengths metres, energies joules (the reference's units are identity
+newtypes over SI `f64`, crates/pantometry-units/src/lib.rs:95-122).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+from typing import Callable, Optional, Tuple, Union
+
+import numpy as np
+
+from . import _capi
+from ._capi import check, dptr, lib
+from .core import ENERGY, Exchange, HEAT, Kind, Ledger, Violation
+
+STABLE_FOURIER_3D = 1.0 / 6.0  # solid.rs:84
+
+
+def celsius(v: float) -> float:
+    """Temperature::celsius, pantometry-units/src/lib.rs:729."""
+    return v + 273.15
+
+
+def mm(v: float) -> float:
+    """Length::mm, pantometry-units/src/lib.rs:669."""
+    return v * 1e-3
+
+
+@dataclass(frozen=True)
+class Substance:
+    """The thermal part of Substance (crates/pantometry-core/src/substance.rs).  None for a
+    conductivity or specific heat is a substance that does not say."""
+
+    name: str
+    density: float
+    conductivity: Optional[float]
+    specific_heat: Optional[float]
+
+    @staticmethod
+    def aluminium_6061() -> "Substance":
+        return Substance("Al 6061", 2.70 * 1e3, 167.0, 896.0)       # substance.rs:362-382
+
+    @staticmethod
+    def borosilicate_crown() -> "Substance":
+        return Substance("N-BK7", 2.51 * 1e3, 1.114, 858.0)         # substance.rs:336-357
+
+    @staticmethod
+    def of(name: str, density: float, conductivity: float, specific_heat: float) -> "Substance":
+        return Substance(name, density, conductivity, specific_heat)
+
+    def diffusivity(self) -> Optional[float]:
+        """k/(rho*c_p), substance.rs:211-216."""
+        if self.conductivity is None or self.specific_heat is None:
+            return None
+        return self.conductivity / (self.density * self.specific_heat)
+
+    def heat_capacity(self, volume: float) -> Optional[float]:
+        """(rho*volume)*c_p, substance.rs:219-222."""
+        if self.specific_heat is None:
+            return None
+        return (self.density * volume) * self.specific_heat
+
+    def _c(self) -> _capi.pm_substance:
+        return _capi.pm_substance(
+            self.density,
+            math.nan if self.conductivity is None else self.conductivity,
+            math.nan if self.specific_heat is None else self.specific_heat)
+
+
+def _phase(m: int, n: int, idx: np.ndarray) -> np.ndarray:
+    """cos(m*PI*(i+0.5)/n) through libm, or 1 on a one-cell axis (solid.rs:1481-1487)."""
+    if n <= 1:
+        return np.ones(len(idx))
+    return np.array([math.cos(float(m) * math.pi * (float(i) + 0.5) / float(n)) for i in idx])
+
+
+class Solid3D:
+    """A block of cubic cells conducting in three dimensions (solid.rs:173-287), on the GPU."""
+
+    def __init__(self, name: str, substance: Substance, counts: Tuple[int, int, int], dx: float,
+                 initial: float):
+        self._name = name
+        self._substances = [substance]
+        self._dx = float(dx)
+        self._initial = float(initial)
+        handle = C.c_void_p()
+        check(lib().pm_solid3d_create(name.encode(), C.byref(substance._c()), int(counts[0]),
+                                      int(counts[1]), int(counts[2]), self._dx, self._initial,
+                                      C.byref(handle)), "pm_solid3d_create")
+        self._h = handle
+        out = (C.c_size_t * 3)()
+        check(lib().pm_solid3d_counts(self._h, out), "pm_solid3d_counts")
+        self._counts = (out[0], out[1], out[2])
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().pm_solid3d_destroy(h)
+            self._h = None
+
+    # ---- construction ------------------------------------------------------------------
+    def fill(self, substance: Substance,
+             which: Union[Callable[[int, int, int], bool], np.ndarray]) -> "Solid3D":
+        """Solid3D::fill, solid.rs:519-543."""
+        nx, ny, nz = self._counts
+        if callable(which):
+            mask = np.zeros((nz, ny, nx), dtype=np.uint8)
+            for k in range(nz):
+                for j in range(ny):
+                    for i in range(nx):
+                        mask[k, j, i] = 1 if which(i, j, k) else 0
+        else:
+            mask = np.ascontiguousarray(np.asarray(which).reshape(nz, ny, nx), dtype=np.uint8)
+        if substance not in self._substances:
+            self._substances.append(substance)
+        check(lib().pm_solid3d_fill(self._h, C.byref(substance._c()),
+                                    mask.ctypes.data_as(C.POINTER(C.c_uint8))), "pm_solid3d_fill")
+        return self
+
+    def with_conductivity(self, conductivity: np.ndarray) -> "Solid3D":
+        """Per-cell conductivity, the capacity unchanged (no reference counterpart; equal to
+        one fill per distinct value)."""
+        k = np.ascontiguousarray(conductivity, dtype=np.float64).reshape(-1)
+        if k.size != self.cell_count():
+            raise ValueError("one conductivity per cell")
+        check(lib().pm_solid3d_set_conductivity(self._h, dptr(k)), "pm_solid3d_set_conductivity")
+        return self
+
+    def substances(self) -> int:
+        return len(self._substances)
+
+    # ---- state -------------------------------------------------------------------------
+    def name(self) -> str:
+        return self._name
+
+    def counts(self) -> Tuple[int, int, int]:
+        return self._counts
+
+    def spacing(self) -> float:
+        return self._dx
+
+    def cell_count(self) -> int:
+        nx, ny, nz = self._counts
+        return nx * ny * nz
+
+    def volume(self) -> float:
+        nx, ny, nz = self._counts
+        return float(nx * ny * nz) * (self._dx * self._dx * self._dx)
+
+    def index(self, i: int, j: int, k: int) -> Optional[int]:
+        nx, ny, nz = self._counts
+        return i + nx * (j + ny * k) if (i < nx and j < ny and k < nz) else None
+
+    def cells(self) -> np.ndarray:
+        """Every temperature, x-fastest, shape (nz, ny, nx)."""
+        nx, ny, nz = self._counts
+        out = np.empty(nx * ny * nz, dtype=np.float64)
+        check(lib().pm_solid3d_get_cells(self._h, dptr(out)), "pm_solid3d_get_cells")
+        return out.reshape(nz, ny, nx)
+
+    def set_cells(self, cells: np.ndarray) -> None:
+        c = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1)
+        if c.size != self.cell_count():
+            raise ValueError("one temperature per cell")
+        check(lib().pm_solid3d_set_cells(self._h, dptr(c)), "pm_solid3d_set_cells")
+
+    def set_temperature(self, i: int, j: int, k: int, t: float) -> None:
+        check(lib().pm_solid3d_set_temperature(self._h, i, j, k, float(t)),
+              "pm_solid3d_set_temperature")
+
+    def deposit(self, i: int, j: int, k: int, joules: float) -> None:
+        check(lib().pm_solid3d_deposit(self._h, i, j, k, float(joules)), "pm_solid3d_deposit")
+
+    def temperature_at(self, i: int, j: int, k: int) -> float:
+        nx, ny, nz = self._counts
+        return float(self.cells()[min(k, nz - 1), min(j, ny - 1), min(i, nx - 1)])
+
+    def release_mode(self, mode: Tuple[int, int, int], mean: float, amplitude: float) -> None:
+        """Solid3D::release_mode, solid.rs:1478-1500: mean + ((amp*px)*py)*pz per cell."""
+        nx, ny, nz = self._counts
+        px = _phase(mode[0], nx, range(nx))
+        py = _phase(mode[1], ny, range(ny))
+        pz = _phase(mode[2], nz, range(nz))
+        a = amplitude * px
+        cells = mean + (a[None, None, :] * py[None, :, None]) * pz[:, None, None]
+        self.set_cells(cells)
+        # `self.saved.clone_from(&self.cells)`: a restore returns to the released mode.
+        self.checkpoint()
+
+    def mode_amplitude(self, mode: Tuple[int, int, int]) -> float:
+        """Solid3D::mode_amplitude, solid.rs:1507-1535 (left fold in index order)."""
+        nx, ny, nz = self._counts
+        px = _phase(mode[0], nx, range(nx))
+        py = _phase(mode[1], ny, range(ny))
+        pz = _phase(mode[2], nz, range(nz))
+        norm = 1.0
+        for m, n in zip(mode, self._counts):
+            norm = norm * (1.0 if (n <= 1 or m == 0) else 0.5)
+        terms = ((self.cells() * px[None, None, :]) * py[None, :, None]) * pz[:, None, None]
+        total = float(np.cumsum(terms.reshape(-1))[-1])
+        return total / (float(nx * ny * nz) * norm)
+
+    def fourier_number(self, dt: float) -> float:
+        """alpha*dt/dx^2 for the constructor's substance, solid.rs:1423-1428."""
+        alpha = self._substances[0].diffusivity()
+        if alpha is None:
+            return math.inf
+        return alpha * dt / (self._dx * self._dx)
+
+    def mode_amplification(self, mode: Tuple[int, int, int], dt: float) -> float:
+        """1 + F * eigenvalue * dx^2, solid.rs:1455-1474."""
+        f = self.fourier_number(dt)
+        total = 0.0
+        terms = []
+        for m, n in zip(mode, self._counts):
+            if n <= 1:
+                terms.append(0.0)
+            else:
+                s = math.sin(float(m) * math.pi / (2.0 * float(n)))
+                terms.append(-4.0 * s * s)
+        total = terms[0] + terms[1] + terms[2]
+        return 1.0 + f * total
+
+    def mean_temperature(self) -> float:
+        c = self.cells().reshape(-1)
+        return float(np.cumsum(c)[-1]) / c.size
+
+    def peak_temperature(self) -> float:
+        return float(self.cells().max())
+
+    def coldest_temperature(self) -> float:
+        return float(self.cells().min())
+
+    def heat_capacity(self) -> float:
+        return float(lib().pm_solid3d_heat_capacity(self._h))
+
+    def absorbed_energy(self) -> float:
+        out = (C.c_double * 4)()
+        check(lib().pm_solid3d_readings(self._h, out), "pm_solid3d_readings")
+        return out[3]
+
+    # ---- Domain ------------------------------------------------------------------------
+    def kind(self) -> Kind:
+        return Kind.EVOLVING
+
+    def books_balance(self) -> bool:
+        return True
+
+    def supports_restore(self) -> bool:
+        return True
+
+    def residual(self) -> float:
+        return 0.0
+
+    def max_stable_dt(self, now: float = 0.0) -> float:
+        return float(lib().pm_solid3d_max_stable_dt(self._h))
+
+    def stability_ratio(self, dt: float) -> float:
+        return float(lib().pm_solid3d_stability_ratio(self._h, dt))
+
+    def step(self, t: float, dt: float, bus: Exchange) -> None:
+        """Domain::step, solid.rs:1717-1901: refuse first, then take the bus share, then sweep."""
+        v = _capi.pm_violation()
+        check(lib().pm_solid3d_refuses(self._h, float(dt), C.byref(v)), "pm_solid3d_refuses", v)
+        gained = bus.take_share(HEAT, dt)
+        check(lib().pm_solid3d_step(self._h, float(dt), float(gained), C.byref(v)),
+              "pm_solid3d_step", v)
+
+    def books(self) -> Tuple[float, float, float]:
+        b = _capi.pm_books()
+        check(lib().pm_solid3d_ledger(self._h, C.byref(b)), "pm_solid3d_ledger")
+        return b.stored, b.lost, b.supplied
+
+    def ledger(self) -> Ledger:
+        stored, lost, supplied = self.books()
+        return Ledger().with_(ENERGY, stored).with_(ENERGY, lost).with_(ENERGY, -supplied)
+
+    def checkpoint(self) -> None:
+        check(lib().pm_solid3d_checkpoint(self._h), "pm_solid3d_checkpoint")
+
+    def restore(self) -> None:
+        check(lib().pm_solid3d_restore(self._h), "pm_solid3d_restore")
+
+    def readings(self):
+        out = (C.c_double * 4)()
+        check(lib().pm_solid3d_readings(self._h, out), "pm_solid3d_readings")
+        return [(self._name, "peak", out[0], "°C"), (self._name, "mean", out[1], "°C"),
+                (self._name, "coldest", out[2], "°C"), (self._name, "absorbed", out[3], "J")]
+
+    # ---- device-resident loop ----------------------------------------------------------
+    def advance(self, dt: float, steps: int, tolerance: float = 1e-9, want_totals: bool = True):
+        """`steps` steps of `dt` with the audit after each, on the device.  Returns
+        (ledger totals after each step, device milliseconds).  Raises Violation on a refused
+        step or an audit breach (the state is left as the breaking step made it)."""
+        v = _capi.pm_violation()
+        totals = np.empty(max(steps, 1), dtype=np.float64) if want_totals else None
+        ms = C.c_double(0.0)
+        check(lib().pm_solid3d_advance(self._h, float(dt), int(steps), float(tolerance),
+                                       dptr(totals) if want_totals else None, C.byref(ms),
+                                       C.byref(v)), "pm_solid3d_advance", v)
+        return (totals[:steps] if want_totals else None), ms.value
+
+    def set_tile(self, tile_y: int = 0, z_chunk: int = 0) -> None:
+        check(lib().pm_solid3d_set_tile(self._h, tile_y, z_chunk), "pm_solid3d_set_tile")
+
+    def set_profiling(self, on: bool) -> None:
+        check(lib().pm_solid3d_set_profiling(self._h, 1 if on else 0), "pm_solid3d_set_profiling")
+
+    def kernel_times(self) -> Tuple[float, float]:
+        """Mean (sweep, audit) device milliseconds per step over the last profiled advance."""
+        sw, au = C.c_double(), C.c_double()
+        check(lib().pm_solid3d_kernel_times(self._h, C.byref(sw), C.byref(au)),
+              "pm_solid3d_kernel_times")
+        return sw.value, au.value

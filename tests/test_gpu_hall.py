@@ -1,0 +1,130 @@
+"""Hall on the GPU against the CPU oracle: bit-exact fields, energy to 1e-12, through the C ABI."""
+import math
+
+import numpy as np
+import pytest
+
+from tests import hall_cases
+from tests.conftest import gpu_case
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).reshape(-1).view(np.uint64)
+
+
+@pytest.mark.parametrize("case", hall_cases.HALL_CASES, ids=lambda c: c.__name__)
+def test_closed_forms_on_the_gpu(case):
+    gpu_case(f"hall closed form {case.__name__}")
+    case(hall_cases.GpuRoom)
+
+
+def _pair(width, height, depth, across, speed=343.0, density=1.204):
+    import pantometry_amd as pm
+    from oracle.oracle import OracleHall
+    return (pm.Hall("hall", width, height, depth, across, speed, density),
+            OracleHall(width, height, depth, across, speed, density))
+
+
+def _same_fields(g, o):
+    p, pp, vx, vy, vz = g.fields()
+    assert np.array_equal(bits(p), bits(o.p))
+    assert np.array_equal(bits(pp), bits(o.pp))
+    n = len(vx)
+    assert np.array_equal(bits(vx), bits(o.vx[:n]))
+    assert np.array_equal(bits(vy), bits(o.vy[:len(vy)]))
+    assert np.array_equal(bits(vz), bits(o.vz[:len(vz)]))
+
+
+@pytest.mark.parametrize("dims,across,mode", [
+    ((4.4, 3.1, 2.4), 13, (1, 1, 1)),
+    ((4.4, 3.1, 2.4), 45, (3, 2, 1)),
+    ((4.4, 3.1, 0.0), 45, (1, 1, 0)),
+    ((0.5, 0.07, 0.3), 130, (2, 1, 3)),
+], ids=str)
+def test_fields_are_bit_exact(dims, across, mode):
+    gpu_case(f"hall parity {dims} {across}")
+    g, o = _pair(*dims, across)
+    assert g.nodes() == o.nodes()
+    g.released_in_mode(mode, 2.0)
+    o.release(o.mode(mode, 2.0))
+    assert np.array_equal(bits(g.fields()[0]), bits(o.p))
+    dt = 0.9 * o.max_stable_dt()
+    assert g.max_stable_dt() == o.max_stable_dt()
+    import pantometry_amd as pm
+    for n in range(150):
+        g.step(0.0, dt, pm.Exchange())
+        o.step(dt, threads_=16)
+    _same_fields(g, o)
+    e_ref = o.energy()
+    assert abs(g.energy() - e_ref) <= 1e-12 * abs(e_ref)
+    assert abs(g.startup_adjustment() - o.offset) <= 1e-12 * abs(e_ref)
+
+
+def test_a_sourced_hall_is_bit_exact_and_books_its_source():
+    gpu_case("hall with source 41^3")
+    import pantometry_amd as pm
+    from pantometry_amd.workloads import ricker
+    g, o = _pair(0.04, 0.04, 0.04, 41, 343.0, 1.2)
+    g.released_from(np.zeros(41 ** 3))
+    o.release(np.zeros(41 ** 3))
+    dt = 0.5 * o.max_stable_dt()
+    f = 343.0 / (20.0 * o.dx)
+    inc = np.array([ricker((n + 1) * dt, f, 1.0 / f) for n in range(200)])
+    node = 17 + 41 * (23 + 41 * 11)
+    g.set_source(node, inc)
+    o.set_source(node, inc)
+    totals, _ = g.advance(dt, 260, tolerance=1e-12)
+    for _ in range(260):
+        o.step(dt)
+    _same_fields(g, o)
+    ref_total, ref_scale = o.ledger()
+    assert abs(totals[-1] - ref_total) <= 1e-12 * ref_scale
+    assert np.all(np.abs(totals - totals[0]) <= 1e-12 * ref_scale)
+    assert abs(g.supplied() - o.supplied) <= 1e-12 * abs(o.supplied)
+
+
+def test_checkpoint_and_restore():
+    gpu_case("hall checkpoint")
+    import pantometry_amd as pm
+    g, _ = _pair(4.4, 3.1, 2.4, 25)
+    g.released_in_mode((1, 1, 1), 1.0)
+    dt = 0.9 * g.max_stable_dt()
+    for _ in range(5):
+        g.step(0.0, dt, pm.Exchange())
+    g.checkpoint()
+    saved = g.fields()
+    for _ in range(9):
+        g.step(0.0, dt, pm.Exchange())
+    after = g.fields()
+    g.restore()
+    p, pp, vx, vy, vz = g.fields()
+    assert np.array_equal(bits(p), bits(saved[0])) and np.array_equal(bits(pp), bits(saved[0]))
+    assert np.array_equal(bits(vx), bits(saved[2]))
+
+
+@pytest.mark.timeout(900)
+def test_the_acoustic_hall_at_384():
+    """385^3 nodes: 20 steps bit-exact against the oracle, then 4000 audited steps."""
+    gpu_case("hall 385^3")
+    from oracle.oracle import OracleHall
+    from pantometry_amd import workloads as w
+    case = w.acoustic_hall(384, 4000)
+    g = case.hall
+    o = OracleHall(384 * w.DX, 384 * w.DX, 384 * w.DX, 385, 343.0, 1.2)
+    assert o.nodes() == g.nodes()
+    o.release(np.zeros(385 ** 3))
+    o.set_source(case.source_node, case.increments)
+    g.checkpoint()
+    totals, ms = g.advance(case.dt, 20, tolerance=1e-12)
+    for _ in range(20):
+        o.step(case.dt, threads_=16)
+    _same_fields(g, o)
+    t_ref, s_ref = o.ledger()
+    assert abs(totals[-1] - t_ref) <= 1e-12 * s_ref
+    del o
+    totals, ms = g.advance(case.dt, 3980, tolerance=1e-12)
+    scale = max(abs(g.energy()), abs(g.supplied()))
+    assert g.supplied() > 0.0
+    assert np.all(np.abs(totals - t_ref) <= 1e-12 * scale)

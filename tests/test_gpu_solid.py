@@ -1,0 +1,233 @@
+"""Solid3D on the GPU against the CPU oracle: bit-exact fields, ledgers to 1e-12, through the C ABI.
+
+Small grids cover every operator form (uniform, per-cell conductivity, general fill), partial
+tiles, one-cell-thick blocks and every tile shape; the full 512^3 heterogeneous block is checked
+bit-exact over 10 steps against the oracle and over its full 2000-step audited run through
+size-independent properties (the audit holds at 1e-12 every step, a rerun is bit-identical).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from tests import solid_cases
+from tests.conftest import gpu_case
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).reshape(-1).view(np.uint64)
+
+
+@pytest.mark.parametrize("case", solid_cases.SOLID_CASES, ids=lambda c: c.__name__)
+def test_closed_forms_on_the_gpu(case):
+    gpu_case(f"solid closed form {case.__name__}")
+    case(solid_cases.GpuBlock)
+
+
+def _pair(counts, k=None, t=None, sub=(4000.0, 1.0, 1000.0), fills=()):
+    """A GPU block and an oracle block with the same operator and cells."""
+    import pantometry_amd as pm
+    from oracle.oracle import OracleSolid3D
+    g = pm.Solid3D("block", pm.Substance("s", *sub), counts, 1e-3, 300.0)
+    o = OracleSolid3D("block", sub[0], sub[1], sub[2], counts, 1e-3, 300.0)
+    if k is not None:
+        g.with_conductivity(k)
+        o.with_conductivity(k)
+    for fsub, mask in fills:
+        g.fill(pm.Substance("f", *fsub), mask)
+        o.fill(fsub[0], fsub[1], fsub[2], mask)
+    if t is not None:
+        g.set_cells(t)
+        o.set_cells(t)
+    return g, o
+
+
+def _step_both(g, o, dt, steps, threads=16):
+    import pantometry_amd as pm
+    for _ in range(steps):
+        g.step(0.0, dt, pm.Exchange())
+        o.step(dt, threads_=threads)
+
+
+SHAPES = [(7, 5, 3), (1, 1, 24), (24, 1, 1), (1, 24, 1), (70, 37, 23), (64, 8, 32), (65, 9, 33),
+          (130, 3, 5)]
+
+
+@pytest.mark.parametrize("counts", SHAPES, ids=str)
+def test_uniform_form_is_bit_exact(counts, oracle):
+    gpu_case(f"uniform form {counts}")
+    t = oracle.hot_spot_cells(counts, sigma=2.0)
+    g, o = _pair(counts, t=t, sub=(4000.0, 400.0, 1000.0))
+    dt = 0.9 * o.max_stable_dt()
+    assert g.max_stable_dt() == o.max_stable_dt()
+    _step_both(g, o, dt, 40)
+    assert np.array_equal(bits(g.cells()), bits(o.cells))
+
+
+@pytest.mark.parametrize("counts", SHAPES, ids=str)
+def test_per_cell_conductivity_is_bit_exact(counts, oracle):
+    gpu_case(f"per-cell conductivity {counts}")
+    k, t = oracle.heterogeneous_fields(counts, spots=3)
+    g, o = _pair(counts, k=k, t=t)
+    assert g.max_stable_dt() == o.max_stable_dt()
+    dt = 0.9 * o.max_stable_dt()
+    _step_both(g, o, dt, 40)
+    assert np.array_equal(bits(g.cells()), bits(o.cells))
+
+
+def test_a_general_fill_is_bit_exact(oracle):
+    gpu_case("general fill 40x30x20")
+    counts = (40, 30, 20)
+    nx, ny, nz = counts
+    k, t = oracle.heterogeneous_fields(counts, spots=2)
+    mask = np.zeros((nz, ny, nx), dtype=bool)
+    mask[5:15, 10:25, 3:30] = True
+    mask2 = np.zeros((nz, ny, nx), dtype=bool)
+    mask2[:, :, 33:] = True
+    fills = [((2.51e3, 1.114, 858.0), mask), ((8960.0, 401.0, 385.0), mask2)]
+    g, o = _pair(counts, t=t, sub=(2700.0, 167.0, 896.0), fills=fills)
+    assert g.max_stable_dt() == o.max_stable_dt()
+    dt = 0.9 * o.max_stable_dt()
+    _step_both(g, o, dt, 60)
+    assert np.array_equal(bits(g.cells()), bits(o.cells))
+    assert abs(g.books()[0] - o.stored_heat()) <= 1e-12 * abs(o.stored_heat())
+
+
+@pytest.mark.parametrize("tile_y,z_chunk", [(4, 1), (4, 7), (8, 3), (8, 32), (16, 5), (16, 64)])
+def test_every_tile_shape_gives_the_same_bits(tile_y, z_chunk, oracle):
+    gpu_case(f"tile {tile_y} x {z_chunk}")
+    counts = (70, 37, 23)
+    k, t = oracle.heterogeneous_fields(counts, spots=3)
+    g, o = _pair(counts, k=k, t=t)
+    g.set_tile(tile_y, z_chunk)
+    dt = 0.9 * o.max_stable_dt()
+    _step_both(g, o, dt, 15)
+    assert np.array_equal(bits(g.cells()), bits(o.cells))
+
+
+def test_bus_heat_spreads_as_the_reference_does(oracle):
+    gpu_case("bus heat 20x11x7")
+    import pantometry_amd as pm
+    counts = (20, 11, 7)
+    k, t = oracle.heterogeneous_fields(counts, spots=2)
+    g, o = _pair(counts, k=k, t=t)
+    dt = 0.5 * o.max_stable_dt()
+    for n in range(5):
+        bus = pm.Exchange()
+        bus.publish(pm.HEAT, 3.0 + n)
+        g.step(0.0, dt, bus)
+        o.step(dt, gained=3.0 + n)
+    assert np.array_equal(bits(g.cells()), bits(o.cells))
+    assert g.absorbed_energy() == o.absorbed
+
+
+def test_the_uniform_block_over_1000_steps(oracle):
+    """64^3, alpha = 1e-4, 0.9 of the limit, 1000 steps, audit every 100 steps."""
+    gpu_case("uniform 64^3 x 1000")
+    from pantometry_amd import workloads as w
+    g, dt = w.uniform_block(64)
+    t = oracle.hot_spot_cells((64, 64, 64))
+    assert np.array_equal(bits(g.cells()), bits(t))
+    o = oracle.OracleSolid3D("block", w.RHO, 1e-4 * w.RHO * w.CP, w.CP, (64, 64, 64), w.DX, 300.0)
+    o.set_cells(t)
+    assert dt == 0.9 * o.max_stable_dt()
+    led0 = o.stored_heat()
+    for block in range(10):
+        totals, _ = g.advance(dt, 100, tolerance=1e-12)
+        for _ in range(100):
+            o.step(dt, threads_=16)
+        ref = o.stored_heat()
+        assert abs(totals[-1] - ref) <= 1e-12 * abs(ref)
+        assert abs(ref - led0) <= 1e-12 * abs(led0)
+    assert np.array_equal(bits(g.cells()), bits(o.cells))
+
+
+def test_a_breach_stops_the_run_where_it_broke(oracle):
+    gpu_case("audit breach")
+    import pantometry_amd as pm
+    counts = (33, 17, 9)
+    k, t = oracle.heterogeneous_fields(counts, spots=2)
+    g, o = _pair(counts, k=k, t=t)
+    dt = 0.9 * o.max_stable_dt()
+    # A negative tolerance breaks on the first audit: the state must be one step on, no further.
+    with pytest.raises(pm.Violation) as err:
+        g.advance(dt, 10, tolerance=-1.0)
+    assert err.value.step == 0 and err.value.quantity == "energy"
+    o.step(dt)
+    assert np.array_equal(bits(g.cells()), bits(o.cells))
+    # And a refused step mutates nothing.
+    before = g.cells().copy()
+    with pytest.raises(pm.Violation) as err:
+        g.advance(1.5 * g.max_stable_dt(), 3)
+    assert err.value.quantity == "Fourier number"
+    assert np.array_equal(bits(g.cells()), bits(before))
+
+
+def test_checkpoint_and_restore(oracle):
+    gpu_case("checkpoint/restore")
+    import pantometry_amd as pm
+    counts = (21, 13, 8)
+    k, t = oracle.heterogeneous_fields(counts, spots=2)
+    g, _ = _pair(counts, k=k, t=t)
+    dt = 0.9 * g.max_stable_dt()
+    g.checkpoint()
+    saved = g.cells().copy()
+    for _ in range(7):
+        g.step(0.0, dt, pm.Exchange())
+    moved = g.cells().copy()
+    g.restore()
+    assert np.array_equal(bits(g.cells()), bits(saved))
+    for _ in range(7):
+        g.step(0.0, dt, pm.Exchange())
+    assert np.array_equal(bits(g.cells()), bits(moved))
+
+
+@pytest.mark.timeout(600)
+def test_the_heterogeneous_block_at_128_over_2000_steps(oracle):
+    gpu_case("heterogeneous 128^3 x 2000")
+    counts = (128, 128, 128)
+    k, t = oracle.heterogeneous_fields(counts)
+    g, o = _pair(counts, k=k, t=t)
+    dt = 0.9 * o.max_stable_dt()
+    totals, _ = g.advance(dt, 2000, tolerance=1e-12)
+    for _ in range(2000):
+        o.step(dt, threads_=16)
+    assert np.array_equal(bits(g.cells()), bits(o.cells))
+    ref = o.stored_heat()
+    comp = o.stored_heat_compensated()
+    assert abs(totals[-1] - ref) <= 1e-12 * abs(ref)
+    assert abs(totals[-1] - comp) <= 1e-12 * abs(comp)
+
+
+@pytest.mark.timeout(900)
+def test_the_heterogeneous_block_at_512(oracle):
+    """The full block: 10 steps bit-exact against the oracle, then the full 2000-step audited
+    run (every step's audit at 1e-12) twice from the same start, bit-identical."""
+    gpu_case("heterogeneous 512^3")
+    from pantometry_amd import workloads as w
+    g, dt = w.heterogeneous_block(512)
+    k, t = oracle.heterogeneous_fields((512, 512, 512))
+    o = oracle.OracleSolid3D("block", w.RHO, 1.0, w.CP, (512, 512, 512), w.DX, 300.0)
+    o.with_conductivity(k)
+    o.set_cells(t)
+    del k
+    assert dt == 0.9 * o.max_stable_dt()
+    start = g.cells().copy()
+    assert np.array_equal(bits(start), bits(t))
+    del t
+    g.advance(dt, 10, tolerance=1e-12)
+    for _ in range(10):
+        o.step(dt, threads_=16)
+    assert np.array_equal(bits(g.cells()), bits(o.cells))
+    ref = o.stored_heat()
+    del o
+    g.set_cells(start)
+    totals, _ = g.advance(dt, 2000, tolerance=1e-12)
+    first = g.cells().copy()
+    assert abs(totals[9] - ref) <= 1e-12 * abs(ref)
+    assert np.all(np.abs(totals - totals[0]) <= 1e-12 * abs(totals[0]))
+    g.set_cells(start)
+    g.advance(dt, 2000, tolerance=1e-12)
+    assert np.array_equal(bits(g.cells()), bits(first))

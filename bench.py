@@ -118,7 +118,7 @@ def main() -> int:
         "config": {
             "workload": "heat 512^3 f64, heterogeneous conductivity, harmonic faces, "
                         "audit every step (0.9 of the stability limit)",
-            "cells": cells, "dt": dt, "tile_y": args.tile_y or 8, "z_chunk": args.z_chunk or 32,
+            "cells": cells, "dt": dt, "tile_y": args.tile_y or 8, "z_chunk": args.z_chunk or 128,
         },
         "roofline": {
             "bound": "hbm",
@@ -127,7 +127,7 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": _traffic_from_profiles(),
-            "kernel": "heat_sweep<1,0,8> (per-cell k, uniform C)",
+            "kernel": "heat_sweep<1,0,8> (per-cell k, uniform C, 64x8 tile)",
             "kernel_ms": round(sweep_ms, 5),
             "audit_kernel_ms": round(audit_ms, 5),
             "bytes_per_cell": BYTES_PER_CELL,

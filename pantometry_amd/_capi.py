@@ -73,7 +73,7 @@ SIGNATURES = {
     "pm_solid3d_restore": (_I, [_VP]),
     "pm_solid3d_readings": (_I, [_VP, _DP]),
     "pm_solid3d_heat_capacity": (_D, [_VP]),
-    "pm_solid3d_set_tile": (_I, [_VP, _I, _I]),
+    "pm_solid3d_set_tile": (_I, [_VP, _I, _I, _I]),
     "pm_solid3d_set_profiling": (_I, [_VP, _I]),
     "pm_solid3d_kernel_times": (_I, [_VP, _DP, _DP]),
     "pm_hall_create": (_I, [C.c_char_p, _D, _D, _D, _S, _D, _D, C.POINTER(_VP)]),

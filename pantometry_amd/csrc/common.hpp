@@ -106,6 +106,15 @@ struct AuditState {
 
 // ---- device helpers ------------------------------------------------------------------------
 
+// A workgroup barrier for data exchanged through LDS only: waits for this wave's LDS operations
+// (lgkmcnt) and meets the other waves, but leaves global loads in flight.  __syncthreads()
+// carries a workgroup-scope release fence, which also drains every outstanding global load
+// (vmcnt(0)) and so cancels any prefetch across the barrier.  The memory clobber keeps the
+// compiler from moving LDS accesses across it.
+__device__ __forceinline__ void lds_barrier() {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 // The harmonic face mean of solid.rs:706-713, `2.0 * a * b / sum` guarded at zero.  IEEE
 // division (the correctly rounded gfx950 sequence); contraction is off for the whole library.
 __device__ __forceinline__ double series(double a, double b) {

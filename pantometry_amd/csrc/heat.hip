@@ -46,35 +46,87 @@ struct HeatArgs {
     double cap0;                       // uniform capacity
     double tref;                       // ledger reference temperature
     int nx, ny, nz, zc;
+    int gx, gy, gz;                    // tiles along x, y and z
 };
 
 enum { K_UNIFORM = 0, K_PER_CELL = 1 };
 enum { C_UNIFORM = 0, C_PER_CELL = 1 };
 
+// Workgroups of one launch are spread over the XCDs round-robin (workgroup i runs on XCD i % 8).
+// The sweep gives each XCD a contiguous run of tiles in x-fastest order, so a tile's x and y
+// neighbours, whose rows it reads as halo, run on the same XCD and are found in its L2.
+constexpr int XCDS = 8;
+
+// The faces of one plane from the conductivities staged in LDS: each thread builds its own low x
+// and y faces; the first TY threads add the tile's right-edge x faces and the last row of threads
+// the top-edge y faces.  Face g along an axis of n cells lies between cells g-1 and g and is zero
+// unless 0 < g < n (solid.rs:716-734).
+// The harmonic face mean with the zero guard as a select: for two zero conductivities the
+// division yields NaN and the select returns 0, as series() does.
+__device__ __forceinline__ double series_nz(double a, double b) {
+    const double sum = a + b;
+    const double q = 2.0 * a * b / sum;
+    return sum > 0.0 ? q : 0.0;
+}
+
+template <int TY>
+__device__ __forceinline__ void build_faces(const double (*sk)[66], double (*fx)[65],
+                                            double (*fy)[64], int tx, int ty, int tid,
+                                            bool fxl_on, bool fyl_on, bool fxe_on, bool fye_on) {
+    constexpr int TX = 64;
+    const double c = sk[ty + 1][tx + 1];
+    const double vx = series_nz(sk[ty + 1][tx], c), vy = series_nz(sk[ty][tx + 1], c);
+    fx[ty][tx] = fxl_on ? vx : 0.0;
+    fy[ty][tx] = fyl_on ? vy : 0.0;
+    if (tid < TY) {
+        const double v = series_nz(sk[tid + 1][TX], sk[tid + 1][TX + 1]);
+        fx[tid][TX] = fxe_on ? v : 0.0;
+    }
+    if (ty == TY - 1) {
+        const double v = series_nz(sk[TY][tx + 1], sk[TY + 1][tx + 1]);
+        fy[TY][tx] = fye_on ? v : 0.0;
+    }
+}
+
+// One sweep (solid.rs:1776-1826) over a 64 x TY column tile and a chunk of z-planes, with the
+// audit's C*(T'-T_ref) summed as the cells are stored.
+//
+// Per plane z a thread stages T(z) and, in the per-cell-conductivity form, k(z+1) in LDS, then
+// one barrier: the x/y faces of plane z were built from k(z) in the previous iteration and the
+// faces of plane z+1 are built now, so each plane needs a single barrier.  The own column's
+// T(z-1..z+2) and k(z..z+3) live in four-register rings and the halo values in two-register
+// rings; the z loop is unrolled by four so every ring slot has a fixed register and a load
+// issued for a later plane is not waited on until that plane's iteration uses it.
 template <int KMODE, int CMODE, int TY>
 __global__ __launch_bounds__(64 * TY) void heat_sweep(HeatArgs a) {
     constexpr int TX = 64;
     constexpr int NT = TX * TY;
+    constexpr bool KC = KMODE == K_PER_CELL;
     __shared__ double sT[2][TY + 2][TX + 2];
-    __shared__ double sK[KMODE ? 2 : 1][KMODE ? TY + 2 : 1][KMODE ? TX + 2 : 1];
-    __shared__ double fx[KMODE ? TY : 1][KMODE ? TX + 1 : 1];
-    __shared__ double fy[KMODE ? TY + 1 : 1][KMODE ? TX : 1];
+    __shared__ double sK[KC ? 2 : 1][KC ? TY + 2 : 1][TX + 2];
+    __shared__ double fx[KC ? 2 : 1][KC ? TY : 1][TX + 1];
+    __shared__ double fy[KC ? 2 : 1][KC ? TY + 1 : 1][TX];
     __shared__ double red[NT / 64];
+
+    const int64_t ntiles = (int64_t)a.gx * a.gy * a.gz;
+    const int64_t chunk = (ntiles + XCDS - 1) / XCDS;
+    const int64_t tile = (int64_t)(blockIdx.x % XCDS) * chunk + blockIdx.x / XCDS;
+    if (tile >= ntiles) return;
+    const int64_t bx = tile % a.gx, by = (tile / a.gx) % a.gy, bz = tile / ((int64_t)a.gx * a.gy);
 
     const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TX * ty;
     const int64_t nx = a.nx, ny = a.ny, nz = a.nz, plane = nx * ny;
-    const int64_t x0 = (int64_t)blockIdx.x * TX, y0 = (int64_t)blockIdx.y * TY;
-    const int64_t z0 = (int64_t)blockIdx.z * a.zc, z1 = std::min<int64_t>(z0 + a.zc, nz);
+    const int64_t x0 = bx * TX, y0 = by * TY;
+    const int64_t z0 = bz * a.zc, z1 = std::min<int64_t>(z0 + a.zc, nz);
     const int64_t gi = x0 + tx, gj = y0 + ty;
     const bool inside = gi < nx && gj < ny;
     const int64_t col = std::min(gi, nx - 1) + nx * std::min(gj, ny - 1);
-    const int64_t bid = blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
 
     if (a.audit->halted) {
         // A breach froze the run: carry the state over unchanged so the ping-pong stays valid.
         if (inside)
             for (int64_t z = z0; z < z1; z++) a.tnew[col + plane * z] = a.told[col + plane * z];
-        if (tid == 0) a.partials[bid] = 0.0;
+        if (tid == 0) a.partials[tile] = 0.0;
         return;
     }
 
@@ -99,104 +151,128 @@ __global__ __launch_bounds__(64 * TY) void heat_sweep(HeatArgs a) {
 
     const double *__restrict__ T = a.told;
     const double *__restrict__ K = a.kcell;
-    auto zcl = [&](int64_t z) { return std::min(std::max<int64_t>(z, 0), nz - 1); };
+    // In-plane offsets fit 32 bits (the block is limited to 2^31 cells); the plane's base pointer
+    // is uniform across the workgroup and lives in scalar registers.
+    const uint32_t ucol = (uint32_t)col, uh = (uint32_t)hidx;
+    // Plane q clamped to [0, min(z1, nz-1)]: a plane past what the chunk uses is never fetched
+    // from HBM, only re-read from a plane the chunk already holds.
+    const int64_t top = std::min(z1, nz - 1);
+    auto at = [&](const double *p, int64_t q) -> const double * {
+        return p + plane * std::min(std::max<int64_t>(q, 0), top);
+    };
 
-    double t_m = T[col + plane * zcl(z0 - 1)];
-    double t_c = T[col + plane * z0];
-    double t_p = T[col + plane * zcl(z0 + 1)];
-    double h_t = halo ? T[hidx + plane * z0] : 0.0;
-    double k_c = 0.0, k_p = 0.0, h_k = 0.0, kz_lo = 0.0;
-    if (KMODE == K_PER_CELL) {
-        double k_m = K[col + plane * zcl(z0 - 1)];
-        k_c = K[col + plane * z0];
-        k_p = K[col + plane * zcl(z0 + 1)];
-        h_k = halo ? K[hidx + plane * z0] : 0.0;
-        kz_lo = z0 > 0 ? series(k_m, k_c) : 0.0;
+    // Faces on the block's outer boundary are zero (solid.rs:189-195); which ones a thread meets
+    // does not change along z.
+    const bool fxl_on = gi > 0 && gi < nx, fyl_on = gj > 0 && gj < ny;
+    const bool fxe_on = x0 + TX < nx, fye_on = y0 + TY < ny;
+
+    // Rings, indexed by (z - z0) modulo their length.
+    double RT[4], RK[4], HT[2], HK[2], KZ[2];
+    RT[3] = at(T, z0 - 1)[ucol];
+    RT[0] = at(T, z0)[ucol];
+    RT[1] = at(T, z0 + 1)[ucol];
+    RT[2] = at(T, z0 + 2)[ucol];
+    HT[0] = halo ? at(T, z0)[uh] : 0.0;
+    HT[1] = halo ? at(T, z0 + 1)[uh] : 0.0;
+    if constexpr (KC) {
+        const double k_m = at(K, z0 - 1)[ucol];
+        RK[0] = at(K, z0)[ucol];
+        RK[1] = at(K, z0 + 1)[ucol];
+        RK[2] = at(K, z0 + 2)[ucol];
+        RK[3] = at(K, z0 + 3)[ucol];
+        const double h_k0 = halo ? at(K, z0)[uh] : 0.0;
+        HK[0] = halo ? at(K, z0 + 1)[uh] : 0.0;
+        HK[1] = halo ? at(K, z0 + 2)[uh] : 0.0;
+        const double lo = series_nz(k_m, RK[0]), hi = series_nz(RK[0], RK[1]);
+        KZ[0] = z0 > 0 ? lo : 0.0;
+        KZ[1] = z0 + 1 < nz ? hi : 0.0;
+        // The faces of plane z0, from k(z0) staged in the second buffer.
+        sK[1][ty + 1][tx + 1] = RK[0];
+        if (halo) sK[1][hr][hc] = h_k0;
+        lds_barrier();
+        build_faces<TY>(sK[1], fx[0], fy[0], tx, ty, tid, fxl_on, fyl_on, fxe_on, fye_on);
     }
 
     // Uniform-form faces depend only on position.
-    const double kxl_u = (gi > 0 && gi < nx) ? a.kface : 0.0;
+    const double kxl_u = fxl_on ? a.kface : 0.0;
     const double kxh_u = (gi + 1 < nx) ? a.kface : 0.0;
-    const double kyl_u = (gj > 0 && gj < ny) ? a.kface : 0.0;
+    const double kyl_u = fyl_on ? a.kface : 0.0;
     const double kyh_u = (gj + 1 < ny) ? a.kface : 0.0;
+    double *__restrict__ Tout = a.tnew;
 
     double acc = 0.0;
-    for (int64_t z = z0; z < z1; z++) {
-        const int buf = (int)((z - z0) & 1);
-        // Prefetch: own column two planes ahead, halo one plane ahead.
-        const int64_t zpp = zcl(z + 2), zn = zcl(z + 1);
-        double t_pp = T[col + plane * zpp];
-        double h_t_n = halo ? T[hidx + plane * zn] : 0.0;
-        double k_pp = 0.0, h_k_n = 0.0;
-        if (KMODE == K_PER_CELL) {
-            k_pp = K[col + plane * zpp];
-            h_k_n = halo ? K[hidx + plane * zn] : 0.0;
-        }
+    for (int64_t zb = z0; zb < z1; zb += 4) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            // A chunk whose length is not a multiple of four runs up to three idle planes at its
+            // end (loads clamped, nothing stored) so the ring keeps its fixed registers.
+            const int64_t z = zb + i;
+            const bool live = z < z1;
+            const int b = i & 1;
+            const double t_m = RT[(i + 3) & 3], t_c = RT[i], t_p = RT[(i + 1) & 3];
 
-        sT[buf][ty + 1][tx + 1] = t_c;
-        if (halo) sT[buf][hr][hc] = h_t;
-        if (KMODE == K_PER_CELL) {
-            sK[buf][ty + 1][tx + 1] = k_c;
-            if (halo) sK[buf][hr][hc] = h_k;
-        }
-        __syncthreads();
-
-        double kxl, kxh, kyl, kyh, kzl, kzh;
-        if (KMODE == K_PER_CELL) {
-            // Each thread builds its own low x and y faces; wave 0 adds the tile's right-edge x
-            // faces and the last wave the top-edge y faces.  Face g along an axis of n cells
-            // lies between cells g-1 and g and is zero unless 0 < g < n.
-            fx[ty][tx] = (gi > 0 && gi < nx) ? series(sK[buf][ty + 1][tx], sK[buf][ty + 1][tx + 1]) : 0.0;
-            fy[ty][tx] = (gj > 0 && gj < ny) ? series(sK[buf][ty][tx + 1], sK[buf][ty + 1][tx + 1]) : 0.0;
-            if (tid < TY) {
-                const int r = tid;
-                const int64_t g = x0 + TX;
-                fx[r][TX] = (g < nx) ? series(sK[buf][r + 1][TX], sK[buf][r + 1][TX + 1]) : 0.0;
+            sT[b][ty + 1][tx + 1] = t_c;
+            if (halo) {
+                sT[b][hr][hc] = HT[b];
+                HT[b] = at(T, z + 2)[uh];
             }
-            if (ty == TY - 1) {
-                const int64_t g = y0 + TY;
-                fy[TY][tx] = (g < ny) ? series(sK[buf][TY][tx + 1], sK[buf][TY + 1][tx + 1]) : 0.0;
+            if constexpr (KC) {
+                sK[b][ty + 1][tx + 1] = RK[(i + 1) & 3];
+                if (halo) {
+                    sK[b][hr][hc] = HK[b];
+                    HK[b] = at(K, z + 3)[uh];
+                }
             }
-            kzh = (z + 1 < nz) ? series(k_c, k_p) : 0.0;
-            kzl = kz_lo;
-            __syncthreads();
-            kxl = fx[ty][tx], kxh = fx[ty][tx + 1];
-            kyl = fy[ty][tx], kyh = fy[ty + 1][tx];
-        } else {
-            kxl = kxl_u, kxh = kxh_u, kyl = kyl_u, kyh = kyh_u;
-            kzl = z > 0 ? a.kface : 0.0;
-            kzh = z + 1 < nz ? a.kface : 0.0;
-        }
+            lds_barrier();
 
-        const double tw = sT[buf][ty + 1][tx], te = sT[buf][ty + 1][tx + 2];
-        const double ts = sT[buf][ty][tx + 1], tn = sT[buf][ty + 2][tx + 1];
-        // solid.rs:1781-1798, in the reference's order: x-, x+, y-, y+, z-, z+.
-        double flux = 0.0;
-        flux = flux + kxl * (tw - t_c);
-        flux = flux + kxh * (te - t_c);
-        flux = flux + kyl * (ts - t_c);
-        flux = flux + kyh * (tn - t_c);
-        flux = flux + kzl * (t_m - t_c);
-        flux = flux + kzh * (t_p - t_c);
-        if (inside) {
-            const int64_t c = col + plane * z;
-            double rise;
-            if (CMODE == C_PER_CELL)
-                rise = (a.dts * a.mob[c]) * flux;
-            else
-                rise = a.f * flux;
-            const double t_new = t_c + rise;
-            a.tnew[c] = t_new;
-            const double cap = (CMODE == C_PER_CELL) ? a.cap[c] : a.cap0;
-            acc += cap * (t_new - a.tref);
-        }
+            double kxl, kxh, kyl, kyh, kzl, kzh;
+            if constexpr (KC) {
+                build_faces<TY>(sK[b], fx[b ^ 1], fy[b ^ 1], tx, ty, tid, fxl_on, fyl_on, fxe_on,
+                                fye_on);
+                kxl = fx[b][ty][tx], kxh = fx[b][ty][tx + 1];
+                kyl = fy[b][ty][tx], kyh = fy[b][ty + 1][tx];
+                kzl = KZ[b], kzh = KZ[b ^ 1];
+                const double zf = series_nz(RK[(i + 1) & 3], RK[(i + 2) & 3]);
+                KZ[b] = (z + 2 < nz) ? zf : 0.0;  // the face between z+1 and z+2
+                RK[i] = at(K, z + 4)[ucol];        // k(z) is no longer needed
+            } else {
+                kxl = kxl_u, kxh = kxh_u, kyl = kyl_u, kyh = kyh_u;
+                kzl = z > 0 ? a.kface : 0.0;
+                kzh = z + 1 < nz ? a.kface : 0.0;
+            }
 
-        t_m = t_c, t_c = t_p, t_p = t_pp, h_t = h_t_n;
-        if (KMODE == K_PER_CELL) k_c = k_p, k_p = k_pp, h_k = h_k_n, kz_lo = kzh;
+            const double tw = sT[b][ty + 1][tx], te = sT[b][ty + 1][tx + 2];
+            const double ts = sT[b][ty][tx + 1], tn = sT[b][ty + 2][tx + 1];
+            // solid.rs:1781-1798, in the reference's order: x-, x+, y-, y+, z-, z+.
+            double flux = 0.0;
+            flux = flux + kxl * (tw - t_c);
+            flux = flux + kxh * (te - t_c);
+            flux = flux + kyl * (ts - t_c);
+            flux = flux + kyh * (tn - t_c);
+            flux = flux + kzl * (t_m - t_c);
+            flux = flux + kzh * (t_p - t_c);
+            RT[(i + 3) & 3] = at(T, z + 3)[ucol];  // T(z-1) is no longer needed
+            if (inside && live) {
+                const int64_t zoff = plane * z;
+                double rise;
+                if constexpr (CMODE == C_PER_CELL)
+                    rise = (a.dts * (a.mob + zoff)[ucol]) * flux;
+                else
+                    rise = a.f * flux;
+                const double t_new = t_c + rise;
+                __builtin_nontemporal_store(t_new, Tout + zoff + ucol);
+                double cap;
+                if constexpr (CMODE == C_PER_CELL)
+                    cap = (a.cap + zoff)[ucol];
+                else
+                    cap = a.cap0;
+                acc += cap * (t_new - a.tref);
+            }
+        }
     }
 
     const double total = block_sum<NT>(acc, red);
-    if (tid == 0) a.partials[bid] = total;
+    if (tid == 0) a.partials[tile] = total;
 }
 
 // Ledger total and audit of the step that just ran: stored = sum of partials; the ledger is
@@ -335,7 +411,7 @@ struct pm_solid3d {
     bool saved_valid = false;
     DevBuf partials, audit, scalars, stored_dev;
     int64_t partial_count = 0;
-    int tile_y = 8, z_chunk = 32;
+    int tile_y = 8, z_chunk = 128, prefetch = 2;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // Per-launch events of a profiled advance: [3*s] before the sweep, [3*s+1] between sweep and
     // audit, [3*s+2] after the audit.
@@ -368,12 +444,20 @@ static int ensure_partials(pm_solid3d *s) {
     return PM_OK;
 }
 
+template <int KM, int CM, int TY>
+static void launch_sweep_pd(const pm_solid3d *s, const HeatArgs &a, dim3 g) {
+    // One-dimensional launch: the kernel maps workgroup ids to tiles XCD by XCD.
+    const int64_t ntiles = (int64_t)g.x * g.y * g.z;
+    const unsigned blocks = (unsigned)(XCDS * ((ntiles + XCDS - 1) / XCDS));
+    heat_sweep<KM, CM, TY><<<blocks, dim3(64, TY), 0, s->stream>>>(a);
+}
+
 template <int KM, int CM>
 static void launch_sweep_t(const pm_solid3d *s, const HeatArgs &a, dim3 g) {
     switch (s->tile_y) {
-        case 4: heat_sweep<KM, CM, 4><<<g, dim3(64, 4), 0, s->stream>>>(a); break;
-        case 16: heat_sweep<KM, CM, 16><<<g, dim3(64, 16), 0, s->stream>>>(a); break;
-        default: heat_sweep<KM, CM, 8><<<g, dim3(64, 8), 0, s->stream>>>(a); break;
+        case 4: launch_sweep_pd<KM, CM, 4>(s, a, g); break;
+        case 8: launch_sweep_pd<KM, CM, 8>(s, a, g); break;
+        default: launch_sweep_pd<KM, CM, 16>(s, a, g); break;
     }
 }
 
@@ -396,6 +480,7 @@ static int sweep_once(pm_solid3d *s, double dt, int64_t step, double *totals_dev
     a.cap0 = s->cap0;
     a.tref = s->reference;
     a.nx = (int)s->nx, a.ny = (int)s->ny, a.nz = (int)s->nz, a.zc = s->z_chunk;
+    a.gx = (int)g.x, a.gy = (int)g.y, a.gz = (int)g.z;
     const bool prof = s->profiling && step >= 0 && 3 * (step + 1) <= (int64_t)s->prof.size();
     if (prof) PM_HIP(hipEventRecord(s->prof[3 * step], s->stream));
     if (s->kmode == K_UNIFORM && s->cmode == C_UNIFORM)
@@ -887,19 +972,19 @@ double pm_solid3d_heat_capacity(pm_solid3d *s) {
     return total;
 }
 
-int pm_solid3d_set_tile(pm_solid3d *s, int tile_y, int z_chunk) {
+int pm_solid3d_set_tile(pm_solid3d *s, int tile_y, int z_chunk, int prefetch) {
     if (!s) return PM_ERR_ARGUMENT;
-    if (tile_y) {
-        if (tile_y != 4 && tile_y != 8 && tile_y != 16) {
-            set_error("tile_y must be 4, 8 or 16");
-            return PM_ERR_ARGUMENT;
-        }
-        s->tile_y = tile_y;
+    if (tile_y && tile_y != 4 && tile_y != 8 && tile_y != 16) {
+        set_error("tile_y must be 4, 8 or 16");
+        return PM_ERR_ARGUMENT;
     }
-    if (z_chunk) {
-        if (z_chunk < 1) return PM_ERR_ARGUMENT;
-        s->z_chunk = z_chunk;
+    if (z_chunk < 0 || prefetch < 0 || prefetch > 3) {
+        set_error("z_chunk must be positive and prefetch 1, 2 or 3");
+        return PM_ERR_ARGUMENT;
     }
+    if (tile_y) s->tile_y = tile_y;
+    if (z_chunk) s->z_chunk = z_chunk;
+    if (prefetch) s->prefetch = prefetch;
     s->partial_count = 0;
     return PM_OK;
 }

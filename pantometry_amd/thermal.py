@@ -304,8 +304,8 @@ class Solid3D:
                                        C.byref(v)), "pm_solid3d_advance", v)
         return (totals[:steps] if want_totals else None), ms.value
 
-    def set_tile(self, tile_y: int = 0, z_chunk: int = 0) -> None:
-        check(lib().pm_solid3d_set_tile(self._h, tile_y, z_chunk), "pm_solid3d_set_tile")
+    def set_tile(self, tile_y: int = 0, z_chunk: int = 0, prefetch: int = 0) -> None:
+        check(lib().pm_solid3d_set_tile(self._h, tile_y, z_chunk, prefetch), "pm_solid3d_set_tile")
 
     def set_profiling(self, on: bool) -> None:
         check(lib().pm_solid3d_set_profiling(self._h, 1 if on else 0), "pm_solid3d_set_profiling")

@@ -72,6 +72,12 @@ const char *pm_last_error(void);
 /* Name, compute-unit count and HBM size of the current device; PM_ERR_DEVICE without one. */
 int pm_device_info(char *name, size_t name_len, int *compute_units, size_t *hbm_bytes);
 
+/* A streaming kernel of known traffic, c[i] = a[i] + b[i] over n doubles (16 B read, 8 B
+ * written per element), run `reps` times; the mean device time per launch.  Calibrates the HBM
+ * counters and measures the bandwidth a read-read-write stream reaches.  No reference
+ * counterpart (measurement only). */
+int pm_stream_copy(size_t n, int reps, double *ms_per_launch);
+
 /* ---------------------------------------------------------------------------------------- */
 /* Rng, crates/pantometry-core/src/rng.rs:37-152 (host-side: seeded inputs are generated once */
 /* on the host and uploaded, so libm's exp/ln/cos are never recomputed on the device).       */

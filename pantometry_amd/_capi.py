@@ -45,6 +45,7 @@ _VIOL = C.POINTER(pm_violation)
 SIGNATURES = {
     "pm_last_error": (C.c_char_p, []),
     "pm_device_info": (_I, [C.c_char_p, _S, C.POINTER(_I), C.POINTER(_S)]),
+    "pm_stream_copy": (_I, [_S, _I, _DP]),
     "pm_rng_new": (_U64, [_U64]),
     "pm_rng_for_index": (_U64, [_U64, _U64]),
     "pm_rng_unit": (_D, [C.POINTER(_U64)]),

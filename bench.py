@@ -137,7 +137,8 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the heat block is not sharded across GPUs "
-              "(see DESIGN.md for how it would be); run with --gpus 1", file=sys.stderr)
+              "(it would shard by z-slabs with one plane of halo exchanged per step); "
+              "run with --gpus 1", file=sys.stderr)
         return 2
 
     import pantometry_amd as pm

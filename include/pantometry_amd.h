@@ -6,8 +6,8 @@
  *
  * Every entry point takes plain pointers and sizes.  Each declaration cites the reference
  * interface it replaces (paths relative to the pantometry repository).  The Domain trait those
- * interfaces implement is crates/pantometry-core/src/sim.rs:77-252; its Rust binding over this
- * header is shown in INTEGRATION.md.
+ * interfaces implement is crates/pantometry-core/src/sim.rs:77-252; a Rust binding implements
+ * Domain by calling these entry points across extern "C".
  *
  * Conventions
  *   - Status: PM_OK (0); PM_VIOLATION (1) when a step is refused or an audit breaks, with the
@@ -129,6 +129,9 @@ int pm_solid3d_set_conductivity(pm_solid3d *solid, const double *conductivity);
  * Solid3D::temperature_at over every cell (solid.rs:1323). */
 int pm_solid3d_set_cells(pm_solid3d *solid, const double *cells);
 int pm_solid3d_get_cells(pm_solid3d *solid, double *cells);
+/* dst's temperatures := src's, on the device (same extents).  No reference counterpart: Solid3D
+ * is Clone (solid.rs:172-173); this copies the field without the operator. */
+int pm_solid3d_copy_cells(pm_solid3d *dst, pm_solid3d *src);
 int pm_solid3d_set_temperature(pm_solid3d *solid, size_t i, size_t j, size_t k, double kelvin);
 /* Solid3D::deposit, solid.rs:1368-1375: joules into one cell, counted as absorbed. */
 int pm_solid3d_deposit(pm_solid3d *solid, size_t i, size_t j, size_t k, double joules);
@@ -250,6 +253,63 @@ int pm_ensemble_set_schedule(pm_ensemble *ens, int resident_edge, int z_chunk);
  * and keep the fastest.  `report` (optional, `report_len` bytes) receives one line per
  * candidate. */
 int pm_ensemble_tune(pm_ensemble *ens, char *report, size_t report_len);
+
+/* ---- The learned heat surrogate ------------------------------------------------------------
+ * No reference counterpart: pantometry has no learned component.  A four-level 3D U-Net
+ * (channels 16-32-64-128, two 3x3x3 convolutions with tanh-GELU per level, 2x2x2 average pool,
+ * nearest upsampling with skip concatenation, 1x1 head; 1,466,065 parameters) mapping a cubic
+ * block's (T, log k) and the stencil's per-step coupling a = dt * dx / C to (T_16 - T_0) / 10 K,
+ * the change over 16 stencil steps, trained with Adam on bf16 matrix-core kernels (fp32
+ * accumulation, fp32 master weights).  Its sixteen input channels are (T - 300 K) / 100 K,
+ * log k / 0.5, and per face (x-, x+, y-, y+, z-, z+) 10 a k_f and a k_f (T_nb - T) * 16 / 10 K
+ * (k_f the harmonic face mean, zero on the insulated surface), their sum, and a constant one.
+ * Edges must be powers of two from 8.  Parameters are one flat fp32 array, layer by layer:
+ * weights [Cout][27][Cin] (tap t = 9 dz + 3 dy + dx) then biases [Cout]; the head's 16 weights
+ * and 1 bias last. */
+typedef struct pm_surrogate pm_surrogate;
+
+int pm_surrogate_create(uint64_t seed, pm_surrogate **out);
+void pm_surrogate_destroy(pm_surrogate *s);
+int64_t pm_surrogate_parameter_count(const pm_surrogate *s);
+int pm_surrogate_get_parameters(pm_surrogate *s, float *out);
+int pm_surrogate_set_parameters(pm_surrogate *s, const float *in);
+/* Reserves a device-resident training set of `samples` pairs of edge^3 cells (T_0 and k as fp32,
+ * the normalised change as fp32, the coupling per pair). */
+int pm_surrogate_reserve_set(pm_surrogate *s, int64_t samples, int64_t edge);
+/* Pair `index`: T_0, conductivity and T_16 (kelvin, W/m/K; x fastest) and the coupling
+ * a = dt * dx / C of the steps between them. */
+int pm_surrogate_set_sample(pm_surrogate *s, int64_t index, const double *t0, const double *k,
+                            const double *tk, double coupling);
+/* Pair `index` taken from two Solid3D blocks on the device: the state before and after 16
+ * stencil steps of `dt` (both cubic with the set's edge, the same per-cell conductivity and a
+ * uniform capacity). */
+int pm_surrogate_set_sample_from(pm_surrogate *s, int64_t index, pm_solid3d *before,
+                                 pm_solid3d *after, double dt);
+/* One Adam step (beta 0.9/0.999, eps 1e-8) on the set's pairs `indices[0..batch)`; `loss`
+ * receives 0.5 * mean squared error of the normalised change, before the step. */
+int pm_surrogate_train_step(pm_surrogate *s, const int *indices, int batch, float lr,
+                            double *loss);
+/* The loss and its gradient with respect to every parameter (flat layout), no step taken. */
+int pm_surrogate_gradients(pm_surrogate *s, const int *indices, int batch, float *grads,
+                           double *loss);
+/* The network's normalised output for one block of edge^3 cells and its coupling. */
+int pm_surrogate_forward(pm_surrogate *s, int64_t edge, const double *t, const double *k,
+                         double coupling, float *y);
+/* One surrogate step, standing in for 16 stencil steps of `dt`, of a cubic block with per-cell
+ * conductivity and uniform capacity, in place: T <- T + 10 K * net, shifted uniformly on the
+ * device so the block's total heat is unchanged.  `drift` receives the relative change of
+ * sum(T - reference) across the step, `elapsed_ms` the device time. */
+int pm_surrogate_step_solid(pm_surrogate *s, pm_solid3d *block, double dt, double reference,
+                            double *drift, double *elapsed_ms);
+/* ||T_a - T_b||_2 / ||T_b - reference||_2 over two blocks of the same extents. */
+int pm_solid3d_relative_l2(pm_solid3d *a, pm_solid3d *b, double reference, double *rel);
+/* One bf16 convolution on host fp32 arrays (rounded to bf16 on the way in): x [B][D][H][W][Cin],
+ * w [Cout][27][Cin], bias [Cout] -> y [B][D][H][W][Cout] before the activation. */
+int pm_nn_conv3d(int B, int D, int H, int W, int Cin, int Cout, const float *x, const float *w,
+                 const float *bias, float *y);
+/* Its gradients for an upstream gradient dy: dx (bf16-rounded), dw and db (fp32). */
+int pm_nn_conv3d_backward(int B, int D, int H, int W, int Cin, int Cout, const float *x,
+                          const float *w, const float *dy, float *dx, float *dw, float *db);
 
 #ifdef __cplusplus
 }

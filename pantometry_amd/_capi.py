@@ -35,6 +35,7 @@ class pm_member(C.Structure):
 
 _D = C.c_double
 _DP = C.POINTER(C.c_double)
+_FP = C.POINTER(C.c_float)
 _VP = C.c_void_p
 _S = C.c_size_t
 _U64 = C.c_uint64
@@ -100,6 +101,22 @@ SIGNATURES = {
     "pm_ensemble_ledgers": (_I, [_VP, _DP]),
     "pm_ensemble_set_schedule": (_I, [_VP, _I, _I]),
     "pm_ensemble_tune": (_I, [_VP, C.c_char_p, _S]),
+    "pm_solid3d_copy_cells": (_I, [_VP, _VP]),
+    "pm_surrogate_create": (_I, [_U64, C.POINTER(_VP)]),
+    "pm_surrogate_destroy": (None, [_VP]),
+    "pm_surrogate_parameter_count": (C.c_int64, [_VP]),
+    "pm_surrogate_get_parameters": (_I, [_VP, _FP]),
+    "pm_surrogate_set_parameters": (_I, [_VP, _FP]),
+    "pm_surrogate_reserve_set": (_I, [_VP, C.c_int64, C.c_int64]),
+    "pm_surrogate_set_sample": (_I, [_VP, C.c_int64, _DP, _DP, _DP, _D]),
+    "pm_surrogate_set_sample_from": (_I, [_VP, C.c_int64, _VP, _VP, _D]),
+    "pm_surrogate_train_step": (_I, [_VP, C.POINTER(_I), _I, C.c_float, _DP]),
+    "pm_surrogate_gradients": (_I, [_VP, C.POINTER(_I), _I, _FP, _DP]),
+    "pm_surrogate_forward": (_I, [_VP, C.c_int64, _DP, _DP, _D, _FP]),
+    "pm_surrogate_step_solid": (_I, [_VP, _VP, _D, _D, _DP, _DP]),
+    "pm_solid3d_relative_l2": (_I, [_VP, _VP, _D, _DP]),
+    "pm_nn_conv3d": (_I, [_I, _I, _I, _I, _I, _I, _FP, _FP, _FP, _FP]),
+    "pm_nn_conv3d_backward": (_I, [_I, _I, _I, _I, _I, _I, _FP, _FP, _FP, _FP, _FP, _FP]),
 }
 
 _lib = None
@@ -145,3 +162,8 @@ def check(status: int, what: str, violation: pm_violation | None = None) -> None
 def dptr(a) -> "C.POINTER(C.c_double)":
     """A double* into a C-contiguous float64 numpy array."""
     return a.ctypes.data_as(_DP)
+
+
+def fptr(a) -> "C.POINTER(C.c_float)":
+    """A float* into a C-contiguous float32 numpy array."""
+    return a.ctypes.data_as(_FP)

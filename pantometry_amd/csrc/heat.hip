@@ -799,6 +799,19 @@ int pm_solid3d_set_cells(pm_solid3d *s, const double *cells) {
     return PM_OK;
 }
 
+int pm_solid3d_copy_cells(pm_solid3d *dst, pm_solid3d *src) {
+    if (!dst || !src) return PM_ERR_ARGUMENT;
+    if (dst->nx != src->nx || dst->ny != src->ny || dst->nz != src->nz) {
+        set_error("pm_solid3d_copy_cells: extents differ");
+        return PM_ERR_ARGUMENT;
+    }
+    if (dst == src) return PM_OK;
+    PM_HIP(hipStreamSynchronize(src->stream));
+    PM_HIP(hipMemcpyAsync(dst->T(), src->T(), sizeof(double) * (size_t)dst->n(), hipMemcpyDeviceToDevice, dst->stream));
+    PM_HIP(hipStreamSynchronize(dst->stream));
+    return PM_OK;
+}
+
 int pm_solid3d_get_cells(pm_solid3d *s, double *cells) {
     if (!s || !cells) return PM_ERR_ARGUMENT;
     PM_HIP(hipMemcpyAsync(cells, s->T(), sizeof(double) * (size_t)s->n(), hipMemcpyDeviceToHost, s->stream));
@@ -1003,3 +1016,22 @@ int pm_solid3d_kernel_times(pm_solid3d *s, double *sweep_ms, double *audit_ms) {
 }
 
 }  // extern "C"
+
+namespace pm {
+
+// Device view of a block for the surrogate: the current temperatures, the per-cell conductivity
+// (null when uniform), the extents, the block's stream, whether the capacity is uniform and, if
+// it is, the mobility dx / C.
+int solid_device_state(pm_solid3d *s, double **cells, const double **kcell, int64_t counts[3],
+                       hipStream_t *stream, bool *uniform_capacity, double *mobility) {
+    if (!s) return PM_ERR_ARGUMENT;
+    *cells = s->T();
+    *kcell = s->kmode == K_PER_CELL ? s->kcell.d() : nullptr;
+    counts[0] = s->nx, counts[1] = s->ny, counts[2] = s->nz;
+    *stream = s->stream;
+    if (uniform_capacity) *uniform_capacity = s->cmode == C_UNIFORM;
+    if (mobility) *mobility = s->mob0;
+    return PM_OK;
+}
+
+}  // namespace pm

@@ -1,0 +1,314 @@
+// Implicit-GEMM 3x3x3 convolutions on the gfx950 bf16 matrix cores, and their gradients.
+//
+// Forward: a workgroup of four waves computes a 64-position M-tile (a TD x TH x TW box of one
+// sample) against an N-block of up to 64 output channels.  For each 16-channel chunk of the
+// input, the box's input neighbourhood ((TD+2) x (TH+2) x (TW+2) positions, zero outside the
+// volume) is staged once in LDS, then every wave runs the 27 taps as MFMA k-steps: the A fragment
+// (16 positions x 16 channels) comes from LDS at the tap's shift, the B fragment (16 channels x 16
+// output channels) straight from the weight tensor.  One staged value feeds 27 taps.
+//
+// MFMA v_mfma_f32_16x16x16_bf16 lane layout (checked on the device): lane l holds A[l%16][4(l/16)
+// + j], B[4(l/16) + j][l%16] for j < 4, and C[4(l/16) + r][l%16] for r < 4.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+
+#include "nn.hpp"
+
+namespace pm {
+namespace nn {
+
+typedef short short4v __attribute__((ext_vector_type(4)));
+typedef float float4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float4v mfma16(short4v a, short4v b, float4v c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// Decomposes an M-tile index into the box origin (b, d0, h0, w0).
+__device__ __forceinline__ void tile_origin(const Conv &g, int64_t tile, int &b, int &d0, int &h0,
+                                            int &w0) {
+    const int tw = g.W / g.TW, th = g.H / g.TH, td = g.D / g.TD;
+    int64_t t = tile;
+    const int wt = (int)(t % tw);
+    t /= tw;
+    const int ht = (int)(t % th);
+    t /= th;
+    const int dt = (int)(t % td);
+    b = (int)(t / td);
+    d0 = dt * g.TD;
+    h0 = ht * g.TH;
+    w0 = wt * g.TW;
+}
+
+// Stages channels [c0, c0+16) of the box's neighbourhood in x (channel stride cs) into `s`:
+// s[((pz * (TH+2) + py) * (TW+2) + px) * 16 + c], zero outside the volume.
+__device__ __forceinline__ void stage_neighbourhood(const Conv &g, const bf16 *__restrict__ x,
+                                                   int cs, int c0, int b, int d0, int h0, int w0,
+                                                   bf16 *s, int tid, int nthreads) {
+    const int SH = g.TH + 2, SW = g.TW + 2;
+    const int npos = (g.TD + 2) * SH * SW;
+    // Two 16-byte halves per position.
+    for (int e = tid; e < npos * 2; e += nthreads) {
+        const int half = e & 1, p = e >> 1;
+        const int px = p % SW, py = (p / SW) % SH, pz = p / (SW * SH);
+        const int z = d0 + pz - 1, y = h0 + py - 1, xx = w0 + px - 1;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (z >= 0 && z < g.D && y >= 0 && y < g.H && xx >= 0 && xx < g.W) {
+            const int64_t pos = (((int64_t)b * g.D + z) * g.H + y) * g.W + xx;
+            v = *(const uint4 *)(x + pos * cs + c0 + half * 8);
+        }
+        *(uint4 *)(s + p * 16 + half * 8) = v;
+    }
+}
+
+// Box coordinates of tile position q (q < tile_positions()).
+__device__ __forceinline__ void box_coords(const Conv &g, int q, int &qd, int &qh, int &qw) {
+    qw = q % g.TW;
+    qh = (q / g.TW) % g.TH;
+    qd = q / (g.TW * g.TH);
+}
+
+template <int NB>
+__global__ __launch_bounds__(256) void conv_fwd_kernel(Conv g, const bf16 *__restrict__ x,
+                                                       const bf16 *__restrict__ w,
+                                                       const float *__restrict__ bias,
+                                                       bf16 *__restrict__ zout,
+                                                       bf16 *__restrict__ aout, int gelu_out,
+                                                       int ostride, int ooff) {
+    constexpr int NT = NB / 16;  // MFMA column tiles per wave
+    extern __shared__ __align__(16) bf16 smem[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t tile = blockIdx.x;
+    const int n0 = blockIdx.y * NB;
+    int b, d0, h0, w0;
+    tile_origin(g, tile, b, d0, h0, w0);
+    const int SH = g.TH + 2, SW = g.TW + 2;
+    const int TP = g.tile_positions();
+
+    // This lane's A row: position p = wave*16 + lane%16 of the box; rows past the box are zero.
+    const int p = wave * 16 + (lane & 15);
+    const bool row_live = p < TP;
+    int pd, ph, pw;
+    box_coords(g, row_live ? p : 0, pd, ph, pw);
+    const int kq = (lane >> 4) * 4;  // the four k (channel) entries this lane carries
+
+    float4v acc[NT];
+#pragma unroll
+    for (int n = 0; n < NT; n++) acc[n] = float4v{0.f, 0.f, 0.f, 0.f};
+
+    const int64_t wrow = (int64_t)27 * g.Cin;
+    for (int c0 = 0; c0 < g.Cin; c0 += 16) {
+        __syncthreads();
+        stage_neighbourhood(g, x, g.Cin, c0, b, d0, h0, w0, smem, tid, 256);
+        __syncthreads();
+#pragma unroll 3
+        for (int t = 0; t < 27; t++) {
+            const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;
+            const int sp = ((pd + dz) * SH + ph + dy) * SW + pw + dx;
+            short4v a = *(const short4v *)(smem + sp * 16 + kq);
+            if (!row_live) a = short4v{0, 0, 0, 0};
+#pragma unroll
+            for (int n = 0; n < NT; n++) {
+                const int co = n0 + n * 16 + (lane & 15);
+                const short4v bv = *(const short4v *)(w + co * wrow + t * g.Cin + c0 + kq);
+                acc[n] = mfma16(a, bv, acc[n]);
+            }
+        }
+    }
+
+    // Epilogue: rows are positions 4*(lane/16) + r of this wave's 16, the column is one channel.
+#pragma unroll
+    for (int n = 0; n < NT; n++) {
+        const int co = n0 + n * 16 + (lane & 15);
+        const float bb = bias ? bias[co] : 0.0f;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int q = wave * 16 + (lane >> 4) * 4 + r;
+            if (q >= TP) continue;
+            int qd, qh, qw;
+            box_coords(g, q, qd, qh, qw);
+            const int64_t pos = (((int64_t)b * g.D + d0 + qd) * g.H + h0 + qh) * g.W + w0 + qw;
+            const float zz = acc[n][r] + bb;
+            const int64_t o = pos * ostride + ooff + co;
+            if (zout) zout[o] = to_bf16(zz);
+            if (aout) aout[o] = to_bf16(gelu_out ? gelu(zz) : zz);
+        }
+    }
+}
+
+// W'[ci][t][co] = W[co][26 - t][ci]: the data gradient is the forward convolution of dy with the
+// flipped, transposed weights.
+__global__ void flip_weights(const bf16 *__restrict__ w, bf16 *__restrict__ wf, int Cin, int Cout) {
+    const int64_t n = (int64_t)Cout * 27 * Cin;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int ci = (int)(i % Cin);
+        const int t = (int)((i / Cin) % 27);
+        const int co = (int)(i / ((int64_t)Cin * 27));
+        wf[((int64_t)ci * 27 + (26 - t)) * Cout + co] = w[i];
+    }
+}
+
+// Weight gradient for a 16 (co) x 16 (ci) block over all 27 taps: wave w owns taps w, w+4, ...
+// The workgroup walks the M-tiles tile = split, split + splits, ...; per tile it stages the input
+// neighbourhood (ci block) and dy (co block) in LDS, then each wave runs one k-step per 16
+// positions of the box per tap.  Partials go to scratch[split][co][t][ci].
+__global__ __launch_bounds__(256) void conv_wgrad_kernel(Conv g, const bf16 *__restrict__ x,
+                                                         const bf16 *__restrict__ dy,
+                                                         float *__restrict__ part, int splits) {
+    extern __shared__ __align__(16) bf16 smem[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int ci0 = blockIdx.x * 16, co0 = blockIdx.y * 16, split = blockIdx.z;
+    const int SH = g.TH + 2, SW = g.TW + 2;
+    const int TP = g.tile_positions();
+    const int KS = (TP + 15) / 16;
+    bf16 *sx = smem;                                  // neighbourhood, 16 channels per position
+    bf16 *sd = smem + (g.TD + 2) * SH * SW * 16;      // dy box, [64 positions][16 co]
+    float4v acc[7];
+#pragma unroll
+    for (int i = 0; i < 7; i++) acc[i] = float4v{0.f, 0.f, 0.f, 0.f};
+    const int kq = (lane >> 4) * 4;
+    const int64_t ntiles = g.tiles();
+    for (int64_t tile = split; tile < ntiles; tile += splits) {
+        int b, d0, h0, w0;
+        tile_origin(g, tile, b, d0, h0, w0);
+        __syncthreads();
+        stage_neighbourhood(g, x, g.Cin, ci0, b, d0, h0, w0, sx, tid, 256);
+        for (int e = tid; e < 64 * 2; e += 256) {
+            const int half = e & 1, q = e >> 1;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (q < TP) {
+                int qd, qh, qw;
+                box_coords(g, q, qd, qh, qw);
+                const int64_t pos = (((int64_t)b * g.D + d0 + qd) * g.H + h0 + qh) * g.W + w0 + qw;
+                v = *(const uint4 *)(dy + pos * g.Cout + co0 + half * 8);
+            }
+            *(uint4 *)(sd + q * 16 + half * 8) = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 7; i++) {
+            const int t = wave + 4 * i;
+            if (t < 27) {
+                const int dz = t / 9, dyy = (t / 3) % 3, dx = t % 3;
+                for (int ks = 0; ks < KS; ks++) {
+                    short4v av, bv;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const int q = ks * 16 + kq + j;  // position within the box
+                        int qd, qh, qw;
+                        box_coords(g, q < TP ? q : 0, qd, qh, qw);
+                        av[j] = (short)sd[q * 16 + (lane & 15)];  // zero past the box
+                        const int sp = ((qd + dz) * SH + qh + dyy) * SW + qw + dx;
+                        bv[j] = (short)sx[sp * 16 + (lane & 15)];
+                    }
+                    acc[i] = mfma16(av, bv, acc[i]);
+                }
+            }
+        }
+    }
+    // C[row = co][col = ci]: lane holds rows 4*(lane/16) + r, column lane%16.
+#pragma unroll
+    for (int i = 0; i < 7; i++) {
+        const int t = wave + 4 * i;
+        if (t >= 27) continue;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int co = co0 + (lane >> 4) * 4 + r, ci = ci0 + (lane & 15);
+            part[(((int64_t)split * g.Cout + co) * 27 + t) * g.Cin + ci] = acc[i][r];
+        }
+    }
+}
+
+// dw[i] = sum over splits of part[s][i], in split order.
+__global__ void reduce_splits(const float *__restrict__ part, float *__restrict__ out, int64_t n,
+                              int splits) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        float s = 0.0f;
+        for (int k = 0; k < splits; k++) s += part[k * n + i];
+        out[i] = s;
+    }
+}
+
+// db[c] = sum over positions of dy[p][c]: one workgroup per channel, fixed order.
+__global__ __launch_bounds__(256) void bias_grad(const bf16 *__restrict__ dy, int64_t npos, int C,
+                                                 float *__restrict__ db) {
+    __shared__ float red[256];
+    const int c = blockIdx.x;
+    float s = 0.0f;
+    for (int64_t p = threadIdx.x; p < npos; p += 256) s += from_bf16(dy[p * C + c]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) db[c] = red[0];
+}
+
+static unsigned blocks_for(int64_t n, int threads) {
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + threads - 1) / threads, 1 << 20));
+}
+
+// The kernels' assumptions: channel counts in whole 16-channel blocks, a box that tiles the
+// volume with at most 64 positions, and a box of 64 unless it is the whole sample.
+static bool conv_shape_ok(const Conv &g) {
+    const bool ok = g.B > 0 && g.Cin > 0 && g.Cout > 0 && g.Cin % 16 == 0 && g.Cout % 16 == 0 &&
+                    g.TD > 0 && g.TH > 0 && g.TW > 0 && g.D % g.TD == 0 && g.H % g.TH == 0 &&
+                    g.W % g.TW == 0 && g.tile_positions() <= 64 &&
+                    (g.tile_positions() == 64 || (g.TD == g.D && g.TH == g.H && g.TW == g.W));
+    if (!ok) set_error("convolution shape %dx%dx%dx%d, %d->%d channels, box %dx%dx%d unsupported",
+                       g.B, g.D, g.H, g.W, g.Cin, g.Cout, g.TD, g.TH, g.TW);
+    return ok;
+}
+
+int conv_forward(const Conv &g, const bf16 *x, const bf16 *w, const float *bias, bf16 *z, bf16 *a,
+                 bool gelu_out, int ostride, int ooff, hipStream_t stream) {
+    if (!conv_shape_ok(g)) return PM_ERR_ARGUMENT;
+    const size_t lds = (size_t)(g.TD + 2) * (g.TH + 2) * (g.TW + 2) * 16 * sizeof(bf16);
+    const int nb = g.Cout % 64 == 0 ? 64 : g.Cout == 48 ? 48 : g.Cout % 32 == 0 ? 32 : 16;
+    dim3 grid((unsigned)g.tiles(), (unsigned)(g.Cout / nb));
+    switch (nb) {
+        case 16: conv_fwd_kernel<16><<<grid, 256, lds, stream>>>(g, x, w, bias, z, a, gelu_out, ostride, ooff); break;
+        case 32: conv_fwd_kernel<32><<<grid, 256, lds, stream>>>(g, x, w, bias, z, a, gelu_out, ostride, ooff); break;
+        case 48: conv_fwd_kernel<48><<<grid, 256, lds, stream>>>(g, x, w, bias, z, a, gelu_out, ostride, ooff); break;
+        default: conv_fwd_kernel<64><<<grid, 256, lds, stream>>>(g, x, w, bias, z, a, gelu_out, ostride, ooff); break;
+    }
+    PM_HIP(hipGetLastError());
+    return PM_OK;
+}
+
+int conv_backward_data(const Conv &g, const bf16 *dy, const bf16 *w, bf16 *wf, bf16 *dx,
+                       hipStream_t stream) {
+    const int64_t n = (int64_t)g.Cout * 27 * g.Cin;
+    flip_weights<<<blocks_for(n, 256), 256, 0, stream>>>(w, wf, g.Cin, g.Cout);
+    PM_HIP(hipGetLastError());
+    Conv t = g;
+    t.Cin = g.Cout;
+    t.Cout = g.Cin;
+    return conv_forward(t, dy, wf, nullptr, nullptr, dx, false, t.Cout, 0, stream);
+}
+
+int conv_backward_weight(const Conv &g, const bf16 *x, const bf16 *dy, float *dw, float *db,
+                         float *scratch, int splits, hipStream_t stream) {
+    if (!conv_shape_ok(g) || splits < 1) return PM_ERR_ARGUMENT;
+    const size_t lds = ((size_t)(g.TD + 2) * (g.TH + 2) * (g.TW + 2) * 16 + 64 * 16) * sizeof(bf16);
+    dim3 grid((unsigned)(g.Cin / 16), (unsigned)(g.Cout / 16), (unsigned)splits);
+    conv_wgrad_kernel<<<grid, 256, lds, stream>>>(g, x, dy, scratch, splits);
+    PM_HIP(hipGetLastError());
+    const int64_t n = (int64_t)g.Cout * 27 * g.Cin;
+    reduce_splits<<<blocks_for(n, 256), 256, 0, stream>>>(scratch, dw, n, splits);
+    PM_HIP(hipGetLastError());
+    if (db) {
+        bias_grad<<<g.Cout, 256, 0, stream>>>(dy, g.positions(), g.Cout, db);
+        PM_HIP(hipGetLastError());
+    }
+    return PM_OK;
+}
+
+}  // namespace nn
+}  // namespace pm

@@ -1,0 +1,83 @@
+// Building blocks of the learned heat surrogate on gfx950: 3x3x3 convolutions as implicit GEMMs
+// on the bf16 matrix cores (v_mfma_f32_16x16x16_bf16, fp32 accumulation), and the element-wise,
+// pooling and optimizer kernels around them.
+//
+// Activations are channels-last bf16, [B][D][H][W][C] with C a multiple of 16; weights are
+// [Cout][27][Cin] bf16 (tap t = (dz*3 + dy)*3 + dx, offsets dz-1, dy-1, dx-1), with fp32 master
+// copies for the optimizer.  Padding is zero ("same" convolution).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "common.hpp"
+
+namespace pm {
+namespace nn {
+
+using bf16 = uint16_t;
+
+// Round-to-nearest-even fp32 -> bf16 and exact bf16 -> fp32 (finite values).
+__host__ __device__ __forceinline__ bf16 to_bf16(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (bf16)(u >> 16);
+}
+__host__ __device__ __forceinline__ float from_bf16(bf16 h) {
+    return __builtin_bit_cast(float, (uint32_t)h << 16);
+}
+
+// tanh-form GELU and its derivative.
+__device__ __forceinline__ float gelu(float z) {
+    const float k = 0.7978845608028654f;
+    return 0.5f * z * (1.0f + tanhf(k * (z + 0.044715f * z * z * z)));
+}
+__device__ __forceinline__ float gelu_grad(float z) {
+    const float k = 0.7978845608028654f;
+    const float t = tanhf(k * (z + 0.044715f * z * z * z));
+    return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * k * (1.0f + 3.0f * 0.044715f * z * z);
+}
+
+// Shape of one convolution: the output has the input's spatial extent.  An M-tile is a box of
+// TD x TH x TW output positions of one sample, at most 64 of them (fewer only when the whole
+// volume is smaller); rows of the 64-row MFMA tile past the box are masked.
+struct Conv {
+    int B, D, H, W;
+    int Cin, Cout;
+    int TD, TH, TW;
+    __host__ __device__ int64_t positions() const { return (int64_t)B * D * H * W; }
+    __host__ __device__ int tile_positions() const { return TD * TH * TW; }
+    __host__ __device__ int64_t tiles() const {
+        return (int64_t)B * (D / TD) * (H / TH) * (W / TW);
+    }
+};
+
+// Chooses the M-tile: as much of a row as fits, then rows, then planes.  Every extent here is a
+// power of two, so the box divides the volume.
+inline Conv make_conv(int B, int D, int H, int W, int Cin, int Cout) {
+    Conv c{B, D, H, W, Cin, Cout, 1, 1, 64};
+    c.TW = W < 64 ? W : 64;
+    c.TH = H < 64 / c.TW ? H : 64 / c.TW;
+    c.TD = D < 64 / (c.TW * c.TH) ? D : 64 / (c.TW * c.TH);
+    return c;
+}
+
+// Host launchers (surrogate.hip).  All pointers are device pointers; `stream` orders them.
+//
+// y = conv(x, w) + bias; z (pre-activation) and a = act(z) stored as bf16 when non-null, with
+// act = GELU when `gelu_out`, identity otherwise.  Output channel c is written at
+// (position * ostride + ooff + c), so a convolution can write into a wider buffer.
+int conv_forward(const Conv &g, const bf16 *x, const bf16 *w, const float *bias, bf16 *z,
+                 bf16 *a, bool gelu_out, int ostride, int ooff, hipStream_t stream);
+// dx = conv(dy, flip(w)^T): the gradient of the input, fp32 accumulation, bf16 out.
+int conv_backward_data(const Conv &g, const bf16 *dy, const bf16 *w, bf16 *w_flip_scratch,
+                       bf16 *dx, hipStream_t stream);
+// dw[co][t][ci] = sum_p dy[p][co] * x[p + off(t)][ci] (fp32), db[co] = sum_p dy[p][co];
+// split over position tiles and reduced in a fixed order.  `scratch` holds
+// splits * Cout * 27 * Cin floats.
+int conv_backward_weight(const Conv &g, const bf16 *x, const bf16 *dy, float *dw, float *db,
+                         float *scratch, int splits, hipStream_t stream);
+
+}  // namespace nn
+}  // namespace pm

@@ -168,6 +168,17 @@ class Solid3D:
             raise ValueError("one temperature per cell")
         check(lib().pm_solid3d_set_cells(self._h, dptr(c)), "pm_solid3d_set_cells")
 
+    def copy_cells_from(self, other: "Solid3D") -> None:
+        """This block's temperatures := `other`'s, device to device (same extents)."""
+        check(lib().pm_solid3d_copy_cells(self._h, other._h), "pm_solid3d_copy_cells")
+
+    def relative_l2(self, other: "Solid3D", reference: float) -> float:
+        """||T - T_other|| / ||T_other - reference||, computed on the device."""
+        out = C.c_double(0.0)
+        check(lib().pm_solid3d_relative_l2(self._h, other._h, float(reference), C.byref(out)),
+              "pm_solid3d_relative_l2")
+        return out.value
+
     def set_temperature(self, i: int, j: int, k: int, t: float) -> None:
         check(lib().pm_solid3d_set_temperature(self._h, i, j, k, float(t)),
               "pm_solid3d_set_temperature")

@@ -1,0 +1,226 @@
+"""The surrogate's gfx950 kernels against the CPU oracle (oracle/unet.py): one convolution and its
+gradients over every tile shape the network uses, the whole network's forward pass and loss
+gradient, the device initialisation, the conservative surrogate step, training from GPU-stencil
+data, and the audit-vetoed rollout.
+
+Tolerances.  The device rounds to bf16 wherever it stores an activation or gradient and
+accumulates in fp32; the oracle, run with the same bf16 rounding points in fp64, differs only by
+accumulation order.  A single convolution output is within one bf16 rounding (2^-9 relative) plus
+the fp32 sum's error; the fp32 weight gradients within 1e-4.  Through the fourteen layers a
+differing fp32 sum can flip an occasional bf16 rounding by one ulp (2^-8), which the 1e-2 relative
+L2 bound on each layer's gradient leaves ample room for."""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from tests.conftest import gpu_case
+from oracle import unet as U
+
+pytestmark = pytest.mark.gpu
+
+CONV_SHAPES = [
+    (2, 8, 8, 8, 16, 16),      # 8x8 planes: 8x8x1 boxes
+    (1, 4, 16, 16, 32, 48),    # 16x4x1 boxes, N-block of 48
+    (1, 2, 32, 32, 16, 32),    # 32x2x1 boxes
+    (1, 2, 8, 64, 48, 16),     # full 64-wide rows
+    (1, 4, 4, 64, 192, 64),    # the widest input (decoder level 2's concatenation)
+    (1, 4, 4, 4, 64, 96),      # 4x4x4 boxes, N-blocks of 32
+    (3, 2, 2, 2, 16, 128),     # 8-position samples: masked rows
+    (2, 1, 1, 1, 16, 16),      # one position per sample
+]
+
+
+def _f(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _conv_inputs(shape, seed):
+    B, D, H, W, Cin, Cout = shape
+    g = np.random.default_rng(seed)
+    x = _f(g.standard_normal((B, D, H, W, Cin)))
+    w = _f(g.standard_normal((Cout, 27, Cin)) * math.sqrt(2.0 / (27 * Cin)))
+    b = _f(0.1 * g.standard_normal(Cout))
+    dy = _f(g.standard_normal((B, D, H, W, Cout)))
+    return x, w, b, dy
+
+
+def _fp(a):
+    from pantometry_amd._capi import fptr
+    return fptr(a)
+
+
+@pytest.mark.parametrize("shape", CONV_SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_conv_forward_matches_oracle(shape):
+    gpu_case(f"conv forward {shape}")
+    from pantometry_amd._capi import check, lib
+    B, D, H, W, Cin, Cout = shape
+    x, w, b, _ = _conv_inputs(shape, 1)
+    y = np.empty((B, D, H, W, Cout), dtype=np.float32)
+    check(lib().pm_nn_conv3d(B, D, H, W, Cin, Cout, _fp(x), _fp(w), _fp(b), _fp(y)), "conv")
+    ref = U.conv3d(U.round_bf16(x), U.round_bf16(w), b.astype(np.float64))
+    rms = math.sqrt(float(np.mean(ref ** 2)))
+    assert np.all(np.abs(y - ref) <= 2.0 ** -8 * np.abs(ref) + 1e-4 * rms)
+
+
+@pytest.mark.parametrize("shape", CONV_SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_conv_backward_matches_oracle(shape):
+    gpu_case(f"conv backward {shape}")
+    from pantometry_amd._capi import check, lib
+    B, D, H, W, Cin, Cout = shape
+    x, w, _, dy = _conv_inputs(shape, 2)
+    dx = np.empty_like(x)
+    dw = np.empty_like(w)
+    db = np.empty(Cout, dtype=np.float32)
+    check(lib().pm_nn_conv3d_backward(B, D, H, W, Cin, Cout, _fp(x), _fp(w), _fp(dy), _fp(dx),
+                                      _fp(dw), _fp(db)), "conv backward")
+    rdx, rdw, rdb = U.conv3d_backward(U.round_bf16(x), U.round_bf16(w), U.round_bf16(dy))
+    rms = math.sqrt(float(np.mean(rdx ** 2)))
+    assert np.all(np.abs(dx - rdx) <= 2.0 ** -8 * np.abs(rdx) + 1e-4 * rms)
+    np.testing.assert_allclose(dw, rdw, rtol=1e-4, atol=1e-5 * float(np.abs(rdw).max()))
+    np.testing.assert_allclose(db, rdb, rtol=1e-4, atol=1e-5 * float(np.abs(rdb).max()))
+
+
+def _random_params(seed):
+    g = np.random.default_rng(seed)
+    lay, head, total = U.parameter_layout()
+    p = np.zeros(total, dtype=np.float32)
+    for i, ((cin, cout, _), (wo, bo)) in enumerate(zip(U.LAYERS, lay)):
+        w = g.uniform(-1, 1, (cout, 27, cin)) * math.sqrt(6.0 / (27 * cin))
+        p[wo:wo + cout * 27 * cin] = w.reshape(-1)
+        p[bo:bo + cout] = g.uniform(-0.1, 0.1, cout)
+    p[head:head + 17] = g.uniform(-0.5, 0.5, 17)
+    return p
+
+
+COUPLING = 0.02   # dt * dx / C of a typical heterogeneous block at 0.9 of its limit
+
+
+def _fields(edge, batch, seed):
+    g = np.random.default_rng(seed)
+    t0 = 300.0 + 150.0 * g.random((batch, edge, edge, edge))
+    k = np.exp(0.5 * g.standard_normal((batch, edge, edge, edge)))
+    tk = t0 + 5.0 * g.standard_normal((batch, edge, edge, edge))
+    return t0, k, tk
+
+
+def test_device_initialisation_matches_oracle():
+    gpu_case("surrogate init")
+    from pantometry_amd.surrogate import Surrogate
+    s = Surrogate(seed=12345)
+    assert s.parameter_count == U.parameter_layout()[2]
+    assert np.array_equal(s.parameters(), U.init_parameters(12345))
+
+
+@pytest.mark.parametrize("edge,batch", [(16, 2), (32, 1)])
+def test_network_gradients_match_oracle(edge, batch):
+    gpu_case(f"surrogate gradients edge {edge} batch {batch}")
+    from pantometry_amd.surrogate import Surrogate
+    p = _random_params(3)
+    s = Surrogate()
+    s.set_parameters(p)
+    t0, k, tk = _fields(edge, batch, 4)
+    s.reserve(batch, edge)
+    for i in range(batch):
+        s.set_sample(i, t0[i], k[i], tk[i], COUPLING * (1 + 0.1 * i))
+    loss, grad = s.gradients(list(range(batch)))
+
+    # the training set holds T_0 and k as fp32; the features are computed from those
+    t32 = t0.astype(np.float32).astype(np.float64)
+    k32 = k.astype(np.float32).astype(np.float64)
+    a = np.array([COUPLING * (1 + 0.1 * i) for i in range(batch)])
+    x = U.pack_input(t32, k32, a, bf16=True)
+    target = ((tk - t0) / U.OUT_SCALE).astype(np.float32).astype(np.float64)
+    y, cache = U.forward(p.astype(np.float64), x, bf16=True)
+    ref = U.backward(p.astype(np.float64), cache, y, target, bf16=True)
+    assert abs(loss - U.loss(y, target)) <= 1e-3 * U.loss(y, target)
+    lay, head, total = U.parameter_layout()
+    spans = [(wo, wo + cout * 27 * cin) for (cin, cout, _), (wo, _) in zip(U.LAYERS, lay)]
+    spans += [(bo, bo + cout) for (cin, cout, _), (_, bo) in zip(U.LAYERS, lay)]
+    spans += [(head, total)]
+    for lo, hi in spans:
+        r = ref[lo:hi]
+        assert np.linalg.norm(grad[lo:hi] - r) <= 1e-2 * np.linalg.norm(r) + 1e-12, (lo, hi)
+
+    # and against the exact (fp64, unrounded) network: the direction agrees
+    x64 = U.pack_input(t32, k32, a)
+    y64, c64 = U.forward(p.astype(np.float64), x64)
+    g64 = U.backward(p.astype(np.float64), c64, y64, target)
+    cos = float(np.dot(grad, g64) / (np.linalg.norm(grad) * np.linalg.norm(g64)))
+    assert cos > 0.99
+
+
+def test_forward_matches_oracle():
+    gpu_case("surrogate forward edge 32")
+    from pantometry_amd.surrogate import Surrogate
+    p = _random_params(5)
+    s = Surrogate()
+    s.set_parameters(p)
+    t0, k, _ = _fields(32, 1, 6)
+    y = s.forward(t0[0], k[0], COUPLING)
+    ref, _ = U.forward(p.astype(np.float64), U.pack_input(t0, k, COUPLING, bf16=True), bf16=True)
+    assert np.linalg.norm(y - ref[0]) <= 1e-2 * np.linalg.norm(ref[0])
+
+
+def test_surrogate_step_is_conservative_and_applies_the_prediction():
+    gpu_case("surrogate step edge 32")
+    from pantometry_amd.surrogate import Surrogate, coupling, heterogeneous_pair_block
+    p = _random_params(7)
+    p[U.parameter_layout()[1]:] *= 0.05   # a small head: changes of a few kelvin
+    s = Surrogate()
+    s.set_parameters(p)
+    block, k, dt = heterogeneous_pair_block(32, 900)
+    t0 = block.cells().copy()
+    y = s.forward(t0, k, coupling(dt))
+    drift, ms = s.step(block, dt, 300.0)
+    t1 = block.cells()
+    assert ms > 0.0
+    assert drift <= 1e-12
+    assert abs(math.fsum((t1 - 300.0).ravel()) - math.fsum((t0 - 300.0).ravel())) <= \
+        1e-12 * math.fsum(np.abs(t0 - 300.0).ravel())
+    shift = t1 - (t0 + 10.0 * y.astype(np.float64))
+    assert np.ptp(shift) <= 1e-9 * np.abs(t0).max()
+
+
+def test_rollout_veto_falls_back_to_the_exact_stencil():
+    gpu_case("rollout veto edge 32")
+    from pantometry_amd.surrogate import Surrogate, heterogeneous_pair_block, rollout
+    s = Surrogate()
+    block, k, dt = heterogeneous_pair_block(32, 901)
+    t0 = block.cells()
+    rep = rollout(s, k, t0, dt, steps=64, tolerance=0.0)
+    assert rep.chunks == 4 and rep.checked == 4 and rep.fallbacks == 4
+    assert rep.final_error == 0.0        # every chunk replaced: bit-exact to the plain stencil
+    rep = rollout(s, k, t0, dt, steps=64, tolerance=math.inf, check_every=2)
+    assert rep.fallbacks == 0 and rep.checked == 2
+    assert rep.max_drift <= 1e-12
+    # the untrained network predicts no change: its error is the persistence error
+    assert rep.final_error > 0.0
+
+
+def test_training_on_stencil_data_beats_persistence():
+    """What the trained network gains over the method without it: after a short training run on
+    GPU-stencil pairs, its 16-step prediction on held-out trajectories is far closer to the
+    stencil than persistence (no change, which is what the untrained network predicts)."""
+    gpu_case("surrogate training edge 32")
+    from pantometry_amd.surrogate import (Surrogate, build_training_set, coupling,
+                                          heterogeneous_pair_block, train)
+    s = Surrogate()
+    n = 96
+    build_training_set(s, n, edge=32, chunks=16)
+    losses = train(s, n, steps=800, batch=8, lr=1e-3)
+    assert np.mean(losses[-50:]) < 0.1 * np.mean(losses[:5])
+    gains = []
+    for i in range(4):
+        block, k, dt = heterogeneous_pair_block(32, 5000 + i)
+        block.advance(dt, 16 * (2 + 3 * i), tolerance=1e-9, want_totals=False)
+        t0 = block.cells().copy()
+        block.advance(dt, 16, tolerance=1e-9, want_totals=False)
+        tk = block.cells()
+        pred = t0 + 10.0 * s.forward(t0, k, coupling(dt)).astype(np.float64)
+        gains.append(np.linalg.norm(t0 - tk) / np.linalg.norm(pred - tk))
+    print(f"[surrogate] persistence/network error ratios {gains}", flush=True)
+    assert min(gains) > 3.0

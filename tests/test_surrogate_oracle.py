@@ -1,0 +1,171 @@
+"""The surrogate's CPU oracle (oracle/unet.py) pinned by mathematics: library correlation, finite
+differences, adjoint identities and hand-computed cases.  No reference counterpart exists for
+the learned part, so these checks are what makes the oracle trustworthy."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+from scipy import ndimage
+
+from oracle import unet as U
+
+
+def _rng(seed=0):
+    return np.random.default_rng(seed)
+
+
+def test_round_bf16_nearest_even():
+    x = np.array([1.0, 1.0 + 2.0 ** -8, 1.0 + 3 * 2.0 ** -8, -2.5, 1.0 + 2.0 ** -7, 3.0e-3])
+    r = U.round_bf16(x)
+    assert r[0] == 1.0
+    assert r[1] == 1.0                      # tie, to even
+    assert r[2] == 1.0 + 2.0 ** -6          # tie, to even (upward)
+    assert r[3] == -2.5
+    assert r[4] == 1.0 + 2.0 ** -7          # representable
+    assert abs(r[5] - 3.0e-3) <= 3.0e-3 * 2.0 ** -8
+
+
+def test_conv3d_matches_scipy_correlate():
+    g = _rng(1)
+    x = g.standard_normal((2, 5, 6, 7, 3))
+    w = g.standard_normal((4, 27, 3))
+    b = g.standard_normal(4)
+    y = U.conv3d(x, w, b)
+    for bb in range(2):
+        for co in range(4):
+            ref = np.full((5, 6, 7), b[co])
+            for ci in range(3):
+                ref += ndimage.correlate(x[bb, ..., ci], w[co, :, ci].reshape(3, 3, 3),
+                                         mode="constant", cval=0.0)
+            np.testing.assert_allclose(y[bb, ..., co], ref, rtol=1e-12, atol=1e-12)
+
+
+def test_conv3d_tap_order():
+    # A single tap t = 9*dz + 3*dy + dx reads x at offset (dz-1, dy-1, dx-1).
+    x = np.zeros((1, 4, 4, 4, 1))
+    x[0, 2, 1, 3, 0] = 1.0
+    w = np.zeros((1, 27, 1))
+    w[0, 9 * 2 + 3 * 0 + 1, 0] = 1.0   # offset (+1, -1, 0)
+    y = U.conv3d(x, w)
+    assert y[0, 1, 2, 3, 0] == 1.0 and np.count_nonzero(y) == 1
+
+
+def test_conv3d_backward_adjoint_and_fd():
+    g = _rng(2)
+    x = g.standard_normal((1, 4, 4, 4, 3))
+    w = g.standard_normal((2, 27, 3))
+    dy = g.standard_normal((1, 4, 4, 4, 2))
+    dx, dw, db = U.conv3d_backward(x, w, dy)
+    # <conv(x), dy> = <x, conv^T(dy)>
+    assert np.isclose(np.sum(U.conv3d(x, w) * dy), np.sum(x * dx), rtol=1e-12)
+    # d/dw and d/db by central differences (the loss is linear in w, so they are exact)
+    for idx in [(0, 0, 0), (1, 13, 2), (0, 26, 1), (1, 4, 0)]:
+        e = np.zeros_like(w)
+        e[idx] = 1e-3
+        fd = (np.sum(U.conv3d(x, w + e) * dy) - np.sum(U.conv3d(x, w - e) * dy)) / 2e-3
+        assert np.isclose(dw[idx], fd, rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(db, dy.reshape(-1, 2).sum(0), rtol=1e-12)
+
+
+def test_pool_and_upsample():
+    x = np.arange(8.0).reshape(1, 2, 2, 2, 1)
+    assert U.pool(x)[0, 0, 0, 0, 0] == 3.5
+    y = U.upsample(np.array([5.0]).reshape(1, 1, 1, 1, 1))
+    assert y.shape == (1, 2, 2, 2, 1) and np.all(y == 5.0)
+    g = _rng(3)
+    a = g.standard_normal((2, 4, 4, 4, 3))
+    c = g.standard_normal((2, 2, 2, 2, 3))
+    assert np.isclose(np.sum(U.pool(a) * c), np.sum(a * U.pool_backward(c)), rtol=1e-12)
+    assert np.isclose(np.sum(U.upsample(c) * a), np.sum(c * U.upsample_backward(a)), rtol=1e-12)
+
+
+def test_gelu_grad_matches_fd():
+    z = np.linspace(-6, 6, 101)
+    h = 1e-5
+    fd = (U.gelu(z + h) - U.gelu(z - h)) / (2 * h)
+    np.testing.assert_allclose(U.gelu_grad(z), fd, rtol=1e-7, atol=1e-9)
+    # the tanh form's known values
+    assert U.gelu(0.0) == 0.0
+    assert abs(U.gelu(1.0) - 0.8411919906082768) < 1e-12
+
+
+def test_parameter_layout():
+    lay, head, total = U.parameter_layout()
+    assert total == 1466065
+    assert head == total - 17
+    assert lay[0] == (0, 16 * 27 * 16)
+
+
+def _random_params(seed):
+    g = _rng(seed)
+    lay, head, total = U.parameter_layout()
+    p = np.zeros(total)
+    for (cin, cout, _), (wo, bo) in zip(U.LAYERS, lay):
+        p[wo:wo + cout * 27 * cin] = g.uniform(-1, 1, cout * 27 * cin) * np.sqrt(6.0 / (27 * cin))
+        p[bo:bo + cout] = g.uniform(-0.1, 0.1, cout)
+    p[head:head + 17] = g.uniform(-0.5, 0.5, 17)
+    return p
+
+
+def test_network_gradient_matches_fd():
+    g = _rng(4)
+    p = _random_params(5)
+    t = 300.0 + 100.0 * g.random((1, 8, 8, 8))
+    k = np.exp(0.5 * g.standard_normal((1, 8, 8, 8)))
+    x = U.pack_input(t, k, 0.02)
+    target = 0.1 * g.standard_normal((1, 8, 8, 8))
+    y, cache = U.forward(p, x)
+    grad = U.backward(p, cache, y, target)
+    lay, head, total = U.parameter_layout()
+    picks = [head + 3, head + 16]
+    for i, ((cin, cout, _), (wo, bo)) in enumerate(zip(U.LAYERS, lay)):
+        co, t, ci = int(g.integers(cout)), int(g.integers(27)), int(g.integers(cin))
+        if i == 0:
+            ci = ci % 2  # the live input channels
+        picks.append(wo + (co * 27 + t) * cin + ci)
+        picks.append(bo + int(g.integers(cout)))
+    for j in picks:
+        h = 1e-6 * max(1.0, abs(p[j]))
+        pp, pm = p.copy(), p.copy()
+        pp[j] += h
+        pm[j] -= h
+        fd = (U.loss(U.forward(pp, x)[0], target) - U.loss(U.forward(pm, x)[0], target)) / (2 * h)
+        assert np.isclose(grad[j], fd, rtol=2e-5, atol=1e-10), (j, grad[j], fd)
+
+
+def test_bf16_emulation_stays_close_to_exact():
+    g = _rng(6)
+    p = _random_params(7)
+    t = 300.0 + 100.0 * g.random((1, 8, 8, 8))
+    k = np.exp(0.5 * g.standard_normal((1, 8, 8, 8)))
+    target = 0.1 * g.standard_normal((1, 8, 8, 8))
+    y64, c64 = U.forward(p, U.pack_input(t, k, 0.02))
+    y16, c16 = U.forward(p, U.pack_input(t, k, 0.02, bf16=True), bf16=True)
+    assert np.linalg.norm(y16 - y64) / np.linalg.norm(y64) < 3e-2
+    g64 = U.backward(p, c64, y64, target)
+    g16 = U.backward(p, c16, y16, target, bf16=True)
+    assert np.dot(g64, g16) / (np.linalg.norm(g64) * np.linalg.norm(g16)) > 0.99
+
+
+def test_input_features_are_the_explicit_stencil_step(oracle):
+    """Channel 14 (the summed face fluxes) is 16/10 K times the first stencil step's change, as the
+    reference-pinned stencil oracle computes it; channels 8..13 split it by face."""
+    from pantometry_amd import workloads as w
+    n = 8
+    k, t = oracle.heterogeneous_fields((n, n, n), spots=3)
+    k, t = k.reshape(n, n, n), t.reshape(n, n, n)
+    b = oracle.OracleSolid3D("b", w.RHO, 1.0, w.CP, (n, n, n), w.DX, 300.0)
+    b.with_conductivity(k)
+    b.set_cells(t)
+    dt = 0.9 * b.max_stable_dt()
+    b.step(dt)
+    a = dt * w.DX / ((w.RHO * w.DX ** 3) * w.CP)
+    x = U.pack_input(t[None], k[None], a)
+    change = (b.cells.reshape(n, n, n) - t) * (U.K_STEPS / U.OUT_SCALE)
+    scale = np.abs(change).max()
+    np.testing.assert_allclose(x[0, ..., 14], change, rtol=0, atol=2e-6 * scale)
+    np.testing.assert_allclose(x[0, ..., 8:14].sum(-1), x[0, ..., 14], rtol=0, atol=1e-6 * scale)
+    # surface faces carry nothing: the x- face of the first column, the z+ face of the top plane
+    assert np.all(x[0, :, :, 0, 2] == 0.0) and np.all(x[0, -1, :, :, 7] == 0.0)
+    assert np.all(x[0, :, :, 1:, 2] > 0.0)
+    assert np.all(x[0, ..., 15] == 1.0)

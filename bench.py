@@ -124,6 +124,48 @@ def bench_ensemble(args) -> dict:
     }
 
 
+def bench_surrogate(args) -> dict:
+    """The surrogate-assisted rollout: a heterogeneous 128^3 block advanced 1024 stencil steps by
+    64 surrogate steps of 16, with the FP64 stencil run beside it on every fourth chunk as the
+    judge (a checked chunk over 1e-3 relative L2 falls back to the stencil's result), against
+    the plain GPU stencil over the same 1024 steps."""
+    import numpy as np
+    from pantometry_amd import surrogate as S
+    s = S.Surrogate()
+    s.load(S.WEIGHTS)
+    block, k, dt = S.heterogeneous_pair_block(128, 424242)
+    t0 = block.cells().copy()
+    del block
+    runs = {}
+    for name, tol, every in (("checked_every_4th", 1e-3, 4), ("checked_every_chunk", 1e-3, 1),
+                             ("unchecked", float("inf"), 0)):
+        rep = S.rollout(s, k, t0, dt, steps=1024, tolerance=tol, check_every=every)
+        runs[name] = {"speedup": round(rep.speedup, 4), "rollout_s": round(rep.rollout_s, 4),
+                      "stencil_s": round(rep.stencil_s, 4), "checked": rep.checked,
+                      "fallbacks": rep.fallbacks, "max_chunk_error": rep.max_error,
+                      "final_error": rep.final_error, "max_drift": rep.max_drift,
+                      "surrogate_step_ms": round(rep.surrogate_ms, 4)}
+    main_run = runs["checked_every_4th"]
+    flops = S.forward_flops(128)
+    tflops = flops / (runs["unchecked"]["surrogate_step_ms"] * 1e-3) / 1e12
+    return {
+        "metric": "surrogate-assisted rollout speedup vs the plain FP64 GPU stencil, heat 128^3, "
+                  "1024 steps, veto at 1e-3 relative L2",
+        "value": main_run["speedup"], "unit": "x", "n_gpus": 1, "steps": 1024, "warmup": 1,
+        "ms_per_step": round(main_run["rollout_s"] * 1e3 / 1024, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (network), f64 (stencil, state)",
+        "data": "synthetic: seeded heterogeneous block; network trained on GPU-stencil pairs",
+        "config": {"workload": "heat 128^3 heterogeneous, 64 surrogate steps of 16 stencil steps",
+                   "weights": S.WEIGHTS.name},
+        "roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": 2500.0,
+                     "unit": "TFLOP/s", "frac": round(tflops / 2500.0, 5), "traffic": None,
+                     "note": "one surrogate step (network forward, features and conservative "
+                             f"shift) over {flops / 1e9:.1f} GFLOP of convolutions"},
+        "runs": runs,
+        "cpu_baseline": None,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,8 +174,10 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--tile-y", type=int, default=0)
     ap.add_argument("--z-chunk", type=int, default=0)
-    ap.add_argument("--workload", choices=["heat", "hall", "ensemble", "copy"], default="heat",
-                    help="heat (the headline 512^3 block), or the hall / ensemble secondaries")
+    ap.add_argument("--workload", choices=["heat", "hall", "ensemble", "surrogate", "copy"],
+                    default="heat",
+                    help="heat (the headline 512^3 block), or the hall / ensemble / surrogate "
+                         "secondaries")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the heat block is not sharded across GPUs "
@@ -149,6 +193,9 @@ def main() -> int:
         return 0
     if args.workload == "ensemble":
         print(json.dumps(bench_ensemble(args)), flush=True)
+        return 0
+    if args.workload == "surrogate":
+        print(json.dumps(bench_surrogate(args)), flush=True)
         return 0
     if args.workload == "copy":
         import ctypes as C

@@ -167,8 +167,9 @@ int pm_solid3d_readings(pm_solid3d *solid, double out[4]);
 /* Solid3D::heat_capacity, solid.rs:1441-1443: the left-fold sum of the cell capacities. */
 double pm_solid3d_heat_capacity(pm_solid3d *solid);
 /* Shape of the sweep: rows per 64-wide tile (4, 8 or 16), z-planes per workgroup, and how many
- * planes ahead each thread prefetches (1-3); 0 keeps the current value.  No reference
- * counterpart. */
+ * planes ahead each thread prefetches (1-3); 0 keeps the current value.  Until set, the z-planes
+ * per workgroup follow the extents: at most 128, at least 4, and about 1024 workgroups per
+ * launch.  No reference counterpart. */
 int pm_solid3d_set_tile(pm_solid3d *solid, int tile_y, int z_chunk, int prefetch);
 /* When `on`, pm_solid3d_advance brackets every sweep launch and every audit launch with HIP
  * events on the handle's stream; pm_solid3d_kernel_times then gives the mean device duration of

@@ -411,7 +411,7 @@ struct pm_solid3d {
     bool saved_valid = false;
     DevBuf partials, audit, scalars, stored_dev;
     int64_t partial_count = 0;
-    int tile_y = 8, z_chunk = 128, prefetch = 2;
+    int tile_y = 8, z_chunk = 0, prefetch = 2;  // z_chunk 0: chosen from the extents
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // Per-launch events of a profiled advance: [3*s] before the sweep, [3*s+1] between sweep and
     // audit, [3*s+2] after the audit.
@@ -426,10 +426,23 @@ struct pm_solid3d {
 
 namespace pm {
 
+// The z-planes one workgroup marches through.  Unless fixed by set_tile, chunks are as long as
+// possible (at most 128 planes, so the two planes of halo a chunk re-reads stay a small share)
+// while the launch still has about 1024 workgroups, four per compute unit; short chunks (at least
+// 4 planes) are what fills the chip on small blocks.
+static int z_chunk_of(const pm_solid3d *s) {
+    if (s->z_chunk > 0) return s->z_chunk;
+    const int64_t xy = ((s->nx + 63) / 64) * ((s->ny + s->tile_y - 1) / s->tile_y);
+    const int64_t chunks = std::max<int64_t>(1, (1024 + xy - 1) / xy);
+    const int64_t zc = (s->nz + chunks - 1) / chunks;
+    return (int)std::min<int64_t>(128, std::max<int64_t>(4, zc));
+}
+
 static int heat_grid(const pm_solid3d *s, dim3 *grid, dim3 *block) {
+    const int zc = z_chunk_of(s);
     *block = dim3(64, (unsigned)s->tile_y, 1);
     *grid = dim3((unsigned)((s->nx + 63) / 64), (unsigned)((s->ny + s->tile_y - 1) / s->tile_y),
-                 (unsigned)((s->nz + s->z_chunk - 1) / s->z_chunk));
+                 (unsigned)((s->nz + zc - 1) / zc));
     return PM_OK;
 }
 
@@ -479,7 +492,7 @@ static int sweep_once(pm_solid3d *s, double dt, int64_t step, double *totals_dev
     a.dts = dt;
     a.cap0 = s->cap0;
     a.tref = s->reference;
-    a.nx = (int)s->nx, a.ny = (int)s->ny, a.nz = (int)s->nz, a.zc = s->z_chunk;
+    a.nx = (int)s->nx, a.ny = (int)s->ny, a.nz = (int)s->nz, a.zc = z_chunk_of(s);
     a.gx = (int)g.x, a.gy = (int)g.y, a.gz = (int)g.z;
     const bool prof = s->profiling && step >= 0 && 3 * (step + 1) <= (int64_t)s->prof.size();
     if (prof) PM_HIP(hipEventRecord(s->prof[3 * step], s->stream));

@@ -155,10 +155,12 @@ __global__ void flip_weights(const bf16 *__restrict__ w, bf16 *__restrict__ wf, 
 // Weight gradient for a 16 (co) x 16 (ci) block over all 27 taps: wave w owns taps w, w+4, ...
 // The workgroup walks the M-tiles tile = split, split + splits, ...; per tile it stages the input
 // neighbourhood (ci block) and dy (co block) in LDS, then each wave runs one k-step per 16
-// positions of the box per tap.  Partials go to scratch[split][co][t][ci].
+// positions of the box per tap.  Partials go to part[split][co][t][ci]; the workgroups of the
+// first ci block also sum dy over their tiles for the bias, into bpart[split][co].
 __global__ __launch_bounds__(256) void conv_wgrad_kernel(Conv g, const bf16 *__restrict__ x,
                                                          const bf16 *__restrict__ dy,
-                                                         float *__restrict__ part, int splits) {
+                                                         float *__restrict__ part,
+                                                         float *__restrict__ bpart, int splits) {
     extern __shared__ __align__(16) bf16 smem[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int ci0 = blockIdx.x * 16, co0 = blockIdx.y * 16, split = blockIdx.z;
@@ -172,6 +174,8 @@ __global__ __launch_bounds__(256) void conv_wgrad_kernel(Conv g, const bf16 *__r
     for (int i = 0; i < 7; i++) acc[i] = float4v{0.f, 0.f, 0.f, 0.f};
     const int kq = (lane >> 4) * 4;
     const int64_t ntiles = g.tiles();
+    const bool bias_lane = blockIdx.x == 0 && tid < 16;
+    float bsum = 0.0f;
     for (int64_t tile = split; tile < ntiles; tile += splits) {
         int b, d0, h0, w0;
         tile_origin(g, tile, b, d0, h0, w0);
@@ -189,6 +193,8 @@ __global__ __launch_bounds__(256) void conv_wgrad_kernel(Conv g, const bf16 *__r
             *(uint4 *)(sd + q * 16 + half * 8) = v;
         }
         __syncthreads();
+        if (bias_lane)
+            for (int q = 0; q < TP; q++) bsum += from_bf16(sd[q * 16 + tid]);
 #pragma unroll
         for (int i = 0; i < 7; i++) {
             const int t = wave + 4 * i;
@@ -221,6 +227,7 @@ __global__ __launch_bounds__(256) void conv_wgrad_kernel(Conv g, const bf16 *__r
             part[(((int64_t)split * g.Cout + co) * 27 + t) * g.Cin + ci] = acc[i][r];
         }
     }
+    if (bias_lane) bpart[(int64_t)split * g.Cout + co0 + tid] = bsum;
 }
 
 // dw[i] = sum over splits of part[s][i], in split order.
@@ -232,22 +239,6 @@ __global__ void reduce_splits(const float *__restrict__ part, float *__restrict_
         for (int k = 0; k < splits; k++) s += part[k * n + i];
         out[i] = s;
     }
-}
-
-// db[c] = sum over positions of dy[p][c]: one workgroup per channel, fixed order.
-__global__ __launch_bounds__(256) void bias_grad(const bf16 *__restrict__ dy, int64_t npos, int C,
-                                                 float *__restrict__ db) {
-    __shared__ float red[256];
-    const int c = blockIdx.x;
-    float s = 0.0f;
-    for (int64_t p = threadIdx.x; p < npos; p += 256) s += from_bf16(dy[p * C + c]);
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int k = 128; k > 0; k >>= 1) {
-        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) db[c] = red[0];
 }
 
 static unsigned blocks_for(int64_t n, int threads) {
@@ -295,18 +286,16 @@ int conv_backward_data(const Conv &g, const bf16 *dy, const bf16 *w, bf16 *wf, b
 
 int conv_backward_weight(const Conv &g, const bf16 *x, const bf16 *dy, float *dw, float *db,
                          float *scratch, int splits, hipStream_t stream) {
-    if (!conv_shape_ok(g) || splits < 1) return PM_ERR_ARGUMENT;
+    if (!conv_shape_ok(g) || splits < 1 || !db) return PM_ERR_ARGUMENT;
     const size_t lds = ((size_t)(g.TD + 2) * (g.TH + 2) * (g.TW + 2) * 16 + 64 * 16) * sizeof(bf16);
-    dim3 grid((unsigned)(g.Cin / 16), (unsigned)(g.Cout / 16), (unsigned)splits);
-    conv_wgrad_kernel<<<grid, 256, lds, stream>>>(g, x, dy, scratch, splits);
-    PM_HIP(hipGetLastError());
     const int64_t n = (int64_t)g.Cout * 27 * g.Cin;
-    reduce_splits<<<blocks_for(n, 256), 256, 0, stream>>>(scratch, dw, n, splits);
+    float *bpart = scratch + (int64_t)splits * n;
+    dim3 grid((unsigned)(g.Cin / 16), (unsigned)(g.Cout / 16), (unsigned)splits);
+    conv_wgrad_kernel<<<grid, 256, lds, stream>>>(g, x, dy, scratch, bpart, splits);
     PM_HIP(hipGetLastError());
-    if (db) {
-        bias_grad<<<g.Cout, 256, 0, stream>>>(dy, g.positions(), g.Cout, db);
-        PM_HIP(hipGetLastError());
-    }
+    reduce_splits<<<blocks_for(n, 256), 256, 0, stream>>>(scratch, dw, n, splits);
+    reduce_splits<<<blocks_for(g.Cout, 256), 256, 0, stream>>>(bpart, db, g.Cout, splits);
+    PM_HIP(hipGetLastError());
     return PM_OK;
 }
 

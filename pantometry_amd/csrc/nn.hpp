@@ -75,7 +75,7 @@ int conv_backward_data(const Conv &g, const bf16 *dy, const bf16 *w, bf16 *w_fli
                        bf16 *dx, hipStream_t stream);
 // dw[co][t][ci] = sum_p dy[p][co] * x[p + off(t)][ci] (fp32), db[co] = sum_p dy[p][co];
 // split over position tiles and reduced in a fixed order.  `scratch` holds
-// splits * Cout * 27 * Cin floats.
+// splits * Cout * (27 * Cin + 1) floats.
 int conv_backward_weight(const Conv &g, const bf16 *x, const bf16 *dy, float *dw, float *db,
                          float *scratch, int splits, hipStream_t stream);
 

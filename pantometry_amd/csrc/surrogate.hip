@@ -517,14 +517,16 @@ int size_buffers(pm_surrogate *s, int R, int B) {
     return PM_OK;
 }
 
-int run_forward(pm_surrogate *s) {
+// The forward pass over xin into y.  With `training` the pre-activations are kept for the
+// backward pass; inference writes only the activations.
+int run_forward(pm_surrogate *s, bool training) {
     hipStream_t st = s->stream;
     const float *P = s->params.as<float>();
     const bf16 *WB = s->wbf.as<bf16>();
     auto conv = [&](int li, const bf16 *in, Act &o) -> int {
         const nn::Conv g = conv_of(s, li);
-        return nn::conv_forward(g, in, WB + s->layers[li].wbf_off, P + s->layers[li].b_off, o.z,
-                                o.a, true, g.Cout, 0, st);
+        return nn::conv_forward(g, in, WB + s->layers[li].wbf_off, P + s->layers[li].b_off,
+                                training ? o.z : nullptr, o.a, true, g.Cout, 0, st);
     };
     auto R = [&](int l) { return s->R >> l; };
     const int B = s->batch;
@@ -570,7 +572,7 @@ int conv_back(pm_surrogate *s, int li, const bf16 *in, bf16 *din) {
     const int64_t tiles = g.tiles();
     const int64_t blocks_ct = (int64_t)(g.Cin / 16) * (g.Cout / 16);
     int splits = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, std::max<int64_t>(1, 2048 / blocks_ct)));
-    while ((int64_t)splits * g.Cout * 27 * g.Cin > s->wscratch_floats) splits /= 2;
+    while ((int64_t)splits * g.Cout * (27 * g.Cin + 1) > s->wscratch_floats) splits /= 2;
     splits = std::max(splits, 1);
     PM_CHECK(nn::conv_backward_weight(g, in, o.da, G + s->layers[li].w_off, G + s->layers[li].b_off,
                                       s->wscratch.as<float>(), splits, st));
@@ -823,7 +825,7 @@ int batch_forward_loss(pm_surrogate *s, const int *indices, int batch, unsigned 
                                                      s->set_edge, batch, s->xin.as<bf16>());
     nn::gather_target<<<grid_of(n * batch), 256, 0, st>>>(s->set_t.as<float>(), s->idx.as<int>(), n, batch, s->target.as<float>());
     PM_HIP(hipGetLastError());
-    PM_CHECK(run_forward(s));
+    PM_CHECK(run_forward(s, true));
     const int64_t n0 = npos(s, 0);
     const unsigned lb = std::min<unsigned>(grid_of(n0), 4096);
     nn::loss_fwd_bwd<<<lb, 256, 0, st>>>(s->y.as<float>(), s->target.as<float>(), n0, s->dy.as<float>(), s->parts.as<float>());
@@ -893,7 +895,7 @@ int pm_surrogate_forward(pm_surrogate *s, int64_t edge, const double *t, const d
     PM_HIP(hipMemcpyAsync(dt.d(), t, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, s->stream));
     PM_HIP(hipMemcpyAsync(dk.d(), k, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, s->stream));
     nn::pack_input<<<grid_of(n), 256, 0, s->stream>>>(dt.d(), dk.d(), (int)edge, (float)coupling, s->xin.as<bf16>());
-    PM_CHECK(run_forward(s));
+    PM_CHECK(run_forward(s, false));
     PM_HIP(hipMemcpyAsync(y, s->y.d(), sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, s->stream));
     PM_HIP(hipStreamSynchronize(s->stream));
     return PM_OK;
@@ -933,7 +935,7 @@ int pm_surrogate_step_solid(pm_surrogate *s, pm_solid3d *solid, double dt, doubl
     const unsigned rb = std::min<unsigned>(grid_of(n), MAXB);
     PM_HIP(hipEventRecord(s->ev0, st));
     nn::pack_input<<<grid_of(n), 256, 0, st>>>(cells, kc, (int)edge, (float)(dt * mob), s->xin.as<bf16>());
-    PM_CHECK(run_forward(s));
+    PM_CHECK(run_forward(s, false));
     nn::sum_f64<<<rb, 256, 0, st>>>(cells, n, part0);
     nn::apply_prediction_sum<<<rb, 256, 0, st>>>(cells, s->y.as<float>(), n, part1);
     nn::shift_finalize<<<1, 1024, 0, st>>>(part0, part1, rb, n, scal);
@@ -1024,7 +1026,7 @@ int pm_nn_conv3d_backward(int B, int D, int H, int W, int Cin, int Cout, const f
     PM_CHECK(gw.alloc(sizeof(float) * (size_t)nw));
     PM_CHECK(gb.alloc(sizeof(float) * (size_t)Cout));
     const int splits = (int)std::min<int64_t>(16, g.tiles());
-    PM_CHECK(scratch.alloc(sizeof(float) * (size_t)(splits * nw)));
+    PM_CHECK(scratch.alloc(sizeof(float) * (size_t)(splits * (nw + Cout))));
     PM_HIP(hipMemcpy(ddx.d(), hx.data(), sizeof(bf16) * hx.size(), hipMemcpyHostToDevice));
     PM_HIP(hipMemcpy(ddw.d(), hw.data(), sizeof(bf16) * hw.size(), hipMemcpyHostToDevice));
     PM_HIP(hipMemcpy(ddy.d(), hd.data(), sizeof(bf16) * hd.size(), hipMemcpyHostToDevice));

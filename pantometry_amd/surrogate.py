@@ -132,6 +132,20 @@ class Surrogate:
 
 K_STEPS = 16   # stencil steps one surrogate step stands in for
 
+# (cin, cout, level) of the fourteen 3x3x3 convolutions, encoder then decoder; level l runs at
+# edge / 2^l.  The 1x1 head (16 -> 1) follows the last.
+LAYERS = [(16, 16, 0), (16, 16, 0), (16, 32, 1), (32, 32, 1), (32, 64, 2), (64, 64, 2),
+          (64, 128, 3), (128, 128, 3), (192, 64, 2), (64, 64, 2), (96, 32, 1), (32, 32, 1),
+          (48, 16, 0), (16, 16, 0)]
+
+
+def forward_flops(edge: int) -> int:
+    """Multiply-add operations (x2) of one forward pass over an edge^3 block."""
+    total = 2 * 16 * edge ** 3
+    for cin, cout, level in LAYERS:
+        total += 2 * 27 * cin * cout * (edge >> level) ** 3
+    return total
+
 
 def coupling(dt: float, dx: float = None, density: float = None, specific_heat: float = None) -> float:
     """dt * dx / C for a cubic cell of side dx: the stencil's per-step coupling scale."""

@@ -224,3 +224,25 @@ def test_training_on_stencil_data_beats_persistence():
         gains.append(np.linalg.norm(t0 - tk) / np.linalg.norm(pred - tk))
     print(f"[surrogate] persistence/network error ratios {gains}", flush=True)
     assert min(gains) > 3.0
+
+
+def test_committed_weights_predict_held_out_chunks():
+    """The shipped network (trained by pantometry_amd.train_surrogate on 4096 GPU-stencil pairs)
+    on trajectories it never saw: one 16-step chunk, eight chunks in, within 3e-3 relative L2 of
+    the stencil (||T_s - T_h|| / ||T_h - 300 K||) and at least five times closer than
+    persistence."""
+    gpu_case("surrogate committed weights edge 64")
+    from pantometry_amd.surrogate import (WEIGHTS, Surrogate, coupling, heterogeneous_pair_block)
+    s = Surrogate()
+    s.load(WEIGHTS)
+    for i in range(3):
+        block, k, dt = heterogeneous_pair_block(64, 6000 + i)
+        block.advance(dt, 16 * 8, tolerance=1e-9, want_totals=False)
+        t0 = block.cells().copy()
+        block.advance(dt, 16, tolerance=1e-9, want_totals=False)
+        tk = block.cells()
+        pred = t0 + 10.0 * s.forward(t0, k, coupling(dt)).astype(np.float64)
+        err = np.linalg.norm(pred - tk) / np.linalg.norm(tk - 300.0)
+        gain = np.linalg.norm(t0 - tk) / np.linalg.norm(pred - tk)
+        print(f"[surrogate] held-out {i}: error {err:.2e}, persistence/network {gain:.1f}", flush=True)
+        assert err <= 3e-3 and gain >= 5.0

@@ -155,7 +155,9 @@ int pm_solid3d_ledger(pm_solid3d *solid, pm_books *books);
  * ledger total after every step is written to totals[0..steps] when totals is not NULL.  On a
  * breach the run stops with the state as the breaking step left it, and PM_VIOLATION names the
  * step.  `elapsed_ms` (optional) receives the device time of the loop, from HIP events on the
- * handle's stream. */
+ * handle's stream.  With per-cell conductivity and uniform capacity the loop runs two steps per
+ * pass over the grid (see pm_solid3d_set_steps_per_pass) with the same arithmetic, results and
+ * per-step audits. */
 int pm_solid3d_advance(pm_solid3d *solid, double dt, uint64_t steps, double tolerance,
                        double *totals, double *elapsed_ms, pm_violation *v);
 /* Domain::checkpoint / Domain::restore, solid.rs:1935-1950. */
@@ -171,11 +173,20 @@ double pm_solid3d_heat_capacity(pm_solid3d *solid);
  * per workgroup follow the extents: at most 128, at least 4, and about 1024 workgroups per
  * launch.  No reference counterpart. */
 int pm_solid3d_set_tile(pm_solid3d *solid, int tile_y, int z_chunk, int prefetch);
+/* How many steps pm_solid3d_advance takes per pass over the grid: 1, or 2 (the default), which
+ * applies to per-cell conductivity with uniform capacity and runs consecutive pairs of steps in
+ * one kernel (temporal blocking: each cell's T and k are read once per two steps; bit-identical
+ * fields and the same audit after every step).  pm_solid3d_steps_per_pass reports what advance
+ * will use for this block.  No reference counterpart. */
+int pm_solid3d_set_steps_per_pass(pm_solid3d *solid, int steps);
+int pm_solid3d_steps_per_pass(const pm_solid3d *solid);
 /* When `on`, pm_solid3d_advance brackets every sweep launch and every audit launch with HIP
  * events on the handle's stream; pm_solid3d_kernel_times then gives the mean device duration of
- * each over the last advance.  No reference counterpart (measurement only). */
+ * each per launch over the last advance, and the steps one such launch advanced (a trailing
+ * single step of a paired run is left out).  No reference counterpart (measurement only). */
 int pm_solid3d_set_profiling(pm_solid3d *solid, int on);
-int pm_solid3d_kernel_times(pm_solid3d *solid, double *sweep_ms, double *audit_ms);
+int pm_solid3d_kernel_times(pm_solid3d *solid, double *sweep_ms, double *audit_ms,
+                            int *steps_per_launch);
 
 /* ---------------------------------------------------------------------------------------- */
 /* Hall, crates/pantometry-acoustic/src/hall.rs                                             */

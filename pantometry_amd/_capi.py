@@ -77,7 +77,9 @@ SIGNATURES = {
     "pm_solid3d_heat_capacity": (_D, [_VP]),
     "pm_solid3d_set_tile": (_I, [_VP, _I, _I, _I]),
     "pm_solid3d_set_profiling": (_I, [_VP, _I]),
-    "pm_solid3d_kernel_times": (_I, [_VP, _DP, _DP]),
+    "pm_solid3d_kernel_times": (_I, [_VP, _DP, _DP, C.POINTER(_I)]),
+    "pm_solid3d_set_steps_per_pass": (_I, [_VP, _I]),
+    "pm_solid3d_steps_per_pass": (_I, [_VP]),
     "pm_hall_create": (_I, [C.c_char_p, _D, _D, _D, _S, _D, _D, C.POINTER(_VP)]),
     "pm_hall_destroy": (None, [_VP]),
     "pm_hall_nodes": (_I, [_VP, C.POINTER(_S)]),

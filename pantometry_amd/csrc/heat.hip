@@ -47,6 +47,7 @@ struct HeatArgs {
     double tref;                       // ledger reference temperature
     int nx, ny, nz, zc;
     int gx, gy, gz;                    // tiles along x, y and z
+    int force;                         // run even if an audit breach froze the run
 };
 
 enum { K_UNIFORM = 0, K_PER_CELL = 1 };
@@ -122,13 +123,9 @@ __global__ __launch_bounds__(64 * TY) void heat_sweep(HeatArgs a) {
     const bool inside = gi < nx && gj < ny;
     const int64_t col = std::min(gi, nx - 1) + nx * std::min(gj, ny - 1);
 
-    if (a.audit->halted) {
-        // A breach froze the run: carry the state over unchanged so the ping-pong stays valid.
-        if (inside)
-            for (int64_t z = z0; z < z1; z++) a.tnew[col + plane * z] = a.told[col + plane * z];
-        if (tid == 0) a.partials[tile] = 0.0;
-        return;
-    }
+    // A breach froze the run: later launches leave both buffers alone, and the host takes the
+    // state from the launch that broke.
+    if (a.audit->halted && !a.force) return;
 
     // Halo role: one halo cell per thread for the first 2*TX + 2*TY threads (bottom row, top row,
     // left column, right column), read from the clamped index so every LDS value is finite.
@@ -275,6 +272,279 @@ __global__ __launch_bounds__(64 * TY) void heat_sweep(HeatArgs a) {
     if (tid == 0) a.partials[tile] = total;
 }
 
+// ---- Two steps per pass -------------------------------------------------------------------
+//
+// The per-cell-conductivity, uniform-capacity sweep advanced two steps in one pass over HBM
+// (temporal blocking).  Every cell-step is computed with exactly the arithmetic of heat_sweep,
+// so the fields stay bit-identical; what changes is that T is read and written once per two
+// steps (12 B per cell-step instead of 24), and that each face conductivity, built once per plane
+// in LDS, serves both steps.
+//
+// A workgroup owns a 64 x TY column tile and a z-chunk.  Per plane z it computes the first step,
+// T1(z), on the tile widened by one cell (E1, 66 x (TY+2)): the tile's own cells by their own
+// threads, the ring of 2*66 + 2*TY widened cells by the first threads of the workgroup (role
+// "ring 1", each keeping its own column's z-rings).  That needs T0 and k on the tile widened by
+// two (E2, 68 x (TY+4)); the outer ring of E2 is fetched by the next threads (role "ring 2").
+// With T1(z) in LDS and T1(z-2), T1(z-1) in registers, the second step T2(z-1) follows on the tile
+// in the same iteration.  The chunk's first step starts one plane early and ends one plane late
+// (planes z0-1 .. z1), which the neighbouring chunks also compute: identical arithmetic on
+// identical inputs, so the overlap agrees bit for bit.
+//
+// LDS per plane: T0(z) and k(z+1) on E2 (double-buffered), the x and y faces of a plane on E1
+// (three buffers: planes z-1, z and z+1 are live at once) and T1 on E1 (double-buffered); one
+// barrier per plane.  The audit of each step is summed as its cells are produced: step one over
+// the owned planes of T1, step two over the stored T2.
+template <int TY>
+__global__ __launch_bounds__(64 * TY) void heat_sweep2(HeatArgs a) {
+    constexpr int TX = 64, NT = TX * TY;
+    constexpr int E1X = TX + 2, E1Y = TY + 2, E2X = TX + 4, E2Y = TY + 4;
+    constexpr int R1 = 2 * E1X + 2 * TY;         // ring-1 cells: the first step only
+    constexpr int R2 = 2 * E2X + 2 * (E2Y - 2);  // ring-2 cells: loaded only
+    constexpr int NFX = E1Y * (E1X + 1), NFY = (E1Y + 1) * E1X;
+    static_assert(R1 + R2 <= NT, "ring roles exceed the workgroup");
+    __shared__ double sT[2][E2Y][E2X];
+    __shared__ double sK[2][E2Y][E2X];
+    __shared__ double fx[3][E1Y][E1X + 1];
+    __shared__ double fy[3][E1Y + 1][E1X];
+    __shared__ double s1[2][E1Y][E1X];
+    __shared__ double red[NT / 64];
+
+    const int64_t ntiles = (int64_t)a.gx * a.gy * a.gz;
+    const int64_t chunk = (ntiles + XCDS - 1) / XCDS;
+    const int64_t tile = (int64_t)(blockIdx.x % XCDS) * chunk + blockIdx.x / XCDS;
+    if (tile >= ntiles) return;
+    if (a.audit->halted && !a.force) return;  // a breach froze the run
+    const int64_t bx = tile % a.gx, by = (tile / a.gx) % a.gy, bz = tile / ((int64_t)a.gx * a.gy);
+
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TX * ty;
+    const int64_t nx = a.nx, ny = a.ny, nz = a.nz, plane = nx * ny;
+    const int64_t x0 = bx * TX, y0 = by * TY;
+    const int64_t z0 = bz * a.zc, z1 = std::min<int64_t>(z0 + a.zc, nz);
+    auto colidx = [&](int64_t x, int64_t y) -> uint32_t {
+        return (uint32_t)(std::min(std::max<int64_t>(x, 0), nx - 1) +
+                          nx * std::min(std::max<int64_t>(y, 0), ny - 1));
+    };
+
+    // Interior cell, E1 coordinates (ty+1, tx+1).
+    const int64_t gi = x0 + tx, gj = y0 + ty;
+    const bool inside = gi < nx && gj < ny;
+    const uint32_t ucol = colidx(gi, gj);
+    // Ring 1: the widened cells of E1 around the tile, E1 coordinates (r1r, r1c).
+    int r1r = -1, r1c = -1;
+    if (tid < E1X) r1r = 0, r1c = tid;
+    else if (tid < 2 * E1X) r1r = E1Y - 1, r1c = tid - E1X;
+    else if (tid < 2 * E1X + TY) r1r = 1 + (tid - 2 * E1X), r1c = 0;
+    else if (tid < R1) r1r = 1 + (tid - 2 * E1X - TY), r1c = E1X - 1;
+    const bool ring1 = r1r >= 0;
+    const uint32_t r1col = ring1 ? colidx(x0 - 1 + r1c, y0 - 1 + r1r) : 0u;
+    // Ring 2: the outer ring of E2, E2 coordinates (r2r, r2c).
+    int r2r = -1, r2c = -1;
+    {
+        const int e = tid - R1;
+        if (e >= 0 && e < E2X) r2r = 0, r2c = e;
+        else if (e >= E2X && e < 2 * E2X) r2r = E2Y - 1, r2c = e - E2X;
+        else if (e >= 2 * E2X && e < 2 * E2X + E2Y - 2) r2r = 1 + (e - 2 * E2X), r2c = 0;
+        else if (e >= 2 * E2X + E2Y - 2 && e < R2) r2r = 1 + (e - 2 * E2X - (E2Y - 2)), r2c = E2X - 1;
+    }
+    const bool ring2 = r2r >= 0;
+    const uint32_t r2col = ring2 ? colidx(x0 - 2 + r2c, y0 - 2 + r2r) : 0u;
+
+    const double *__restrict__ T = a.told;
+    const double *__restrict__ K = a.kcell;
+    // The chunk reads planes z0-2 .. z1+1; later planes are clamped to what it already reads.
+    const int64_t top = std::min(z1 + 1, nz - 1);
+    auto at = [&](const double *p, int64_t q) -> const double * {
+        return p + plane * std::min(std::max<int64_t>(q, 0), top);
+    };
+    // The z face below plane q (between q-1 and q) from the two conductivities.
+    auto zface = [&](int64_t q, double klo, double khi) -> double {
+        const double v = series_nz(klo, khi);
+        return (q > 0 && q < nz) ? v : 0.0;
+    };
+    // Faces of one plane on E1 from the conductivities staged on E2: x face (r, c) lies between
+    // the E1 cells (r, c-1) and (r, c), y face (r, c) between (r-1, c) and (r, c).
+    auto build = [&](const double (*k)[E2X], double (*gx)[E1X + 1], double (*gy)[E1X]) {
+        for (int e = tid; e < NFX + NFY; e += NT) {
+            if (e < NFX) {
+                const int r = e / (E1X + 1), c = e % (E1X + 1);
+                const int64_t x = x0 - 1 + c;
+                const double v = series_nz(k[r + 1][c], k[r + 1][c + 1]);
+                gx[r][c] = (x > 0 && x < nx) ? v : 0.0;
+            } else {
+                const int f = e - NFX;
+                const int r = f / E1X, c = f % E1X;
+                const int64_t y = y0 - 1 + r;
+                const double v = series_nz(k[r][c + 1], k[r + 1][c + 1]);
+                gy[r][c] = (y > 0 && y < ny) ? v : 0.0;
+            }
+        }
+    };
+    // One explicit update of E1 cell (r, c) from its four in-plane neighbours in `t`, indexed
+    // with offset (o, o) from E1 coordinates, and its own column (solid.rs:1781-1826 order).
+    auto update = [&](const double *t, int stride, int o, const double (*gx)[E1X + 1],
+                      const double (*gy)[E1X], int r, int c, double t_m, double t_c, double t_p,
+                      double kzl, double kzh) -> double {
+        const double tw = t[(r + o) * stride + c + o - 1], te = t[(r + o) * stride + c + o + 1];
+        const double ts = t[(r + o - 1) * stride + c + o], tn = t[(r + o + 1) * stride + c + o];
+        double flux = 0.0;
+        flux = flux + gx[r][c] * (tw - t_c);
+        flux = flux + gx[r][c + 1] * (te - t_c);
+        flux = flux + gy[r][c] * (ts - t_c);
+        flux = flux + gy[r + 1][c] * (tn - t_c);
+        flux = flux + kzl * (t_m - t_c);
+        flux = flux + kzh * (t_p - t_c);
+        return t_c + a.f * flux;
+    };
+
+    // Rings, indexed by (plane - zs) mod 4.
+    const int64_t zs = z0 - 1;
+    double RT[4], RK[4], KZ[4], U1[4];  // interior column: T0, k, z face below, T1
+    double QT[4], QK[4], QZ[4];         // ring-1 column
+    double HT[2], HK[2];                // ring-2 column: T0(z), k(z+1) for the next staging
+    RT[3] = at(T, zs - 1)[ucol];
+    RT[0] = at(T, zs)[ucol];
+    RT[1] = at(T, zs + 1)[ucol];
+    RT[2] = at(T, zs + 2)[ucol];
+    const double k_m = at(K, zs - 1)[ucol];
+    RK[0] = at(K, zs)[ucol];
+    RK[1] = at(K, zs + 1)[ucol];
+    RK[2] = at(K, zs + 2)[ucol];
+    RK[3] = at(K, zs + 3)[ucol];
+    KZ[0] = zface(zs, k_m, RK[0]);
+    KZ[1] = zface(zs + 1, RK[0], RK[1]);
+    KZ[2] = 0.0;
+    KZ[3] = 0.0;
+    U1[0] = U1[1] = U1[2] = U1[3] = 0.0;
+    QT[0] = QT[1] = QT[2] = QT[3] = 0.0;
+    QK[0] = QK[1] = QK[2] = QK[3] = 0.0;
+    QZ[0] = QZ[1] = QZ[2] = QZ[3] = 0.0;
+    HT[0] = HT[1] = HK[0] = HK[1] = 0.0;
+    double hk0 = 0.0;
+    if (ring1) {
+        QT[3] = at(T, zs - 1)[r1col];
+        QT[0] = at(T, zs)[r1col];
+        QT[1] = at(T, zs + 1)[r1col];
+        QT[2] = at(T, zs + 2)[r1col];
+        const double q_m = at(K, zs - 1)[r1col];
+        QK[0] = at(K, zs)[r1col];
+        QK[1] = at(K, zs + 1)[r1col];
+        QK[2] = at(K, zs + 2)[r1col];
+        QK[3] = at(K, zs + 3)[r1col];
+        QZ[0] = zface(zs, q_m, QK[0]);
+        QZ[1] = zface(zs + 1, QK[0], QK[1]);
+    }
+    if (ring2) {
+        HT[0] = at(T, zs)[r2col];
+        HT[1] = at(T, zs + 1)[r2col];
+        hk0 = at(K, zs)[r2col];
+        HK[0] = at(K, zs + 1)[r2col];
+        HK[1] = at(K, zs + 2)[r2col];
+    }
+    // The faces of plane zs, from k(zs) staged in the second buffer.
+    sK[1][ty + 2][tx + 2] = RK[0];
+    if (ring1) sK[1][r1r + 1][r1c + 1] = QK[0];
+    if (ring2) sK[1][r2r][r2c] = hk0;
+    lds_barrier();
+    build(sK[1], fx[0], fy[0]);
+
+    double *__restrict__ Tout = a.tnew;
+    double acc1 = 0.0, acc2 = 0.0;
+    int fb = 0;  // face buffer of plane z
+    for (int64_t zb = zs; zb <= z1; zb += 4) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int64_t z = zb + i;
+            const bool live = z <= z1;
+            const int b = i & 1;
+            const int fbn = fb == 2 ? 0 : fb + 1, fbp = fb == 0 ? 2 : fb - 1;
+
+            // Stage T0(z) and k(z+1) on E2.
+            sT[b][ty + 2][tx + 2] = RT[i];
+            sK[b][ty + 2][tx + 2] = RK[(i + 1) & 3];
+            if (ring1) {
+                sT[b][r1r + 1][r1c + 1] = QT[i];
+                sK[b][r1r + 1][r1c + 1] = QK[(i + 1) & 3];
+            }
+            if (ring2) {
+                sT[b][r2r][r2c] = HT[b];
+                sK[b][r2r][r2c] = HK[b];
+                HT[b] = at(T, z + 2)[r2col];
+                HK[b] = at(K, z + 3)[r2col];
+            }
+            lds_barrier();
+
+            // Faces of plane z+1, and the z face between planes z+1 and z+2.
+            build(sK[b], fx[fbn], fy[fbn]);
+            KZ[(i + 2) & 3] = zface(z + 2, RK[(i + 1) & 3], RK[(i + 2) & 3]);
+            RK[i] = at(K, z + 4)[ucol];  // k(z) is no longer needed
+            if (ring1) {
+                QZ[(i + 2) & 3] = zface(z + 2, QK[(i + 1) & 3], QK[(i + 2) & 3]);
+                QK[i] = at(K, z + 4)[r1col];
+            }
+
+            // First step: T1(z) on E1.
+            const double t1 = update(&sT[b][0][0], E2X, 1, fx[fb], fy[fb], ty + 1, tx + 1,
+                                     RT[(i + 3) & 3], RT[i], RT[(i + 1) & 3], KZ[i],
+                                     KZ[(i + 1) & 3]);
+            U1[i] = t1;
+            s1[b][ty + 1][tx + 1] = t1;
+            if (ring1)
+                s1[b][r1r][r1c] = update(&sT[b][0][0], E2X, 1, fx[fb], fy[fb], r1r, r1c,
+                                         QT[(i + 3) & 3], QT[i], QT[(i + 1) & 3], QZ[i],
+                                         QZ[(i + 1) & 3]);
+            RT[(i + 3) & 3] = at(T, z + 3)[ucol];  // T0(z-1) is no longer needed
+            if (ring1) QT[(i + 3) & 3] = at(T, z + 3)[r1col];
+            if (inside && z >= z0 && z < z1) acc1 += a.cap0 * (t1 - a.tref);
+
+            // Second step: T2(z-1) on the tile, from T1(z-1) staged in the previous iteration.
+            const double t2 = update(&s1[b ^ 1][0][0], E1X, 0, fx[fbp], fy[fbp], ty + 1, tx + 1,
+                                     U1[(i + 2) & 3], U1[(i + 3) & 3], U1[i], KZ[(i + 3) & 3],
+                                     KZ[i]);
+            if (inside && live && z - 1 >= z0) {
+                __builtin_nontemporal_store(t2, Tout + plane * (z - 1) + ucol);
+                acc2 += a.cap0 * (t2 - a.tref);
+            }
+            fb = fbn;
+        }
+    }
+
+    const double total1 = block_sum<NT>(acc1, red);
+    __syncthreads();
+    const double total2 = block_sum<NT>(acc2, red);
+    if (tid == 0) {
+        a.partials[tile] = total1;
+        a.partials[ntiles + tile] = total2;
+    }
+}
+
+// The ledger totals and audits of the two steps of a pass, in order: the first step's breach
+// freezes the run before the second is audited.
+__global__ __launch_bounds__(1024) void heat_finalize2(const double *partials, int64_t n,
+                                                       AuditState *st, double lost,
+                                                       double supplied, int64_t step,
+                                                       double *totals) {
+    __shared__ double scratch[16];
+    if (st->halted) return;
+    const double stored1 = fixed_order_sum_1024(partials, n, scratch);
+    __syncthreads();
+    const double stored2 = fixed_order_sum_1024(partials + n, n, scratch);
+    if (threadIdx.x == 0) {
+        double t1 = 0.0;
+        t1 += stored1;
+        t1 += lost;
+        t1 += -supplied;
+        const double s1 = rmax(rmax(rmax(0.0, std::fabs(stored1)), std::fabs(lost)), std::fabs(supplied));
+        audit_step(st, t1, s1, step, totals);
+        if (st->halted) return;
+        double t2 = 0.0;
+        t2 += stored2;
+        t2 += lost;
+        t2 += -supplied;
+        const double s2 = rmax(rmax(rmax(0.0, std::fabs(stored2)), std::fabs(lost)), std::fabs(supplied));
+        audit_step(st, t2, s2, step + 1, totals);
+    }
+}
+
 // Ledger total and audit of the step that just ran: stored = sum of partials; the ledger is
 // (0 + stored) + lost + (-supplied) with scale max(|stored|, |lost|, |supplied|).
 __global__ __launch_bounds__(1024) void heat_finalize(const double *partials, int64_t n,
@@ -412,12 +682,15 @@ struct pm_solid3d {
     DevBuf partials, audit, scalars, stored_dev;
     int64_t partial_count = 0;
     int tile_y = 8, z_chunk = 0, prefetch = 2;  // z_chunk 0: chosen from the extents
+    int steps_per_pass = 2;  // 2: pairs of steps in one pass where the form allows it
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // Per-launch events of a profiled advance: [3*s] before the sweep, [3*s+1] between sweep and
     // audit, [3*s+2] after the audit.
     bool profiling = false;
     std::vector<hipEvent_t> prof;
+    std::vector<int> prof_steps;  // steps each profiled launch advanced
     int64_t prof_used = 0;
+    int prof_launch_steps = 1;    // steps per launch of the launches sweep_ms averages
     double sweep_ms = 0.0, audit_ms = 0.0;
 
     int64_t n() const { return nx * ny * nz; }
@@ -430,26 +703,35 @@ namespace pm {
 // possible (at most 128 planes, so the two planes of halo a chunk re-reads stay a small share)
 // while the launch still has about 1024 workgroups, four per compute unit; short chunks (at least
 // 4 planes) are what fills the chip on small blocks.
-static int z_chunk_of(const pm_solid3d *s) {
+static int z_chunk_of(const pm_solid3d *s, int tile_y) {
     if (s->z_chunk > 0) return s->z_chunk;
-    const int64_t xy = ((s->nx + 63) / 64) * ((s->ny + s->tile_y - 1) / s->tile_y);
+    const int64_t xy = ((s->nx + 63) / 64) * ((s->ny + tile_y - 1) / tile_y);
     const int64_t chunks = std::max<int64_t>(1, (1024 + xy - 1) / xy);
     const int64_t zc = (s->nz + chunks - 1) / chunks;
     return (int)std::min<int64_t>(128, std::max<int64_t>(4, zc));
 }
 
-static int heat_grid(const pm_solid3d *s, dim3 *grid, dim3 *block) {
-    const int zc = z_chunk_of(s);
-    *block = dim3(64, (unsigned)s->tile_y, 1);
-    *grid = dim3((unsigned)((s->nx + 63) / 64), (unsigned)((s->ny + s->tile_y - 1) / s->tile_y),
+static int heat_grid(const pm_solid3d *s, dim3 *grid, dim3 *block, int tile_y = 0) {
+    const int ty = tile_y ? tile_y : s->tile_y;
+    const int zc = z_chunk_of(s, ty);
+    *block = dim3(64, (unsigned)ty, 1);
+    *grid = dim3((unsigned)((s->nx + 63) / 64), (unsigned)((s->ny + ty - 1) / ty),
                  (unsigned)((s->nz + zc - 1) / zc));
     return PM_OK;
 }
 
+// The two-steps-per-pass sweep exists for per-cell conductivity with uniform capacity, in
+// 64 x 8 tiles.
+constexpr int PAIR_TY = 8;
+static bool pairs_enabled(const pm_solid3d *s) {
+    return s->steps_per_pass == 2 && s->kmode == K_PER_CELL && s->cmode == C_UNIFORM;
+}
+
 static int ensure_partials(pm_solid3d *s) {
-    dim3 g, b;
+    dim3 g, b, g2;
     heat_grid(s, &g, &b);
-    int64_t need = (int64_t)g.x * g.y * g.z;
+    heat_grid(s, &g2, &b, PAIR_TY);
+    int64_t need = std::max<int64_t>((int64_t)g.x * g.y * g.z, 2 * (int64_t)g2.x * g2.y * g2.z);
     if (s->partial_count < need || !s->partials) {
         PM_CHECK(s->partials.alloc(sizeof(double) * (size_t)need));
         s->partial_count = need;
@@ -474,12 +756,20 @@ static void launch_sweep_t(const pm_solid3d *s, const HeatArgs &a, dim3 g) {
     }
 }
 
-// One sweep from t[cur] into t[cur^1] plus the finalize/audit; then swaps.
-static int sweep_once(pm_solid3d *s, double dt, int64_t step, double *totals_dev) {
+// Events around a launch of a profiled advance: [3l] before the sweep, [3l+1] between sweep and
+// finalize, [3l+2] after the finalize, for launch l.
+static bool prof_slot(pm_solid3d *s, int64_t launch) {
+    return s->profiling && launch >= 0 && 3 * (launch + 1) <= (int64_t)s->prof.size();
+}
+
+// One sweep from t[cur] into t[cur^1] plus the finalize/audit; then swaps.  `launch` indexes the
+// profiling events (-1: none); `force` runs the sweep even after a breach, without the audit.
+static int sweep_once(pm_solid3d *s, double dt, int64_t step, double *totals_dev,
+                      int64_t launch = -1, bool force = false) {
     PM_CHECK(ensure_partials(s));
     dim3 g, b;
     heat_grid(s, &g, &b);
-    HeatArgs a;
+    HeatArgs a{};
     a.told = s->t[s->cur].d();
     a.tnew = s->t[s->cur ^ 1].d();
     a.kcell = s->kcell.d();
@@ -492,10 +782,11 @@ static int sweep_once(pm_solid3d *s, double dt, int64_t step, double *totals_dev
     a.dts = dt;
     a.cap0 = s->cap0;
     a.tref = s->reference;
-    a.nx = (int)s->nx, a.ny = (int)s->ny, a.nz = (int)s->nz, a.zc = z_chunk_of(s);
+    a.nx = (int)s->nx, a.ny = (int)s->ny, a.nz = (int)s->nz, a.zc = z_chunk_of(s, s->tile_y);
     a.gx = (int)g.x, a.gy = (int)g.y, a.gz = (int)g.z;
-    const bool prof = s->profiling && step >= 0 && 3 * (step + 1) <= (int64_t)s->prof.size();
-    if (prof) PM_HIP(hipEventRecord(s->prof[3 * step], s->stream));
+    a.force = force ? 1 : 0;
+    const bool prof = prof_slot(s, launch);
+    if (prof) PM_HIP(hipEventRecord(s->prof[3 * launch], s->stream));
     if (s->kmode == K_UNIFORM && s->cmode == C_UNIFORM)
         launch_sweep_t<K_UNIFORM, C_UNIFORM>(s, a, g);
     else if (s->cmode == C_UNIFORM)
@@ -503,30 +794,77 @@ static int sweep_once(pm_solid3d *s, double dt, int64_t step, double *totals_dev
     else
         launch_sweep_t<K_PER_CELL, C_PER_CELL>(s, a, g);
     PM_HIP(hipGetLastError());
-    if (prof) PM_HIP(hipEventRecord(s->prof[3 * step + 1], s->stream));
-    heat_finalize<<<1, 1024, 0, s->stream>>>(s->partials.d(), (int64_t)g.x * g.y * g.z,
-                                             s->audit.as<AuditState>(), s->lost, s->supplied,
-                                             step, totals_dev, s->stored_dev.d());
-    PM_HIP(hipGetLastError());
+    if (prof) PM_HIP(hipEventRecord(s->prof[3 * launch + 1], s->stream));
+    if (!force) {
+        heat_finalize<<<1, 1024, 0, s->stream>>>(s->partials.d(), (int64_t)g.x * g.y * g.z,
+                                                 s->audit.as<AuditState>(), s->lost, s->supplied,
+                                                 step, totals_dev, s->stored_dev.d());
+        PM_HIP(hipGetLastError());
+    }
     if (prof) {
-        PM_HIP(hipEventRecord(s->prof[3 * step + 2], s->stream));
-        s->prof_used = std::max<int64_t>(s->prof_used, step + 1);
+        PM_HIP(hipEventRecord(s->prof[3 * launch + 2], s->stream));
+        s->prof_steps[(size_t)launch] = 1;
+        s->prof_used = std::max<int64_t>(s->prof_used, launch + 1);
     }
     s->cur ^= 1;
     return PM_OK;
 }
 
-// Mean sweep and audit durations over the profiled steps of the last advance.
+// Steps `step` and `step + 1` in one pass from t[cur] into t[cur^1], then both audits; swaps.
+static int sweep_pair(pm_solid3d *s, double dt, int64_t step, double *totals_dev, int64_t launch) {
+    PM_CHECK(ensure_partials(s));
+    dim3 g, b;
+    heat_grid(s, &g, &b, PAIR_TY);
+    HeatArgs a{};
+    a.told = s->t[s->cur].d();
+    a.tnew = s->t[s->cur ^ 1].d();
+    a.kcell = s->kcell.d();
+    a.partials = s->partials.d();
+    a.audit = s->audit.as<AuditState>();
+    a.f = dt * s->mob0;
+    a.dts = dt;
+    a.cap0 = s->cap0;
+    a.tref = s->reference;
+    a.nx = (int)s->nx, a.ny = (int)s->ny, a.nz = (int)s->nz, a.zc = z_chunk_of(s, PAIR_TY);
+    a.gx = (int)g.x, a.gy = (int)g.y, a.gz = (int)g.z;
+    const int64_t ntiles = (int64_t)g.x * g.y * g.z;
+    const bool prof = prof_slot(s, launch);
+    if (prof) PM_HIP(hipEventRecord(s->prof[3 * launch], s->stream));
+    const unsigned blocks = (unsigned)(XCDS * ((ntiles + XCDS - 1) / XCDS));
+    heat_sweep2<PAIR_TY><<<blocks, dim3(64, PAIR_TY), 0, s->stream>>>(a);
+    PM_HIP(hipGetLastError());
+    if (prof) PM_HIP(hipEventRecord(s->prof[3 * launch + 1], s->stream));
+    heat_finalize2<<<1, 1024, 0, s->stream>>>(s->partials.d(), ntiles, s->audit.as<AuditState>(),
+                                              s->lost, s->supplied, step, totals_dev);
+    PM_HIP(hipGetLastError());
+    if (prof) {
+        PM_HIP(hipEventRecord(s->prof[3 * launch + 2], s->stream));
+        s->prof_steps[(size_t)launch] = 2;
+        s->prof_used = std::max<int64_t>(s->prof_used, launch + 1);
+    }
+    s->cur ^= 1;
+    return PM_OK;
+}
+
+// Mean sweep and finalize durations per launch over the profiled launches of the last advance
+// that advanced the most common number of steps (a trailing single step of a paired run is left
+// out of the mean).
 static int collect_profile(pm_solid3d *s) {
+    int most = 1;
+    for (int64_t n = 0; n < s->prof_used; n++) most = std::max(most, s->prof_steps[(size_t)n]);
     double sw = 0.0, au = 0.0;
+    int64_t used = 0;
     for (int64_t n = 0; n < s->prof_used; n++) {
+        if (s->prof_steps[(size_t)n] != most) continue;
         float a = 0.0f, b = 0.0f;
         PM_HIP(hipEventElapsedTime(&a, s->prof[3 * n], s->prof[3 * n + 1]));
         PM_HIP(hipEventElapsedTime(&b, s->prof[3 * n + 1], s->prof[3 * n + 2]));
         sw += a, au += b;
+        used++;
     }
-    s->sweep_ms = s->prof_used ? sw / (double)s->prof_used : 0.0;
-    s->audit_ms = s->prof_used ? au / (double)s->prof_used : 0.0;
+    s->sweep_ms = used ? sw / (double)used : 0.0;
+    s->audit_ms = used ? au / (double)used : 0.0;
+    s->prof_launch_steps = most;
     return PM_OK;
 }
 
@@ -924,10 +1262,30 @@ int pm_solid3d_advance(pm_solid3d *s, double dt, uint64_t steps, double toleranc
             PM_HIP(hipEventCreate(&ev));
             s->prof.push_back(ev);
         }
+        s->prof_steps.assign((size_t)steps, 0);
         s->prof_used = 0;
     }
+    // Launches in order: (first step, steps, buffer read).
+    struct Launch {
+        int64_t first;
+        int steps;
+        int cur;
+    };
+    std::vector<Launch> launches;
+    const bool pairs = pairs_enabled(s);
     PM_HIP(hipEventRecord(s->ev0, s->stream));
-    for (uint64_t n = 0; n < steps; n++) PM_CHECK(sweep_once(s, dt, (int64_t)n, totals ? dtot.d() : nullptr));
+    for (uint64_t n = 0; n < steps;) {
+        const int64_t l = (int64_t)launches.size();
+        if (pairs && n + 1 < steps) {
+            launches.push_back({(int64_t)n, 2, s->cur});
+            PM_CHECK(sweep_pair(s, dt, (int64_t)n, totals ? dtot.d() : nullptr, l));
+            n += 2;
+        } else {
+            launches.push_back({(int64_t)n, 1, s->cur});
+            PM_CHECK(sweep_once(s, dt, (int64_t)n, totals ? dtot.d() : nullptr, l));
+            n += 1;
+        }
+    }
     PM_HIP(hipEventRecord(s->ev1, s->stream));
     AuditState st;
     PM_HIP(hipMemcpyAsync(&st, s->audit.d(), sizeof st, hipMemcpyDeviceToHost, s->stream));
@@ -940,7 +1298,20 @@ int pm_solid3d_advance(pm_solid3d *s, double dt, uint64_t steps, double toleranc
     }
     if (s->profiling) PM_CHECK(collect_profile(s));
     if (st.halted) {
-        // Steps after the breach copied the state unchanged; the ping-pong parity is preserved.
+        // Launches after the breach did nothing: the state is what the breaking step made it,
+        // in the buffer its launch wrote.  A breach on the first step of a pair is recomputed
+        // alone from the pair's input, which nothing has overwritten.
+        for (const Launch &l : launches) {
+            if (st.breach_step < l.first || st.breach_step >= l.first + l.steps) continue;
+            s->cur = l.cur;
+            if (l.steps == 2 && st.breach_step == l.first) {
+                PM_CHECK(sweep_once(s, dt, -1, nullptr, -1, true));
+                PM_HIP(hipStreamSynchronize(s->stream));
+            } else {
+                s->cur ^= 1;
+            }
+            break;
+        }
         fill_violation(v, "energy", "simulation", st.before, st.after, st.scale, tolerance, st.breach_step);
         return PM_VIOLATION;
     }
@@ -1015,16 +1386,33 @@ int pm_solid3d_set_tile(pm_solid3d *s, int tile_y, int z_chunk, int prefetch) {
     return PM_OK;
 }
 
+int pm_solid3d_set_steps_per_pass(pm_solid3d *s, int steps) {
+    if (!s) return PM_ERR_ARGUMENT;
+    if (steps != 1 && steps != 2) {
+        set_error("steps per pass must be 1 or 2");
+        return PM_ERR_ARGUMENT;
+    }
+    s->steps_per_pass = steps;
+    return PM_OK;
+}
+
+int pm_solid3d_steps_per_pass(const pm_solid3d *s) {
+    if (!s) return -1;
+    return pairs_enabled(s) ? 2 : 1;
+}
+
 int pm_solid3d_set_profiling(pm_solid3d *s, int on) {
     if (!s) return PM_ERR_ARGUMENT;
     s->profiling = on != 0;
     return PM_OK;
 }
 
-int pm_solid3d_kernel_times(pm_solid3d *s, double *sweep_ms, double *audit_ms) {
+int pm_solid3d_kernel_times(pm_solid3d *s, double *sweep_ms, double *audit_ms,
+                            int *steps_per_launch) {
     if (!s) return PM_ERR_ARGUMENT;
     if (sweep_ms) *sweep_ms = s->sweep_ms;
     if (audit_ms) *audit_ms = s->audit_ms;
+    if (steps_per_launch) *steps_per_launch = s->prof_launch_steps;
     return PM_OK;
 }
 

@@ -321,9 +321,18 @@ class Solid3D:
     def set_profiling(self, on: bool) -> None:
         check(lib().pm_solid3d_set_profiling(self._h, 1 if on else 0), "pm_solid3d_set_profiling")
 
-    def kernel_times(self) -> Tuple[float, float]:
-        """Mean (sweep, audit) device milliseconds per step over the last profiled advance."""
-        sw, au = C.c_double(), C.c_double()
-        check(lib().pm_solid3d_kernel_times(self._h, C.byref(sw), C.byref(au)),
+    def kernel_times(self) -> Tuple[float, float, int]:
+        """Mean (sweep, audit) device milliseconds per launch over the last profiled advance,
+        and the steps one sweep launch advanced."""
+        sw, au, k = C.c_double(), C.c_double(), C.c_int()
+        check(lib().pm_solid3d_kernel_times(self._h, C.byref(sw), C.byref(au), C.byref(k)),
               "pm_solid3d_kernel_times")
-        return sw.value, au.value
+        return sw.value, au.value, k.value
+
+    def set_steps_per_pass(self, steps: int) -> None:
+        """1, or 2 (default): advance runs pairs of steps in one pass where the form allows."""
+        check(lib().pm_solid3d_set_steps_per_pass(self._h, int(steps)),
+              "pm_solid3d_set_steps_per_pass")
+
+    def steps_per_pass(self) -> int:
+        return int(lib().pm_solid3d_steps_per_pass(self._h))

@@ -26,7 +26,11 @@ def test_solid_fixture_on_the_gpu(name):
     assert dt == 0.9 * b.max_stable_dt(0.0)
     totals, _ = b.advance(dt, f["steps"], tolerance=1e-12)
     assert np.array_equal(bits(b.cells()), bits(arr(f["final"])))
-    assert abs(totals[-1] - num(f["stored_after"])) <= 1e-12 * abs(num(f["stored_after"]))
+    # the audit sums in another order than the oracle's left fold: rounding at the magnitude of
+    # C |T| per cell (a mode about the reference stores almost nothing, so not relative to it)
+    cap = (f["density"] * f["dx"] ** 3) * f["specific_heat"]
+    scale = cap * float(np.sum(np.abs(arr(f["final"]))))
+    assert abs(totals[-1] - num(f["stored_after"])) <= 1e-14 * scale
 
 
 def test_hall_fixture_on_the_gpu():

@@ -165,6 +165,70 @@ def test_a_breach_stops_the_run_where_it_broke(oracle):
     assert np.array_equal(bits(g.cells()), bits(before))
 
 
+@pytest.mark.parametrize("counts", SHAPES, ids=str)
+def test_paired_advance_is_bit_exact(counts, oracle):
+    """advance runs pairs of steps per pass for per-cell conductivity: an odd step count (pairs
+    plus a trailing single step) against the oracle, bits and per-step audit totals."""
+    gpu_case(f"paired advance {counts}")
+    k, t = oracle.heterogeneous_fields(counts, spots=3)
+    g, o = _pair(counts, k=k, t=t)
+    assert g.steps_per_pass() == 2
+    dt = 0.9 * o.max_stable_dt()
+    totals, _ = g.advance(dt, 41, tolerance=1e-12)
+    for n in range(41):
+        o.step(dt, threads_=16)
+        ref = o.stored_heat_compensated()
+        assert abs(totals[n] - ref) <= 1e-12 * max(abs(ref), 1e-300), n
+    assert np.array_equal(bits(g.cells()), bits(o.cells))
+
+
+@pytest.mark.parametrize("z_chunk", [1, 2, 3, 5, 8, 128])
+def test_paired_advance_every_z_chunk(z_chunk, oracle):
+    gpu_case(f"paired advance z_chunk {z_chunk}")
+    counts = (70, 37, 23)
+    k, t = oracle.heterogeneous_fields(counts, spots=3)
+    g, o = _pair(counts, k=k, t=t)
+    g.set_tile(0, z_chunk)
+    dt = 0.9 * o.max_stable_dt()
+    g.advance(dt, 14, tolerance=1e-12)
+    for _ in range(14):
+        o.step(dt, threads_=16)
+    assert np.array_equal(bits(g.cells()), bits(o.cells))
+    # one step per pass gives the same bits
+    g1, _ = _pair(counts, k=k, t=t)
+    g1.set_steps_per_pass(1)
+    assert g1.steps_per_pass() == 1
+    g1.advance(dt, 14, tolerance=1e-12)
+    assert np.array_equal(bits(g1.cells()), bits(o.cells))
+
+
+def test_a_breach_on_the_second_step_of_a_pair(oracle):
+    """The audit after each step of a pair: a tolerance that the first few steps meet and a
+    second step of a pair breaks stops the run there, with the state that step made."""
+    gpu_case("audit breach inside a pair")
+    import pantometry_amd as pm
+    counts = (33, 17, 9)
+    k, t = oracle.heterogeneous_fields(counts, spots=2)
+    g, o = _pair(counts, k=k, t=t)
+    dt = 0.9 * o.max_stable_dt()
+    start = g.books()[0]
+    totals, _ = g.advance(dt, 40, tolerance=1.0)
+    prev = np.concatenate([[start], totals[:-1]])
+    drift = np.abs(totals - prev) / np.maximum(np.abs(totals), np.abs(prev))
+    # the first step j (second of its pair) whose drift exceeds every earlier one
+    js = [j for j in range(1, 40, 2) if drift[j] > drift[:j].max()]
+    assert js, drift
+    j = js[0]
+    tol = 0.5 * (drift[j] + drift[:j].max())
+    g.set_cells(t)
+    with pytest.raises(pm.Violation) as err:
+        g.advance(dt, 40, tolerance=tol)
+    assert err.value.step == j
+    for _ in range(j + 1):
+        o.step(dt, threads_=16)
+    assert np.array_equal(bits(g.cells()), bits(o.cells))
+
+
 def test_checkpoint_and_restore(oracle):
     gpu_case("checkpoint/restore")
     import pantometry_amd as pm

@@ -281,31 +281,40 @@ __global__ __launch_bounds__(64 * TY) void heat_sweep(HeatArgs a) {
 // in LDS, serves both steps.
 //
 // A workgroup owns a 64 x TY column tile and a z-chunk.  Per plane z it computes the first step,
-// T1(z), on the tile widened by one cell (E1, 66 x (TY+2)): the tile's own cells by their own
-// threads, the ring of 2*66 + 2*TY widened cells by the first threads of the workgroup (role
-// "ring 1", each keeping its own column's z-rings).  That needs T0 and k on the tile widened by
-// two (E2, 68 x (TY+4)); the outer ring of E2 is fetched by the next threads (role "ring 2").
-// With T1(z) in LDS and T1(z-2), T1(z-1) in registers, the second step T2(z-1) follows on the tile
-// in the same iteration.  The chunk's first step starts one plane early and ends one plane late
-// (planes z0-1 .. z1), which the neighbouring chunks also compute: identical arithmetic on
-// identical inputs, so the overlap agrees bit for bit.
+// T1(z), on the tile widened by one cell (E1, 66 x (TY+2)) less its four corners, which no
+// second-step stencil reaches: the tile's cells by their own threads, the 144-cell ring around
+// them by the last threads of the workgroup (role "ring 1", each keeping its own column's
+// z-rings).  That needs T0 and k on the tile widened by two (E2, 68 x (TY+4)); the outer ring's
+// needed cells are fetched by the first threads (role "ring 2").  With T1(z) in LDS and T1(z-2),
+// T1(z-1) in registers, the second step T2(z-1) follows on the tile in the same iteration, with
+// the faces of plane z-1 still in LDS.  The chunk's first
+// step starts one plane early and ends one plane late (planes z0-1 .. z1), which the
+// neighbouring chunks also compute: identical arithmetic on identical inputs, so the overlap
+// agrees bit for bit.
 //
-// LDS per plane: T0(z) and k(z+1) on E2 (double-buffered), the x and y faces of a plane on E1
-// (three buffers: planes z-1, z and z+1 are live at once) and T1 on E1 (double-buffered); one
-// barrier per plane.  The audit of each step is summed as its cells are produced: step one over
-// the owned planes of T1, step two over the stored T2.
+// Per plane every wave performs four rounds of divisions: its own column's z face, and three
+// rounds shared out over the 1388 in-plane faces of E1 and the ring's z faces, so no wave
+// waits on another at the plane's one barrier.  LDS: T0(z) and k(z+1) on E2 and T1 on E1,
+// double-buffered, and the faces of planes z-1, z and z+1 on E1.  The audit of each step is summed as its cells are
+// produced: step one over the owned planes of T1, step two over the stored T2.
 template <int TY>
-__global__ __launch_bounds__(64 * TY) void heat_sweep2(HeatArgs a) {
+__global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
     constexpr int TX = 64, NT = TX * TY;
     constexpr int E1X = TX + 2, E1Y = TY + 2, E2X = TX + 4, E2Y = TY + 4;
-    constexpr int R1 = 2 * E1X + 2 * TY;         // ring-1 cells: the first step only
-    constexpr int R2 = 2 * E2X + 2 * (E2Y - 2);  // ring-2 cells: loaded only
-    constexpr int NFX = E1Y * (E1X + 1), NFY = (E1Y + 1) * E1X;
-    static_assert(R1 + R2 <= NT, "ring roles exceed the workgroup");
+    constexpr int R1 = 2 * (E1X - 2) + 2 * TY;         // ring-1 cells (E1 less the tile, corners)
+    // Ring-2 cells: those of E2's outer ring the ring-1 stencils reach, and E1's four corners,
+    // whose T0 and k the ring-1 stencils and faces next to them read.
+    constexpr int R2 = 2 * (E2X - 4) + 2 * (E2Y - 4) + 4;
+    constexpr int FXS = E1Y * (E1X + 1), FYS = (E1Y + 1) * E1X;  // face layout: x rows, y rows
+    constexpr int NFX = FXS - 4, NFY = FYS - 4;        // faces needed: no corner cell's
+    constexpr int NF = NFX + NFY;
+    constexpr int F2 = NF - 2 * NT;                    // faces of the third round
+    constexpr int RING1_FIRST = NT - R1;
+    static_assert(R1 <= NT && R2 <= RING1_FIRST && F2 > 0 && F2 <= RING1_FIRST,
+                  "roles do not fit");
     __shared__ double sT[2][E2Y][E2X];
     __shared__ double sK[2][E2Y][E2X];
-    __shared__ double fx[3][E1Y][E1X + 1];
-    __shared__ double fy[3][E1Y + 1][E1X];
+    __shared__ double sF[3][FXS + FYS];
     __shared__ double s1[2][E1Y][E1X];
     __shared__ double red[NT / 64];
 
@@ -325,29 +334,73 @@ __global__ __launch_bounds__(64 * TY) void heat_sweep2(HeatArgs a) {
                           nx * std::min(std::max<int64_t>(y, 0), ny - 1));
     };
 
-    // Interior cell, E1 coordinates (ty+1, tx+1).
+    // Tile cell, E1 coordinates (ty+1, tx+1).
     const int64_t gi = x0 + tx, gj = y0 + ty;
     const bool inside = gi < nx && gj < ny;
     const uint32_t ucol = colidx(gi, gj);
-    // Ring 1: the widened cells of E1 around the tile, E1 coordinates (r1r, r1c).
+    // Ring 1 (the last R1 threads): E1 coordinates (r1r, r1c), corners excluded.
     int r1r = -1, r1c = -1;
-    if (tid < E1X) r1r = 0, r1c = tid;
-    else if (tid < 2 * E1X) r1r = E1Y - 1, r1c = tid - E1X;
-    else if (tid < 2 * E1X + TY) r1r = 1 + (tid - 2 * E1X), r1c = 0;
-    else if (tid < R1) r1r = 1 + (tid - 2 * E1X - TY), r1c = E1X - 1;
+    {
+        const int q = tid - RING1_FIRST;
+        if (q >= 0 && q < E1X - 2) r1r = 0, r1c = 1 + q;
+        else if (q >= E1X - 2 && q < 2 * (E1X - 2)) r1r = E1Y - 1, r1c = 1 + q - (E1X - 2);
+        else if (q >= 2 * (E1X - 2) && q < 2 * (E1X - 2) + TY) r1r = 1 + q - 2 * (E1X - 2), r1c = 0;
+        else if (q >= 2 * (E1X - 2) + TY && q < R1) r1r = 1 + q - 2 * (E1X - 2) - TY, r1c = E1X - 1;
+    }
     const bool ring1 = r1r >= 0;
     const uint32_t r1col = ring1 ? colidx(x0 - 1 + r1c, y0 - 1 + r1r) : 0u;
-    // Ring 2: the outer ring of E2, E2 coordinates (r2r, r2c).
+    // Ring 2 (the first R2 threads): E2 coordinates (r2r, r2c).
     int r2r = -1, r2c = -1;
     {
-        const int e = tid - R1;
-        if (e >= 0 && e < E2X) r2r = 0, r2c = e;
-        else if (e >= E2X && e < 2 * E2X) r2r = E2Y - 1, r2c = e - E2X;
-        else if (e >= 2 * E2X && e < 2 * E2X + E2Y - 2) r2r = 1 + (e - 2 * E2X), r2c = 0;
-        else if (e >= 2 * E2X + E2Y - 2 && e < R2) r2r = 1 + (e - 2 * E2X - (E2Y - 2)), r2c = E2X - 1;
+        constexpr int SIDES = 2 * (E2X - 4) + 2 * (E2Y - 4);
+        if (tid < E2X - 4) r2r = 0, r2c = 2 + tid;
+        else if (tid < 2 * (E2X - 4)) r2r = E2Y - 1, r2c = 2 + tid - (E2X - 4);
+        else if (tid < 2 * (E2X - 4) + (E2Y - 4)) r2r = 2 + tid - 2 * (E2X - 4), r2c = 0;
+        else if (tid < SIDES) r2r = 2 + tid - 2 * (E2X - 4) - (E2Y - 4), r2c = E2X - 1;
+        else if (tid < R2) {
+            const int q = tid - SIDES;  // E1's corners
+            r2r = (q & 2) ? E2Y - 2 : 1;
+            r2c = (q & 1) ? E2X - 2 : 1;
+        }
     }
     const bool ring2 = r2r >= 0;
     const uint32_t r2col = ring2 ? colidx(x0 - 2 + r2c, y0 - 2 + r2r) : 0u;
+
+    // This thread's in-plane faces, one per round (the third only for the first F2 threads),
+    // packed as: bits 0-11 the first conductivity's offset in a k plane on E2, bits 12-23 the
+    // face's offset in a face buffer, bit 24 set for a y face (second conductivity one row on,
+    // else one column on), bit 25 set when the face lies inside the block (faces on its surface
+    // are zero).
+    static_assert(E2X * E2Y < 4096 && FXS + FYS < 4096, "face descriptors need 12-bit offsets");
+    uint32_t fdesc[3];
+#pragma unroll
+    for (int rnd = 0; rnd < 3; rnd++) {
+        int j = rnd * NT + tid;
+        if (j >= NF) j = NF - 1;  // idle lane of the third round: a valid face, never stored
+        if (j < NFX) {
+            // x faces in row-major order over E1Y x (E1X+1), skipping the corner cells' faces
+            int ex = j + 1;
+            if (ex >= E1X) ex++;
+            if (ex >= FXS - (E1X + 1)) ex++;
+            const int r = ex / (E1X + 1), c = ex % (E1X + 1);
+            const int64_t x = x0 - 1 + c;
+            fdesc[rnd] = (uint32_t)((r + 1) * E2X + c) | ((uint32_t)ex << 12) |
+                         ((x > 0 && x < nx) ? (1u << 25) : 0u);
+        } else {
+            int ey = j - NFX + 1;
+            if (ey >= E1X - 1) ey++;
+            if (ey >= FYS - E1X) ey++;
+            const int r = ey / E1X, c = ey % E1X;
+            const int64_t y = y0 - 1 + r;
+            fdesc[rnd] = (uint32_t)(r * E2X + c + 1) | ((uint32_t)(FXS + ey) << 12) | (1u << 24) |
+                         ((y > 0 && y < ny) ? (1u << 25) : 0u);
+        }
+    }
+    const bool third = tid < F2;
+    auto fk = [](uint32_t d) -> int { return (int)(d & 0xFFFu); };
+    auto fk2 = [](uint32_t d) -> int { return (int)(d & 0xFFFu) + ((d >> 24) & 1u ? E2X : 1); };
+    auto fdst = [](uint32_t d) -> int { return (int)((d >> 12) & 0xFFFu); };
+    auto fon = [](uint32_t d) -> bool { return (d >> 25) & 1u; };
 
     const double *__restrict__ T = a.told;
     const double *__restrict__ K = a.kcell;
@@ -356,51 +409,33 @@ __global__ __launch_bounds__(64 * TY) void heat_sweep2(HeatArgs a) {
     auto at = [&](const double *p, int64_t q) -> const double * {
         return p + plane * std::min(std::max<int64_t>(q, 0), top);
     };
-    // The z face below plane q (between q-1 and q) from the two conductivities.
-    auto zface = [&](int64_t q, double klo, double khi) -> double {
-        const double v = series_nz(klo, khi);
-        return (q > 0 && q < nz) ? v : 0.0;
-    };
-    // Faces of one plane on E1 from the conductivities staged on E2: x face (r, c) lies between
-    // the E1 cells (r, c-1) and (r, c), y face (r, c) between (r-1, c) and (r, c).
-    auto build = [&](const double (*k)[E2X], double (*gx)[E1X + 1], double (*gy)[E1X]) {
-        for (int e = tid; e < NFX + NFY; e += NT) {
-            if (e < NFX) {
-                const int r = e / (E1X + 1), c = e % (E1X + 1);
-                const int64_t x = x0 - 1 + c;
-                const double v = series_nz(k[r + 1][c], k[r + 1][c + 1]);
-                gx[r][c] = (x > 0 && x < nx) ? v : 0.0;
-            } else {
-                const int f = e - NFX;
-                const int r = f / E1X, c = f % E1X;
-                const int64_t y = y0 - 1 + r;
-                const double v = series_nz(k[r][c + 1], k[r + 1][c + 1]);
-                gy[r][c] = (y > 0 && y < ny) ? v : 0.0;
-            }
-        }
-    };
-    // One explicit update of E1 cell (r, c) from its four in-plane neighbours in `t`, indexed
-    // with offset (o, o) from E1 coordinates, and its own column (solid.rs:1781-1826 order).
-    auto update = [&](const double *t, int stride, int o, const double (*gx)[E1X + 1],
-                      const double (*gy)[E1X], int r, int c, double t_m, double t_c, double t_p,
-                      double kzl, double kzh) -> double {
-        const double tw = t[(r + o) * stride + c + o - 1], te = t[(r + o) * stride + c + o + 1];
-        const double ts = t[(r + o - 1) * stride + c + o], tn = t[(r + o + 1) * stride + c + o];
+    auto zon = [&](int64_t q) -> bool { return q > 0 && q < nz; };  // face between q-1 and q
+    // One explicit update (solid.rs:1781-1826 order) from the in-plane neighbours in `t` around
+    // position p (row stride `stride`), the four in-plane faces and the own column.
+    auto update = [&](const double *t, int stride, int p, double kxl, double kxh, double kyl,
+                      double kyh, double t_m, double t_c, double t_p, double kzl,
+                      double kzh) -> double {
+        const double tw = t[p - 1], te = t[p + 1], ts = t[p - stride], tn = t[p + stride];
         double flux = 0.0;
-        flux = flux + gx[r][c] * (tw - t_c);
-        flux = flux + gx[r][c + 1] * (te - t_c);
-        flux = flux + gy[r][c] * (ts - t_c);
-        flux = flux + gy[r + 1][c] * (tn - t_c);
+        flux = flux + kxl * (tw - t_c);
+        flux = flux + kxh * (te - t_c);
+        flux = flux + kyl * (ts - t_c);
+        flux = flux + kyh * (tn - t_c);
         flux = flux + kzl * (t_m - t_c);
         flux = flux + kzh * (t_p - t_c);
         return t_c + a.f * flux;
     };
+    // The x faces of E1 cell (r, c) are at r*(E1X+1) + c and + 1, the y faces at
+    // FXS + r*E1X + c and FXS + (r+1)*E1X + c.
+    const int fxo = (ty + 1) * (E1X + 1) + tx + 1, fyo = FXS + (ty + 1) * E1X + tx + 1;
+    const int r1fx = ring1 ? r1r * (E1X + 1) + r1c : 0, r1fy = ring1 ? FXS + r1r * E1X + r1c : 0;
 
     // Rings, indexed by (plane - zs) mod 4.
     const int64_t zs = z0 - 1;
-    double RT[4], RK[4], KZ[4], U1[4];  // interior column: T0, k, z face below, T1
-    double QT[4], QK[4], QZ[4];         // ring-1 column
-    double HT[2], HK[2];                // ring-2 column: T0(z), k(z+1) for the next staging
+    double RT[4], RK[4], KZ[4], U1[4];  // tile column: T0, k, z face below, T1
+    // Ring-1 column: T0, k and z faces.  A ring-2 thread, never a ring-1 thread, keeps the T0(z)
+    // and k(z+1) of its next staging in QT[0..1] and QK[0..1] (indexed by plane parity).
+    double QT[4], QK[4], QZ[4];
     RT[3] = at(T, zs - 1)[ucol];
     RT[0] = at(T, zs)[ucol];
     RT[1] = at(T, zs + 1)[ucol];
@@ -410,15 +445,16 @@ __global__ __launch_bounds__(64 * TY) void heat_sweep2(HeatArgs a) {
     RK[1] = at(K, zs + 1)[ucol];
     RK[2] = at(K, zs + 2)[ucol];
     RK[3] = at(K, zs + 3)[ucol];
-    KZ[0] = zface(zs, k_m, RK[0]);
-    KZ[1] = zface(zs + 1, RK[0], RK[1]);
-    KZ[2] = 0.0;
-    KZ[3] = 0.0;
+    {
+        const double lo = series_nz(k_m, RK[0]), hi = series_nz(RK[0], RK[1]);
+        KZ[0] = zon(zs) ? lo : 0.0;
+        KZ[1] = zon(zs + 1) ? hi : 0.0;
+    }
+    KZ[2] = KZ[3] = 0.0;
     U1[0] = U1[1] = U1[2] = U1[3] = 0.0;
     QT[0] = QT[1] = QT[2] = QT[3] = 0.0;
     QK[0] = QK[1] = QK[2] = QK[3] = 0.0;
     QZ[0] = QZ[1] = QZ[2] = QZ[3] = 0.0;
-    HT[0] = HT[1] = HK[0] = HK[1] = 0.0;
     double hk0 = 0.0;
     if (ring1) {
         QT[3] = at(T, zs - 1)[r1col];
@@ -430,26 +466,36 @@ __global__ __launch_bounds__(64 * TY) void heat_sweep2(HeatArgs a) {
         QK[1] = at(K, zs + 1)[r1col];
         QK[2] = at(K, zs + 2)[r1col];
         QK[3] = at(K, zs + 3)[r1col];
-        QZ[0] = zface(zs, q_m, QK[0]);
-        QZ[1] = zface(zs + 1, QK[0], QK[1]);
+        const double lo = series_nz(q_m, QK[0]), hi = series_nz(QK[0], QK[1]);
+        QZ[0] = zon(zs) ? lo : 0.0;
+        QZ[1] = zon(zs + 1) ? hi : 0.0;
     }
     if (ring2) {
-        HT[0] = at(T, zs)[r2col];
-        HT[1] = at(T, zs + 1)[r2col];
+        QT[0] = at(T, zs)[r2col];
+        QT[1] = at(T, zs + 1)[r2col];
         hk0 = at(K, zs)[r2col];
-        HK[0] = at(K, zs + 1)[r2col];
-        HK[1] = at(K, zs + 2)[r2col];
+        QK[0] = at(K, zs + 1)[r2col];
+        QK[1] = at(K, zs + 2)[r2col];
     }
     // The faces of plane zs, from k(zs) staged in the second buffer.
     sK[1][ty + 2][tx + 2] = RK[0];
     if (ring1) sK[1][r1r + 1][r1c + 1] = QK[0];
     if (ring2) sK[1][r2r][r2c] = hk0;
     lds_barrier();
-    build(sK[1], fx[0], fy[0]);
+    {
+        const double *k = &sK[1][0][0];
+        double *f = sF[0];
+#pragma unroll
+        for (int rnd = 0; rnd < 3; rnd++) {
+            const uint32_t d = fdesc[rnd];
+            const double v = series_nz(k[fk(d)], k[fk2(d)]);
+            if (rnd < 2 || third) f[fdst(d)] = fon(d) ? v : 0.0;
+        }
+    }
 
     double *__restrict__ Tout = a.tnew;
     double acc1 = 0.0, acc2 = 0.0;
-    int fb = 0;  // face buffer of plane z
+    int fb = 0;  // face buffer of plane z; planes z-1, z and z+1 are live at once
     for (int64_t zb = zs; zb <= z1; zb += 4) {
 #pragma unroll
         for (int i = 0; i < 4; i++) {
@@ -466,40 +512,59 @@ __global__ __launch_bounds__(64 * TY) void heat_sweep2(HeatArgs a) {
                 sK[b][r1r + 1][r1c + 1] = QK[(i + 1) & 3];
             }
             if (ring2) {
-                sT[b][r2r][r2c] = HT[b];
-                sK[b][r2r][r2c] = HK[b];
-                HT[b] = at(T, z + 2)[r2col];
-                HK[b] = at(K, z + 3)[r2col];
+                sT[b][r2r][r2c] = QT[b];
+                sK[b][r2r][r2c] = QK[b];
+                QT[b] = at(T, z + 2)[r2col];
+                QK[b] = at(K, z + 3)[r2col];
             }
             lds_barrier();
 
-            // Faces of plane z+1, and the z face between planes z+1 and z+2.
-            build(sK[b], fx[fbn], fy[fbn]);
-            KZ[(i + 2) & 3] = zface(z + 2, RK[(i + 1) & 3], RK[(i + 2) & 3]);
-            RK[i] = at(K, z + 4)[ucol];  // k(z) is no longer needed
-            if (ring1) {
-                QZ[(i + 2) & 3] = zface(z + 2, QK[(i + 1) & 3], QK[(i + 2) & 3]);
-                QK[i] = at(K, z + 4)[r1col];
+            // Four rounds of divisions: the tile column's z face between planes z+1 and z+2, two
+            // rounds of the faces of plane z+1, and a third shared between the remaining faces
+            // and the ring-1 columns' z faces.
+            {
+                const double v = series_nz(RK[(i + 1) & 3], RK[(i + 2) & 3]);
+                KZ[(i + 2) & 3] = zon(z + 2) ? v : 0.0;
             }
+            {
+                const double *k = &sK[b][0][0];
+                double *f = sF[fbn];
+#pragma unroll
+                for (int rnd = 0; rnd < 2; rnd++) {
+                    const uint32_t d = fdesc[rnd];
+                    const double v = series_nz(k[fk(d)], k[fk2(d)]);
+                    f[fdst(d)] = fon(d) ? v : 0.0;
+                }
+                const uint32_t d = fdesc[2];
+                const double ka = third ? k[fk(d)] : QK[(i + 1) & 3];
+                const double kb = third ? k[fk2(d)] : QK[(i + 2) & 3];
+                const double v = series_nz(ka, kb);
+                if (third) f[fdst(d)] = fon(d) ? v : 0.0;
+                if (ring1) QZ[(i + 2) & 3] = zon(z + 2) ? v : 0.0;
+            }
+            RK[i] = at(K, z + 4)[ucol];  // k(z) is no longer needed
+            if (ring1) QK[i] = at(K, z + 4)[r1col];
 
-            // First step: T1(z) on E1.
-            const double t1 = update(&sT[b][0][0], E2X, 1, fx[fb], fy[fb], ty + 1, tx + 1,
-                                     RT[(i + 3) & 3], RT[i], RT[(i + 1) & 3], KZ[i],
-                                     KZ[(i + 1) & 3]);
+            // First step: T1(z) on E1 (its corners aside).
+            const double *f = sF[fb];
+            const double t1 = update(&sT[b][0][0], E2X, (ty + 2) * E2X + tx + 2, f[fxo],
+                                     f[fxo + 1], f[fyo], f[fyo + E1X], RT[(i + 3) & 3], RT[i],
+                                     RT[(i + 1) & 3], KZ[i], KZ[(i + 1) & 3]);
             U1[i] = t1;
             s1[b][ty + 1][tx + 1] = t1;
             if (ring1)
-                s1[b][r1r][r1c] = update(&sT[b][0][0], E2X, 1, fx[fb], fy[fb], r1r, r1c,
-                                         QT[(i + 3) & 3], QT[i], QT[(i + 1) & 3], QZ[i],
-                                         QZ[(i + 1) & 3]);
+                s1[b][r1r][r1c] = update(&sT[b][0][0], E2X, (r1r + 1) * E2X + r1c + 1, f[r1fx],
+                                         f[r1fx + 1], f[r1fy], f[r1fy + E1X], QT[(i + 3) & 3],
+                                         QT[i], QT[(i + 1) & 3], QZ[i], QZ[(i + 1) & 3]);
             RT[(i + 3) & 3] = at(T, z + 3)[ucol];  // T0(z-1) is no longer needed
             if (ring1) QT[(i + 3) & 3] = at(T, z + 3)[r1col];
             if (inside && z >= z0 && z < z1) acc1 += a.cap0 * (t1 - a.tref);
 
             // Second step: T2(z-1) on the tile, from T1(z-1) staged in the previous iteration.
-            const double t2 = update(&s1[b ^ 1][0][0], E1X, 0, fx[fbp], fy[fbp], ty + 1, tx + 1,
-                                     U1[(i + 2) & 3], U1[(i + 3) & 3], U1[i], KZ[(i + 3) & 3],
-                                     KZ[i]);
+            const double *g = sF[fbp];
+            const double t2 = update(&s1[b ^ 1][0][0], E1X, (ty + 1) * E1X + tx + 1, g[fxo],
+                                     g[fxo + 1], g[fyo], g[fyo + E1X], U1[(i + 2) & 3],
+                                     U1[(i + 3) & 3], U1[i], KZ[(i + 3) & 3], KZ[i]);
             if (inside && live && z - 1 >= z0) {
                 __builtin_nontemporal_store(t2, Tout + plane * (z - 1) + ucol);
                 acc2 += a.cap0 * (t2 - a.tref);

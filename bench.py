@@ -28,9 +28,11 @@ HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
 METRIC = "GCUPS + %HBM roofline, FP64 heat 512³; surrogate vs stencil speedup @1e-3 L2"
 
 
-def _traffic_from_profiles():
-    """HBM bytes per sweep launch, from the committed rocprofv3 counter summary, if present."""
-    p = ROOT / "profiles" / "heat_sweep_512_counters.json"
+def _traffic_from_profiles(steps_per_launch: int = 1):
+    """HBM bytes per sweep launch, from the committed rocprofv3 counter summary of the kernel
+    that ran (one or two steps per launch), if present."""
+    name = "heat_sweep2_512_counters.json" if steps_per_launch == 2 else "heat_sweep_512_counters.json"
+    p = ROOT / "profiles" / name
     if not p.exists():
         return None
     try:
@@ -174,6 +176,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--tile-y", type=int, default=0)
     ap.add_argument("--z-chunk", type=int, default=0)
+    ap.add_argument("--steps-per-pass", type=int, choices=[1, 2], default=2,
+                    help="2 (default): pairs of steps per pass over HBM; 1: one step per pass")
     ap.add_argument("--workload", choices=["heat", "hall", "ensemble", "surrogate", "copy"],
                     default="heat",
                     help="heat (the headline 512^3 block), or the hall / ensemble / surrogate "
@@ -210,6 +214,7 @@ def main() -> int:
     block, dt = w.heterogeneous_block(N)
     if args.tile_y or args.z_chunk:
         block.set_tile(args.tile_y, args.z_chunk)
+    block.set_steps_per_pass(args.steps_per_pass)
     if args.warmup:
         block.advance(dt, args.warmup, tolerance=1e-12, want_totals=False)
     block.set_profiling(True)
@@ -252,7 +257,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": _traffic_from_profiles(),
+            "traffic": _traffic_from_profiles(per_launch),
             "kernel": ("heat_sweep2<8> (per-cell k, uniform C, 64x8 tile, two steps per pass)"
                        if per_launch == 2 else "heat_sweep<1,0,8> (per-cell k, uniform C, 64x8 tile)"),
             "kernel_ms": round(sweep_ms, 5),

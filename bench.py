@@ -73,6 +73,8 @@ def bench_hall(args) -> dict:
     steps = args.steps
     case = w.acoustic_hall(384, steps + args.warmup + 1)
     h = case.hall
+    if args.tile_y or args.z_chunk:
+        h.set_tile(args.tile_y, args.z_chunk)
     h.advance(case.dt, 1 + args.warmup, tolerance=1e-12)  # startup step + warm-up
     t0 = time.perf_counter()
     totals, ms = h.advance(case.dt, steps, tolerance=1e-12)

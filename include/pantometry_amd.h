@@ -225,11 +225,21 @@ int pm_hall_advance(pm_hall *hall, double dt, uint64_t steps, double tolerance, 
 /* All five arrays: p and pressure_prev [nx ny nz], vx [(nx-1) ny nz], vy [nx (ny-1) nz],
  * vz [nx ny (nz-1)].  Any pointer may be NULL. */
 int pm_hall_get_fields(pm_hall *hall, double *p, double *pp, double *vx, double *vy, double *vz);
-/* Domain::checkpoint / restore, hall.rs:536-555. */
+/* Domain::checkpoint / restore, hall.rs:536-555.  With a source set, its booked energy and its
+ * position in the increments go back with the fields. */
 int pm_hall_checkpoint(pm_hall *hall);
 int pm_hall_restore(pm_hall *hall);
 /* Hall::peak_pressure, hall.rs:242-244. */
 double pm_hall_peak_pressure(pm_hall *hall);
+/* Shape of the step: rows per 64-wide column tile (8 or 16; 0 keeps the current) and z-planes
+ * per workgroup (0: chosen from the extents, at most 128 and about 1024 workgroups).  No
+ * reference counterpart. */
+int pm_hall_set_tile(pm_hall *hall, int tile_y, int z_chunk);
+/* How many leapfrog steps pm_hall_advance takes per pass over the grid: 1, or 2 (the default):
+ * pairs of steps in one kernel (temporal blocking: p, vx, vy, vz read once per two steps), the
+ * same arithmetic, fields and per-step audits.  The half step that starts the leapfrog always
+ * runs alone.  No reference counterpart. */
+int pm_hall_set_steps_per_pass(pm_hall *hall, int steps);
 
 /* ---------------------------------------------------------------------------------------- */
 /* Ensemble of independent heat grids, crates/pantometry-core/src/ensemble.rs:102-147       */

@@ -210,6 +210,14 @@ class Hall:
         p = self.fields()[0]
         return float(p[self.node_index(min(i, nx - 1), min(j, ny - 1), min(k, nz - 1))])
 
+    def set_tile(self, tile_y: int = 0, z_chunk: int = 0) -> None:
+        """Rows per 64-wide column tile (8 or 16) and z-planes per workgroup (0: automatic)."""
+        check(lib().pm_hall_set_tile(self._h, int(tile_y), int(z_chunk)), "pm_hall_set_tile")
+
+    def set_steps_per_pass(self, steps: int) -> None:
+        """1, or 2 (default): advance runs pairs of leapfrog steps in one pass."""
+        check(lib().pm_hall_set_steps_per_pass(self._h, int(steps)), "pm_hall_set_steps_per_pass")
+
     def peak_pressure(self) -> float:
         return float(lib().pm_hall_peak_pressure(self._h))
 

@@ -12,7 +12,8 @@
 // neighbours), from the old pressure, exactly as hall.rs:433-457 does; then the new pressure from
 // those six new velocities (hall.rs:465-505).  A face's new velocity is the same expression
 // whichever thread evaluates it, so the redundancy is bit-identical.  The thread then adds its
-// node's share of the invariant and the shares of the faces it owns.
+// node's share of the invariant and the shares of the faces it owns.  The kernel marches in z
+// over column tiles, so each array is read from HBM once per step.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -43,69 +44,437 @@ struct HallArgs {
     double source;       // its increment this step
     double source_prev;  // its increment the step before
     int nx, ny, nz;
+    // Two steps per pass: the pressure after the first step (the second step's pressure_prev),
+    // the source of the second step (node or -1, increment) and its supplied energy.
+    double *__restrict__ p_mid;
+    int64_t src2;
+    double source2;
+    double *__restrict__ supplied_step2;
+    int force;  // run even if an audit breach froze the run
 };
 
 __device__ __forceinline__ double wall(int64_t k, int64_t n) { return (k == 0 || k + 1 == n) ? 2.0 : 1.0; }
 
+// Workgroups of one launch go round-robin over the eight XCDs; the marching kernel gives each XCD
+// a contiguous run of tiles so a tile's neighbours (whose rows it reads as halo) share its L2.
+constexpr int XCDS_HALL = 8;
+
+// The z-marching form of the same step.  A workgroup owns a 64 x TY column tile of nodes and a
+// chunk of z-planes; each thread keeps its own column's p(k-1..k+2) and vz(k-1..k+1) in
+// registers, so every array is read from HBM once per step.  Per plane the workgroup stages p(k)
+// with a one-node halo, vx(k) with the face left of the tile and vy(k) with the face below it in
+// LDS (double-buffered, one barrier per plane) for the in-plane neighbours.  The arithmetic
+// follows hall.rs:433-505 expression for expression.
 template <int TY>
-__global__ __launch_bounds__(64 * TY) void hall_step_kernel(HallArgs a) {
-    constexpr int NT = 64 * TY;
+__global__ __launch_bounds__(64 * TY) void hall_march(HallArgs a, int gx, int gy, int gz, int zc) {
+    constexpr int TX = 64, NT = TX * TY;
+    __shared__ double sP[2][TY + 2][TX + 2];
+    __shared__ double sVX[2][TY][TX + 1];  // faces x0-1 .. x0+63 of each row
+    __shared__ double sVY[2][TY + 1][TX];  // faces y0-1 .. y0+TY-1 of each column
     __shared__ double red[NT / 64];
-    const int64_t nx = a.nx, ny = a.ny, nz = a.nz;
-    const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
-    const int64_t j = (int64_t)blockIdx.y * TY + threadIdx.y;
-    const int64_t k = blockIdx.z;
-    const int tid = threadIdx.x + 64 * threadIdx.y;
-    const int64_t bid = blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
-    double acc = 0.0;
-    if (a.audit->halted) {
-        if (i < nx && j < ny) {
-            const int64_t n = i + nx * (j + ny * k);
-            a.p_out[n] = a.p[n];
-            if (i + 1 < nx) a.vx_out[i + (nx - 1) * (j + ny * k)] = a.vx[i + (nx - 1) * (j + ny * k)];
-            if (j + 1 < ny) a.vy_out[i + nx * (j + (ny - 1) * k)] = a.vy[i + nx * (j + (ny - 1) * k)];
-            if (k + 1 < nz) a.vz_out[n] = a.vz[n];
-        }
-        if (tid == 0) a.partials[bid] = 0.0;
-        return;
+
+    const int64_t ntiles = (int64_t)gx * gy * gz;
+    const int64_t chunk = (ntiles + XCDS_HALL - 1) / XCDS_HALL;
+    const int64_t tile = (int64_t)(blockIdx.x % XCDS_HALL) * chunk + blockIdx.x / XCDS_HALL;
+    if (tile >= ntiles) return;
+    const int64_t bx = tile % gx, by = (tile / gx) % gy, bz = tile / ((int64_t)gx * gy);
+
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TX * ty;
+    const int64_t nx = a.nx, ny = a.ny, nz = a.nz, plane = nx * ny;
+    const int64_t x0 = bx * TX, y0 = by * TY;
+    const int64_t z0 = bz * zc, z1 = std::min<int64_t>(z0 + zc, nz);
+    const int64_t i = x0 + tx, j = y0 + ty;
+    const bool inside = i < nx && j < ny;
+    const int64_t ci = std::min(i, nx - 1), cj = std::min(j, ny - 1);
+    const int64_t col = ci + nx * cj;
+    const bool has_vx = nx > 1, has_vy = ny > 1, has_vz = nz > 1;
+    // Face offsets within a plane of each velocity array (clamped to a face that exists).
+    const int64_t vxp = (nx - 1) * ny, vyp = nx * (ny - 1);
+    const int64_t vxcol = std::min(ci, nx - 2) + (nx - 1) * cj;
+    const int64_t vycol = ci + nx * std::min(cj, ny - 2);
+    auto kc = [&](int64_t q) { return std::min(std::max<int64_t>(q, 0), nz - 1); };
+    auto kz = [&](int64_t q) { return std::min(std::max<int64_t>(q, 0), nz - 2); };
+
+    // A breach froze the run: later launches leave every buffer alone, and the host takes the
+    // state from the launch that broke.
+    if (a.audit->halted && !a.force) return;
+
+    // Halo roles: p one node around the tile (bottom row, top row, left and right columns), the
+    // vx face left of each row (first TY threads), the vy face below each column (first row).
+    int hr = -1, hc = -1;
+    int64_t hidx = 0;
+    if (tid < TX) {
+        hr = 0, hc = tid + 1;
+        hidx = std::min(x0 + tid, nx - 1) + nx * std::max<int64_t>(y0 - 1, 0);
+    } else if (tid < 2 * TX) {
+        hr = TY + 1, hc = tid - TX + 1;
+        hidx = std::min(x0 + tid - TX, nx - 1) + nx * std::min(y0 + TY, ny - 1);
+    } else if (tid < 2 * TX + TY) {
+        hr = tid - 2 * TX + 1, hc = 0;
+        hidx = std::max<int64_t>(x0 - 1, 0) + nx * std::min(y0 + hr - 1, ny - 1);
+    } else if (tid < 2 * TX + 2 * TY) {
+        hr = tid - 2 * TX - TY + 1, hc = TX + 1;
+        hidx = std::min(x0 + TX, nx - 1) + nx * std::min(y0 + hr - 1, ny - 1);
     }
-    if (i < nx && j < ny) {
-        const int64_t n = i + nx * (j + ny * k);
-        const double pc = a.p[n];
-        const double g = a.gain;
-        // New face velocities, hall.rs:433-457: v -= gain * (p[hi] - p[lo]).
+    const bool halo = hr >= 0;
+    const bool vxh = tid < TY && x0 > 0 && has_vx;  // face x0-1 of row y0+tid
+    const int64_t vxhidx = (x0 - 1 >= 0 ? x0 - 1 : 0) + (nx - 1) * std::min(y0 + tid, ny - 1);
+    const bool vyh = ty == 0 && y0 > 0 && has_vy;   // face y0-1 of column x0+tx
+    const int64_t vyhidx = ci + nx * (y0 - 1 >= 0 ? y0 - 1 : 0);
+
+    const double *__restrict__ P = a.p;
+    auto ldp = [&](int64_t idx, int64_t k) { return P[idx + plane * kc(k)]; };
+    auto ldvx = [&](bool ok, int64_t idx, int64_t k) { return ok ? a.vx[idx + vxp * kc(k)] : 0.0; };
+    auto ldvy = [&](bool ok, int64_t idx, int64_t k) { return ok ? a.vy[idx + vyp * kc(k)] : 0.0; };
+    auto ldvz = [&](int64_t k) { return has_vz ? a.vz[col + plane * kz(k)] : 0.0; };
+
+    // Own column: p(k-1), p(k), p(k+1), p(k+2); vz(k-1), vz(k), vz(k+1); vx(k), vx(k+1);
+    // vy(k), vy(k+1).  Halos: the values of planes k and k+1.
+    double pm_ = ldp(col, z0 - 1), pc_ = ldp(col, z0), pp_ = ldp(col, z0 + 1), pq_ = ldp(col, z0 + 2);
+    double zm_ = ldvz(z0 - 1), zc_ = ldvz(z0), zp_ = ldvz(z0 + 1);
+    double xc_ = ldvx(has_vx, vxcol, z0), xp_ = ldvx(has_vx, vxcol, z0 + 1);
+    double yc_ = ldvy(has_vy, vycol, z0), yp_ = ldvy(has_vy, vycol, z0 + 1);
+    double hp0 = halo ? ldp(hidx, z0) : 0.0, hp1 = halo ? ldp(hidx, z0 + 1) : 0.0;
+    double hx0 = ldvx(vxh, vxhidx, z0), hx1 = ldvx(vxh, vxhidx, z0 + 1);
+    double hy0 = ldvy(vyh, vyhidx, z0), hy1 = ldvy(vyh, vyhidx, z0 + 1);
+
+    const double g = a.gain;
+    const double wi = wall(i, nx), wj = wall(j, ny);
+    double acc = 0.0;
+    for (int64_t k = z0; k < z1; k++) {
+        const int b = (int)((k - z0) & 1);
+        sP[b][ty + 1][tx + 1] = pc_;
+        sVX[b][ty][tx + 1] = xc_;
+        sVY[b][ty + 1][tx] = yc_;
+        if (halo) sP[b][hr][hc] = hp0;
+        if (tid < TY) sVX[b][tid][0] = hx0;
+        if (ty == 0) sVY[b][0][tx] = hy0;
+        lds_barrier();
+
+        const double pc = pc_;
         double hi_x = 0.0, lo_x = 0.0, hi_y = 0.0, lo_y = 0.0, hi_z = 0.0, lo_z = 0.0;
-        const int64_t fxi = i + (nx - 1) * (j + ny * k);
-        const int64_t fyi = i + nx * (j + (ny - 1) * k);
-        if (i + 1 < nx) hi_x = a.vx[fxi] - g * (a.p[n + 1] - pc);
-        if (i > 0) lo_x = a.vx[fxi - 1] - g * (pc - a.p[n - 1]);
-        if (j + 1 < ny) hi_y = a.vy[fyi] - g * (a.p[n + nx] - pc);
-        if (j > 0) lo_y = a.vy[fyi - nx] - g * (pc - a.p[n - nx]);
-        if (k + 1 < nz) hi_z = a.vz[n] - g * (a.p[n + nx * ny] - pc);
-        if (k > 0) lo_z = a.vz[n - nx * ny] - g * (pc - a.p[n - nx * ny]);
-        const double wi = wall(i, nx), wj = wall(j, ny), wk = wall(k, nz);
-        // Pressure, hall.rs:494-504: p -= scale * ((wi dx + wj dy) + wk dz).
+        if (i + 1 < nx) hi_x = sVX[b][ty][tx + 1] - g * (sP[b][ty + 1][tx + 2] - pc);
+        if (i > 0) lo_x = sVX[b][ty][tx] - g * (pc - sP[b][ty + 1][tx]);
+        if (j + 1 < ny) hi_y = sVY[b][ty + 1][tx] - g * (sP[b][ty + 2][tx + 1] - pc);
+        if (j > 0) lo_y = sVY[b][ty][tx] - g * (pc - sP[b][ty][tx + 1]);
+        if (k + 1 < nz) hi_z = zc_ - g * (pp_ - pc);
+        if (k > 0) lo_z = zm_ - g * (pc - pm_);
+        const double wk = wall(k, nz);
         double pn = pc - a.scale * (wi * (hi_x - lo_x) + wj * (hi_y - lo_y) + wk * (hi_z - lo_z));
-        if (n == a.src) {
-            pn = pn + a.source;
-            // An increment d at step n moves the invariant by d*p^(n-1) at once and by d*p^n at
-            // step n+1 (whose energy identity uses p^n - p^(n-1)); both use this step's p_old.
+        const int64_t n = col + plane * k;
+        if (inside) {
+            if (n == a.src) {
+                pn = pn + a.source;
+                const double share = 1.0 / (wi * wj * wk);
+                *a.supplied_step = share * (a.source + a.source_prev) * pc / a.two_rc2 * a.volume;
+            }
+            a.p_out[n] = pn;
+            if (i + 1 < nx) a.vx_out[vxcol + vxp * k] = hi_x;
+            if (j + 1 < ny) a.vy_out[vycol + vyp * k] = hi_y;
+            if (k + 1 < nz) a.vz_out[n] = hi_z;
             const double share = 1.0 / (wi * wj * wk);
-            *a.supplied_step = share * (a.source + a.source_prev) * pc / a.two_rc2 * a.volume;
+            acc += share * pn * pc / a.two_rc2 * a.volume;
+            if (i + 1 < nx) acc += 1.0 / (wj * wk) * a.half_rho * hi_x * hi_x * a.volume;
+            if (j + 1 < ny) acc += 1.0 / (wi * wk) * a.half_rho * hi_y * hi_y * a.volume;
+            if (k + 1 < nz) acc += 1.0 / (wi * wj) * a.half_rho * hi_z * hi_z * a.volume;
         }
-        a.p_out[n] = pn;
-        if (i + 1 < nx) a.vx_out[fxi] = hi_x;
-        if (j + 1 < ny) a.vy_out[fyi] = hi_y;
-        if (k + 1 < nz) a.vz_out[n] = hi_z;
-        // The invariant's shares, hall.rs:346-381: the node's, then each owned face's.
-        const double share = 1.0 / (wi * wj * wk);
-        acc += share * pn * pc / a.two_rc2 * a.volume;
-        if (i + 1 < nx) acc += 1.0 / (wj * wk) * a.half_rho * hi_x * hi_x * a.volume;
-        if (j + 1 < ny) acc += 1.0 / (wi * wk) * a.half_rho * hi_y * hi_y * a.volume;
-        if (k + 1 < nz) acc += 1.0 / (wi * wj) * a.half_rho * hi_z * hi_z * a.volume;
+
+        // Advance the rings and fetch the planes two ahead.
+        pm_ = pc_, pc_ = pp_, pp_ = pq_, pq_ = ldp(col, k + 3);
+        zm_ = zc_, zc_ = zp_, zp_ = ldvz(k + 2);
+        xc_ = xp_, xp_ = ldvx(has_vx, vxcol, k + 2);
+        yc_ = yp_, yp_ = ldvy(has_vy, vycol, k + 2);
+        hp0 = hp1, hp1 = halo ? ldp(hidx, k + 2) : 0.0;
+        hx0 = hx1, hx1 = ldvx(vxh, vxhidx, k + 2);
+        hy0 = hy1, hy1 = ldvy(vyh, vyhidx, k + 2);
     }
     const double total = block_sum<NT>(acc, red);
-    if (tid == 0) a.partials[bid] = total;
+    if (tid == 0) a.partials[tile] = total;
+}
+
+// Two steps per pass (temporal blocking) of the marching step.  Per plane k the workgroup
+// computes the first step on the tile widened by one node (E1, 66 x (TY+2), its four corners
+// aside, which no second-step stencil reaches): each node's six new face velocities v' and its
+// new pressure p' (hall.rs:433-505), from p, vx, vy staged on E2 (the tile widened by two) and its
+// own column's p and vz in registers.  The tile's threads keep their nodes' v' in registers and
+// p' in LDS; in the same iteration they take the second step on plane k-1 from those.  A face's
+// v' is computed by both nodes it joins and a ring node's p' by two workgroups: the same
+// expression on the same inputs each time, so every copy agrees bit for bit.  One pass reads p,
+// vx, vy, vz once and writes p'' (the new pressure), p' (its pressure_prev) and the v'' faces:
+// 72 B per two node-steps instead of 128.  The first step's audit sums the owned nodes' p'p and
+// the owned faces' v'^2 as they are produced, the second step's those of p''p' and v''^2.
+template <int TY>
+__global__ __launch_bounds__(64 * TY, 4) void hall_march2(HallArgs a, int gx, int gy, int gz, int zc) {
+    constexpr int TX = 64, NT = TX * TY;
+    constexpr int E1X = TX + 2, E1Y = TY + 2, E2X = TX + 4, E2Y = TY + 4;
+    constexpr int R1 = 2 * (E1X - 2) + 2 * TY;             // E1 ring nodes, corners aside
+    constexpr int R2 = 2 * (E2X - 4) + 2 * (E2Y - 4) + 4;  // E2 ring nodes reached, E1 corners
+    constexpr int RING1_FIRST = NT - R1;
+    static_assert(R1 <= NT && R2 <= RING1_FIRST, "roles do not fit");
+    __shared__ double sP[2][E2Y][E2X];      // p(k) on E2
+    __shared__ double sVX[2][E1Y][E2X - 1];  // vx(k) of E1 rows: face c between E2 columns c, c+1
+    __shared__ double sVY[2][E2Y - 1][E1X];  // vy(k) of E1 columns: face r between E2 rows r, r+1
+    __shared__ double sQ[2][E1Y][E1X];      // p'(k) on E1
+    __shared__ double sHX[2][E1Y][E1X];     // v'x(k) of each E1 node's +x face
+    __shared__ double sHY[2][E1Y][E1X];     // v'y(k) of each E1 node's +y face
+    __shared__ double red[NT / 64];
+
+    const int64_t ntiles = (int64_t)gx * gy * gz;
+    const int64_t chunk = (ntiles + XCDS_HALL - 1) / XCDS_HALL;
+    const int64_t tile = (int64_t)(blockIdx.x % XCDS_HALL) * chunk + blockIdx.x / XCDS_HALL;
+    if (tile >= ntiles) return;
+    if (a.audit->halted && !a.force) return;  // a breach froze the run
+    const int64_t bx = tile % gx, by = (tile / gx) % gy, bz = tile / ((int64_t)gx * gy);
+
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TX * ty;
+    const int64_t nx = a.nx, ny = a.ny, nz = a.nz, plane = nx * ny;
+    const int64_t x0 = bx * TX, y0 = by * TY;
+    const int64_t z0 = bz * zc, z1 = std::min<int64_t>(z0 + zc, nz);
+    const bool has_vx = nx > 1, has_vy = ny > 1, has_vz = nz > 1;
+    const int64_t vxp = (nx - 1) * ny, vyp = nx * (ny - 1);
+    auto cx = [&](int64_t x) { return std::min(std::max<int64_t>(x, 0), nx - 1); };
+    auto cy = [&](int64_t y) { return std::min(std::max<int64_t>(y, 0), ny - 1); };
+    auto kc = [&](int64_t q) { return std::min(std::max<int64_t>(q, 0), nz - 1); };
+    auto kz = [&](int64_t q) { return std::min(std::max<int64_t>(q, 0), nz - 2); };
+    // A node's column offsets in p, vx (its +x face), vy (its +y face), clamped to ones that exist.
+    // (In-plane offsets fit 32 bits: the hall is limited to 2^31 nodes a plane.)
+    struct Col {
+        int x, y;
+        uint32_t p, vx, vy;
+    };
+    auto column = [&](int64_t x, int64_t y) {
+        Col c;
+        c.x = (int)x, c.y = (int)y;
+        c.p = (uint32_t)(cx(x) + nx * cy(y));
+        c.vx = (uint32_t)(std::min(cx(x), nx - 2) + (nx - 1) * cy(y));
+        c.vy = (uint32_t)(cx(x) + nx * std::min(cy(y), ny - 2));
+        return c;
+    };
+
+    // Roles.  Every thread: its tile node, E1 (ty+1, tx+1).  The last R1 threads: an E1 ring node
+    // (r1r, r1c).  The first R2 threads: an E2 node whose p (and, on the left column and bottom
+    // row and at E1's corners, whose +x or +y face) the ring stencils read.
+    const Col tc = column(x0 + tx, y0 + ty);
+    const bool inside = tc.x < nx && tc.y < ny;
+    int r1r = -1, r1c = -1;
+    {
+        const int q = tid - RING1_FIRST;
+        if (q >= 0 && q < E1X - 2) r1r = 0, r1c = 1 + q;
+        else if (q >= E1X - 2 && q < 2 * (E1X - 2)) r1r = E1Y - 1, r1c = 1 + q - (E1X - 2);
+        else if (q >= 2 * (E1X - 2) && q < 2 * (E1X - 2) + TY) r1r = 1 + q - 2 * (E1X - 2), r1c = 0;
+        else if (q >= 2 * (E1X - 2) + TY && q < R1) r1r = 1 + q - 2 * (E1X - 2) - TY, r1c = E1X - 1;
+    }
+    const bool ring1 = r1r >= 0;
+    const Col rc = column(x0 - 1 + r1c, y0 - 1 + r1r);
+    int r2r = -1, r2c = -1;
+    {
+        constexpr int SIDES = 2 * (E2X - 4) + 2 * (E2Y - 4);
+        if (tid < E2X - 4) r2r = 0, r2c = 2 + tid;
+        else if (tid < 2 * (E2X - 4)) r2r = E2Y - 1, r2c = 2 + tid - (E2X - 4);
+        else if (tid < 2 * (E2X - 4) + (E2Y - 4)) r2r = 2 + tid - 2 * (E2X - 4), r2c = 0;
+        else if (tid < SIDES) r2r = 2 + tid - 2 * (E2X - 4) - (E2Y - 4), r2c = E2X - 1;
+        else if (tid < R2) {
+            const int q = tid - SIDES;
+            r2r = (q & 2) ? E2Y - 2 : 1;
+            r2c = (q & 1) ? E2X - 2 : 1;
+        }
+    }
+    const bool ring2 = r2r >= 0;
+    const bool corner = ring2 && (r2r == 1 || r2r == E2Y - 2) && (r2c == 1 || r2c == E2X - 2);
+    const Col hc = column(x0 - 2 + r2c, y0 - 2 + r2r);
+    // which faces a ring-2 thread stages: the +x face on E2's left column and at E1's corners,
+    // the +y face on E2's bottom row and at E1's corners (when those faces exist)
+    const bool h_vx = ring2 && (r2c == 0 || corner) && has_vx && hc.x >= 0 && hc.x + 1 < nx;
+    const bool h_vy = ring2 && (r2r == 0 || corner) && has_vy && hc.y >= 0 && hc.y + 1 < ny;
+
+    const double *__restrict__ P = a.p;
+    auto ldp = [&](const Col &c, int64_t k) { return P[c.p + plane * kc(k)]; };
+    auto ldvx = [&](bool ok, const Col &c, int64_t k) { return ok ? a.vx[c.vx + vxp * kc(k)] : 0.0; };
+    auto ldvy = [&](bool ok, const Col &c, int64_t k) { return ok ? a.vy[c.vy + vyp * kc(k)] : 0.0; };
+    auto ldvz = [&](const Col &c, int64_t k) { return has_vz ? a.vz[c.p + plane * kz(k)] : 0.0; };
+    const bool t_vx = has_vx, t_vy = has_vy;  // loads of a node's own +x/+y face (clamped)
+
+    const double g = a.gain, sc = a.scale;
+    // The first step at E1 node (r, c) of plane k: new face velocities and pressure.
+    struct Step {
+        double hx, lx, hy, ly, hz, lz, p;
+    };
+    auto first = [&](int b, int r, int c, const Col &cc, int64_t k, double pm, double pcur,
+                     double pp, double vzm, double vzc, int64_t src, double source) {
+        Step o;
+        const int64_t x = cc.x, y = cc.y;
+        o.hx = 0.0, o.lx = 0.0, o.hy = 0.0, o.ly = 0.0, o.hz = 0.0, o.lz = 0.0;
+        if (x + 1 < nx) o.hx = sVX[b][r][c + 1] - g * (sP[b][r + 1][c + 2] - pcur);
+        if (x > 0) o.lx = sVX[b][r][c] - g * (pcur - sP[b][r + 1][c]);
+        if (y + 1 < ny) o.hy = sVY[b][r + 1][c] - g * (sP[b][r + 2][c + 1] - pcur);
+        if (y > 0) o.ly = sVY[b][r][c] - g * (pcur - sP[b][r][c + 1]);
+        if (k + 1 < nz) o.hz = vzc - g * (pp - pcur);
+        if (k > 0) o.lz = vzm - g * (pcur - pm);
+        const double wi = wall(x, nx), wj = wall(y, ny), wk = wall(k, nz);
+        double pn = pcur - sc * (wi * (o.hx - o.lx) + wj * (o.hy - o.ly) + wk * (o.hz - o.lz));
+        if (x + nx * (y + ny * k) == src) pn = pn + source;
+        o.p = pn;
+        return o;
+    };
+
+    const int64_t zs = z0 - 1;
+    // Own columns: p(k-1..k+1), vz(k-1..k), vx(k), vy(k), fetched one plane ahead; the tile node
+    // also p'(k-2..k) and the previous plane's v'.  Ring-2: p, vx, vy of plane k.
+    double tpm = ldp(tc, zs - 1), tpc = ldp(tc, zs), tpp = ldp(tc, zs + 1);
+    double tzm = ldvz(tc, zs - 1), tzc = ldvz(tc, zs);
+    double txc = ldvx(t_vx, tc, zs), tyc = ldvy(t_vy, tc, zs);
+    // Ring-1 column registers.  A ring-2 thread, never a ring-1 thread, keeps its p, vx, vy of
+    // plane k in the first three of them.
+    double rpm = 0.0, rpc = 0.0, rpp = 0.0, rzm = 0.0, rzc = 0.0, rxc = 0.0, ryc = 0.0;
+    double &hp0 = rpm, &hx0 = rpc, &hy0 = rpp;
+    if (ring1) {
+        rpm = ldp(rc, zs - 1), rpc = ldp(rc, zs), rpp = ldp(rc, zs + 1);
+        rzm = ldvz(rc, zs - 1), rzc = ldvz(rc, zs);
+        rxc = ldvx(t_vx, rc, zs), ryc = ldvy(t_vy, rc, zs);
+    }
+    if (ring2) {
+        hp0 = ldp(hc, zs);
+        hx0 = ldvx(h_vx, hc, zs);
+        hy0 = ldvy(h_vy, hc, zs);
+    }
+    double qm = 0.0, qc = 0.0;  // the tile node's p'(k-2), p'(k-1)
+    double plz = 0.0;           // its v'z below plane k-1 (the one above is this plane's lz)
+
+    const double wi = wall(tc.x, nx), wj = wall(tc.y, ny);
+    double acc1 = 0.0, acc2 = 0.0;
+    for (int64_t k = zs; k <= z1; k++) {
+        const int b = (int)((k - zs) & 1);
+        // Stage p(k), vx(k), vy(k) on E2.
+        sP[b][ty + 2][tx + 2] = tpc;
+        sVX[b][ty + 1][tx + 2] = txc;
+        sVY[b][ty + 2][tx + 1] = tyc;
+        if (ring1) {
+            sP[b][r1r + 1][r1c + 1] = rpc;
+            sVX[b][r1r][r1c + 1] = rxc;
+            sVY[b][r1r + 1][r1c] = ryc;
+        }
+        if (ring2) {
+            sP[b][r2r][r2c] = hp0;
+            if (r2c == 0 || corner) sVX[b][r2r - 1][r2c] = hx0;
+            if (r2r == 0 || corner) sVY[b][r2r][r2c - 1] = hy0;
+        }
+        lds_barrier();
+
+        // First step on plane k.
+        const Step t1 = first(b, ty + 1, tx + 1, tc, k, tpm, tpc, tpp, tzm, tzc, a.src, a.source);
+        sQ[b][ty + 1][tx + 1] = t1.p;
+        sHX[b][ty + 1][tx + 1] = t1.hx;
+        sHY[b][ty + 1][tx + 1] = t1.hy;
+        if (ring1) {
+            const Step r1 = first(b, r1r, r1c, rc, k, rpm, rpc, rpp, rzm, rzc, a.src, a.source);
+            sQ[b][r1r][r1c] = r1.p;
+            sHX[b][r1r][r1c] = r1.hx;
+            sHY[b][r1r][r1c] = r1.hy;
+        }
+        const int64_t n1 = tc.p + plane * k;
+        if (inside && k >= z0 && k < z1) {
+            const double wk = wall(k, nz);
+            const double share = 1.0 / (wi * wj * wk);
+            if (n1 == a.src)
+                *a.supplied_step = share * (a.source + a.source_prev) * tpc / a.two_rc2 * a.volume;
+            acc1 += share * t1.p * tpc / a.two_rc2 * a.volume;
+            if (tc.x + 1 < nx) acc1 += 1.0 / (wj * wk) * a.half_rho * t1.hx * t1.hx * a.volume;
+            if (tc.y + 1 < ny) acc1 += 1.0 / (wi * wk) * a.half_rho * t1.hy * t1.hy * a.volume;
+            if (k + 1 < nz) acc1 += 1.0 / (wi * wj) * a.half_rho * t1.hz * t1.hz * a.volume;
+        }
+
+        // Second step on plane k-1, from p' of planes k-2, k-1 (registers, LDS) and k, and the
+        // v' faces of plane k-1: in-plane ones staged by their nodes, z ones in registers.
+        {
+            const int64_t k2 = k - 1;
+            const double pc = qc;
+            const int r = ty + 1, c = tx + 1;
+            double hx = 0.0, lx = 0.0, hy = 0.0, ly = 0.0, hz = 0.0, lz = 0.0;
+            if (tc.x + 1 < nx) hx = sHX[b ^ 1][r][c] - g * (sQ[b ^ 1][r][c + 1] - pc);
+            if (tc.x > 0) lx = sHX[b ^ 1][r][c - 1] - g * (pc - sQ[b ^ 1][r][c - 1]);
+            if (tc.y + 1 < ny) hy = sHY[b ^ 1][r][c] - g * (sQ[b ^ 1][r + 1][c] - pc);
+            if (tc.y > 0) ly = sHY[b ^ 1][r - 1][c] - g * (pc - sQ[b ^ 1][r - 1][c]);
+            if (k2 + 1 < nz) hz = t1.lz - g * (t1.p - pc);
+            if (k2 > 0) lz = plz - g * (pc - qm);
+            const double wk = wall(k2, nz);
+            double pn = pc - sc * (wi * (hx - lx) + wj * (hy - ly) + wk * (hz - lz));
+            const int64_t n2 = tc.p + plane * k2;
+            if (n2 == a.src2) pn = pn + a.source2;
+            if (inside && k2 >= z0 && k2 < z1) {
+                const double share = 1.0 / (wi * wj * wk);
+                if (n2 == a.src2)
+                    *a.supplied_step2 = share * (a.source2 + a.source) * pc / a.two_rc2 * a.volume;
+                __builtin_nontemporal_store(pn, a.p_out + n2);
+                __builtin_nontemporal_store(pc, a.p_mid + n2);
+                if (tc.x + 1 < nx) __builtin_nontemporal_store(hx, a.vx_out + tc.vx + vxp * k2);
+                if (tc.y + 1 < ny) __builtin_nontemporal_store(hy, a.vy_out + tc.vy + vyp * k2);
+                if (k2 + 1 < nz) __builtin_nontemporal_store(hz, a.vz_out + n2);
+                acc2 += share * pn * pc / a.two_rc2 * a.volume;
+                if (tc.x + 1 < nx) acc2 += 1.0 / (wj * wk) * a.half_rho * hx * hx * a.volume;
+                if (tc.y + 1 < ny) acc2 += 1.0 / (wi * wk) * a.half_rho * hy * hy * a.volume;
+                if (k2 + 1 < nz) acc2 += 1.0 / (wi * wj) * a.half_rho * hz * hz * a.volume;
+            }
+        }
+        qm = qc, qc = t1.p;
+        plz = t1.lz;
+
+        // Advance the rings; fetch the next plane.
+        tpm = tpc, tpc = tpp, tpp = ldp(tc, k + 2);
+        tzm = tzc, tzc = ldvz(tc, k + 1);
+        txc = ldvx(t_vx, tc, k + 1);
+        tyc = ldvy(t_vy, tc, k + 1);
+        if (ring1) {
+            rpm = rpc, rpc = rpp, rpp = ldp(rc, k + 2);
+            rzm = rzc, rzc = ldvz(rc, k + 1);
+            rxc = ldvx(t_vx, rc, k + 1);
+            ryc = ldvy(t_vy, rc, k + 1);
+        }
+        if (ring2) {
+            hp0 = ldp(hc, k + 1);
+            hx0 = ldvx(h_vx, hc, k + 1);
+            hy0 = ldvy(h_vy, hc, k + 1);
+        }
+    }
+    const double total1 = block_sum<NT>(acc1, red);
+    __syncthreads();
+    const double total2 = block_sum<NT>(acc2, red);
+    if (tid == 0) {
+        a.partials[tile] = total1;
+        a.partials[ntiles + tile] = total2;
+    }
+}
+
+// The audits of the two steps of a pass, in order: supplied energy and ledger of the first step,
+// then of the second (a breach on the first freezes the run before the second is booked).
+__global__ __launch_bounds__(1024) void hall_finalize2(const double *partials, int64_t n,
+                                                       AuditState *st, double offset,
+                                                       double *supplied_total,
+                                                       const double *supplied1,
+                                                       const double *supplied2, int64_t step,
+                                                       double *totals) {
+    __shared__ double scratch[16];
+    if (st->halted) return;
+    const double inv1 = fixed_order_sum_1024(partials, n, scratch);
+    __syncthreads();
+    const double inv2 = fixed_order_sum_1024(partials + n, n, scratch);
+    if (threadIdx.x == 0) {
+        double supplied = *supplied_total;
+        if (supplied1) supplied += *supplied1;
+        double energy = inv1 + offset;
+        double total = 0.0;
+        total += energy;
+        total += -supplied;
+        audit_step(st, total, rmax(rmax(0.0, std::fabs(energy)), std::fabs(supplied)), step, totals);
+        *supplied_total = supplied;
+        if (st->halted) return;
+        if (supplied2) supplied += *supplied2;
+        energy = inv2 + offset;
+        total = 0.0;
+        total += energy;
+        total += -supplied;
+        audit_step(st, total, rmax(rmax(0.0, std::fabs(energy)), std::fabs(supplied)), step + 1, totals);
+        *supplied_total = supplied;
+    }
 }
 
 // The invariant of the current state without stepping (the release datum, ledger reads).
@@ -191,10 +560,15 @@ struct pm_hall {
     bool staggered = false;
     double energy_offset = 0.0, energy_datum = 0.0;
     hipStream_t stream = nullptr;
-    DevBuf p[2], vx[2], vy[2], vz[2];
-    int cur = 0;
+    // Pressure rotates through three buffers: current, previous (pressure_prev) and a spare the
+    // two-steps-per-pass kernel writes its intermediate pressure into.  Velocities ping-pong.
+    DevBuf p[3], vx[2], vy[2], vz[2];
+    int ip = 0, iq = 1, is = 2, iv = 0;
+    int steps_per_pass = 2;
     DevBuf saved_p, saved_vx, saved_vy, saved_vz;
     bool saved_valid = false, saved_staggered = false;
+    double saved_supplied = 0.0;  // the source's booked energy and cursor at the checkpoint
+    uint64_t saved_cursor = 0;
     DevBuf partials, audit, scalars;  // scalars: [0] invariant, [1] supplied total, [2] supplied step
     int64_t partial_count = 0;
     DevBuf src_inc;
@@ -203,6 +577,7 @@ struct pm_hall {
     std::vector<double> src_host;
     double supplied_host = 0.0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int tile_y = 8, z_chunk = 0;  // marching tiles; z_chunk 0: chosen from the extents
 
     int64_t nodes() const { return nx * ny * nz; }
     int64_t nfx() const { return (nx - 1) * ny * nz; }
@@ -212,10 +587,25 @@ struct pm_hall {
 
 namespace pm {
 
-static constexpr int HALL_TY = 4;
+// Tiles of the marching step: 64 x tile_y columns and, unless fixed, z-chunks of at most 128
+// planes, short enough that a launch has about 1024 workgroups (four per compute unit), at
+// least 4 planes.
+struct HallGrid {
+    int gx, gy, gz, zc;
+    int64_t tiles() const { return (int64_t)gx * gy * gz; }
+};
 
-static dim3 hall_grid(const pm_hall *h) {
-    return dim3((unsigned)((h->nx + 63) / 64), (unsigned)((h->ny + HALL_TY - 1) / HALL_TY), (unsigned)h->nz);
+static HallGrid hall_grid(const pm_hall *h, int tile_y = 0) {
+    const int ty = tile_y ? tile_y : h->tile_y;
+    HallGrid g;
+    g.gx = (int)((h->nx + 63) / 64);
+    g.gy = (int)((h->ny + ty - 1) / ty);
+    const int64_t xy = (int64_t)g.gx * g.gy;
+    const int64_t chunks = std::max<int64_t>((1024 + xy - 1) / xy, (h->nz + 127) / 128);
+    g.zc = h->z_chunk > 0 ? h->z_chunk
+                          : (int)std::max<int64_t>(4, (h->nz + chunks - 1) / chunks);
+    g.gz = (int)((h->nz + g.zc - 1) / g.zc);
+    return g;
 }
 
 static int hall_invariant(pm_hall *h, const double *p, const double *pp, const double *vx,
@@ -242,8 +632,8 @@ static int hall_invariant(pm_hall *h, const double *p, const double *pp, const d
 }
 
 static int hall_invariant_now(pm_hall *h, double *out) {
-    return hall_invariant(h, h->p[h->cur].d(), h->p[h->cur ^ 1].d(), h->vx[h->cur].d(),
-                          h->vy[h->cur].d(), h->vz[h->cur].d(), out);
+    return hall_invariant(h, h->p[h->ip].d(), h->p[h->iq].d(), h->vx[h->iv].d(),
+                          h->vy[h->iv].d(), h->vz[h->iv].d(), out);
 }
 
 static int hall_check(const pm_hall *h, double dt, pm_violation *v, int64_t step) {
@@ -256,50 +646,113 @@ static int hall_check(const pm_hall *h, double dt, pm_violation *v, int64_t step
     return PM_OK;
 }
 
-// One fused step from buffer `cur` into `cur^1`, then the finalize (audit when step >= 0).
-static int hall_sweep(pm_hall *h, double hstep, double vh, int64_t step, double *totals_dev, double offset) {
-    const dim3 g = hall_grid(h);
-    const int64_t nparts = (int64_t)g.x * g.y * g.z;
-    if (h->partial_count < nparts) {
-        PM_CHECK(h->partials.alloc(sizeof(double) * (size_t)nparts));
-        h->partial_count = nparts;
+// The source terms of the step the cursor is at: whether the source acts, its increment and
+// the previous increment (hall.rs has no source; the extension books it as supplied energy).
+struct SourceStep {
+    int64_t node;
+    double inc, prev;
+};
+static SourceStep source_at(const pm_hall *h, uint64_t cursor) {
+    SourceStep s{-1, 0.0, 0.0};
+    if (h->src >= 0 && cursor <= h->src_count) {
+        s.node = h->src;
+        if (cursor < h->src_count) s.inc = h->src_host[cursor];
+        if (cursor > 0) s.prev = h->src_host[cursor - 1];
     }
+    return s;
+}
+
+static HallArgs hall_args(pm_hall *h, double hstep, double vh) {
     const double rc2 = h->density * h->speed * h->speed;
-    HallArgs a;
-    const int c = h->cur, o = h->cur ^ 1;
-    a.p = h->p[c].d(), a.vx = h->vx[c].d(), a.vy = h->vy[c].d(), a.vz = h->vz[c].d();
-    a.p_out = h->p[o].d(), a.vx_out = h->vx[o].d(), a.vy_out = h->vy[o].d(), a.vz_out = h->vz[o].d();
+    HallArgs a{};
     a.partials = h->partials.d();
     a.supplied_step = h->scalars.d() + 2;
+    a.supplied_step2 = h->scalars.d() + 3;
     a.audit = h->audit.as<AuditState>();
     a.gain = vh / (h->density * h->dx);
     a.scale = hstep * rc2 / h->dx;
     a.two_rc2 = 2.0 * rc2;
     a.volume = h->dx * h->dx * h->dx;
     a.half_rho = 0.5 * h->density;
-    a.src = -1;
-    a.source = 0.0;
-    a.source_prev = 0.0;
-    bool sourced = false;
-    if (h->src >= 0 && h->src_cursor <= h->src_count) {
-        a.src = h->src;
-        if (h->src_cursor < h->src_count) a.source = h->src_host[h->src_cursor];
-        if (h->src_cursor > 0) a.source_prev = h->src_host[h->src_cursor - 1];
-        sourced = true;
-    }
-    h->src_cursor++;
     a.nx = (int)h->nx, a.ny = (int)h->ny, a.nz = (int)h->nz;
+    a.src = a.src2 = -1;
+    return a;
+}
+
+static int hall_partials(pm_hall *h, int64_t count) {
+    if (h->partial_count < count) {
+        PM_CHECK(h->partials.alloc(sizeof(double) * (size_t)count));
+        h->partial_count = count;
+    }
+    return PM_OK;
+}
+
+// One fused step from the current buffers into the others, then the finalize (audit when
+// step >= 0); `force` runs it even after a breach, without the finalize.
+static int hall_sweep(pm_hall *h, double hstep, double vh, int64_t step, double *totals_dev,
+                      double offset, bool force = false) {
+    const HallGrid g = hall_grid(h);
+    const int64_t nparts = g.tiles();
+    PM_CHECK(hall_partials(h, nparts));
+    HallArgs a = hall_args(h, hstep, vh);
+    const int v = h->iv, vo = h->iv ^ 1;
+    a.p = h->p[h->ip].d(), a.vx = h->vx[v].d(), a.vy = h->vy[v].d(), a.vz = h->vz[v].d();
+    a.p_out = h->p[h->iq].d(), a.vx_out = h->vx[vo].d(), a.vy_out = h->vy[vo].d(), a.vz_out = h->vz[vo].d();
+    a.force = force ? 1 : 0;
+    const SourceStep src = source_at(h, h->src_cursor);
+    a.src = src.node, a.source = src.inc, a.source_prev = src.prev;
+    const bool sourced = src.node >= 0;
+    h->src_cursor++;
     if (sourced) {
         hall_zero<<<1, 64, 0, h->stream>>>(h->scalars.d() + 2, 1);
         PM_HIP(hipGetLastError());
     }
-    hall_step_kernel<HALL_TY><<<g, dim3(64, HALL_TY), 0, h->stream>>>(a);
+    const unsigned blocks = (unsigned)(XCDS_HALL * ((nparts + XCDS_HALL - 1) / XCDS_HALL));
+    if (h->tile_y == 16)
+        hall_march<16><<<blocks, dim3(64, 16), 0, h->stream>>>(a, g.gx, g.gy, g.gz, g.zc);
+    else
+        hall_march<8><<<blocks, dim3(64, 8), 0, h->stream>>>(a, g.gx, g.gy, g.gz, g.zc);
     PM_HIP(hipGetLastError());
-    hall_finalize<<<1, 1024, 0, h->stream>>>(h->partials.d(), nparts, h->audit.as<AuditState>(), offset,
-                                             h->scalars.d() + 1, sourced ? h->scalars.d() + 2 : nullptr, step,
-                                             totals_dev, h->scalars.d());
+    if (!force) {
+        hall_finalize<<<1, 1024, 0, h->stream>>>(h->partials.d(), nparts, h->audit.as<AuditState>(), offset,
+                                                 h->scalars.d() + 1, sourced ? h->scalars.d() + 2 : nullptr, step,
+                                                 totals_dev, h->scalars.d());
+        PM_HIP(hipGetLastError());
+    }
+    std::swap(h->ip, h->iq);
+    h->iv = vo;
+    return PM_OK;
+}
+
+// Steps `step` and `step + 1` (both full leapfrog steps) in one pass, then both audits.
+static constexpr int HALL_PAIR_TY = 8;
+static int hall_sweep_pair(pm_hall *h, double hstep, int64_t step, double *totals_dev, double offset) {
+    const HallGrid g = hall_grid(h, HALL_PAIR_TY);
+    const int64_t nparts = g.tiles();
+    PM_CHECK(hall_partials(h, 2 * nparts));
+    HallArgs a = hall_args(h, hstep, hstep);
+    const int v = h->iv, vo = h->iv ^ 1;
+    a.p = h->p[h->ip].d(), a.vx = h->vx[v].d(), a.vy = h->vy[v].d(), a.vz = h->vz[v].d();
+    a.p_out = h->p[h->iq].d(), a.p_mid = h->p[h->is].d();
+    a.vx_out = h->vx[vo].d(), a.vy_out = h->vy[vo].d(), a.vz_out = h->vz[vo].d();
+    const SourceStep s1 = source_at(h, h->src_cursor), s2 = source_at(h, h->src_cursor + 1);
+    a.src = s1.node, a.source = s1.inc, a.source_prev = s1.prev;
+    a.src2 = s2.node, a.source2 = s2.inc;
+    h->src_cursor += 2;
+    if (s1.node >= 0 || s2.node >= 0) {
+        hall_zero<<<1, 64, 0, h->stream>>>(h->scalars.d() + 2, 2);
+        PM_HIP(hipGetLastError());
+    }
+    const unsigned blocks = (unsigned)(XCDS_HALL * ((nparts + XCDS_HALL - 1) / XCDS_HALL));
+    hall_march2<HALL_PAIR_TY><<<blocks, dim3(64, HALL_PAIR_TY), 0, h->stream>>>(a, g.gx, g.gy, g.gz, g.zc);
     PM_HIP(hipGetLastError());
-    h->cur = o;
+    hall_finalize2<<<1, 1024, 0, h->stream>>>(h->partials.d(), nparts, h->audit.as<AuditState>(), offset,
+                                              h->scalars.d() + 1, s1.node >= 0 ? h->scalars.d() + 2 : nullptr,
+                                              s2.node >= 0 ? h->scalars.d() + 3 : nullptr, step, totals_dev);
+    PM_HIP(hipGetLastError());
+    const int ip = h->ip, iq = h->iq, is = h->is;
+    h->ip = iq, h->iq = is, h->is = ip;
+    h->iv = vo;
     return PM_OK;
 }
 
@@ -364,6 +817,8 @@ int pm_hall_create(const char *name, double width, double height, double depth, 
     PM_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     PM_HIP(hipEventCreate(&h->ev0));
     PM_HIP(hipEventCreate(&h->ev1));
+    PM_CHECK(h->p[2].alloc(sizeof(double) * (size_t)h->nodes()));
+    PM_HIP(hipMemsetAsync(h->p[2].d(), 0, h->p[2].bytes(), h->stream));
     for (int b = 0; b < 2; b++) {
         PM_CHECK(h->p[b].alloc(sizeof(double) * (size_t)h->nodes()));
         PM_CHECK(h->vx[b].alloc(sizeof(double) * (size_t)std::max<int64_t>(h->nfx(), 1)));
@@ -403,7 +858,8 @@ double pm_hall_spacing(const pm_hall *h) { return h ? h->dx : NAN; }
 
 int pm_hall_release(pm_hall *h, const double *pressure) {
     if (!h || !pressure) return PM_ERR_ARGUMENT;
-    h->cur = 0;
+    h->ip = 0, h->iq = 1, h->is = 2, h->iv = 0;
+    PM_HIP(hipMemcpyAsync(h->p[2].d(), pressure, sizeof(double) * (size_t)h->nodes(), hipMemcpyHostToDevice, h->stream));
     for (int b = 0; b < 2; b++) {
         PM_HIP(hipMemcpyAsync(h->p[b].d(), pressure, sizeof(double) * (size_t)h->nodes(), hipMemcpyHostToDevice, h->stream));
         PM_HIP(hipMemsetAsync(h->vx[b].d(), 0, h->vx[b].bytes(), h->stream));
@@ -499,7 +955,26 @@ int pm_hall_advance(pm_hall *h, double dt, uint64_t steps, double tolerance, dou
     const double scale0 = rmax(rmax(0.0, std::fabs(energy)), std::fabs(h->supplied_host));
     if (first == 1 && totals) totals[0] = total0;
     PM_CHECK(hall_reset_audit(h, total0, scale0, tolerance));
-    for (uint64_t n = first; n < steps; n++) PM_CHECK(hall_sweep(h, dt, dt, (int64_t)n, dtot.d(), h->energy_offset));
+    // Launches in order, with the buffers and source cursor each started from.
+    struct Launch {
+        int64_t first;
+        int steps;
+        int ip, iq, is, iv;
+        uint64_t cursor;
+    };
+    std::vector<Launch> launches;
+    const bool pairs = h->steps_per_pass == 2;
+    for (uint64_t n = first; n < steps;) {
+        launches.push_back({(int64_t)n, 1, h->ip, h->iq, h->is, h->iv, h->src_cursor});
+        if (pairs && n + 1 < steps) {
+            launches.back().steps = 2;
+            PM_CHECK(hall_sweep_pair(h, dt, (int64_t)n, dtot.d(), h->energy_offset));
+            n += 2;
+        } else {
+            PM_CHECK(hall_sweep(h, dt, dt, (int64_t)n, dtot.d(), h->energy_offset));
+            n += 1;
+        }
+    }
     PM_HIP(hipEventRecord(h->ev1, h->stream));
     AuditState st;
     PM_HIP(hipMemcpyAsync(&st, h->audit.d(), sizeof st, hipMemcpyDeviceToHost, h->stream));
@@ -515,17 +990,49 @@ int pm_hall_advance(pm_hall *h, double dt, uint64_t steps, double tolerance, dou
         *elapsed_ms = ms;
     }
     if (st.halted) {
+        // Launches after the breach did nothing: the state is what the breaking step made it.
+        // A breach on the first step of a pair is recomputed alone from the pair's input, which
+        // nothing has overwritten.
+        for (const Launch &l : launches) {
+            if (st.breach_step < l.first || st.breach_step >= l.first + l.steps) continue;
+            h->ip = l.ip, h->iq = l.iq, h->is = l.is, h->iv = l.iv;
+            h->src_cursor = l.cursor;
+            if (l.steps == 2 && st.breach_step == l.first) {
+                PM_CHECK(hall_sweep(h, dt, dt, -1, nullptr, h->energy_offset, true));
+                PM_HIP(hipStreamSynchronize(h->stream));
+            } else if (l.steps == 2) {
+                const int ip = h->ip, iq = h->iq, is = h->is;
+                h->ip = iq, h->iq = is, h->is = ip;
+                h->iv ^= 1;
+                h->src_cursor += 2;
+            } else {
+                std::swap(h->ip, h->iq);
+                h->iv ^= 1;
+                h->src_cursor += 1;
+            }
+            break;
+        }
         fill_violation(v, "energy", "simulation", st.before, st.after, st.scale, tolerance, st.breach_step);
         return PM_VIOLATION;
     }
     return PM_OK;
 }
 
+int pm_hall_set_steps_per_pass(pm_hall *h, int steps) {
+    if (!h) return PM_ERR_ARGUMENT;
+    if (steps != 1 && steps != 2) {
+        set_error("steps per pass must be 1 or 2");
+        return PM_ERR_ARGUMENT;
+    }
+    h->steps_per_pass = steps;
+    return PM_OK;
+}
+
 int pm_hall_get_fields(pm_hall *h, double *p, double *pp, double *vx, double *vy, double *vz) {
     if (!h) return PM_ERR_ARGUMENT;
-    const int c = h->cur;
-    if (p) PM_HIP(hipMemcpyAsync(p, h->p[c].d(), sizeof(double) * (size_t)h->nodes(), hipMemcpyDeviceToHost, h->stream));
-    if (pp) PM_HIP(hipMemcpyAsync(pp, h->p[c ^ 1].d(), sizeof(double) * (size_t)h->nodes(), hipMemcpyDeviceToHost, h->stream));
+    const int c = h->iv;
+    if (p) PM_HIP(hipMemcpyAsync(p, h->p[h->ip].d(), sizeof(double) * (size_t)h->nodes(), hipMemcpyDeviceToHost, h->stream));
+    if (pp) PM_HIP(hipMemcpyAsync(pp, h->p[h->iq].d(), sizeof(double) * (size_t)h->nodes(), hipMemcpyDeviceToHost, h->stream));
     if (vx && h->nfx()) PM_HIP(hipMemcpyAsync(vx, h->vx[c].d(), sizeof(double) * (size_t)h->nfx(), hipMemcpyDeviceToHost, h->stream));
     if (vy && h->nfy()) PM_HIP(hipMemcpyAsync(vy, h->vy[c].d(), sizeof(double) * (size_t)h->nfy(), hipMemcpyDeviceToHost, h->stream));
     if (vz && h->nfz()) PM_HIP(hipMemcpyAsync(vz, h->vz[c].d(), sizeof(double) * (size_t)h->nfz(), hipMemcpyDeviceToHost, h->stream));
@@ -541,28 +1048,45 @@ int pm_hall_checkpoint(pm_hall *h) {
         PM_CHECK(h->saved_vy.alloc(h->vy[0].bytes()));
         PM_CHECK(h->saved_vz.alloc(h->vz[0].bytes()));
     }
-    const int c = h->cur;
-    PM_CHECK(hall_copy(h, h->saved_p, h->p[c], h->nodes()));
+    const int c = h->iv;
+    PM_CHECK(hall_copy(h, h->saved_p, h->p[h->ip], h->nodes()));
     PM_CHECK(hall_copy(h, h->saved_vx, h->vx[c], h->nfx()));
     PM_CHECK(hall_copy(h, h->saved_vy, h->vy[c], h->nfy()));
     PM_CHECK(hall_copy(h, h->saved_vz, h->vz[c], h->nfz()));
     PM_HIP(hipStreamSynchronize(h->stream));
     h->saved_valid = true;
     h->saved_staggered = h->staggered;
+    h->saved_supplied = h->supplied_host;
+    h->saved_cursor = h->src_cursor;
     return PM_OK;
 }
 
 int pm_hall_restore(pm_hall *h) {
     if (!h) return PM_ERR_ARGUMENT;
     if (!h->saved_valid) return PM_OK;  // `if let Some(..) = self.saved` (hall.rs:547)
-    const int c = h->cur;
-    PM_CHECK(hall_copy(h, h->p[c], h->saved_p, h->nodes()));
-    PM_CHECK(hall_copy(h, h->p[c ^ 1], h->saved_p, h->nodes()));  // pressure_prev = p
+    const int c = h->iv;
+    PM_CHECK(hall_copy(h, h->p[h->ip], h->saved_p, h->nodes()));
+    PM_CHECK(hall_copy(h, h->p[h->iq], h->saved_p, h->nodes()));  // pressure_prev = p
     PM_CHECK(hall_copy(h, h->vx[c], h->saved_vx, h->nfx()));
     PM_CHECK(hall_copy(h, h->vy[c], h->saved_vy, h->nfy()));
     PM_CHECK(hall_copy(h, h->vz[c], h->saved_vz, h->nfz()));
+    // The source's books go back with the fields.
+    h->supplied_host = h->saved_supplied;
+    h->src_cursor = h->saved_cursor;
+    PM_HIP(hipMemcpyAsync(h->scalars.d() + 1, &h->saved_supplied, sizeof(double), hipMemcpyHostToDevice, h->stream));
     PM_HIP(hipStreamSynchronize(h->stream));
     h->staggered = h->saved_staggered;
+    return PM_OK;
+}
+
+int pm_hall_set_tile(pm_hall *h, int tile_y, int z_chunk) {
+    if (!h) return PM_ERR_ARGUMENT;
+    if ((tile_y && tile_y != 8 && tile_y != 16) || z_chunk < 0) {
+        set_error("hall tiles: tile_y must be 8 or 16 and z_chunk positive");
+        return PM_ERR_ARGUMENT;
+    }
+    if (tile_y) h->tile_y = tile_y;
+    h->z_chunk = z_chunk;
     return PM_OK;
 }
 

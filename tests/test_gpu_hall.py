@@ -62,6 +62,89 @@ def test_fields_are_bit_exact(dims, across, mode):
     assert abs(g.startup_adjustment() - o.offset) <= 1e-12 * abs(e_ref)
 
 
+@pytest.mark.parametrize("tile_y,z_chunk", [(8, 1), (8, 3), (16, 0), (16, 5), (8, 64)])
+def test_every_tile_shape_gives_the_same_bits(tile_y, z_chunk):
+    gpu_case(f"hall tile {tile_y} x {z_chunk}")
+    import pantometry_amd as pm
+    g, o = _pair(0.5, 0.07, 0.3, 130)
+    g.set_tile(tile_y, z_chunk)
+    g.released_in_mode((2, 1, 3), 2.0)
+    o.release(o.mode((2, 1, 3), 2.0))
+    dt = 0.9 * o.max_stable_dt()
+    for n in range(40):
+        g.step(0.0, dt, pm.Exchange())
+        o.step(dt, threads_=16)
+    _same_fields(g, o)
+
+
+@pytest.mark.parametrize("dims,across,mode", [
+    ((4.4, 3.1, 2.4), 13, (1, 1, 1)),
+    ((4.4, 3.1, 0.0), 45, (1, 1, 0)),
+    ((0.5, 0.07, 0.3), 130, (2, 1, 3)),
+], ids=str)
+@pytest.mark.parametrize("per_pass", [1, 2])
+def test_advance_is_bit_exact(dims, across, mode, per_pass):
+    """advance runs pairs of leapfrog steps per pass by default: an odd number of audited
+    steps (the half step, pairs, a trailing single step) against the oracle."""
+    gpu_case(f"hall advance {dims} {across} per pass {per_pass}")
+    g, o = _pair(*dims, across)
+    g.set_steps_per_pass(per_pass)
+    g.released_in_mode(mode, 2.0)
+    o.release(o.mode(mode, 2.0))
+    dt = 0.9 * o.max_stable_dt()
+    totals, _ = g.advance(dt, 37, tolerance=1e-12)
+    for n in range(37):
+        o.step(dt, threads_=16)
+        t_ref, s_ref = o.ledger()
+        assert abs(totals[n] - t_ref) <= 1e-12 * s_ref, n
+    _same_fields(g, o)
+
+
+def test_a_breach_inside_a_pair_leaves_the_state_of_the_breaking_step():
+    gpu_case("hall breach inside a pair")
+    import pantometry_amd as pm
+    g, o = _pair(4.4, 3.1, 2.4, 31)
+    g.released_in_mode((2, 1, 1), 1.0)
+    p0 = o.mode((2, 1, 1), 1.0)
+    dt = 0.9 * g.max_stable_dt()
+    # a negative tolerance breaks on the first audit of the device loop: step 1, the first of a
+    # pair (step 0, the half step, runs alone)
+    g.checkpoint()
+    with pytest.raises(pm.Violation) as err:
+        g.advance(dt, 20, tolerance=-1.0)
+    assert err.value.step == 1
+    o.release(p0)
+    for _ in range(2):
+        o.step(dt, threads_=16)
+    _same_fields(g, o)
+
+    # A tolerance met by every earlier step and broken by a second step of a pair (step 2, 4,
+    # ...).  A hall released in a mode with a little seeded noise, whose ledger (its energy alone)
+    # moves by rounding every step, gives such a step.
+    g, o = _pair(4.4, 3.1, 2.4, 31)
+    nx, ny, nz = g.nodes()
+    p0 = o.mode((2, 1, 1), 1.0) + 1e-3 * np.random.default_rng(7).standard_normal(nx * ny * nz)
+    g.released_from(p0.reshape(nz, ny, nx))
+    dt = 0.9 * g.max_stable_dt()
+    g.checkpoint()
+    totals, _ = g.advance(dt, 80, tolerance=1.0)
+    prev = np.concatenate([[totals[0]], totals[:-1]])
+    scale = np.maximum(np.abs(totals), np.abs(prev))
+    drift = np.where(scale > 0, np.abs(totals - prev) / np.where(scale > 0, scale, 1.0), 0.0)
+    js = [j for j in range(2, 80, 2) if drift[j] > drift[1:j].max()]
+    assert js, drift
+    j = js[0]
+    tol = 0.5 * (drift[j] + drift[1:j].max())
+    g.restore()
+    with pytest.raises(pm.Violation) as err:
+        g.advance(dt, 80, tolerance=tol)
+    assert err.value.step == j
+    o.release(p0)
+    for _ in range(j + 1):
+        o.step(dt, threads_=16)
+    _same_fields(g, o)
+
+
 def test_a_sourced_hall_is_bit_exact_and_books_its_source():
     gpu_case("hall with source 41^3")
     import pantometry_amd as pm

@@ -71,70 +71,117 @@ __device__ __forceinline__ void box_coords(const Conv &g, int q, int &qd, int &q
     qd = q / (g.TW * g.TH);
 }
 
-template <int NB>
-__global__ __launch_bounds__(256) void conv_fwd_kernel(Conv g, const bf16 *__restrict__ x,
-                                                       const bf16 *__restrict__ w,
-                                                       const float *__restrict__ bias,
-                                                       bf16 *__restrict__ zout,
-                                                       bf16 *__restrict__ aout, int gelu_out,
-                                                       int ostride, int ooff) {
-    constexpr int NT = NB / 16;  // MFMA column tiles per wave
+// The forward kernel.  A workgroup of four waves computes a 256-position M-tile (a TD x TH x TW
+// box of one sample, smaller only when the volume is) against an N-block of 16*NT output
+// channels; each wave owns 64 positions as four 16-row MFMA blocks, so every B fragment (16
+// channels x 16 output channels of one tap, read from the weights) feeds four MFMAs and every A
+// fragment (16 positions x 16 channels, read from LDS at the tap's shift) feeds NT.  Per
+// 16-channel input chunk the box's neighbourhood is staged in LDS once, row by row; the 27 taps
+// are unrolled so each tap's offset in the staged box is a constant plus the rows' strides.  The
+// epilogue adds the bias, applies GELU, and writes bf16 through LDS so each position's channels
+// leave in 16-byte stores.
+template <int NT>
+__global__ __launch_bounds__(256) void conv_fwd2(Conv g, const bf16 *__restrict__ x,
+                                                 const bf16 *__restrict__ w,
+                                                 const float *__restrict__ bias,
+                                                 bf16 *__restrict__ zout,
+                                                 bf16 *__restrict__ aout, int gelu_out,
+                                                 int ostride, int ooff) {
+    constexpr int NB = 16 * NT;
     extern __shared__ __align__(16) bf16 smem[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t tile = blockIdx.x;
-    const int n0 = blockIdx.y * NB;
     int b, d0, h0, w0;
-    tile_origin(g, tile, b, d0, h0, w0);
-    const int SH = g.TH + 2, SW = g.TW + 2;
+    tile_origin(g, blockIdx.x, b, d0, h0, w0);
+    const int n0 = blockIdx.y * NB;
+    const int SH = g.TH + 2, SW = g.TW + 2, SHW = SH * SW;
     const int TP = g.tile_positions();
+    const int kq = (lane >> 4) * 4;
 
-    // This lane's A row: position p = wave*16 + lane%16 of the box; rows past the box are zero.
-    const int p = wave * 16 + (lane & 15);
-    const bool row_live = p < TP;
-    int pd, ph, pw;
-    box_coords(g, row_live ? p : 0, pd, ph, pw);
-    const int kq = (lane >> 4) * 4;  // the four k (channel) entries this lane carries
-
-    float4v acc[NT];
+    // This lane's four A rows: positions wave*64 + rb*16 + lane%16 of the box.
+    int base[4];
+    bool live[4];
 #pragma unroll
-    for (int n = 0; n < NT; n++) acc[n] = float4v{0.f, 0.f, 0.f, 0.f};
+    for (int rb = 0; rb < 4; rb++) {
+        const int p = wave * 64 + rb * 16 + (lane & 15);
+        live[rb] = p < TP;
+        int pd, ph, pw;
+        box_coords(g, live[rb] ? p : 0, pd, ph, pw);
+        base[rb] = ((pd * SH + ph) * SW + pw) * 16 + kq;
+    }
+
+    float4v acc[4][NT];
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+        for (int n = 0; n < NT; n++) acc[rb][n] = float4v{0.f, 0.f, 0.f, 0.f};
 
     const int64_t wrow = (int64_t)27 * g.Cin;
+    const bf16 *wl = w + (int64_t)(n0 + (lane & 15)) * wrow + kq;  // this lane's B column
+    const int rows = (g.TD + 2) * SH;
     for (int c0 = 0; c0 < g.Cin; c0 += 16) {
         __syncthreads();
-        stage_neighbourhood(g, x, g.Cin, c0, b, d0, h0, w0, smem, tid, 256);
+        // Stage channels [c0, c0+16) of the neighbourhood, one row of SW positions at a time.
+        for (int row = wave; row < rows; row += 4) {
+            const int pz = row / SH, py = row - pz * SH;
+            const int z = d0 + pz - 1, y = h0 + py - 1;
+            const bool zy = z >= 0 && z < g.D && y >= 0 && y < g.H;
+            const int64_t rbase = (((int64_t)b * g.D + z) * g.H + y) * g.W;
+            for (int e = lane; e < 2 * SW; e += 64) {
+                const int px = e >> 1, half = e & 1;
+                const int xx = w0 + px - 1;
+                uint4 v = make_uint4(0, 0, 0, 0);
+                if (zy && xx >= 0 && xx < g.W)
+                    v = *(const uint4 *)(x + (rbase + xx) * g.Cin + c0 + half * 8);
+                *(uint4 *)(smem + (row * SW + px) * 16 + half * 8) = v;
+            }
+        }
         __syncthreads();
-#pragma unroll 3
+#pragma unroll
         for (int t = 0; t < 27; t++) {
             const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;
-            const int sp = ((pd + dz) * SH + ph + dy) * SW + pw + dx;
-            short4v a = *(const short4v *)(smem + sp * 16 + kq);
-            if (!row_live) a = short4v{0, 0, 0, 0};
+            const int toff = (dz * SHW + dy * SW + dx) * 16;
+            short4v bv[NT];
 #pragma unroll
-            for (int n = 0; n < NT; n++) {
-                const int co = n0 + n * 16 + (lane & 15);
-                const short4v bv = *(const short4v *)(w + co * wrow + t * g.Cin + c0 + kq);
-                acc[n] = mfma16(a, bv, acc[n]);
+            for (int n = 0; n < NT; n++)
+                bv[n] = *(const short4v *)(wl + (int64_t)n * 16 * wrow + t * g.Cin + c0);
+#pragma unroll
+            for (int rb = 0; rb < 4; rb++) {
+                short4v a = *(const short4v *)(smem + base[rb] + toff);
+                if (!live[rb]) a = short4v{0, 0, 0, 0};
+#pragma unroll
+                for (int n = 0; n < NT; n++) acc[rb][n] = mfma16(a, bv[n], acc[rb][n]);
             }
         }
     }
 
-    // Epilogue: rows are positions 4*(lane/16) + r of this wave's 16, the column is one channel.
+    // Epilogue through LDS: [position][NB] bf16, then 16-byte stores per position.
+    float bb[NT];
 #pragma unroll
-    for (int n = 0; n < NT; n++) {
-        const int co = n0 + n * 16 + (lane & 15);
-        const float bb = bias ? bias[co] : 0.0f;
+    for (int n = 0; n < NT; n++) bb[n] = bias ? bias[n0 + n * 16 + (lane & 15)] : 0.0f;
+    bf16 *outs[2] = {zout, aout};
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int q = wave * 16 + (lane >> 4) * 4 + r;
-            if (q >= TP) continue;
+    for (int pass = 0; pass < 2; pass++) {
+        bf16 *out = outs[pass];
+        if (!out) continue;
+        __syncthreads();
+#pragma unroll
+        for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+            for (int n = 0; n < NT; n++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int q = wave * 64 + rb * 16 + (lane >> 4) * 4 + r;
+                    const float zz = acc[rb][n][r] + bb[n];
+                    const float v = pass == 1 && gelu_out ? gelu(zz) : zz;
+                    smem[q * NB + n * 16 + (lane & 15)] = to_bf16(v);
+                }
+        __syncthreads();
+        for (int e = tid; e < TP * (NB / 8); e += 256) {
+            const int q = e / (NB / 8), part = e % (NB / 8);
             int qd, qh, qw;
             box_coords(g, q, qd, qh, qw);
             const int64_t pos = (((int64_t)b * g.D + d0 + qd) * g.H + h0 + qh) * g.W + w0 + qw;
-            const float zz = acc[n][r] + bb;
-            const int64_t o = pos * ostride + ooff + co;
-            if (zout) zout[o] = to_bf16(zz);
-            if (aout) aout[o] = to_bf16(gelu_out ? gelu(zz) : zz);
+            *(uint4 *)(out + pos * ostride + ooff + n0 + part * 8) = *(const uint4 *)(smem + q * NB + part * 8);
         }
     }
 }
@@ -257,17 +304,40 @@ static bool conv_shape_ok(const Conv &g) {
     return ok;
 }
 
-int conv_forward(const Conv &g, const bf16 *x, const bf16 *w, const float *bias, bf16 *z, bf16 *a,
+// The forward kernel's M-tile: a box of up to 256 positions, 16 wide, 4 deep, as tall as fills
+// it (the whole sample when the volume is smaller).
+static Conv forward_box(const Conv &g) {
+    Conv f = g;
+    f.TW = g.W < 16 ? g.W : 16;
+    f.TD = g.D < 4 ? g.D : 4;
+    f.TH = g.H < 256 / (f.TW * f.TD) ? g.H : 256 / (f.TW * f.TD);
+    f.TD = g.D < 256 / (f.TW * f.TH) ? g.D : 256 / (f.TW * f.TH);
+    return f;
+}
+
+int conv_forward(const Conv &g0, const bf16 *x, const bf16 *w, const float *bias, bf16 *z, bf16 *a,
                  bool gelu_out, int ostride, int ooff, hipStream_t stream) {
-    if (!conv_shape_ok(g)) return PM_ERR_ARGUMENT;
-    const size_t lds = (size_t)(g.TD + 2) * (g.TH + 2) * (g.TW + 2) * 16 * sizeof(bf16);
-    const int nb = g.Cout % 64 == 0 ? 64 : g.Cout == 48 ? 48 : g.Cout % 32 == 0 ? 32 : 16;
-    dim3 grid((unsigned)g.tiles(), (unsigned)(g.Cout / nb));
-    switch (nb) {
-        case 16: conv_fwd_kernel<16><<<grid, 256, lds, stream>>>(g, x, w, bias, z, a, gelu_out, ostride, ooff); break;
-        case 32: conv_fwd_kernel<32><<<grid, 256, lds, stream>>>(g, x, w, bias, z, a, gelu_out, ostride, ooff); break;
-        case 48: conv_fwd_kernel<48><<<grid, 256, lds, stream>>>(g, x, w, bias, z, a, gelu_out, ostride, ooff); break;
-        default: conv_fwd_kernel<64><<<grid, 256, lds, stream>>>(g, x, w, bias, z, a, gelu_out, ostride, ooff); break;
+    const Conv g = forward_box(g0);
+    const bool ok = g.B > 0 && g.Cin > 0 && g.Cout > 0 && g.Cin % 16 == 0 && g.Cout % 16 == 0 &&
+                    g.D % g.TD == 0 && g.H % g.TH == 0 && g.W % g.TW == 0 &&
+                    g.tile_positions() <= 256 &&
+                    (g.tile_positions() == 256 || (g.TD == g.D && g.TH == g.H && g.TW == g.W)) &&
+                    ostride % 8 == 0 && ooff % 8 == 0;
+    if (!ok) {
+        set_error("conv_forward: shape %dx%dx%dx%d, %d->%d channels unsupported", g.B, g.D, g.H,
+                  g.W, g.Cin, g.Cout);
+        return PM_ERR_ARGUMENT;
+    }
+    const int nt = g.Cout % 64 == 0 ? 4 : g.Cout == 48 ? 3 : g.Cout % 32 == 0 ? 2 : 1;
+    const size_t stage = (size_t)(g.TD + 2) * (g.TH + 2) * (g.TW + 2) * 16 * sizeof(bf16);
+    const size_t epi = (size_t)256 * 16 * nt * sizeof(bf16);
+    const size_t lds = std::max(stage, epi);
+    dim3 grid((unsigned)g.tiles(), (unsigned)(g.Cout / (16 * nt)));
+    switch (nt) {
+        case 1: conv_fwd2<1><<<grid, 256, lds, stream>>>(g, x, w, bias, z, a, gelu_out, ostride, ooff); break;
+        case 2: conv_fwd2<2><<<grid, 256, lds, stream>>>(g, x, w, bias, z, a, gelu_out, ostride, ooff); break;
+        case 3: conv_fwd2<3><<<grid, 256, lds, stream>>>(g, x, w, bias, z, a, gelu_out, ostride, ooff); break;
+        default: conv_fwd2<4><<<grid, 256, lds, stream>>>(g, x, w, bias, z, a, gelu_out, ostride, ooff); break;
     }
     PM_HIP(hipGetLastError());
     return PM_OK;

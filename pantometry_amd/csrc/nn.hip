@@ -71,16 +71,16 @@ __device__ __forceinline__ void box_coords(const Conv &g, int q, int &qd, int &q
     qd = q / (g.TW * g.TH);
 }
 
-// The forward kernel.  A workgroup of four waves computes a 256-position M-tile (a TD x TH x TW
+// The forward kernel.  A workgroup of four waves computes a 64*RB-position M-tile (a TD x TH x TW
 // box of one sample, smaller only when the volume is) against an N-block of 16*NT output
-// channels; each wave owns 64 positions as four 16-row MFMA blocks, so every B fragment (16
-// channels x 16 output channels of one tap, read from the weights) feeds four MFMAs and every A
+// channels; each wave owns 16*RB positions as RB 16-row MFMA blocks, so every B fragment (16
+// channels x 16 output channels of one tap, read from the weights) feeds RB MFMAs and every A
 // fragment (16 positions x 16 channels, read from LDS at the tap's shift) feeds NT.  Per
 // 16-channel input chunk the box's neighbourhood is staged in LDS once, row by row; the 27 taps
 // are unrolled so each tap's offset in the staged box is a constant plus the rows' strides.  The
 // epilogue adds the bias, applies GELU, and writes bf16 through LDS so each position's channels
 // leave in 16-byte stores.
-template <int NT>
+template <int NT, int RB>
 __global__ __launch_bounds__(256) void conv_fwd2(Conv g, const bf16 *__restrict__ x,
                                                  const bf16 *__restrict__ w,
                                                  const float *__restrict__ bias,
@@ -97,21 +97,21 @@ __global__ __launch_bounds__(256) void conv_fwd2(Conv g, const bf16 *__restrict_
     const int TP = g.tile_positions();
     const int kq = (lane >> 4) * 4;
 
-    // This lane's four A rows: positions wave*64 + rb*16 + lane%16 of the box.
-    int base[4];
-    bool live[4];
+    // This lane's RB A rows: positions wave*16*RB + rb*16 + lane%16 of the box.
+    int base[RB];
+    bool live[RB];
 #pragma unroll
-    for (int rb = 0; rb < 4; rb++) {
-        const int p = wave * 64 + rb * 16 + (lane & 15);
+    for (int rb = 0; rb < RB; rb++) {
+        const int p = wave * 16 * RB + rb * 16 + (lane & 15);
         live[rb] = p < TP;
         int pd, ph, pw;
         box_coords(g, live[rb] ? p : 0, pd, ph, pw);
         base[rb] = ((pd * SH + ph) * SW + pw) * 16 + kq;
     }
 
-    float4v acc[4][NT];
+    float4v acc[RB][NT];
 #pragma unroll
-    for (int rb = 0; rb < 4; rb++)
+    for (int rb = 0; rb < RB; rb++)
 #pragma unroll
         for (int n = 0; n < NT; n++) acc[rb][n] = float4v{0.f, 0.f, 0.f, 0.f};
 
@@ -145,7 +145,7 @@ __global__ __launch_bounds__(256) void conv_fwd2(Conv g, const bf16 *__restrict_
             for (int n = 0; n < NT; n++)
                 bv[n] = *(const short4v *)(wl + (int64_t)n * 16 * wrow + t * g.Cin + c0);
 #pragma unroll
-            for (int rb = 0; rb < 4; rb++) {
+            for (int rb = 0; rb < RB; rb++) {
                 short4v a = *(const short4v *)(smem + base[rb] + toff);
                 if (!live[rb]) a = short4v{0, 0, 0, 0};
 #pragma unroll
@@ -165,12 +165,12 @@ __global__ __launch_bounds__(256) void conv_fwd2(Conv g, const bf16 *__restrict_
         if (!out) continue;
         __syncthreads();
 #pragma unroll
-        for (int rb = 0; rb < 4; rb++)
+        for (int rb = 0; rb < RB; rb++)
 #pragma unroll
             for (int n = 0; n < NT; n++)
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    const int q = wave * 64 + rb * 16 + (lane >> 4) * 4 + r;
+                    const int q = wave * 16 * RB + rb * 16 + (lane >> 4) * 4 + r;
                     const float zz = acc[rb][n][r] + bb[n];
                     const float v = pass == 1 && gelu_out ? gelu(zz) : zz;
                     smem[q * NB + n * 16 + (lane & 15)] = to_bf16(v);
@@ -304,40 +304,69 @@ static bool conv_shape_ok(const Conv &g) {
     return ok;
 }
 
-// The forward kernel's M-tile: a box of up to 256 positions, 16 wide, 4 deep, as tall as fills
-// it (the whole sample when the volume is smaller).
-static Conv forward_box(const Conv &g) {
+// The forward kernel's M-tile: a box of up to m positions (256 or 64), at most 16 wide and 4
+// deep, as tall as fills it (the whole sample when the volume is smaller).
+static Conv forward_box(const Conv &g, int m) {
     Conv f = g;
     f.TW = g.W < 16 ? g.W : 16;
     f.TD = g.D < 4 ? g.D : 4;
-    f.TH = g.H < 256 / (f.TW * f.TD) ? g.H : 256 / (f.TW * f.TD);
-    f.TD = g.D < 256 / (f.TW * f.TH) ? g.D : 256 / (f.TW * f.TH);
+    if (f.TW * f.TD > m) f.TD = m / f.TW;
+    f.TH = g.H < m / (f.TW * f.TD) ? g.H : m / (f.TW * f.TD);
+    f.TD = g.D < m / (f.TW * f.TH) ? g.D : m / (f.TW * f.TH);
     return f;
+}
+
+template <int NT, int RB>
+static void launch_fwd(const Conv &g, dim3 grid, size_t lds, hipStream_t stream, const bf16 *x,
+                       const bf16 *w, const float *bias, bf16 *z, bf16 *a, bool gelu_out,
+                       int ostride, int ooff) {
+    conv_fwd2<NT, RB><<<grid, 256, lds, stream>>>(g, x, w, bias, z, a, gelu_out, ostride, ooff);
 }
 
 int conv_forward(const Conv &g0, const bf16 *x, const bf16 *w, const float *bias, bf16 *z, bf16 *a,
                  bool gelu_out, int ostride, int ooff, hipStream_t stream) {
-    const Conv g = forward_box(g0);
+    // Most reuse first (256-position boxes, the widest N-block); fall back to narrower N-blocks
+    // and 64-position boxes while the launch would have fewer than 1024 workgroups.
+    const int nt_max = g0.Cout % 64 == 0 ? 4 : g0.Cout == 48 ? 3 : g0.Cout % 32 == 0 ? 2 : 1;
+    const int opts[4][2] = {{4, nt_max}, {4, 1}, {1, nt_max}, {1, 1}};
+    int rb = 4, nt = nt_max;
+    int64_t best = -1;
+    for (const auto &o : opts) {
+        const Conv f = forward_box(g0, 64 * o[0]);
+        if (f.TW <= 0 || f.TH <= 0 || f.TD <= 0) continue;
+        const int64_t wgs = f.tiles() * (g0.Cout / (16 * o[1]));
+        if (wgs > best) best = wgs, rb = o[0], nt = o[1];
+        if (wgs >= 1024) {
+            rb = o[0], nt = o[1];
+            break;
+        }
+    }
+    const int m = 64 * rb;
+    const Conv g = forward_box(g0, m);
     const bool ok = g.B > 0 && g.Cin > 0 && g.Cout > 0 && g.Cin % 16 == 0 && g.Cout % 16 == 0 &&
-                    g.D % g.TD == 0 && g.H % g.TH == 0 && g.W % g.TW == 0 &&
-                    g.tile_positions() <= 256 &&
-                    (g.tile_positions() == 256 || (g.TD == g.D && g.TH == g.H && g.TW == g.W)) &&
+                    g.TW > 0 && g.TH > 0 && g.TD > 0 && g.D % g.TD == 0 && g.H % g.TH == 0 &&
+                    g.W % g.TW == 0 && g.tile_positions() <= m &&
+                    (g.tile_positions() == m || (g.TD == g.D && g.TH == g.H && g.TW == g.W)) &&
                     ostride % 8 == 0 && ooff % 8 == 0;
     if (!ok) {
         set_error("conv_forward: shape %dx%dx%dx%d, %d->%d channels unsupported", g.B, g.D, g.H,
                   g.W, g.Cin, g.Cout);
         return PM_ERR_ARGUMENT;
     }
-    const int nt = g.Cout % 64 == 0 ? 4 : g.Cout == 48 ? 3 : g.Cout % 32 == 0 ? 2 : 1;
     const size_t stage = (size_t)(g.TD + 2) * (g.TH + 2) * (g.TW + 2) * 16 * sizeof(bf16);
-    const size_t epi = (size_t)256 * 16 * nt * sizeof(bf16);
+    const size_t epi = (size_t)m * 16 * nt * sizeof(bf16);
     const size_t lds = std::max(stage, epi);
     dim3 grid((unsigned)g.tiles(), (unsigned)(g.Cout / (16 * nt)));
-    switch (nt) {
-        case 1: conv_fwd2<1><<<grid, 256, lds, stream>>>(g, x, w, bias, z, a, gelu_out, ostride, ooff); break;
-        case 2: conv_fwd2<2><<<grid, 256, lds, stream>>>(g, x, w, bias, z, a, gelu_out, ostride, ooff); break;
-        case 3: conv_fwd2<3><<<grid, 256, lds, stream>>>(g, x, w, bias, z, a, gelu_out, ostride, ooff); break;
-        default: conv_fwd2<4><<<grid, 256, lds, stream>>>(g, x, w, bias, z, a, gelu_out, ostride, ooff); break;
+    const int key = nt * 10 + rb;
+    switch (key) {
+        case 14: launch_fwd<1, 4>(g, grid, lds, stream, x, w, bias, z, a, gelu_out, ostride, ooff); break;
+        case 24: launch_fwd<2, 4>(g, grid, lds, stream, x, w, bias, z, a, gelu_out, ostride, ooff); break;
+        case 34: launch_fwd<3, 4>(g, grid, lds, stream, x, w, bias, z, a, gelu_out, ostride, ooff); break;
+        case 44: launch_fwd<4, 4>(g, grid, lds, stream, x, w, bias, z, a, gelu_out, ostride, ooff); break;
+        case 11: launch_fwd<1, 1>(g, grid, lds, stream, x, w, bias, z, a, gelu_out, ostride, ooff); break;
+        case 21: launch_fwd<2, 1>(g, grid, lds, stream, x, w, bias, z, a, gelu_out, ostride, ooff); break;
+        case 31: launch_fwd<3, 1>(g, grid, lds, stream, x, w, bias, z, a, gelu_out, ostride, ooff); break;
+        default: launch_fwd<4, 1>(g, grid, lds, stream, x, w, bias, z, a, gelu_out, ostride, ooff); break;
     }
     PM_HIP(hipGetLastError());
     return PM_OK;

@@ -110,34 +110,57 @@ __global__ void gather_target(const float *__restrict__ set, const int *__restri
     }
 }
 
-// 2x2x2 average pool, channels-last.
+// Eight bf16 channels as one 16-byte word: element-wise kernels move channel groups of eight
+// (every channel count and offset here is a multiple of 16).
+__device__ __forceinline__ void unpack8(uint4 u, float f[8]) {
+    const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        f[2 * q] = from_bf16((bf16)(w[q] & 0xFFFFu));
+        f[2 * q + 1] = from_bf16((bf16)(w[q] >> 16));
+    }
+}
+__device__ __forceinline__ uint4 pack8(const float f[8]) {
+    uint32_t w[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) w[q] = (uint32_t)to_bf16(f[2 * q]) | ((uint32_t)to_bf16(f[2 * q + 1]) << 16);
+    return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// 2x2x2 average pool, channels-last: (sum of the eight fine values) * 0.125, rounded once.
 __global__ void pool_fwd(const bf16 *__restrict__ x, bf16 *__restrict__ y, int B, int D, int H,
                          int W, int C) {
-    const int d2 = D / 2, h2 = H / 2, w2 = W / 2;
-    GRID_STRIDE(i, (int64_t)B * d2 * h2 * w2 * C) {
-        const int c = (int)(i % C);
-        int64_t p = i / C;
+    const int d2 = D / 2, h2 = H / 2, w2 = W / 2, G = C / 8;
+    GRID_STRIDE(i, (int64_t)B * d2 * h2 * w2 * G) {
+        const int g = (int)(i % G);
+        int64_t p = i / G;
         const int w = (int)(p % w2);
         p /= w2;
         const int h = (int)(p % h2);
         p /= h2;
         const int d = (int)(p % d2);
         const int b = (int)(p / d2);
-        float s = 0.0f;
+        float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         for (int k = 0; k < 8; k++) {
             const int64_t q = (((int64_t)b * D + 2 * d + (k >> 2)) * H + 2 * h + ((k >> 1) & 1)) * W + 2 * w + (k & 1);
-            s += from_bf16(x[q * C + c]);
+            float f[8];
+            unpack8(*(const uint4 *)(x + q * C + g * 8), f);
+#pragma unroll
+            for (int c = 0; c < 8; c++) s[c] += f[c];
         }
-        y[i] = to_bf16(s * 0.125f);
+#pragma unroll
+        for (int c = 0; c < 8; c++) s[c] *= 0.125f;
+        *(uint4 *)(y + (i / G) * C + g * 8) = pack8(s);
     }
 }
 
 // Gradient of the pool, added to dx (the skip path has already written its part there).
 __global__ void pool_bwd_add(const bf16 *__restrict__ dy, bf16 *__restrict__ dx, int B, int D,
                              int H, int W, int C) {
-    GRID_STRIDE(i, (int64_t)B * D * H * W * C) {
-        const int c = (int)(i % C);
-        int64_t p = i / C;
+    const int G = C / 8;
+    GRID_STRIDE(i, (int64_t)B * D * H * W * G) {
+        const int g = (int)(i % G);
+        int64_t p = i / G;
         const int w = (int)(p % W);
         p /= W;
         const int h = (int)(p % H);
@@ -145,7 +168,13 @@ __global__ void pool_bwd_add(const bf16 *__restrict__ dy, bf16 *__restrict__ dx,
         const int d = (int)(p % D);
         const int b = (int)(p / D);
         const int64_t q = (((int64_t)b * (D / 2) + d / 2) * (H / 2) + h / 2) * (W / 2) + w / 2;
-        dx[i] = to_bf16(from_bf16(dx[i]) + 0.125f * from_bf16(dy[q * C + c]));
+        const int64_t o = (i / G) * C + g * 8;
+        float a[8], c[8];
+        unpack8(*(const uint4 *)(dx + o), a);
+        unpack8(*(const uint4 *)(dy + q * C + g * 8), c);
+#pragma unroll
+        for (int k = 0; k < 8; k++) a[k] = a[k] + 0.125f * c[k];
+        *(uint4 *)(dx + o) = pack8(a);
     }
 }
 
@@ -153,9 +182,10 @@ __global__ void pool_bwd_add(const bf16 *__restrict__ dy, bf16 *__restrict__ dx,
 __global__ void up_fwd(const bf16 *__restrict__ x, bf16 *__restrict__ y, int B, int D, int H,
                        int W, int C, int Ctot) {
     // x is [B][D][H][W][C]; y is [B][2D][2H][2W][Ctot]
-    GRID_STRIDE(i, (int64_t)B * 8 * D * H * W * C) {
-        const int c = (int)(i % C);
-        int64_t p = i / C;
+    const int G = C / 8;
+    GRID_STRIDE(i, (int64_t)B * 8 * D * H * W * G) {
+        const int g = (int)(i % G);
+        int64_t p = i / G;
         const int w = (int)(p % (2 * W));
         p /= 2 * W;
         const int h = (int)(p % (2 * H));
@@ -163,29 +193,32 @@ __global__ void up_fwd(const bf16 *__restrict__ x, bf16 *__restrict__ y, int B, 
         const int d = (int)(p % (2 * D));
         const int b = (int)(p / (2 * D));
         const int64_t src = (((int64_t)b * D + d / 2) * H + h / 2) * W + w / 2;
-        const int64_t dst = (((int64_t)b * 2 * D + d) * 2 * H + h) * 2 * W + w;
-        y[dst * Ctot + c] = x[src * C + c];
+        *(uint4 *)(y + (i / G) * Ctot + g * 8) = *(const uint4 *)(x + src * C + g * 8);
     }
 }
 
-// Gradient of the upsampling: dx[p][c] = sum of the eight fine cells' dy[.][c].
+// Gradient of the upsampling: dx[p][c] = sum of the eight fine cells' dy[.][c], rounded once.
 __global__ void up_bwd(const bf16 *__restrict__ dy, bf16 *__restrict__ dx, int B, int D, int H,
                        int W, int C, int Ctot) {
-    GRID_STRIDE(i, (int64_t)B * D * H * W * C) {
-        const int c = (int)(i % C);
-        int64_t p = i / C;
+    const int G = C / 8;
+    GRID_STRIDE(i, (int64_t)B * D * H * W * G) {
+        const int g = (int)(i % G);
+        int64_t p = i / G;
         const int w = (int)(p % W);
         p /= W;
         const int h = (int)(p % H);
         p /= H;
         const int d = (int)(p % D);
         const int b = (int)(p / D);
-        float s = 0.0f;
+        float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         for (int k = 0; k < 8; k++) {
             const int64_t q = (((int64_t)b * 2 * D + 2 * d + (k >> 2)) * 2 * H + 2 * h + ((k >> 1) & 1)) * 2 * W + 2 * w + (k & 1);
-            s += from_bf16(dy[q * Ctot + c]);
+            float f[8];
+            unpack8(*(const uint4 *)(dy + q * Ctot + g * 8), f);
+#pragma unroll
+            for (int c = 0; c < 8; c++) s[c] += f[c];
         }
-        dx[i] = to_bf16(s);
+        *(uint4 *)(dx + (i / G) * C + g * 8) = pack8(s);
     }
 }
 
@@ -193,18 +226,35 @@ __global__ void up_bwd(const bf16 *__restrict__ dy, bf16 *__restrict__ dx, int B
 // those channels of y into x (the skip's share of a concatenation gradient).
 __global__ void channel_copy(const bf16 *__restrict__ src, bf16 *__restrict__ dst, int64_t npos,
                              int C, int sstride, int soff, int dstride, int doff, int add) {
-    GRID_STRIDE(i, npos * C) {
-        const int c = (int)(i % C);
-        const int64_t p = i / C;
-        const bf16 v = src[p * sstride + soff + c];
-        bf16 &o = dst[p * dstride + doff + c];
-        o = add ? to_bf16(from_bf16(o) + from_bf16(v)) : v;
+    const int G = C / 8;
+    GRID_STRIDE(i, npos * G) {
+        const int g = (int)(i % G);
+        const int64_t p = i / G;
+        const uint4 v = *(const uint4 *)(src + p * sstride + soff + g * 8);
+        uint4 *o = (uint4 *)(dst + p * dstride + doff + g * 8);
+        if (add) {
+            float a[8], b[8];
+            unpack8(*o, a);
+            unpack8(v, b);
+#pragma unroll
+            for (int k = 0; k < 8; k++) a[k] = a[k] + b[k];
+            *o = pack8(a);
+        } else {
+            *o = v;
+        }
     }
 }
 
-// dz = da * gelu'(z), in place into da.
+// dz = da * gelu'(z), in place into da, eight values per thread.
 __global__ void gelu_bwd(const bf16 *__restrict__ z, bf16 *__restrict__ da, int64_t n) {
-    GRID_STRIDE(i, n) { da[i] = to_bf16(from_bf16(da[i]) * gelu_grad(from_bf16(z[i]))); }
+    GRID_STRIDE(i, n / 8) {
+        float a[8], zz[8];
+        unpack8(*(const uint4 *)(da + i * 8), a);
+        unpack8(*(const uint4 *)(z + i * 8), zz);
+#pragma unroll
+        for (int k = 0; k < 8; k++) a[k] = a[k] * gelu_grad(zz[k]);
+        *(uint4 *)(da + i * 8) = pack8(a);
+    }
 }
 
 // The 1x1 head: y[p] = sum_c hw[c] a[p][c] + hb.
@@ -532,26 +582,26 @@ int run_forward(pm_surrogate *s, bool training) {
     const int B = s->batch;
     PM_CHECK(conv(0, s->xin.as<bf16>(), s->act[0]));
     PM_CHECK(conv(1, s->act[0].a, s->act[1]));
-    nn::pool_fwd<<<grid_of(npos(s, 1) * 16), 256, 0, st>>>(s->act[1].a, s->pool[0], B, R(0), R(0), R(0), 16);
+    nn::pool_fwd<<<grid_of(npos(s, 1) * 16 / 8), 256, 0, st>>>(s->act[1].a, s->pool[0], B, R(0), R(0), R(0), 16);
     PM_CHECK(conv(2, s->pool[0], s->act[2]));
     PM_CHECK(conv(3, s->act[2].a, s->act[3]));
-    nn::pool_fwd<<<grid_of(npos(s, 2) * 32), 256, 0, st>>>(s->act[3].a, s->pool[1], B, R(1), R(1), R(1), 32);
+    nn::pool_fwd<<<grid_of(npos(s, 2) * 32 / 8), 256, 0, st>>>(s->act[3].a, s->pool[1], B, R(1), R(1), R(1), 32);
     PM_CHECK(conv(4, s->pool[1], s->act[4]));
     PM_CHECK(conv(5, s->act[4].a, s->act[5]));
-    nn::pool_fwd<<<grid_of(npos(s, 3) * 64), 256, 0, st>>>(s->act[5].a, s->pool[2], B, R(2), R(2), R(2), 64);
+    nn::pool_fwd<<<grid_of(npos(s, 3) * 64 / 8), 256, 0, st>>>(s->act[5].a, s->pool[2], B, R(2), R(2), R(2), 64);
     PM_CHECK(conv(6, s->pool[2], s->act[6]));
     PM_CHECK(conv(7, s->act[6].a, s->act[7]));
     // decoder level 2: up(enc4) ++ enc3
-    nn::up_fwd<<<grid_of(npos(s, 2) * 128), 256, 0, st>>>(s->act[7].a, s->cat[2], B, R(3), R(3), R(3), 128, 192);
-    nn::channel_copy<<<grid_of(npos(s, 2) * 64), 256, 0, st>>>(s->act[5].a, s->cat[2], npos(s, 2), 64, 64, 0, 192, 128, 0);
+    nn::up_fwd<<<grid_of(npos(s, 2) * 128 / 8), 256, 0, st>>>(s->act[7].a, s->cat[2], B, R(3), R(3), R(3), 128, 192);
+    nn::channel_copy<<<grid_of(npos(s, 2) * 64 / 8), 256, 0, st>>>(s->act[5].a, s->cat[2], npos(s, 2), 64, 64, 0, 192, 128, 0);
     PM_CHECK(conv(8, s->cat[2], s->act[8]));
     PM_CHECK(conv(9, s->act[8].a, s->act[9]));
-    nn::up_fwd<<<grid_of(npos(s, 1) * 64), 256, 0, st>>>(s->act[9].a, s->cat[1], B, R(2), R(2), R(2), 64, 96);
-    nn::channel_copy<<<grid_of(npos(s, 1) * 32), 256, 0, st>>>(s->act[3].a, s->cat[1], npos(s, 1), 32, 32, 0, 96, 64, 0);
+    nn::up_fwd<<<grid_of(npos(s, 1) * 64 / 8), 256, 0, st>>>(s->act[9].a, s->cat[1], B, R(2), R(2), R(2), 64, 96);
+    nn::channel_copy<<<grid_of(npos(s, 1) * 32 / 8), 256, 0, st>>>(s->act[3].a, s->cat[1], npos(s, 1), 32, 32, 0, 96, 64, 0);
     PM_CHECK(conv(10, s->cat[1], s->act[10]));
     PM_CHECK(conv(11, s->act[10].a, s->act[11]));
-    nn::up_fwd<<<grid_of(npos(s, 0) * 32), 256, 0, st>>>(s->act[11].a, s->cat[0], B, R(1), R(1), R(1), 32, 48);
-    nn::channel_copy<<<grid_of(npos(s, 0) * 16), 256, 0, st>>>(s->act[1].a, s->cat[0], npos(s, 0), 16, 16, 0, 48, 32, 0);
+    nn::up_fwd<<<grid_of(npos(s, 0) * 32 / 8), 256, 0, st>>>(s->act[11].a, s->cat[0], B, R(1), R(1), R(1), 32, 48);
+    nn::channel_copy<<<grid_of(npos(s, 0) * 16 / 8), 256, 0, st>>>(s->act[1].a, s->cat[0], npos(s, 0), 16, 16, 0, 48, 32, 0);
     PM_CHECK(conv(12, s->cat[0], s->act[12]));
     PM_CHECK(conv(13, s->act[12].a, s->act[13]));
     nn::head_fwd<<<grid_of(npos(s, 0)), 256, 0, st>>>(s->act[13].a, P + s->head_off, P + s->head_off + 16, npos(s, 0), s->y.as<float>());
@@ -566,7 +616,7 @@ int conv_back(pm_surrogate *s, int li, const bf16 *in, bf16 *din) {
     Act &o = s->act[li];
     const nn::Conv g = conv_of(s, li);
     const int64_t n = g.positions() * g.Cout;
-    nn::gelu_bwd<<<grid_of(n), 256, 0, st>>>(o.z, o.da, n);
+    nn::gelu_bwd<<<grid_of(n / 8), 256, 0, st>>>(o.z, o.da, n);
     PM_HIP(hipGetLastError());
     float *G = s->grads.as<float>();
     const int64_t tiles = g.tiles();
@@ -597,30 +647,30 @@ int run_backward(pm_surrogate *s) {
     // decoder level 0
     PM_CHECK(conv_back(s, 13, s->act[12].a, s->act[12].da));
     PM_CHECK(conv_back(s, 12, s->cat[0], s->dcat[0]));
-    nn::up_bwd<<<grid_of(npos(s, 1) * 32), 256, 0, st>>>(s->dcat[0], s->act[11].da, B, R(1), R(1), R(1), 32, 48);
-    nn::channel_copy<<<grid_of(n0 * 16), 256, 0, st>>>(s->dcat[0], s->act[1].da, n0, 16, 48, 32, 16, 0, 0);
+    nn::up_bwd<<<grid_of(npos(s, 1) * 32 / 8), 256, 0, st>>>(s->dcat[0], s->act[11].da, B, R(1), R(1), R(1), 32, 48);
+    nn::channel_copy<<<grid_of(n0 * 16 / 8), 256, 0, st>>>(s->dcat[0], s->act[1].da, n0, 16, 48, 32, 16, 0, 0);
     // decoder level 1
     PM_CHECK(conv_back(s, 11, s->act[10].a, s->act[10].da));
     PM_CHECK(conv_back(s, 10, s->cat[1], s->dcat[1]));
-    nn::up_bwd<<<grid_of(npos(s, 2) * 64), 256, 0, st>>>(s->dcat[1], s->act[9].da, B, R(2), R(2), R(2), 64, 96);
-    nn::channel_copy<<<grid_of(npos(s, 1) * 32), 256, 0, st>>>(s->dcat[1], s->act[3].da, npos(s, 1), 32, 96, 64, 32, 0, 0);
+    nn::up_bwd<<<grid_of(npos(s, 2) * 64 / 8), 256, 0, st>>>(s->dcat[1], s->act[9].da, B, R(2), R(2), R(2), 64, 96);
+    nn::channel_copy<<<grid_of(npos(s, 1) * 32 / 8), 256, 0, st>>>(s->dcat[1], s->act[3].da, npos(s, 1), 32, 96, 64, 32, 0, 0);
     // decoder level 2
     PM_CHECK(conv_back(s, 9, s->act[8].a, s->act[8].da));
     PM_CHECK(conv_back(s, 8, s->cat[2], s->dcat[2]));
-    nn::up_bwd<<<grid_of(npos(s, 3) * 128), 256, 0, st>>>(s->dcat[2], s->act[7].da, B, R(3), R(3), R(3), 128, 192);
-    nn::channel_copy<<<grid_of(npos(s, 2) * 64), 256, 0, st>>>(s->dcat[2], s->act[5].da, npos(s, 2), 64, 192, 128, 64, 0, 0);
+    nn::up_bwd<<<grid_of(npos(s, 3) * 128 / 8), 256, 0, st>>>(s->dcat[2], s->act[7].da, B, R(3), R(3), R(3), 128, 192);
+    nn::channel_copy<<<grid_of(npos(s, 2) * 64 / 8), 256, 0, st>>>(s->dcat[2], s->act[5].da, npos(s, 2), 64, 192, 128, 64, 0, 0);
     // encoder level 3
     PM_CHECK(conv_back(s, 7, s->act[6].a, s->act[6].da));
     PM_CHECK(conv_back(s, 6, s->pool[2], s->dpool[2]));
-    nn::pool_bwd_add<<<grid_of(npos(s, 2) * 64), 256, 0, st>>>(s->dpool[2], s->act[5].da, B, R(2), R(2), R(2), 64);
+    nn::pool_bwd_add<<<grid_of(npos(s, 2) * 64 / 8), 256, 0, st>>>(s->dpool[2], s->act[5].da, B, R(2), R(2), R(2), 64);
     // encoder level 2
     PM_CHECK(conv_back(s, 5, s->act[4].a, s->act[4].da));
     PM_CHECK(conv_back(s, 4, s->pool[1], s->dpool[1]));
-    nn::pool_bwd_add<<<grid_of(npos(s, 1) * 32), 256, 0, st>>>(s->dpool[1], s->act[3].da, B, R(1), R(1), R(1), 32);
+    nn::pool_bwd_add<<<grid_of(npos(s, 1) * 32 / 8), 256, 0, st>>>(s->dpool[1], s->act[3].da, B, R(1), R(1), R(1), 32);
     // encoder level 1
     PM_CHECK(conv_back(s, 3, s->act[2].a, s->act[2].da));
     PM_CHECK(conv_back(s, 2, s->pool[0], s->dpool[0]));
-    nn::pool_bwd_add<<<grid_of(n0 * 16), 256, 0, st>>>(s->dpool[0], s->act[1].da, B, R(0), R(0), R(0), 16);
+    nn::pool_bwd_add<<<grid_of(n0 * 16 / 8), 256, 0, st>>>(s->dpool[0], s->act[1].da, B, R(0), R(0), R(0), 16);
     // encoder level 0
     PM_CHECK(conv_back(s, 1, s->act[0].a, s->act[0].da));
     PM_CHECK(conv_back(s, 0, s->xin.as<bf16>(), nullptr));

@@ -271,6 +271,10 @@ int pm_ensemble_ledgers(pm_ensemble *ens, double *stored);
  * fit in LDS) run resident, the rest streamed in z-chunks of `z_chunk` planes.  No reference
  * counterpart. */
 int pm_ensemble_set_schedule(pm_ensemble *ens, int resident_edge, int z_chunk);
+/* Steps per launch of the streamed members: 2 (default; temporal blocking, each launch advances
+ * two steps with the same arithmetic) or 1.  Results are identical either way.  No reference
+ * counterpart. */
+int pm_ensemble_set_steps_per_pass(pm_ensemble *ens, int steps);
 /* Time the candidate schedules on this ensemble (a few steps each, state restored afterwards)
  * and keep the fastest.  `report` (optional, `report_len` bytes) receives one line per
  * candidate. */

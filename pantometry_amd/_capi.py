@@ -104,6 +104,7 @@ SIGNATURES = {
     "pm_ensemble_run": (_I, [_VP, _U64, _DP, _VIOL]),
     "pm_ensemble_ledgers": (_I, [_VP, _DP]),
     "pm_ensemble_set_schedule": (_I, [_VP, _I, _I]),
+    "pm_ensemble_set_steps_per_pass": (_I, [_VP, _I]),
     "pm_ensemble_tune": (_I, [_VP, C.c_char_p, _S]),
     "pm_solid3d_copy_cells": (_I, [_VP, _VP]),
     "pm_surrogate_create": (_I, [_U64, C.POINTER(_VP)]),

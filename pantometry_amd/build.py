@@ -35,8 +35,8 @@ def _compile(src: str) -> Path:
     obj = BUILD / (src.replace(".", "_") + ".o")
     cmd = [HIPCC, *FLAGS, "-c", str(CSRC / src), "-o", str(obj)]
     if src.endswith(".cpp"):
-        cmd = [HIPCC, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
-               "-c", str(CSRC / src), "-o", str(obj)]
+        cmd = [HIPCC, "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+               "-Wall", "-c", str(CSRC / src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"compiling {src} failed:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")

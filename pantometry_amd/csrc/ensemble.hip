@@ -8,9 +8,11 @@
 //   resident   a member small enough for both of its temperature copies to fit in one
 //              workgroup's LDS is stepped entirely in LDS: one workgroup per member runs every
 //              step of the run in a single launch and touches HBM only to load and store.
-//   streamed   every other member is cut into work items (256 xy-columns by a z-chunk); one
-//              launch per step covers every item of every streamed member, and the launches of
-//              a run are captured once into a hipGraph and replayed.
+//   streamed   every other member is cut into tiles (whole rows of the xy plane, or 64 x 8
+//              for rows over 128 cells) by a z-chunk; one launch covers every tile of every
+//              streamed member and advances it two steps (temporal blocking in LDS: T crosses
+//              HBM once per two steps), and the launches of a run are captured once into a
+//              hipGraph and replayed.
 //
 // The two schedules run on two streams joined inside the captured graph.  Which members are
 // resident and how deep the z-chunks are is chosen by pm_ensemble_tune, which times the
@@ -33,18 +35,29 @@ namespace pm {
 
 struct Member {
     int64_t off;          // first cell in the concatenated arrays
-    int nx, ny, nz, pad;
+    int nx, ny, nz;
+    int tx, ty;           // streamed tile: tx columns by ty rows
+    int pad;
     double kface;         // series(k, k)
     double f;             // dt * dx / C
     double cap;           // C
     double tref;          // ledger reference
 };
 
-struct Item {
+// A streamed work item: one tile of one member over the planes [z0, z1).  member < 0 pads the
+// grid so that every XCD gets the same number of blocks.
+struct Tile {
     int member;
-    int col0;  // first xy column (i + nx*j)
+    int x0, y0;
     int z0, z1;
+    int pad;
 };
+
+constexpr int ENS_NT = 512;    // threads per streamed workgroup; a tile holds at most this many
+constexpr int ENS_W0 = 1024;   // most cells of a staged plane (tile + two-cell halo)
+constexpr int ENS_W1 = 768;    // most cells of a first-step plane (tile + one-cell halo)
+constexpr int ENS_XCDS = 8;
+constexpr int ENS_D = 3;       // planes prefetched ahead in registers
 
 // One cell's update in the uniform form; neighbours off the block carry a zero face.
 __device__ __forceinline__ double uniform_cell(double t, double tw, double te, double ts, double tn,
@@ -61,34 +74,143 @@ __device__ __forceinline__ double uniform_cell(double t, double tw, double te, d
     return t + f * flux;
 }
 
-// Streamed schedule: one step of every work item.  A thread owns one xy column of its item and
-// marches z with its own column in registers; x and y neighbours come through the L1/L2.
-__global__ __launch_bounds__(256) void ens_step(const Member *__restrict__ members,
-                                                const Item *__restrict__ items,
-                                                const double *__restrict__ told,
-                                                double *__restrict__ tnew) {
-    const Item it = items[blockIdx.x];
+// The uniform update of the cell at index c of a plane staged in LDS with row stride w; `mask`
+// says which in-plane neighbours exist (bit 0 x-1, 1 x+1, 2 y-1, 3 y+1); pd/pu are the planes
+// below and above (the plane itself at the block's z faces).
+__device__ __forceinline__ double staged_cell(const double *pc, const double *pd, const double *pu,
+                                              int c, int w, unsigned mask, double kface,
+                                              double kzl, double kzh, double f) {
+    const double t = pc[c];
+    const double tw = pc[c - (int)(mask & 1u)];
+    const double te = pc[c + (int)((mask >> 1) & 1u)];
+    const double ts = pc[(mask & 4u) ? c - w : c];
+    const double tn = pc[(mask & 8u) ? c + w : c];
+    return uniform_cell(t, tw, te, ts, tn, pd[c], pu[c], (mask & 1u) ? kface : 0.0,
+                        (mask & 2u) ? kface : 0.0, (mask & 4u) ? kface : 0.0,
+                        (mask & 8u) ? kface : 0.0, kzl, kzh, f);
+}
+
+__device__ __forceinline__ unsigned face_mask(int x, int y, int nx, int ny) {
+    return (x > 0 ? 1u : 0u) | (x + 1 < nx ? 2u : 0u) | (y > 0 ? 4u : 0u) | (y + 1 < ny ? 8u : 0u);
+}
+
+// Streamed schedule: S (1 or 2) steps of every work item per launch.  A workgroup marches its
+// tile in z.  Per plane i it stages T(i) (tile + S-cell halo, prefetched D planes ahead in
+// registers) in an LDS ring of four planes; for S = 2 it computes the first step on the tile
+// widened by one cell at plane i-2 into a second LDS ring and the second step on the tile at
+// plane i-4, so the three parts of a plane touch disjoint ring slots and one LDS barrier per
+// plane orders them.  The planes next to a z-chunk are recomputed by both chunks (same inputs,
+// same operations, the same bits).  T crosses HBM once per S steps.
+template <int S, int D>
+__global__ __launch_bounds__(ENS_NT, 4) void ens_tile(const Member *__restrict__ members,
+                                                      const Tile *__restrict__ tiles,
+                                                      const double *__restrict__ told,
+                                                      double *__restrict__ tnew) {
+    __shared__ double sT[4][ENS_W0];
+    __shared__ double s1[S == 2 ? 4 : 1][S == 2 ? ENS_W1 : 1];
+    const Tile it = tiles[blockIdx.x];
+    if (it.member < 0) return;  // the whole workgroup leaves before any barrier
     const Member m = members[it.member];
-    const int64_t nx = m.nx, ny = m.ny, nz = m.nz, plane = nx * ny;
-    const int64_t col = it.col0 + threadIdx.x;
-    if (col >= plane) return;
-    const int64_t i = col % nx, j = col / nx;
+    const int nx = m.nx, ny = m.ny, nz = m.nz;
+    const int64_t plane = (int64_t)nx * ny;
+    const int ox0 = it.x0, ox1 = min(it.x0 + m.tx, nx), oy0 = it.y0, oy1 = min(it.y0 + m.ty, ny);
+    const int lx0 = max(ox0 - S, 0), lx1 = min(ox1 + S, nx);
+    const int ly0 = max(oy0 - S, 0), ly1 = min(oy1 + S, ny);
+    const int ax0 = max(ox0 - (S - 1), 0), ax1 = min(ox1 + (S - 1), nx);
+    const int ay0 = max(oy0 - (S - 1), 0), ay1 = min(oy1 + (S - 1), ny);
+    const int w0 = lx1 - lx0, n0 = w0 * (ly1 - ly0);
+    const int w1 = ax1 - ax0, n1 = w1 * (ay1 - ay0);
+    const int wo = ox1 - ox0, no = wo * (oy1 - oy0);
+    const int t = threadIdx.x;
+    const double kf = m.kface, f = m.f;
     const double *T = told + m.off;
     double *Tn = tnew + m.off;
-    const double kxl = i > 0 ? m.kface : 0.0, kxh = i + 1 < nx ? m.kface : 0.0;
-    const double kyl = j > 0 ? m.kface : 0.0, kyh = j + 1 < ny ? m.kface : 0.0;
-    const int64_t dw = i > 0 ? -1 : 0, de = i + 1 < nx ? 1 : 0;
-    const int64_t ds = j > 0 ? -nx : 0, dn = j + 1 < ny ? nx : 0;
-    const int64_t z0 = it.z0, z1 = it.z1;
-    double t_m = T[col + plane * (z0 > 0 ? z0 - 1 : 0)];
-    double t_c = T[col + plane * z0];
-    for (int64_t z = z0; z < z1; z++) {
-        const int64_t c = col + plane * z;
-        const double t_p = T[col + plane * (z + 1 < nz ? z + 1 : z)];
-        const double kzl = z > 0 ? m.kface : 0.0, kzh = z + 1 < nz ? m.kface : 0.0;
-        Tn[c] = uniform_cell(t_c, T[c + dw], T[c + de], T[c + ds], T[c + dn], t_m, t_p, kxl, kxh,
-                             kyl, kyh, kzl, kzh, m.f);
-        t_m = t_c, t_c = t_p;
+
+    // Staged cells this thread loads (in-plane offsets, -1 for none).
+    int gl[2];
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        const int q = t + r * ENS_NT;
+        gl[r] = q < n0 ? (ly0 + q / w0) * nx + lx0 + q % w0 : -1;
+    }
+    // First-step cells: index in the staged plane, face mask, and (S = 1) the output offset.
+    int c1[2], g1[2];
+    unsigned f1[2];
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        const int q = t + r * ENS_NT;
+        c1[r] = -1, g1[r] = 0, f1[r] = 0;
+        if (q < n1) {
+            const int x = ax0 + q % w1, y = ay0 + q / w1;
+            c1[r] = (y - ly0) * w0 + (x - lx0);
+            f1[r] = face_mask(x, y, nx, ny);
+            g1[r] = y * nx + x;
+        }
+    }
+    // Second-step cell (S = 2): index in the first-step plane and output offset.
+    int c2 = -1, g2 = 0;
+    unsigned f2 = 0;
+    if (S == 2 && t < no) {
+        const int x = ox0 + t % wo, y = oy0 + t / wo;
+        c2 = (y - ay0) * w1 + (x - ax0);
+        f2 = face_mask(x, y, nx, ny);
+        g2 = y * nx + x;
+    }
+
+    const int z0 = it.z0, z1 = it.z1;
+    const int lo_load = max(z0 - S, 0), hi_load = min(z1 - 1 + S, nz - 1);
+    const int lo1 = max(z0 - (S - 1), 0), hi1 = min(z1 - 1 + (S - 1), nz - 1);
+    const int iend = z1 - 1 + 2 * S;
+
+    double pre[D][2];
+#pragma unroll
+    for (int u = 0; u < D; u++)
+#pragma unroll
+        for (int r = 0; r < 2; r++)
+            pre[u][r] = (lo_load + u <= hi_load && gl[r] >= 0) ? T[(lo_load + u) * plane + gl[r]] : 0.0;
+
+    auto phase = [&](int i, double (&reg)[2]) {
+        if (i <= hi_load) {
+#pragma unroll
+            for (int r = 0; r < 2; r++)
+                if (gl[r] >= 0) sT[i & 3][t + r * ENS_NT] = reg[r];
+            if (i + D <= hi_load) {
+#pragma unroll
+                for (int r = 0; r < 2; r++)
+                    if (gl[r] >= 0) reg[r] = T[(i + D) * plane + gl[r]];
+            }
+        }
+        const int p = i - 2;
+        if (p >= lo1 && p <= hi1) {
+            const double *pc = sT[p & 3], *pd = sT[max(p - 1, 0) & 3], *pu = sT[min(p + 1, nz - 1) & 3];
+            const double kzl = p > 0 ? kf : 0.0, kzh = p + 1 < nz ? kf : 0.0;
+#pragma unroll
+            for (int r = 0; r < 2; r++) {
+                if (c1[r] < 0) continue;
+                const double v = staged_cell(pc, pd, pu, c1[r], w0, f1[r], kf, kzl, kzh, f);
+                if constexpr (S == 2)
+                    s1[p & 3][t + r * ENS_NT] = v;
+                else
+                    __builtin_nontemporal_store(v, Tn + p * plane + g1[r]);
+            }
+        }
+        if constexpr (S == 2) {
+            const int p2 = i - 4;
+            if (p2 >= z0 && p2 < z1 && c2 >= 0) {
+                const double *pc = s1[p2 & 3], *pd = s1[max(p2 - 1, 0) & 3],
+                             *pu = s1[min(p2 + 1, nz - 1) & 3];
+                const double kzl = p2 > 0 ? kf : 0.0, kzh = p2 + 1 < nz ? kf : 0.0;
+                const double v = staged_cell(pc, pd, pu, c2, w1, f2, kf, kzl, kzh, f);
+                __builtin_nontemporal_store(v, Tn + p2 * plane + g2);
+            }
+        }
+        lds_barrier();
+    };
+
+    for (int i = lo_load; i <= iend; i += D) {
+#pragma unroll
+        for (int u = 0; u < D; u++)
+            if (i + u <= iend) phase(i + u, pre[u]);
     }
 }
 
@@ -148,10 +270,11 @@ struct pm_ensemble {
     hipEvent_t fork = nullptr, join = nullptr, ev0 = nullptr, ev1 = nullptr;
     DevBuf t[2];
     int cur = 0;
-    DevBuf dmembers, ditems, dresident, dstored;
+    DevBuf dmembers, dtiles, dresident, dstored;
     int resident_edge = 0;  // members with max edge <= this run resident
-    int z_chunk = 16;
-    int64_t item_count = 0, resident_count = 0;
+    int z_chunk = 32;       // target planes per streamed tile
+    int steps_per_pass = 2;
+    int64_t tile_blocks = 0, tile_count = 0, resident_count = 0;
     size_t lds_limit = 0;
     hipGraphExec_t graph = nullptr;
     uint64_t graph_steps = 0;
@@ -162,32 +285,60 @@ namespace pm {
 
 static size_t resident_bytes(const pm_member &m) { return 2 * sizeof(double) * m.nx * m.ny * m.nz; }
 
+// The streamed tile of a member: whole rows when a row is at most 128 cells (no x halo), else
+// 64 x 8.  Either way the tile holds at most ENS_NT cells, its two-cell-halo plane at most
+// ENS_W0 and its one-cell-halo plane at most ENS_W1.
+static void tile_shape(int nx, int ny, int *tx, int *ty) {
+    if (nx <= 128) {
+        *tx = nx;
+        *ty = std::max(1, std::min(ny, ENS_NT / nx));
+    } else {
+        *tx = 64;
+        *ty = std::min(ny, 8);
+    }
+}
+
 static int ens_plan(pm_ensemble *e) {
-    std::vector<Item> items;
     std::vector<int> resident;
-    // Largest members first, so the long items start early.
-    std::vector<size_t> order(e->spec.size());
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
-        return e->spec[a].nx * e->spec[a].ny * e->spec[a].nz > e->spec[b].nx * e->spec[b].ny * e->spec[b].nz;
-    });
-    for (size_t idx : order) {
+    // Streamed tiles are grouped by (member, z-chunk); the groups go, largest first, to the XCD
+    // with the least work so far, so the tiles that share halo rows run on one XCD's L2 and
+    // the XCDs finish together.  Launch block b runs on XCD b mod 8.
+    struct Group { int member, z0, z1; double cost; };
+    std::vector<Group> groups;
+    for (size_t idx = 0; idx < e->spec.size(); idx++) {
         const pm_member &s = e->spec[idx];
         const size_t edge = std::max(s.nx, std::max(s.ny, s.nz));
         if ((int)edge <= e->resident_edge && resident_bytes(s) <= e->lds_limit) {
             resident.push_back((int)idx);
             continue;
         }
-        const int plane = (int)(s.nx * s.ny);
-        for (int z0 = 0; z0 < (int)s.nz; z0 += e->z_chunk)
-            for (int c0 = 0; c0 < plane; c0 += 256)
-                items.push_back({(int)idx, c0, z0, std::min<int>(z0 + e->z_chunk, (int)s.nz)});
+        const int nz = (int)s.nz;
+        const int chunks = (nz + e->z_chunk - 1) / e->z_chunk;
+        const int zc = (nz + chunks - 1) / chunks;
+        for (int z0 = 0; z0 < nz; z0 += zc)
+            groups.push_back({(int)idx, z0, std::min(z0 + zc, nz), (double)s.nx * s.ny * std::min(zc, nz - z0)});
     }
-    e->item_count = (int64_t)items.size();
+    std::stable_sort(groups.begin(), groups.end(), [](const Group &a, const Group &b) { return a.cost > b.cost; });
+    std::vector<std::vector<Tile>> per(ENS_XCDS);
+    std::vector<double> load(ENS_XCDS, 0.0);
+    for (const Group &g : groups) {
+        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        const Member &m = e->members[g.member];
+        for (int y0 = 0; y0 < m.ny; y0 += m.ty)
+            for (int x0 = 0; x0 < m.nx; x0 += m.tx) per[x].push_back({g.member, x0, y0, g.z0, g.z1, 0});
+        load[x] += g.cost;
+    }
+    size_t depth = 0, count = 0;
+    for (const auto &v : per) depth = std::max(depth, v.size()), count += v.size();
+    std::vector<Tile> tiles(depth * ENS_XCDS, Tile{-1, 0, 0, 0, 0, 0});
+    for (int x = 0; x < ENS_XCDS; x++)
+        for (size_t j = 0; j < per[x].size(); j++) tiles[j * ENS_XCDS + x] = per[x][j];
+    e->tile_blocks = (int64_t)tiles.size();
+    e->tile_count = (int64_t)count;
     e->resident_count = (int64_t)resident.size();
-    if (!items.empty()) {
-        PM_CHECK(e->ditems.alloc(sizeof(Item) * items.size()));
-        PM_HIP(hipMemcpy(e->ditems.d(), items.data(), sizeof(Item) * items.size(), hipMemcpyHostToDevice));
+    if (!tiles.empty()) {
+        PM_CHECK(e->dtiles.alloc(sizeof(Tile) * tiles.size()));
+        PM_HIP(hipMemcpy(e->dtiles.d(), tiles.data(), sizeof(Tile) * tiles.size(), hipMemcpyHostToDevice));
     }
     if (!resident.empty()) {
         PM_CHECK(e->dresident.alloc(sizeof(int) * resident.size()));
@@ -223,10 +374,18 @@ static int ens_capture(pm_ensemble *e, uint64_t steps) {
     if (e->resident_count > 0)
         ens_resident<<<(unsigned)e->resident_count, 1024, max_lds, e->side>>>(
             e->dmembers.as<Member>(), e->dresident.as<int>(), e->t[c0].d(), e->t[cend].d(), (int)steps);
+    // Two-step passes in an even number, so that after `steps` steps the state sits in the same
+    // buffer as after `steps` single steps; the rest (at most three) as single steps.
+    const uint64_t pairs = e->steps_per_pass == 2 ? (steps / 2) & ~(uint64_t)1 : 0;
+    const uint64_t singles = steps - 2 * pairs;
     int c = c0;
-    for (uint64_t s = 0; s < steps && e->item_count > 0; s++) {
-        ens_step<<<(unsigned)e->item_count, 256, 0, e->stream>>>(e->dmembers.as<Member>(), e->ditems.as<Item>(),
-                                                                  e->t[c].d(), e->t[c ^ 1].d());
+    for (uint64_t s = 0; s < pairs + singles && e->tile_blocks > 0; s++) {
+        if (s < pairs)
+            ens_tile<2, ENS_D><<<(unsigned)e->tile_blocks, ENS_NT, 0, e->stream>>>(
+                e->dmembers.as<Member>(), e->dtiles.as<Tile>(), e->t[c].d(), e->t[c ^ 1].d());
+        else
+            ens_tile<1, ENS_D><<<(unsigned)e->tile_blocks, ENS_NT, 0, e->stream>>>(
+                e->dmembers.as<Member>(), e->dtiles.as<Tile>(), e->t[c].d(), e->t[c ^ 1].d());
         c ^= 1;
     }
     PM_HIP(hipEventRecord(e->join, e->side));
@@ -272,13 +431,14 @@ int pm_ensemble_create(const pm_member *members, size_t count, pm_ensemble **out
     for (size_t m = 0; m < count; m++) {
         pm_member &s = e->spec[m];
         s.nx = std::max<size_t>(s.nx, 1), s.ny = std::max<size_t>(s.ny, 1), s.nz = std::max<size_t>(s.nz, 1);
-        if (s.nx * s.ny > (1u << 30) || s.nz > (1u << 30)) {
+        if (s.nx * s.ny > (1u << 30) || s.nz > (1u << 30) || s.nx * s.ny * s.nz > ((size_t)1 << 40)) {
             set_error("pm_ensemble_create: member %zu is too large", m);
             return PM_ERR_ARGUMENT;
         }
         Member d;
         d.off = off;
         d.nx = (int)s.nx, d.ny = (int)s.ny, d.nz = (int)s.nz, d.pad = 0;
+        tile_shape(d.nx, d.ny, &d.tx, &d.ty);
         const double k = s.substance.conductivity;
         const double sum = k + k;
         d.kface = sum > 0.0 ? 2.0 * k * k / sum : 0.0;
@@ -399,6 +559,19 @@ int pm_ensemble_set_schedule(pm_ensemble *e, int resident_edge, int z_chunk) {
     return ens_plan(e);
 }
 
+int pm_ensemble_set_steps_per_pass(pm_ensemble *e, int steps) {
+    if (!e || (steps != 1 && steps != 2)) {
+        set_error("pm_ensemble_set_steps_per_pass: steps must be 1 or 2");
+        return PM_ERR_ARGUMENT;
+    }
+    e->steps_per_pass = steps;
+    if (e->graph) {
+        (void)hipGraphExecDestroy(e->graph);
+        e->graph = nullptr;
+    }
+    return PM_OK;
+}
+
 int pm_ensemble_tune(pm_ensemble *e, char *report, size_t report_len) {
     if (!e) return PM_ERR_ARGUMENT;
     if (e->spec.empty()) return PM_OK;
@@ -410,7 +583,7 @@ int pm_ensemble_tune(pm_ensemble *e, char *report, size_t report_len) {
     struct Cand { int edge, zc; };
     std::vector<Cand> cands;
     for (int edge : {0, 16, 20, 24})
-        for (int zc : {8, 16, 32}) cands.push_back({edge, zc});
+        for (int zc : {16, 32, 64}) cands.push_back({edge, zc});
     const uint64_t trial = 20;
     double best = INFINITY;
     Cand pick = {e->resident_edge, e->z_chunk};
@@ -422,8 +595,8 @@ int pm_ensemble_tune(pm_ensemble *e, char *report, size_t report_len) {
         PM_CHECK(ens_run(e, trial, &ms));   // warm (includes capture)
         PM_CHECK(ens_run(e, trial, &ms2));  // timed replay
         char line[160];
-        std::snprintf(line, sizeof line, "resident_edge=%d z_chunk=%d resident=%lld items=%lld: %.4f ms/step\n",
-                      c.edge, c.zc, (long long)e->resident_count, (long long)e->item_count, ms2 / trial);
+        std::snprintf(line, sizeof line, "resident_edge=%d z_chunk=%d resident=%lld tiles=%lld: %.4f ms/step\n",
+                      c.edge, c.zc, (long long)e->resident_count, (long long)e->tile_count, ms2 / trial);
         text += line;
         if (ms2 < best) best = ms2, pick = c;
     }

@@ -75,6 +75,11 @@ class Ensemble:
         check(lib().pm_ensemble_set_schedule(self._h, resident_edge, z_chunk),
               "pm_ensemble_set_schedule")
 
+    def set_steps_per_pass(self, steps: int) -> None:
+        """Streamed members advance 2 steps per launch (default) or 1; the results are the same."""
+        check(lib().pm_ensemble_set_steps_per_pass(self._h, int(steps)),
+              "pm_ensemble_set_steps_per_pass")
+
     def tune(self) -> str:
         buf = C.create_string_buffer(8192)
         check(lib().pm_ensemble_tune(self._h, buf, len(buf)), "pm_ensemble_tune")

@@ -22,12 +22,14 @@ def _oracle_member(oracle, edge, alpha, seed, steps, dt):
     return o
 
 
-@pytest.mark.parametrize("resident_edge,z_chunk", [(0, 8), (0, 32), (24, 16), (40, 5)])
-def test_members_are_bit_exact_under_every_schedule(resident_edge, z_chunk, oracle):
-    gpu_case(f"ensemble schedule {resident_edge}/{z_chunk}")
+@pytest.mark.parametrize("resident_edge,z_chunk,per_pass",
+                         [(0, 8, 2), (0, 32, 2), (24, 16, 2), (40, 5, 2), (0, 16, 1), (20, 3, 1)])
+def test_members_are_bit_exact_under_every_schedule(resident_edge, z_chunk, per_pass, oracle):
+    gpu_case(f"ensemble schedule {resident_edge}/{z_chunk}/{per_pass}")
     from pantometry_amd import workloads as w
     ens, shapes = w.heat_ensemble(24, lo_edge=14.0, hi_edge=40.0)
     ens.set_schedule(resident_edge, z_chunk)
+    ens.set_steps_per_pass(per_pass)
     before = ens.ledgers()
     ens.run(37)
     after = ens.ledgers()
@@ -49,6 +51,54 @@ def test_runs_continue_across_calls(oracle):
     for m, (edge, alpha) in enumerate(shapes):
         o = _oracle_member(oracle, edge, alpha, w.ENSEMBLE_SEED + 1 + m, 17, ens.members[m].dt)
         assert np.array_equal(bits(ens.cells(m)), bits(o.cells)), m
+
+
+# Uneven shapes: rows over 128 cells (64 x 8 tiles, partial in x and y), rows of 1-3 cells
+# (tiles of hundreds of rows), single-plane and single-column members, z-chunks of one plane.
+ODD_SHAPES = [(130, 9, 7), (200, 17, 3), (3, 50, 20), (1, 1, 40), (97, 6, 33), (128, 5, 9),
+              (40, 1, 1), (2, 2, 2), (65, 70, 2)]
+
+
+@pytest.mark.parametrize("steps", [1, 2, 3, 4, 7])
+@pytest.mark.parametrize("z_chunk", [1, 16])
+def test_uneven_members_are_bit_exact(steps, z_chunk, oracle):
+    gpu_case(f"ensemble uneven members steps={steps} z_chunk={z_chunk}")
+    from pantometry_amd import workloads as w
+    from pantometry_amd.ensemble import Ensemble, Member
+    from pantometry_amd.thermal import Substance
+    members, blocks = [], []
+    for n, counts in enumerate(ODD_SHAPES):
+        alpha = 2e-5 * (1.0 + 0.37 * n)
+        o = oracle.OracleSolid3D("m", w.RHO, alpha * w.RHO * w.CP, w.CP, counts, w.DX, 300.0)
+        o.set_cells(oracle.hot_spot_cells(counts, seed=0x0DD0 + n))
+        dt = 0.9 * o.max_stable_dt()
+        members.append(Member(counts, w.DX, Substance.of("m", w.RHO, alpha * w.RHO * w.CP, w.CP),
+                              300.0, dt))
+        blocks.append((o, dt))
+    ens = Ensemble(members)
+    for n, (o, _) in enumerate(blocks):
+        ens.set_cells(n, o.cells)
+    ens.set_schedule(0, z_chunk)
+    ens.run(steps)
+    for n, (o, dt) in enumerate(blocks):
+        for _ in range(steps):
+            o.step(dt, threads_=16)
+        assert np.array_equal(bits(ens.cells(n)), bits(o.cells)), ODD_SHAPES[n]
+
+
+def test_one_and_two_steps_per_pass_agree():
+    gpu_case("ensemble per-pass 1 vs 2")
+    from pantometry_amd import workloads as w
+    runs = []
+    for per_pass in (1, 2):
+        ens, shapes = w.heat_ensemble(40, lo_edge=16.0, hi_edge=90.0)
+        ens.set_schedule(0, 24)
+        ens.set_steps_per_pass(per_pass)
+        ens.run(9)
+        ens.run(12)
+        runs.append([ens.cells(m).copy() for m in range(len(shapes))])
+    for a, b in zip(*runs):
+        assert np.array_equal(bits(a), bits(b))
 
 
 @pytest.mark.timeout(900)

@@ -41,6 +41,17 @@ def _traffic_from_profiles(steps_per_launch: int = 1):
         return None
 
 
+def _ensemble_traffic(steps_per_launch: int):
+    """HBM bytes per ens_tile launch from the committed counter summary, if present."""
+    p = ROOT / "profiles" / "ensemble_512_counters.json"
+    if steps_per_launch != 2 or not p.exists():
+        return None
+    try:
+        return json.loads(p.read_text()).get("hbm_bytes_per_launch")
+    except (ValueError, OSError):
+        return None
+
+
 def cpu_baseline(threads: int, budget_s: float = 12.0):
     """The oracle's sweep (the reference's arithmetic, OpenMP over z-planes) on the heterogeneous
     block at 512^3: whole steps until about `budget_s` seconds of stepping have run."""
@@ -101,7 +112,12 @@ def bench_ensemble(args) -> dict:
     """512 uniform heat blocks, edges 16-96, 500 steps each in one graph launch."""
     from pantometry_amd import workloads as w
     ens, shapes = w.heat_ensemble(512)
-    report = ens.tune()
+    ens.set_steps_per_pass(args.steps_per_pass)
+    if args.z_chunk > 0:
+        ens.set_schedule(0, args.z_chunk)
+        report = f"fixed resident_edge=0 z_chunk={args.z_chunk}"
+    else:
+        report = ens.tune()
     cells = sum(e ** 3 for e, _ in shapes)
     ens.run(args.warmup)
     before = ens.ledgers()
@@ -121,8 +137,14 @@ def bench_ensemble(args) -> dict:
         "roofline": {"bound": "hbm", "achieved": round(16 * cells / (ms / args.steps * 1e-3) / 1e9, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(16 * cells / (ms / args.steps * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                     "traffic": None, "bytes_per_cell": 16,
-                     "note": "graph launch time per step; resident members touch HBM only at the ends"},
+                     "traffic": _ensemble_traffic(args.steps_per_pass), "bytes_per_cell": 16,
+                     "steps_per_launch": args.steps_per_pass,
+                     "kernel": f"ens_tile<{args.steps_per_pass}> (whole-row tiles, z-marching, "
+                               "LDS-staged planes)",
+                     "note": "graph launch time per step (one launch per two steps: temporal "
+                             "blocking, so 'achieved' counts 16 B per cell-step while 'traffic' "
+                             "is the HBM bytes of one launch); resident members touch HBM only "
+                             "at the ends"},
         "audit_drift": float((abs(after - before) / abs(before)).max()),
         "cpu_baseline": None,
     }

@@ -53,11 +53,13 @@ struct Tile {
     int pad;
 };
 
-constexpr int ENS_NT = 512;    // threads per streamed workgroup; a tile holds at most this many
+constexpr int ENS_TILE = 512;  // most cells of a streamed tile
 constexpr int ENS_W0 = 1024;   // most cells of a staged plane (tile + two-cell halo)
-constexpr int ENS_W1 = 768;    // most cells of a first-step plane (tile + one-cell halo)
+constexpr int ENS_NT = 1024;    // threads per streamed workgroup
+constexpr int ENS_R = ENS_W0 / ENS_NT;  // staged cells per thread
+constexpr int ENS_WAVES = 8;   // waves per SIMD the register budget is set for
 constexpr int ENS_XCDS = 8;
-constexpr int ENS_D = 3;       // planes prefetched ahead in registers
+constexpr int ENS_D = 2;       // planes prefetched ahead in registers
 
 // One cell's update in the uniform form; neighbours off the block carry a zero face.
 __device__ __forceinline__ double uniform_cell(double t, double tw, double te, double ts, double tn,
@@ -74,18 +76,18 @@ __device__ __forceinline__ double uniform_cell(double t, double tw, double te, d
     return t + f * flux;
 }
 
-// The uniform update of the cell at index c of a plane staged in LDS with row stride w; `mask`
-// says which in-plane neighbours exist (bit 0 x-1, 1 x+1, 2 y-1, 3 y+1); pd/pu are the planes
-// below and above (the plane itself at the block's z faces).
-__device__ __forceinline__ double staged_cell(const double *pc, const double *pd, const double *pu,
-                                              int c, int w, unsigned mask, double kface,
+// The uniform update of a cell whose own value t and z neighbours td/tu are in registers and
+// whose in-plane neighbours sit at index c of a plane staged in LDS with row stride w; `mask`
+// says which in-plane neighbours exist (bit 0 x-1, 1 x+1, 2 y-1, 3 y+1; a missing one reads
+// the cell itself, as in the uniform sweep).
+__device__ __forceinline__ double staged_cell(const double *pc, int c, int w, unsigned mask,
+                                              double t, double td, double tu, double kface,
                                               double kzl, double kzh, double f) {
-    const double t = pc[c];
     const double tw = pc[c - (int)(mask & 1u)];
     const double te = pc[c + (int)((mask >> 1) & 1u)];
     const double ts = pc[(mask & 4u) ? c - w : c];
     const double tn = pc[(mask & 8u) ? c + w : c];
-    return uniform_cell(t, tw, te, ts, tn, pd[c], pu[c], (mask & 1u) ? kface : 0.0,
+    return uniform_cell(t, tw, te, ts, tn, td, tu, (mask & 1u) ? kface : 0.0,
                         (mask & 2u) ? kface : 0.0, (mask & 4u) ? kface : 0.0,
                         (mask & 8u) ? kface : 0.0, kzl, kzh, f);
 }
@@ -94,20 +96,25 @@ __device__ __forceinline__ unsigned face_mask(int x, int y, int nx, int ny) {
     return (x > 0 ? 1u : 0u) | (x + 1 < nx ? 2u : 0u) | (y > 0 ? 4u : 0u) | (y + 1 < ny ? 8u : 0u);
 }
 
+constexpr unsigned IN_FIRST = 16u;  // the cell takes the first step (tile + S-1 halo)
+constexpr unsigned IN_TILE = 32u;   // the cell is the workgroup's own
+
 // Streamed schedule: S (1 or 2) steps of every work item per launch.  A workgroup marches its
-// tile in z.  Per plane i it stages T(i) (tile + S-cell halo, prefetched D planes ahead in
-// registers) in an LDS ring of four planes; for S = 2 it computes the first step on the tile
-// widened by one cell at plane i-2 into a second LDS ring and the second step on the tile at
-// plane i-4, so the three parts of a plane touch disjoint ring slots and one LDS barrier per
-// plane orders them.  The planes next to a z-chunk are recomputed by both chunks (same inputs,
-// same operations, the same bits).  T crosses HBM once per S steps.
+// tile in z.  Each thread owns up to two cells of the staged plane (tile + S-cell halo) and
+// keeps their z columns in registers: T prefetched D planes ahead, then T(i-2..i), and for
+// S = 2 the first-step values T1(i-3..i-1).  Per plane i it stages T(i) in LDS (for the
+// in-plane neighbours; double-buffered), computes the first step at plane i-1 on the cells of
+// the tile widened by S-1 (into LDS for the second step's neighbours) and, for S = 2, the
+// second step at plane i-2 on the tile's own cells; one LDS barrier per plane.  The planes next
+// to a z-chunk are recomputed by both chunks (same inputs, same operations, the same bits).
+// T crosses HBM once per S steps.
 template <int S, int D>
-__global__ __launch_bounds__(ENS_NT, 4) void ens_tile(const Member *__restrict__ members,
+__global__ __launch_bounds__(ENS_NT, ENS_WAVES) void ens_tile(const Member *__restrict__ members,
                                                       const Tile *__restrict__ tiles,
                                                       const double *__restrict__ told,
                                                       double *__restrict__ tnew) {
-    __shared__ double sT[4][ENS_W0];
-    __shared__ double s1[S == 2 ? 4 : 1][S == 2 ? ENS_W1 : 1];
+    __shared__ double sT[2][ENS_W0];
+    __shared__ double s1[S == 2 ? 2 : 1][S == 2 ? ENS_W0 : 1];
     const Tile it = tiles[blockIdx.x];
     if (it.member < 0) return;  // the whole workgroup leaves before any barrier
     const Member m = members[it.member];
@@ -116,92 +123,90 @@ __global__ __launch_bounds__(ENS_NT, 4) void ens_tile(const Member *__restrict__
     const int ox0 = it.x0, ox1 = min(it.x0 + m.tx, nx), oy0 = it.y0, oy1 = min(it.y0 + m.ty, ny);
     const int lx0 = max(ox0 - S, 0), lx1 = min(ox1 + S, nx);
     const int ly0 = max(oy0 - S, 0), ly1 = min(oy1 + S, ny);
-    const int ax0 = max(ox0 - (S - 1), 0), ax1 = min(ox1 + (S - 1), nx);
-    const int ay0 = max(oy0 - (S - 1), 0), ay1 = min(oy1 + (S - 1), ny);
     const int w0 = lx1 - lx0, n0 = w0 * (ly1 - ly0);
-    const int w1 = ax1 - ax0, n1 = w1 * (ay1 - ay0);
-    const int wo = ox1 - ox0, no = wo * (oy1 - oy0);
     const int t = threadIdx.x;
     const double kf = m.kface, f = m.f;
     const double *T = told + m.off;
     double *Tn = tnew + m.off;
 
-    // Staged cells this thread loads (in-plane offsets, -1 for none).
-    int gl[2];
+    // This thread's cells: in-plane offset (-1 for none) and role bits.
+    int g[ENS_R];
+    unsigned info[ENS_R];
 #pragma unroll
-    for (int r = 0; r < 2; r++) {
+    for (int r = 0; r < ENS_R; r++) {
         const int q = t + r * ENS_NT;
-        gl[r] = q < n0 ? (ly0 + q / w0) * nx + lx0 + q % w0 : -1;
-    }
-    // First-step cells: index in the staged plane, face mask, and (S = 1) the output offset.
-    int c1[2], g1[2];
-    unsigned f1[2];
-#pragma unroll
-    for (int r = 0; r < 2; r++) {
-        const int q = t + r * ENS_NT;
-        c1[r] = -1, g1[r] = 0, f1[r] = 0;
-        if (q < n1) {
-            const int x = ax0 + q % w1, y = ay0 + q / w1;
-            c1[r] = (y - ly0) * w0 + (x - lx0);
-            f1[r] = face_mask(x, y, nx, ny);
-            g1[r] = y * nx + x;
+        g[r] = -1, info[r] = 0;
+        if (q < n0) {
+            const int x = lx0 + q % w0, y = ly0 + q / w0;
+            g[r] = y * nx + x;
+            info[r] = face_mask(x, y, nx, ny);
+            if (x >= ox0 - (S - 1) && x < ox1 + (S - 1) && y >= oy0 - (S - 1) && y < oy1 + (S - 1))
+                info[r] |= IN_FIRST;
+            if (x >= ox0 && x < ox1 && y >= oy0 && y < oy1) info[r] |= IN_TILE;
         }
-    }
-    // Second-step cell (S = 2): index in the first-step plane and output offset.
-    int c2 = -1, g2 = 0;
-    unsigned f2 = 0;
-    if (S == 2 && t < no) {
-        const int x = ox0 + t % wo, y = oy0 + t / wo;
-        c2 = (y - ay0) * w1 + (x - ax0);
-        f2 = face_mask(x, y, nx, ny);
-        g2 = y * nx + x;
     }
 
     const int z0 = it.z0, z1 = it.z1;
     const int lo_load = max(z0 - S, 0), hi_load = min(z1 - 1 + S, nz - 1);
     const int lo1 = max(z0 - (S - 1), 0), hi1 = min(z1 - 1 + (S - 1), nz - 1);
-    const int iend = z1 - 1 + 2 * S;
+    const int iend = z1 - 1 + S;
 
-    double pre[D][2];
+    double pre[D][ENS_R];
 #pragma unroll
     for (int u = 0; u < D; u++)
 #pragma unroll
-        for (int r = 0; r < 2; r++)
-            pre[u][r] = (lo_load + u <= hi_load && gl[r] >= 0) ? T[(lo_load + u) * plane + gl[r]] : 0.0;
+        for (int r = 0; r < ENS_R; r++)
+            pre[u][r] = (lo_load + u <= hi_load && g[r] >= 0) ? T[(lo_load + u) * plane + g[r]] : 0.0;
+    double zm[ENS_R] = {}, zc[ENS_R] = {}, zp[ENS_R] = {};  // T(i-2), T(i-1), T(i)
+    double ym[ENS_R] = {}, yc[ENS_R] = {}, yp[ENS_R] = {};  // T1(i-3), T1(i-2), T1(i-1)
 
-    auto phase = [&](int i, double (&reg)[2]) {
+    auto phase = [&](int i, double (&reg)[ENS_R]) {
+#pragma unroll
+        for (int r = 0; r < ENS_R; r++) zm[r] = zc[r], zc[r] = zp[r], zp[r] = reg[r];
         if (i <= hi_load) {
 #pragma unroll
-            for (int r = 0; r < 2; r++)
-                if (gl[r] >= 0) sT[i & 3][t + r * ENS_NT] = reg[r];
+            for (int r = 0; r < ENS_R; r++)
+                if (g[r] >= 0) sT[i & 1][t + r * ENS_NT] = zp[r];
             if (i + D <= hi_load) {
 #pragma unroll
-                for (int r = 0; r < 2; r++)
-                    if (gl[r] >= 0) reg[r] = T[(i + D) * plane + gl[r]];
+                for (int r = 0; r < ENS_R; r++)
+                    if (g[r] >= 0) reg[r] = T[(i + D) * plane + g[r]];
             }
         }
-        const int p = i - 2;
+        const int p = i - 1;
         if (p >= lo1 && p <= hi1) {
-            const double *pc = sT[p & 3], *pd = sT[max(p - 1, 0) & 3], *pu = sT[min(p + 1, nz - 1) & 3];
+            const double *pc = sT[p & 1];
             const double kzl = p > 0 ? kf : 0.0, kzh = p + 1 < nz ? kf : 0.0;
 #pragma unroll
-            for (int r = 0; r < 2; r++) {
-                if (c1[r] < 0) continue;
-                const double v = staged_cell(pc, pd, pu, c1[r], w0, f1[r], kf, kzl, kzh, f);
-                if constexpr (S == 2)
-                    s1[p & 3][t + r * ENS_NT] = v;
-                else
-                    __builtin_nontemporal_store(v, Tn + p * plane + g1[r]);
+            for (int r = 0; r < ENS_R; r++) {
+                if (!(info[r] & IN_FIRST)) continue;
+                const double v = staged_cell(pc, t + r * ENS_NT, w0, info[r], zc[r],
+                                             p > 0 ? zm[r] : zc[r], p + 1 < nz ? zp[r] : zc[r],
+                                             kf, kzl, kzh, f);
+                if constexpr (S == 2) {
+                    s1[p & 1][t + r * ENS_NT] = v;
+                    ym[r] = yc[r], yc[r] = yp[r], yp[r] = v;
+                } else {
+                    __builtin_nontemporal_store(v, Tn + p * plane + g[r]);
+                }
             }
+        } else if constexpr (S == 2) {
+#pragma unroll
+            for (int r = 0; r < ENS_R; r++) ym[r] = yc[r], yc[r] = yp[r];
         }
         if constexpr (S == 2) {
-            const int p2 = i - 4;
-            if (p2 >= z0 && p2 < z1 && c2 >= 0) {
-                const double *pc = s1[p2 & 3], *pd = s1[max(p2 - 1, 0) & 3],
-                             *pu = s1[min(p2 + 1, nz - 1) & 3];
+            const int p2 = i - 2;
+            if (p2 >= z0 && p2 < z1) {
+                const double *pc = s1[p2 & 1];
                 const double kzl = p2 > 0 ? kf : 0.0, kzh = p2 + 1 < nz ? kf : 0.0;
-                const double v = staged_cell(pc, pd, pu, c2, w1, f2, kf, kzl, kzh, f);
-                __builtin_nontemporal_store(v, Tn + p2 * plane + g2);
+#pragma unroll
+                for (int r = 0; r < ENS_R; r++) {
+                    if (!(info[r] & IN_TILE)) continue;
+                    const double v = staged_cell(pc, t + r * ENS_NT, w0, info[r], yc[r],
+                                                 p2 > 0 ? ym[r] : yc[r],
+                                                 p2 + 1 < nz ? yp[r] : yc[r], kf, kzl, kzh, f);
+                    __builtin_nontemporal_store(v, Tn + p2 * plane + g[r]);
+                }
             }
         }
         lds_barrier();
@@ -286,16 +291,21 @@ namespace pm {
 static size_t resident_bytes(const pm_member &m) { return 2 * sizeof(double) * m.nx * m.ny * m.nz; }
 
 // The streamed tile of a member: whole rows when a row is at most 128 cells (no x halo), else
-// 64 x 8.  Either way the tile holds at most ENS_NT cells, its two-cell-halo plane at most
-// ENS_W0 and its one-cell-halo plane at most ENS_W1.
+// 64 x 8.  Either way the tile holds at most ENS_TILE cells and its plane with a two-cell halo
+// at most ENS_W0.
 static void tile_shape(int nx, int ny, int *tx, int *ty) {
     if (nx <= 128) {
         *tx = nx;
-        *ty = std::max(1, std::min(ny, ENS_NT / nx));
+        *ty = std::max(1, std::min(ny, ENS_TILE / nx));
     } else {
         *tx = 64;
         *ty = std::min(ny, 8);
     }
+}
+
+static bool tile_fits(const Member &m) {
+    const int w0 = std::min(m.tx + 4, m.nx), h0 = std::min(m.ty + 4, m.ny);
+    return m.tx * m.ty <= ENS_TILE && w0 * h0 <= ENS_W0;
 }
 
 static int ens_plan(pm_ensemble *e) {
@@ -439,6 +449,10 @@ int pm_ensemble_create(const pm_member *members, size_t count, pm_ensemble **out
         d.off = off;
         d.nx = (int)s.nx, d.ny = (int)s.ny, d.nz = (int)s.nz, d.pad = 0;
         tile_shape(d.nx, d.ny, &d.tx, &d.ty);
+        if (!tile_fits(d)) {
+            set_error("pm_ensemble_create: no streamed tile fits member %zu", m);
+            return PM_ERR_ARGUMENT;
+        }
         const double k = s.substance.conductivity;
         const double sum = k + k;
         d.kface = sum > 0.0 ? 2.0 * k * k / sum : 0.0;
@@ -583,7 +597,7 @@ int pm_ensemble_tune(pm_ensemble *e, char *report, size_t report_len) {
     struct Cand { int edge, zc; };
     std::vector<Cand> cands;
     for (int edge : {0, 16, 20, 24})
-        for (int zc : {16, 32, 64}) cands.push_back({edge, zc});
+        for (int zc : {16, 32, 64, 128}) cands.push_back({edge, zc});
     const uint64_t trial = 20;
     double best = INFINITY;
     Cand pick = {e->resident_edge, e->z_chunk};

@@ -330,12 +330,14 @@ int pm_surrogate_step_solid(pm_surrogate *s, pm_solid3d *block, double dt, doubl
 /* ||T_a - T_b||_2 / ||T_b - reference||_2 over two blocks of the same extents. */
 int pm_solid3d_relative_l2(pm_solid3d *a, pm_solid3d *b, double reference, double *rel);
 /* One bf16 convolution on host fp32 arrays (rounded to bf16 on the way in): x [B][D][H][W][Cin],
- * w [Cout][27][Cin], bias [Cout] -> y [B][D][H][W][Cout] before the activation. */
+ * w [Cout][27][Cin], bias [Cout] -> y [B][D][H][W][Cout] before the activation.  `split` lets a
+ * small grid use the split-reduction workspace (as the surrogate does); 0 runs it without. */
 int pm_nn_conv3d(int B, int D, int H, int W, int Cin, int Cout, const float *x, const float *w,
-                 const float *bias, float *y);
+                 const float *bias, float *y, int split);
 /* Its gradients for an upstream gradient dy: dx (bf16-rounded), dw and db (fp32). */
 int pm_nn_conv3d_backward(int B, int D, int H, int W, int Cin, int Cout, const float *x,
-                          const float *w, const float *dy, float *dx, float *dw, float *db);
+                          const float *w, const float *dy, float *dx, float *dw, float *db,
+                          int split);
 
 #ifdef __cplusplus
 }

@@ -120,8 +120,8 @@ SIGNATURES = {
     "pm_surrogate_forward": (_I, [_VP, C.c_int64, _DP, _DP, _D, _FP]),
     "pm_surrogate_step_solid": (_I, [_VP, _VP, _D, _D, _DP, _DP]),
     "pm_solid3d_relative_l2": (_I, [_VP, _VP, _D, _DP]),
-    "pm_nn_conv3d": (_I, [_I, _I, _I, _I, _I, _I, _FP, _FP, _FP, _FP]),
-    "pm_nn_conv3d_backward": (_I, [_I, _I, _I, _I, _I, _I, _FP, _FP, _FP, _FP, _FP, _FP]),
+    "pm_nn_conv3d": (_I, [_I, _I, _I, _I, _I, _I, _FP, _FP, _FP, _FP, _I]),
+    "pm_nn_conv3d_backward": (_I, [_I, _I, _I, _I, _I, _I, _FP, _FP, _FP, _FP, _FP, _FP, _I]),
 }
 
 _lib = None

@@ -22,6 +22,14 @@ namespace nn {
 
 typedef short short4v __attribute__((ext_vector_type(4)));
 typedef float float4v __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// gfx950's 16x16x32 bf16 MFMA: lane l holds A[l%16][8(l/16)+j] and B[8(l/16)+j][l%16] (j < 8)
+// and receives C[4(l/16)+r][l%16] (r < 4), the layout of the 16x16x16 form with K doubled.
+__device__ __forceinline__ float4v mfma32(uint4 a, uint4 b, float4v c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                   __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
 
 __device__ __forceinline__ float4v mfma16(short4v a, short4v b, float4v c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
@@ -86,7 +94,8 @@ __global__ __launch_bounds__(256) void conv_fwd2(Conv g, const bf16 *__restrict_
                                                  const float *__restrict__ bias,
                                                  bf16 *__restrict__ zout,
                                                  bf16 *__restrict__ aout, int gelu_out,
-                                                 int ostride, int ooff) {
+                                                 int ostride, int ooff,
+                                                 float *__restrict__ part) {
     constexpr int NB = 16 * NT;
     extern __shared__ __align__(16) bf16 smem[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -95,7 +104,12 @@ __global__ __launch_bounds__(256) void conv_fwd2(Conv g, const bf16 *__restrict_
     const int n0 = blockIdx.y * NB;
     const int SH = g.TH + 2, SW = g.TW + 2, SHW = SH * SW;
     const int TP = g.tile_positions();
-    const int kq = (lane >> 4) * 4;
+    // The staged neighbourhood is two planes (channels 0-7 and 8-15 of the chunk) of 16-byte
+    // positions, so the 16 lanes of a row group read 256 contiguous bytes.  Lane group
+    // lg = lane/16 takes tap 2p + lg/2 of a tap pair p and channel half lg%2.
+    const int lg = lane >> 4, qh = lg & 1, qt = lg >> 1;
+    const int npos = (g.TD + 2) * SHW;
+    const uint4 *sA = (const uint4 *)smem + qh * npos;
 
     // This lane's RB A rows: positions wave*16*RB + rb*16 + lane%16 of the box.
     int base[RB];
@@ -106,7 +120,7 @@ __global__ __launch_bounds__(256) void conv_fwd2(Conv g, const bf16 *__restrict_
         live[rb] = p < TP;
         int pd, ph, pw;
         box_coords(g, live[rb] ? p : 0, pd, ph, pw);
-        base[rb] = ((pd * SH + ph) * SW + pw) * 16 + kq;
+        base[rb] = (pd * SH + ph) * SW + pw;
     }
 
     float4v acc[RB][NT];
@@ -116,9 +130,18 @@ __global__ __launch_bounds__(256) void conv_fwd2(Conv g, const bf16 *__restrict_
         for (int n = 0; n < NT; n++) acc[rb][n] = float4v{0.f, 0.f, 0.f, 0.f};
 
     const int64_t wrow = (int64_t)27 * g.Cin;
-    const bf16 *wl = w + (int64_t)(n0 + (lane & 15)) * wrow + kq;  // this lane's B column
+    const bf16 *wl = w + (int64_t)(n0 + (lane & 15)) * wrow + qh * 8;  // this lane's B column
     const int rows = (g.TD + 2) * SH;
-    for (int c0 = 0; c0 < g.Cin; c0 += 16) {
+    // The reduction runs over units (16-channel chunk, kernel plane dz); split z of gridDim.z
+    // takes units [u0, u1).
+    const int units = g.Cin / 16 * 3;
+    const int u0 = (int)((int64_t)blockIdx.z * units / gridDim.z);
+    const int u1 = (int)((int64_t)(blockIdx.z + 1) * units / gridDim.z);
+    int staged = -1;
+    for (int u = u0; u < u1; u++) {
+        const int c0 = (u / 3) * 16, dz = u - (u / 3) * 3;
+        if (c0 != staged) {
+        staged = c0;
         __syncthreads();
         // Stage channels [c0, c0+16) of the neighbourhood, one row of SW positions at a time.
         for (int row = wave; row < rows; row += 4) {
@@ -127,31 +150,52 @@ __global__ __launch_bounds__(256) void conv_fwd2(Conv g, const bf16 *__restrict_
             const bool zy = z >= 0 && z < g.D && y >= 0 && y < g.H;
             const int64_t rbase = (((int64_t)b * g.D + z) * g.H + y) * g.W;
             for (int e = lane; e < 2 * SW; e += 64) {
-                const int px = e >> 1, half = e & 1;
+                const int half = e >= SW ? 1 : 0, px = e - half * SW;
                 const int xx = w0 + px - 1;
                 uint4 v = make_uint4(0, 0, 0, 0);
                 if (zy && xx >= 0 && xx < g.W)
                     v = *(const uint4 *)(x + (rbase + xx) * g.Cin + c0 + half * 8);
-                *(uint4 *)(smem + (row * SW + px) * 16 + half * 8) = v;
+                ((uint4 *)smem)[half * npos + row * SW + px] = v;
             }
         }
         __syncthreads();
+        }
 #pragma unroll
-        for (int t = 0; t < 27; t++) {
-            const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;
-            const int toff = (dz * SHW + dy * SW + dx) * 16;
-            short4v bv[NT];
+        for (int pr = 0; pr < 5; pr++) {
+            // Taps 2pr and 2pr+1 of this kernel plane in one K=32 step; the tenth is padding.
+            const int mt = 2 * pr + qt;
+            const bool tv = mt < 9;
+            const int toff = dz * SHW + (mt / 3) * SW + (mt % 3);
+            uint4 bv[NT];
 #pragma unroll
             for (int n = 0; n < NT; n++)
-                bv[n] = *(const short4v *)(wl + (int64_t)n * 16 * wrow + t * g.Cin + c0);
+                bv[n] = tv ? *(const uint4 *)(wl + (int64_t)n * 16 * wrow + (dz * 9 + mt) * g.Cin + c0)
+                           : make_uint4(0, 0, 0, 0);
 #pragma unroll
             for (int rb = 0; rb < RB; rb++) {
-                short4v a = *(const short4v *)(smem + base[rb] + toff);
-                if (!live[rb]) a = short4v{0, 0, 0, 0};
+                const uint4 a = tv && live[rb] ? sA[base[rb] + toff] : make_uint4(0, 0, 0, 0);
 #pragma unroll
-                for (int n = 0; n < NT; n++) acc[rb][n] = mfma16(a, bv[n], acc[rb][n]);
+                for (int n = 0; n < NT; n++) acc[rb][n] = mfma32(a, bv[n], acc[rb][n]);
             }
         }
+    }
+
+    if (part) {
+        // Split: fp32 partials [split][position][Cout], bias and activation left to the reduction.
+        float *pz = part + (int64_t)blockIdx.z * g.positions() * g.Cout;
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int q = wave * 16 * RB + rb * 16 + (lane >> 4) * 4 + r;
+                if (q >= TP) continue;
+                int qd, qh, qw;
+                box_coords(g, q, qd, qh, qw);
+                const int64_t pos = (((int64_t)b * g.D + d0 + qd) * g.H + h0 + qh) * g.W + w0 + qw;
+#pragma unroll
+                for (int n = 0; n < NT; n++) pz[pos * g.Cout + n0 + n * 16 + (lane & 15)] = acc[rb][n][r];
+            }
+        return;
     }
 
     // Epilogue through LDS: [position][NB] bf16, then 16-byte stores per position.
@@ -319,28 +363,94 @@ static Conv forward_box(const Conv &g, int m) {
 template <int NT, int RB>
 static void launch_fwd(const Conv &g, dim3 grid, size_t lds, hipStream_t stream, const bf16 *x,
                        const bf16 *w, const float *bias, bf16 *z, bf16 *a, bool gelu_out,
-                       int ostride, int ooff) {
-    conv_fwd2<NT, RB><<<grid, 256, lds, stream>>>(g, x, w, bias, z, a, gelu_out, ostride, ooff);
+                       int ostride, int ooff, float *part) {
+    conv_fwd2<NT, RB><<<grid, 256, lds, stream>>>(g, x, w, bias, z, a, gelu_out, ostride, ooff, part);
+}
+
+// Sums the split partials of conv_fwd2 in split order, adds the bias and writes z and act(z) as
+// bf16, eight channels (one 16-byte store) per thread.
+__global__ void conv_split_reduce(const float *__restrict__ part, int splits, int64_t P, int Cout,
+                                  const float *__restrict__ bias, bf16 *__restrict__ zout,
+                                  bf16 *__restrict__ aout, int gelu_out, int ostride, int ooff) {
+    const int G = Cout / 8;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < P * G;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t pos = i / G;
+        const int c = (int)(i - pos * G) * 8;
+        float v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = 0.0f;
+        for (int z = 0; z < splits; z++) {
+            const float *src = part + ((int64_t)z * P + pos) * Cout + c;
+            const float4 lo = *(const float4 *)src, hi = *(const float4 *)(src + 4);
+            v[0] += lo.x, v[1] += lo.y, v[2] += lo.z, v[3] += lo.w;
+            v[4] += hi.x, v[5] += hi.y, v[6] += hi.z, v[7] += hi.w;
+        }
+        if (bias) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) v[k] += bias[c + k];
+        }
+        bf16 zb[8], ab[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            zb[k] = to_bf16(v[k]);
+            ab[k] = to_bf16(gelu_out ? gelu(v[k]) : v[k]);
+        }
+        if (zout) *(uint4 *)(zout + pos * ostride + ooff + c) = *(const uint4 *)zb;
+        if (aout) *(uint4 *)(aout + pos * ostride + ooff + c) = *(const uint4 *)ab;
+    }
+}
+
+struct FwdPlan {
+    int rb, nt, splits;
+};
+
+// Most reuse first: 256-position boxes and the widest N-block.  While that launch would have
+// fewer than 1024 workgroups, split it over the reduction (when a workspace may be used), else
+// fall back to narrower N-blocks and 64-position boxes.
+static FwdPlan plan_fwd(const Conv &g0, bool can_split) {
+    const int nt_max = g0.Cout % 64 == 0 ? 4 : g0.Cout == 48 ? 3 : g0.Cout % 32 == 0 ? 2 : 1;
+    const Conv f = forward_box(g0, 256);
+    if (f.TW > 0 && f.TH > 0 && f.TD > 0) {
+        const int64_t base = f.tiles() * (g0.Cout / (16 * nt_max));
+        if (base >= 1024) return {4, nt_max, 1};
+        const int units = g0.Cin / 16 * 3;
+        const int splits = (int)std::min<int64_t>(units, (1024 + base - 1) / base);
+        if (can_split && splits > 1) return {4, nt_max, splits};
+    }
+    const int opts[4][2] = {{4, nt_max}, {4, 1}, {1, nt_max}, {1, 1}};
+    FwdPlan p{4, nt_max, 1};
+    int64_t best = -1;
+    for (const auto &o : opts) {
+        const Conv c = forward_box(g0, 64 * o[0]);
+        if (c.TW <= 0 || c.TH <= 0 || c.TD <= 0) continue;
+        const int64_t wgs = c.tiles() * (g0.Cout / (16 * o[1]));
+        if (wgs > best) best = wgs, p = {o[0], o[1], 1};
+        if (wgs >= 1024) return {o[0], o[1], 1};
+    }
+    return p;
+}
+
+int64_t conv_forward_workspace(const Conv &g0) {
+    const FwdPlan p = plan_fwd(g0, true);
+    return p.splits > 1 ? (int64_t)p.splits * g0.positions() * g0.Cout : 0;
+}
+
+Conv transposed(const Conv &g) {
+    Conv t = g;
+    t.Cin = g.Cout;
+    t.Cout = g.Cin;
+    return t;
 }
 
 int conv_forward(const Conv &g0, const bf16 *x, const bf16 *w, const float *bias, bf16 *z, bf16 *a,
-                 bool gelu_out, int ostride, int ooff, hipStream_t stream) {
-    // Most reuse first (256-position boxes, the widest N-block); fall back to narrower N-blocks
-    // and 64-position boxes while the launch would have fewer than 1024 workgroups.
-    const int nt_max = g0.Cout % 64 == 0 ? 4 : g0.Cout == 48 ? 3 : g0.Cout % 32 == 0 ? 2 : 1;
-    const int opts[4][2] = {{4, nt_max}, {4, 1}, {1, nt_max}, {1, 1}};
-    int rb = 4, nt = nt_max;
-    int64_t best = -1;
-    for (const auto &o : opts) {
-        const Conv f = forward_box(g0, 64 * o[0]);
-        if (f.TW <= 0 || f.TH <= 0 || f.TD <= 0) continue;
-        const int64_t wgs = f.tiles() * (g0.Cout / (16 * o[1]));
-        if (wgs > best) best = wgs, rb = o[0], nt = o[1];
-        if (wgs >= 1024) {
-            rb = o[0], nt = o[1];
-            break;
-        }
-    }
+                 bool gelu_out, int ostride, int ooff, hipStream_t stream, float *work,
+                 int64_t work_floats) {
+    FwdPlan plan = plan_fwd(g0, true);
+    if (plan.splits > 1 && (!work || work_floats < (int64_t)plan.splits * g0.positions() * g0.Cout))
+        plan = plan_fwd(g0, false);
+    const int rb = plan.rb, nt = plan.nt;
+    float *part = plan.splits > 1 ? work : nullptr;
     const int m = 64 * rb;
     const Conv g = forward_box(g0, m);
     const bool ok = g.B > 0 && g.Cin > 0 && g.Cout > 0 && g.Cin % 16 == 0 && g.Cout % 16 == 0 &&
@@ -356,31 +466,35 @@ int conv_forward(const Conv &g0, const bf16 *x, const bf16 *w, const float *bias
     const size_t stage = (size_t)(g.TD + 2) * (g.TH + 2) * (g.TW + 2) * 16 * sizeof(bf16);
     const size_t epi = (size_t)m * 16 * nt * sizeof(bf16);
     const size_t lds = std::max(stage, epi);
-    dim3 grid((unsigned)g.tiles(), (unsigned)(g.Cout / (16 * nt)));
+    dim3 grid((unsigned)g.tiles(), (unsigned)(g.Cout / (16 * nt)), (unsigned)plan.splits);
     const int key = nt * 10 + rb;
     switch (key) {
-        case 14: launch_fwd<1, 4>(g, grid, lds, stream, x, w, bias, z, a, gelu_out, ostride, ooff); break;
-        case 24: launch_fwd<2, 4>(g, grid, lds, stream, x, w, bias, z, a, gelu_out, ostride, ooff); break;
-        case 34: launch_fwd<3, 4>(g, grid, lds, stream, x, w, bias, z, a, gelu_out, ostride, ooff); break;
-        case 44: launch_fwd<4, 4>(g, grid, lds, stream, x, w, bias, z, a, gelu_out, ostride, ooff); break;
-        case 11: launch_fwd<1, 1>(g, grid, lds, stream, x, w, bias, z, a, gelu_out, ostride, ooff); break;
-        case 21: launch_fwd<2, 1>(g, grid, lds, stream, x, w, bias, z, a, gelu_out, ostride, ooff); break;
-        case 31: launch_fwd<3, 1>(g, grid, lds, stream, x, w, bias, z, a, gelu_out, ostride, ooff); break;
-        default: launch_fwd<4, 1>(g, grid, lds, stream, x, w, bias, z, a, gelu_out, ostride, ooff); break;
+        case 14: launch_fwd<1, 4>(g, grid, lds, stream, x, w, bias, z, a, gelu_out, ostride, ooff, part); break;
+        case 24: launch_fwd<2, 4>(g, grid, lds, stream, x, w, bias, z, a, gelu_out, ostride, ooff, part); break;
+        case 34: launch_fwd<3, 4>(g, grid, lds, stream, x, w, bias, z, a, gelu_out, ostride, ooff, part); break;
+        case 44: launch_fwd<4, 4>(g, grid, lds, stream, x, w, bias, z, a, gelu_out, ostride, ooff, part); break;
+        case 11: launch_fwd<1, 1>(g, grid, lds, stream, x, w, bias, z, a, gelu_out, ostride, ooff, part); break;
+        case 21: launch_fwd<2, 1>(g, grid, lds, stream, x, w, bias, z, a, gelu_out, ostride, ooff, part); break;
+        case 31: launch_fwd<3, 1>(g, grid, lds, stream, x, w, bias, z, a, gelu_out, ostride, ooff, part); break;
+        default: launch_fwd<4, 1>(g, grid, lds, stream, x, w, bias, z, a, gelu_out, ostride, ooff, part); break;
     }
     PM_HIP(hipGetLastError());
+    if (part) {
+        const int64_t items = g.positions() * (g.Cout / 8);
+        conv_split_reduce<<<blocks_for(items, 256), 256, 0, stream>>>(
+            part, plan.splits, g.positions(), g.Cout, bias, z, a, gelu_out, ostride, ooff);
+        PM_HIP(hipGetLastError());
+    }
     return PM_OK;
 }
 
 int conv_backward_data(const Conv &g, const bf16 *dy, const bf16 *w, bf16 *wf, bf16 *dx,
-                       hipStream_t stream) {
+                       hipStream_t stream, float *work, int64_t work_floats) {
     const int64_t n = (int64_t)g.Cout * 27 * g.Cin;
     flip_weights<<<blocks_for(n, 256), 256, 0, stream>>>(w, wf, g.Cin, g.Cout);
     PM_HIP(hipGetLastError());
-    Conv t = g;
-    t.Cin = g.Cout;
-    t.Cout = g.Cin;
-    return conv_forward(t, dy, wf, nullptr, nullptr, dx, false, t.Cout, 0, stream);
+    const Conv t = transposed(g);
+    return conv_forward(t, dy, wf, nullptr, nullptr, dx, false, t.Cout, 0, stream, work, work_floats);
 }
 
 int conv_backward_weight(const Conv &g, const bf16 *x, const bf16 *dy, float *dw, float *db,

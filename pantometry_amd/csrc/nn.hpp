@@ -68,11 +68,22 @@ inline Conv make_conv(int B, int D, int H, int W, int Cin, int Cout) {
 // y = conv(x, w) + bias; z (pre-activation) and a = act(z) stored as bf16 when non-null, with
 // act = GELU when `gelu_out`, identity otherwise.  Output channel c is written at
 // (position * ostride + ooff + c), so a convolution can write into a wider buffer.
+//
+// A grid too small to give 1024 workgroups with full-size MFMA blocks is split over the
+// reduction (16-channel chunk x kernel plane): each split writes fp32 partials to `work`, and a
+// reduction sums them in split order, so results do not depend on scheduling.  Without a
+// `work` of at least conv_forward_workspace(g) floats the launch narrows its blocks instead.
 int conv_forward(const Conv &g, const bf16 *x, const bf16 *w, const float *bias, bf16 *z,
-                 bf16 *a, bool gelu_out, int ostride, int ooff, hipStream_t stream);
+                 bf16 *a, bool gelu_out, int ostride, int ooff, hipStream_t stream,
+                 float *work = nullptr, int64_t work_floats = 0);
+// Floats of `work` conv_forward would use for this shape (0: it does not split).
+int64_t conv_forward_workspace(const Conv &g);
 // dx = conv(dy, flip(w)^T): the gradient of the input, fp32 accumulation, bf16 out.
 int conv_backward_data(const Conv &g, const bf16 *dy, const bf16 *w, bf16 *w_flip_scratch,
-                       bf16 *dx, hipStream_t stream);
+                       bf16 *dx, hipStream_t stream, float *work = nullptr,
+                       int64_t work_floats = 0);
+// The transposed convolution conv_backward_data runs (Cin and Cout exchanged).
+Conv transposed(const Conv &g);
 // dw[co][t][ci] = sum_p dy[p][co] * x[p + off(t)][ci] (fp32), db[co] = sum_p dy[p][co];
 // split over position tiles and reduced in a fixed order.  `scratch` holds
 // splits * Cout * (27 * Cin + 1) floats.

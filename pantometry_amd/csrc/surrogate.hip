@@ -477,8 +477,8 @@ struct pm_surrogate {
     int batch = 1;
     std::vector<ConvLayer> layers;
     int64_t nparams = 0, nwbf = 0;
-    DevBuf params, grads, m, v, wbf, wflip, wscratch;
-    int64_t wscratch_floats = 0;
+    DevBuf params, grads, m, v, wbf, wflip, wscratch, fsplit;
+    int64_t wscratch_floats = 0, fsplit_floats = 0;
     int64_t head_off = 0;  // 16 weights then 1 bias
     int adam_t = 0;
     hipStream_t stream = nullptr;
@@ -556,6 +556,15 @@ int size_buffers(pm_surrogate *s, int R, int B) {
         PM_CHECK(alloc_bf(s, npos(s, l + 1) * pool_c[l], &s->pool[l]));
         PM_CHECK(alloc_bf(s, npos(s, l + 1) * pool_c[l], &s->dpool[l]));
     }
+    // The split-reduction workspace of the small-grid convolutions, forward and data gradient.
+    int64_t need = 0;
+    for (int li = 0; li < 14; li++) {
+        const nn::Conv g = conv_of(s, li);
+        need = std::max(need, nn::conv_forward_workspace(g));
+        need = std::max(need, nn::conv_forward_workspace(nn::transposed(g)));
+    }
+    s->fsplit_floats = need;
+    PM_CHECK(s->fsplit.alloc(sizeof(float) * (size_t)std::max<int64_t>(need, 1)));
     const int64_t n0 = npos(s, 0);
     PM_CHECK(s->xin.alloc((size_t)n0 * 16 * sizeof(bf16)));
     PM_CHECK(s->y.alloc((size_t)n0 * sizeof(float)));
@@ -576,7 +585,8 @@ int run_forward(pm_surrogate *s, bool training) {
     auto conv = [&](int li, const bf16 *in, Act &o) -> int {
         const nn::Conv g = conv_of(s, li);
         return nn::conv_forward(g, in, WB + s->layers[li].wbf_off, P + s->layers[li].b_off,
-                                training ? o.z : nullptr, o.a, true, g.Cout, 0, st);
+                                training ? o.z : nullptr, o.a, true, g.Cout, 0, st,
+                                s->fsplit.as<float>(), s->fsplit_floats);
     };
     auto R = [&](int l) { return s->R >> l; };
     const int B = s->batch;
@@ -628,7 +638,8 @@ int conv_back(pm_surrogate *s, int li, const bf16 *in, bf16 *din) {
                                       s->wscratch.as<float>(), splits, st));
     if (din)
         PM_CHECK(nn::conv_backward_data(g, o.da, s->wbf.as<bf16>() + s->layers[li].wbf_off,
-                                        s->wflip.as<bf16>(), din, st));
+                                        s->wflip.as<bf16>(), din, st, s->fsplit.as<float>(),
+                                        s->fsplit_floats));
     return PM_OK;
 }
 
@@ -1034,7 +1045,7 @@ int pm_solid3d_relative_l2(pm_solid3d *a, pm_solid3d *b, double reference, doubl
 // Test entry points for one convolution on host fp32 arrays (converted to bf16 on the way in):
 // x [B][D][H][W][Cin], w [Cout][27][Cin], bias [Cout], y [B][D][H][W][Cout] (pre-activation).
 int pm_nn_conv3d(int B, int D, int H, int W, int Cin, int Cout, const float *x, const float *w,
-                 const float *bias, float *y) {
+                 const float *bias, float *y, int split) {
     PM_CHECK(require_device());
     const nn::Conv g = nn::make_conv(B, D, H, W, Cin, Cout);
     const int64_t nx = g.positions() * Cin, ny = g.positions() * Cout, nw = (int64_t)Cout * 27 * Cin;
@@ -1049,7 +1060,11 @@ int pm_nn_conv3d(int B, int D, int H, int W, int Cin, int Cout, const float *x, 
     PM_HIP(hipMemcpy(dx.d(), hx.data(), sizeof(bf16) * hx.size(), hipMemcpyHostToDevice));
     PM_HIP(hipMemcpy(dw.d(), hw.data(), sizeof(bf16) * hw.size(), hipMemcpyHostToDevice));
     PM_HIP(hipMemcpy(db.d(), bias, sizeof(float) * (size_t)Cout, hipMemcpyHostToDevice));
-    PM_CHECK(nn::conv_forward(g, dx.as<bf16>(), dw.as<bf16>(), db.as<float>(), dz.as<bf16>(), nullptr, false, Cout, 0, nullptr));
+    const int64_t nwork = split ? nn::conv_forward_workspace(g) : 0;
+    DevBuf work;
+    PM_CHECK(work.alloc(sizeof(float) * (size_t)std::max<int64_t>(nwork, 1)));
+    PM_CHECK(nn::conv_forward(g, dx.as<bf16>(), dw.as<bf16>(), db.as<float>(), dz.as<bf16>(), nullptr, false, Cout, 0, nullptr,
+                              work.as<float>(), nwork));
     std::vector<bf16> hz((size_t)ny);
     PM_HIP(hipMemcpy(hz.data(), dz.d(), sizeof(bf16) * hz.size(), hipMemcpyDeviceToHost));
     for (int64_t i = 0; i < ny; i++) y[i] = from_bf16(hz[(size_t)i]);
@@ -1059,7 +1074,8 @@ int pm_nn_conv3d(int B, int D, int H, int W, int Cin, int Cout, const float *x, 
 // Gradients of sum(y * dy) for the convolution above: dx [B][D][H][W][Cin], dw [Cout][27][Cin],
 // db [Cout].
 int pm_nn_conv3d_backward(int B, int D, int H, int W, int Cin, int Cout, const float *x,
-                          const float *w, const float *dy, float *dx, float *dw, float *db) {
+                          const float *w, const float *dy, float *dx, float *dw, float *db,
+                          int split) {
     PM_CHECK(require_device());
     const nn::Conv g = nn::make_conv(B, D, H, W, Cin, Cout);
     const int64_t nx = g.positions() * Cin, ny = g.positions() * Cout, nw = (int64_t)Cout * 27 * Cin;
@@ -1080,7 +1096,11 @@ int pm_nn_conv3d_backward(int B, int D, int H, int W, int Cin, int Cout, const f
     PM_HIP(hipMemcpy(ddx.d(), hx.data(), sizeof(bf16) * hx.size(), hipMemcpyHostToDevice));
     PM_HIP(hipMemcpy(ddw.d(), hw.data(), sizeof(bf16) * hw.size(), hipMemcpyHostToDevice));
     PM_HIP(hipMemcpy(ddy.d(), hd.data(), sizeof(bf16) * hd.size(), hipMemcpyHostToDevice));
-    PM_CHECK(nn::conv_backward_data(g, ddy.as<bf16>(), ddw.as<bf16>(), wf.as<bf16>(), gx.as<bf16>(), nullptr));
+    const int64_t nwork = split ? nn::conv_forward_workspace(nn::transposed(g)) : 0;
+    DevBuf work;
+    PM_CHECK(work.alloc(sizeof(float) * (size_t)std::max<int64_t>(nwork, 1)));
+    PM_CHECK(nn::conv_backward_data(g, ddy.as<bf16>(), ddw.as<bf16>(), wf.as<bf16>(), gx.as<bf16>(), nullptr,
+                                    work.as<float>(), nwork));
     PM_CHECK(nn::conv_backward_weight(g, ddx.as<bf16>(), ddy.as<bf16>(), gw.as<float>(), gb.as<float>(), scratch.as<float>(), splits, nullptr));
     std::vector<bf16> hgx((size_t)nx);
     PM_HIP(hipMemcpy(hgx.data(), gx.d(), sizeof(bf16) * hgx.size(), hipMemcpyDeviceToHost));

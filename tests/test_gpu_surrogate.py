@@ -53,22 +53,24 @@ def _fp(a):
     return fptr(a)
 
 
+@pytest.mark.parametrize("split", [1, 0], ids=["split", "narrow"])
 @pytest.mark.parametrize("shape", CONV_SHAPES, ids=lambda s: "x".join(map(str, s)))
-def test_conv_forward_matches_oracle(shape):
-    gpu_case(f"conv forward {shape}")
+def test_conv_forward_matches_oracle(shape, split):
+    gpu_case(f"conv forward {shape} split={split}")
     from pantometry_amd._capi import check, lib
     B, D, H, W, Cin, Cout = shape
     x, w, b, _ = _conv_inputs(shape, 1)
     y = np.empty((B, D, H, W, Cout), dtype=np.float32)
-    check(lib().pm_nn_conv3d(B, D, H, W, Cin, Cout, _fp(x), _fp(w), _fp(b), _fp(y)), "conv")
+    check(lib().pm_nn_conv3d(B, D, H, W, Cin, Cout, _fp(x), _fp(w), _fp(b), _fp(y), split), "conv")
     ref = U.conv3d(U.round_bf16(x), U.round_bf16(w), b.astype(np.float64))
     rms = math.sqrt(float(np.mean(ref ** 2)))
     assert np.all(np.abs(y - ref) <= 2.0 ** -8 * np.abs(ref) + 1e-4 * rms)
 
 
+@pytest.mark.parametrize("split", [1, 0], ids=["split", "narrow"])
 @pytest.mark.parametrize("shape", CONV_SHAPES, ids=lambda s: "x".join(map(str, s)))
-def test_conv_backward_matches_oracle(shape):
-    gpu_case(f"conv backward {shape}")
+def test_conv_backward_matches_oracle(shape, split):
+    gpu_case(f"conv backward {shape} split={split}")
     from pantometry_amd._capi import check, lib
     B, D, H, W, Cin, Cout = shape
     x, w, _, dy = _conv_inputs(shape, 2)
@@ -76,7 +78,7 @@ def test_conv_backward_matches_oracle(shape):
     dw = np.empty_like(w)
     db = np.empty(Cout, dtype=np.float32)
     check(lib().pm_nn_conv3d_backward(B, D, H, W, Cin, Cout, _fp(x), _fp(w), _fp(dy), _fp(dx),
-                                      _fp(dw), _fp(db)), "conv backward")
+                                      _fp(dw), _fp(db), split), "conv backward")
     rdx, rdw, rdb = U.conv3d_backward(U.round_bf16(x), U.round_bf16(w), U.round_bf16(dy))
     rms = math.sqrt(float(np.mean(rdx ** 2)))
     assert np.all(np.abs(dx - rdx) <= 2.0 ** -8 * np.abs(rdx) + 1e-4 * rms)

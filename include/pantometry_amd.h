@@ -50,12 +50,14 @@ typedef struct pm_violation {
 } pm_violation;
 
 /* The thermal part of Substance, crates/pantometry-core/src/substance.rs:43-239: density in
- * kg/m^3, conductivity in W/m/K, specific heat in J/kg/K.  NaN conductivity or specific heat is a
- * substance that does not say (Substance::thermal = None). */
+ * kg/m^3, conductivity in W/m/K, specific heat in J/kg/K, and the surface's emissivity (0..1,
+ * ThermalProps::emissivity) that an exposed face radiates with.  NaN conductivity or specific
+ * heat is a substance that does not say (Substance::thermal = None). */
 typedef struct pm_substance {
     double density;
     double conductivity;
     double specific_heat;
+    double emissivity;
 } pm_substance;
 
 /* The three entries of Solid3D::ledger, crates/pantometry-thermal/src/solid.rs:1904-1927:
@@ -161,6 +163,28 @@ int pm_solid3d_ledger(pm_solid3d *solid, pm_books *books);
 int pm_solid3d_advance(pm_solid3d *solid, double dt, uint64_t steps, double tolerance,
                        double *totals, double *elapsed_ms, pm_violation *v);
 /* Domain::checkpoint / Domain::restore, solid.rs:1935-1950. */
+/* An environment a face loses heat to (Environment, crates/pantometry-thermal/src/lib.rs:38-49):
+ * ambient temperature in kelvin, convective coefficient in W/m^2/K, and the face's whole area in
+ * m^2 (each boundary cell is charged area / cells on the face). */
+typedef struct pm_environment {
+    double ambient;
+    double convection_w_per_m2_k;
+    double area;
+} pm_environment;
+
+/* Solid3D::losing_from, crates/pantometry-thermal/src/solid.rs:1038-1042: face 0..5 (XMin, XMax,
+ * YMin, YMax, ZMin, ZMax) loses heat to `env` by convection and by radiation at each boundary
+ * cell's own temperature and its substance's emissivity, inside the sweep (film_flux,
+ * solid.rs:1192-1221); a later call for the same face replaces its environment.  The stability
+ * limit then depends on the state (worst_rate_now, solid.rs:1148-1185) and is read on the device
+ * before every step; what the film removes is kept in the `lost` entry of the books. */
+int pm_solid3d_losing_from(pm_solid3d *solid, int face, const pm_environment *env);
+/* Solid3D::dissipating, solid.rs:1058-1083: `watts` in total spread evenly over the cells `mask`
+ * selects (x-fastest bytes, nonzero = selected), replacing what they generated before; added
+ * after each sweep as (W*dt)/C and kept in the `supplied` entry of the books. */
+int pm_solid3d_dissipating(pm_solid3d *solid, double watts, const uint8_t *mask);
+/* Solid3D::generated_power, solid.rs:1119-1121: the watts all cells generate. */
+int pm_solid3d_generated_power(const pm_solid3d *solid, double *watts);
 int pm_solid3d_checkpoint(pm_solid3d *solid);
 int pm_solid3d_restore(pm_solid3d *solid);
 /* Domain::readings, solid.rs:1962-2012: peak, mean and coldest in degrees Celsius, then the

@@ -24,6 +24,7 @@ _DP = C.POINTER(C.c_double)
 _S = C.c_size_t
 _U64 = C.c_uint64
 _I = C.c_int
+_IP = C.POINTER(C.c_int)
 
 _SIG = {
     "orc_mix64": (_U64, [_U64]),
@@ -53,6 +54,14 @@ _SIG = {
     "orc_workload_hot_spot": (None, [_S, _S, _S, _D, _D, _D, _D, _U64, _DP, _I]),
     "orc_workload_heterogeneous": (None, [_S, _S, _S, _U64, _U64, _I, _DP, _DP, _I]),
     "orc_workload_member": (None, [_U64, _U64, _D, _D, _D, _D, C.POINTER(C.c_longlong), _DP]),
+    "orc_loss_from": (_D, [_D, _D, _D, _D, _D]),
+    "orc_series_with_half_cell": (_D, [_D, _D, _D, _D]),
+    "orc_film_flux": (_D, [_S, _S, _S, _D, _S, _S, _S, _D, _IP, _DP, _DP, _DP, _D, _D]),
+    "orc_loss_conductance": (_D, [_S, _S, _S, _D, _S, _S, _S, _D, _IP, _DP, _DP, _DP, _D, _D]),
+    "orc_worst_rate_now": (_D, [_S, _S, _S, _D, _D, _DP, _DP, _DP, _IP, _DP, _DP, _DP, _DP,
+                                _DP]),
+    "orc_solid_step_ext": (None, [_S, _S, _S, _D, _DP, _DP, _DP, _DP, _DP, _D, _DP, _DP, _IP,
+                                  _DP, _DP, _DP, _DP, _DP, _DP, _DP, _DP]),
 }
 
 _lib = None
@@ -85,6 +94,16 @@ def threads() -> int:
 
 def p(a: np.ndarray):
     return a.ctypes.data_as(_DP)
+
+
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(_IP)
+
+
+def _thermal_or(value: float, conductivity: float, missing: float) -> float:
+    """A thermal property, or what the reference uses when the substance has no thermal
+    properties at all (`thermal.map_or(missing, ...)`; a NaN conductivity marks that here)."""
+    return missing if math.isnan(conductivity) else float(value)
 
 
 # ---- Rng ----------------------------------------------------------------------------------
@@ -140,7 +159,8 @@ class OracleSolid3D:
     """Solid3D's plain branch on the CPU, as the reference computes it."""
 
     def __init__(self, name: str, density: float, conductivity: float, specific_heat: float,
-                 counts: Tuple[int, int, int], dx: float, initial: float):
+                 counts: Tuple[int, int, int], dx: float, initial: float,
+                 emissivity: float = 0.0):
         self.name_ = name
         self.counts_ = tuple(max(1, int(c)) for c in counts)
         nx, ny, nz = self.counts_
@@ -155,6 +175,17 @@ class OracleSolid3D:
         self.first = (density, conductivity, specific_heat)
         self.absorbed = 0.0
         self._old = np.empty(self.n)
+        # The film's per-cell inputs are the cell's substance's emissivity and conductivity
+        # (0 and +inf for a substance that does not say, solid.rs:1193-1195).
+        self.emis = np.full(self.n, _thermal_or(emissivity, conductivity, 0.0))
+        self.ksub = np.full(self.n, _thermal_or(conductivity, conductivity, math.inf))
+        self.on = np.zeros(6, dtype=np.int32)      # exposed faces, Face order
+        self.ambient = np.zeros(6)
+        self.h = np.zeros(6)
+        self.area = np.zeros(6)
+        self.source = np.zeros(self.n)             # watts per cell
+        self.lost = 0.0
+        self.supplied = 0.0
         self.resolve()
 
     def resolve(self) -> None:
@@ -168,12 +199,43 @@ class OracleSolid3D:
                                             p(self.kx), p(self.ky), p(self.kz), p(self.mob),
                                             p(self.face_sum))
 
-    def fill(self, density: float, conductivity: float, specific_heat: float, mask: np.ndarray):
+    def fill(self, density: float, conductivity: float, specific_heat: float, mask: np.ndarray,
+             emissivity: float = 0.0):
         m = np.asarray(mask).reshape(-1).astype(bool)
         self.kcell[m] = conductivity
         self.cap[m] = lib().orc_heat_capacity(density, specific_heat, self.dx)
+        self.emis[m] = _thermal_or(emissivity, conductivity, 0.0)
+        self.ksub[m] = _thermal_or(conductivity, conductivity, math.inf)
         self.resolve()
         return self
+
+    def losing_from(self, face: int, ambient: float, convection: float, area: float):
+        """Solid3D::losing_from, solid.rs:1038-1042: the face (0..5, XMin..ZMax) loses to an
+        environment; a later call for the same face replaces it."""
+        self.on[face] = 1
+        self.ambient[face] = ambient
+        self.h[face] = convection
+        self.area[face] = area
+        return self
+
+    def dissipating(self, watts: float, mask: np.ndarray):
+        """Solid3D::dissipating, solid.rs:1058-1083 (no void cells): `watts` spread evenly over the
+        selected cells, replacing what they generated before."""
+        m = np.asarray(mask).reshape(-1).astype(bool)
+        chosen = int(m.sum())
+        if chosen:
+            self.source[m] = watts / float(chosen)
+        return self
+
+    def exposed(self) -> bool:
+        return bool(self.on.any())
+
+    def worst_rate_now(self) -> float:
+        nx, ny, nz = self.counts_
+        return lib().orc_worst_rate_now(nx, ny, nz, self.dx, self.worst_rate, p(self.mob),
+                                        p(self.face_sum), p(self.cells), _ip(self.on),
+                                        p(self.ambient), p(self.h), p(self.area), p(self.emis),
+                                        p(self.ksub))
 
     def with_conductivity(self, k: np.ndarray):
         self.kcell = np.ascontiguousarray(k, dtype=np.float64).reshape(-1).copy()
@@ -184,12 +246,13 @@ class OracleSolid3D:
         self.cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1).copy()
 
     def max_stable_dt(self, now: float = 0.0) -> float:
-        if math.isnan(self.worst_rate):
+        rate = self.worst_rate_now()
+        if math.isnan(rate):
             return math.inf
-        return math.inf if self.worst_rate == 0.0 else 1.0 / self.worst_rate
+        return math.inf if rate == 0.0 else 1.0 / rate
 
     def refuses(self, dt: float) -> Optional[str]:
-        ratio = dt * self.worst_rate
+        ratio = dt * self.worst_rate_now()
         if math.isnan(ratio):
             return "substance has no diffusivity"
         if ratio > 1.0 + 6e-12:
@@ -204,7 +267,15 @@ class OracleSolid3D:
         if gained != 0.0:
             self.absorbed += gained
             lib().orc_solid_bus_heat(self.n, p(self.cap), gained, p(self.cells))
-        if threads_ > 1:
+        if self.exposed() or self.source.any():
+            lost, supplied = C.c_double(self.lost), C.c_double(self.supplied)
+            lib().orc_solid_step_ext(nx, ny, nz, self.dx, p(self.kx), p(self.ky), p(self.kz),
+                                     p(self.mob), p(self.cap), dt, p(self.cells), p(self._old),
+                                     _ip(self.on), p(self.ambient), p(self.h), p(self.area),
+                                     p(self.emis), p(self.ksub), p(self.source), C.byref(lost),
+                                     C.byref(supplied))
+            self.lost, self.supplied = lost.value, supplied.value
+        elif threads_ > 1:
             lib().orc_solid_step_omp(nx, ny, nz, p(self.kx), p(self.ky), p(self.kz), p(self.mob),
                                      dt, p(self.cells), p(self._old), threads_)
         else:
@@ -219,7 +290,7 @@ class OracleSolid3D:
                                                  self.reference)
 
     def ledger(self) -> Tuple[float, float]:
-        return ledger_total([self.stored_heat(), 0.0, -0.0])
+        return ledger_total([self.stored_heat(), self.lost, -self.supplied])
 
     def release_mode(self, mode, mean: float, amplitude: float) -> None:
         nx, ny, nz = self.counts_

@@ -187,6 +187,156 @@ void orc_solid_step_omp(size_t nx, size_t ny, size_t nz, const double *kx, const
                     orc_cell(nx, ny, nz, i, j, (size_t)k, kx, ky, kz, mobility, dts, old);
 }
 
+/* ---- Exposed faces and volumetric sources ------------------------------------------------ */
+
+/* STEFAN_BOLTZMANN, crates/pantometry-units/src/lib.rs:899. */
+#define ORC_SIGMA 5.670374419e-8
+
+/* x.powi(4) and x.powi(3) with a constant exponent, which Rust lowers to repeated squaring (the
+ * same products compiler-rt's __powidf2 forms): (x*x)*(x*x) and x*(x*x). */
+static inline double orc_pow4(double x) {
+    double x2 = x * x;
+    return x2 * x2;
+}
+static inline double orc_pow3(double x) { return x * (x * x); }
+
+/* Environment::loss_from, crates/pantometry-thermal/src/lib.rs:104-110. */
+double orc_loss_from(double t, double ambient, double h, double area, double emissivity) {
+    double convective = h * area * (t - ambient);
+    double radiative = emissivity * ORC_SIGMA * area * (orc_pow4(t) - orc_pow4(ambient));
+    return convective + radiative;
+}
+
+/* series_with_half_cell, solid.rs:110-122. */
+double orc_series_with_half_cell(double film, double conductivity, double area, double dx) {
+    if (film <= 0.0) return 0.0;
+    double half = 2.0 * conductivity * area / dx;
+    if (!isfinite(half)) return film;
+    if (half <= 0.0) return 0.0;
+    return 1.0 / (1.0 / film + 1.0 / half);
+}
+
+/* Face::holds, solid.rs:158-169; faces in the enum's order XMin, XMax, YMin, YMax, ZMin, ZMax. */
+static inline int orc_holds(int f, size_t i, size_t j, size_t k, size_t nx, size_t ny, size_t nz) {
+    switch (f) {
+        case 0: return i == 0;
+        case 1: return i + 1 == nx;
+        case 2: return j == 0;
+        case 3: return j + 1 == ny;
+        case 4: return k == 0;
+        default: return k + 1 == nz;
+    }
+}
+
+/* cells_on(face).max(1) as f64, solid.rs:1224-1231. */
+static inline double orc_cells_on(int f, size_t nx, size_t ny, size_t nz) {
+    size_t n = f < 2 ? ny * nz : f < 4 ? nx * nz : nx * ny;
+    return (double)(n > 1 ? n : 1);
+}
+
+/* Solid3D::film_flux, solid.rs:1192-1221, for cell (i,j,k) at temperature t.  `on`, `ambient`,
+ * `h`, `area` describe the six faces (the BTreeMap `exposed`, iterated in Face order);
+ * `emissivity` and `conductivity` are the cell's substance's (0 and +inf when it does not say). */
+double orc_film_flux(size_t nx, size_t ny, size_t nz, double dx, size_t i, size_t j, size_t k,
+                     double t, const int *on, const double *ambient, const double *h,
+                     const double *area, double emissivity, double conductivity) {
+    double out = 0.0;
+    for (int f = 0; f < 6; f++) {
+        if (!on[f] || !orc_holds(f, i, j, k, nx, ny, nz)) continue;
+        double share = area[f] / orc_cells_on(f, nx, ny, nz);
+        double gap = t - ambient[f];
+        if (gap == 0.0) continue;
+        double bare = orc_loss_from(t, ambient[f], h[f], share, emissivity);
+        double g = orc_series_with_half_cell(bare / gap, conductivity, share, dx);
+        out += g * gap / dx;
+    }
+    return out;
+}
+
+/* Solid3D::loss_conductance_at, solid.rs:1262-1283: the film's tangent conductance at
+ * max(at, ambient), in series with the half cell, summed over the exposed faces the cell is on. */
+double orc_loss_conductance(size_t nx, size_t ny, size_t nz, double dx, size_t i, size_t j,
+                            size_t k, double at, const int *on, const double *ambient,
+                            const double *h, const double *area, double emissivity,
+                            double conductivity) {
+    double sum = 0.0;
+    for (int f = 0; f < 6; f++) {
+        if (!on[f] || !orc_holds(f, i, j, k, nx, ny, nz)) continue;
+        double share = area[f] / orc_cells_on(f, nx, ny, nz);
+        double hot = fmax(at, ambient[f]);
+        double hh = h[f] + 4.0 * emissivity * ORC_SIGMA * orc_pow3(hot);
+        sum += orc_series_with_half_cell(hh * share, conductivity, share, dx);
+    }
+    return sum;
+}
+
+/* Solid3D::worst_rate_now, solid.rs:1148-1185, for a block with exposed faces and no gaps. */
+double orc_worst_rate_now(size_t nx, size_t ny, size_t nz, double dx, double worst_rate,
+                          const double *mobility, const double *face_sum, const double *cells,
+                          const int *on, const double *ambient, const double *h,
+                          const double *area, const double *emissivity,
+                          const double *conductivity) {
+    int any = 0;
+    for (int f = 0; f < 6; f++) any |= on[f];
+    if (!any || isnan(worst_rate)) return worst_rate;
+    double worst = worst_rate;
+    for (size_t k = 0; k < nz; k++)
+        for (size_t j = 0; j < ny; j++)
+            for (size_t i = 0; i < nx; i++) {
+                size_t c = i + nx * (j + ny * k);
+                double loss = orc_loss_conductance(nx, ny, nz, dx, i, j, k, cells[c], on, ambient,
+                                                   h, area, emissivity[c], conductivity[c]);
+                if (loss == 0.0) continue;
+                worst = fmax(worst, mobility[c] * (face_sum[c] + loss / dx));
+            }
+    return worst;
+}
+
+/* Solid3D::step's sweep with the film inside it and the sources after it, solid.rs:1767-1864
+ * (plain branch: no void, no gaps, no phase change).  `source` (watts per cell) may be NULL.
+ * Adds what the film removed to *lost and what the sources generated to *supplied, each a left
+ * fold in cell order as the reference accumulates them. */
+void orc_solid_step_ext(size_t nx, size_t ny, size_t nz, double dx, const double *kx,
+                        const double *ky, const double *kz, const double *mobility,
+                        const double *cap, double dts, double *cells, double *old, const int *on,
+                        const double *ambient, const double *h, const double *area,
+                        const double *emissivity, const double *conductivity,
+                        const double *source, double *lost, double *supplied) {
+    memcpy(old, cells, sizeof(double) * nx * ny * nz);
+    int exposed = 0;
+    for (int f = 0; f < 6; f++) exposed |= on[f];
+    for (size_t k = 0; k < nz; k++)
+        for (size_t j = 0; j < ny; j++)
+            for (size_t i = 0; i < nx; i++) {
+                size_t c = i + nx * (j + ny * k);
+                double t = old[c];
+                size_t xr = (nx + 1) * (j + ny * k), yr = nx * (j + (ny + 1) * k);
+                double flux = 0.0;
+                if (i > 0) flux += kx[i + xr] * (old[c - 1] - t);
+                if (i + 1 < nx) flux += kx[i + 1 + xr] * (old[c + 1] - t);
+                if (j > 0) flux += ky[i + yr] * (old[c - nx] - t);
+                if (j + 1 < ny) flux += ky[i + nx + yr] * (old[c + nx] - t);
+                if (k > 0) flux += kz[c] * (old[c - nx * ny] - t);
+                if (k + 1 < nz) flux += kz[c + nx * ny] * (old[c + nx * ny] - t);
+                double shed = exposed ? orc_film_flux(nx, ny, nz, dx, i, j, k, t, on, ambient, h,
+                                                      area, emissivity[c], conductivity[c])
+                                      : 0.0;
+                cells[c] += dts * mobility[c] * (flux - shed);
+                if (shed != 0.0) *lost += dts * mobility[c] * shed * cap[c];
+            }
+    if (!source) return;
+    size_t n = nx * ny * nz;
+    int supplies = 0;
+    for (size_t c = 0; c < n && !supplies; c++) supplies = source[c] != 0.0;
+    if (!supplies) return;
+    for (size_t c = 0; c < n; c++) {
+        if (source[c] == 0.0) continue;
+        double joules = source[c] * dts;
+        cells[c] += joules / cap[c];
+        *supplied += joules;
+    }
+}
+
 /* Placeless bus heat, solid.rs:1752-1765: a uniform rise gained / sum(capacity), the sum a left
  * fold in index order (`self.capacity.iter().sum::<f64>()`). */
 void orc_solid_bus_heat(size_t n, const double *cap, double gained, double *cells) {

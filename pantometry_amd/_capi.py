@@ -21,7 +21,12 @@ class pm_violation(C.Structure):
 
 class pm_substance(C.Structure):
     _fields_ = [("density", C.c_double), ("conductivity", C.c_double),
-                ("specific_heat", C.c_double)]
+                ("specific_heat", C.c_double), ("emissivity", C.c_double)]
+
+
+class pm_environment(C.Structure):
+    _fields_ = [("ambient", C.c_double), ("convection_w_per_m2_k", C.c_double),
+                ("area", C.c_double)]
 
 
 class pm_books(C.Structure):
@@ -72,6 +77,9 @@ SIGNATURES = {
     "pm_solid3d_ledger": (_I, [_VP, C.POINTER(pm_books)]),
     "pm_solid3d_advance": (_I, [_VP, _D, _U64, _D, _DP, _DP, _VIOL]),
     "pm_solid3d_checkpoint": (_I, [_VP]),
+    "pm_solid3d_losing_from": (_I, [_VP, _I, C.POINTER(pm_environment)]),
+    "pm_solid3d_dissipating": (_I, [_VP, _D, C.POINTER(C.c_uint8)]),
+    "pm_solid3d_generated_power": (_I, [_VP, _DP]),
     "pm_solid3d_restore": (_I, [_VP]),
     "pm_solid3d_readings": (_I, [_VP, _DP]),
     "pm_solid3d_heat_capacity": (_D, [_VP]),

@@ -24,6 +24,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstring>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -757,9 +759,28 @@ struct pm_solid3d {
     int64_t prof_used = 0;
     int prof_launch_steps = 1;    // steps per launch of the launches sweep_ms averages
     double sweep_ms = 0.0, audit_ms = 0.0;
+    // Exposed faces and volumetric sources, the film and source branches of the sweep
+    // (solid.rs:1192-1288, 1800-1864).  A block with either steps through step_ext.
+    struct Film {
+        int on = 0;
+        double ambient = 0.0, h = 0.0, area = 0.0;
+    };
+    Film film[6];                      // Face order: XMin, XMax, YMin, YMax, ZMin, ZMax
+    std::vector<pm_substance> subs;    // subs[0] is the constructor's
+    std::vector<uint16_t> which;       // each cell's substance; empty while all are subs[0]
+    std::map<int64_t, double> source;  // watts of each generating cell
+    DevBuf bcell, bemis, bkcond, shed, src_c, src_w, ext_part;
+    int64_t nboundary = 0, nsource = 0;
+    bool tables_valid = false;
 
     int64_t n() const { return nx * ny * nz; }
     double *T() const { return t[cur].d(); }
+    bool exposed() const {
+        for (const Film &f : film)
+            if (f.on) return true;
+        return false;
+    }
+    bool extended() const { return exposed() || !source.empty(); }
 };
 
 namespace pm {
@@ -1044,8 +1065,331 @@ static int total_capacity(pm_solid3d *s, double *out) {
     return PM_OK;
 }
 
-static int check_ratio(const pm_solid3d *s, double dt, pm_violation *v, int64_t step) {
-    const double ratio = dt * s->worst_rate;
+// ---- Exposed faces and volumetric sources ---------------------------------------------------
+//
+// The film acts only on the cells of exposed faces, so those cells are listed once (index order,
+// with their substance's emissivity and conductivity) and three small kernels serve them: the
+// state-dependent stability rate (solid.rs:1148-1185), the film's flux out of each listed cell
+// from the step's input (solid.rs:1192-1221), written into a dense `shed` array the sweep reads.
+// The sweep of a block with a film or a source is heat_step_ext, one thread per cell, with the
+// film inside the update and the sources added after it, as solid.rs:1767-1864 orders them.
+
+struct FilmFaces {
+    int on[6];
+    double ambient[6], h[6], share[6];  // share: the face's area over its cell count
+    double dx;
+};
+
+constexpr double STEFAN_BOLTZMANN = 5.670374419e-8;  // crates/pantometry-units/src/lib.rs:899
+
+// t.powi(4) and t.powi(3) with a constant exponent: repeated squaring, as Rust lowers them.
+__device__ __forceinline__ double pow4(double x) {
+    const double x2 = x * x;
+    return x2 * x2;
+}
+__device__ __forceinline__ double pow3(double x) { return x * (x * x); }
+
+// Environment::loss_from, crates/pantometry-thermal/src/lib.rs:104-110.
+__device__ __forceinline__ double loss_from(double t, double ambient, double h, double area,
+                                            double emissivity) {
+    const double convective = h * area * (t - ambient);
+    const double radiative = emissivity * STEFAN_BOLTZMANN * area * (pow4(t) - pow4(ambient));
+    return convective + radiative;
+}
+
+// series_with_half_cell, solid.rs:110-122.
+__device__ __forceinline__ double series_with_half_cell(double film, double conductivity,
+                                                        double area, double dx) {
+    if (film <= 0.0) return 0.0;
+    const double half = 2.0 * conductivity * area / dx;
+    if (!isfinite(half)) return film;
+    if (half <= 0.0) return 0.0;
+    return 1.0 / (1.0 / film + 1.0 / half);
+}
+
+// Face::holds, solid.rs:158-169.
+__device__ __forceinline__ bool holds(int f, int64_t i, int64_t j, int64_t k, int64_t nx,
+                                      int64_t ny, int64_t nz) {
+    switch (f) {
+        case 0: return i == 0;
+        case 1: return i + 1 == nx;
+        case 2: return j == 0;
+        case 3: return j + 1 == ny;
+        case 4: return k == 0;
+        default: return k + 1 == nz;
+    }
+}
+
+// The six faces of a cell in resolve's order and their left-fold sum (solid.rs:757-763).
+__device__ __forceinline__ double face_sum_of(const double *kc, double kface, int64_t c, int64_t i,
+                                              int64_t j, int64_t k, int64_t nx, int64_t ny,
+                                              int64_t nz) {
+    const int64_t plane = nx * ny;
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0, f3 = 0.0, f4 = 0.0, f5 = 0.0;
+    if (kc) {
+        const double kk = kc[c];
+        if (i > 0) f0 = series_nz(kc[c - 1], kk);
+        if (i + 1 < nx) f1 = series_nz(kk, kc[c + 1]);
+        if (j > 0) f2 = series_nz(kc[c - nx], kk);
+        if (j + 1 < ny) f3 = series_nz(kk, kc[c + nx]);
+        if (k > 0) f4 = series_nz(kc[c - plane], kk);
+        if (k + 1 < nz) f5 = series_nz(kk, kc[c + plane]);
+    } else {
+        f0 = i > 0 ? kface : 0.0, f1 = i + 1 < nx ? kface : 0.0;
+        f2 = j > 0 ? kface : 0.0, f3 = j + 1 < ny ? kface : 0.0;
+        f4 = k > 0 ? kface : 0.0, f5 = k + 1 < nz ? kface : 0.0;
+    }
+    return f0 + f1 + f2 + f3 + f4 + f5;
+}
+
+// The stability rate of each listed cell at its present temperature,
+// mob * (face_sum + loss_conductance_at(T)/dx), skipped where the loss is zero; per-block maxima
+// (NaN rates lose every comparison, as in f64::max).
+__global__ __launch_bounds__(256) void film_rates(FilmFaces F, const int64_t *__restrict__ bcell,
+                                                  const double *__restrict__ bemis,
+                                                  const double *__restrict__ bk, int64_t nb,
+                                                  const double *__restrict__ T,
+                                                  const double *__restrict__ kc, double kface,
+                                                  const double *__restrict__ mob, double mob0,
+                                                  int64_t nx, int64_t ny, int64_t nz,
+                                                  double *__restrict__ part) {
+    __shared__ double red[256];
+    double worst = -INFINITY;
+    for (int64_t b = blockIdx.x * 256 + threadIdx.x; b < nb; b += (int64_t)gridDim.x * 256) {
+        const int64_t c = bcell[b];
+        const int64_t i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+        const double at = T[c];
+        double loss = 0.0;
+        for (int f = 0; f < 6; f++) {
+            if (!F.on[f] || !holds(f, i, j, k, nx, ny, nz)) continue;
+            const double hot = fmax(at, F.ambient[f]);
+            const double h = F.h[f] + 4.0 * bemis[b] * STEFAN_BOLTZMANN * pow3(hot);
+            loss += series_with_half_cell(h * F.share[f], bk[b], F.share[f], F.dx);
+        }
+        if (loss == 0.0) continue;
+        const double rate = (mob ? mob[c] : mob0) * (face_sum_of(kc, kface, c, i, j, k, nx, ny, nz) + loss / F.dx);
+        worst = fmax(worst, rate);
+    }
+    red[threadIdx.x] = worst;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// Solid3D::film_flux of every listed cell, from the step's input temperatures.
+__global__ __launch_bounds__(256) void film_shed(FilmFaces F, const int64_t *__restrict__ bcell,
+                                                 const double *__restrict__ bemis,
+                                                 const double *__restrict__ bk, int64_t nb,
+                                                 const double *__restrict__ T,
+                                                 double *__restrict__ shed, int64_t nx, int64_t ny,
+                                                 int64_t nz) {
+    for (int64_t b = blockIdx.x * 256 + threadIdx.x; b < nb; b += (int64_t)gridDim.x * 256) {
+        const int64_t c = bcell[b];
+        const int64_t i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+        const double t = T[c];
+        double out = 0.0;
+        for (int f = 0; f < 6; f++) {
+            if (!F.on[f] || !holds(f, i, j, k, nx, ny, nz)) continue;
+            const double share = F.share[f];
+            const double gap = t - F.ambient[f];
+            if (gap == 0.0) continue;
+            const double bare = loss_from(t, F.ambient[f], F.h[f], share, bemis[b]);
+            const double g = series_with_half_cell(bare / gap, bk[b], share, F.dx);
+            out += g * gap / F.dx;
+        }
+        shed[c] = out;
+    }
+}
+
+// One step of a block with a film or sources: the plain sweep term by term (faces recomputed
+// from the per-cell conductivity with series, or the uniform face), the film's shed inside the
+// update, rise = (dt*mob)*(flux - shed) (solid.rs:1776-1818).  What the film removed,
+// ((dt*mob)*shed)*C per cell (solid.rs:1821-1823), is summed per block in a fixed order.
+__global__ __launch_bounds__(256) void heat_step_ext(const double *__restrict__ told,
+                                                     double *__restrict__ tnew,
+                                                     const double *__restrict__ kc, double kface,
+                                                     const double *__restrict__ mob, double mob0,
+                                                     const double *__restrict__ cap, double cap0,
+                                                     const double *__restrict__ shed, double dts,
+                                                     int64_t nx, int64_t ny, int64_t nz,
+                                                     double *__restrict__ lost_part) {
+    __shared__ double red[4];
+    const int64_t plane = nx * ny, n = plane * nz;
+    double lost = 0.0;
+    for (int64_t c = blockIdx.x * 256 + threadIdx.x; c < n; c += (int64_t)gridDim.x * 256) {
+        const int64_t i = c % nx, j = (c / nx) % ny, k = c / plane;
+        const double t = told[c];
+        const double kk = kc ? kc[c] : 0.0;
+        double flux = 0.0;
+        if (i > 0) flux = flux + (kc ? series_nz(kc[c - 1], kk) : kface) * (told[c - 1] - t);
+        if (i + 1 < nx) flux = flux + (kc ? series_nz(kk, kc[c + 1]) : kface) * (told[c + 1] - t);
+        if (j > 0) flux = flux + (kc ? series_nz(kc[c - nx], kk) : kface) * (told[c - nx] - t);
+        if (j + 1 < ny) flux = flux + (kc ? series_nz(kk, kc[c + nx]) : kface) * (told[c + nx] - t);
+        if (k > 0) flux = flux + (kc ? series_nz(kc[c - plane], kk) : kface) * (told[c - plane] - t);
+        if (k + 1 < nz) flux = flux + (kc ? series_nz(kk, kc[c + plane]) : kface) * (told[c + plane] - t);
+        const double sh = shed ? shed[c] : 0.0;
+        const double dtm = dts * (mob ? mob[c] : mob0);
+        tnew[c] = t + dtm * (flux - sh);
+        if (sh != 0.0) lost += dtm * sh * (cap ? cap[c] : cap0);
+    }
+    const double total = block_sum<256>(lost, red);
+    if (threadIdx.x == 0) lost_part[blockIdx.x] = total;
+}
+
+// The sources, after the sweep: each generating cell gains (W*dt)/C (solid.rs:1854-1864).
+__global__ void heat_sources(double *__restrict__ T, const int64_t *__restrict__ sc,
+                             const double *__restrict__ sw, int64_t ns,
+                             const double *__restrict__ cap, double cap0, double dts) {
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < ns;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = sc[q];
+        const double joules = sw[q] * dts;
+        T[c] = T[c] + joules / (cap ? cap[c] : cap0);
+    }
+}
+
+constexpr int EXT_BLOCKS = 1024;  // fixed grid of the extended sweep, so its sums are reproducible
+
+static FilmFaces film_faces(const pm_solid3d *s) {
+    FilmFaces F{};
+    for (int f = 0; f < 6; f++) {
+        const int64_t cells = f < 2 ? s->ny * s->nz : f < 4 ? s->nx * s->nz : s->nx * s->ny;
+        F.on[f] = s->film[f].on;
+        F.ambient[f] = s->film[f].ambient;
+        F.h[f] = s->film[f].h;
+        // env.area / cells_on(face).max(1) as f64, solid.rs:1201
+        F.share[f] = s->film[f].area / (double)std::max<int64_t>(cells, 1);
+    }
+    F.dx = s->dx;
+    return F;
+}
+
+// The listed cells of the exposed faces and the generating cells, on the device.
+static int ensure_tables(pm_solid3d *s) {
+    if (s->tables_valid) return PM_OK;
+    const int64_t nx = s->nx, ny = s->ny, nz = s->nz;
+    std::vector<int64_t> cells;
+    for (int f = 0; f < 6; f++) {
+        if (!s->film[f].on) continue;
+        const int axis = f / 2;
+        const int64_t at = (f % 2 == 0) ? 0 : (axis == 0 ? nx : axis == 1 ? ny : nz) - 1;
+        for (int64_t k = 0; k < nz; k++)
+            for (int64_t j = 0; j < ny; j++)
+                for (int64_t i = 0; i < nx; i++) {
+                    const int64_t q = axis == 0 ? i : axis == 1 ? j : k;
+                    if (q == at) cells.push_back(i + nx * (j + ny * k));
+                }
+    }
+    std::sort(cells.begin(), cells.end());
+    cells.erase(std::unique(cells.begin(), cells.end()), cells.end());
+    s->nboundary = (int64_t)cells.size();
+    if (!cells.empty()) {
+        std::vector<double> emis(cells.size()), kcond(cells.size());
+        for (size_t b = 0; b < cells.size(); b++) {
+            const pm_substance &sub = s->subs[s->which.empty() ? 0 : s->which[(size_t)cells[b]]];
+            // thermal.map_or(0.0, emissivity) and thermal.map_or(INFINITY, conductivity),
+            // solid.rs:1193-1195: a substance that does not say has neither.
+            const bool thermal = !std::isnan(sub.conductivity) && !std::isnan(sub.specific_heat);
+            emis[b] = thermal ? sub.emissivity : 0.0;
+            kcond[b] = thermal ? sub.conductivity : INFINITY;
+        }
+        PM_CHECK(s->bcell.alloc(sizeof(int64_t) * cells.size()));
+        PM_CHECK(s->bemis.alloc(sizeof(double) * cells.size()));
+        PM_CHECK(s->bkcond.alloc(sizeof(double) * cells.size()));
+        PM_HIP(hipMemcpy(s->bcell.d(), cells.data(), sizeof(int64_t) * cells.size(), hipMemcpyHostToDevice));
+        PM_HIP(hipMemcpy(s->bemis.d(), emis.data(), sizeof(double) * cells.size(), hipMemcpyHostToDevice));
+        PM_HIP(hipMemcpy(s->bkcond.d(), kcond.data(), sizeof(double) * cells.size(), hipMemcpyHostToDevice));
+        if (!s->shed) {
+            PM_CHECK(s->shed.alloc(sizeof(double) * (size_t)s->n()));
+            PM_HIP(hipMemset(s->shed.d(), 0, sizeof(double) * (size_t)s->n()));
+        }
+    }
+    s->nsource = (int64_t)s->source.size();
+    if (s->nsource) {
+        std::vector<int64_t> sc;
+        std::vector<double> sw;
+        for (const auto &e : s->source) sc.push_back(e.first), sw.push_back(e.second);
+        PM_CHECK(s->src_c.alloc(sizeof(int64_t) * sc.size()));
+        PM_CHECK(s->src_w.alloc(sizeof(double) * sw.size()));
+        PM_HIP(hipMemcpy(s->src_c.d(), sc.data(), sizeof(int64_t) * sc.size(), hipMemcpyHostToDevice));
+        PM_HIP(hipMemcpy(s->src_w.d(), sw.data(), sizeof(double) * sw.size(), hipMemcpyHostToDevice));
+    }
+    if (!s->ext_part) PM_CHECK(s->ext_part.alloc(sizeof(double) * EXT_BLOCKS));
+    s->tables_valid = true;
+    return PM_OK;
+}
+
+// Solid3D::worst_rate_now, solid.rs:1148-1185: the resolved limit, raised by each exposed cell's
+// film at its present temperature.  Synchronises the handle's stream.
+static int worst_rate_now(pm_solid3d *s, double *out) {
+    *out = s->worst_rate;
+    if (!s->exposed() || std::isnan(s->worst_rate)) return PM_OK;
+    PM_CHECK(ensure_tables(s));
+    if (s->nboundary == 0) return PM_OK;
+    const int blocks = (int)std::min<int64_t>(EXT_BLOCKS, (s->nboundary + 255) / 256);
+    film_rates<<<blocks, 256, 0, s->stream>>>(
+        film_faces(s), s->bcell.as<int64_t>(), s->bemis.d(), s->bkcond.d(), s->nboundary, s->T(),
+        s->kmode == K_PER_CELL ? s->kcell.d() : nullptr, s->kface,
+        s->cmode == C_PER_CELL ? s->mob.d() : nullptr, s->mob0, s->nx, s->ny, s->nz,
+        s->ext_part.d());
+    PM_HIP(hipGetLastError());
+    std::vector<double> part((size_t)blocks);
+    PM_HIP(hipMemcpyAsync(part.data(), s->ext_part.d(), sizeof(double) * part.size(), hipMemcpyDeviceToHost, s->stream));
+    PM_HIP(hipStreamSynchronize(s->stream));
+    double worst = s->worst_rate;
+    for (double p : part) worst = std::fmax(worst, p);
+    *out = worst;
+    return PM_OK;
+}
+
+// One step of a block with a film or sources (the caller has checked the limit and applied the
+// bus heat): the film's shed from the input, the sweep into the other buffer, the sources, and
+// the books: `lost` by the fixed-order sum of what the film removed, `supplied` by the
+// reference's own left fold over the generating cells (solid.rs:1856-1862).
+static int step_ext(pm_solid3d *s, double dt) {
+    PM_CHECK(ensure_tables(s));
+    const bool film = s->exposed() && s->nboundary > 0;
+    if (film) {
+        const int blocks = (int)std::min<int64_t>(EXT_BLOCKS, (s->nboundary + 255) / 256);
+        film_shed<<<blocks, 256, 0, s->stream>>>(film_faces(s), s->bcell.as<int64_t>(),
+                                                 s->bemis.d(), s->bkcond.d(), s->nboundary,
+                                                 s->T(), s->shed.d(), s->nx, s->ny, s->nz);
+        PM_HIP(hipGetLastError());
+    }
+    const double *kc = s->kmode == K_PER_CELL ? s->kcell.d() : nullptr;
+    const double *mob = s->cmode == C_PER_CELL ? s->mob.d() : nullptr;
+    const double *cap = s->cmode == C_PER_CELL ? s->cap.d() : nullptr;
+    double *out = s->t[s->cur ^ 1].d();
+    heat_step_ext<<<EXT_BLOCKS, 256, 0, s->stream>>>(s->T(), out, kc, s->kface, mob, s->mob0, cap,
+                                                     s->cap0, film ? s->shed.d() : nullptr, dt,
+                                                     s->nx, s->ny, s->nz, s->ext_part.d());
+    PM_HIP(hipGetLastError());
+    if (s->nsource) {
+        heat_sources<<<grid_for(s->nsource, 256), 256, 0, s->stream>>>(
+            out, s->src_c.as<int64_t>(), s->src_w.d(), s->nsource, cap, s->cap0, dt);
+        PM_HIP(hipGetLastError());
+    }
+    std::vector<double> part(EXT_BLOCKS);
+    PM_HIP(hipMemcpyAsync(part.data(), s->ext_part.d(), sizeof(double) * EXT_BLOCKS, hipMemcpyDeviceToHost, s->stream));
+    PM_HIP(hipStreamSynchronize(s->stream));
+    s->cur ^= 1;
+    double lost = 0.0;
+    for (double p : part) lost += p;
+    s->lost += lost;
+    for (const auto &e : s->source) s->supplied += e.second * dt;
+    return PM_OK;
+}
+
+static int check_ratio(const pm_solid3d *cs, double dt, pm_violation *v, int64_t step) {
+    // The limit of a block with an exposed face depends on its state; reading it runs a kernel
+    // on the handle's stream, which a const query may do (Domain::max_stable_dt is &self).
+    pm_solid3d *s = const_cast<pm_solid3d *>(cs);
+    double rate = s->worst_rate;
+    PM_CHECK(worst_rate_now(s, &rate));
+    const double ratio = dt * rate;
     std::string site = s->name + " (explicit 3D conduction)";
     if (std::isnan(ratio)) {
         violation_at(v, s->name.c_str(), "substance has no diffusivity", INFINITY, step);
@@ -1113,6 +1457,7 @@ int pm_solid3d_create(const char *name, const pm_substance *substance, size_t nx
     }
     std::unique_ptr<pm_solid3d> s(new pm_solid3d());
     s->name = name ? name : "block";
+    s->subs.push_back(*substance);
     s->nx = (int64_t)nx, s->ny = (int64_t)ny, s->nz = (int64_t)nz;
     s->dx = dx;
     s->reference = initial;
@@ -1171,6 +1516,26 @@ int pm_solid3d_fill(pm_solid3d *s, const pm_substance *sub, const uint8_t *mask)
     if (!any) return PM_OK;
     bool all = true;
     for (int64_t i = 0; i < n && all; i++) all = mask[i] != 0;
+    // Which substance each cell holds, for the film's emissivity and conductivity.
+    if (all) {
+        s->subs.assign(1, *sub);
+        s->which.clear();
+    } else {
+        size_t id = 0;
+        while (id < s->subs.size() && std::memcmp(&s->subs[id], sub, sizeof *sub) != 0) id++;
+        if (id == s->subs.size()) {
+            if (id > 65535) {
+                set_error("pm_solid3d_fill: more than 65536 substances in one block");
+                return PM_ERR_ARGUMENT;
+            }
+            s->subs.push_back(*sub);
+        }
+        if (s->which.empty() && id != 0) s->which.assign((size_t)n, 0);
+        if (!s->which.empty())
+            for (int64_t i = 0; i < n; i++)
+                if (mask[i]) s->which[(size_t)i] = (uint16_t)id;
+    }
+    s->tables_valid = false;
     const bool k_same = s->kmode == K_UNIFORM && same_bits(k, s->k0);
     const bool c_same = s->cmode == C_UNIFORM && same_bits(c, s->cap0);
     if (all && (s->kmode == K_UNIFORM) && (s->cmode == C_UNIFORM)) {
@@ -1262,13 +1627,17 @@ int pm_solid3d_deposit(pm_solid3d *s, size_t i, size_t j, size_t k, double joule
 
 double pm_solid3d_max_stable_dt(const pm_solid3d *s) {
     if (!s) return NAN;
-    if (std::isnan(s->worst_rate)) return INFINITY;
-    return 1.0 / s->worst_rate;
+    double rate = s->worst_rate;
+    if (worst_rate_now(const_cast<pm_solid3d *>(s), &rate) != PM_OK) return NAN;
+    if (std::isnan(rate)) return INFINITY;
+    return 1.0 / rate;
 }
 
 double pm_solid3d_stability_ratio(const pm_solid3d *s, double dt) {
     if (!s) return NAN;
-    return dt * s->worst_rate;
+    double rate = s->worst_rate;
+    if (worst_rate_now(const_cast<pm_solid3d *>(s), &rate) != PM_OK) return NAN;
+    return dt * rate;
 }
 
 int pm_solid3d_refuses(const pm_solid3d *s, double dt, pm_violation *v) {
@@ -1287,6 +1656,7 @@ int pm_solid3d_step(pm_solid3d *s, double dt, double gained, pm_violation *v) {
         add_uniform<<<grid_for(s->n(), 256), 256, 0, s->stream>>>(s->T(), s->n(), rise);
         PM_HIP(hipGetLastError());
     }
+    if (s->extended()) return step_ext(s, dt);
     PM_CHECK(reset_audit(s, 0.0, 0.0, INFINITY));
     PM_CHECK(sweep_once(s, dt, -1, nullptr));
     PM_HIP(hipStreamSynchronize(s->stream));
@@ -1303,9 +1673,52 @@ int pm_solid3d_ledger(pm_solid3d *s, pm_books *books) {
     return PM_OK;
 }
 
+// advance for a block with a film or sources: the limit is checked before every step (it moves
+// with the state), each step runs through step_ext, and the books are audited on the host after
+// every step, stored + lost - supplied against the step before (conserved.rs:323-363).  A
+// refusal or a breach stops the run with the state the last step left.
+static int advance_ext(pm_solid3d *s, double dt, uint64_t steps, double tolerance, double *totals,
+                       double *elapsed_ms, pm_violation *v) {
+    double stored = 0.0;
+    PM_CHECK(heat_stored(s, &stored));
+    auto books = [&](double st, double *total, double *scale) {
+        *total = 0.0;
+        *total += st;
+        *total += s->lost;
+        *total += -s->supplied;
+        *scale = rmax(rmax(rmax(0.0, std::fabs(st)), std::fabs(s->lost)), std::fabs(s->supplied));
+    };
+    double before = 0.0, sb = 0.0;
+    books(stored, &before, &sb);
+    PM_HIP(hipEventRecord(s->ev0, s->stream));
+    for (uint64_t n = 0; n < steps; n++) {
+        PM_CHECK(check_ratio(s, dt, v, (int64_t)n));
+        PM_CHECK(step_ext(s, dt));
+        PM_CHECK(heat_stored(s, &stored));
+        double after = 0.0, sa = 0.0;
+        books(stored, &after, &sa);
+        if (totals) totals[n] = after;
+        const double scale = rmax(rmax(rmax(std::fabs(before), std::fabs(after)), sb), sa);
+        if (scale >= 1e-300 && std::fabs(after - before) / scale > tolerance) {
+            fill_violation(v, "energy", "simulation", before, after, scale, tolerance, (int64_t)n);
+            return PM_VIOLATION;
+        }
+        before = after, sb = sa;
+    }
+    PM_HIP(hipEventRecord(s->ev1, s->stream));
+    PM_HIP(hipStreamSynchronize(s->stream));
+    if (elapsed_ms) {
+        float ms = 0.0f;
+        PM_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        *elapsed_ms = ms;
+    }
+    return PM_OK;
+}
+
 int pm_solid3d_advance(pm_solid3d *s, double dt, uint64_t steps, double tolerance,
                        double *totals, double *elapsed_ms, pm_violation *v) {
     if (!s) return PM_ERR_ARGUMENT;
+    if (s->extended()) return advance_ext(s, dt, steps, tolerance, totals, elapsed_ms, v);
     PM_CHECK(check_ratio(s, dt, v, 0));
     // The opening ledger.
     double stored0 = 0.0;
@@ -1380,6 +1793,48 @@ int pm_solid3d_advance(pm_solid3d *s, double dt, uint64_t steps, double toleranc
         fill_violation(v, "energy", "simulation", st.before, st.after, st.scale, tolerance, st.breach_step);
         return PM_VIOLATION;
     }
+    return PM_OK;
+}
+
+int pm_solid3d_losing_from(pm_solid3d *s, int face, const pm_environment *env) {
+    if (!s || !env || face < 0 || face > 5) {
+        set_error("pm_solid3d_losing_from: face must be 0..5 (XMin, XMax, YMin, YMax, ZMin, ZMax)");
+        return PM_ERR_ARGUMENT;
+    }
+    s->film[face].on = 1;
+    s->film[face].ambient = env->ambient;
+    s->film[face].h = env->convection_w_per_m2_k;
+    s->film[face].area = env->area;
+    s->tables_valid = false;
+    return PM_OK;
+}
+
+int pm_solid3d_dissipating(pm_solid3d *s, double watts, const uint8_t *mask) {
+    if (!s || !mask) {
+        set_error("pm_solid3d_dissipating: null argument");
+        return PM_ERR_ARGUMENT;
+    }
+    int64_t chosen = 0;
+    for (int64_t c = 0; c < s->n(); c++) chosen += mask[c] != 0;
+    if (chosen == 0) return PM_OK;
+    const double each = watts / (double)chosen;
+    for (int64_t c = 0; c < s->n(); c++) {
+        if (!mask[c]) continue;
+        if (each == 0.0)
+            s->source.erase(c);
+        else
+            s->source[c] = each;
+    }
+    s->tables_valid = false;
+    return PM_OK;
+}
+
+int pm_solid3d_generated_power(const pm_solid3d *s, double *watts) {
+    if (!s || !watts) return PM_ERR_ARGUMENT;
+    // `self.source.iter().sum()` over every cell in index order; zeros add nothing.
+    double total = 0.0;
+    for (const auto &e : s->source) total += e.second;
+    *watts = total;
     return PM_OK;
 }
 

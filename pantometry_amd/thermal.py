@@ -9,6 +9,7 @@ from __future__ import annotations
 import ctypes as C
 import math
 from dataclasses import dataclass
+from enum import IntEnum
 from typing import Callable, Optional, Tuple, Union
 
 import numpy as np
@@ -39,18 +40,28 @@ class Substance:
     density: float
     conductivity: Optional[float]
     specific_heat: Optional[float]
+    emissivity: float = 0.0
 
     @staticmethod
     def aluminium_6061() -> "Substance":
-        return Substance("Al 6061", 2.70 * 1e3, 167.0, 896.0)       # substance.rs:362-382
+        return Substance("Al 6061", 2.70 * 1e3, 167.0, 896.0, 0.09)  # substance.rs:362-382
 
     @staticmethod
     def borosilicate_crown() -> "Substance":
-        return Substance("N-BK7", 2.51 * 1e3, 1.114, 858.0)         # substance.rs:336-357
+        return Substance("N-BK7", 2.51 * 1e3, 1.114, 858.0, 0.90)    # substance.rs:336-357
 
     @staticmethod
-    def of(name: str, density: float, conductivity: float, specific_heat: float) -> "Substance":
-        return Substance(name, density, conductivity, specific_heat)
+    def of(name: str, density: float, conductivity: float, specific_heat: float,
+           emissivity: float = 0.0) -> "Substance":
+        return Substance(name, density, conductivity, specific_heat, emissivity)
+
+    def with_emissivity(self, emissivity: float) -> "Substance":
+        """The same substance with another surface finish, clamped to 0..1 (substance.rs:398-403);
+        a substance with no thermal properties is returned unchanged."""
+        if self.conductivity is None or self.specific_heat is None:
+            return self
+        return Substance(self.name, self.density, self.conductivity, self.specific_heat,
+                         min(max(emissivity, 0.0), 1.0))
 
     def diffusivity(self) -> Optional[float]:
         """k/(rho*c_p), substance.rs:211-216."""
@@ -68,7 +79,36 @@ class Substance:
         return _capi.pm_substance(
             self.density,
             math.nan if self.conductivity is None else self.conductivity,
-            math.nan if self.specific_heat is None else self.specific_heat)
+            math.nan if self.specific_heat is None else self.specific_heat,
+            self.emissivity)
+
+
+class Face(IntEnum):
+    """One outer face of a block, solid.rs:128-150, in the enum's order."""
+    XMIN = 0
+    XMAX = 1
+    YMIN = 2
+    YMAX = 3
+    ZMIN = 4
+    ZMAX = 5
+
+
+@dataclass(frozen=True)
+class Environment:
+    """What a face loses heat to (crates/pantometry-thermal/src/lib.rs:38-49): the ambient in
+    kelvin, the convective coefficient in W/m^2/K and the whole face's area in m^2."""
+
+    ambient: float
+    convection_w_per_m2_k: float
+    area: float
+
+    @staticmethod
+    def still_air(ambient: float, area: float) -> "Environment":
+        """A part in still room air, h = 7 W/m^2/K (lib.rs:52-59)."""
+        return Environment(ambient, 7.0, area)
+
+    def _c(self) -> _capi.pm_environment:
+        return _capi.pm_environment(self.ambient, self.convection_w_per_m2_k, self.area)
 
 
 def _phase(m: int, n: int, idx: np.ndarray) -> np.ndarray:
@@ -280,6 +320,44 @@ class Solid3D:
         gained = bus.take_share(HEAT, dt)
         check(lib().pm_solid3d_step(self._h, float(dt), float(gained), C.byref(v)),
               "pm_solid3d_step", v)
+
+    def losing_from(self, face: Face, environment: Environment) -> "Solid3D":
+        """Solid3D::losing_from (solid.rs:1038-1042): the face loses heat to the environment by
+        convection and radiation, inside the sweep; a later call for the face replaces it."""
+        env = environment._c()
+        check(lib().pm_solid3d_losing_from(self._h, int(face), C.byref(env)),
+              "pm_solid3d_losing_from")
+        return self
+
+    def dissipating(self, watts: float, where) -> "Solid3D":
+        """Solid3D::dissipating (solid.rs:1058-1083): `watts` in total spread evenly over the cells
+        `where` selects (a boolean array over the cells, or a function of (i, j, k)), replacing
+        what they generated before."""
+        nx, ny, nz = self._counts
+        if callable(where):
+            k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+            mask = np.vectorize(where)(i, j, k)
+        else:
+            mask = np.asarray(where)
+        m = np.ascontiguousarray(mask.reshape(-1), dtype=np.uint8)
+        if m.size != nx * ny * nz:
+            raise ValueError("dissipating: the selection must cover every cell")
+        check(lib().pm_solid3d_dissipating(self._h, float(watts),
+                                           m.ctypes.data_as(C.POINTER(C.c_uint8))),
+              "pm_solid3d_dissipating")
+        return self
+
+    def generated_power(self) -> float:
+        out = C.c_double(0.0)
+        check(lib().pm_solid3d_generated_power(self._h, C.byref(out)),
+              "pm_solid3d_generated_power")
+        return out.value
+
+    def generated_energy(self) -> float:
+        return self.books()[2]
+
+    def lost_energy(self) -> float:
+        return self.books()[1]
 
     def books(self) -> Tuple[float, float, float]:
         b = _capi.pm_books()

@@ -183,6 +183,11 @@ int pm_solid3d_losing_from(pm_solid3d *solid, int face, const pm_environment *en
  * selects (x-fastest bytes, nonzero = selected), replacing what they generated before; added
  * after each sweep as (W*dt)/C and kept in the `supplied` entry of the books. */
 int pm_solid3d_dissipating(pm_solid3d *solid, double watts, const uint8_t *mask);
+/* ScalarField::at for Solid3D, crates/pantometry-thermal/src/solid.rs:2045-2102: the temperature
+ * (kelvin) at `count` points given as x, y, z triples in metres from the block's corner,
+ * trilinear between cell centres and clamped at the outer half cells; NaN for a NaN point.  What
+ * scene::capture reads a block through. */
+int pm_solid3d_sample(pm_solid3d *solid, const double *xyz, size_t count, double *out);
 /* Solid3D::generated_power, solid.rs:1119-1121: the watts all cells generate. */
 int pm_solid3d_generated_power(const pm_solid3d *solid, double *watts);
 int pm_solid3d_checkpoint(pm_solid3d *solid);

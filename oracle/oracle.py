@@ -55,6 +55,7 @@ _SIG = {
     "orc_workload_heterogeneous": (None, [_S, _S, _S, _U64, _U64, _I, _DP, _DP, _I]),
     "orc_workload_member": (None, [_U64, _U64, _D, _D, _D, _D, C.POINTER(C.c_longlong), _DP]),
     "orc_loss_from": (_D, [_D, _D, _D, _D, _D]),
+    "orc_sample": (_D, [_S, _S, _S, _D, _DP, _D, _D, _D]),
     "orc_series_with_half_cell": (_D, [_D, _D, _D, _D]),
     "orc_film_flux": (_D, [_S, _S, _S, _D, _S, _S, _S, _D, _IP, _DP, _DP, _DP, _D, _D]),
     "orc_loss_conductance": (_D, [_S, _S, _S, _D, _S, _S, _S, _D, _IP, _DP, _DP, _DP, _D, _D]),
@@ -319,6 +320,11 @@ class OracleSolid3D:
     def grid(self) -> np.ndarray:
         nx, ny, nz = self.counts_
         return self.cells.reshape(nz, ny, nx)
+
+    def sample(self, x: float, y: float, z: float) -> float:
+        """ScalarField::at (solid.rs:2045-2102)."""
+        nx, ny, nz = self.counts_
+        return lib().orc_sample(nx, ny, nz, self.dx, p(self.cells), x, y, z)
 
 
 # ---- Hall ----------------------------------------------------------------------------------

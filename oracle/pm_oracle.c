@@ -337,6 +337,44 @@ void orc_solid_step_ext(size_t nx, size_t ny, size_t nz, double dx, const double
     }
 }
 
+/* ScalarField::at for Solid3D, solid.rs:2045-2102 (no void cells): the temperature at point
+ * (x, y, z) in metres, trilinear between cell centres, clamped at the outer half cells, the
+ * weights renormalised over the corners that carry weight. */
+double orc_sample(size_t nx, size_t ny, size_t nz, double dx, const double *cells, double x,
+                  double y, double z) {
+    double q[3] = {x / dx - 0.5, y / dx - 0.5, z / dx - 0.5};
+    if (isnan(q[0]) || isnan(q[1]) || isnan(q[2])) return NAN;
+    size_t n[3] = {nx, ny, nz}, at[3];
+    double fr[3];
+    for (int a = 0; a < 3; a++) {
+        size_t last = n[a] > 0 ? n[a] - 1 : 0;
+        if (q[a] <= 0.0) {
+            at[a] = 0, fr[a] = 0.0;
+        } else if (q[a] >= (double)last) {
+            at[a] = last, fr[a] = 0.0;
+        } else {
+            double fl = floor(q[a]);
+            at[a] = (size_t)fl, fr[a] = q[a] - fl;
+        }
+    }
+    size_t i1 = at[0] + 1 < nx ? at[0] + 1 : nx - 1;
+    size_t j1 = at[1] + 1 < ny ? at[1] + 1 : ny - 1;
+    size_t k1 = at[2] + 1 < nz ? at[2] + 1 : nz - 1;
+    size_t ia[2] = {at[0], i1}, jb[2] = {at[1], j1}, kc[2] = {at[2], k1};
+    double wa[2] = {1.0 - fr[0], fr[0]}, wb[2] = {1.0 - fr[1], fr[1]}, wc[2] = {1.0 - fr[2], fr[2]};
+    double sum = 0.0, weight = 0.0;
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++)
+            for (int c = 0; c < 2; c++) {
+                double w = wa[a] * wb[b] * wc[c];
+                if (w > 0.0) {
+                    sum += w * cells[ia[a] + nx * (jb[b] + ny * kc[c])];
+                    weight += w;
+                }
+            }
+    return weight > 0.0 ? sum / weight : NAN;
+}
+
 /* Placeless bus heat, solid.rs:1752-1765: a uniform rise gained / sum(capacity), the sum a left
  * fold in index order (`self.capacity.iter().sum::<f64>()`). */
 void orc_solid_bus_heat(size_t n, const double *cap, double gained, double *cells) {

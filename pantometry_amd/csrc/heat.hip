@@ -1251,6 +1251,51 @@ __global__ void heat_sources(double *__restrict__ T, const int64_t *__restrict__
     }
 }
 
+// ScalarField::at for Solid3D, solid.rs:2045-2102 (no void cells): trilinear between cell
+// centres, clamped at the outer half cells, weights renormalised over the corners with weight.
+__global__ void heat_sample(const double *__restrict__ T, int64_t nx, int64_t ny, int64_t nz,
+                            double dx, const double *__restrict__ xyz, int64_t n,
+                            double *__restrict__ out) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        const double q[3] = {xyz[3 * p] / dx - 0.5, xyz[3 * p + 1] / dx - 0.5, xyz[3 * p + 2] / dx - 0.5};
+        if (isnan(q[0]) || isnan(q[1]) || isnan(q[2])) {
+            out[p] = NAN;
+            continue;
+        }
+        const int64_t len[3] = {nx, ny, nz};
+        int64_t at[3];
+        double fr[3];
+        for (int a = 0; a < 3; a++) {
+            const int64_t last = len[a] - 1;
+            if (q[a] <= 0.0) {
+                at[a] = 0, fr[a] = 0.0;
+            } else if (q[a] >= (double)last) {
+                at[a] = last, fr[a] = 0.0;
+            } else {
+                const double fl = floor(q[a]);
+                at[a] = (int64_t)fl, fr[a] = q[a] - fl;
+            }
+        }
+        const int64_t ia[2] = {at[0], min(at[0] + 1, nx - 1)};
+        const int64_t jb[2] = {at[1], min(at[1] + 1, ny - 1)};
+        const int64_t kc[2] = {at[2], min(at[2] + 1, nz - 1)};
+        const double wa[2] = {1.0 - fr[0], fr[0]}, wb[2] = {1.0 - fr[1], fr[1]};
+        const double wc[2] = {1.0 - fr[2], fr[2]};
+        double sum = 0.0, weight = 0.0;
+        for (int a = 0; a < 2; a++)
+            for (int b = 0; b < 2; b++)
+                for (int c = 0; c < 2; c++) {
+                    const double w = wa[a] * wb[b] * wc[c];
+                    if (w > 0.0) {
+                        sum += w * T[ia[a] + nx * (jb[b] + ny * kc[c])];
+                        weight += w;
+                    }
+                }
+        out[p] = weight > 0.0 ? sum / weight : NAN;
+    }
+}
+
 constexpr int EXT_BLOCKS = 1024;  // fixed grid of the extended sweep, so its sums are reproducible
 
 static FilmFaces film_faces(const pm_solid3d *s) {
@@ -1826,6 +1871,21 @@ int pm_solid3d_dissipating(pm_solid3d *s, double watts, const uint8_t *mask) {
             s->source[c] = each;
     }
     s->tables_valid = false;
+    return PM_OK;
+}
+
+int pm_solid3d_sample(pm_solid3d *s, const double *xyz, size_t count, double *out) {
+    if (!s || (count && (!xyz || !out))) return PM_ERR_ARGUMENT;
+    if (count == 0) return PM_OK;
+    DevBuf dp, dv;
+    PM_CHECK(dp.alloc(sizeof(double) * 3 * count));
+    PM_CHECK(dv.alloc(sizeof(double) * count));
+    PM_HIP(hipMemcpyAsync(dp.d(), xyz, sizeof(double) * 3 * count, hipMemcpyHostToDevice, s->stream));
+    heat_sample<<<grid_for((int64_t)count, 256), 256, 0, s->stream>>>(s->T(), s->nx, s->ny, s->nz,
+                                                                      s->dx, dp.d(), (int64_t)count, dv.d());
+    PM_HIP(hipGetLastError());
+    PM_HIP(hipMemcpyAsync(out, dv.d(), sizeof(double) * count, hipMemcpyDeviceToHost, s->stream));
+    PM_HIP(hipStreamSynchronize(s->stream));
     return PM_OK;
 }
 

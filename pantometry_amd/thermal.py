@@ -347,6 +347,15 @@ class Solid3D:
               "pm_solid3d_dissipating")
         return self
 
+    def sample(self, points) -> np.ndarray:
+        """ScalarField::at (solid.rs:2045-2102) at each point (x, y, z in metres from the block's
+        corner): trilinear between cell centres, kelvin."""
+        pts = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1, 3))
+        out = np.empty(len(pts))
+        check(lib().pm_solid3d_sample(self._h, dptr(pts), len(pts), dptr(out)),
+              "pm_solid3d_sample")
+        return out
+
     def generated_power(self) -> float:
         out = C.c_double(0.0)
         check(lib().pm_solid3d_generated_power(self._h, C.byref(out)),
@@ -377,8 +386,12 @@ class Solid3D:
     def readings(self):
         out = (C.c_double * 4)()
         check(lib().pm_solid3d_readings(self._h, out), "pm_solid3d_readings")
-        return [(self._name, "peak", out[0], "°C"), (self._name, "mean", out[1], "°C"),
-                (self._name, "coldest", out[2], "°C"), (self._name, "absorbed", out[3], "J")]
+        # Solid3D::readings, solid.rs:1962-2002: "generated" only for a block with a source.
+        out_ = [(self._name, "peak", out[0], "C"), (self._name, "mean", out[1], "C"),
+                (self._name, "coldest", out[2], "C"), (self._name, "absorbed", out[3], "J")]
+        if self.generated_power() != 0.0:
+            out_.append((self._name, "generated", self.generated_energy(), "J"))
+        return out_
 
     # ---- device-resident loop ----------------------------------------------------------
     def advance(self, dt: float, steps: int, tolerance: float = 1e-9, want_totals: bool = True):

@@ -10,11 +10,11 @@ from .core import (ENERGY, HEAT, Exchange, Kind, Ledger, Report, Schedule, Simul
                    Tolerances, Violation, audit, audit_with, substeps_for)
 from .ensemble import Ensemble, Member
 from .rng import Rng
-from .thermal import STABLE_FOURIER_3D, Solid3D, Substance, celsius, mm
+from .thermal import STABLE_FOURIER_3D, Environment, Face, Solid3D, Substance, celsius, mm
 
 __all__ = [
     "DeviceError", "LIB_PATH", "lib", "Hall", "ENERGY", "HEAT", "Exchange", "Kind", "Ledger",
     "Report", "Schedule", "Simulation", "Tolerances", "Violation", "audit", "audit_with",
-    "substeps_for", "Ensemble", "Member", "Rng", "STABLE_FOURIER_3D", "Solid3D", "Substance",
-    "celsius", "mm",
+    "substeps_for", "Ensemble", "Member", "Rng", "STABLE_FOURIER_3D", "Environment", "Face",
+    "Solid3D", "Substance", "celsius", "mm",
 ]

@@ -243,6 +243,10 @@ double pm_hall_courant(const pm_hall *hall, double dt);
 /* Domain::step, hall.rs:408-530: Courant guard, half step the first time, velocity then
  * pressure sweeps, and the startup check of the first step. */
 int pm_hall_step(pm_hall *hall, double dt, pm_violation *v);
+/* ScalarField::at for Hall, crates/pantometry-acoustic/src/hall.rs:590-617: the pressure (Pa) at
+ * `count` points given as x, y, z triples in metres from the corner node, trilinear between nodes
+ * and clamped at the walls (p[0] for a NaN point). */
+int pm_hall_sample(pm_hall *hall, const double *xyz, size_t count, double *out);
 /* Hall::energy (hall.rs:249-251) split into the invariant (hall.rs:334-384) and the startup
  * offset, plus the energy a source has supplied. */
 int pm_hall_energy(pm_hall *hall, double *invariant, double *offset, double *supplied);

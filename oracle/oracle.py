@@ -56,6 +56,7 @@ _SIG = {
     "orc_workload_member": (None, [_U64, _U64, _D, _D, _D, _D, C.POINTER(C.c_longlong), _DP]),
     "orc_loss_from": (_D, [_D, _D, _D, _D, _D]),
     "orc_sample": (_D, [_S, _S, _S, _D, _DP, _D, _D, _D]),
+    "orc_hall_sample": (_D, [_S, _S, _S, _D, _DP, _D, _D, _D]),
     "orc_series_with_half_cell": (_D, [_D, _D, _D, _D]),
     "orc_film_flux": (_D, [_S, _S, _S, _D, _S, _S, _S, _D, _IP, _DP, _DP, _DP, _D, _D]),
     "orc_loss_conductance": (_D, [_S, _S, _S, _D, _S, _S, _S, _D, _IP, _DP, _DP, _DP, _D, _D]),
@@ -391,6 +392,10 @@ class OracleHall:
 
     def max_stable_dt(self) -> float:
         return self.dx / (self.speed * math.sqrt(3.0))
+
+    def sample(self, x: float, y: float, z: float) -> float:
+        """ScalarField::at (hall.rs:590-617)."""
+        return lib().orc_hall_sample(self.nx, self.ny, self.nz, self.dx, p(self.p), x, y, z)
 
     def courant(self, dt: float) -> float:
         return self.speed * dt * math.sqrt(3.0) / self.dx

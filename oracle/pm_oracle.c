@@ -375,6 +375,43 @@ double orc_sample(size_t nx, size_t ny, size_t nz, double dx, const double *cell
     return weight > 0.0 ? sum / weight : NAN;
 }
 
+/* clamp_index, crates/pantometry-acoustic/src/hall.rs:684-694. */
+static void orc_clamp_index(double q, size_t n, size_t *i, double *f) {
+    if (isnan(q) || q <= 0.0) {
+        *i = 0, *f = 0.0;
+        return;
+    }
+    double last = (double)(n - 1);
+    if (q >= last) {
+        *i = n - 1, *f = 0.0;
+        return;
+    }
+    double fl = floor(q);
+    *i = (size_t)fl, *f = q - fl;
+}
+
+static inline double orc_lerp(double lo, double hi, double t) { return lo * (1.0 - t) + hi * t; }
+
+/* ScalarField::at for Hall, hall.rs:590-617: the pressure at (x, y, z) metres from the corner
+ * node, trilinear between nodes (x, then y, then z), clamped at the walls; p[0] for a NaN point. */
+double orc_hall_sample(size_t nx, size_t ny, size_t nz, double dx, const double *p, double x,
+                       double y, double z) {
+    double qx = x / dx, qy = y / dx, qz = z / dx;
+    if (isnan(qx) || isnan(qy) || isnan(qz)) return p[0];
+    size_t i, j, k;
+    double fx, fy, fz;
+    orc_clamp_index(qx, nx, &i, &fx);
+    orc_clamp_index(qy, ny, &j, &fy);
+    orc_clamp_index(qz, nz, &k, &fz);
+    size_t i1 = i + 1 < nx ? i + 1 : nx - 1, j1 = j + 1 < ny ? j + 1 : ny - 1;
+    size_t k1 = k + 1 < nz ? k + 1 : nz - 1;
+#define G(a, b, c) p[(a) + nx * ((b) + ny * (c))]
+    double z0 = orc_lerp(orc_lerp(G(i, j, k), G(i1, j, k), fx), orc_lerp(G(i, j1, k), G(i1, j1, k), fx), fy);
+    double z1 = orc_lerp(orc_lerp(G(i, j, k1), G(i1, j, k1), fx), orc_lerp(G(i, j1, k1), G(i1, j1, k1), fx), fy);
+#undef G
+    return orc_lerp(z0, z1, fz);
+}
+
 /* Placeless bus heat, solid.rs:1752-1765: a uniform rise gained / sum(capacity), the sum a left
  * fold in index order (`self.capacity.iter().sum::<f64>()`). */
 void orc_solid_bus_heat(size_t n, const double *cap, double gained, double *cells) {

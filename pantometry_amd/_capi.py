@@ -81,6 +81,7 @@ SIGNATURES = {
     "pm_solid3d_dissipating": (_I, [_VP, _D, C.POINTER(C.c_uint8)]),
     "pm_solid3d_generated_power": (_I, [_VP, _DP]),
     "pm_solid3d_sample": (_I, [_VP, _DP, _S, _DP]),
+    "pm_hall_sample": (_I, [_VP, _DP, _S, _DP]),
     "pm_solid3d_restore": (_I, [_VP]),
     "pm_solid3d_readings": (_I, [_VP, _DP]),
     "pm_solid3d_heat_capacity": (_D, [_VP]),

@@ -221,6 +221,14 @@ class Hall:
     def peak_pressure(self) -> float:
         return float(lib().pm_hall_peak_pressure(self._h))
 
+    def sample(self, points) -> np.ndarray:
+        """ScalarField::at (hall.rs:590-617) at each point (x, y, z in metres from the corner
+        node): trilinear between nodes, pascals."""
+        pts = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1, 3))
+        out = np.empty(len(pts))
+        check(lib().pm_hall_sample(self._h, dptr(pts), len(pts), dptr(out)), "pm_hall_sample")
+        return out
+
     def readings(self):
         return [(self._name, "peak", self.peak_pressure(), "Pa")]
 

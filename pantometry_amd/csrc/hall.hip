@@ -553,6 +553,52 @@ __global__ void hall_audit_reset(AuditState *st, double total, double scale, dou
 
 using namespace pm;
 
+namespace pm {
+
+// clamp_index, hall.rs:684-694.
+__device__ __forceinline__ void clamp_index(double q, int64_t n, int64_t *i, double *f) {
+    if (isnan(q) || q <= 0.0) {
+        *i = 0, *f = 0.0;
+        return;
+    }
+    const double last = (double)(n - 1);
+    if (q >= last) {
+        *i = n - 1, *f = 0.0;
+        return;
+    }
+    const double fl = floor(q);
+    *i = (int64_t)fl, *f = q - fl;
+}
+
+__device__ __forceinline__ double lerp(double lo, double hi, double t) { return lo * (1.0 - t) + hi * t; }
+
+// ScalarField::at for Hall, hall.rs:590-617: trilinear between nodes (x, then y, then z),
+// clamped at the walls; p[0] for a NaN point.
+__global__ void hall_sample(const double *__restrict__ p, int64_t nx, int64_t ny, int64_t nz,
+                            double dx, const double *__restrict__ xyz, int64_t n,
+                            double *__restrict__ out) {
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const double qx = xyz[3 * q] / dx, qy = xyz[3 * q + 1] / dx, qz = xyz[3 * q + 2] / dx;
+        if (isnan(qx) || isnan(qy) || isnan(qz)) {
+            out[q] = p[0];
+            continue;
+        }
+        int64_t i, j, k;
+        double fx, fy, fz;
+        clamp_index(qx, nx, &i, &fx);
+        clamp_index(qy, ny, &j, &fy);
+        clamp_index(qz, nz, &k, &fz);
+        const int64_t i1 = min(i + 1, nx - 1), j1 = min(j + 1, ny - 1), k1 = min(k + 1, nz - 1);
+        auto g = [&](int64_t a, int64_t b, int64_t c) { return p[a + nx * (b + ny * c)]; };
+        const double z0 = lerp(lerp(g(i, j, k), g(i1, j, k), fx), lerp(g(i, j1, k), g(i1, j1, k), fx), fy);
+        const double z1 = lerp(lerp(g(i, j, k1), g(i1, j, k1), fx), lerp(g(i, j1, k1), g(i1, j1, k1), fx), fy);
+        out[q] = lerp(z0, z1, fz);
+    }
+}
+
+}  // namespace pm
+
 struct pm_hall {
     std::string name;
     int64_t nx = 2, ny = 1, nz = 1;
@@ -915,6 +961,21 @@ int pm_hall_step(pm_hall *h, double dt, pm_violation *v) {
     PM_HIP(hipMemcpyAsync(&supplied, h->scalars.d() + 1, sizeof(double), hipMemcpyDeviceToHost, h->stream));
     PM_HIP(hipStreamSynchronize(h->stream));
     h->supplied_host = supplied;
+    return PM_OK;
+}
+
+int pm_hall_sample(pm_hall *h, const double *xyz, size_t count, double *out) {
+    if (!h || (count && (!xyz || !out))) return PM_ERR_ARGUMENT;
+    if (count == 0) return PM_OK;
+    DevBuf dp, dv;
+    PM_CHECK(dp.alloc(sizeof(double) * 3 * count));
+    PM_CHECK(dv.alloc(sizeof(double) * count));
+    PM_HIP(hipMemcpyAsync(dp.d(), xyz, sizeof(double) * 3 * count, hipMemcpyHostToDevice, h->stream));
+    hall_sample<<<(unsigned)std::min<size_t>((count + 255) / 256, 4096), 256, 0, h->stream>>>(
+        h->p[h->ip].d(), h->nx, h->ny, h->nz, h->dx, dp.d(), (int64_t)count, dv.d());
+    PM_HIP(hipGetLastError());
+    PM_HIP(hipMemcpyAsync(out, dv.d(), sizeof(double) * count, hipMemcpyDeviceToHost, h->stream));
+    PM_HIP(hipStreamSynchronize(h->stream));
     return PM_OK;
 }
 

@@ -305,6 +305,11 @@ class World:
                 self.extent[d.name()] = (size, cells)
             elif kind == "hall":
                 d = build_hall(spec)
+                nx, ny, nz = d.nodes()
+                dx = d.spacing()
+                # Placement::field(Extent::volume(((n-1) dx per axis), nodes)), lib.rs:3461-3482
+                size = ((nx - 1) * dx, (ny - 1) * dx, (nz - 1) * dx)
+                self.extent[d.name()] = (size, (nx, ny, nz))
             else:
                 raise SceneError(f"{spec.get('name', '?')}: a {kind!r} domain is not on this "
                                  f"backend's path (block and hall are)")
@@ -315,14 +320,16 @@ class World:
         self._pts = {n: _extent_points(*e) for n, e in self.extent.items()}
 
     def capture(self) -> Frame:
-        """scene::capture (scene/src/lib.rs:337-357): every domain's readings, and each block's
-        temperature sampled over its extent through ScalarField::at on the device."""
+        """scene::capture (scene/src/lib.rs:337-357): every domain's readings, and each field
+        (a block's temperature, a hall's pressure) sampled over its extent through
+        ScalarField::at on the device."""
         frame = Frame(self.sim.t)
         for d in self.sim.domains:
             frame.readings.extend(d.readings())
             if d.name() in self._pts:
                 (size, (nx, ny, nz)) = self.extent[d.name()]
-                frame.panels.append(Panel(d.name(), "K", nx, ny, nz, d.sample(self._pts[d.name()])))
+                unit = "Pa" if isinstance(d, Hall) else "K"
+                frame.panels.append(Panel(d.name(), unit, nx, ny, nz, d.sample(self._pts[d.name()])))
         return frame
 
     def advance(self, dt: float):

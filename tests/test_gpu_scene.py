@@ -88,15 +88,32 @@ def test_block_scenes_match_the_oracle(oracle, name, frames):
     assert len(doc["frames"]) == 2
 
 
-def test_the_hall_scene_runs_and_conserves():
+def test_the_hall_scene_matches_the_oracle(oracle):
+    """Scene 16: every frame's pressure field and captured panel bit-identical to the oracle's
+    hall released in the same mode and advanced by the same multirate substeps."""
     gpu_case("scene 16 hall")
     from pantometry_amd import scene as S
-    world = S.build(SCENES / "16-a-room-with-a-ceiling.json")
-    frames = world.run()
-    assert len(frames) == world.scene.frames + 1
-    peaks = [f.readings[0][2] for f in frames]
-    assert peaks[0] == pytest.approx(1.0, rel=1e-12)
-    assert all(0.0 < p <= 1.0 + 1e-9 for p in peaks)
+    from pantometry_amd.core import substeps_for
+    sc = S.load(SCENES / "16-a-room-with-a-ceiling.json")
+    spec = sc.domains[0]
+    world = S.build(sc)
+    o = oracle.OracleHall(spec["width_m"], spec["height_m"], spec["depth_m"], spec["nodes_across"],
+                          343.0, 1.204)
+    o.release(o.mode(tuple(spec["mode"]), spec["amplitude_pa"]))
+    dt = sc.duration_s / sc.frames
+    for n in range(sc.frames):
+        world.advance(dt)
+        m = substeps_for(dt, o.max_stable_dt())
+        for _ in range(m):
+            o.step(dt / m)
+        g = world.sim.domain(spec["name"])
+        assert np.array_equal(bits(g.fields()[0]), bits(o.p)), n
+    frame = world.capture()
+    panel = frame.panels[0]
+    assert panel.unit == "Pa" and (panel.nx, panel.ny, panel.nz) == (o.nx, o.ny, o.nz)
+    expect = np.array([o.sample(*pt) for pt in world._pts[spec["name"]]])
+    assert np.array_equal(bits(panel.values), bits(expect))
+    assert frame.readings[0][1] == "peak"
 
 
 def test_scenes_off_the_path_are_refused():

@@ -44,3 +44,29 @@ def test_outside_the_centres_is_flat_and_nan_is_nan(oracle):
     assert b.sample(-1.0, -1.0, -1.0) == b.cells[0]
     assert b.sample(1.0, 1.0, 1.0) == b.cells[63]
     assert math.isnan(b.sample(math.nan, 0.0, 0.0))
+
+
+def _hall(oracle, nodes_across, p):
+    h = oracle.OracleHall(0.4, 0.3, 0.2, nodes_across, 343.0, 1.204)
+    h.p = np.ascontiguousarray(p(h), dtype=np.float64)
+    return h
+
+
+def test_hall_nodes_are_exact_and_a_linear_field_is_reproduced(oracle):
+    """Hall's ScalarField::at (hall.rs:590-617): nodes exact, trilinear reproduces a field linear
+    along each axis, clamped beyond the walls, p[0] for a NaN point."""
+    def linear(h):
+        k, j, i = np.meshgrid(np.arange(h.nz), np.arange(h.ny), np.arange(h.nx), indexing="ij")
+        return (1.0 + 0.5 * i - 0.25 * j + 2.0 * k).reshape(-1)
+
+    h = _hall(oracle, 9, linear)
+    dx = h.dx
+    for (i, j, k) in [(0, 0, 0), (3, 2, 1), (h.nx - 1, h.ny - 1, h.nz - 1)]:
+        assert h.sample(i * dx, j * dx, k * dx) == h.p[i + h.nx * (j + h.ny * k)]
+    rng = np.random.default_rng(3)
+    for _ in range(100):
+        q = rng.uniform(0.0, 1.0, 3) * (np.array([h.nx, h.ny, h.nz]) - 1)
+        assert abs(h.sample(*(q * dx)) - (1.0 + 0.5 * q[0] - 0.25 * q[1] + 2.0 * q[2])) < 1e-12
+    assert h.sample(-1.0, -1.0, -1.0) == h.p[0]
+    assert h.sample(math.nan, 0.0, 0.0) == h.p[0]
+    assert h.sample(9.0, 9.0, 9.0) == h.p[-1]

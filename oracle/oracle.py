@@ -57,6 +57,7 @@ _SIG = {
     "orc_loss_from": (_D, [_D, _D, _D, _D, _D]),
     "orc_sample": (_D, [_S, _S, _S, _D, _DP, _D, _D, _D]),
     "orc_hall_sample": (_D, [_S, _S, _S, _D, _DP, _D, _D, _D]),
+    "orc_room_invariant": (_D, [_S, _S, _D, _D, _D, _D, _DP, _DP, _DP, _DP]),
     "orc_series_with_half_cell": (_D, [_D, _D, _D, _D]),
     "orc_film_flux": (_D, [_S, _S, _S, _D, _S, _S, _S, _D, _IP, _DP, _DP, _DP, _D, _D]),
     "orc_loss_conductance": (_D, [_S, _S, _S, _D, _S, _S, _S, _D, _IP, _DP, _DP, _DP, _D, _D]),
@@ -433,6 +434,51 @@ class OracleHall:
         if self.supplied != 0.0:
             entries.append(-self.supplied)
         return ledger_total(entries)
+
+
+class OracleRoom(OracleHall):
+    """Room (crates/pantometry-acoustic/src/room.rs) on the CPU: the same staggered scheme as a
+    hall one node deep (Room::new's node counts, room.rs:90-120), stepped under the
+    two-dimensional Courant limit dx/(c*sqrt 2) (room.rs:336-338, 361-366), its energy the
+    Room's own invariant (room.rs:243-281) for a room `depth` metres deep."""
+
+    def __init__(self, width: float, height: float, cells_across: int, depth: float,
+                 speed: float, density: float):
+        super().__init__(width, height, 0.0, max(cells_across, 3), speed, density)
+        nx = max(cells_across, 3)
+        dx = width / float(nx - 1)
+        ny = max(int(_round_half_away(height / dx)) + 1, 3)
+        self.nx, self.ny, self.nz, self.dx = nx, ny, 1, dx
+        self.depth_m = float(depth)
+        self.p = np.zeros(nx * ny)
+        self.pp = np.zeros(nx * ny)
+        self.vx = np.zeros((nx - 1) * ny)
+        self.vy = np.zeros(nx * (ny - 1))
+        self.vz = np.zeros(1)
+
+    def invariant(self) -> float:
+        return lib().orc_room_invariant(self.nx, self.ny, self.dx, self.depth_m, self.speed,
+                                        self.density, p(self.p), p(self.pp), p(self.vx),
+                                        p(self.vy))
+
+    def max_stable_dt(self) -> float:
+        if self.speed <= 0.0:
+            return math.inf
+        return self.dx / (self.speed * math.sqrt(2.0))
+
+    def courant(self, dt: float) -> float:
+        return self.speed * dt * math.sqrt(2.0) / self.dx
+
+    def mode2(self, nx_mode: int, ny_mode: int, amplitude: float) -> np.ndarray:
+        """Room::released_in_mode's field (room.rs:166-182): ((a*px)*py)."""
+        lx, ly = (self.nx - 1) * self.dx, (self.ny - 1) * self.dx
+        out = np.empty(self.nx * self.ny)
+        for j in range(self.ny):
+            for i in range(self.nx):
+                px = math.cos(float(nx_mode) * math.pi * (float(i) * self.dx) / lx) if lx > 0 else 1.0
+                py = math.cos(float(ny_mode) * math.pi * (float(j) * self.dx) / ly) if ly > 0 else 1.0
+                out[i + self.nx * j] = amplitude * px * py
+        return out
 
 
 def _round_half_away(x: float) -> float:

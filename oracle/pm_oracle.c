@@ -375,6 +375,30 @@ double orc_sample(size_t nx, size_t ny, size_t nz, double dx, const double *cell
     return weight > 0.0 ? sum / weight : NAN;
 }
 
+static inline double orc_wall(size_t k, size_t n);
+
+/* Room::invariant_si, crates/pantometry-acoustic/src/room.rs:243-281: the scheme's conserved
+ * energy of a two-dimensional room `depth` deep (nodes x-fastest, nx*ny; vx (nx-1) per row, vy
+ * ny-1 rows), each group a left fold in storage order, then potential + kinetic_x + kinetic_y. */
+double orc_room_invariant(size_t nx, size_t ny, double dx, double depth, double speed,
+                          double density, const double *p, const double *pp, const double *vx,
+                          const double *vy) {
+    double volume = dx * dx * depth;
+    double rc2 = density * speed * speed;
+    if (rc2 <= 0.0) return 0.0;
+    double potential = 0.0, kx = 0.0, ky = 0.0;
+    for (size_t k = 0; k < nx * ny; k++) {
+        size_t i = k % nx, j = k / nx;
+        double share = 1.0 / (orc_wall(i, nx) * orc_wall(j, ny));
+        potential += share * p[k] * pp[k] / (2.0 * rc2) * volume;
+    }
+    for (size_t k = 0; k < (nx - 1) * ny; k++)
+        kx += density * vx[k] * vx[k] / 2.0 * volume / orc_wall(k / (nx - 1), ny);
+    for (size_t k = 0; k < nx * (ny - 1); k++)
+        ky += density * vy[k] * vy[k] / 2.0 * volume / orc_wall(k % nx, nx);
+    return potential + kx + ky;
+}
+
 /* clamp_index, crates/pantometry-acoustic/src/hall.rs:684-694. */
 static void orc_clamp_index(double q, size_t n, size_t *i, double *f) {
     if (isnan(q) || q <= 0.0) {

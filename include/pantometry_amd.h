@@ -275,6 +275,21 @@ int pm_hall_set_tile(pm_hall *hall, int tile_y, int z_chunk);
 int pm_hall_set_steps_per_pass(pm_hall *hall, int steps);
 
 /* ---------------------------------------------------------------------------------------- */
+/* Room, crates/pantometry-acoustic/src/room.rs: a hall one node deep                       */
+/* ---------------------------------------------------------------------------------------- */
+
+/* Room::new, room.rs:90-120: nx = max(cells_across, 3), dx = width/(nx-1),
+ * ny = max(round(height/dx) + 1, 3), `depth` metres deep.  The handle is a pm_hall with nz = 1
+ * and every pm_hall_* call applies, under the room's rules: the Courant number
+ * c dt sqrt(2)/dx and its limit dx/(c sqrt 2) (room.rs:336-338, 361-366, 369-379), the room's
+ * invariant for energy and startup (room.rs:243-281, 460-475), sampling in x and y only
+ * (room.rs:587-600), no source.  pm_hall_advance audits each step's energy on the host. */
+int pm_room_create(const char *name, double width, double height, size_t cells_across,
+                   double depth, double density, double speed, pm_hall **out);
+/* 2 for a room, 3 for a hall. */
+int pm_hall_dimensions(const pm_hall *hall);
+
+/* ---------------------------------------------------------------------------------------- */
 /* Ensemble of independent heat grids, crates/pantometry-core/src/ensemble.rs:102-147       */
 /* ---------------------------------------------------------------------------------------- */
 

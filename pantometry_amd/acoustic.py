@@ -1,5 +1,5 @@
-"""Hall on the GPU: the Domain of crates/pantometry-acoustic/src/hall.rs, stepped by the gfx950
-library through its C ABI.  Pressures in pascal, lengths in metres, energies in joules."""
+"""Hall and Room on the GPU: the Domains of crates/pantometry-acoustic/src/hall.rs and room.rs,
+stepped by the gfx950 library through its C ABI.  Pressures in pascal, lengths in metres, energies in joules."""
 from __future__ import annotations
 
 import ctypes as C
@@ -241,3 +241,105 @@ class Hall:
         check(lib().pm_hall_advance(self._h, float(dt), int(steps), float(tolerance), dptr(totals),
                                     C.byref(ms), C.byref(v)), "pm_hall_advance", v)
         return totals[:steps], ms.value
+
+
+class Room(Hall):
+    """Room (crates/pantometry-acoustic/src/room.rs): a two-dimensional rigid-walled room, `depth`
+    metres deep, on the hall's kernels as a hall one node deep.  Its own node counts (room.rs:
+    90-120), Courant limit dx/(c sqrt 2) (room.rs:361-366) and energy invariant (room.rs:243-281)
+    are applied by the library; the fields are those of the hall's sweep with no z faces."""
+
+    def __init__(self, name: str, width: float, height: float, cells_across: int, depth: float,
+                 density: float, speed: float):
+        self._name = name
+        self._speed = float(speed)
+        self._density = float(density)
+        self._depth = float(depth)
+        handle = C.c_void_p()
+        check(lib().pm_room_create(name.encode(), float(width), float(height), int(cells_across),
+                                   self._depth, self._density, self._speed, C.byref(handle)),
+              "pm_room_create")
+        self._h = handle
+        out = (C.c_size_t * 3)()
+        check(lib().pm_hall_nodes(self._h, out), "pm_hall_nodes")
+        self._nodes = (out[0], out[1], out[2])
+        self._dx = float(lib().pm_hall_spacing(self._h))
+
+    @staticmethod
+    def of_air(name: str, width: float, height: float, cells_across: int) -> "Room":
+        """Air at 20 C, one metre deep: 1.204 kg/m^3, 343 m/s (room.rs:122-137)."""
+        return Room(name, width, height, cells_across, 1.0, 1.204, 343.0)
+
+    def cells(self) -> Tuple[int, int]:
+        return self._nodes[0], self._nodes[1]
+
+    def room_depth(self) -> float:
+        return self._depth
+
+    def cell_area(self) -> float:
+        """room.rs:283-285."""
+        return self._dx * self._depth
+
+    def released_from(self, profile) -> "Room":
+        """Room::released_from, room.rs:140-160; the profile sees node positions (i, j)*dx."""
+        nx, ny = self.cells()
+        if callable(profile):
+            p = np.empty((ny, nx), dtype=np.float64)
+            for j in range(ny):
+                for i in range(nx):
+                    p[j, i] = profile(float(i) * self._dx, float(j) * self._dx)
+        else:
+            p = np.ascontiguousarray(profile, dtype=np.float64).reshape(ny, nx)
+        p = np.ascontiguousarray(p.reshape(-1))
+        check(lib().pm_hall_release(self._h, dptr(p)), "pm_hall_release")
+        return self
+
+    def mode_profile(self, mode: Tuple[int, int], amplitude: float) -> np.ndarray:
+        """The pressure of room.rs:166-182 at every node: (a*px)*py."""
+        m = tuple(mode) + (0,) * (3 - len(tuple(mode)))
+        return super().mode_profile((m[0], m[1], 0), amplitude).reshape(self._nodes[1],
+                                                                          self._nodes[0])
+
+    def released_in_mode(self, mode: Tuple[int, int], amplitude: float) -> "Room":
+        return self.released_from(self.mode_profile(mode, amplitude))
+
+    def set_source(self, node: int, increments: Optional[np.ndarray]) -> None:
+        raise TypeError("a room has no source")
+
+    def mode_frequency(self, mode: Tuple[int, int]) -> float:
+        """room.rs:291-300: (c/2) sqrt((nx/Lx)^2 + (ny/Ly)^2), zero for a degenerate room."""
+        lx, ly = self.width(), self.height()
+        if lx <= 0.0 or ly <= 0.0:
+            return 0.0
+        a = float(mode[0]) / lx
+        b = float(mode[1]) / ly
+        return self._speed / 2.0 * math.sqrt(a * a + b * b)
+
+    def modes_below(self, limit: float) -> List[Tuple[int, int, float]]:
+        """room.rs:306-332."""
+        def bound(length: float) -> int:
+            return 0 if length <= 0.0 else int(math.floor(2.0 * limit * length / self._speed))
+
+        found = []
+        for a in range(bound(self.width()) + 1):
+            for b in range(bound(self.height()) + 1):
+                if a == 0 and b == 0:
+                    continue
+                f = self.mode_frequency((a, b))
+                if f <= limit:
+                    found.append((a, b, f))
+        found.sort(key=lambda x: x[2])
+        return found
+
+    def pressure_at(self, i: int, j: int, k: int = 0) -> float:
+        """room.rs:200-204, clamped to the walls."""
+        return super().pressure_at(i, j, 0)
+
+    def sample(self, points) -> np.ndarray:
+        """ScalarField::at (room.rs:587-600) at each (x, y) point, or (x, y, z) with z ignored:
+        bilinear between nodes, clamped to the walls, pascals."""
+        pts = np.asarray(points, dtype=np.float64)
+        pts = pts.reshape(-1, pts.shape[-1] if pts.ndim > 1 else 2)
+        xyz = np.zeros((len(pts), 3))
+        xyz[:, :2] = pts[:, :2]
+        return super().sample(xyz)

@@ -509,6 +509,35 @@ __global__ __launch_bounds__(256) void hall_invariant_kernel(const double *p, co
     if (threadIdx.x == 0) partials[blockIdx.x] = total;
 }
 
+// Room::invariant_si, room.rs:243-281: the potential share of every node, then the kinetic share
+// of every vx face and every vy face; blocks [0, bp) potential, [bp, bp+bx) vx, the rest vy.
+__global__ __launch_bounds__(256) void room_invariant_kernel(const double *p, const double *pp,
+                                                             const double *vx, const double *vy,
+                                                             int nx, int ny, double rc2,
+                                                             double volume, double density,
+                                                             double *partials) {
+    __shared__ double red[4];
+    const int64_t np = (int64_t)nx * ny, nfx = (int64_t)(nx - 1) * ny, nfy = (int64_t)nx * (ny - 1);
+    const int64_t bp = (np + 255) / 256, bx = (nfx + 255) / 256;
+    const int64_t b = blockIdx.x;
+    double acc = 0.0;
+    if (b < bp) {
+        const int64_t k = b * 256 + threadIdx.x;
+        if (k < np) {
+            const double share = 1.0 / (wall(k % nx, nx) * wall(k / nx, ny));
+            acc = share * p[k] * pp[k] / (2.0 * rc2) * volume;
+        }
+    } else if (b < bp + bx) {
+        const int64_t k = (b - bp) * 256 + threadIdx.x;
+        if (k < nfx) acc = density * vx[k] * vx[k] / 2.0 * volume / wall(k / (nx - 1), ny);
+    } else {
+        const int64_t k = (b - bp - bx) * 256 + threadIdx.x;
+        if (k < nfy) acc = density * vy[k] * vy[k] / 2.0 * volume / wall(k % nx, nx);
+    }
+    const double total = block_sum<256>(acc, red);
+    if (threadIdx.x == 0) partials[b] = total;
+}
+
 // Sums the step's partials into the invariant and audits energy + offset - supplied.
 __global__ __launch_bounds__(1024) void hall_finalize(const double *partials, int64_t n,
                                                       AuditState *st, double offset,
@@ -605,6 +634,11 @@ struct pm_hall {
     double dx = 0.0, speed = 0.0, density = 0.0;
     bool staggered = false;
     double energy_offset = 0.0, energy_datum = 0.0;
+    // A Room (crates/pantometry-acoustic/src/room.rs) is a hall one node deep: the same
+    // arithmetic, under the two-dimensional Courant limit, with the Room's own invariant for a
+    // room `room_depth` metres deep.
+    int dims = 3;
+    double room_depth = 0.0;
     hipStream_t stream = nullptr;
     // Pressure rotates through three buffers: current, previous (pressure_prev) and a spare the
     // two-steps-per-pass kernel writes its intermediate pressure into.  Velocities ping-pong.
@@ -661,6 +695,26 @@ static int hall_invariant(pm_hall *h, const double *p, const double *pp, const d
         *out = 0.0;
         return PM_OK;
     }
+    if (h->dims == 2) {
+        // The Room's invariant: three groups, each summed in a fixed order, then added as
+        // potential + kinetic_x + kinetic_y (room.rs:280).
+        const int64_t bp = (h->nodes() + 255) / 256, bx = (h->nfx() + 255) / 256;
+        const int64_t by = (h->nfy() + 255) / 256;
+        const int64_t blocks = bp + bx + by;
+        DevBuf part;
+        PM_CHECK(part.alloc(sizeof(double) * (size_t)blocks));
+        const double volume = h->dx * h->dx * h->room_depth;
+        room_invariant_kernel<<<(unsigned)blocks, 256, 0, h->stream>>>(
+            p, pp, vx, vy, (int)h->nx, (int)h->ny, rc2, volume, h->density, part.d());
+        PM_HIP(hipGetLastError());
+        std::vector<double> hp((size_t)blocks);
+        PM_HIP(hipMemcpyAsync(hp.data(), part.d(), sizeof(double) * hp.size(), hipMemcpyDeviceToHost, h->stream));
+        PM_HIP(hipStreamSynchronize(h->stream));
+        double g[3] = {0.0, 0.0, 0.0};
+        for (int64_t b = 0; b < blocks; b++) g[b < bp ? 0 : b < bp + bx ? 1 : 2] += hp[(size_t)b];
+        *out = g[0] + g[1] + g[2];
+        return PM_OK;
+    }
     const int64_t n = h->nodes();
     const int64_t blocks = (n + 255) / 256;
     DevBuf part;
@@ -683,9 +737,10 @@ static int hall_invariant_now(pm_hall *h, double *out) {
 }
 
 static int hall_check(const pm_hall *h, double dt, pm_violation *v, int64_t step) {
-    const double courant = h->speed * dt * std::sqrt(3.0) / h->dx;
+    const double courant = h->speed * dt * std::sqrt((double)h->dims) / h->dx;
     if (courant > 1.0 + 1e-12) {
-        std::string site = h->name + " (three-dimensional wave equation)";
+        std::string site = h->name + (h->dims == 2 ? " (two-dimensional wave equation)"
+                                                   : " (three-dimensional wave equation)");
         fill_violation(v, "Courant number", site.c_str(), 1.0, courant, 1.0, 1e-12, step);
         return PM_VIOLATION;
     }
@@ -825,6 +880,8 @@ static int hall_first_step(pm_hall *h, double hstep, pm_violation *v, int64_t st
     PM_HIP(hipMemcpyAsync(&supplied_step, h->scalars.d() + 1, sizeof(double), hipMemcpyDeviceToHost, h->stream));
     PM_HIP(hipStreamSynchronize(h->stream));
     h->supplied_host = supplied_step;
+    // The fused sum is the hall's invariant; a room measures its own.
+    if (h->dims == 2) PM_CHECK(hall_invariant_now(h, &inv));
     // The startup offset is measured on the leapfrog alone: a source's first increment is
     // booked as supplied, not folded into the offset.
     const double inv_scheme = inv - supplied_step;
@@ -884,6 +941,34 @@ int pm_hall_create(const char *name, double width, double height, double depth, 
     return PM_OK;
 }
 
+int pm_room_create(const char *name, double width, double height, size_t cells_across,
+                   double depth, double density, double speed, pm_hall **out) {
+    if (!out) return PM_ERR_ARGUMENT;
+    *out = nullptr;
+    // Room::new, room.rs:90-120: at least three nodes each way, square cells from the width.
+    const int64_t nx = (int64_t)std::max<size_t>(cells_across, 3);
+    const double dx = width / (double)(nx - 1);
+    const double rows = std::round(height / dx);
+    const int64_t ny = std::max<int64_t>((rows > 0.0 ? (int64_t)rows : 0) + 1, 3);
+    // The hall's constructor quantises a height to the same rows; the node counts are then
+    // set to the room's own, whatever the hall's rules would have made of them.
+    pm_hall *h = nullptr;
+    PM_CHECK(pm_hall_create(name ? name : "room", width, (double)(ny - 1) * dx, 0.0, (size_t)nx, speed,
+                            density, &h));
+    if (h->nx != nx || h->ny != ny || h->nz != 1 || h->dx != dx) {
+        pm_hall_destroy(h);
+        set_error("pm_room_create: node counts (%lld, %lld) do not map onto one plane",
+                  (long long)nx, (long long)ny);
+        return PM_ERR_ARGUMENT;
+    }
+    h->dims = 2;
+    h->room_depth = depth;
+    *out = h;
+    return PM_OK;
+}
+
+int pm_hall_dimensions(const pm_hall *h) { return h ? h->dims : 0; }
+
 void pm_hall_destroy(pm_hall *h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
@@ -928,6 +1013,10 @@ int pm_hall_set_source(pm_hall *h, size_t node, const double *increments, uint64
         h->src_host.clear();
         return PM_OK;
     }
+    if (h->dims == 2) {
+        set_error("pm_hall_set_source: a room has no source (room.rs)");
+        return PM_ERR_ARGUMENT;
+    }
     if ((int64_t)node >= h->nodes()) {
         set_error("pm_hall_set_source: node %zu is outside the hall", node);
         return PM_ERR_ARGUMENT;
@@ -942,12 +1031,12 @@ int pm_hall_set_source(pm_hall *h, size_t node, const double *increments, uint64
 double pm_hall_max_stable_dt(const pm_hall *h) {
     if (!h) return NAN;
     if (h->speed <= 0.0) return INFINITY;
-    return h->dx / (h->speed * std::sqrt(3.0));
+    return h->dx / (h->speed * std::sqrt((double)h->dims));
 }
 
 double pm_hall_courant(const pm_hall *h, double dt) {
     if (!h) return NAN;
-    return h->speed * dt * std::sqrt(3.0) / h->dx;
+    return h->speed * dt * std::sqrt((double)h->dims) / h->dx;
 }
 
 int pm_hall_step(pm_hall *h, double dt, pm_violation *v) {
@@ -989,11 +1078,57 @@ int pm_hall_energy(pm_hall *h, double *invariant, double *offset, double *suppli
     return PM_OK;
 }
 
+namespace pm {
+
+// A room's steps: the hall's sweep, then the room's own invariant and the audit of its ledger
+// (energy only: a room has no source), on the host, step by step.  The device audit of the
+// hall's loop sums the three-dimensional invariant and cannot serve.
+static int room_advance(pm_hall *h, double dt, uint64_t steps, double tolerance, double *totals,
+                        double *elapsed_ms, pm_violation *v) {
+    PM_HIP(hipEventRecord(h->ev0, h->stream));
+    uint64_t first = 0;
+    if (!h->staggered) {
+        int st = hall_first_step(h, dt, v, 0);
+        if (st != PM_OK) return st;
+        first = 1;
+    }
+    double inv = 0.0;
+    PM_CHECK(hall_invariant_now(h, &inv));
+    double prev_total = inv + h->energy_offset;
+    double prev_scale = rmax(0.0, std::fabs(prev_total));
+    if (first == 1 && totals) totals[0] = prev_total;
+    for (uint64_t n = first; n < steps; n++) {
+        PM_CHECK(hall_sweep(h, dt, dt, (int64_t)n, nullptr, h->energy_offset, true));
+        PM_CHECK(hall_invariant_now(h, &inv));
+        const double total = inv + h->energy_offset;
+        const double entries = rmax(0.0, std::fabs(total));
+        // Ledger::audit_with (conserved.rs:323-363), as the device audit applies it.
+        const double scale = rmax(rmax(rmax(std::fabs(prev_total), std::fabs(total)), prev_scale), entries);
+        if (totals) totals[n] = total;
+        if (!(scale < 1e-300) && std::fabs(total - prev_total) / scale > tolerance) {
+            fill_violation(v, "energy", "simulation", prev_total, total, scale, tolerance, (int64_t)n);
+            return PM_VIOLATION;
+        }
+        prev_total = total, prev_scale = entries;
+    }
+    PM_HIP(hipEventRecord(h->ev1, h->stream));
+    PM_HIP(hipEventSynchronize(h->ev1));
+    if (elapsed_ms) {
+        float ms = 0.0f;
+        PM_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        *elapsed_ms = ms;
+    }
+    return PM_OK;
+}
+
+}  // namespace pm
+
 int pm_hall_advance(pm_hall *h, double dt, uint64_t steps, double tolerance, double *totals,
                     double *elapsed_ms, pm_violation *v) {
     if (!h) return PM_ERR_ARGUMENT;
     PM_CHECK(hall_check(h, dt, v, 0));
     if (dt <= 0.0 || steps == 0) return PM_OK;
+    if (h->dims == 2) return room_advance(h, dt, steps, tolerance, totals, elapsed_ms, v);
     DevBuf dtot;
     PM_CHECK(dtot.alloc(sizeof(double) * (size_t)steps));
     PM_HIP(hipMemsetAsync(dtot.d(), 0xFF, sizeof(double) * (size_t)steps, h->stream));

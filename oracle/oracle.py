@@ -58,6 +58,11 @@ _SIG = {
     "orc_sample": (_D, [_S, _S, _S, _D, _DP, _D, _D, _D]),
     "orc_hall_sample": (_D, [_S, _S, _S, _D, _DP, _D, _D, _D]),
     "orc_room_invariant": (_D, [_S, _S, _D, _D, _D, _D, _DP, _DP, _DP, _DP]),
+    "orc_bar_step": (None, [_S, _D, _D, _D, _DP, _DP, _DP]),
+    "orc_bar_stored": (_D, [_S, _D, _DP, _D]),
+    "orc_bar_sample": (_D, [_S, _D, _DP, _D]),
+    "orc_tube_invariant": (_D, [_S, _D, _D, _D, _D, _DP, _DP, _DP]),
+    "orc_tube_sweep": (_D, [_S, _D, _D, _D, _D, _D, _D, _I, _D, _I, _D, _DP, _DP, _DP]),
     "orc_series_with_half_cell": (_D, [_D, _D, _D, _D]),
     "orc_film_flux": (_D, [_S, _S, _S, _D, _S, _S, _S, _D, _IP, _DP, _DP, _DP, _D, _D]),
     "orc_loss_conductance": (_D, [_S, _S, _S, _D, _S, _S, _S, _D, _IP, _DP, _DP, _DP, _D, _D]),
@@ -479,6 +484,144 @@ class OracleRoom(OracleHall):
                 py = math.cos(float(ny_mode) * math.pi * (float(j) * self.dx) / ly) if ly > 0 else 1.0
                 out[i + self.nx * j] = amplitude * px * py
         return out
+
+
+class OracleBar1D:
+    """Bar1D (crates/pantometry-thermal/src/lib.rs:425-660) on the CPU: cells of a uniform
+    substance, insulated ends, the bus share into cell 0 and per-face joules into their cells."""
+
+    def __init__(self, cells: int, dx: float, area: float, density: float, conductivity: float,
+                 specific_heat: float, initial: float):
+        n = max(cells, 2)
+        self.n, self.dx, self.area = n, float(dx), float(area)
+        self.cells = np.full(n, float(initial))
+        self.saved = self.cells.copy()
+        self.reference = float(initial)
+        self.absorbed = 0.0
+        self.alpha = conductivity / (density * specific_heat)
+        self.capacity = (density * (self.area * self.dx)) * specific_heat
+        self._prev = np.empty(n)
+
+    def fourier_number(self, dt: float) -> float:
+        """lib.rs:546-551."""
+        return self.alpha * dt / (self.dx * self.dx)
+
+    def max_stable_dt(self) -> float:
+        """lib.rs:563-568."""
+        return self.dx * self.dx / (2.0 * self.alpha)
+
+    def step(self, dt: float, gained: float = 0.0, joules=None) -> None:
+        f = self.fourier_number(dt)
+        if not math.isfinite(f):
+            raise ValueError("substance has no diffusivity")
+        if f > 0.5 + 1e-12:
+            raise ValueError("Fourier number")
+        self.absorbed += gained
+        j = None
+        if joules is not None:
+            j = np.ascontiguousarray(joules, dtype=np.float64)
+        lib().orc_bar_step(self.n, f, self.capacity, gained, p(j) if j is not None else None,
+                           p(self.cells), p(self._prev))
+        if j is not None:
+            total = 0.0
+            for v in j:
+                total += float(v)
+            self.absorbed += total
+
+    def stored(self) -> float:
+        return lib().orc_bar_stored(self.n, self.capacity, p(self.cells), self.reference)
+
+    def ledger(self) -> Tuple[float, float]:
+        return ledger_total([self.stored()])
+
+    def sample(self, x: float) -> float:
+        return lib().orc_bar_sample(self.n, self.dx, p(self.cells), x)
+
+
+class OracleTube:
+    """Tube (crates/pantometry-acoustic/src/lib.rs:136-563) on the CPU.  Ends: 0 closed, 1 open,
+    2 matched to an impedance."""
+
+    def __init__(self, length: float, cells: int, area: float, density: float, speed: float,
+                 left=(0, 0.0), right=(0, 0.0)):
+        n = max(cells, 3)
+        self.n = n
+        self.dx = length / float(n - 1)
+        self.area, self.density, self.speed = float(area), float(density), float(speed)
+        self.left, self.right = left, right
+        self.p = np.zeros(n)
+        self.pp = np.zeros(n)
+        self.u = np.zeros(n - 1)
+        self.staggered = False
+        self.offset = 0.0
+        self.datum = 0.0
+        self.radiated = 0.0
+
+    def impedance(self) -> float:
+        return self.density * self.speed
+
+    def release(self, pressure) -> None:
+        self.p = np.ascontiguousarray(pressure, dtype=np.float64).copy()
+        self.pp = self.p.copy()
+        self.u[:] = 0.0
+        self.staggered = False
+        self.offset = 0.0
+        self.datum = self.invariant()
+
+    def invariant(self) -> float:
+        return lib().orc_tube_invariant(self.n, self.dx, self.area, self.speed, self.density,
+                                        p(self.p), p(self.pp), p(self.u))
+
+    def energy(self) -> float:
+        return self.invariant() + self.offset
+
+    def courant(self, dt: float) -> float:
+        return self.speed * dt / self.dx
+
+    def max_stable_dt(self) -> float:
+        """lib.rs:486-514."""
+        if self.speed <= 0.0:
+            return math.inf
+        wave = self.dx / self.speed
+        rc2 = self.density * self.speed * self.speed
+
+        def drain(end):
+            kind, z = end
+            if kind == 0:
+                return math.inf
+            if kind == 1:
+                return self.dx * self.impedance() / (2.0 * rc2)
+            zz = z if z > 0.0 else self.impedance()
+            return self.dx * zz / (2.0 * rc2)
+
+        return min(min(wave, drain(self.left)), drain(self.right))
+
+    def step(self, dt: float) -> float:
+        """Domain::step, lib.rs:516-533; returns the joules published this step."""
+        if self.courant(dt) > 1.0 + 1e-12:
+            raise ValueError("Courant number")
+        if dt <= 0.0:
+            return 0.0
+        vh = dt if self.staggered else 0.5 * dt
+        starting = not self.staggered
+        self.staggered = True
+        absorbed = lib().orc_tube_sweep(self.n, self.dx, self.area, self.speed, self.density, dt,
+                                        vh, self.left[0], self.left[1], self.right[0],
+                                        self.right[1], p(self.p), p(self.pp), p(self.u))
+        published = 0.0
+        if absorbed > 0.0:
+            self.radiated += absorbed
+            published = absorbed
+        if starting:
+            inv = self.invariant()
+            off = self.datum - inv
+            scale = max(abs(self.datum), abs(inv))
+            if scale > 0.0 and abs(off) / scale <= 0.25:
+                self.offset = off
+        return published
+
+    def ledger(self) -> Tuple[float, float]:
+        return ledger_total([self.energy()])
 
 
 def _round_half_away(x: float) -> float:

@@ -719,3 +719,92 @@ void orc_workload_member(uint64_t seed, uint64_t i, double lo_edge, double hi_ed
     *edge = (long long)round(exp(orc_rng_range(&s, log(lo_edge), log(hi_edge))));
     *alpha = exp(orc_rng_range(&s, log(lo_alpha), log(hi_alpha)));
 }
+
+/* ------------------------------------------------------------------------------------------ */
+/* Bar1D, crates/pantometry-thermal/src/lib.rs:425-660                                       */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Bar1D::step after its guard, lib.rs:584-626: the bus share into cell 0, per-face joules into
+ * their cells, then the insulated three-point sweep on a copy of the previous cells. */
+void orc_bar_step(size_t n, double f, double capacity, double gained, const double *joules,
+                  double *cells, double *prev) {
+    cells[0] += gained / capacity;
+    if (joules)
+        for (size_t i = 0; i < n; i++) cells[i] += joules[i] / capacity;
+    memcpy(prev, cells, n * sizeof(double));
+    size_t last = n - 1;
+    for (size_t i = 0; i <= last; i++) {
+        double left = prev[i == 0 ? 0 : i - 1];
+        double right = prev[i + 1 < last ? i + 1 : last];
+        cells[i] = prev[i] + f * (left - 2.0 * prev[i] + right);
+    }
+}
+
+/* Bar1D::stored_heat, lib.rs:535-538: capacity times the left fold of (t - reference). */
+double orc_bar_stored(size_t n, double capacity, const double *cells, double reference) {
+    double s = 0.0;
+    for (size_t i = 0; i < n; i++) s += cells[i] - reference;
+    return capacity * s;
+}
+
+/* ScalarField::at for Bar1D, lib.rs:736-751: linear between cell centres, held past the ends. */
+double orc_bar_sample(size_t n, double dx, const double *cells, double x) {
+    size_t last = n - 1;
+    double u = x / dx - 0.5;
+    if (isnan(u) || u <= 0.0) return cells[0];
+    if (u >= (double)last) return cells[last];
+    size_t i = (size_t)floor(u);
+    double f = u - (double)i;
+    return cells[i] * (1.0 - f) + cells[i + 1] * f;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Tube, crates/pantometry-acoustic/src/lib.rs:136-563                                        */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Tube::wall_velocity, lib.rs:371-385: end 0 closed, 1 open, 2 matched to `z`. */
+static double orc_wall_velocity(int end, double z, double impedance, double p, double outward) {
+    if (end == 0) return 0.0;
+    if (end == 1 || z <= 0.0) return outward * p / impedance;
+    return outward * p / z;
+}
+
+/* Tube::invariant_si, lib.rs:300-327. */
+double orc_tube_invariant(size_t n, double dx, double area, double speed, double density,
+                          const double *p, const double *pp, const double *u) {
+    double volume = area * dx;
+    double rc2 = density * speed * speed;
+    if (rc2 <= 0.0) return 0.0;
+    size_t last = n - 1;
+    double potential = 0.0, kinetic = 0.0;
+    for (size_t i = 0; i < n; i++) {
+        double share = (i == 0 || i == last) ? 0.5 : 1.0;
+        potential += share * p[i] * pp[i] / (2.0 * rc2) * volume;
+    }
+    for (size_t i = 0; i + 1 < n; i++) kinetic += density * u[i] * u[i] / 2.0 * volume;
+    return potential + kinetic;
+}
+
+/* Tube::advance's sweeps, lib.rs:389-446: faces, wall velocities from the pressures before the
+ * update, cells with the end cells' half width, then the power through each wall.  Returns the
+ * joules the walls absorbed this step (lib.rs:436-446). */
+double orc_tube_sweep(size_t n, double dx, double area, double speed, double density, double dt,
+                      double vh, int left, double zl, int right, double zr, double *p, double *pp,
+                      double *u) {
+    double rc2 = density * speed * speed;
+    double impedance = density * speed;
+    for (size_t i = 0; i + 1 < n; i++) u[i] -= vh / (density * dx) * (p[i + 1] - p[i]);
+    double lw = orc_wall_velocity(left, zl, impedance, p[0], -1.0);
+    double rw = orc_wall_velocity(right, zr, impedance, p[n - 1], 1.0);
+    memcpy(pp, p, n * sizeof(double));
+    for (size_t i = 0; i < n; i++) {
+        double inflow = i == 0 ? lw : u[i - 1];
+        double outflow = i == n - 1 ? rw : u[i];
+        double share = (i == 0 || i == n - 1) ? 2.0 : 1.0;
+        p[i] -= dt * rc2 / dx * share * (outflow - inflow);
+    }
+    double absorbed = 0.0;
+    absorbed += fabs(p[0] * lw) * area * dt;
+    absorbed += fabs(p[n - 1] * rw) * area * dt;
+    return absorbed;
+}

@@ -290,6 +290,87 @@ int pm_room_create(const char *name, double width, double height, size_t cells_a
 int pm_hall_dimensions(const pm_hall *hall);
 
 /* ---------------------------------------------------------------------------------------- */
+/* Bar1D, crates/pantometry-thermal/src/lib.rs:425-660                                      */
+/* ---------------------------------------------------------------------------------------- */
+
+typedef struct pm_bar1d pm_bar1d;
+
+/* Bar1D::new, lib.rs:443-467: max(cells, 2) cells of `dx` metres and `area` m^2, all at
+ * `initial` kelvin, which is also the datum of the stored heat (lib.rs:535-538).  A substance
+ * whose conductivity or specific heat is NaN has no diffusivity (its steps are refused). */
+int pm_bar1d_create(const char *name, const pm_substance *substance, size_t cells, double dx,
+                    double area, double initial, pm_bar1d **out);
+void pm_bar1d_destroy(pm_bar1d *bar);
+size_t pm_bar1d_cells(const pm_bar1d *bar);
+/* Cell temperatures in and out (kelvin, cell 0 first). */
+int pm_bar1d_set_cells(pm_bar1d *bar, const double *cells);
+int pm_bar1d_get_cells(pm_bar1d *bar, double *cells);
+/* Bar1D::fourier_number (lib.rs:546-551) and Domain::max_stable_dt (lib.rs:563-568). */
+double pm_bar1d_fourier_number(const pm_bar1d *bar, double dt);
+double pm_bar1d_max_stable_dt(const pm_bar1d *bar);
+/* The guard of Bar1D::step (lib.rs:571-587) alone, so a binding can refuse before it takes its
+ * bus share: PM_VIOLATION with "Fourier number" past alpha dt/dx^2 = 1/2. */
+int pm_bar1d_refuses(const pm_bar1d *bar, double dt, pm_violation *v);
+/* Domain::step for Bar1D, lib.rs:571-627: `gained` joules (the bus share) into cell 0, then
+ * joules[i] into cell i when `joules` is not NULL (the resolved flux, one value per cell), then
+ * the insulated three-point sweep.  Both count as absorbed. */
+int pm_bar1d_step(pm_bar1d *bar, double dt, double gained, const double *joules, pm_violation *v);
+/* Bar1D::ledger (lib.rs:629-631): the stored heat, and the absorbed total (lib.rs:541-543). */
+int pm_bar1d_ledger(pm_bar1d *bar, double *stored, double *absorbed);
+/* Device-resident loop: `steps` steps with no bus input in one launch per 65536 steps, the
+ * stored heat audited after every step against the step before (relative tolerance), totals
+ * after each step to totals[0..steps] when not NULL.  A breach stops with the breaking step's
+ * state. */
+int pm_bar1d_advance(pm_bar1d *bar, double dt, uint64_t steps, double tolerance, double *totals,
+                     double *elapsed_ms, pm_violation *v);
+/* ScalarField::at for Bar1D, lib.rs:736-751: the temperature (K) at `count` positions x in
+ * metres along the bar, linear between cell centres and held past the ends. */
+int pm_bar1d_sample(pm_bar1d *bar, const double *x, size_t count, double *out);
+/* What Bar1D::readings reports (lib.rs:644-660), in kelvin and joules: the mean temperature
+ * (left fold over the cells, over their count), the peak, and the absorbed total. */
+int pm_bar1d_readings(pm_bar1d *bar, double out[3]);
+/* Domain::checkpoint / restore, lib.rs:633-639 (the saved cells start as the initial ones). */
+int pm_bar1d_checkpoint(pm_bar1d *bar);
+int pm_bar1d_restore(pm_bar1d *bar);
+
+/* ---------------------------------------------------------------------------------------- */
+/* Tube, crates/pantometry-acoustic/src/lib.rs:136-563                                      */
+/* ---------------------------------------------------------------------------------------- */
+
+typedef struct pm_tube pm_tube;
+
+/* Tube::new, lib.rs:167-193: max(cells, 3) pressure samples over `length` metres, closed ends,
+ * at rest. */
+int pm_tube_create(const char *name, double length, size_t cells, double area, double density,
+                   double speed, pm_tube **out);
+void pm_tube_destroy(pm_tube *tube);
+size_t pm_tube_cells(const pm_tube *tube);
+double pm_tube_spacing(const pm_tube *tube);
+/* Tube::with_ends, lib.rs:211-215: each end 0 closed, 1 open, 2 matched to the given
+ * impedance (Pa s/m; <= 0 means the tube's own, lib.rs:380-383). */
+int pm_tube_set_ends(pm_tube *tube, int left, double z_left, int right, double z_right);
+/* Tube::released_from, lib.rs:226-236, with the profile evaluated at every sample. */
+int pm_tube_release(pm_tube *tube, const double *pressure);
+/* Domain::max_stable_dt (lib.rs:486-514) and Tube::courant (lib.rs:358-360). */
+double pm_tube_max_stable_dt(const pm_tube *tube);
+double pm_tube_courant(const pm_tube *tube, double dt);
+/* Domain::step, lib.rs:516-533: the Courant guard, then Tube::advance (lib.rs:389-464);
+ * `published` gets the joules the ends absorbed, which the caller puts on the bus. */
+int pm_tube_step(pm_tube *tube, double dt, double *published, pm_violation *v);
+/* Tube::energy split into the invariant (lib.rs:300-327) and the startup offset, plus
+ * Tube::radiated_energy (lib.rs:330-332). */
+int pm_tube_energy(pm_tube *tube, double *invariant, double *offset, double *radiated);
+/* Device-resident loop: `steps` steps, each audited (energy plus what the ends radiated,
+ * against the step before), after the first step's startup.  Totals as pm_hall_advance. */
+int pm_tube_advance(pm_tube *tube, double dt, uint64_t steps, double tolerance, double *totals,
+                    double *elapsed_ms, pm_violation *v);
+/* pressure [n], pressure_prev [n], velocity [n-1]; any pointer may be NULL. */
+int pm_tube_get_fields(pm_tube *tube, double *p, double *pp, double *u);
+/* Domain::checkpoint / restore, lib.rs:539-557. */
+int pm_tube_checkpoint(pm_tube *tube);
+int pm_tube_restore(pm_tube *tube);
+
+/* ---------------------------------------------------------------------------------------- */
 /* Ensemble of independent heat grids, crates/pantometry-core/src/ensemble.rs:102-147       */
 /* ---------------------------------------------------------------------------------------- */
 

@@ -5,15 +5,15 @@ hand-written HIP kernels through the C ABI of ``include/pantometry_amd.h``, and 
 types they are driven through (`Ledger`, `audit_with`, `Exchange`, `Simulation`).
 """
 from ._capi import DeviceError, LIB_PATH, lib
-from .acoustic import Hall, Room
+from .acoustic import End, Hall, Room, Tube
 from .core import (ENERGY, HEAT, Exchange, Kind, Ledger, Report, Schedule, Simulation,
                    Tolerances, Violation, audit, audit_with, substeps_for)
 from .ensemble import Ensemble, Member
 from .rng import Rng
-from .thermal import STABLE_FOURIER_3D, Environment, Face, Solid3D, Substance, celsius, mm
+from .thermal import STABLE_FOURIER_3D, Bar1D, Environment, Face, Solid3D, Substance, celsius, mm
 
 __all__ = [
-    "DeviceError", "LIB_PATH", "lib", "Hall", "Room", "ENERGY", "HEAT", "Exchange", "Kind", "Ledger",
+    "DeviceError", "LIB_PATH", "lib", "Hall", "Room", "Tube", "End", "Bar1D", "ENERGY", "HEAT", "Exchange", "Kind", "Ledger",
     "Report", "Schedule", "Simulation", "Tolerances", "Violation", "audit", "audit_with",
     "substeps_for", "Ensemble", "Member", "Rng", "STABLE_FOURIER_3D", "Environment", "Face",
     "Solid3D", "Substance", "celsius", "mm",

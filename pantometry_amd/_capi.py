@@ -109,6 +109,35 @@ SIGNATURES = {
     "pm_hall_set_steps_per_pass": (_I, [_VP, _I]),
     "pm_room_create": (_I, [C.c_char_p, _D, _D, _S, _D, _D, _D, C.POINTER(_VP)]),
     "pm_hall_dimensions": (_I, [_VP]),
+    "pm_bar1d_create": (_I, [C.c_char_p, C.POINTER(pm_substance), _S, _D, _D, _D, C.POINTER(_VP)]),
+    "pm_bar1d_destroy": (None, [_VP]),
+    "pm_bar1d_cells": (_S, [_VP]),
+    "pm_bar1d_set_cells": (_I, [_VP, _DP]),
+    "pm_bar1d_get_cells": (_I, [_VP, _DP]),
+    "pm_bar1d_fourier_number": (_D, [_VP, _D]),
+    "pm_bar1d_max_stable_dt": (_D, [_VP]),
+    "pm_bar1d_refuses": (_I, [_VP, _D, _VIOL]),
+    "pm_bar1d_step": (_I, [_VP, _D, _D, _DP, _VIOL]),
+    "pm_bar1d_ledger": (_I, [_VP, _DP, _DP]),
+    "pm_bar1d_advance": (_I, [_VP, _D, _U64, _D, _DP, _DP, _VIOL]),
+    "pm_bar1d_sample": (_I, [_VP, _DP, _S, _DP]),
+    "pm_bar1d_readings": (_I, [_VP, _DP]),
+    "pm_bar1d_checkpoint": (_I, [_VP]),
+    "pm_bar1d_restore": (_I, [_VP]),
+    "pm_tube_create": (_I, [C.c_char_p, _D, _S, _D, _D, _D, C.POINTER(_VP)]),
+    "pm_tube_destroy": (None, [_VP]),
+    "pm_tube_cells": (_S, [_VP]),
+    "pm_tube_spacing": (_D, [_VP]),
+    "pm_tube_set_ends": (_I, [_VP, _I, _D, _I, _D]),
+    "pm_tube_release": (_I, [_VP, _DP]),
+    "pm_tube_max_stable_dt": (_D, [_VP]),
+    "pm_tube_courant": (_D, [_VP, _D]),
+    "pm_tube_step": (_I, [_VP, _D, _DP, _VIOL]),
+    "pm_tube_energy": (_I, [_VP, _DP, _DP, _DP]),
+    "pm_tube_advance": (_I, [_VP, _D, _U64, _D, _DP, _DP, _VIOL]),
+    "pm_tube_get_fields": (_I, [_VP, _DP, _DP, _DP]),
+    "pm_tube_checkpoint": (_I, [_VP]),
+    "pm_tube_restore": (_I, [_VP]),
     "pm_ensemble_create": (_I, [C.POINTER(pm_member), _S, C.POINTER(_VP)]),
     "pm_ensemble_destroy": (None, [_VP]),
     "pm_ensemble_set_cells": (_I, [_VP, _S, _DP]),

@@ -343,3 +343,210 @@ class Room(Hall):
         xyz = np.zeros((len(pts), 3))
         xyz[:, :2] = pts[:, :2]
         return super().sample(xyz)
+
+
+def impedance(density: float, sound_speed: float) -> float:
+    """rho c, crates/pantometry-acoustic/src/lib.rs:69-71."""
+    return density * sound_speed
+
+
+def reflection_coefficient(z_from: float, z_into: float) -> float:
+    """(Z2 - Z1)/(Z2 + Z1), lib.rs:86-92; zero for a pair summing to zero."""
+    if z_from + z_into == 0.0:
+        return 0.0
+    return (z_into - z_from) / (z_into + z_from)
+
+
+def power_transmission(z_from: float, z_into: float) -> float:
+    """1 - R^2 clamped to [0, 1], lib.rs:100-103."""
+    r = reflection_coefficient(z_from, z_into)
+    return min(max(1.0 - r * r, 0.0), 1.0)
+
+
+class End:
+    """What an end of a tube does to a wave (lib.rs:106-116): closed, open, or matched to an
+    impedance."""
+
+    CLOSED, OPEN, MATCHED = 0, 1, 2
+
+    def __init__(self, kind: int, z: float = 0.0):
+        self.kind, self.z = kind, float(z)
+
+    @staticmethod
+    def closed() -> "End":
+        return End(End.CLOSED)
+
+    @staticmethod
+    def open() -> "End":
+        return End(End.OPEN)
+
+    @staticmethod
+    def matched(z: float) -> "End":
+        return End(End.MATCHED, z)
+
+    def __eq__(self, other) -> bool:
+        return isinstance(other, End) and (self.kind, self.z) == (other.kind, other.z)
+
+    def __repr__(self) -> str:
+        return ("Closed", "Open", f"Matched({self.z})")[self.kind]
+
+
+class Tube:
+    """Tube (crates/pantometry-acoustic/src/lib.rs:136-563): pressure at the samples, particle
+    velocity on the faces between them, leapfrogged, with closed, open or matched ends."""
+
+    def __init__(self, name: str, length: float, cells: int, area: float, density: float,
+                 speed: float):
+        self._name = name
+        self._density, self._speed, self._area = float(density), float(speed), float(area)
+        handle = C.c_void_p()
+        check(lib().pm_tube_create(name.encode(), float(length), int(cells), self._area,
+                                   self._density, self._speed, C.byref(handle)), "pm_tube_create")
+        self._h = handle
+        self._n = int(lib().pm_tube_cells(self._h))
+        self._dx = float(lib().pm_tube_spacing(self._h))
+        self._ends = (End.closed(), End.closed())
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().pm_tube_destroy(h)
+            self._h = None
+
+    @staticmethod
+    def of_air(name: str, length: float, cells: int, area: float) -> "Tube":
+        """Air at 20 C: 1.204 kg/m^3, 343 m/s (lib.rs:196-205)."""
+        return Tube(name, length, cells, area, 1.204, 343.0)
+
+    def with_ends(self, left: End, right: End) -> "Tube":
+        check(lib().pm_tube_set_ends(self._h, left.kind, left.z, right.kind, right.z),
+              "pm_tube_set_ends")
+        self._ends = (left, right)
+        return self
+
+    def released_from(self, profile) -> "Tube":
+        """Tube::released_from, lib.rs:226-236; the profile sees sample positions i*dx."""
+        if callable(profile):
+            p = np.array([profile(float(i) * self._dx) for i in range(self._n)], dtype=np.float64)
+        else:
+            p = np.ascontiguousarray(profile, dtype=np.float64).reshape(-1)
+        check(lib().pm_tube_release(self._h, dptr(p)), "pm_tube_release")
+        return self
+
+    # ---- geometry and closed forms ---------------------------------------------------------
+    def name(self) -> str:
+        return self._name
+
+    def cells(self) -> int:
+        return self._n
+
+    def spacing(self) -> float:
+        return self._dx
+
+    def length(self) -> float:
+        return float(self._n - 1) * self._dx
+
+    def sound_speed(self) -> float:
+        return self._speed
+
+    def impedance(self) -> float:
+        return impedance(self._density, self._speed)
+
+    def mode_frequency(self, n: int) -> float:
+        """lib.rs:340-355: n c/2L for like ends, odd multiples of c/4L otherwise."""
+        length = self.length()
+        if length <= 0.0 or n == 0:
+            return 0.0
+        kinds = (self._ends[0].kind, self._ends[1].kind)
+        if kinds in ((End.CLOSED, End.CLOSED), (End.OPEN, End.OPEN)):
+            return float(n) * self._speed / (2.0 * length)
+        return float(2 * n - 1) * self._speed / (4.0 * length)
+
+    def courant(self, dt: float) -> float:
+        return float(lib().pm_tube_courant(self._h, float(dt)))
+
+    # ---- state ---------------------------------------------------------------------------
+    def fields(self):
+        """(pressure, pressure_prev, velocity)."""
+        p, pp, u = np.empty(self._n), np.empty(self._n), np.empty(self._n - 1)
+        check(lib().pm_tube_get_fields(self._h, dptr(p), dptr(pp), dptr(u)), "pm_tube_get_fields")
+        return p, pp, u
+
+    def pressure(self) -> np.ndarray:
+        return self.fields()[0]
+
+    def pressure_at(self, cell: int) -> float:
+        return float(self.pressure()[min(cell, self._n - 1)])
+
+    def velocity_at(self, face: int) -> float:
+        return float(self.fields()[2][min(face, self._n - 2)])
+
+    def peak_pressure(self) -> float:
+        peak = 0.0
+        for v in self.pressure():
+            peak = max(peak, abs(float(v)))
+        return peak
+
+    def energy_parts(self) -> Tuple[float, float, float]:
+        inv, off, rad = C.c_double(), C.c_double(), C.c_double()
+        check(lib().pm_tube_energy(self._h, C.byref(inv), C.byref(off), C.byref(rad)),
+              "pm_tube_energy")
+        return inv.value, off.value, rad.value
+
+    def energy(self) -> float:
+        """Tube::energy, lib.rs:288-290: the invariant plus the startup offset."""
+        inv, off, _ = self.energy_parts()
+        return inv + off
+
+    def startup_adjustment(self) -> float:
+        return self.energy_parts()[1]
+
+    def radiated_energy(self) -> float:
+        return self.energy_parts()[2]
+
+    # ---- Domain ----------------------------------------------------------------------------
+    def kind(self) -> Kind:
+        return Kind.EVOLVING
+
+    def books_balance(self) -> bool:
+        return True
+
+    def supports_restore(self) -> bool:
+        return True
+
+    def residual(self) -> float:
+        return 0.0
+
+    def max_stable_dt(self, now: float = 0.0) -> float:
+        return float(lib().pm_tube_max_stable_dt(self._h))
+
+    def step(self, t: float, dt: float, bus: Exchange) -> None:
+        """Domain::step, lib.rs:516-533: what the ends absorbed goes onto the bus as energy."""
+        v = _capi.pm_violation()
+        published = C.c_double(0.0)
+        check(lib().pm_tube_step(self._h, float(dt), C.byref(published), C.byref(v)),
+              "pm_tube_step", v)
+        if published.value > 0.0:
+            bus.publish(ENERGY, published.value)
+
+    def ledger(self) -> Ledger:
+        return Ledger().with_(ENERGY, self.energy())
+
+    def checkpoint(self) -> None:
+        check(lib().pm_tube_checkpoint(self._h), "pm_tube_checkpoint")
+
+    def restore(self) -> None:
+        check(lib().pm_tube_restore(self._h), "pm_tube_restore")
+
+    def readings(self):
+        return [(self._name, "peak", self.peak_pressure(), "Pa")]
+
+    def advance(self, dt: float, steps: int, tolerance: float = 1e-9):
+        """`steps` steps, each audited on the device (energy plus what the ends radiated).
+        Returns (books after each step, device milliseconds)."""
+        v = _capi.pm_violation()
+        totals = np.empty(max(steps, 1))
+        ms = C.c_double(0.0)
+        check(lib().pm_tube_advance(self._h, float(dt), int(steps), float(tolerance), dptr(totals),
+                                    C.byref(ms), C.byref(v)), "pm_tube_advance", v)
+        return totals[:steps], ms.value

@@ -16,7 +16,7 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 BUILD = HERE / "_build"
 LIB = HERE / "libpantometry_amd.so"
-SOURCES = ["host.cpp", "heat.hip", "hall.hip", "ensemble.hip", "measure.hip", "nn.hip", "surrogate.hip"]
+SOURCES = ["host.cpp", "heat.hip", "hall.hip", "ensemble.hip", "measure.hip", "nn.hip", "surrogate.hip", "line.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -427,3 +427,150 @@ class Solid3D:
 
     def steps_per_pass(self) -> int:
         return int(lib().pm_solid3d_steps_per_pass(self._h))
+
+
+class Bar1D:
+    """Bar1D (crates/pantometry-thermal/src/lib.rs:425-660): explicit one-dimensional conduction
+    with insulated ends, on the GPU.  Temperatures in kelvin at cell centres (i + 1/2) dx."""
+
+    def __init__(self, name: str, substance: Substance, cells: int, dx: float, area: float,
+                 initial: float):
+        self._name = name
+        self._substance = substance
+        self._dx = float(dx)
+        self._area = float(area)
+        handle = C.c_void_p()
+        sub = substance._c()
+        check(lib().pm_bar1d_create(name.encode(), C.byref(sub), int(cells), self._dx,
+                                    self._area, float(initial), C.byref(handle)),
+              "pm_bar1d_create")
+        self._h = handle
+        self._n = int(lib().pm_bar1d_cells(self._h))
+        self._face_area: Optional[float] = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().pm_bar1d_destroy(h)
+            self._h = None
+
+    def exposing(self, face_area: float) -> "Bar1D":
+        """Bar1D::exposing (lib.rs:483-486): each cell has a face of `face_area` that heat can be
+        delivered onto, one value per cell, by `step_with_flux`."""
+        self._face_area = float(face_area)
+        return self
+
+    # ---- state ---------------------------------------------------------------------------
+    def name(self) -> str:
+        return self._name
+
+    def cell_count(self) -> int:
+        return self._n
+
+    def cells(self) -> np.ndarray:
+        out = np.empty(self._n)
+        check(lib().pm_bar1d_get_cells(self._h, dptr(out)), "pm_bar1d_get_cells")
+        return out
+
+    def set_cells(self, cells) -> None:
+        c = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1)
+        if c.size != self._n:
+            raise ValueError(f"{c.size} temperatures for {self._n} cells")
+        check(lib().pm_bar1d_set_cells(self._h, dptr(c)), "pm_bar1d_set_cells")
+
+    def temperature_at(self, index: int) -> float:
+        """lib.rs:494-496: clamped to the ends."""
+        return float(self.cells()[min(index, self._n - 1)])
+
+    def _readings(self) -> Tuple[float, float, float]:
+        out = (C.c_double * 3)()
+        check(lib().pm_bar1d_readings(self._h, out), "pm_bar1d_readings")
+        return out[0], out[1], out[2]
+
+    def mean_temperature(self) -> float:
+        """lib.rs:504-506: the left fold over the cells, over their count."""
+        return self._readings()[0]
+
+    def end_to_end(self) -> float:
+        c = self.cells()
+        return float(c[-1] - c[0])
+
+    def absorbed_energy(self) -> float:
+        absorbed = C.c_double()
+        check(lib().pm_bar1d_ledger(self._h, None, C.byref(absorbed)), "pm_bar1d_ledger")
+        return absorbed.value
+
+    def fourier_number(self, dt: float) -> float:
+        return float(lib().pm_bar1d_fourier_number(self._h, float(dt)))
+
+    # ---- Domain ----------------------------------------------------------------------------
+    def kind(self) -> Kind:
+        return Kind.EVOLVING
+
+    def books_balance(self) -> bool:
+        return True
+
+    def supports_restore(self) -> bool:
+        return True
+
+    def residual(self) -> float:
+        return 0.0
+
+    def max_stable_dt(self, now: float = 0.0) -> float:
+        return float(lib().pm_bar1d_max_stable_dt(self._h))
+
+    def step(self, t: float, dt: float, bus: Exchange, joules=None) -> None:
+        """Domain::step, lib.rs:571-627: refuse first, then take the bus share into cell 0 and
+        any per-cell joules into their cells, then conduct."""
+        v = _capi.pm_violation()
+        check(lib().pm_bar1d_refuses(self._h, float(dt), C.byref(v)), "pm_bar1d_refuses", v)
+        gained = bus.take_share(HEAT, dt)
+        j = None
+        if joules is not None:
+            j = np.ascontiguousarray(joules, dtype=np.float64).reshape(-1)
+            if j.size != self._n:
+                raise Violation.at(self._name, "flux on a different grid", float(j.size))
+        check(lib().pm_bar1d_step(self._h, float(dt), float(gained),
+                                  dptr(j) if j is not None else None, C.byref(v)),
+              "pm_bar1d_step", v)
+
+    def stored_heat(self) -> float:
+        stored = C.c_double()
+        check(lib().pm_bar1d_ledger(self._h, C.byref(stored), None), "pm_bar1d_ledger")
+        return stored.value
+
+    def ledger(self) -> Ledger:
+        """lib.rs:629-631: the heat stored since the start."""
+        return Ledger().with_(ENERGY, self.stored_heat())
+
+    def checkpoint(self) -> None:
+        check(lib().pm_bar1d_checkpoint(self._h), "pm_bar1d_checkpoint")
+
+    def restore(self) -> None:
+        check(lib().pm_bar1d_restore(self._h), "pm_bar1d_restore")
+
+    def readings(self):
+        """lib.rs:644-660: mean and peak in celsius, and what it has absorbed."""
+        mean, peak, absorbed = self._readings()
+        return [(self._name, "mean", mean - 273.15, "C"),
+                (self._name, "peak", peak - 273.15, "C"),
+                (self._name, "absorbed", absorbed, "J")]
+
+    def sample(self, xs) -> np.ndarray:
+        """ScalarField::at (lib.rs:736-751) at each x in metres (or each (x, y, z) point, y and z
+        ignored): linear between cell centres, held past the ends."""
+        pts = np.asarray(xs, dtype=np.float64)
+        x = np.ascontiguousarray(pts[:, 0] if pts.ndim == 2 else pts.reshape(-1))
+        out = np.empty(len(x))
+        check(lib().pm_bar1d_sample(self._h, dptr(x), len(x), dptr(out)), "pm_bar1d_sample")
+        return out
+
+    def advance(self, dt: float, steps: int, tolerance: float = 1e-9):
+        """`steps` steps with no bus input, each audited on the device.  Returns (stored heat
+        after each step, device milliseconds)."""
+        v = _capi.pm_violation()
+        totals = np.empty(max(steps, 1))
+        ms = C.c_double(0.0)
+        check(lib().pm_bar1d_advance(self._h, float(dt), int(steps), float(tolerance), dptr(totals),
+                                     C.byref(ms), C.byref(v)), "pm_bar1d_advance", v)
+        return totals[:steps], ms.value

@@ -192,6 +192,66 @@ def bench_surrogate(args) -> dict:
     }
 
 
+def bench_lines(args) -> dict:
+    """The lower-dimensional domains at the reference's own sizes and at the largest line one
+    workgroup holds in LDS: device-resident advance (audit every step) against the oracle's
+    step on this host, both per step."""
+    import math
+    import numpy as np
+    import pantometry_amd as pm
+    from oracle.oracle import OracleBar1D, OracleRoom, OracleTube
+    steps = max(args.steps, 100)
+    rows = {}
+
+    def cpu_time(step, budget=1.5):
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < budget and n < 20000:
+            step()
+            n += 1
+        return (time.perf_counter() - t0) / n * 1e6
+
+    al = pm.Substance.aluminium_6061()
+    for cells in (21, 2048):
+        bar = pm.Bar1D("bar", al, cells, 1e-3, 1e-4, 293.15)
+        c = np.full(cells, 293.15)
+        c[cells // 2] += 40.0
+        bar.set_cells(c)
+        dt = 0.9 * bar.max_stable_dt()
+        bar.advance(dt, args.warmup)
+        _, ms = bar.advance(dt, steps)
+        o = OracleBar1D(cells, 1e-3, 1e-4, al.density, al.conductivity, al.specific_heat, 293.15)
+        o.cells = c.copy()
+        rows[f"bar {cells} cells"] = {"gpu_us_per_step": round(ms * 1e3 / steps, 3),
+                                      "cpu_us_per_step": round(cpu_time(lambda: o.step(dt)), 3)}
+    for cells in (201, 2048):
+        tube = pm.Tube.of_air("tube", 1.0, cells, 1e-4)
+        tube.released_from(lambda x: 200.0 * math.exp(-((x - 0.5) / 0.08) ** 2))
+        dt = tube.max_stable_dt()
+        tube.advance(dt, args.warmup)
+        _, ms = tube.advance(dt, steps)
+        o = OracleTube(1.0, cells, 1e-4, 1.204, 343.0)
+        o.release(np.array([200.0 * math.exp(-((i * o.dx - 0.5) / 0.08) ** 2) for i in range(o.n)]))
+        rows[f"tube {cells} cells"] = {"gpu_us_per_step": round(ms * 1e3 / steps, 3),
+                                       "cpu_us_per_step": round(cpu_time(lambda: o.step(dt)), 3)}
+    for cells in (81, 241):
+        room = pm.Room.of_air("room", 4.4, 3.1, cells)
+        room.released_in_mode((1, 1), 1.0)
+        dt = room.max_stable_dt()
+        room.advance(dt, args.warmup)
+        rsteps = min(steps, 500)
+        _, ms = room.advance(dt, rsteps)
+        o = OracleRoom(4.4, 3.1, cells, 1.0, 343.0, 1.204)
+        o.release(o.mode2(1, 1, 1.0))
+        rows[f"room {cells} nodes across"] = {
+            "gpu_us_per_step": round(ms * 1e3 / rsteps, 3),
+            "cpu_us_per_step": round(cpu_time(lambda: (o.step(dt), o.invariant())), 3)}
+    return {"metric": "microseconds per audited step, lower-dimensional domains",
+            "workload": "lines", "steps": steps, "rows": rows,
+            "note": "GPU: device-resident advance with the audit every step (a line: one "
+                    "launch for all steps; a room: three launches a step); CPU: the oracle's "
+                    "step (plus the room's invariant) from Python on one core"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -202,7 +262,7 @@ def main() -> int:
     ap.add_argument("--z-chunk", type=int, default=0)
     ap.add_argument("--steps-per-pass", type=int, choices=[1, 2], default=2,
                     help="2 (default): pairs of steps per pass over HBM; 1: one step per pass")
-    ap.add_argument("--workload", choices=["heat", "hall", "ensemble", "surrogate", "copy"],
+    ap.add_argument("--workload", choices=["heat", "hall", "ensemble", "surrogate", "copy", "lines"],
                     default="heat",
                     help="heat (the headline 512^3 block), or the hall / ensemble / surrogate "
                          "secondaries")
@@ -221,6 +281,9 @@ def main() -> int:
         return 0
     if args.workload == "ensemble":
         print(json.dumps(bench_ensemble(args)), flush=True)
+        return 0
+    if args.workload == "lines":
+        print(json.dumps(bench_lines(args)), flush=True)
         return 0
     if args.workload == "surrogate":
         print(json.dumps(bench_surrogate(args)), flush=True)

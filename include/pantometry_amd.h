@@ -283,7 +283,7 @@ int pm_hall_set_steps_per_pass(pm_hall *hall, int steps);
  * and every pm_hall_* call applies, under the room's rules: the Courant number
  * c dt sqrt(2)/dx and its limit dx/(c sqrt 2) (room.rs:336-338, 361-366, 369-379), the room's
  * invariant for energy and startup (room.rs:243-281, 460-475), sampling in x and y only
- * (room.rs:587-600), no source.  pm_hall_advance audits each step's energy on the host. */
+ * (room.rs:587-600), no source.  pm_hall_advance audits each step's energy on the device. */
 int pm_room_create(const char *name, double width, double height, size_t cells_across,
                    double depth, double density, double speed, pm_hall **out);
 /* 2 for a room, 3 for a hall. */

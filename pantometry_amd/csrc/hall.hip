@@ -538,6 +538,30 @@ __global__ __launch_bounds__(256) void room_invariant_kernel(const double *p, co
     if (threadIdx.x == 0) partials[b] = total;
 }
 
+// The room's invariant from room_invariant_kernel's partials: each group summed in a fixed
+// order, added as potential + kinetic_x + kinetic_y (room.rs:280) into out[0]; with an audit
+// state, the ledger (energy = invariant + offset) is audited as step `step`.  Nothing runs once
+// an audit has halted the run.
+__global__ __launch_bounds__(1024) void room_total(const double *part, int64_t bp, int64_t bx,
+                                                   int64_t by, AuditState *st, double offset,
+                                                   int64_t step, double *totals, double *out) {
+    __shared__ double red[16];
+    if (st && st->halted) return;
+    const double potential = fixed_order_sum_1024(part, bp, red);
+    __syncthreads();
+    const double kinetic_x = fixed_order_sum_1024(part + bp, bx, red);
+    __syncthreads();
+    const double kinetic_y = fixed_order_sum_1024(part + bp + bx, by, red);
+    if (threadIdx.x == 0) {
+        const double inv = potential + kinetic_x + kinetic_y;
+        out[0] = inv;
+        if (st) {
+            const double energy = inv + offset;
+            audit_step(st, energy, std::fabs(energy), step, totals);
+        }
+    }
+}
+
 // Sums the step's partials into the invariant and audits energy + offset - supplied.
 __global__ __launch_bounds__(1024) void hall_finalize(const double *partials, int64_t n,
                                                       AuditState *st, double offset,
@@ -639,6 +663,8 @@ struct pm_hall {
     // room `room_depth` metres deep.
     int dims = 3;
     double room_depth = 0.0;
+    DevBuf room_part;  // the room invariant's per-block partials
+    int64_t room_groups[3] = {0, 0, 0};
     hipStream_t stream = nullptr;
     // Pressure rotates through three buffers: current, previous (pressure_prev) and a spare the
     // two-steps-per-pass kernel writes its intermediate pressure into.  Velocities ping-pong.
@@ -688,6 +714,22 @@ static HallGrid hall_grid(const pm_hall *h, int tile_y = 0) {
     return g;
 }
 
+// The room's invariant, per block of 256 terms, into room_part (potential blocks, then vx, then
+// vy; the three counts in room_groups).
+static int room_partials(pm_hall *h, const double *p, const double *pp, const double *vx,
+                         const double *vy) {
+    const int64_t bp = (h->nodes() + 255) / 256, bx = (h->nfx() + 255) / 256;
+    const int64_t by = (h->nfy() + 255) / 256;
+    if (!h->room_part) PM_CHECK(h->room_part.alloc(sizeof(double) * (size_t)(bp + bx + by)));
+    h->room_groups[0] = bp, h->room_groups[1] = bx, h->room_groups[2] = by;
+    const double rc2 = h->density * h->speed * h->speed;
+    room_invariant_kernel<<<(unsigned)(bp + bx + by), 256, 0, h->stream>>>(
+        p, pp, vx, vy, (int)h->nx, (int)h->ny, rc2, h->dx * h->dx * h->room_depth, h->density,
+        h->room_part.d());
+    PM_HIP(hipGetLastError());
+    return PM_OK;
+}
+
 static int hall_invariant(pm_hall *h, const double *p, const double *pp, const double *vx,
                           const double *vy, const double *vz, double *out) {
     const double rc2 = h->density * h->speed * h->speed;
@@ -696,23 +738,12 @@ static int hall_invariant(pm_hall *h, const double *p, const double *pp, const d
         return PM_OK;
     }
     if (h->dims == 2) {
-        // The Room's invariant: three groups, each summed in a fixed order, then added as
-        // potential + kinetic_x + kinetic_y (room.rs:280).
-        const int64_t bp = (h->nodes() + 255) / 256, bx = (h->nfx() + 255) / 256;
-        const int64_t by = (h->nfy() + 255) / 256;
-        const int64_t blocks = bp + bx + by;
-        DevBuf part;
-        PM_CHECK(part.alloc(sizeof(double) * (size_t)blocks));
-        const double volume = h->dx * h->dx * h->room_depth;
-        room_invariant_kernel<<<(unsigned)blocks, 256, 0, h->stream>>>(
-            p, pp, vx, vy, (int)h->nx, (int)h->ny, rc2, volume, h->density, part.d());
+        PM_CHECK(room_partials(h, p, pp, vx, vy));
+        room_total<<<1, 1024, 0, h->stream>>>(h->room_part.d(), h->room_groups[0], h->room_groups[1],
+                                              h->room_groups[2], nullptr, 0.0, -1, nullptr, h->scalars.d());
         PM_HIP(hipGetLastError());
-        std::vector<double> hp((size_t)blocks);
-        PM_HIP(hipMemcpyAsync(hp.data(), part.d(), sizeof(double) * hp.size(), hipMemcpyDeviceToHost, h->stream));
+        PM_HIP(hipMemcpyAsync(out, h->scalars.d(), sizeof(double), hipMemcpyDeviceToHost, h->stream));
         PM_HIP(hipStreamSynchronize(h->stream));
-        double g[3] = {0.0, 0.0, 0.0};
-        for (int64_t b = 0; b < blocks; b++) g[b < bp ? 0 : b < bp + bx ? 1 : 2] += hp[(size_t)b];
-        *out = g[0] + g[1] + g[2];
         return PM_OK;
     }
     const int64_t n = h->nodes();
@@ -814,7 +845,7 @@ static int hall_sweep(pm_hall *h, double hstep, double vh, int64_t step, double 
     else
         hall_march<8><<<blocks, dim3(64, 8), 0, h->stream>>>(a, g.gx, g.gy, g.gz, g.zc);
     PM_HIP(hipGetLastError());
-    if (!force) {
+    if (!force && h->dims == 3) {  // a room audits its own invariant (room_total)
         hall_finalize<<<1, 1024, 0, h->stream>>>(h->partials.d(), nparts, h->audit.as<AuditState>(), offset,
                                                  h->scalars.d() + 1, sourced ? h->scalars.d() + 2 : nullptr, step,
                                                  totals_dev, h->scalars.d());
@@ -1080,9 +1111,10 @@ int pm_hall_energy(pm_hall *h, double *invariant, double *offset, double *suppli
 
 namespace pm {
 
-// A room's steps: the hall's sweep, then the room's own invariant and the audit of its ledger
-// (energy only: a room has no source), on the host, step by step.  The device audit of the
-// hall's loop sums the three-dimensional invariant and cannot serve.
+// A room's steps: the hall's sweep (its fused three-dimensional audit left out), then the
+// room's own invariant and the audit of its ledger (energy only: a room has no source), all on
+// the device.  The marching kernel returns at once after a breach, so the buffers that step
+// wrote hold the state the breaking step made; the host finds them from the step index.
 static int room_advance(pm_hall *h, double dt, uint64_t steps, double tolerance, double *totals,
                         double *elapsed_ms, pm_violation *v) {
     PM_HIP(hipEventRecord(h->ev0, h->stream));
@@ -1094,29 +1126,41 @@ static int room_advance(pm_hall *h, double dt, uint64_t steps, double tolerance,
     }
     double inv = 0.0;
     PM_CHECK(hall_invariant_now(h, &inv));
-    double prev_total = inv + h->energy_offset;
-    double prev_scale = rmax(0.0, std::fabs(prev_total));
-    if (first == 1 && totals) totals[0] = prev_total;
+    const double total0 = inv + h->energy_offset;
+    if (first == 1 && totals) totals[0] = total0;
+    PM_CHECK(hall_reset_audit(h, total0, std::fabs(total0), tolerance));
+    DevBuf dtot;
+    PM_CHECK(dtot.alloc(sizeof(double) * (size_t)steps));
+    struct Buffers {
+        int ip, iq, is, iv;
+    };
+    std::vector<Buffers> after;  // the buffers holding the state after each step
     for (uint64_t n = first; n < steps; n++) {
-        PM_CHECK(hall_sweep(h, dt, dt, (int64_t)n, nullptr, h->energy_offset, true));
-        PM_CHECK(hall_invariant_now(h, &inv));
-        const double total = inv + h->energy_offset;
-        const double entries = rmax(0.0, std::fabs(total));
-        // Ledger::audit_with (conserved.rs:323-363), as the device audit applies it.
-        const double scale = rmax(rmax(rmax(std::fabs(prev_total), std::fabs(total)), prev_scale), entries);
-        if (totals) totals[n] = total;
-        if (!(scale < 1e-300) && std::fabs(total - prev_total) / scale > tolerance) {
-            fill_violation(v, "energy", "simulation", prev_total, total, scale, tolerance, (int64_t)n);
-            return PM_VIOLATION;
-        }
-        prev_total = total, prev_scale = entries;
+        PM_CHECK(hall_sweep(h, dt, dt, (int64_t)n, nullptr, h->energy_offset));
+        PM_CHECK(room_partials(h, h->p[h->ip].d(), h->p[h->iq].d(), h->vx[h->iv].d(), h->vy[h->iv].d()));
+        room_total<<<1, 1024, 0, h->stream>>>(h->room_part.d(), h->room_groups[0], h->room_groups[1],
+                                              h->room_groups[2], h->audit.as<AuditState>(), h->energy_offset,
+                                              (int64_t)n, dtot.d(), h->scalars.d());
+        PM_HIP(hipGetLastError());
+        after.push_back({h->ip, h->iq, h->is, h->iv});
     }
     PM_HIP(hipEventRecord(h->ev1, h->stream));
-    PM_HIP(hipEventSynchronize(h->ev1));
+    AuditState st;
+    PM_HIP(hipMemcpyAsync(&st, h->audit.d(), sizeof st, hipMemcpyDeviceToHost, h->stream));
+    PM_HIP(hipStreamSynchronize(h->stream));
+    const uint64_t done = st.halted ? (uint64_t)st.breach_step + 1 : steps;
+    if (totals && done > first)
+        PM_HIP(hipMemcpy(totals + first, dtot.d() + first, sizeof(double) * (done - first), hipMemcpyDeviceToHost));
     if (elapsed_ms) {
         float ms = 0.0f;
         PM_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
         *elapsed_ms = ms;
+    }
+    if (st.halted) {
+        const Buffers &b = after[(size_t)(st.breach_step - (int64_t)first)];
+        h->ip = b.ip, h->iq = b.iq, h->is = b.is, h->iv = b.iv;
+        fill_violation(v, "energy", "simulation", st.before, st.after, st.scale, tolerance, st.breach_step);
+        return PM_VIOLATION;
     }
     return PM_OK;
 }

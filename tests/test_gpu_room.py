@@ -196,3 +196,28 @@ def test_checkpoint_restore_and_degenerate_rooms():
     assert not math.isfinite(silent.max_stable_dt())
     silent.step(0.0, 1.0, pm.Exchange())
     assert silent.peak_pressure() == 0.0
+
+
+def test_a_breach_leaves_the_state_of_the_breaking_step():
+    # The room's audit runs on the device; once it halts, later sweeps do nothing and the handle
+    # points at the buffers the breaking step wrote.  A tolerance of 1e-300 breaks on the
+    # first step whose energy moves by rounding at all.
+    gpu_case("room breach")
+    from pantometry_amd.core import Violation
+    g, o = _pair(4.4, 3.1, 37)
+    p0 = o.mode2(2, 1, 5.0) + _gauss(o, 1.7, 1.2, 0.2)
+    g.released_from(p0.reshape(o.ny, o.nx))
+    o.release(p0)
+    dt = 0.8 * o.max_stable_dt()
+    g.advance(dt, 3, 1e-9)
+    for _ in range(3):
+        o.step(dt)
+    try:
+        g.advance(dt, 40, 1e-300)
+        taken = 40
+    except Violation as err:
+        assert err.quantity == "energy"
+        taken = err.step + 1
+    for _ in range(taken):
+        o.step(dt)
+    _same_fields(g, o)

@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import sys
 import os
 import subprocess
 from pathlib import Path
@@ -25,6 +26,8 @@ _S = C.c_size_t
 _U64 = C.c_uint64
 _I = C.c_int
 _IP = C.POINTER(C.c_int)
+_U8P = C.POINTER(C.c_ubyte)
+_SP = C.POINTER(C.c_size_t)
 
 _SIG = {
     "orc_mix64": (_U64, [_U64]),
@@ -58,6 +61,12 @@ _SIG = {
     "orc_sample": (_D, [_S, _S, _S, _D, _DP, _D, _D, _D]),
     "orc_hall_sample": (_D, [_S, _S, _S, _D, _DP, _D, _D, _D]),
     "orc_room_invariant": (_D, [_S, _S, _D, _D, _D, _D, _DP, _DP, _DP, _DP]),
+    "orc_resolve_void": (_D, [_S, _S, _S, _D, _DP, _DP, _U8P, _DP, _DP, _DP, _DP, _DP]),
+    "orc_solid_bus_heat_void": (None, [_S, _DP, _U8P, _D, _DP]),
+    "orc_find_gaps": (_S, [_S, _S, _S, _D, _U8P, _DP, _SP, _SP, _DP]),
+    "orc_gap_exchange": (None, [_S, _SP, _SP, _DP, _DP, _DP, _D, _DP]),
+    "orc_gap_rates": (_D, [_S, _SP, _SP, _DP, _DP, _DP, _D]),
+    "orc_sample_void": (_D, [_S, _S, _S, _D, _DP, _U8P, _D, _D, _D]),
     "orc_bar_step": (None, [_S, _D, _D, _D, _DP, _DP, _DP]),
     "orc_bar_stored": (_D, [_S, _D, _DP, _D]),
     "orc_bar_sample": (_D, [_S, _D, _DP, _D]),
@@ -98,6 +107,14 @@ def lib() -> C.CDLL:
 def threads() -> int:
     n = os.cpu_count() or 1
     return max(1, min(16, n))
+
+
+def _u8(a: np.ndarray):
+    return a.ctypes.data_as(_U8P)
+
+
+def _sp(a: np.ndarray):
+    return a.ctypes.data_as(_SP)
 
 
 def p(a: np.ndarray):
@@ -194,21 +211,71 @@ class OracleSolid3D:
         self.source = np.zeros(self.n)             # watts per cell
         self.lost = 0.0
         self.supplied = 0.0
+        # Void cells (solid.rs:238, 992-1005) and the gaps between solid cells across them.
+        self.voids = np.zeros(self.n, dtype=np.uint8)
+        self.gap_a = np.zeros(0, dtype=np.uintp)
+        self.gap_b = np.zeros(0, dtype=np.uintp)
+        self.gap_coef = np.zeros(0)
         self.resolve()
 
     def resolve(self) -> None:
+        """Solid3D::resolve (solid.rs:588-784) and find_gaps (solid.rs:791-847)."""
         nx, ny, nz = self.counts_
+        self.kcell[self.voids != 0] = 0.0
+        self.cap[self.voids != 0] = 0.0
         self.kx = np.empty((nx + 1) * ny * nz)
         self.ky = np.empty(nx * (ny + 1) * nz)
         self.kz = np.empty(nx * ny * (nz + 1))
         self.mob = np.empty(self.n)
         self.face_sum = np.empty(self.n)
-        self.worst_rate = lib().orc_resolve(nx, ny, nz, self.dx, p(self.kcell), p(self.cap),
-                                            p(self.kx), p(self.ky), p(self.kz), p(self.mob),
-                                            p(self.face_sum))
+        self.worst_rate = lib().orc_resolve_void(nx, ny, nz, self.dx, p(self.kcell), p(self.cap),
+                                                 _u8(self.voids), p(self.kx), p(self.ky),
+                                                 p(self.kz), p(self.mob), p(self.face_sum))
+        a = np.zeros(3 * self.n, dtype=np.uintp)
+        b = np.zeros(3 * self.n, dtype=np.uintp)
+        coef = np.zeros(3 * self.n)
+        count = lib().orc_find_gaps(nx, ny, nz, self.dx, _u8(self.voids), p(self.emis), _sp(a),
+                                    _sp(b), p(coef))
+        self.gap_a, self.gap_b, self.gap_coef = a[:count].copy(), b[:count].copy(), coef[:count].copy()
+
+    def empty(self, mask: np.ndarray):
+        """Solid3D::empty, solid.rs:992-1005: the selected cells hold nothing from now on."""
+        m = np.asarray(mask).reshape(-1).astype(bool)
+        self.voids[m] = 1
+        self.resolve()
+        return self
+
+    def is_void(self, i: int, j: int, k: int) -> bool:
+        nx, ny, nz = self.counts_
+        if i >= nx or j >= ny or k >= nz:
+            return False
+        return bool(self.voids[i + nx * (j + ny * k)])
+
+    def void_cells(self) -> int:
+        return int(np.count_nonzero(self.voids))
+
+    def temperature_at(self, i: int, j: int, k: int) -> float:
+        """solid.rs:1323-1335: clamped, NaN for a void cell."""
+        nx, ny, nz = self.counts_
+        c = min(i, nx - 1) + nx * (min(j, ny - 1) + ny * min(k, nz - 1))
+        return math.nan if self.voids[c] else float(self.cells[c])
+
+    def statistics(self) -> Tuple[float, float, float]:
+        """(peak, mean, coldest) over the cells that are there (solid.rs:1377-1405)."""
+        peak, coldest, total, count = -sys.float_info.max, sys.float_info.max, 0.0, 0
+        for c in range(self.n):
+            if self.voids[c]:
+                continue
+            t = float(self.cells[c])
+            peak = max(peak, t) if not math.isnan(t) else peak
+            coldest = min(coldest, t) if not math.isnan(t) else coldest
+            total += t
+            count += 1
+        return peak, (total / count if count else math.nan), coldest
 
     def fill(self, density: float, conductivity: float, specific_heat: float, mask: np.ndarray,
              emissivity: float = 0.0):
+        """Solid3D::fill, solid.rs:519-543: a void cell stays void."""
         m = np.asarray(mask).reshape(-1).astype(bool)
         self.kcell[m] = conductivity
         self.cap[m] = lib().orc_heat_capacity(density, specific_heat, self.dx)
@@ -229,7 +296,7 @@ class OracleSolid3D:
     def dissipating(self, watts: float, mask: np.ndarray):
         """Solid3D::dissipating, solid.rs:1058-1083 (no void cells): `watts` spread evenly over the
         selected cells, replacing what they generated before."""
-        m = np.asarray(mask).reshape(-1).astype(bool)
+        m = np.asarray(mask).reshape(-1).astype(bool) & (self.voids == 0)
         chosen = int(m.sum())
         if chosen:
             self.source[m] = watts / float(chosen)
@@ -239,6 +306,14 @@ class OracleSolid3D:
         return bool(self.on.any())
 
     def worst_rate_now(self) -> float:
+        nx, ny, nz = self.counts_
+        if len(self.gap_a) and not math.isnan(self.worst_rate):
+            w = self._film_rate()
+            return lib().orc_gap_rates(len(self.gap_a), _sp(self.gap_a), _sp(self.gap_b),
+                                       p(self.gap_coef), p(self.cells), p(self.cap), w)
+        return self._film_rate()
+
+    def _film_rate(self) -> float:
         nx, ny, nz = self.counts_
         return lib().orc_worst_rate_now(nx, ny, nz, self.dx, self.worst_rate, p(self.mob),
                                         p(self.face_sum), p(self.cells), _ip(self.on),
@@ -274,8 +349,10 @@ class OracleSolid3D:
         nx, ny, nz = self.counts_
         if gained != 0.0:
             self.absorbed += gained
-            lib().orc_solid_bus_heat(self.n, p(self.cap), gained, p(self.cells))
-        if self.exposed() or self.source.any():
+            lib().orc_solid_bus_heat_void(self.n, p(self.cap), _u8(self.voids), gained,
+                                          p(self.cells))
+        gaps = len(self.gap_a)
+        if self.exposed() or self.source.any() or gaps:
             lost, supplied = C.c_double(self.lost), C.c_double(self.supplied)
             lib().orc_solid_step_ext(nx, ny, nz, self.dx, p(self.kx), p(self.ky), p(self.kz),
                                      p(self.mob), p(self.cap), dt, p(self.cells), p(self._old),
@@ -283,6 +360,9 @@ class OracleSolid3D:
                                      p(self.emis), p(self.ksub), p(self.source), C.byref(lost),
                                      C.byref(supplied))
             self.lost, self.supplied = lost.value, supplied.value
+            if gaps:
+                lib().orc_gap_exchange(gaps, _sp(self.gap_a), _sp(self.gap_b), p(self.gap_coef),
+                                       p(self._old), p(self.cap), dt, p(self.cells))
         elif threads_ > 1:
             lib().orc_solid_step_omp(nx, ny, nz, p(self.kx), p(self.ky), p(self.kz), p(self.mob),
                                      dt, p(self.cells), p(self._old), threads_)
@@ -331,6 +411,9 @@ class OracleSolid3D:
     def sample(self, x: float, y: float, z: float) -> float:
         """ScalarField::at (solid.rs:2045-2102)."""
         nx, ny, nz = self.counts_
+        if self.voids.any():
+            return lib().orc_sample_void(nx, ny, nz, self.dx, p(self.cells), _u8(self.voids), x,
+                                         y, z)
         return lib().orc_sample(nx, ny, nz, self.dx, p(self.cells), x, y, z)
 
 

@@ -104,9 +104,22 @@ double orc_series(double a, double b) {
  *   cap[N]     per-cell heat capacity
  * Out: kx[(nx+1)ny nz], ky[nx(ny+1)nz], kz[nx ny(nz+1)], mobility[N], face_sum[N];
  * returns worst_rate (NaN when any rate is NaN, solid.rs:771-782). */
+double orc_resolve_void(size_t nx, size_t ny, size_t nz, double dx, const double *kcell,
+                        const double *cap, const unsigned char *voids, double *kx, double *ky,
+                        double *kz, double *mobility, double *face_sum);
+
 double orc_resolve(size_t nx, size_t ny, size_t nz, double dx, const double *kcell,
                    const double *cap, double *kx, double *ky, double *kz, double *mobility,
                    double *face_sum) {
+    return orc_resolve_void(nx, ny, nz, dx, kcell, cap, NULL, kx, ky, kz, mobility, face_sum);
+}
+
+/* The same with void cells (solid.rs:686-702, 753-756): a void cell conducts and holds nothing
+ * (its k and capacity are zero, so every face it touches is zero through `series`), and its
+ * mobility is zero rather than dx/0.  `voids` may be NULL. */
+double orc_resolve_void(size_t nx, size_t ny, size_t nz, double dx, const double *kcell,
+                        const double *cap, const unsigned char *voids, double *kx, double *ky,
+                        double *kz, double *mobility, double *face_sum) {
     memset(kx, 0, sizeof(double) * (nx + 1) * ny * nz);
     memset(ky, 0, sizeof(double) * nx * (ny + 1) * nz);
     memset(kz, 0, sizeof(double) * nx * ny * (nz + 1));
@@ -124,7 +137,7 @@ double orc_resolve(size_t nx, size_t ny, size_t nz, double dx, const double *kce
         for (size_t j = 0; j < ny; j++)
             for (size_t i = 0; i < nx; i++) {
                 size_t c = i + nx * (j + ny * k);
-                mobility[c] = dx / cap[c];
+                mobility[c] = (voids && voids[c]) ? 0.0 : dx / cap[c];
                 double sum = kx[i + (nx + 1) * (j + ny * k)] + kx[i + 1 + (nx + 1) * (j + ny * k)] +
                              ky[i + nx * (j + (ny + 1) * k)] + ky[i + nx * (j + 1 + (ny + 1) * k)] +
                              kz[i + nx * (j + ny * k)] + kz[i + nx * (j + ny * (k + 1))];
@@ -443,6 +456,16 @@ void orc_solid_bus_heat(size_t n, const double *cap, double gained, double *cell
     for (size_t c = 0; c < n; c++) total += cap[c];
     double rise = gained / total;
     for (size_t c = 0; c < n; c++) cells[c] += rise;
+}
+
+/* The same with void cells: void takes no share of the rise (solid.rs:1756-1764). */
+void orc_solid_bus_heat_void(size_t n, const double *cap, const unsigned char *voids,
+                             double gained, double *cells) {
+    double total = 0.0;
+    for (size_t c = 0; c < n; c++) total += cap[c];
+    double rise = gained / total;
+    for (size_t c = 0; c < n; c++)
+        if (!voids[c]) cells[c] += rise;
 }
 
 /* Solid3D::stored_heat, solid.rs:1557-1561 with enthalpy_joules' latent <= 0 branch
@@ -807,4 +830,140 @@ double orc_tube_sweep(size_t n, double dx, double area, double speed, double den
     absorbed += fabs(p[0] * lw) * area * dt;
     absorbed += fabs(p[n - 1] * rw) * area * dt;
     return absorbed;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Void cells and the gaps between them, crates/pantometry-thermal/src/solid.rs               */
+/* ------------------------------------------------------------------------------------------ */
+
+/* One grid line of Solid3D::find_gaps, solid.rs:803-831: a run of void with solid at both ends
+ * pairs the two solid cells, coupled by sigma*dx^2/(1/ea + 1/eb - 1) when both radiate. */
+static size_t orc_walk(const size_t *line, size_t len, const unsigned char *voids,
+                       const double *emissivity, double area, size_t count, size_t *a,
+                       size_t *b, double *coef) {
+    long last = -1;
+    int between = 0;
+    for (size_t q = 0; q < len; q++) {
+        size_t c = line[q];
+        if (voids[c]) {
+            between = last >= 0;
+            continue;
+        }
+        if (last >= 0 && between) {
+            double ea = emissivity[last], eb = emissivity[c];
+            if (ea > 0.0 && eb > 0.0) {
+                double resistance = 1.0 / ea + 1.0 / eb - 1.0;
+                if (resistance > 0.0) {
+                    a[count] = (size_t)last, b[count] = c;
+                    coef[count] = ORC_SIGMA * area / resistance;
+                    count++;
+                }
+            }
+        }
+        last = (long)c;
+        between = 0;
+    }
+    return count;
+}
+
+/* Solid3D::find_gaps, solid.rs:791-847: x lines, then y lines, then z lines.  `emissivity` is
+ * each cell's substance's (0 for one that does not say).  The outputs hold at most
+ * 3*nx*ny*nz pairs; returns the count. */
+size_t orc_find_gaps(size_t nx, size_t ny, size_t nz, double dx, const unsigned char *voids,
+                     const double *emissivity, size_t *a, size_t *b, double *coef) {
+    size_t n = nx * ny * nz, count = 0;
+    int any = 0;
+    for (size_t c = 0; c < n && !any; c++) any = voids[c];
+    if (!any) return 0;
+    double area = dx * dx;
+    size_t longest = nx > ny ? (nx > nz ? nx : nz) : (ny > nz ? ny : nz);
+    size_t *line = (size_t *)malloc(sizeof(size_t) * longest);
+    for (size_t k = 0; k < nz; k++)
+        for (size_t j = 0; j < ny; j++) {
+            for (size_t i = 0; i < nx; i++) line[i] = i + nx * (j + ny * k);
+            count = orc_walk(line, nx, voids, emissivity, area, count, a, b, coef);
+        }
+    for (size_t k = 0; k < nz; k++)
+        for (size_t i = 0; i < nx; i++) {
+            for (size_t j = 0; j < ny; j++) line[j] = i + nx * (j + ny * k);
+            count = orc_walk(line, ny, voids, emissivity, area, count, a, b, coef);
+        }
+    for (size_t j = 0; j < ny; j++)
+        for (size_t i = 0; i < nx; i++) {
+            for (size_t k = 0; k < nz; k++) line[k] = i + nx * (j + ny * k);
+            count = orc_walk(line, nz, voids, emissivity, area, count, a, b, coef);
+        }
+    free(line);
+    return count;
+}
+
+/* The gaps' exchange in Solid3D::step, solid.rs:1866-1878: in pair order, read off the step's
+ * input `old`, antisymmetric. */
+void orc_gap_exchange(size_t count, const size_t *a, const size_t *b, const double *coef,
+                      const double *old, const double *cap, double dts, double *cells) {
+    for (size_t g = 0; g < count; g++) {
+        double ta = old[a[g]], tb = old[b[g]];
+        double watts = coef[g] * (orc_pow4(ta) - orc_pow4(tb));
+        if (watts == 0.0) continue;
+        double joules = watts * dts;
+        cells[a[g]] += -joules / cap[a[g]];
+        cells[b[g]] += joules / cap[b[g]];
+    }
+}
+
+/* The gaps' part of Solid3D::worst_rate_now, solid.rs:1159-1171: each pair's radiative
+ * conductance linearised at the hotter cell, over each cell's capacity. */
+double orc_gap_rates(size_t count, const size_t *a, const size_t *b, const double *coef,
+                     const double *cells, const double *cap, double worst) {
+    for (size_t g = 0; g < count; g++) {
+        double hotter = fmax(cells[a[g]], cells[b[g]]);
+        double gc = 4.0 * coef[g] * orc_pow3(hotter);
+        if (cap[a[g]] > 0.0) worst = fmax(worst, gc / cap[a[g]]);
+        if (cap[b[g]] > 0.0) worst = fmax(worst, gc / cap[b[g]]);
+    }
+    return worst;
+}
+
+/* ScalarField::at for Solid3D with void, solid.rs:2045-2102: NaN when the cell the point is in
+ * (rounded, half away from zero) is void; otherwise the trilinear weights over the solid corners
+ * that carry weight, renormalised. */
+double orc_sample_void(size_t nx, size_t ny, size_t nz, double dx, const double *cells,
+                       const unsigned char *voids, double x, double y, double z) {
+    double q[3] = {x / dx - 0.5, y / dx - 0.5, z / dx - 0.5};
+    if (isnan(q[0]) || isnan(q[1]) || isnan(q[2])) return NAN;
+    size_t n[3] = {nx, ny, nz}, at[3], near[3];
+    double fr[3];
+    for (int a = 0; a < 3; a++) {
+        size_t last = n[a] > 0 ? n[a] - 1 : 0;
+        if (q[a] <= 0.0) {
+            at[a] = 0, fr[a] = 0.0;
+        } else if (q[a] >= (double)last) {
+            at[a] = last, fr[a] = 0.0;
+        } else {
+            double fl = floor(q[a]);
+            at[a] = (size_t)fl, fr[a] = q[a] - fl;
+        }
+        double r = round(q[a]);
+        if (!(r > 0.0)) r = 0.0;
+        near[a] = (size_t)r;
+        if (near[a] > last) near[a] = last;
+    }
+    if (voids[near[0] + nx * (near[1] + ny * near[2])]) return NAN;
+    size_t i1 = at[0] + 1 < nx ? at[0] + 1 : nx - 1;
+    size_t j1 = at[1] + 1 < ny ? at[1] + 1 : ny - 1;
+    size_t k1 = at[2] + 1 < nz ? at[2] + 1 : nz - 1;
+    size_t ia[2] = {at[0], i1}, jb[2] = {at[1], j1}, kc[2] = {at[2], k1};
+    double wa[2] = {1.0 - fr[0], fr[0]}, wb[2] = {1.0 - fr[1], fr[1]}, wc[2] = {1.0 - fr[2], fr[2]};
+    double sum = 0.0, weight = 0.0;
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++)
+            for (int c = 0; c < 2; c++) {
+                double w = wa[a] * wb[b] * wc[c];
+                size_t idx = ia[a] + nx * (jb[b] + ny * kc[c]);
+                if (w > 0.0 && !voids[idx]) {
+                    sum += w * cells[idx];
+                    weight += w;
+                }
+            }
+    return weight > 0.0 ? sum / weight : NAN;
 }

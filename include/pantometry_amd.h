@@ -183,6 +183,20 @@ int pm_solid3d_losing_from(pm_solid3d *solid, int face, const pm_environment *en
  * selects (x-fastest bytes, nonzero = selected), replacing what they generated before; added
  * after each sweep as (W*dt)/C and kept in the `supplied` entry of the books. */
 int pm_solid3d_dissipating(pm_solid3d *solid, double watts, const uint8_t *mask);
+/* Solid3D::empty, solid.rs:992-1005: the cells under `mask` hold nothing from now on (a fill
+ * does not bring them back).  A void cell conducts and stores nothing; solid cells facing each
+ * other across a run of void along a grid line exchange radiation sigma A (Ta^4 - Tb^4)/
+ * (1/ea + 1/eb - 1) inside the step (solid.rs:791-847, 1866-1878), which also bounds the step
+ * (solid.rs:1159-1171).  Void takes no share of the bus heat, generates nothing, has no
+ * temperature in the readings or a sample. */
+int pm_solid3d_empty(pm_solid3d *solid, const uint8_t *mask);
+/* Solid3D::void_cells and is_void, solid.rs:1007-1015. */
+size_t pm_solid3d_void_cells(const pm_solid3d *solid);
+int pm_solid3d_is_void(const pm_solid3d *solid, size_t i, size_t j, size_t k);
+/* The gap pairs in the reference's order (cell a, cell b, coefficient in W/K^4); at most
+ * `capacity` written, the number of pairs returned. */
+size_t pm_solid3d_gap_pairs(const pm_solid3d *solid, int64_t *a, int64_t *b, double *coef,
+                            size_t capacity);
 /* ScalarField::at for Solid3D, crates/pantometry-thermal/src/solid.rs:2045-2102: the temperature
  * (kelvin) at `count` points given as x, y, z triples in metres from the block's corner,
  * trilinear between cell centres and clamped at the outer half cells; NaN for a NaN point.  What
@@ -195,6 +209,9 @@ int pm_solid3d_restore(pm_solid3d *solid);
 /* Domain::readings, solid.rs:1962-2012: peak, mean and coldest in degrees Celsius, then the
  * absorbed joules. */
 int pm_solid3d_readings(pm_solid3d *solid, double out[4]);
+/* peak_temperature, mean_temperature and coldest_temperature (solid.rs:1377-1405) in kelvin, over
+ * the cells that are not void (the mean NaN when every cell is). */
+int pm_solid3d_statistics(pm_solid3d *solid, double out[3]);
 /* Solid3D::heat_capacity, solid.rs:1441-1443: the left-fold sum of the cell capacities. */
 double pm_solid3d_heat_capacity(pm_solid3d *solid);
 /* Shape of the sweep: rows per 64-wide tile (4, 8 or 16), z-planes per workgroup, and how many

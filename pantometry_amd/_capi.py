@@ -81,9 +81,14 @@ SIGNATURES = {
     "pm_solid3d_dissipating": (_I, [_VP, _D, C.POINTER(C.c_uint8)]),
     "pm_solid3d_generated_power": (_I, [_VP, _DP]),
     "pm_solid3d_sample": (_I, [_VP, _DP, _S, _DP]),
+    "pm_solid3d_empty": (_I, [_VP, C.POINTER(C.c_uint8)]),
+    "pm_solid3d_void_cells": (_S, [_VP]),
+    "pm_solid3d_is_void": (_I, [_VP, _S, _S, _S]),
+    "pm_solid3d_gap_pairs": (_S, [_VP, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _DP, _S]),
     "pm_hall_sample": (_I, [_VP, _DP, _S, _DP]),
     "pm_solid3d_restore": (_I, [_VP]),
     "pm_solid3d_readings": (_I, [_VP, _DP]),
+    "pm_solid3d_statistics": (_I, [_VP, _DP]),
     "pm_solid3d_heat_capacity": (_D, [_VP]),
     "pm_solid3d_set_tile": (_I, [_VP, _I, _I, _I]),
     "pm_solid3d_set_profiling": (_I, [_VP, _I]),

@@ -708,6 +708,21 @@ __global__ void add_uniform(double *p, int64_t n, double rise) {
         p[i] += rise;
 }
 
+// The bus heat with void cells, which take no share (solid.rs:1756-1764).
+__global__ void add_uniform_void(double *p, const uint8_t *vd, int64_t n, double rise) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        if (!vd[i]) p[i] += rise;
+}
+
+// A void cell conducts and holds nothing: zero conductivity and capacity, and zero mobility
+// rather than dx/0 (solid.rs:686-702, 753-756).
+__global__ void void_zero(const uint8_t *vd, int64_t n, double *kcell, double *cap, double *mob) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        if (vd[i]) kcell[i] = 0.0, cap[i] = 0.0, mob[i] = 0.0;
+}
+
 __global__ void audit_reset(AuditState *st, double total, double scale, double tol) {
     st->prev_total = total;
     st->prev_scale = scale;
@@ -772,6 +787,20 @@ struct pm_solid3d {
     DevBuf bcell, bemis, bkcond, shed, src_c, src_w, ext_part;
     int64_t nboundary = 0, nsource = 0;
     bool tables_valid = false;
+    // Void cells (solid.rs:238, 992-1005) and the pairs of solid cells facing each other across
+    // them (solid.rs:239-245, 791-847).  Each gap cell lists the pairs it belongs to in pair
+    // order: gent[gstart[q] .. gstart[q+1]) holds 2*pair + (0 for the pair's first cell, 1 for
+    // its second).
+    std::vector<uint8_t> voids;  // empty until a cell is emptied
+    int64_t nvoid = 0;
+    DevBuf dvoid;
+    struct Gap {
+        int64_t a, b;
+        double coef;
+    };
+    std::vector<Gap> gaps;
+    DevBuf gap_a, gap_b, gap_coef, gcell, gstart, gent, gap_part;
+    int64_t ngcell = 0;
 
     int64_t n() const { return nx * ny * nz; }
     double *T() const { return t[cur].d(); }
@@ -780,7 +809,8 @@ struct pm_solid3d {
             if (f.on) return true;
         return false;
     }
-    bool extended() const { return exposed() || !source.empty(); }
+    bool extended() const { return exposed() || !source.empty() || !gaps.empty(); }
+    bool is_void(int64_t c) const { return nvoid && voids[(size_t)c]; }
 };
 
 namespace pm {
@@ -1251,11 +1281,13 @@ __global__ void heat_sources(double *__restrict__ T, const int64_t *__restrict__
     }
 }
 
-// ScalarField::at for Solid3D, solid.rs:2045-2102 (no void cells): trilinear between cell
-// centres, clamped at the outer half cells, weights renormalised over the corners with weight.
-__global__ void heat_sample(const double *__restrict__ T, int64_t nx, int64_t ny, int64_t nz,
-                            double dx, const double *__restrict__ xyz, int64_t n,
-                            double *__restrict__ out) {
+// ScalarField::at for Solid3D, solid.rs:2045-2102: trilinear between cell centres, clamped at
+// the outer half cells, weights renormalised over the corners with weight.  With void cells
+// (`vd` not null): NaN when the cell the point rounds into is void, and void corners carry no
+// weight.
+__global__ void heat_sample(const double *__restrict__ T, const uint8_t *__restrict__ vd,
+                            int64_t nx, int64_t ny, int64_t nz, double dx,
+                            const double *__restrict__ xyz, int64_t n, double *__restrict__ out) {
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n;
          p += (int64_t)gridDim.x * blockDim.x) {
         const double q[3] = {xyz[3 * p] / dx - 0.5, xyz[3 * p + 1] / dx - 0.5, xyz[3 * p + 2] / dx - 0.5};
@@ -1277,6 +1309,19 @@ __global__ void heat_sample(const double *__restrict__ T, int64_t nx, int64_t ny
                 at[a] = (int64_t)fl, fr[a] = q[a] - fl;
             }
         }
+        if (vd) {
+            // `(v.round().max(0.0) as usize).min(n - 1)`, solid.rs:2076: half away from zero.
+            int64_t near[3];
+            for (int a = 0; a < 3; a++) {
+                double r = round(q[a]);
+                if (!(r > 0.0)) r = 0.0;
+                near[a] = min((int64_t)r, len[a] - 1);
+            }
+            if (vd[near[0] + nx * (near[1] + ny * near[2])]) {
+                out[p] = NAN;
+                continue;
+            }
+        }
         const int64_t ia[2] = {at[0], min(at[0] + 1, nx - 1)};
         const int64_t jb[2] = {at[1], min(at[1] + 1, ny - 1)};
         const int64_t kc[2] = {at[2], min(at[2] + 1, nz - 1)};
@@ -1287,8 +1332,9 @@ __global__ void heat_sample(const double *__restrict__ T, int64_t nx, int64_t ny
             for (int b = 0; b < 2; b++)
                 for (int c = 0; c < 2; c++) {
                     const double w = wa[a] * wb[b] * wc[c];
-                    if (w > 0.0) {
-                        sum += w * T[ia[a] + nx * (jb[b] + ny * kc[c])];
+                    const int64_t at_ = ia[a] + nx * (jb[b] + ny * kc[c]);
+                    if (w > 0.0 && !(vd && vd[at_])) {
+                        sum += w * T[at_];
                         weight += w;
                     }
                 }
@@ -1297,6 +1343,60 @@ __global__ void heat_sample(const double *__restrict__ T, int64_t nx, int64_t ny
 }
 
 constexpr int EXT_BLOCKS = 1024;  // fixed grid of the extended sweep, so its sums are reproducible
+
+// The gaps' exchange, solid.rs:1866-1878: each gap cell applies the pairs it belongs to in pair
+// order, each pair's watts coef*(Ta^4 - Tb^4) from the step's input, so a cell in several pairs
+// sums them in the reference's order; the pair's two cells evaluate the same expression.
+__global__ __launch_bounds__(256) void gap_exchange(const int64_t *__restrict__ gcell,
+                                                    const int64_t *__restrict__ gstart,
+                                                    const int64_t *__restrict__ gent, int64_t ngcell,
+                                                    const int64_t *__restrict__ ga,
+                                                    const int64_t *__restrict__ gb,
+                                                    const double *__restrict__ gcoef,
+                                                    const double *__restrict__ told,
+                                                    const double *__restrict__ cap, double dts,
+                                                    double *__restrict__ tnew) {
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < ngcell;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = gcell[q];
+        double t = tnew[c];
+        for (int64_t e = gstart[q]; e < gstart[q + 1]; e++) {
+            const int64_t g = gent[e] >> 1;
+            const double ta = told[ga[g]], tb = told[gb[g]];
+            const double watts = gcoef[g] * (pow4(ta) - pow4(tb));
+            if (watts == 0.0) continue;
+            const double joules = watts * dts;
+            t = t + ((gent[e] & 1) ? joules / cap[c] : -joules / cap[c]);
+        }
+        tnew[c] = t;
+    }
+}
+
+// The gaps' part of worst_rate_now, solid.rs:1159-1171: 4 coef T_hot^3 over each cell's
+// capacity, as a per-block maximum.
+__global__ __launch_bounds__(256) void gap_rates(const int64_t *__restrict__ ga,
+                                                 const int64_t *__restrict__ gb,
+                                                 const double *__restrict__ gcoef, int64_t npairs,
+                                                 const double *__restrict__ T,
+                                                 const double *__restrict__ cap,
+                                                 double *__restrict__ block_max) {
+    __shared__ double smax[256];
+    double worst = 0.0;
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < npairs;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const double hotter = rmax(T[ga[g]], T[gb[g]]);
+        const double gc = 4.0 * gcoef[g] * pow3(hotter);
+        if (cap[ga[g]] > 0.0) worst = rmax(worst, gc / cap[ga[g]]);
+        if (cap[gb[g]] > 0.0) worst = rmax(worst, gc / cap[gb[g]]);
+    }
+    smax[threadIdx.x] = worst;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) smax[threadIdx.x] = rmax(smax[threadIdx.x], smax[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) block_max[blockIdx.x] = smax[0];
+}
 
 static FilmFaces film_faces(const pm_solid3d *s) {
     FilmFaces F{};
@@ -1352,11 +1452,13 @@ static int ensure_tables(pm_solid3d *s) {
             PM_HIP(hipMemset(s->shed.d(), 0, sizeof(double) * (size_t)s->n()));
         }
     }
-    s->nsource = (int64_t)s->source.size();
+    s->nsource = 0;
+    for (const auto &e : s->source) s->nsource += !s->is_void(e.first);
     if (s->nsource) {
         std::vector<int64_t> sc;
         std::vector<double> sw;
-        for (const auto &e : s->source) sc.push_back(e.first), sw.push_back(e.second);
+        for (const auto &e : s->source)
+            if (!s->is_void(e.first)) sc.push_back(e.first), sw.push_back(e.second);
         PM_CHECK(s->src_c.alloc(sizeof(int64_t) * sc.size()));
         PM_CHECK(s->src_w.alloc(sizeof(double) * sw.size()));
         PM_HIP(hipMemcpy(s->src_c.d(), sc.data(), sizeof(int64_t) * sc.size(), hipMemcpyHostToDevice));
@@ -1367,11 +1469,112 @@ static int ensure_tables(pm_solid3d *s) {
     return PM_OK;
 }
 
+// Void cells hold and conduct nothing: the general form with their k, C and mobility zeroed.
+// Called after anything that writes the per-cell arrays (a void cell stays void, solid.rs:519).
+static int apply_voids(pm_solid3d *s) {
+    if (!s->nvoid) return PM_OK;
+    PM_CHECK(promote_c(s));
+    void_zero<<<grid_for(s->n(), 256), 256, 0, s->stream>>>(s->dvoid.as<uint8_t>(), s->n(),
+                                                            s->kcell.d(), s->cap.d(), s->mob.d());
+    PM_HIP(hipGetLastError());
+    s->heat_capacity_valid = false;
+    return PM_OK;
+}
+
+// Solid3D::find_gaps, solid.rs:791-847, on the host: along every x line, then y, then z, a run of
+// void with solid at both ends pairs the two cells, coupled by sigma dx^2/(1/ea + 1/eb - 1) when
+// both radiate.  Then the per-cell lists the exchange kernel walks.
+static int find_gaps(pm_solid3d *s) {
+    s->gaps.clear();
+    s->ngcell = 0;
+    s->tables_valid = false;
+    if (!s->nvoid) return PM_OK;
+    const int64_t nx = s->nx, ny = s->ny, nz = s->nz;
+    const double area = s->dx * s->dx;
+    auto emissivity = [&](int64_t c) {
+        const pm_substance &sub = s->subs[s->which.empty() ? 0 : s->which[(size_t)c]];
+        const bool thermal = !std::isnan(sub.conductivity) && !std::isnan(sub.specific_heat);
+        return thermal ? sub.emissivity : 0.0;  // thermal.map_or(0.0, emissivity)
+    };
+    auto walk = [&](int64_t first, int64_t stride, int64_t len) {
+        int64_t last = -1;
+        bool between = false;
+        for (int64_t q = 0; q < len; q++) {
+            const int64_t c = first + q * stride;
+            if (s->voids[(size_t)c]) {
+                between = last >= 0;
+                continue;
+            }
+            if (last >= 0 && between) {
+                const double ea = emissivity(last), eb = emissivity(c);
+                if (ea > 0.0 && eb > 0.0) {
+                    const double resistance = 1.0 / ea + 1.0 / eb - 1.0;
+                    if (resistance > 0.0) s->gaps.push_back({last, c, STEFAN_BOLTZMANN * area / resistance});
+                }
+            }
+            last = c;
+            between = false;
+        }
+    };
+    for (int64_t k = 0; k < nz; k++)
+        for (int64_t j = 0; j < ny; j++) walk(nx * (j + ny * k), 1, nx);
+    for (int64_t k = 0; k < nz; k++)
+        for (int64_t i = 0; i < nx; i++) walk(i + nx * ny * k, nx, ny);
+    for (int64_t j = 0; j < ny; j++)
+        for (int64_t i = 0; i < nx; i++) walk(i + nx * j, nx * ny, nz);
+    if (s->gaps.empty()) return PM_OK;
+    const size_t np = s->gaps.size();
+    std::vector<int64_t> ga(np), gb(np);
+    std::vector<double> gc(np);
+    std::map<int64_t, std::vector<int64_t>> per_cell;  // each cell's entries in pair order
+    for (size_t g = 0; g < np; g++) {
+        ga[g] = s->gaps[g].a, gb[g] = s->gaps[g].b, gc[g] = s->gaps[g].coef;
+        per_cell[ga[g]].push_back(2 * (int64_t)g);
+        per_cell[gb[g]].push_back(2 * (int64_t)g + 1);
+    }
+    std::vector<int64_t> cells, start{0}, ent;
+    for (const auto &e : per_cell) {
+        cells.push_back(e.first);
+        ent.insert(ent.end(), e.second.begin(), e.second.end());
+        start.push_back((int64_t)ent.size());
+    }
+    s->ngcell = (int64_t)cells.size();
+    PM_CHECK(s->gap_a.alloc(sizeof(int64_t) * np));
+    PM_CHECK(s->gap_b.alloc(sizeof(int64_t) * np));
+    PM_CHECK(s->gap_coef.alloc(sizeof(double) * np));
+    PM_CHECK(s->gcell.alloc(sizeof(int64_t) * cells.size()));
+    PM_CHECK(s->gstart.alloc(sizeof(int64_t) * start.size()));
+    PM_CHECK(s->gent.alloc(sizeof(int64_t) * ent.size()));
+    PM_CHECK(s->gap_part.alloc(sizeof(double) * EXT_BLOCKS));
+    PM_HIP(hipMemcpy(s->gap_a.d(), ga.data(), sizeof(int64_t) * np, hipMemcpyHostToDevice));
+    PM_HIP(hipMemcpy(s->gap_b.d(), gb.data(), sizeof(int64_t) * np, hipMemcpyHostToDevice));
+    PM_HIP(hipMemcpy(s->gap_coef.d(), gc.data(), sizeof(double) * np, hipMemcpyHostToDevice));
+    PM_HIP(hipMemcpy(s->gcell.d(), cells.data(), sizeof(int64_t) * cells.size(), hipMemcpyHostToDevice));
+    PM_HIP(hipMemcpy(s->gstart.d(), start.data(), sizeof(int64_t) * start.size(), hipMemcpyHostToDevice));
+    PM_HIP(hipMemcpy(s->gent.d(), ent.data(), sizeof(int64_t) * ent.size(), hipMemcpyHostToDevice));
+    return PM_OK;
+}
+
 // Solid3D::worst_rate_now, solid.rs:1148-1185: the resolved limit, raised by each exposed cell's
 // film at its present temperature.  Synchronises the handle's stream.
 static int worst_rate_now(pm_solid3d *s, double *out) {
     *out = s->worst_rate;
-    if (!s->exposed() || std::isnan(s->worst_rate)) return PM_OK;
+    if ((!s->exposed() && s->gaps.empty()) || std::isnan(s->worst_rate)) return PM_OK;
+    double gap_worst = 0.0;
+    if (!s->gaps.empty()) {
+        const int64_t np = (int64_t)s->gaps.size();
+        const int gblocks = (int)std::min<int64_t>(EXT_BLOCKS, (np + 255) / 256);
+        gap_rates<<<gblocks, 256, 0, s->stream>>>(s->gap_a.as<int64_t>(), s->gap_b.as<int64_t>(),
+                                                  s->gap_coef.d(), np, s->T(), s->cap.d(),
+                                                  s->gap_part.d());
+        PM_HIP(hipGetLastError());
+        std::vector<double> part((size_t)gblocks);
+        PM_HIP(hipMemcpyAsync(part.data(), s->gap_part.d(), sizeof(double) * part.size(), hipMemcpyDeviceToHost, s->stream));
+        PM_HIP(hipStreamSynchronize(s->stream));
+        for (double w : part) gap_worst = std::fmax(gap_worst, w);
+        *out = std::fmax(s->worst_rate, gap_worst);
+    }
+    if (!s->exposed()) return PM_OK;
     PM_CHECK(ensure_tables(s));
     if (s->nboundary == 0) return PM_OK;
     const int blocks = (int)std::min<int64_t>(EXT_BLOCKS, (s->nboundary + 255) / 256);
@@ -1384,7 +1587,7 @@ static int worst_rate_now(pm_solid3d *s, double *out) {
     std::vector<double> part((size_t)blocks);
     PM_HIP(hipMemcpyAsync(part.data(), s->ext_part.d(), sizeof(double) * part.size(), hipMemcpyDeviceToHost, s->stream));
     PM_HIP(hipStreamSynchronize(s->stream));
-    double worst = s->worst_rate;
+    double worst = std::fmax(s->worst_rate, gap_worst);
     for (double p : part) worst = std::fmax(worst, p);
     *out = worst;
     return PM_OK;
@@ -1417,6 +1620,12 @@ static int step_ext(pm_solid3d *s, double dt) {
             out, s->src_c.as<int64_t>(), s->src_w.d(), s->nsource, cap, s->cap0, dt);
         PM_HIP(hipGetLastError());
     }
+    if (s->ngcell) {
+        gap_exchange<<<grid_for(s->ngcell, 256), 256, 0, s->stream>>>(
+            s->gcell.as<int64_t>(), s->gstart.as<int64_t>(), s->gent.as<int64_t>(), s->ngcell,
+            s->gap_a.as<int64_t>(), s->gap_b.as<int64_t>(), s->gap_coef.d(), s->T(), s->cap.d(), dt, out);
+        PM_HIP(hipGetLastError());
+    }
     std::vector<double> part(EXT_BLOCKS);
     PM_HIP(hipMemcpyAsync(part.data(), s->ext_part.d(), sizeof(double) * EXT_BLOCKS, hipMemcpyDeviceToHost, s->stream));
     PM_HIP(hipStreamSynchronize(s->stream));
@@ -1424,7 +1633,8 @@ static int step_ext(pm_solid3d *s, double dt) {
     double lost = 0.0;
     for (double p : part) lost += p;
     s->lost += lost;
-    for (const auto &e : s->source) s->supplied += e.second * dt;
+    for (const auto &e : s->source)
+        if (!s->is_void(e.first)) s->supplied += e.second * dt;  // void makes nothing (solid.rs:1857)
     return PM_OK;
 }
 
@@ -1486,6 +1696,8 @@ static int reset_audit(pm_solid3d *s, double total, double scale, double tol) {
 }  // namespace pm
 
 extern "C" {
+
+static int pm_solid3d_readings_kelvin(pm_solid3d *s, double out[4]);
 
 int pm_solid3d_create(const char *name, const pm_substance *substance, size_t nx, size_t ny,
                       size_t nz, double dx, double initial, pm_solid3d **out) {
@@ -1590,7 +1802,7 @@ int pm_solid3d_fill(pm_solid3d *s, const pm_substance *sub, const uint8_t *mask)
         s->kface = sum > 0.0 ? 2.0 * k * k / sum : 0.0;
         return resolve(s);
     }
-    if (k_same && c_same) return PM_OK;
+    if (k_same && c_same) return s->nvoid ? find_gaps(s) : PM_OK;
     if (!c_same) PM_CHECK(promote_c(s));
     if (!k_same) PM_CHECK(promote_k(s));
     DevBuf dmask;
@@ -1601,7 +1813,9 @@ int pm_solid3d_fill(pm_solid3d *s, const pm_substance *sub, const uint8_t *mask)
         s->cmode == C_PER_CELL ? s->cap.d() : nullptr,
         s->cmode == C_PER_CELL ? s->mob.d() : nullptr, k, c, s->dx);
     PM_HIP(hipGetLastError());
+    PM_CHECK(apply_voids(s));
     PM_CHECK(resolve(s));
+    PM_CHECK(find_gaps(s));  // a new substance brings its own emissivity
     PM_HIP(hipStreamSynchronize(s->stream));
     return PM_OK;
 }
@@ -1613,6 +1827,7 @@ int pm_solid3d_set_conductivity(pm_solid3d *s, const double *k) {
         s->kmode = K_PER_CELL;
     }
     PM_HIP(hipMemcpyAsync(s->kcell.d(), k, sizeof(double) * (size_t)s->n(), hipMemcpyHostToDevice, s->stream));
+    PM_CHECK(apply_voids(s));
     PM_CHECK(resolve(s));
     PM_HIP(hipStreamSynchronize(s->stream));
     return PM_OK;
@@ -1698,7 +1913,10 @@ int pm_solid3d_step(pm_solid3d *s, double dt, double gained, pm_violation *v) {
         double total = 0.0;
         PM_CHECK(total_capacity(s, &total));
         const double rise = gained / total;
-        add_uniform<<<grid_for(s->n(), 256), 256, 0, s->stream>>>(s->T(), s->n(), rise);
+        if (s->nvoid)
+            add_uniform_void<<<grid_for(s->n(), 256), 256, 0, s->stream>>>(s->T(), s->dvoid.as<uint8_t>(), s->n(), rise);
+        else
+            add_uniform<<<grid_for(s->n(), 256), 256, 0, s->stream>>>(s->T(), s->n(), rise);
         PM_HIP(hipGetLastError());
     }
     if (s->extended()) return step_ext(s, dt);
@@ -1859,12 +2077,13 @@ int pm_solid3d_dissipating(pm_solid3d *s, double watts, const uint8_t *mask) {
         set_error("pm_solid3d_dissipating: null argument");
         return PM_ERR_ARGUMENT;
     }
+    // Over the selected cells that are there (solid.rs:1066-1075).
     int64_t chosen = 0;
-    for (int64_t c = 0; c < s->n(); c++) chosen += mask[c] != 0;
+    for (int64_t c = 0; c < s->n(); c++) chosen += mask[c] != 0 && !s->is_void(c);
     if (chosen == 0) return PM_OK;
     const double each = watts / (double)chosen;
     for (int64_t c = 0; c < s->n(); c++) {
-        if (!mask[c]) continue;
+        if (!mask[c] || s->is_void(c)) continue;
         if (each == 0.0)
             s->source.erase(c);
         else
@@ -1874,6 +2093,49 @@ int pm_solid3d_dissipating(pm_solid3d *s, double watts, const uint8_t *mask) {
     return PM_OK;
 }
 
+int pm_solid3d_empty(pm_solid3d *s, const uint8_t *mask) {
+    if (!s || !mask) {
+        set_error("pm_solid3d_empty: null argument");
+        return PM_ERR_ARGUMENT;
+    }
+    const int64_t n = s->n();
+    if (s->voids.empty()) s->voids.assign((size_t)n, 0);
+    int64_t count = 0;
+    for (int64_t c = 0; c < n; c++) {
+        if (mask[c]) s->voids[(size_t)c] = 1;
+        count += s->voids[(size_t)c];
+    }
+    if (count == 0) {
+        s->voids.clear();
+        return PM_OK;
+    }
+    s->nvoid = count;
+    if (!s->dvoid) PM_CHECK(s->dvoid.alloc((size_t)n));
+    PM_HIP(hipMemcpyAsync(s->dvoid.d(), s->voids.data(), (size_t)n, hipMemcpyHostToDevice, s->stream));
+    PM_CHECK(apply_voids(s));
+    PM_CHECK(resolve(s));
+    PM_CHECK(find_gaps(s));
+    PM_HIP(hipStreamSynchronize(s->stream));
+    return PM_OK;
+}
+
+size_t pm_solid3d_void_cells(const pm_solid3d *s) { return s ? (size_t)s->nvoid : 0; }
+
+int pm_solid3d_is_void(const pm_solid3d *s, size_t i, size_t j, size_t k) {
+    if (!s || (int64_t)i >= s->nx || (int64_t)j >= s->ny || (int64_t)k >= s->nz) return 0;
+    return s->is_void((int64_t)i + s->nx * ((int64_t)j + s->ny * (int64_t)k)) ? 1 : 0;
+}
+
+size_t pm_solid3d_gap_pairs(const pm_solid3d *s, int64_t *a, int64_t *b, double *coef, size_t capacity) {
+    if (!s) return 0;
+    for (size_t g = 0; g < s->gaps.size() && g < capacity; g++) {
+        if (a) a[g] = s->gaps[g].a;
+        if (b) b[g] = s->gaps[g].b;
+        if (coef) coef[g] = s->gaps[g].coef;
+    }
+    return s->gaps.size();
+}
+
 int pm_solid3d_sample(pm_solid3d *s, const double *xyz, size_t count, double *out) {
     if (!s || (count && (!xyz || !out))) return PM_ERR_ARGUMENT;
     if (count == 0) return PM_OK;
@@ -1881,8 +2143,9 @@ int pm_solid3d_sample(pm_solid3d *s, const double *xyz, size_t count, double *ou
     PM_CHECK(dp.alloc(sizeof(double) * 3 * count));
     PM_CHECK(dv.alloc(sizeof(double) * count));
     PM_HIP(hipMemcpyAsync(dp.d(), xyz, sizeof(double) * 3 * count, hipMemcpyHostToDevice, s->stream));
-    heat_sample<<<grid_for((int64_t)count, 256), 256, 0, s->stream>>>(s->T(), s->nx, s->ny, s->nz,
-                                                                      s->dx, dp.d(), (int64_t)count, dv.d());
+    heat_sample<<<grid_for((int64_t)count, 256), 256, 0, s->stream>>>(
+        s->T(), s->nvoid ? s->dvoid.as<uint8_t>() : nullptr, s->nx, s->ny, s->nz, s->dx, dp.d(),
+        (int64_t)count, dv.d());
     PM_HIP(hipGetLastError());
     PM_HIP(hipMemcpyAsync(out, dv.d(), sizeof(double) * count, hipMemcpyDeviceToHost, s->stream));
     PM_HIP(hipStreamSynchronize(s->stream));
@@ -1924,20 +2187,40 @@ int pm_solid3d_restore(pm_solid3d *s) {
     return PM_OK;
 }
 
+int pm_solid3d_statistics(pm_solid3d *s, double out[3]) {
+    if (!s || !out) return PM_ERR_ARGUMENT;
+    double r[4];
+    PM_CHECK(pm_solid3d_readings_kelvin(s, r));
+    out[0] = r[0], out[1] = r[1], out[2] = r[2];
+    return PM_OK;
+}
+
 int pm_solid3d_readings(pm_solid3d *s, double out[4]) {
+    if (!s || !out) return PM_ERR_ARGUMENT;
+    PM_CHECK(pm_solid3d_readings_kelvin(s, out));
+    out[0] -= 273.15, out[1] -= 273.15, out[2] -= 273.15;
+    return PM_OK;
+}
+
+static int pm_solid3d_readings_kelvin(pm_solid3d *s, double out[4]) {
     if (!s || !out) return PM_ERR_ARGUMENT;
     std::vector<double> h((size_t)s->n());
     PM_CHECK(pm_solid3d_get_cells(s, h.data()));
     // peak_temperature / mean_temperature / coldest_temperature, solid.rs:1378-1405, in celsius.
+    // Over what is there: void cells have no vote (solid.rs:1377-1405).
     double peak = -1.7976931348623157e308, cold = 1.7976931348623157e308, sum = 0.0;
-    for (double t : h) {
+    int64_t count = 0;
+    for (int64_t c = 0; c < s->n(); c++) {
+        if (s->is_void(c)) continue;
+        const double t = h[(size_t)c];
         peak = rmax(peak, t);
         cold = std::fmin(cold, t);
         sum += t;
+        count++;
     }
-    out[0] = peak - 273.15;
-    out[1] = sum / (double)h.size() - 273.15;
-    out[2] = cold - 273.15;
+    out[0] = peak;
+    out[1] = count == 0 ? NAN : sum / (double)count;
+    out[2] = cold;
     out[3] = s->absorbed;
     return PM_OK;
 }

@@ -227,8 +227,46 @@ class Solid3D:
         check(lib().pm_solid3d_deposit(self._h, i, j, k, float(joules)), "pm_solid3d_deposit")
 
     def temperature_at(self, i: int, j: int, k: int) -> float:
+        """solid.rs:1323-1335: clamped to the block, NaN for a void cell."""
         nx, ny, nz = self._counts
-        return float(self.cells()[min(k, nz - 1), min(j, ny - 1), min(i, nx - 1)])
+        i, j, k = min(i, nx - 1), min(j, ny - 1), min(k, nz - 1)
+        if self.is_void(i, j, k):
+            return math.nan
+        return float(self.cells()[k, j, i])
+
+    def empty(self, which: Union[Callable[[int, int, int], bool], np.ndarray]) -> "Solid3D":
+        """Solid3D::empty, solid.rs:992-1005: the selected cells hold nothing from now on; solid
+        cells facing each other across them radiate (solid.rs:791-847, 1866-1878)."""
+        mask = self._mask(which)
+        check(lib().pm_solid3d_empty(self._h, mask.ctypes.data_as(C.POINTER(C.c_uint8))),
+              "pm_solid3d_empty")
+        return self
+
+    def is_void(self, i: int, j: int, k: int) -> bool:
+        return bool(lib().pm_solid3d_is_void(self._h, int(i), int(j), int(k)))
+
+    def void_cells(self) -> int:
+        return int(lib().pm_solid3d_void_cells(self._h))
+
+    def gap_pairs(self):
+        """The radiating pairs in the reference's order: (cell a, cell b, W/K^4)."""
+        n = int(lib().pm_solid3d_gap_pairs(self._h, None, None, None, 0))
+        a, b = np.empty(n, dtype=np.int64), np.empty(n, dtype=np.int64)
+        coef = np.empty(n)
+        lib().pm_solid3d_gap_pairs(self._h, a.ctypes.data_as(C.POINTER(C.c_int64)),
+                                   b.ctypes.data_as(C.POINTER(C.c_int64)), dptr(coef), n)
+        return list(zip(a.tolist(), b.tolist(), coef.tolist()))
+
+    def _mask(self, which) -> np.ndarray:
+        nx, ny, nz = self._counts
+        if callable(which):
+            mask = np.zeros((nz, ny, nx), dtype=np.uint8)
+            for k in range(nz):
+                for j in range(ny):
+                    for i in range(nx):
+                        mask[k, j, i] = 1 if which(i, j, k) else 0
+            return mask.reshape(-1)
+        return np.ascontiguousarray(np.asarray(which).reshape(-1), dtype=np.uint8)
 
     def release_mode(self, mode: Tuple[int, int, int], mean: float, amplitude: float) -> None:
         """Solid3D::release_mode, solid.rs:1478-1500: mean + ((amp*px)*py)*pz per cell."""
@@ -276,15 +314,20 @@ class Solid3D:
         total = terms[0] + terms[1] + terms[2]
         return 1.0 + f * total
 
+    def _statistics(self) -> Tuple[float, float, float]:
+        out = (C.c_double * 3)()
+        check(lib().pm_solid3d_statistics(self._h, out), "pm_solid3d_statistics")
+        return out[0], out[1], out[2]
+
     def mean_temperature(self) -> float:
-        c = self.cells().reshape(-1)
-        return float(np.cumsum(c)[-1]) / c.size
+        """solid.rs:1377-1384: over the cells that are there, NaN when none is."""
+        return self._statistics()[1]
 
     def peak_temperature(self) -> float:
-        return float(self.cells().max())
+        return self._statistics()[0]
 
     def coldest_temperature(self) -> float:
-        return float(self.cells().min())
+        return self._statistics()[2]
 
     def heat_capacity(self) -> float:
         return float(lib().pm_solid3d_heat_capacity(self._h))

@@ -67,6 +67,12 @@ _SIG = {
     "orc_gap_exchange": (None, [_S, _SP, _SP, _DP, _DP, _DP, _D, _DP]),
     "orc_gap_rates": (_D, [_S, _SP, _SP, _DP, _DP, _DP, _D]),
     "orc_sample_void": (_D, [_S, _S, _S, _D, _DP, _U8P, _D, _D, _D]),
+    "orc_stored_heat_phase": (_D, [_S, _DP, _DP, _D, _DP, _DP, _DP, _DP, _DP]),
+    "orc_bus_heat_phase": (None, [_S, _DP, _U8P, _D, _DP, _DP, _DP, _DP, _DP, _DP]),
+    "orc_solid_step_phase": (None, [_S, _S, _S, _D, _DP, _DP, _DP, _DP, _DP, _D, _DP, _DP, _IP,
+                                    _DP, _DP, _DP, _DP, _DP, _DP, _U8P, _S, _SP, _SP, _DP, _DP,
+                                    _DP, _DP, _DP, _DP, _DP, _DP]),
+    "orc_deposit_phase": (None, [_S, _D, _DP, _DP, _DP, _DP, _DP, _DP, _DP]),
     "orc_bar_step": (None, [_S, _D, _D, _D, _DP, _DP, _DP]),
     "orc_bar_stored": (_D, [_S, _D, _DP, _D]),
     "orc_bar_sample": (_D, [_S, _D, _DP, _D]),
@@ -185,7 +191,7 @@ class OracleSolid3D:
 
     def __init__(self, name: str, density: float, conductivity: float, specific_heat: float,
                  counts: Tuple[int, int, int], dx: float, initial: float,
-                 emissivity: float = 0.0):
+                 emissivity: float = 0.0, fusion=None):
         self.name_ = name
         self.counts_ = tuple(max(1, int(c)) for c in counts)
         nx, ny, nz = self.counts_
@@ -216,11 +222,61 @@ class OracleSolid3D:
         self.gap_a = np.zeros(0, dtype=np.uintp)
         self.gap_b = np.zeros(0, dtype=np.uintp)
         self.gap_coef = np.zeros(0)
+        # Phase change (solid.rs:586-680): per-cell solid and liquid conductivity and capacity,
+        # melting point and latent joules, and the melted fraction, once any substance melts.
+        self.phase = False
+        self.two_phase = False
+        self._phase_of(np.ones(self.n, dtype=bool), density, conductivity, specific_heat, fusion)
+        if self.phase:
+            # The fresh build: a cell above its melting point starts liquid (solid.rs:652-665).
+            self.melted = np.where(self.cells > self.point, 1.0, 0.0)
+            self.saved_melted = self.melted.copy()
         self.resolve()
+
+    def _phase_of(self, m, density, conductivity, specific_heat, fusion):
+        """The per-cell phase tables of the cells `m` selects (resolve's props, solid.rs:599-646).
+        `fusion` is (melting point K, latent J/kg, liquid (k, c_p) or None) or None."""
+        thermal = not math.isnan(conductivity)
+        volume = self.dx * self.dx * self.dx
+        cs = (density * volume) * specific_heat
+        if fusion is not None and not self.phase:
+            self.phase = True
+            self.k_solid = self.kcell.copy()
+            self.k_liquid = self.kcell.copy()
+            self.c_solid = self.cap.copy()
+            self.c_liquid = self.cap.copy()
+            self.point = np.full(self.n, math.inf)
+            self.latent = np.zeros(self.n)
+            self.melted = np.zeros(self.n)
+            self.saved_melted = np.zeros(self.n)
+        if fusion is not None and fusion[2] is not None and thermal:
+            if fusion[2][0] != conductivity or fusion[2][1] != specific_heat:
+                self.two_phase = True
+        if not self.phase:
+            return
+        liquid = fusion[2] if fusion is not None else None
+        self.k_solid[m] = conductivity if thermal else math.nan
+        self.c_solid[m] = cs
+        self.k_liquid[m] = liquid[0] if liquid is not None else self.k_solid[m]
+        self.c_liquid[m] = (density * volume) * liquid[1] if liquid is not None else cs
+        if fusion is not None and thermal:
+            self.point[m] = fusion[0]
+            self.latent[m] = (density * volume) * fusion[1]
+        else:
+            self.point[m] = math.inf
+            self.latent[m] = 0.0
+
+    def _phase_args(self):
+        return (p(self.melted), p(self.point), p(self.latent), p(self.c_solid), p(self.c_liquid))
 
     def resolve(self) -> None:
         """Solid3D::resolve (solid.rs:588-784) and find_gaps (solid.rs:791-847)."""
         nx, ny, nz = self.counts_
+        if self.phase:
+            # Mixed by melt fraction, arithmetically (solid.rs:686-702).
+            phi = self.melted
+            self.kcell = self.k_solid + phi * (self.k_liquid - self.k_solid)
+            self.cap = self.c_solid + phi * (self.c_liquid - self.c_solid)
         self.kcell[self.voids != 0] = 0.0
         self.cap[self.voids != 0] = 0.0
         self.kx = np.empty((nx + 1) * ny * nz)
@@ -274,9 +330,10 @@ class OracleSolid3D:
         return peak, (total / count if count else math.nan), coldest
 
     def fill(self, density: float, conductivity: float, specific_heat: float, mask: np.ndarray,
-             emissivity: float = 0.0):
-        """Solid3D::fill, solid.rs:519-543: a void cell stays void."""
+             emissivity: float = 0.0, fusion=None):
+        """Solid3D::fill, solid.rs:519-543: a void cell stays void; a melted fraction is carried."""
         m = np.asarray(mask).reshape(-1).astype(bool)
+        self._phase_of(m, density, conductivity, specific_heat, fusion)
         self.kcell[m] = conductivity
         self.cap[m] = lib().orc_heat_capacity(density, specific_heat, self.dx)
         self.emis[m] = _thermal_or(emissivity, conductivity, 0.0)
@@ -322,11 +379,18 @@ class OracleSolid3D:
 
     def with_conductivity(self, k: np.ndarray):
         self.kcell = np.ascontiguousarray(k, dtype=np.float64).reshape(-1).copy()
+        if self.phase:
+            self.k_solid = self.kcell.copy()
+            self.k_liquid = self.kcell.copy()
         self.resolve()
         return self
 
     def set_cells(self, cells: np.ndarray) -> None:
+        """Every cell at once, as set_temperature would set it (solid.rs:1346-1361)."""
         self.cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1).copy()
+        if self.phase:
+            latent = self.latent > 0.0
+            self.melted[latent] = np.where(self.cells[latent] > self.point[latent], 1.0, 0.0)
 
     def max_stable_dt(self, now: float = 0.0) -> float:
         rate = self.worst_rate_now()
@@ -349,10 +413,25 @@ class OracleSolid3D:
         nx, ny, nz = self.counts_
         if gained != 0.0:
             self.absorbed += gained
-            lib().orc_solid_bus_heat_void(self.n, p(self.cap), _u8(self.voids), gained,
-                                          p(self.cells))
+            if self.phase:
+                lib().orc_bus_heat_phase(self.n, p(self.cap), _u8(self.voids), gained,
+                                         p(self.cells), *self._phase_args())
+            else:
+                lib().orc_solid_bus_heat_void(self.n, p(self.cap), _u8(self.voids), gained,
+                                              p(self.cells))
         gaps = len(self.gap_a)
-        if self.exposed() or self.source.any() or gaps:
+        if self.phase:
+            lost, supplied = C.c_double(self.lost), C.c_double(self.supplied)
+            lib().orc_solid_step_phase(nx, ny, nz, self.dx, p(self.kx), p(self.ky), p(self.kz),
+                                       p(self.mob), p(self.cap), dt, p(self.cells), p(self._old),
+                                       _ip(self.on), p(self.ambient), p(self.h), p(self.area),
+                                       p(self.emis), p(self.ksub), p(self.source), _u8(self.voids),
+                                       gaps, _sp(self.gap_a), _sp(self.gap_b), p(self.gap_coef),
+                                       *self._phase_args(), C.byref(lost), C.byref(supplied))
+            self.lost, self.supplied = lost.value, supplied.value
+            if self.two_phase:
+                self.resolve()
+        elif self.exposed() or self.source.any() or gaps:
             lost, supplied = C.c_double(self.lost), C.c_double(self.supplied)
             lib().orc_solid_step_ext(nx, ny, nz, self.dx, p(self.kx), p(self.ky), p(self.kz),
                                      p(self.mob), p(self.cap), dt, p(self.cells), p(self._old),
@@ -371,7 +450,66 @@ class OracleSolid3D:
                                  p(self.cells), p(self._old))
 
     def stored_heat(self) -> float:
+        if self.phase:
+            return lib().orc_stored_heat_phase(self.n, p(self.cap), p(self.cells), self.reference,
+                                               *self._phase_args())
         return lib().orc_stored_heat(self.n, p(self.cap), p(self.cells), self.reference)
+
+    def set_temperature(self, i: int, j: int, k: int, t: float) -> None:
+        """solid.rs:1346-1361: naming a temperature names the phase of a cell that melts."""
+        nx, ny, nz = self.counts_
+        if i >= nx or j >= ny or k >= nz:
+            return
+        c = i + nx * (j + ny * k)
+        self.cells[c] = t
+        if self.phase and self.latent[c] > 0.0:
+            self.melted[c] = 1.0 if t > self.point[c] else 0.0
+
+    def set_melted_fraction(self, i: int, j: int, k: int, fraction: float) -> None:
+        """solid.rs:448-466."""
+        nx, ny, nz = self.counts_
+        if i >= nx or j >= ny or k >= nz or not self.phase:
+            return
+        c = i + nx * (j + ny * k)
+        if self.latent[c] <= 0.0:
+            return
+        phi = min(max(fraction, 0.0), 1.0)
+        self.melted[c] = phi
+        point = self.point[c]
+        if phi >= 1.0:
+            self.cells[c] = max(self.cells[c], point)
+        elif phi <= 0.0:
+            self.cells[c] = min(self.cells[c], point)
+        else:
+            self.cells[c] = point
+
+    def melted_fraction_at(self, i: int, j: int, k: int) -> float:
+        nx, ny, nz = self.counts_
+        if not self.phase:
+            return 0.0
+        return float(self.melted[min(i, nx - 1) + nx * (min(j, ny - 1) + ny * min(k, nz - 1))])
+
+    def melted_volume(self) -> float:
+        """solid.rs:476-478: the left fold of the fractions times the cell volume."""
+        if not self.phase:
+            return 0.0
+        total = 0.0
+        for v in self.melted:
+            total += float(v)
+        return total * (self.dx * self.dx * self.dx)
+
+    def checkpoint(self) -> None:
+        self.saved = self.cells.copy()
+        if self.phase:
+            self.saved_melted = self.melted.copy()
+        self.saved_books = (self.lost, self.supplied)
+
+    def restore(self) -> None:
+        """solid.rs:1945-1950: the operator is not rebuilt."""
+        self.cells = self.saved.copy()
+        if self.phase:
+            self.melted = self.saved_melted.copy()
+        self.lost, self.supplied = getattr(self, "saved_books", (0.0, 0.0))
 
     def stored_heat_compensated(self) -> float:
         return lib().orc_stored_heat_compensated(self.n, p(self.cap), p(self.cells),
@@ -401,7 +539,12 @@ class OracleSolid3D:
         if i >= nx or j >= ny or k >= nz:
             return
         idx = i + nx * (j + ny * k)
-        self.cells[idx] += joules / self.cap[idx]
+        if self.phase:
+            lib().orc_deposit_phase(idx, joules, p(self.cells), p(self.melted), p(self.cap),
+                                    p(self.point), p(self.latent), p(self.c_solid),
+                                    p(self.c_liquid))
+        else:
+            self.cells[idx] += joules / self.cap[idx]
         self.absorbed += joules
 
     def grid(self) -> np.ndarray:

@@ -967,3 +967,146 @@ double orc_sample_void(size_t nx, size_t ny, size_t nz, double dx, const double 
             }
     return weight > 0.0 ? sum / weight : NAN;
 }
+
+/* ------------------------------------------------------------------------------------------ */
+/* Phase change: the enthalpy method of Solid3D, solid.rs:1563-1648                           */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Per-cell phase state: the melted fraction (written), the mixed capacity the step converts a
+ * rise through, the melting point, the latent joules of the cell, and the solid and liquid
+ * capacities. */
+typedef struct {
+    double *melted;
+    const double *capacity;
+    const double *point;
+    const double *latent;
+    const double *cap_solid;
+    const double *cap_liquid;
+} OrcPhase;
+
+/* Solid3D::enthalpy_joules, solid.rs:1578-1593 (the latent > 0 branches). */
+static double orc_enthalpy(const OrcPhase *ph, size_t c, double t) {
+    double point = ph->point[c], latent = ph->latent[c];
+    if (ph->melted[c] >= 1.0) return latent + ph->cap_liquid[c] * (t - point);
+    if (ph->melted[c] <= 0.0) return ph->cap_solid[c] * (t - point);
+    (void)point;
+    return ph->melted[c] * latent;
+}
+
+/* Solid3D::add_kelvin and state_from_enthalpy, solid.rs:1620-1648. */
+static void orc_add_kelvin(const OrcPhase *ph, size_t c, double rise, double *cells) {
+    if (ph && ph->latent[c] > 0.0) {
+        double joules = rise * ph->capacity[c];
+        double e = orc_enthalpy(ph, c, cells[c]) + joules;
+        double point = ph->point[c], latent = ph->latent[c];
+        if (e <= 0.0) {
+            cells[c] = point + e / ph->cap_solid[c];
+            ph->melted[c] = 0.0;
+        } else if (e >= latent) {
+            cells[c] = point + (e - latent) / ph->cap_liquid[c];
+            ph->melted[c] = 1.0;
+        } else {
+            cells[c] = point;
+            ph->melted[c] = e / latent;
+        }
+    } else {
+        cells[c] += rise;
+    }
+}
+
+/* Solid3D::stored_heat with phases, solid.rs:1557-1614: a left fold of
+ * enthalpy_joules - reference_enthalpy_joules in index order. */
+double orc_stored_heat_phase(size_t n, const double *cap, const double *cells, double reference,
+                             const double *melted, const double *point, const double *latent,
+                             const double *cap_solid, const double *cap_liquid) {
+    OrcPhase ph = {(double *)melted, cap, point, latent, cap_solid, cap_liquid};
+    double acc = 0.0;
+    for (size_t c = 0; c < n; c++) {
+        double e, r;
+        if (latent[c] <= 0.0) {
+            e = cap[c] * (cells[c] - reference);
+            r = 0.0;
+        } else {
+            e = orc_enthalpy(&ph, c, cells[c]);
+            r = reference > point[c] ? latent[c] + cap_liquid[c] * (reference - point[c])
+                                     : cap_solid[c] * (reference - point[c]);
+        }
+        acc += e - r;
+    }
+    return acc;
+}
+
+/* The bus share with phases and void: a uniform rise through add_kelvin (solid.rs:1752-1765). */
+void orc_bus_heat_phase(size_t n, const double *cap, const unsigned char *voids, double gained,
+                        double *cells, double *melted, const double *point, const double *latent,
+                        const double *cap_solid, const double *cap_liquid) {
+    OrcPhase ph = {melted, cap, point, latent, cap_solid, cap_liquid};
+    double total = 0.0;
+    for (size_t c = 0; c < n; c++) total += cap[c];
+    double rise = gained / total;
+    for (size_t c = 0; c < n; c++)
+        if (!voids[c]) orc_add_kelvin(&ph, c, rise, cells);
+}
+
+/* Solid3D::step's sweep, film, sources and gaps with phases and void, solid.rs:1767-1878: every
+ * change to a cell goes through add_kelvin; sources skip void cells; the gaps read `old`. */
+void orc_solid_step_phase(size_t nx, size_t ny, size_t nz, double dx, const double *kx,
+                          const double *ky, const double *kz, const double *mobility,
+                          const double *cap, double dts, double *cells, double *old,
+                          const int *on, const double *ambient, const double *h,
+                          const double *area, const double *emissivity,
+                          const double *conductivity, const double *source,
+                          const unsigned char *voids, size_t gaps, const size_t *ga,
+                          const size_t *gb, const double *gcoef, double *melted,
+                          const double *point, const double *latent, const double *cap_solid,
+                          const double *cap_liquid, double *lost, double *supplied) {
+    OrcPhase ph = {melted, cap, point, latent, cap_solid, cap_liquid};
+    size_t n = nx * ny * nz;
+    memcpy(old, cells, sizeof(double) * n);
+    int exposed = 0;
+    for (int f = 0; f < 6; f++) exposed |= on[f];
+    for (size_t k = 0; k < nz; k++)
+        for (size_t j = 0; j < ny; j++)
+            for (size_t i = 0; i < nx; i++) {
+                size_t c = i + nx * (j + ny * k);
+                double t = old[c];
+                size_t xr = (nx + 1) * (j + ny * k), yr = nx * (j + (ny + 1) * k);
+                double flux = 0.0;
+                if (i > 0) flux += kx[i + xr] * (old[c - 1] - t);
+                if (i + 1 < nx) flux += kx[i + 1 + xr] * (old[c + 1] - t);
+                if (j > 0) flux += ky[i + yr] * (old[c - nx] - t);
+                if (j + 1 < ny) flux += ky[i + nx + yr] * (old[c + nx] - t);
+                if (k > 0) flux += kz[c] * (old[c - nx * ny] - t);
+                if (k + 1 < nz) flux += kz[c + nx * ny] * (old[c + nx * ny] - t);
+                double shed = exposed ? orc_film_flux(nx, ny, nz, dx, i, j, k, t, on, ambient, h,
+                                                      area, emissivity[c], conductivity[c])
+                                      : 0.0;
+                orc_add_kelvin(&ph, c, dts * mobility[c] * (flux - shed), cells);
+                if (shed != 0.0) *lost += dts * mobility[c] * shed * cap[c];
+            }
+    int supplies = 0;
+    for (size_t c = 0; c < n && source && !supplies; c++) supplies = source[c] != 0.0;
+    if (supplies)
+        for (size_t c = 0; c < n; c++) {
+            if (source[c] == 0.0 || (voids && voids[c])) continue;
+            double joules = source[c] * dts;
+            orc_add_kelvin(&ph, c, joules / cap[c], cells);
+            *supplied += joules;
+        }
+    for (size_t g = 0; g < gaps; g++) {
+        double ta = old[ga[g]], tb = old[gb[g]];
+        double watts = gcoef[g] * (orc_pow4(ta) - orc_pow4(tb));
+        if (watts == 0.0) continue;
+        double joules = watts * dts;
+        orc_add_kelvin(&ph, ga[g], -joules / cap[ga[g]], cells);
+        orc_add_kelvin(&ph, gb[g], joules / cap[gb[g]], cells);
+    }
+}
+
+/* Solid3D::deposit with phases, solid.rs:1364-1372. */
+void orc_deposit_phase(size_t c, double joules, double *cells, double *melted, const double *cap,
+                       const double *point, const double *latent, const double *cap_solid,
+                       const double *cap_liquid) {
+    OrcPhase ph = {melted, cap, point, latent, cap_solid, cap_liquid};
+    orc_add_kelvin(&ph, c, joules / cap[c], cells);
+}

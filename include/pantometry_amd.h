@@ -58,6 +58,13 @@ typedef struct pm_substance {
     double conductivity;
     double specific_heat;
     double emissivity;
+    /* FusionProps, crates/pantometry-core/src/substance.rs:119-193: the melting point (K) and
+     * the latent heat (J/kg), both NaN for a substance that does not melt; the liquid's
+     * conductivity and specific heat, both NaN for the one-phase model (liquid = solid). */
+    double melting_point;
+    double latent_heat;
+    double liquid_conductivity;
+    double liquid_specific_heat;
 } pm_substance;
 
 /* The three entries of Solid3D::ledger, crates/pantometry-thermal/src/solid.rs:1904-1927:
@@ -183,6 +190,17 @@ int pm_solid3d_losing_from(pm_solid3d *solid, int face, const pm_environment *en
  * selects (x-fastest bytes, nonzero = selected), replacing what they generated before; added
  * after each sweep as (W*dt)/C and kept in the `supplied` entry of the books. */
 int pm_solid3d_dissipating(pm_solid3d *solid, double watts, const uint8_t *mask);
+/* Phase change (solid.rs:423-478, 1563-1648): a substance with a melting point and a latent heat
+ * carries a melted fraction per cell and every heat that reaches the cell goes through its
+ * enthalpy; a substance with a distinct liquid mixes its conductivity and capacity by that
+ * fraction and the block's operator is rebuilt after every step. */
+/* Solid3D::set_melted_fraction, solid.rs:448-466 (ignored for a cell that does not melt). */
+int pm_solid3d_set_melted_fraction(pm_solid3d *solid, size_t i, size_t j, size_t k, double fraction);
+/* melted_fraction_at (solid.rs:427-430) for every cell into out[n] (zeros for a block that
+ * cannot melt), and melted_volume (solid.rs:476-478) in m^3. */
+int pm_solid3d_melted(pm_solid3d *solid, double *out, double *volume);
+/* How many cells can melt (latent heat above zero): the condition of the "melted" reading. */
+size_t pm_solid3d_melting_cells(const pm_solid3d *solid);
 /* Solid3D::empty, solid.rs:992-1005: the cells under `mask` hold nothing from now on (a fill
  * does not bring them back).  A void cell conducts and stores nothing; solid cells facing each
  * other across a run of void along a grid line exchange radiation sigma A (Ta^4 - Tb^4)/

@@ -21,7 +21,9 @@ class pm_violation(C.Structure):
 
 class pm_substance(C.Structure):
     _fields_ = [("density", C.c_double), ("conductivity", C.c_double),
-                ("specific_heat", C.c_double), ("emissivity", C.c_double)]
+                ("specific_heat", C.c_double), ("emissivity", C.c_double),
+                ("melting_point", C.c_double), ("latent_heat", C.c_double),
+                ("liquid_conductivity", C.c_double), ("liquid_specific_heat", C.c_double)]
 
 
 class pm_environment(C.Structure):
@@ -82,6 +84,9 @@ SIGNATURES = {
     "pm_solid3d_generated_power": (_I, [_VP, _DP]),
     "pm_solid3d_sample": (_I, [_VP, _DP, _S, _DP]),
     "pm_solid3d_empty": (_I, [_VP, C.POINTER(C.c_uint8)]),
+    "pm_solid3d_set_melted_fraction": (_I, [_VP, _S, _S, _S, _D]),
+    "pm_solid3d_melted": (_I, [_VP, _DP, _DP]),
+    "pm_solid3d_melting_cells": (_S, [_VP]),
     "pm_solid3d_void_cells": (_S, [_VP]),
     "pm_solid3d_is_void": (_I, [_VP, _S, _S, _S]),
     "pm_solid3d_gap_pairs": (_S, [_VP, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _DP, _S]),

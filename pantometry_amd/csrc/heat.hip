@@ -732,6 +732,84 @@ __global__ void audit_reset(AuditState *st, double total, double scale, double t
     st->before = st->after = st->scale = 0.0;
 }
 
+// ---- Phase change: the enthalpy method of solid.rs:1563-1648 ------------------------------
+//
+// A cell whose substance melts carries a melted fraction beside its temperature; its state is the
+// one monotone number e (kelvin of sensible heat below the melting point, phi*L inside, L + above),
+// and every heat that reaches it is added to e and inverted back.  `cap` is the capacity the rise
+// was computed with (the mixed one of the last rebuild), cs and cl the solid and liquid ones.
+struct PhaseArgs {
+    double *melted;
+    const double *point, *latent, *cs, *cl;
+};
+
+// Solid3D::enthalpy_joules, solid.rs:1578-1593, latent > 0.
+__device__ __forceinline__ double enthalpy_of(const PhaseArgs &ph, int64_t c, double t) {
+    const double phi = ph.melted[c];
+    if (phi >= 1.0) return ph.latent[c] + ph.cl[c] * (t - ph.point[c]);
+    if (phi <= 0.0) return ph.cs[c] * (t - ph.point[c]);
+    return phi * ph.latent[c];
+}
+
+// Solid3D::add_kelvin and state_from_enthalpy, solid.rs:1620-1648: the cell's new temperature,
+// its fraction written back.
+__device__ __forceinline__ double add_kelvin(const PhaseArgs &ph, int64_t c, double t, double rise,
+                                             double cap) {
+    if (!ph.melted || !(ph.latent[c] > 0.0)) return t + rise;
+    const double joules = rise * cap;
+    const double e = enthalpy_of(ph, c, t) + joules;
+    const double point = ph.point[c], latent = ph.latent[c];
+    if (e <= 0.0) {
+        ph.melted[c] = 0.0;
+        return point + e / ph.cs[c];
+    }
+    if (e >= latent) {
+        ph.melted[c] = 1.0;
+        return point + (e - latent) / ph.cl[c];
+    }
+    ph.melted[c] = e / latent;
+    return point;
+}
+
+// The bus share with phases (and void): a uniform rise through add_kelvin, solid.rs:1752-1765.
+__global__ void add_uniform_phase(double *p, const uint8_t *vd, int64_t n, double rise,
+                                  const double *cap, PhaseArgs ph) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        if (!(vd && vd[i])) p[i] = add_kelvin(ph, i, p[i], rise, cap[i]);
+}
+
+// Solid3D::deposit, solid.rs:1364-1372, for a block that can melt.
+__global__ void deposit_phase(double *T, int64_t c, double joules, const double *cap, PhaseArgs ph) {
+    T[c] = add_kelvin(ph, c, T[c], joules / cap[c], cap[c]);
+}
+
+// A two-phase block's operator from its fractions, solid.rs:686-702 and 753-756: conductivity and
+// capacity mixed arithmetically, void conducting and holding nothing, mobility dx/C.
+__global__ void mix_phases(const double *melted, const double *ks, const double *kl,
+                           const double *cs, const double *cl, const uint8_t *vd, int64_t n,
+                           double dx, double *kcell, double *cap, double *mob) {
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        if (vd && vd[c]) {
+            kcell[c] = 0.0, cap[c] = 0.0, mob[c] = 0.0;
+            continue;
+        }
+        const double phi = melted[c];
+        kcell[c] = ks[c] + phi * (kl[c] - ks[c]);
+        cap[c] = cs[c] + phi * (cl[c] - cs[c]);
+        mob[c] = dx / cap[c];
+    }
+}
+
+// Naming a temperature names the phase of a cell that melts (solid.rs:1352-1360).
+__global__ void phase_from_temperature(const double *T, const double *point, const double *latent,
+                                       int64_t n, double *melted) {
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n;
+         c += (int64_t)gridDim.x * blockDim.x)
+        if (latent[c] > 0.0) melted[c] = T[c] > point[c] ? 1.0 : 0.0;
+}
+
 static inline int grid_for(int64_t n, int threads) {
     int64_t g = (n + threads - 1) / threads;
     return (int)std::max<int64_t>(1, std::min<int64_t>(g, 65536));
@@ -801,6 +879,21 @@ struct pm_solid3d {
     std::vector<Gap> gaps;
     DevBuf gap_a, gap_b, gap_coef, gcell, gstart, gent, gap_part;
     int64_t ngcell = 0;
+    // Phase change (solid.rs:586-680): once any substance melts, per-cell solid and liquid
+    // conductivity and capacity, melting point, latent joules, and the melted fraction (with
+    // the copy a checkpoint keeps).  Host copies of point and latent serve set_temperature.
+    bool phase = false, two_phase = false;
+    DevBuf ph_ks, ph_kl, ph_cs, ph_cl, ph_point, ph_latent, melted, saved_melted;
+    std::vector<double> h_point, h_latent;
+    int64_t nlatent = 0;
+    PhaseArgs phase_args() const {
+        PhaseArgs a{};
+        if (phase) {
+            a.melted = melted.d(), a.point = ph_point.d(), a.latent = ph_latent.d();
+            a.cs = ph_cs.d(), a.cl = ph_cl.d();
+        }
+        return a;
+    }
 
     int64_t n() const { return nx * ny * nz; }
     double *T() const { return t[cur].d(); }
@@ -809,7 +902,7 @@ struct pm_solid3d {
             if (f.on) return true;
         return false;
     }
-    bool extended() const { return exposed() || !source.empty() || !gaps.empty(); }
+    bool extended() const { return exposed() || !source.empty() || !gaps.empty() || nlatent > 0; }
     bool is_void(int64_t c) const { return nvoid && voids[(size_t)c]; }
 };
 
@@ -1245,7 +1338,7 @@ __global__ __launch_bounds__(256) void heat_step_ext(const double *__restrict__ 
                                                      const double *__restrict__ cap, double cap0,
                                                      const double *__restrict__ shed, double dts,
                                                      int64_t nx, int64_t ny, int64_t nz,
-                                                     double *__restrict__ lost_part) {
+                                                     double *__restrict__ lost_part, PhaseArgs ph) {
     __shared__ double red[4];
     const int64_t plane = nx * ny, n = plane * nz;
     double lost = 0.0;
@@ -1262,7 +1355,7 @@ __global__ __launch_bounds__(256) void heat_step_ext(const double *__restrict__ 
         if (k + 1 < nz) flux = flux + (kc ? series_nz(kk, kc[c + plane]) : kface) * (told[c + plane] - t);
         const double sh = shed ? shed[c] : 0.0;
         const double dtm = dts * (mob ? mob[c] : mob0);
-        tnew[c] = t + dtm * (flux - sh);
+        tnew[c] = add_kelvin(ph, c, t, dtm * (flux - sh), cap ? cap[c] : cap0);
         if (sh != 0.0) lost += dtm * sh * (cap ? cap[c] : cap0);
     }
     const double total = block_sum<256>(lost, red);
@@ -1272,12 +1365,13 @@ __global__ __launch_bounds__(256) void heat_step_ext(const double *__restrict__ 
 // The sources, after the sweep: each generating cell gains (W*dt)/C (solid.rs:1854-1864).
 __global__ void heat_sources(double *__restrict__ T, const int64_t *__restrict__ sc,
                              const double *__restrict__ sw, int64_t ns,
-                             const double *__restrict__ cap, double cap0, double dts) {
+                             const double *__restrict__ cap, double cap0, double dts, PhaseArgs ph) {
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < ns;
          q += (int64_t)gridDim.x * blockDim.x) {
         const int64_t c = sc[q];
         const double joules = sw[q] * dts;
-        T[c] = T[c] + joules / (cap ? cap[c] : cap0);
+        const double cc = cap ? cap[c] : cap0;
+        T[c] = add_kelvin(ph, c, T[c], joules / cc, cc);
     }
 }
 
@@ -1355,7 +1449,7 @@ __global__ __launch_bounds__(256) void gap_exchange(const int64_t *__restrict__ 
                                                     const double *__restrict__ gcoef,
                                                     const double *__restrict__ told,
                                                     const double *__restrict__ cap, double dts,
-                                                    double *__restrict__ tnew) {
+                                                    double *__restrict__ tnew, PhaseArgs ph) {
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < ngcell;
          q += (int64_t)gridDim.x * blockDim.x) {
         const int64_t c = gcell[q];
@@ -1366,7 +1460,7 @@ __global__ __launch_bounds__(256) void gap_exchange(const int64_t *__restrict__ 
             const double watts = gcoef[g] * (pow4(ta) - pow4(tb));
             if (watts == 0.0) continue;
             const double joules = watts * dts;
-            t = t + ((gent[e] & 1) ? joules / cap[c] : -joules / cap[c]);
+            t = add_kelvin(ph, c, t, (gent[e] & 1) ? joules / cap[c] : -joules / cap[c], cap[c]);
         }
         tnew[c] = t;
     }
@@ -1466,6 +1560,83 @@ static int ensure_tables(pm_solid3d *s) {
     }
     if (!s->ext_part) PM_CHECK(s->ext_part.alloc(sizeof(double) * EXT_BLOCKS));
     s->tables_valid = true;
+    return PM_OK;
+}
+
+// Whether a substance melts (FusionProps present, with thermal properties to melt with) and
+// whether its liquid differs from its solid (solid.rs:621-629, 640-646).
+static bool melts(const pm_substance &m) {
+    return !std::isnan(m.melting_point) && !std::isnan(m.latent_heat) && !std::isnan(m.conductivity) &&
+           !std::isnan(m.specific_heat);
+}
+static bool two_phased(const pm_substance &m) {
+    return !std::isnan(m.liquid_conductivity) && !std::isnan(m.conductivity) && !std::isnan(m.specific_heat) &&
+           (m.liquid_conductivity != m.conductivity || m.liquid_specific_heat != m.specific_heat);
+}
+
+// The per-cell phase tables of resolve (solid.rs:599-646) from each cell's substance: solid and
+// liquid conductivity and capacity, melting point (INFINITY for a substance that does not melt)
+// and latent joules.  The first build gives every cell its fraction from its temperature when
+// `fresh` (the constructor's), zero otherwise (solid.rs:648-675).
+static int build_phase(pm_solid3d *s, bool fresh, double initial) {
+    const int64_t n = s->n();
+    const double volume = s->dx * s->dx * s->dx;
+    PM_CHECK(promote_c(s));
+    std::vector<double> ks((size_t)n), kl((size_t)n), cs((size_t)n), cl((size_t)n);
+    s->h_point.assign((size_t)n, INFINITY);
+    s->h_latent.assign((size_t)n, 0.0);
+    s->nlatent = 0;
+    for (int64_t c = 0; c < n; c++) {
+        const pm_substance &m = s->subs[s->which.empty() ? 0 : s->which[(size_t)c]];
+        const double k_solid = m.conductivity;           // NaN for a substance that does not say
+        const double c_solid = (m.density * volume) * m.specific_heat;
+        const bool liquid = !std::isnan(m.liquid_conductivity) && !std::isnan(m.liquid_specific_heat);
+        ks[(size_t)c] = k_solid, cs[(size_t)c] = c_solid;
+        kl[(size_t)c] = liquid ? m.liquid_conductivity : k_solid;
+        cl[(size_t)c] = liquid ? (m.density * volume) * m.liquid_specific_heat : c_solid;
+        if (melts(m)) {
+            s->h_point[(size_t)c] = m.melting_point;
+            s->h_latent[(size_t)c] = (m.density * volume) * m.latent_heat;
+            s->nlatent += s->h_latent[(size_t)c] > 0.0;
+        }
+    }
+    const size_t bytes = sizeof(double) * (size_t)n;
+    if (!s->phase) {
+        PM_CHECK(s->ph_ks.alloc(bytes));
+        PM_CHECK(s->ph_kl.alloc(bytes));
+        PM_CHECK(s->ph_cs.alloc(bytes));
+        PM_CHECK(s->ph_cl.alloc(bytes));
+        PM_CHECK(s->ph_point.alloc(bytes));
+        PM_CHECK(s->ph_latent.alloc(bytes));
+        PM_CHECK(s->melted.alloc(bytes));
+        PM_CHECK(s->saved_melted.alloc(bytes));
+        std::vector<double> phi((size_t)n, 0.0);
+        if (fresh)
+            for (int64_t c = 0; c < n; c++) phi[(size_t)c] = initial > s->h_point[(size_t)c] ? 1.0 : 0.0;
+        PM_HIP(hipMemcpy(s->melted.d(), phi.data(), bytes, hipMemcpyHostToDevice));
+        PM_HIP(hipMemcpy(s->saved_melted.d(), phi.data(), bytes, hipMemcpyHostToDevice));
+        s->phase = true;
+    }
+    PM_HIP(hipMemcpy(s->ph_ks.d(), ks.data(), bytes, hipMemcpyHostToDevice));
+    PM_HIP(hipMemcpy(s->ph_kl.d(), kl.data(), bytes, hipMemcpyHostToDevice));
+    PM_HIP(hipMemcpy(s->ph_cs.d(), cs.data(), bytes, hipMemcpyHostToDevice));
+    PM_HIP(hipMemcpy(s->ph_cl.d(), cl.data(), bytes, hipMemcpyHostToDevice));
+    PM_HIP(hipMemcpy(s->ph_point.d(), s->h_point.data(), bytes, hipMemcpyHostToDevice));
+    PM_HIP(hipMemcpy(s->ph_latent.d(), s->h_latent.data(), bytes, hipMemcpyHostToDevice));
+    s->tables_valid = false;
+    return PM_OK;
+}
+
+// A two-phase block's operator from the fractions it carries (solid.rs:686-702), before resolve.
+static int remix(pm_solid3d *s) {
+    if (!s->two_phase) return PM_OK;
+    PM_CHECK(promote_c(s));
+    PM_CHECK(promote_k(s));
+    mix_phases<<<grid_for(s->n(), 256), 256, 0, s->stream>>>(
+        s->melted.d(), s->ph_ks.d(), s->ph_kl.d(), s->ph_cs.d(), s->ph_cl.d(),
+        s->nvoid ? s->dvoid.as<uint8_t>() : nullptr, s->n(), s->dx, s->kcell.d(), s->cap.d(), s->mob.d());
+    PM_HIP(hipGetLastError());
+    s->heat_capacity_valid = false;
     return PM_OK;
 }
 
@@ -1613,17 +1784,19 @@ static int step_ext(pm_solid3d *s, double dt) {
     double *out = s->t[s->cur ^ 1].d();
     heat_step_ext<<<EXT_BLOCKS, 256, 0, s->stream>>>(s->T(), out, kc, s->kface, mob, s->mob0, cap,
                                                      s->cap0, film ? s->shed.d() : nullptr, dt,
-                                                     s->nx, s->ny, s->nz, s->ext_part.d());
+                                                     s->nx, s->ny, s->nz, s->ext_part.d(),
+                                                     s->phase_args());
     PM_HIP(hipGetLastError());
     if (s->nsource) {
         heat_sources<<<grid_for(s->nsource, 256), 256, 0, s->stream>>>(
-            out, s->src_c.as<int64_t>(), s->src_w.d(), s->nsource, cap, s->cap0, dt);
+            out, s->src_c.as<int64_t>(), s->src_w.d(), s->nsource, cap, s->cap0, dt, s->phase_args());
         PM_HIP(hipGetLastError());
     }
     if (s->ngcell) {
         gap_exchange<<<grid_for(s->ngcell, 256), 256, 0, s->stream>>>(
             s->gcell.as<int64_t>(), s->gstart.as<int64_t>(), s->gent.as<int64_t>(), s->ngcell,
-            s->gap_a.as<int64_t>(), s->gap_b.as<int64_t>(), s->gap_coef.d(), s->T(), s->cap.d(), dt, out);
+            s->gap_a.as<int64_t>(), s->gap_b.as<int64_t>(), s->gap_coef.d(), s->T(), s->cap.d(), dt, out,
+            s->phase_args());
         PM_HIP(hipGetLastError());
     }
     std::vector<double> part(EXT_BLOCKS);
@@ -1635,6 +1808,11 @@ static int step_ext(pm_solid3d *s, double dt) {
     s->lost += lost;
     for (const auto &e : s->source)
         if (!s->is_void(e.first)) s->supplied += e.second * dt;  // void makes nothing (solid.rs:1857)
+    // A two-phase block re-derives its operator, because the fractions moved (solid.rs:1897-1899).
+    if (s->two_phase) {
+        PM_CHECK(remix(s));
+        PM_CHECK(resolve(s));
+    }
     return PM_OK;
 }
 
@@ -1658,13 +1836,24 @@ static int check_ratio(const pm_solid3d *cs, double dt, pm_violation *v, int64_t
     return PM_OK;
 }
 
+// Solid3D::stored_heat, solid.rs:1557-1614: each cell's enthalpy above the reference state, in
+// enthalpy for a cell that melts, capacity*(t - reference) otherwise.
 __global__ __launch_bounds__(256) void heat_stored_kernel(const double *T, const double *cap,
                                                           double cap0, double tref, int64_t n,
-                                                          int64_t per_block, double *partials) {
+                                                          int64_t per_block, double *partials,
+                                                          PhaseArgs ph) {
     __shared__ double red[4];
     const int64_t lo = blockIdx.x * per_block, hi = std::min(n, lo + per_block);
     double acc = 0.0;
-    for (int64_t c = lo + threadIdx.x; c < hi; c += 256) acc += (cap ? cap[c] : cap0) * (T[c] - tref);
+    for (int64_t c = lo + threadIdx.x; c < hi; c += 256) {
+        if (ph.melted && ph.latent[c] > 0.0) {
+            const double point = ph.point[c], latent = ph.latent[c];
+            const double r = tref > point ? latent + ph.cl[c] * (tref - point) : ph.cs[c] * (tref - point);
+            acc += enthalpy_of(ph, c, T[c]) - r;
+        } else {
+            acc += (cap ? cap[c] : cap0) * (T[c] - tref);
+        }
+    }
     double total = block_sum<256>(acc, red);
     if (threadIdx.x == 0) partials[blockIdx.x] = total;
 }
@@ -1677,7 +1866,7 @@ int heat_stored(pm_solid3d *s, double *stored) {
     PM_CHECK(part.alloc(sizeof(double) * (size_t)blocks));
     heat_stored_kernel<<<(unsigned)blocks, 256, 0, s->stream>>>(
         s->T(), s->cmode == C_PER_CELL ? s->cap.d() : nullptr, s->cap0, s->reference, n,
-        per_block, part.d());
+        per_block, part.d(), s->phase_args());
     PM_HIP(hipGetLastError());
     heat_finalize<<<1, 1024, 0, s->stream>>>(part.d(), blocks, s->audit.as<AuditState>(), 0.0,
                                              0.0, -1, nullptr, s->stored_dev.d());
@@ -1737,6 +1926,11 @@ int pm_solid3d_create(const char *name, const pm_substance *substance, size_t nx
     fill_value<<<grid_for(s->n(), 256), 256, 0, s->stream>>>(s->t[0].d(), s->n(), initial);
     PM_HIP(hipGetLastError());
     PM_CHECK(reset_audit(s.get(), 0.0, 0.0, 1e-9));
+    if (melts(*substance) || two_phased(*substance)) {
+        s->two_phase = two_phased(*substance);
+        PM_CHECK(build_phase(s.get(), true, initial));
+        PM_CHECK(remix(s.get()));
+    }
     PM_CHECK(resolve(s.get()));
     PM_HIP(hipStreamSynchronize(s->stream));
     *out = s.release();
@@ -1768,6 +1962,8 @@ int pm_solid3d_fill(pm_solid3d *s, const pm_substance *sub, const uint8_t *mask)
     const double k = sub->conductivity;
     const double c = capacity_of(sub, s->dx);
     const int64_t n = s->n();
+    // The block's list of materials grows with every fill, selected cells or not (solid.rs:523).
+    if (two_phased(*sub)) s->two_phase = true;
     bool any = false;
     for (int64_t i = 0; i < n && !any; i++) any = mask[i] != 0;
     if (!any) return PM_OK;
@@ -1795,14 +1991,15 @@ int pm_solid3d_fill(pm_solid3d *s, const pm_substance *sub, const uint8_t *mask)
     s->tables_valid = false;
     const bool k_same = s->kmode == K_UNIFORM && same_bits(k, s->k0);
     const bool c_same = s->cmode == C_UNIFORM && same_bits(c, s->cap0);
-    if (all && (s->kmode == K_UNIFORM) && (s->cmode == C_UNIFORM)) {
+    const bool phased = s->phase || melts(*sub) || s->two_phase;
+    if (all && !phased && (s->kmode == K_UNIFORM) && (s->cmode == C_UNIFORM)) {
         // The whole block becomes the new substance: still one material.
         s->k0 = k, s->cap0 = c, s->mob0 = s->dx / c;
         double sum = k + k;
         s->kface = sum > 0.0 ? 2.0 * k * k / sum : 0.0;
         return resolve(s);
     }
-    if (k_same && c_same) return s->nvoid ? find_gaps(s) : PM_OK;
+    if (k_same && c_same && !phased) return s->nvoid ? find_gaps(s) : PM_OK;
     if (!c_same) PM_CHECK(promote_c(s));
     if (!k_same) PM_CHECK(promote_k(s));
     DevBuf dmask;
@@ -1813,6 +2010,10 @@ int pm_solid3d_fill(pm_solid3d *s, const pm_substance *sub, const uint8_t *mask)
         s->cmode == C_PER_CELL ? s->cap.d() : nullptr,
         s->cmode == C_PER_CELL ? s->mob.d() : nullptr, k, c, s->dx);
     PM_HIP(hipGetLastError());
+    if (s->phase || melts(*sub) || s->two_phase) {
+        PM_CHECK(build_phase(s, false, 0.0));
+        PM_CHECK(remix(s));
+    }
     PM_CHECK(apply_voids(s));
     PM_CHECK(resolve(s));
     PM_CHECK(find_gaps(s));  // a new substance brings its own emissivity
@@ -1827,6 +2028,12 @@ int pm_solid3d_set_conductivity(pm_solid3d *s, const double *k) {
         s->kmode = K_PER_CELL;
     }
     PM_HIP(hipMemcpyAsync(s->kcell.d(), k, sizeof(double) * (size_t)s->n(), hipMemcpyHostToDevice, s->stream));
+    if (s->phase) {
+        // Both phases of every cell take the given conductivity.
+        PM_HIP(hipMemcpyAsync(s->ph_ks.d(), k, sizeof(double) * (size_t)s->n(), hipMemcpyHostToDevice, s->stream));
+        PM_HIP(hipMemcpyAsync(s->ph_kl.d(), k, sizeof(double) * (size_t)s->n(), hipMemcpyHostToDevice, s->stream));
+        PM_CHECK(remix(s));
+    }
     PM_CHECK(apply_voids(s));
     PM_CHECK(resolve(s));
     PM_HIP(hipStreamSynchronize(s->stream));
@@ -1836,6 +2043,11 @@ int pm_solid3d_set_conductivity(pm_solid3d *s, const double *k) {
 int pm_solid3d_set_cells(pm_solid3d *s, const double *cells) {
     if (!s || !cells) return PM_ERR_ARGUMENT;
     PM_HIP(hipMemcpyAsync(s->T(), cells, sizeof(double) * (size_t)s->n(), hipMemcpyHostToDevice, s->stream));
+    if (s->phase) {
+        phase_from_temperature<<<grid_for(s->n(), 256), 256, 0, s->stream>>>(
+            s->T(), s->ph_point.d(), s->ph_latent.d(), s->n(), s->melted.d());
+        PM_HIP(hipGetLastError());
+    }
     PM_HIP(hipStreamSynchronize(s->stream));
     return PM_OK;
 }
@@ -1865,9 +2077,56 @@ int pm_solid3d_set_temperature(pm_solid3d *s, size_t i, size_t j, size_t k, doub
     if ((int64_t)i >= s->nx || (int64_t)j >= s->ny || (int64_t)k >= s->nz) return PM_OK;  // ignored, solid.rs:1346
     const int64_t idx = (int64_t)i + s->nx * ((int64_t)j + s->ny * (int64_t)k);
     PM_HIP(hipMemcpyAsync(s->T() + idx, &kelvin, sizeof(double), hipMemcpyHostToDevice, s->stream));
+    double phi = 0.0;
+    if (s->phase && s->h_latent[(size_t)idx] > 0.0) {
+        // A temperature names the phase of a cell that melts (solid.rs:1352-1360).
+        phi = kelvin > s->h_point[(size_t)idx] ? 1.0 : 0.0;
+        PM_HIP(hipMemcpyAsync(s->melted.d() + idx, &phi, sizeof(double), hipMemcpyHostToDevice, s->stream));
+    }
     PM_HIP(hipStreamSynchronize(s->stream));
     return PM_OK;
 }
+
+int pm_solid3d_set_melted_fraction(pm_solid3d *s, size_t i, size_t j, size_t k, double fraction) {
+    if (!s) return PM_ERR_ARGUMENT;
+    if ((int64_t)i >= s->nx || (int64_t)j >= s->ny || (int64_t)k >= s->nz) return PM_OK;
+    const int64_t idx = (int64_t)i + s->nx * ((int64_t)j + s->ny * (int64_t)k);
+    if (!s->phase || !(s->h_latent[(size_t)idx] > 0.0)) return PM_OK;  // solid.rs:452-454
+    // f64::clamp: NaN stays NaN.
+    const double phi = std::isnan(fraction) ? fraction : std::fmin(std::fmax(fraction, 0.0), 1.0);
+    const double point = s->h_point[(size_t)idx];
+    double t = 0.0;
+    PM_HIP(hipMemcpyAsync(&t, s->T() + idx, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    PM_HIP(hipStreamSynchronize(s->stream));
+    if (phi >= 1.0)
+        t = rmax(t, point);
+    else if (phi <= 0.0)
+        t = std::fmin(t, point);
+    else
+        t = point;
+    PM_HIP(hipMemcpyAsync(s->melted.d() + idx, &phi, sizeof(double), hipMemcpyHostToDevice, s->stream));
+    PM_HIP(hipMemcpyAsync(s->T() + idx, &t, sizeof(double), hipMemcpyHostToDevice, s->stream));
+    PM_HIP(hipStreamSynchronize(s->stream));
+    return PM_OK;
+}
+
+int pm_solid3d_melted(pm_solid3d *s, double *out, double *volume) {
+    if (!s) return PM_ERR_ARGUMENT;
+    std::vector<double> phi((size_t)s->n(), 0.0);
+    if (s->phase) {
+        PM_HIP(hipMemcpyAsync(phi.data(), s->melted.d(), sizeof(double) * phi.size(), hipMemcpyDeviceToHost, s->stream));
+        PM_HIP(hipStreamSynchronize(s->stream));
+    }
+    if (out) std::memcpy(out, phi.data(), sizeof(double) * phi.size());
+    if (volume) {
+        double total = 0.0;  // melted.iter().sum() * cell_volume, solid.rs:476-478
+        for (double v : phi) total += v;
+        *volume = total * (s->dx * s->dx * s->dx);
+    }
+    return PM_OK;
+}
+
+size_t pm_solid3d_melting_cells(const pm_solid3d *s) { return s ? (size_t)s->nlatent : 0; }
 
 int pm_solid3d_deposit(pm_solid3d *s, size_t i, size_t j, size_t k, double joules) {
     if (!s) return PM_ERR_ARGUMENT;
@@ -1877,6 +2136,13 @@ int pm_solid3d_deposit(pm_solid3d *s, size_t i, size_t j, size_t k, double joule
     PM_CHECK(cell_capacity(s, idx, &cap));
     PM_HIP(hipMemcpyAsync(&t, s->T() + idx, sizeof(double), hipMemcpyDeviceToHost, s->stream));
     PM_HIP(hipStreamSynchronize(s->stream));
+    if (s->phase) {
+        deposit_phase<<<1, 1, 0, s->stream>>>(s->T(), idx, joules, s->cap.d(), s->phase_args());
+        PM_HIP(hipGetLastError());
+        PM_HIP(hipStreamSynchronize(s->stream));
+        s->absorbed += joules;
+        return PM_OK;
+    }
     const double rise = joules / cap;
     t += rise;  // add_kelvin, latent <= 0 (solid.rs:1634)
     PM_HIP(hipMemcpyAsync(s->T() + idx, &t, sizeof(double), hipMemcpyHostToDevice, s->stream));
@@ -1913,7 +2179,10 @@ int pm_solid3d_step(pm_solid3d *s, double dt, double gained, pm_violation *v) {
         double total = 0.0;
         PM_CHECK(total_capacity(s, &total));
         const double rise = gained / total;
-        if (s->nvoid)
+        if (s->phase)
+            add_uniform_phase<<<grid_for(s->n(), 256), 256, 0, s->stream>>>(
+                s->T(), s->nvoid ? s->dvoid.as<uint8_t>() : nullptr, s->n(), rise, s->cap.d(), s->phase_args());
+        else if (s->nvoid)
             add_uniform_void<<<grid_for(s->n(), 256), 256, 0, s->stream>>>(s->T(), s->dvoid.as<uint8_t>(), s->n(), rise);
         else
             add_uniform<<<grid_for(s->n(), 256), 256, 0, s->stream>>>(s->T(), s->n(), rise);
@@ -2112,6 +2381,7 @@ int pm_solid3d_empty(pm_solid3d *s, const uint8_t *mask) {
     s->nvoid = count;
     if (!s->dvoid) PM_CHECK(s->dvoid.alloc((size_t)n));
     PM_HIP(hipMemcpyAsync(s->dvoid.d(), s->voids.data(), (size_t)n, hipMemcpyHostToDevice, s->stream));
+    PM_CHECK(remix(s));  // resolve mixes a two-phase block from the fractions it carries now
     PM_CHECK(apply_voids(s));
     PM_CHECK(resolve(s));
     PM_CHECK(find_gaps(s));
@@ -2165,6 +2435,8 @@ int pm_solid3d_checkpoint(pm_solid3d *s) {
     if (!s) return PM_ERR_ARGUMENT;
     if (!s->saved) PM_CHECK(s->saved.alloc(sizeof(double) * (size_t)s->n()));
     PM_HIP(hipMemcpyAsync(s->saved.d(), s->T(), sizeof(double) * (size_t)s->n(), hipMemcpyDeviceToDevice, s->stream));
+    if (s->phase)
+        PM_HIP(hipMemcpyAsync(s->saved_melted.d(), s->melted.d(), sizeof(double) * (size_t)s->n(), hipMemcpyDeviceToDevice, s->stream));
     PM_HIP(hipStreamSynchronize(s->stream));
     s->saved_valid = true;
     s->saved_lost = s->lost;
@@ -2181,6 +2453,9 @@ int pm_solid3d_restore(pm_solid3d *s) {
     } else {
         PM_HIP(hipMemcpyAsync(s->T(), s->saved.d(), sizeof(double) * (size_t)s->n(), hipMemcpyDeviceToDevice, s->stream));
     }
+    // The fractions go back too; the operator is not rebuilt (solid.rs:1945-1950).
+    if (s->phase)
+        PM_HIP(hipMemcpyAsync(s->melted.d(), s->saved_melted.d(), sizeof(double) * (size_t)s->n(), hipMemcpyDeviceToDevice, s->stream));
     PM_HIP(hipStreamSynchronize(s->stream));
     s->lost = s->saved_lost;
     s->supplied = s->saved_supplied;

@@ -41,6 +41,7 @@ class Substance:
     conductivity: Optional[float]
     specific_heat: Optional[float]
     emissivity: float = 0.0
+    fusion: Optional["Fusion"] = None
 
     @staticmethod
     def aluminium_6061() -> "Substance":
@@ -51,9 +52,19 @@ class Substance:
         return Substance("N-BK7", 2.51 * 1e3, 1.114, 858.0, 0.90)    # substance.rs:336-357
 
     @staticmethod
+    def ice() -> "Substance":
+        """substance.rs:613-650: one-phase ice, melting at 0 C with 333.55 kJ/kg."""
+        return Substance("ice", 0.917 * 1e3, 2.22, 2050.0, 0.97,
+                         Fusion(273.15, 333.55 * 1e3))
+
+    @staticmethod
     def of(name: str, density: float, conductivity: float, specific_heat: float,
-           emissivity: float = 0.0) -> "Substance":
-        return Substance(name, density, conductivity, specific_heat, emissivity)
+           emissivity: float = 0.0, fusion: Optional["Fusion"] = None) -> "Substance":
+        return Substance(name, density, conductivity, specific_heat, emissivity, fusion)
+
+    def with_fusion(self, fusion: Optional["Fusion"]) -> "Substance":
+        return Substance(self.name, self.density, self.conductivity, self.specific_heat,
+                         self.emissivity, fusion)
 
     def with_emissivity(self, emissivity: float) -> "Substance":
         """The same substance with another surface finish, clamped to 0..1 (substance.rs:398-403);
@@ -61,7 +72,7 @@ class Substance:
         if self.conductivity is None or self.specific_heat is None:
             return self
         return Substance(self.name, self.density, self.conductivity, self.specific_heat,
-                         min(max(emissivity, 0.0), 1.0))
+                         min(max(emissivity, 0.0), 1.0), self.fusion)
 
     def diffusivity(self) -> Optional[float]:
         """k/(rho*c_p), substance.rs:211-216."""
@@ -76,11 +87,31 @@ class Substance:
         return (self.density * volume) * self.specific_heat
 
     def _c(self) -> _capi.pm_substance:
+        f = self.fusion
+        liquid = f.liquid if f is not None else None
         return _capi.pm_substance(
             self.density,
             math.nan if self.conductivity is None else self.conductivity,
             math.nan if self.specific_heat is None else self.specific_heat,
-            self.emissivity)
+            self.emissivity,
+            math.nan if f is None else f.melting_point,
+            math.nan if f is None else f.latent_heat,
+            math.nan if liquid is None else liquid[0],
+            math.nan if liquid is None else liquid[1])
+
+
+@dataclass(frozen=True)
+class Fusion:
+    """FusionProps (crates/pantometry-core/src/substance.rs:119-193): the melting point in kelvin,
+    the latent heat in J/kg, and the liquid's (conductivity, specific heat) when it differs from
+    the solid's (None: the one-phase model)."""
+
+    melting_point: float
+    latent_heat: float
+    liquid: Optional[Tuple[float, float]] = None
+
+    def with_liquid(self, conductivity: float, specific_heat: float) -> "Fusion":
+        return Fusion(self.melting_point, self.latent_heat, (conductivity, specific_heat))
 
 
 class Face(IntEnum):
@@ -241,6 +272,29 @@ class Solid3D:
         check(lib().pm_solid3d_empty(self._h, mask.ctypes.data_as(C.POINTER(C.c_uint8))),
               "pm_solid3d_empty")
         return self
+
+    def set_melted_fraction(self, i: int, j: int, k: int, fraction: float) -> None:
+        """Solid3D::set_melted_fraction, solid.rs:448-466."""
+        check(lib().pm_solid3d_set_melted_fraction(self._h, int(i), int(j), int(k), float(fraction)),
+              "pm_solid3d_set_melted_fraction")
+
+    def melted_fractions(self) -> np.ndarray:
+        """Every cell's melted fraction, shape (nz, ny, nx)."""
+        nx, ny, nz = self._counts
+        out = np.empty(nx * ny * nz)
+        check(lib().pm_solid3d_melted(self._h, dptr(out), None), "pm_solid3d_melted")
+        return out.reshape(nz, ny, nx)
+
+    def melted_fraction_at(self, i: int, j: int, k: int) -> float:
+        """solid.rs:427-430, clamped."""
+        nx, ny, nz = self._counts
+        return float(self.melted_fractions()[min(k, nz - 1), min(j, ny - 1), min(i, nx - 1)])
+
+    def melted_volume(self) -> float:
+        """solid.rs:476-478, m^3."""
+        v = C.c_double(0.0)
+        check(lib().pm_solid3d_melted(self._h, None, C.byref(v)), "pm_solid3d_melted")
+        return v.value
 
     def is_void(self, i: int, j: int, k: int) -> bool:
         return bool(lib().pm_solid3d_is_void(self._h, int(i), int(j), int(k)))
@@ -434,6 +488,9 @@ class Solid3D:
                 (self._name, "coldest", out[2], "C"), (self._name, "absorbed", out[3], "J")]
         if self.generated_power() != 0.0:
             out_.append((self._name, "generated", self.generated_energy(), "J"))
+        # "melted", only for a block that can melt (solid.rs:2003-2010).
+        if lib().pm_solid3d_melting_cells(self._h) > 0:
+            out_.append((self._name, "melted", self.melted_volume() * 1e9, "mm3"))
         return out_
 
     # ---- device-resident loop ----------------------------------------------------------

@@ -194,6 +194,9 @@ int pm_solid3d_dissipating(pm_solid3d *solid, double watts, const uint8_t *mask)
  * carries a melted fraction per cell and every heat that reaches the cell goes through its
  * enthalpy; a substance with a distinct liquid mixes its conductivity and capacity by that
  * fraction and the block's operator is rebuilt after every step. */
+/* Solid3D::set_temperature (solid.rs:1346-1361) on every cell under `mask`: the region
+ * initial temperature of a scene (crates/pantometry-world/src/lib.rs:2290-2305). */
+int pm_solid3d_set_region(pm_solid3d *solid, const uint8_t *mask, double kelvin);
 /* Solid3D::set_melted_fraction, solid.rs:448-466 (ignored for a cell that does not melt). */
 int pm_solid3d_set_melted_fraction(pm_solid3d *solid, size_t i, size_t j, size_t k, double fraction);
 /* melted_fraction_at (solid.rs:427-430) for every cell into out[n] (zeros for a block that

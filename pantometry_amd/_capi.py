@@ -85,6 +85,7 @@ SIGNATURES = {
     "pm_solid3d_sample": (_I, [_VP, _DP, _S, _DP]),
     "pm_solid3d_empty": (_I, [_VP, C.POINTER(C.c_uint8)]),
     "pm_solid3d_set_melted_fraction": (_I, [_VP, _S, _S, _S, _D]),
+    "pm_solid3d_set_region": (_I, [_VP, C.POINTER(C.c_uint8), _D]),
     "pm_solid3d_melted": (_I, [_VP, _DP, _DP]),
     "pm_solid3d_melting_cells": (_S, [_VP]),
     "pm_solid3d_void_cells": (_S, [_VP]),

@@ -802,6 +802,17 @@ __global__ void mix_phases(const double *melted, const double *ks, const double 
     }
 }
 
+// set_temperature over the cells of a mask (solid.rs:1346-1361), phase included.
+__global__ void set_region(double *T, const uint8_t *mask, int64_t n, double kelvin,
+                           const double *point, const double *latent, double *melted) {
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        if (!mask[c]) continue;
+        T[c] = kelvin;
+        if (melted && latent[c] > 0.0) melted[c] = kelvin > point[c] ? 1.0 : 0.0;
+    }
+}
+
 // Naming a temperature names the phase of a cell that melts (solid.rs:1352-1360).
 __global__ void phase_from_temperature(const double *T, const double *point, const double *latent,
                                        int64_t n, double *melted) {
@@ -2083,6 +2094,19 @@ int pm_solid3d_set_temperature(pm_solid3d *s, size_t i, size_t j, size_t k, doub
         phi = kelvin > s->h_point[(size_t)idx] ? 1.0 : 0.0;
         PM_HIP(hipMemcpyAsync(s->melted.d() + idx, &phi, sizeof(double), hipMemcpyHostToDevice, s->stream));
     }
+    PM_HIP(hipStreamSynchronize(s->stream));
+    return PM_OK;
+}
+
+int pm_solid3d_set_region(pm_solid3d *s, const uint8_t *mask, double kelvin) {
+    if (!s || !mask) return PM_ERR_ARGUMENT;
+    DevBuf dm;
+    PM_CHECK(dm.alloc((size_t)s->n()));
+    PM_HIP(hipMemcpyAsync(dm.d(), mask, (size_t)s->n(), hipMemcpyHostToDevice, s->stream));
+    set_region<<<grid_for(s->n(), 256), 256, 0, s->stream>>>(
+        s->T(), dm.as<uint8_t>(), s->n(), kelvin, s->phase ? s->ph_point.d() : nullptr,
+        s->phase ? s->ph_latent.d() : nullptr, s->phase ? s->melted.d() : nullptr);
+    PM_HIP(hipGetLastError());
     PM_HIP(hipStreamSynchronize(s->stream));
     return PM_OK;
 }

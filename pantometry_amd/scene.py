@@ -2,17 +2,20 @@
 
 A scene is the reference's JSON format (crates/pantometry-world/src/lib.rs:55-130, `Scene`):
 a title, a coupling schedule, a duration cut into frames, a conservation tolerance, declared
-materials and a list of domains.  Two kinds of domain are built here, on the GPU:
+materials and a list of domains.  These kinds of domain are built here, the grids on the GPU:
 
   block   Solid3D (DomainSpec::Block, lib.rs:2185-2400): cells, cell size, starting temperature,
-          bulk material, regions of other materials (later wins) with their own starting
-          temperature, a hot spot, cooled faces (CoolingSpec) and dissipating boxes
-          (DissipationSpec)
+          bulk material, regions of other materials or of void (later wins) with their own
+          starting temperature, a hot spot, cooled faces (CoolingSpec) and dissipating boxes
+          (DissipationSpec); materials may melt (FusionProps)
   hall    Hall (DomainSpec::Hall, lib.rs:1786-1805): an air-filled room released in a mode
+  room    Room (DomainSpec::Room, lib.rs:1405-1437): released in a mode or as a Gaussian pulse
+  bar     Bar1D of aluminium (DomainSpec::Bar, lib.rs:1682-1701)
+  heater  Heater (crates/pantometry-world/src/heater.rs): a reserve of joules spent at a fixed
+          power onto the bus, the heat every thermal domain takes its share of
 
-Every other kind of domain, and the block features outside this path (designed parts from STL,
-void regions, phase-changing materials), is refused with a message that says so rather than
-built approximately.  A run (World::run, lib.rs:3204-3218) advances the simulation by
+Every other kind of domain, and the features outside this path (designed parts from STL,
+composite materials), is refused with a message that says so rather than built approximately.  A run (World::run, lib.rs:3204-3218) advances the simulation by
 duration/frames per frame and captures each domain's readings and its field sampled over its
 extent (scene::capture, crates/pantometry-scene/src/lib.rs:337-404); `to_json` writes the frames
 in the viewer's run-file format (crates/pantometry-view/src/data.rs:55-171).
@@ -27,9 +30,9 @@ from typing import Dict, List, Optional, Tuple, Union
 
 import numpy as np
 
-from .acoustic import Hall
-from .core import Schedule, Simulation, Tolerances
-from .thermal import Environment, Face, Solid3D, Substance, celsius, mm
+from .acoustic import Hall, Room
+from .core import ENERGY, HEAT, Exchange, Kind, Ledger, Schedule, Simulation, Tolerances
+from .thermal import Bar1D, Environment, Face, Fusion, Solid3D, Substance, celsius, mm
 
 FORMAT = 1  # lib.rs:369, the highest scene revision this reads
 
@@ -55,11 +58,17 @@ _FACES = {"x-min": Face.XMIN, "x-max": Face.XMAX, "y-min": Face.YMIN, "y-max": F
 _SCHEDULES = {"one-way": Schedule.ONE_WAY, "staggered": Schedule.STAGGERED,
               "multirate": Schedule.MULTIRATE}
 _SCENE_KEYS = {"format", "title", "schedule", "domains", "duration_s", "frames",
-               "conservation_tolerance", "tolerance_for", "materials"}
+               "conservation_tolerance", "tolerance_for", "materials", "composites"}
 _BLOCK_KEYS = {"kind", "name", "cells", "cell_mm", "initial_c", "material", "regions",
                "hot_spot", "parts", "cooling", "dissipation"}
 _HALL_KEYS = {"kind", "name", "width_m", "height_m", "depth_m", "nodes_across", "mode",
               "amplitude_pa"}
+_ROOM_KEYS = {"kind", "name", "width_m", "height_m", "cells_across", "release"}
+_RELEASE_KEYS = {"mode": {"as", "nx", "ny", "amplitude_pa"},
+                 "pulse": {"as", "x_m", "y_m", "radius_m", "amplitude_pa"}}
+_BAR_KEYS = {"kind", "name", "length_mm", "cells", "area_mm2", "initial_c", "exposes"}
+_HEATER_KEYS = {"kind", "name", "watts", "reserve_j"}
+_FUSION_KEYS = {"melting_point", "latent_heat", "liquid"}
 
 
 class SceneError(ValueError):
@@ -82,6 +91,7 @@ class Scene:
     tolerance_for: Dict[str, float]
     materials: Dict[str, dict]
     domains: List[dict]
+    composites: Dict[str, dict] = field(default_factory=dict)
 
 
 def load(source: Union[str, Path, dict]) -> Scene:
@@ -114,7 +124,8 @@ def load(source: Union[str, Path, dict]) -> Scene:
         raise SceneError("a scene needs at least one frame")
     return Scene(raw["title"], _SCHEDULES[schedule], duration, frames,
                  float(raw.get("conservation_tolerance", 1e-6)),
-                 dict(raw.get("tolerance_for", {})), dict(raw.get("materials", {})), domains)
+                 dict(raw.get("tolerance_for", {})), dict(raw.get("materials", {})), domains,
+                 dict(raw.get("composites", {})))
 
 
 class Palette:
@@ -131,15 +142,16 @@ class Palette:
             density = float(spec.get("density", math.nan))
             if not (density > 0.0 and math.isfinite(density)):
                 raise SceneError(f"materials.{key}: density is {density}")
+            fusion = _fusion(f"materials.{key}", spec.get("fusion"))
             if thermal is None:
-                sub = Substance(spec.get("name", key), density, None, None, 0.0)
+                sub = Substance(spec.get("name", key), density, None, None, 0.0, fusion)
             else:
                 k, cp = float(thermal["conductivity"]), float(thermal["specific_heat"])
                 if not (k >= 0.0 and cp > 0.0):
                     raise SceneError(f"materials.{key}: conductivity {k}, specific heat {cp}")
                 sub = Substance(spec.get("name", key), density, k, cp,
-                                min(max(float(thermal.get("emissivity", 0.0)), 0.0), 1.0))
-            self.declared[key] = (sub, spec.get("fusion") is not None)
+                                min(max(float(thermal.get("emissivity", 0.0)), 0.0), 1.0), fusion)
+            self.declared[key] = (sub, fusion is not None)
 
     def get(self, site: str, name: str) -> Substance:
         self.asked.add(name)
@@ -147,20 +159,35 @@ class Palette:
             sub, freezes = self.declared[name]
         elif name in _CATALOGUE:
             label, dens, k, cp, eps, freezes = _CATALOGUE[name]
-            sub = Substance(label, dens * 1e3, k, cp, eps)
+            sub = Substance.ice() if freezes else Substance(label, dens * 1e3, k, cp, eps)
         else:
             raise SceneError(f"{site}: no material {name!r}; the catalogue has "
                              f"{', '.join(CATALOGUE)} and the scene declares "
                              f"{', '.join(self.declared) or 'none'}")
-        if freezes:
-            raise SceneError(f"{site}: {name!r} changes phase, and enthalpy phase change is not "
-                             f"on this backend's path")
         if sub.conductivity is None or sub.specific_heat is None:
             raise SceneError(f"{site}: {name!r} has no thermal properties")
         return sub
 
     def unused(self) -> List[str]:
         return sorted(k for k in self.declared if k not in self.asked)
+
+
+def _fusion(site: str, spec: Optional[dict]) -> Optional[Fusion]:
+    """FusionProps as a scene declares it (substance.rs:132-154): a melting point in kelvin, a
+    latent heat in J/kg, and optionally the liquid's ThermalProps."""
+    if spec is None:
+        return None
+    _unknown(f"{site}.fusion", spec, _FUSION_KEYS)
+    point, latent = float(spec["melting_point"]), float(spec["latent_heat"])
+    if not (point > 0.0 and math.isfinite(point) and latent > 0.0 and math.isfinite(latent)):
+        raise SceneError(f"{site}.fusion: melting point {point} K, latent heat {latent} J/kg")
+    liquid = spec.get("liquid")
+    if liquid is None:
+        return Fusion(point, latent)
+    kl, cl = float(liquid["conductivity"]), float(liquid["specific_heat"])
+    if not (kl >= 0.0 and cl > 0.0):
+        raise SceneError(f"{site}.fusion.liquid: conductivity {kl}, specific heat {cl}")
+    return Fusion(point, latent, (kl, cl))
 
 
 def _box(counts, lo, hi) -> np.ndarray:
@@ -196,15 +223,18 @@ def build_block(spec: dict, palette: Palette) -> Solid3D:
         _selects(site, r, cells)
         inside = _box(cells, r["from"], r["to"])
         if r["material"] == VOID:
-            raise SceneError(f"{site}: void cells are not on this backend's path")
+            # A box of nothing: an air gap without a mesh (lib.rs:2276-2285).
+            if r.get("initial_c") is not None:
+                raise SceneError(f"{site}: a \"void\" region cannot start at a temperature, "
+                                 f"because there is nothing there to have one")
+            block.empty(inside)
+            continue
         block.fill(palette.get(site, r["material"]), inside)
         if r.get("initial_c") is not None:
             c = float(r["initial_c"])
             if not math.isfinite(c):
                 raise SceneError(f"{site}: initial_c is {c}")
-            t = block.cells().reshape(-1).copy()
-            t[inside] = celsius(c)
-            block.set_cells(t)
+            block.set_region_temperature(inside, celsius(c))
     seen = set()
     for n, cool in enumerate(spec.get("cooling", [])):
         site = f"{name}/cooling[{n}]"
@@ -244,6 +274,93 @@ def build_hall(spec: dict) -> Hall:
                        float(spec["depth_m"]), int(spec["nodes_across"]))
     m = spec["mode"]
     return hall.released_in_mode((int(m[0]), int(m[1]), int(m[2])), float(spec["amplitude_pa"]))
+
+
+def build_room(spec: dict) -> Room:
+    """DomainSpec::Room (lib.rs:1405-1437): Room::of_air released in a mode or as a pulse
+    a*exp(-((x-x0)^2 + (y-y0)^2)/r^2), r at least 1e-9."""
+    name = spec["name"]
+    _unknown(name, spec, _ROOM_KEYS)
+    release = spec.get("release") or {"as": "mode", "nx": 1, "ny": 1, "amplitude_pa": 1.0}
+    kind = release.get("as")
+    if kind not in _RELEASE_KEYS:
+        raise SceneError(f"{name}.release: {kind!r} is not one of mode, pulse")
+    _unknown(f"{name}.release", release, _RELEASE_KEYS[kind])
+    room = Room.of_air(name, float(spec["width_m"]), float(spec["height_m"]),
+                       int(spec["cells_across"]))
+    if kind == "mode":
+        return room.released_in_mode((int(release["nx"]), int(release["ny"])),
+                                     float(release["amplitude_pa"]))
+    cx, cy = float(release["x_m"]), float(release["y_m"])
+    r, a = max(float(release["radius_m"]), 1e-9), float(release["amplitude_pa"])
+
+    def pulse(x, y):
+        dx, dy = x - cx, y - cy
+        return a * math.exp(-(dx * dx + dy * dy) / (r * r))
+
+    return room.released_from(pulse)
+
+
+def build_bar(spec: dict) -> Bar1D:
+    """DomainSpec::Bar (lib.rs:1682-1701): aluminium, length_mm/cells per cell."""
+    name = spec["name"]
+    _unknown(name, spec, _BAR_KEYS)
+    cells = int(spec["cells"])
+    bar = Bar1D(name, Substance.aluminium_6061(), cells, mm(float(spec["length_mm"]) / float(cells)),
+                float(spec["area_mm2"]) * 1e-6, celsius(float(spec["initial_c"])))
+    exposes = spec.get("exposes")
+    if exposes is not None:
+        bar.exposing(float(exposes["face_area_mm2"]) * 1e-6)
+    return bar
+
+
+class Heater:
+    """Heater (crates/pantometry-world/src/heater.rs:17-104): a reserve of joules spent at a
+    fixed power, published on the bus as heat.  Its ledger is the reserve."""
+
+    def __init__(self, name: str, watts: float, reserve: float):
+        self._name = name
+        self.watts = max(watts, 0.0)
+        self.reserve = max(reserve, 0.0)
+        self._saved: Optional[float] = None
+
+    def name(self) -> str:
+        return self._name
+
+    def kind(self) -> Kind:
+        return Kind.QUASI_STATIC
+
+    def max_stable_dt(self, now: float = 0.0) -> float:
+        return math.inf
+
+    def books_balance(self) -> bool:
+        return False  # the Domain default: a heater makes no claim of its own (sim.rs:152-154)
+
+    def step(self, t: float, dt: float, bus: Exchange) -> None:
+        joules = max(min(self.watts * dt, self.reserve), 0.0)
+        self.reserve -= joules
+        bus.publish(HEAT, joules)
+
+    def ledger(self) -> Ledger:
+        return Ledger().with_(ENERGY, self.reserve)
+
+    def checkpoint(self) -> None:
+        self._saved = self.reserve
+
+    def restore(self) -> None:
+        if self._saved is not None:
+            self.reserve = self._saved
+
+    def supports_restore(self) -> bool:
+        return True
+
+    def readings(self):
+        return [(self._name, "reserve", self.reserve, "J")]
+
+
+def build_heater(spec: dict) -> Heater:
+    _unknown(spec["name"], spec, _HEATER_KEYS)
+    return Heater(spec["name"], float(spec["watts"]), float(spec["reserve_j"]))
 
 
 @dataclass
@@ -295,10 +412,28 @@ class World:
         self.sim = Simulation(scene.schedule)
         self.sim.conservation_tol = tol
         palette = Palette(scene.materials)
+        if scene.composites:
+            raise SceneError(f"composites: {', '.join(sorted(scene.composites))} mix substances "
+                             f"by volume (crates/pantometry-core/src/mixture.rs), which is not on "
+                             f"this backend's path")
         self.extent: Dict[str, Tuple[Tuple[float, float, float], Tuple[int, int, int]]] = {}
         for spec in scene.domains:
             kind = spec.get("kind")
-            if kind == "block":
+            if kind == "room":
+                d = build_room(spec)
+                # Placement::field(Extent::rectangle(width, height, nx, ny)), lib.rs:3402-3415
+                nx = max(int(spec["cells_across"]), 3)
+                w, h = float(spec["width_m"]), float(spec["height_m"])
+                ny = max(int(_round_half_away((h / w) * float(nx - 1))) + 1, 3)
+                self.extent[d.name()] = ((w, h, 0.0), (nx, ny, 1))
+            elif kind == "bar":
+                d = build_bar(spec)
+                # Placement::field(Extent::line(length, cells)), lib.rs:3416-3418
+                self.extent[d.name()] = ((float(spec["length_mm"]) * 1e-3, 0.0, 0.0),
+                                         (max(int(spec["cells"]), 2), 1, 1))
+            elif kind == "heater":
+                d = build_heater(spec)
+            elif kind == "block":
                 d = build_block(spec, palette)
                 cells = tuple(int(c) for c in spec["cells"])
                 size = tuple(cells[a] * float(spec["cell_mm"]) * 1e-3 for a in range(3))
@@ -312,7 +447,7 @@ class World:
                 self.extent[d.name()] = (size, (nx, ny, nz))
             else:
                 raise SceneError(f"{spec.get('name', '?')}: a {kind!r} domain is not on this "
-                                 f"backend's path (block and hall are)")
+                                 f"backend's path (block, hall, room, bar and heater are)")
             self.sim.with_(d)
         unused = palette.unused()
         if unused:
@@ -328,7 +463,7 @@ class World:
             frame.readings.extend(d.readings())
             if d.name() in self._pts:
                 (size, (nx, ny, nz)) = self.extent[d.name()]
-                unit = "Pa" if isinstance(d, Hall) else "K"
+                unit = "Pa" if isinstance(d, (Hall, Room)) else "K"
                 frame.panels.append(Panel(d.name(), unit, nx, ny, nz, d.sample(self._pts[d.name()])))
         return frame
 
@@ -343,6 +478,11 @@ class World:
             self.advance(dt)
             frames.append(self.capture())
         return frames
+
+
+def _round_half_away(x: float) -> float:
+    """f64::round: halves away from zero."""
+    return math.floor(abs(x) + 0.5) * (1.0 if x >= 0.0 else -1.0) if abs(x) < 2.0 ** 52 else x
 
 
 def build(source: Union[str, Path, dict, Scene]) -> World:

@@ -273,6 +273,12 @@ class Solid3D:
               "pm_solid3d_empty")
         return self
 
+    def set_region_temperature(self, which, t: float) -> None:
+        """set_temperature on every selected cell (solid.rs:1346-1361), phase included."""
+        mask = self._mask(which)
+        check(lib().pm_solid3d_set_region(self._h, mask.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                          float(t)), "pm_solid3d_set_region")
+
     def set_melted_fraction(self, i: int, j: int, k: int, fraction: float) -> None:
         """Solid3D::set_melted_fraction, solid.rs:448-466."""
         check(lib().pm_solid3d_set_melted_fraction(self._h, int(i), int(j), int(k), float(fraction)),

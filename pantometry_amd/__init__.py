@@ -10,7 +10,7 @@ from .core import (ENERGY, HEAT, Exchange, Kind, Ledger, Report, Schedule, Simul
                    Tolerances, Violation, audit, audit_with, substeps_for)
 from .ensemble import Ensemble, Member
 from .rng import Rng
-from .thermal import STABLE_FOURIER_3D, Bar1D, Environment, Face, Fusion, Solid3D, Substance, celsius, mm
+from .thermal import STABLE_FOURIER_3D, Bar1D, Environment, Face, Fusion, GapPatch, Solid3D, Substance, celsius, mm
 
 __all__ = [
     "DeviceError", "LIB_PATH", "lib", "Hall", "Room", "Tube", "End", "Bar1D", "ENERGY", "HEAT", "Exchange", "Kind", "Ledger",

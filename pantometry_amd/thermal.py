@@ -10,7 +10,7 @@ import ctypes as C
 import math
 from dataclasses import dataclass
 from enum import IntEnum
-from typing import Callable, Optional, Tuple, Union
+from typing import Callable, List, Optional, Tuple, Union
 
 import numpy as np
 
@@ -112,6 +112,85 @@ class Fusion:
 
     def with_liquid(self, conductivity: float, specific_heat: float) -> "Fusion":
         return Fusion(self.melting_point, self.latent_heat, (conductivity, specific_heat))
+
+
+@dataclass(frozen=True)
+class GapPatch:
+    """One patch of a clearance (solid.rs:289-310): how many facing pairs, the bounding span of
+    the facing cells across the gap (m), the separation of the two surfaces (m), and whether
+    the facing cells fill their bounding box."""
+
+    pairs: int
+    span: Tuple[float, float]
+    distance: float
+    rectangular: bool
+
+    def view_factor(self) -> float:
+        """F12 of two equal, parallel, directly opposed rectangles (solid.rs:312-367), with the
+        small-argument series below X = Y = 0.01 and the form's own limits at the ends."""
+        if not math.isfinite(self.distance) or math.isnan(self.span[0]) or math.isnan(self.span[1]):
+            return 0.0
+        if self.distance <= 0.0:
+            return 1.0 if self.span[0] > 0.0 and self.span[1] > 0.0 else 0.0
+        x, y = self.span[0] / self.distance, self.span[1] / self.distance
+        if x <= 0.0 or y <= 0.0:
+            return 0.0
+        x2, y2 = x * x, y * y
+        if x < 0.01 and y < 0.01:
+            return min(max(x * y / math.pi * (1.0 - (x2 + y2) / 3.0), 0.0), 1.0)
+        rx, ry = math.sqrt(1.0 + x2), math.sqrt(1.0 + y2)
+        t = (math.log(math.sqrt((1.0 + x2) * (1.0 + y2) / (1.0 + x2 + y2)))
+             + x * ry * math.atan(x / ry) + y * rx * math.atan(y / rx)
+             - x * math.atan(x) - y * math.atan(y))
+        return min(max(2.0 / (math.pi * x * y) * t, 0.0), 1.0)
+
+
+def gap_patches_of(pairs, counts: Tuple[int, int, int], dx: float) -> List[GapPatch]:
+    """Solid3D::gap_patches (solid.rs:857-940): the gap pairs grouped by axis and separation,
+    then into four-connected sheets of facing cells."""
+    nx, ny, _ = counts
+    strides = ((0, 1), (1, nx), (2, nx * ny))
+    sheets: dict = {}
+    for a, b, _coef in pairs:
+        delta = b - a
+        found = None
+        for axis, stride in strides:
+            limit_ok = delta < nx if axis == 0 else (delta < nx * ny if axis == 1 else True)
+            if delta % stride == 0 and delta // stride > 1 and limit_ok:
+                found = (axis, stride)
+                break
+        if found is None:
+            continue
+        axis, stride = found
+        cells = delta // stride - 1
+        i, j, k = a % nx, (a // nx) % ny, a // (nx * ny)
+        across = (j, k) if axis == 0 else ((i, k) if axis == 1 else (i, j))
+        sheets.setdefault((axis, cells), []).append(across)
+    out = []
+    for (axis, cells) in sorted(sheets):
+        face = sorted(sheets[(axis, cells)])
+        index = {p: n for n, p in enumerate(face)}
+        seen = [False] * len(face)
+        for start in range(len(face)):
+            if seen[start]:
+                continue
+            stack, members = [start], []
+            seen[start] = True
+            while stack:
+                n = stack.pop()
+                u, v = face[n]
+                members.append((u, v))
+                for du, dv in ((1, 0), (-1, 0), (0, 1), (0, -1)):
+                    m = index.get((u + du, v + dv))
+                    if m is not None and not seen[m]:
+                        seen[m] = True
+                        stack.append(m)
+            u0, u1 = min(m[0] for m in members), max(m[0] for m in members)
+            v0, v1 = min(m[1] for m in members), max(m[1] for m in members)
+            wu, wv = u1 + 1 - u0, v1 + 1 - v0
+            out.append(GapPatch(len(members), (wu * dx, wv * dx), cells * dx,
+                                len(members) == wu * wv))
+    return out
 
 
 class Face(IntEnum):
@@ -316,6 +395,10 @@ class Solid3D:
         lib().pm_solid3d_gap_pairs(self._h, a.ctypes.data_as(C.POINTER(C.c_int64)),
                                    b.ctypes.data_as(C.POINTER(C.c_int64)), dptr(coef), n)
         return list(zip(a.tolist(), b.tolist(), coef.tolist()))
+
+    def gap_patches(self) -> List[GapPatch]:
+        """Solid3D::gap_patches (solid.rs:857-940) over the block's radiating pairs."""
+        return gap_patches_of(self.gap_pairs(), self._counts, self._dx)
 
     def _mask(self, which) -> np.ndarray:
         nx, ny, nz = self._counts

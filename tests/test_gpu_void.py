@@ -228,3 +228,18 @@ def test_the_void_tests_on_the_device():
     at = gap_bar.sample([[mm * 1e-3, 5e-3, 5e-3] for mm in (5.0, 35.0, 15.0, 20.0, 25.0, 9.0, 11.0)])
     assert abs(at[0] - (300.0 + C0)) < 1e-12 and abs(at[1] - (20.0 + C0)) < 1e-12
     assert all(math.isnan(v) for v in at[2:5]) and math.isfinite(at[5]) and math.isnan(at[6])
+
+
+def test_gap_patches_on_the_device():
+    # how_much_of_the_gap_is_an_assumption.rs:93-128: the housing's one 4x4 patch, two cells
+    # apart, from the device's pairs.
+    gpu_case("gap patches")
+    import pantometry_amd as pm
+    b = pm.Solid3D("housing", sub(COPPER), (6, 6, 7), 8e-3, 20.0 + C0)
+    b.empty(lambda i, j, k: k < 3 and (i in (0, 5) or j in (0, 5)))
+    b.empty(lambda i, j, k: k in (3, 4))
+    patches = b.gap_patches()
+    assert len(patches) == 1
+    p = patches[0]
+    assert p.pairs == 16 and p.rectangular and abs(p.distance - 0.016) < 1e-12
+    assert abs(p.view_factor() - 0.41525) < 5e-5

@@ -266,3 +266,56 @@ def test_the_gaps_and_their_exchange_to_the_bit():
         cells[a] += -joules / b.cap[a]
         cells[c] += joules / b.cap[c]
     assert np.array_equal(b.cells, np.array(cells))
+
+
+def test_the_view_factor_is_the_parallel_rectangle_form():
+    # how_much_of_the_gap_is_an_assumption.rs:16-90.
+    from pantometry_amd.thermal import GapPatch
+
+    def patch(span, distance):
+        return GapPatch(1, (span, span), distance, True)
+
+    assert abs(patch(1.0, 1.0).view_factor() - 0.19982) < 5e-6
+    previous = 0.0
+    for n in range(1, 7):
+        f = patch(10.0 ** n, 1.0).view_factor()
+        assert f > previous
+        previous = f
+    assert previous > 0.999
+    worst = 0.0
+    for c in (10.0, 30.0, 100.0, 300.0, 1000.0, 10000.0, 1e6):
+        off = abs(patch(1.0, c).view_factor() / (1.0 / (math.pi * c * c)) - 1.0)
+        assert off <= 2.0 / (3.0 * c * c) * 1.05 + 1e-9
+        if c >= 300.0:
+            worst = max(worst, off)
+    assert worst < 1e-5
+    assert patch(0.0, 1.0).view_factor() == 0.0 and patch(1.0, 0.0).view_factor() == 1.0
+    assert math.isfinite(patch(1.0, math.nan).view_factor())
+    assert math.isfinite(patch(math.nan, 1.0).view_factor())
+
+
+def test_a_patch_is_a_sheet_and_clearances_stay_separate():
+    # how_much_of_the_gap_is_an_assumption.rs:93-180, over the oracle's pairs.
+    from pantometry_amd.thermal import gap_patches_of
+    counts = (6, 6, 7)
+    b = block(counts, 8e-3, 20.0)
+    b.empty(mask_of(counts, lambda i, j, k: k < 3 and (i in (0, 5) or j in (0, 5))))
+    b.empty(mask_of(counts, lambda i, j, k: k in (3, 4)))
+    pairs = list(zip(b.gap_a.tolist(), b.gap_b.tolist(), b.gap_coef.tolist()))
+    patches = gap_patches_of(pairs, counts, 8e-3)
+    assert len(patches) == 1
+    p = patches[0]
+    assert p.pairs == 16 and p.rectangular
+    assert abs(p.span[0] - 0.032) < 1e-12 and abs(p.span[1] - 0.032) < 1e-12
+    assert abs(p.distance - 0.016) < 1e-12
+    assert abs(p.view_factor() - 0.41525) < 5e-5
+    two = block((5, 1, 3), 5e-3, 20.0).empty(mask_of((5, 1, 3), lambda i, j, k: k == 1 and i in (0, 4)))
+    ps = sorted(gap_patches_of(list(zip(two.gap_a.tolist(), two.gap_b.tolist(),
+                                        two.gap_coef.tolist())), (5, 1, 3), 5e-3),
+                key=lambda q: q.pairs)
+    assert len(ps) == 2
+    for q in ps:
+        assert q.pairs == 1 and abs(q.distance - 0.005) < 1e-12
+        assert abs(q.view_factor() - 0.19982) < 5e-6
+    solid = block((4, 4, 4), 5e-3, 20.0)
+    assert gap_patches_of([], (4, 4, 4), 5e-3) == [] and len(solid.gap_a) == 0

@@ -60,6 +60,9 @@ enum { C_UNIFORM = 0, C_PER_CELL = 1 };
 // neighbours, whose rows it reads as halo, run on the same XCD and are found in its L2.
 constexpr int XCDS = 8;
 
+// Global (HBM) address space, for pointers whose provenance an empty asm has hidden.
+#define GLOBAL __attribute__((address_space(1)))
+
 // The faces of one plane from the conductivities staged in LDS: each thread builds its own low x
 // and y faces; the first TY threads add the tile's right-edge x faces and the last row of threads
 // the top-edge y faces.  Face g along an axis of n cells lies between cells g-1 and g and is zero
@@ -70,6 +73,13 @@ __device__ __forceinline__ double series_nz(double a, double b) {
     const double sum = a + b;
     const double q = 2.0 * a * b / sum;
     return sum > 0.0 ? q : 0.0;
+}
+
+// series_nz(a, b) for a face that exists (`on`), else 0, with one select for both guards.
+__device__ __forceinline__ double series_on(double a, double b, bool on) {
+    const double sum = a + b;
+    const double q = 2.0 * a * b / sum;
+    return (on && sum > 0.0) ? q : 0.0;
 }
 
 template <int TY>
@@ -316,7 +326,7 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
                   "roles do not fit");
     __shared__ double sT[2][E2Y][E2X];
     __shared__ double sK[2][E2Y][E2X];
-    __shared__ double sF[3][FXS + FYS];
+    __shared__ double sF[4][FXS + FYS];
     __shared__ double s1[2][E1Y][E1X];
     __shared__ double red[NT / 64];
 
@@ -328,9 +338,11 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
     const int64_t bx = tile % a.gx, by = (tile / a.gx) % a.gy, bz = tile / ((int64_t)a.gx * a.gy);
 
     const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TX * ty;
-    const int64_t nx = a.nx, ny = a.ny, nz = a.nz, plane = nx * ny;
+    const int64_t nx = a.nx, ny = a.ny, plane = nx * ny;
     const int64_t x0 = bx * TX, y0 = by * TY;
-    const int64_t z0 = bz * a.zc, z1 = std::min<int64_t>(z0 + a.zc, nz);
+    // Plane indices are uniform 32-bit values, so their clamps and tests stay on the scalar unit.
+    const int nz = a.nz;
+    const int z0 = (int)bz * a.zc, z1 = std::min(z0 + a.zc, nz);
     auto colidx = [&](int64_t x, int64_t y) -> uint32_t {
         return (uint32_t)(std::min(std::max<int64_t>(x, 0), nx - 1) +
                           nx * std::min(std::max<int64_t>(y, 0), ny - 1));
@@ -367,6 +379,9 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
     }
     const bool ring2 = r2r >= 0;
     const uint32_t r2col = ring2 ? colidx(x0 - 2 + r2c, y0 - 2 + r2r) : 0u;
+    // Byte offsets within a plane: they fit 32 bits (the paired sweep takes planes of fewer than
+    // 2^29 cells), so each access is a uniform plane address plus one 32-bit lane offset.
+    const uint32_t ub = ucol * 8u, r1b = r1col * 8u, r2b = r2col * 8u;
 
     // This thread's in-plane faces, one per round (the third only for the first F2 threads),
     // packed as: bits 0-11 the first conductivity's offset in a k plane on E2, bits 12-23 the
@@ -407,11 +422,15 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
     const double *__restrict__ T = a.told;
     const double *__restrict__ K = a.kcell;
     // The chunk reads planes z0-2 .. z1+1; later planes are clamped to what it already reads.
-    const int64_t top = std::min(z1 + 1, nz - 1);
-    auto at = [&](const double *p, int64_t q) -> const double * {
-        return p + plane * std::min(std::max<int64_t>(q, 0), top);
+    const int top = std::min(z1 + 1, nz - 1);
+    // The empty asm pins the plane address to scalar registers, so the load is addressed as
+    // scalar base + 32-bit lane offset with no per-lane 64-bit add.
+    auto ld = [&](const double *p, int q, uint32_t off) -> double {
+        auto base = (const GLOBAL char *)(p + plane * std::min(std::max(q, 0), top));
+        asm("" : "+s"(base));
+        return *(const GLOBAL double *)(base + off);
     };
-    auto zon = [&](int64_t q) -> bool { return q > 0 && q < nz; };  // face between q-1 and q
+    auto zon = [&](int q) -> bool { return q > 0 && q < nz; };  // face between q-1 and q
     // One explicit update (solid.rs:1781-1826 order) from the in-plane neighbours in `t` around
     // position p (row stride `stride`), the four in-plane faces and the own column.
     auto update = [&](const double *t, int stride, int p, double kxl, double kxh, double kyl,
@@ -433,20 +452,20 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
     const int r1fx = ring1 ? r1r * (E1X + 1) + r1c : 0, r1fy = ring1 ? FXS + r1r * E1X + r1c : 0;
 
     // Rings, indexed by (plane - zs) mod 4.
-    const int64_t zs = z0 - 1;
+    const int zs = z0 - 1;
     double RT[4], RK[4], KZ[4], U1[4];  // tile column: T0, k, z face below, T1
     // Ring-1 column: T0, k and z faces.  A ring-2 thread, never a ring-1 thread, keeps the T0(z)
     // and k(z+1) of its next staging in QT[0..1] and QK[0..1] (indexed by plane parity).
     double QT[4], QK[4], QZ[4];
-    RT[3] = at(T, zs - 1)[ucol];
-    RT[0] = at(T, zs)[ucol];
-    RT[1] = at(T, zs + 1)[ucol];
-    RT[2] = at(T, zs + 2)[ucol];
-    const double k_m = at(K, zs - 1)[ucol];
-    RK[0] = at(K, zs)[ucol];
-    RK[1] = at(K, zs + 1)[ucol];
-    RK[2] = at(K, zs + 2)[ucol];
-    RK[3] = at(K, zs + 3)[ucol];
+    RT[3] = ld(T, zs - 1, ub);
+    RT[0] = ld(T, zs, ub);
+    RT[1] = ld(T, zs + 1, ub);
+    RT[2] = ld(T, zs + 2, ub);
+    const double k_m = ld(K, zs - 1, ub);
+    RK[0] = ld(K, zs, ub);
+    RK[1] = ld(K, zs + 1, ub);
+    RK[2] = ld(K, zs + 2, ub);
+    RK[3] = ld(K, zs + 3, ub);
     {
         const double lo = series_nz(k_m, RK[0]), hi = series_nz(RK[0], RK[1]);
         KZ[0] = zon(zs) ? lo : 0.0;
@@ -459,25 +478,25 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
     QZ[0] = QZ[1] = QZ[2] = QZ[3] = 0.0;
     double hk0 = 0.0;
     if (ring1) {
-        QT[3] = at(T, zs - 1)[r1col];
-        QT[0] = at(T, zs)[r1col];
-        QT[1] = at(T, zs + 1)[r1col];
-        QT[2] = at(T, zs + 2)[r1col];
-        const double q_m = at(K, zs - 1)[r1col];
-        QK[0] = at(K, zs)[r1col];
-        QK[1] = at(K, zs + 1)[r1col];
-        QK[2] = at(K, zs + 2)[r1col];
-        QK[3] = at(K, zs + 3)[r1col];
+        QT[3] = ld(T, zs - 1, r1b);
+        QT[0] = ld(T, zs, r1b);
+        QT[1] = ld(T, zs + 1, r1b);
+        QT[2] = ld(T, zs + 2, r1b);
+        const double q_m = ld(K, zs - 1, r1b);
+        QK[0] = ld(K, zs, r1b);
+        QK[1] = ld(K, zs + 1, r1b);
+        QK[2] = ld(K, zs + 2, r1b);
+        QK[3] = ld(K, zs + 3, r1b);
         const double lo = series_nz(q_m, QK[0]), hi = series_nz(QK[0], QK[1]);
         QZ[0] = zon(zs) ? lo : 0.0;
         QZ[1] = zon(zs + 1) ? hi : 0.0;
     }
     if (ring2) {
-        QT[0] = at(T, zs)[r2col];
-        QT[1] = at(T, zs + 1)[r2col];
-        hk0 = at(K, zs)[r2col];
-        QK[0] = at(K, zs + 1)[r2col];
-        QK[1] = at(K, zs + 2)[r2col];
+        QT[0] = ld(T, zs, r2b);
+        QT[1] = ld(T, zs + 1, r2b);
+        hk0 = ld(K, zs, r2b);
+        QK[0] = ld(K, zs + 1, r2b);
+        QK[1] = ld(K, zs + 2, r2b);
     }
     // The faces of plane zs, from k(zs) staged in the second buffer.
     sK[1][ty + 2][tx + 2] = RK[0];
@@ -497,14 +516,14 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
 
     double *__restrict__ Tout = a.tnew;
     double acc1 = 0.0, acc2 = 0.0;
-    int fb = 0;  // face buffer of plane z; planes z-1, z and z+1 are live at once
-    for (int64_t zb = zs; zb <= z1; zb += 4) {
+    // Plane z's faces are in buffer (z - zs) mod 4; planes z-1, z and z+1 are live at once.
+    for (int zb = zs; zb <= z1; zb += 4) {
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const int64_t z = zb + i;
+            const int z = zb + i;
             const bool live = z <= z1;
             const int b = i & 1;
-            const int fbn = fb == 2 ? 0 : fb + 1, fbp = fb == 0 ? 2 : fb - 1;
+            const int fb = i, fbn = (i + 1) & 3, fbp = (i + 3) & 3;
 
             // Stage T0(z) and k(z+1) on E2.
             sT[b][ty + 2][tx + 2] = RT[i];
@@ -516,8 +535,8 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
             if (ring2) {
                 sT[b][r2r][r2c] = QT[b];
                 sK[b][r2r][r2c] = QK[b];
-                QT[b] = at(T, z + 2)[r2col];
-                QK[b] = at(K, z + 3)[r2col];
+                QT[b] = ld(T, z + 2, r2b);
+                QK[b] = ld(K, z + 3, r2b);
             }
             lds_barrier();
 
@@ -525,8 +544,7 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
             // rounds of the faces of plane z+1, and a third shared between the remaining faces
             // and the ring-1 columns' z faces.
             {
-                const double v = series_nz(RK[(i + 1) & 3], RK[(i + 2) & 3]);
-                KZ[(i + 2) & 3] = zon(z + 2) ? v : 0.0;
+                KZ[(i + 2) & 3] = series_on(RK[(i + 1) & 3], RK[(i + 2) & 3], zon(z + 2));
             }
             {
                 const double *k = &sK[b][0][0];
@@ -534,18 +552,17 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
 #pragma unroll
                 for (int rnd = 0; rnd < 2; rnd++) {
                     const uint32_t d = fdesc[rnd];
-                    const double v = series_nz(k[fk(d)], k[fk2(d)]);
-                    f[fdst(d)] = fon(d) ? v : 0.0;
+                    f[fdst(d)] = series_on(k[fk(d)], k[fk2(d)], fon(d));
                 }
                 const uint32_t d = fdesc[2];
                 const double ka = third ? k[fk(d)] : QK[(i + 1) & 3];
                 const double kb = third ? k[fk2(d)] : QK[(i + 2) & 3];
-                const double v = series_nz(ka, kb);
-                if (third) f[fdst(d)] = fon(d) ? v : 0.0;
-                if (ring1) QZ[(i + 2) & 3] = zon(z + 2) ? v : 0.0;
+                const double v = series_on(ka, kb, third ? fon(d) : zon(z + 2));
+                if (third) f[fdst(d)] = v;
+                if (ring1) QZ[(i + 2) & 3] = v;
             }
-            RK[i] = at(K, z + 4)[ucol];  // k(z) is no longer needed
-            if (ring1) QK[i] = at(K, z + 4)[r1col];
+            RK[i] = ld(K, z + 4, ub);  // k(z) is no longer needed
+            if (ring1) QK[i] = ld(K, z + 4, r1b);
 
             // First step: T1(z) on E1 (its corners aside).
             const double *f = sF[fb];
@@ -558,8 +575,8 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
                 s1[b][r1r][r1c] = update(&sT[b][0][0], E2X, (r1r + 1) * E2X + r1c + 1, f[r1fx],
                                          f[r1fx + 1], f[r1fy], f[r1fy + E1X], QT[(i + 3) & 3],
                                          QT[i], QT[(i + 1) & 3], QZ[i], QZ[(i + 1) & 3]);
-            RT[(i + 3) & 3] = at(T, z + 3)[ucol];  // T0(z-1) is no longer needed
-            if (ring1) QT[(i + 3) & 3] = at(T, z + 3)[r1col];
+            RT[(i + 3) & 3] = ld(T, z + 3, ub);  // T0(z-1) is no longer needed
+            if (ring1) QT[(i + 3) & 3] = ld(T, z + 3, r1b);
             if (inside && z >= z0 && z < z1) acc1 += a.cap0 * (t1 - a.tref);
 
             // Second step: T2(z-1) on the tile, from T1(z-1) staged in the previous iteration.
@@ -568,10 +585,11 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
                                      g[fxo + 1], g[fyo], g[fyo + E1X], U1[(i + 2) & 3],
                                      U1[(i + 3) & 3], U1[i], KZ[(i + 3) & 3], KZ[i]);
             if (inside && live && z - 1 >= z0) {
-                __builtin_nontemporal_store(t2, Tout + plane * (z - 1) + ucol);
+                auto row = (GLOBAL char *)(Tout + plane * (z - 1));
+                asm("" : "+s"(row));
+                __builtin_nontemporal_store(t2, (GLOBAL double *)(row + ub));
                 acc2 += a.cap0 * (t2 - a.tref);
             }
-            fb = fbn;
         }
     }
 

@@ -60,8 +60,19 @@ enum { C_UNIFORM = 0, C_PER_CELL = 1 };
 // neighbours, whose rows it reads as halo, run on the same XCD and are found in its L2.
 constexpr int XCDS = 8;
 
-// Global (HBM) address space, for pointers whose provenance an empty asm has hidden.
-#define GLOBAL __attribute__((address_space(1)))
+// Raw buffer accesses at a uniform base address and a 32-bit byte offset below 2^32 - 8.  The
+// resource's last word sets a 32-bit data format (untyped loads and stores need a non-zero one);
+// the record count admits every 32-bit offset.  Aux 2 marks a store non-temporal.
+constexpr int BUF_WORD3 = 0x00020000;
+__device__ __forceinline__ double buffer_load_f64(const double *base, uint32_t off) {
+    const __amdgpu_buffer_rsrc_t r =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(base), (short)0, -1, BUF_WORD3);
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+__device__ __forceinline__ void buffer_store_f64_nt(double v, double *base, uint32_t off) {
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(base, (short)0, -1, BUF_WORD3);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r, (int)off, 0, 2);
+}
 
 // The faces of one plane from the conductivities staged in LDS: each thread builds its own low x
 // and y faces; the first TY threads add the tile's right-edge x faces and the last row of threads
@@ -423,12 +434,10 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
     const double *__restrict__ K = a.kcell;
     // The chunk reads planes z0-2 .. z1+1; later planes are clamped to what it already reads.
     const int top = std::min(z1 + 1, nz - 1);
-    // The empty asm pins the plane address to scalar registers, so the load is addressed as
-    // scalar base + 32-bit lane offset with no per-lane 64-bit add.
+    // Plane accesses go through a buffer resource built on the scalar unit from the plane's
+    // address, with the 32-bit lane offset as the only per-lane operand.
     auto ld = [&](const double *p, int q, uint32_t off) -> double {
-        auto base = (const GLOBAL char *)(p + plane * std::min(std::max(q, 0), top));
-        asm("" : "+s"(base));
-        return *(const GLOBAL double *)(base + off);
+        return buffer_load_f64(p + plane * std::min(std::max(q, 0), top), off);
     };
     auto zon = [&](int q) -> bool { return q > 0 && q < nz; };  // face between q-1 and q
     // One explicit update (solid.rs:1781-1826 order) from the in-plane neighbours in `t` around
@@ -524,6 +533,11 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
             const bool live = z <= z1;
             const int b = i & 1;
             const int fb = i, fbn = (i + 1) & 3, fbp = (i + 3) & 3;
+            // The planes this iteration fetches, computed once on the scalar unit (z + 2 >= 1).
+            const double *T2 = T + plane * std::min(z + 2, top);
+            const double *T3 = T + plane * std::min(z + 3, top);
+            const double *K3 = K + plane * std::min(z + 3, top);
+            const double *K4 = K + plane * std::min(z + 4, top);
 
             // Stage T0(z) and k(z+1) on E2.
             sT[b][ty + 2][tx + 2] = RT[i];
@@ -535,8 +549,8 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
             if (ring2) {
                 sT[b][r2r][r2c] = QT[b];
                 sK[b][r2r][r2c] = QK[b];
-                QT[b] = ld(T, z + 2, r2b);
-                QK[b] = ld(K, z + 3, r2b);
+                QT[b] = buffer_load_f64(T2, r2b);
+                QK[b] = buffer_load_f64(K3, r2b);
             }
             lds_barrier();
 
@@ -561,8 +575,8 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
                 if (third) f[fdst(d)] = v;
                 if (ring1) QZ[(i + 2) & 3] = v;
             }
-            RK[i] = ld(K, z + 4, ub);  // k(z) is no longer needed
-            if (ring1) QK[i] = ld(K, z + 4, r1b);
+            RK[i] = buffer_load_f64(K4, ub);  // k(z) is no longer needed
+            if (ring1) QK[i] = buffer_load_f64(K4, r1b);
 
             // First step: T1(z) on E1 (its corners aside).
             const double *f = sF[fb];
@@ -575,8 +589,8 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
                 s1[b][r1r][r1c] = update(&sT[b][0][0], E2X, (r1r + 1) * E2X + r1c + 1, f[r1fx],
                                          f[r1fx + 1], f[r1fy], f[r1fy + E1X], QT[(i + 3) & 3],
                                          QT[i], QT[(i + 1) & 3], QZ[i], QZ[(i + 1) & 3]);
-            RT[(i + 3) & 3] = ld(T, z + 3, ub);  // T0(z-1) is no longer needed
-            if (ring1) QT[(i + 3) & 3] = ld(T, z + 3, r1b);
+            RT[(i + 3) & 3] = buffer_load_f64(T3, ub);  // T0(z-1) is no longer needed
+            if (ring1) QT[(i + 3) & 3] = buffer_load_f64(T3, r1b);
             if (inside && z >= z0 && z < z1) acc1 += a.cap0 * (t1 - a.tref);
 
             // Second step: T2(z-1) on the tile, from T1(z-1) staged in the previous iteration.
@@ -585,9 +599,7 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
                                      g[fxo + 1], g[fyo], g[fyo + E1X], U1[(i + 2) & 3],
                                      U1[(i + 3) & 3], U1[i], KZ[(i + 3) & 3], KZ[i]);
             if (inside && live && z - 1 >= z0) {
-                auto row = (GLOBAL char *)(Tout + plane * (z - 1));
-                asm("" : "+s"(row));
-                __builtin_nontemporal_store(t2, (GLOBAL double *)(row + ub));
+                buffer_store_f64_nt(t2, Tout + plane * (z - 1), ub);
                 acc2 += a.cap0 * (t2 - a.tref);
             }
         }
@@ -961,8 +973,11 @@ static int heat_grid(const pm_solid3d *s, dim3 *grid, dim3 *block, int tile_y = 
 // The two-steps-per-pass sweep exists for per-cell conductivity with uniform capacity, in
 // 64 x 8 tiles.
 constexpr int PAIR_TY = 8;
+// The paired sweep addresses a plane with 32-bit byte offsets: planes of 2^29 cells or more take
+// the one-step sweep.
 static bool pairs_enabled(const pm_solid3d *s) {
-    return s->steps_per_pass == 2 && s->kmode == K_PER_CELL && s->cmode == C_UNIFORM;
+    return s->steps_per_pass == 2 && s->kmode == K_PER_CELL && s->cmode == C_UNIFORM &&
+           (int64_t)s->nx * s->ny < ((int64_t)1 << 29);
 }
 
 static int ensure_partials(pm_solid3d *s) {

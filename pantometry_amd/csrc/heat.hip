@@ -392,7 +392,10 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
     const uint32_t r2col = ring2 ? colidx(x0 - 2 + r2c, y0 - 2 + r2r) : 0u;
     // Byte offsets within a plane: they fit 32 bits (the paired sweep takes planes of fewer than
     // 2^29 cells), so each access is a uniform plane address plus one 32-bit lane offset.
-    const uint32_t ub = ucol * 8u, r1b = r1col * 8u, r2b = r2col * 8u;
+    // Every lane issues every load (a lane outside a ring re-reads its own column, which the
+    // tile's loads bring into cache), so the count of loads in flight is the same on every path
+    // and the waits before their uses stay exact.
+    const uint32_t ub = ucol * 8u, r1b = ring1 ? r1col * 8u : ub, r2b = ring2 ? r2col * 8u : ub;
 
     // This thread's in-plane faces, one per round (the third only for the first F2 threads),
     // packed as: bits 0-11 the first conductivity's offset in a k plane on E2, bits 12-23 the
@@ -485,7 +488,7 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
     QT[0] = QT[1] = QT[2] = QT[3] = 0.0;
     QK[0] = QK[1] = QK[2] = QK[3] = 0.0;
     QZ[0] = QZ[1] = QZ[2] = QZ[3] = 0.0;
-    double hk0 = 0.0;
+    double hk0 = 0.0, PT[2] = {0.0, 0.0}, PK[2] = {0.0, 0.0};
     if (ring1) {
         QT[3] = ld(T, zs - 1, r1b);
         QT[0] = ld(T, zs, r1b);
@@ -501,11 +504,11 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
         QZ[1] = zon(zs + 1) ? hi : 0.0;
     }
     if (ring2) {
-        QT[0] = ld(T, zs, r2b);
-        QT[1] = ld(T, zs + 1, r2b);
+        PT[0] = ld(T, zs, r2b);
+        PT[1] = ld(T, zs + 1, r2b);
         hk0 = ld(K, zs, r2b);
-        QK[0] = ld(K, zs + 1, r2b);
-        QK[1] = ld(K, zs + 2, r2b);
+        PK[0] = ld(K, zs + 1, r2b);
+        PK[1] = ld(K, zs + 2, r2b);
     }
     // The faces of plane zs, from k(zs) staged in the second buffer.
     sK[1][ty + 2][tx + 2] = RK[0];
@@ -547,11 +550,11 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
                 sK[b][r1r + 1][r1c + 1] = QK[(i + 1) & 3];
             }
             if (ring2) {
-                sT[b][r2r][r2c] = QT[b];
-                sK[b][r2r][r2c] = QK[b];
-                QT[b] = buffer_load_f64(T2, r2b);
-                QK[b] = buffer_load_f64(K3, r2b);
+                sT[b][r2r][r2c] = PT[b];
+                sK[b][r2r][r2c] = PK[b];
             }
+            PT[b] = buffer_load_f64(T2, r2b);
+            PK[b] = buffer_load_f64(K3, r2b);
             lds_barrier();
 
             // Four rounds of divisions: the tile column's z face between planes z+1 and z+2, two
@@ -576,7 +579,7 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
                 if (ring1) QZ[(i + 2) & 3] = v;
             }
             RK[i] = buffer_load_f64(K4, ub);  // k(z) is no longer needed
-            if (ring1) QK[i] = buffer_load_f64(K4, r1b);
+            QK[i] = buffer_load_f64(K4, r1b);
 
             // First step: T1(z) on E1 (its corners aside).
             const double *f = sF[fb];
@@ -590,7 +593,7 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
                                          f[r1fx + 1], f[r1fy], f[r1fy + E1X], QT[(i + 3) & 3],
                                          QT[i], QT[(i + 1) & 3], QZ[i], QZ[(i + 1) & 3]);
             RT[(i + 3) & 3] = buffer_load_f64(T3, ub);  // T0(z-1) is no longer needed
-            if (ring1) QT[(i + 3) & 3] = buffer_load_f64(T3, r1b);
+            QT[(i + 3) & 3] = buffer_load_f64(T3, r1b);
             if (inside && z >= z0 && z < z1) acc1 += a.cap0 * (t1 - a.tref);
 
             // Second step: T2(z-1) on the tile, from T1(z-1) staged in the previous iteration.

@@ -86,11 +86,45 @@ __device__ __forceinline__ double series_nz(double a, double b) {
     return sum > 0.0 ? q : 0.0;
 }
 
-// series_nz(a, b) for a face that exists (`on`), else 0, with one select for both guards.
+// n / d by the steps of the IEEE double division (reciprocal, two Newton steps, one residual
+// correction) without its operand scaling and special-case fix-up.  Those two leave the operands
+// and the quotient unchanged when d is a positive normal number, n a positive number of at least
+// 2^-969 whose exponent exceeds d's by less than 768, and n / d is normal, so there the result is
+// the correctly rounded quotient bit for bit; for n = +0 and a positive normal d it is +0, as the
+// division's is.
+__device__ __forceinline__ double div_unscaled(double n, double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    double e = __builtin_fma(-d, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-d, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    const double q = n * r;
+    e = __builtin_fma(-d, q, n);
+    return __builtin_fma(e, r, q);
+}
+
+// Conductivities for which every harmonic face mean may use div_unscaled: +0, or in
+// [2^-190, 2^190].  Two such values a, b > 0 give 2ab in [2^-379, 2^381] and a + b in
+// [2^-189, 2^191], well inside the range above; a = +0 gives 2ab = +0.
+__device__ __forceinline__ bool k_short_ok(double v) {
+    return __double_as_longlong(v) == 0 || (v >= 0x1p-190 && v <= 0x1p190);
+}
+
+// series_nz(a, b) for a face that exists (`on`), else 0, with one select for both guards; with
+// SHORT, for conductivities that all pass k_short_ok, the division is div_unscaled.
+template <bool SHORT = false>
 __device__ __forceinline__ double series_on(double a, double b, bool on) {
     const double sum = a + b;
-    const double q = 2.0 * a * b / sum;
+    const double q = SHORT ? div_unscaled(2.0 * a * b, sum) : 2.0 * a * b / sum;
     return (on && sum > 0.0) ? q : 0.0;
+}
+
+// Sets *outside when some conductivity fails k_short_ok.
+__global__ __launch_bounds__(256) void k_short_check(const double *k, int64_t n, int *outside) {
+    bool bad = false;
+    for (int64_t c = blockIdx.x * 256 + threadIdx.x; c < n; c += (int64_t)gridDim.x * 256)
+        bad = bad || !k_short_ok(k[c]);
+    if (bad) atomicOr(outside, 1);
 }
 
 template <int TY>
@@ -320,7 +354,7 @@ __global__ __launch_bounds__(64 * TY) void heat_sweep(HeatArgs a) {
 // waits on another at the plane's one barrier.  LDS: T0(z) and k(z+1) on E2 and T1 on E1,
 // double-buffered, and the faces of planes z-1, z and z+1 on E1.  The audit of each step is summed as its cells are
 // produced: step one over the owned planes of T1, step two over the stored T2.
-template <int TY>
+template <int TY, bool SHORT>
 __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
     constexpr int TX = 64, NT = TX * TY;
     constexpr int E1X = TX + 2, E1Y = TY + 2, E2X = TX + 4, E2Y = TY + 4;
@@ -561,7 +595,7 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
             // rounds of the faces of plane z+1, and a third shared between the remaining faces
             // and the ring-1 columns' z faces.
             {
-                KZ[(i + 2) & 3] = series_on(RK[(i + 1) & 3], RK[(i + 2) & 3], zon(z + 2));
+                KZ[(i + 2) & 3] = series_on<SHORT>(RK[(i + 1) & 3], RK[(i + 2) & 3], zon(z + 2));
             }
             {
                 const double *k = &sK[b][0][0];
@@ -569,12 +603,12 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
 #pragma unroll
                 for (int rnd = 0; rnd < 2; rnd++) {
                     const uint32_t d = fdesc[rnd];
-                    f[fdst(d)] = series_on(k[fk(d)], k[fk2(d)], fon(d));
+                    f[fdst(d)] = series_on<SHORT>(k[fk(d)], k[fk2(d)], fon(d));
                 }
                 const uint32_t d = fdesc[2];
                 const double ka = third ? k[fk(d)] : QK[(i + 1) & 3];
                 const double kb = third ? k[fk2(d)] : QK[(i + 2) & 3];
-                const double v = series_on(ka, kb, third ? fon(d) : zon(z + 2));
+                const double v = series_on<SHORT>(ka, kb, third ? fon(d) : zon(z + 2));
                 if (third) f[fdst(d)] = v;
                 if (ring1) QZ[(i + 2) & 3] = v;
             }
@@ -882,6 +916,9 @@ struct pm_solid3d {
     DevBuf t[2];
     int cur = 0;
     DevBuf kcell, cap, mob, saved;
+    // Whether every conductivity passes k_short_ok, known since the field last changed.
+    bool k_checked = false, k_short = false;
+    DevBuf kflag;
     bool saved_valid = false;
     DevBuf partials, audit, scalars, stored_dev;
     int64_t partial_count = 0;
@@ -1066,6 +1103,23 @@ static int sweep_once(pm_solid3d *s, double dt, int64_t step, double *totals_dev
     return PM_OK;
 }
 
+// Finds, once after each change of the conductivity field, whether every conductivity passes
+// k_short_ok, so the paired sweep may build its faces with the unscaled division.
+static int check_k_range(pm_solid3d *s) {
+    if (s->k_checked) return PM_OK;
+    if (!s->kflag) PM_CHECK(s->kflag.alloc(sizeof(int)));
+    PM_HIP(hipMemsetAsync(s->kflag.d(), 0, sizeof(int), s->stream));
+    k_short_check<<<grid_for(s->n(), 256), 256, 0, s->stream>>>(s->kcell.d(), s->n(),
+                                                                 s->kflag.as<int>());
+    PM_HIP(hipGetLastError());
+    int outside = 1;
+    PM_HIP(hipMemcpyAsync(&outside, s->kflag.d(), sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    PM_HIP(hipStreamSynchronize(s->stream));
+    s->k_short = outside == 0;
+    s->k_checked = true;
+    return PM_OK;
+}
+
 // Steps `step` and `step + 1` in one pass from t[cur] into t[cur^1], then both audits; swaps.
 static int sweep_pair(pm_solid3d *s, double dt, int64_t step, double *totals_dev, int64_t launch) {
     PM_CHECK(ensure_partials(s));
@@ -1087,7 +1141,10 @@ static int sweep_pair(pm_solid3d *s, double dt, int64_t step, double *totals_dev
     const bool prof = prof_slot(s, launch);
     if (prof) PM_HIP(hipEventRecord(s->prof[3 * launch], s->stream));
     const unsigned blocks = (unsigned)(XCDS * ((ntiles + XCDS - 1) / XCDS));
-    heat_sweep2<PAIR_TY><<<blocks, dim3(64, PAIR_TY), 0, s->stream>>>(a);
+    if (s->k_short)
+        heat_sweep2<PAIR_TY, true><<<blocks, dim3(64, PAIR_TY), 0, s->stream>>>(a);
+    else
+        heat_sweep2<PAIR_TY, false><<<blocks, dim3(64, PAIR_TY), 0, s->stream>>>(a);
     PM_HIP(hipGetLastError());
     if (prof) PM_HIP(hipEventRecord(s->prof[3 * launch + 1], s->stream));
     heat_finalize2<<<1, 1024, 0, s->stream>>>(s->partials.d(), ntiles, s->audit.as<AuditState>(),
@@ -1188,6 +1245,7 @@ static int promote_k(pm_solid3d *s) {
     if (s->kmode == K_PER_CELL) return PM_OK;
     PM_CHECK(s->kcell.alloc(sizeof(double) * (size_t)s->n()));
     fill_value<<<grid_for(s->n(), 256), 256, 0, s->stream>>>(s->kcell.d(), s->n(), s->k0);
+    s->k_checked = false;
     PM_HIP(hipGetLastError());
     s->kmode = K_PER_CELL;
     return PM_OK;
@@ -1683,6 +1741,7 @@ static int remix(pm_solid3d *s) {
         s->melted.d(), s->ph_ks.d(), s->ph_kl.d(), s->ph_cs.d(), s->ph_cl.d(),
         s->nvoid ? s->dvoid.as<uint8_t>() : nullptr, s->n(), s->dx, s->kcell.d(), s->cap.d(), s->mob.d());
     PM_HIP(hipGetLastError());
+    s->k_checked = false;
     s->heat_capacity_valid = false;
     return PM_OK;
 }
@@ -1695,6 +1754,7 @@ static int apply_voids(pm_solid3d *s) {
     void_zero<<<grid_for(s->n(), 256), 256, 0, s->stream>>>(s->dvoid.as<uint8_t>(), s->n(),
                                                             s->kcell.d(), s->cap.d(), s->mob.d());
     PM_HIP(hipGetLastError());
+    s->k_checked = false;
     s->heat_capacity_valid = false;
     return PM_OK;
 }
@@ -2057,6 +2117,7 @@ int pm_solid3d_fill(pm_solid3d *s, const pm_substance *sub, const uint8_t *mask)
         s->cmode == C_PER_CELL ? s->cap.d() : nullptr,
         s->cmode == C_PER_CELL ? s->mob.d() : nullptr, k, c, s->dx);
     PM_HIP(hipGetLastError());
+    s->k_checked = false;
     if (s->phase || melts(*sub) || s->two_phase) {
         PM_CHECK(build_phase(s, false, 0.0));
         PM_CHECK(remix(s));
@@ -2075,6 +2136,7 @@ int pm_solid3d_set_conductivity(pm_solid3d *s, const double *k) {
         s->kmode = K_PER_CELL;
     }
     PM_HIP(hipMemcpyAsync(s->kcell.d(), k, sizeof(double) * (size_t)s->n(), hipMemcpyHostToDevice, s->stream));
+    s->k_checked = false;
     if (s->phase) {
         // Both phases of every cell take the given conductivity.
         PM_HIP(hipMemcpyAsync(s->ph_ks.d(), k, sizeof(double) * (size_t)s->n(), hipMemcpyHostToDevice, s->stream));
@@ -2343,6 +2405,7 @@ int pm_solid3d_advance(pm_solid3d *s, double dt, uint64_t steps, double toleranc
     };
     std::vector<Launch> launches;
     const bool pairs = pairs_enabled(s);
+    if (pairs && steps > 1) PM_CHECK(check_k_range(s));
     PM_HIP(hipEventRecord(s->ev0, s->stream));
     for (uint64_t n = 0; n < steps;) {
         const int64_t l = (int64_t)launches.size();

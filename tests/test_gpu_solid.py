@@ -202,6 +202,60 @@ def test_paired_advance_every_z_chunk(z_chunk, oracle):
     assert np.array_equal(bits(g1.cells()), bits(o.cells))
 
 
+def _extreme_conductivity(counts, rng, low, high):
+    """Conductivities 2^e * (1 + u) with e uniform over [low, high), the bounds themselves, and
+    zeros (void-like cells), so harmonic faces meet every exponent pairing in the range."""
+    n = counts[0] * counts[1] * counts[2]
+    k = np.ldexp(1.0 + rng.random(n), rng.integers(low, high, size=n))
+    k[rng.random(n) < 0.05] = 0.0
+    k[0], k[1], k[2] = 2.0 ** low, 2.0 ** high, 0.0
+    return k
+
+
+@pytest.mark.parametrize("span", [(-190, 190), (-400, 200), (-30, 30)], ids=str)
+def test_paired_faces_at_extreme_conductivities(span, oracle):
+    """The paired sweep builds faces with a shortened division when every conductivity is +0 or
+    in [2^-190, 2^190] (the first and last spans), and with the full division otherwise (the
+    second): both give the oracle's bits, including a -0 conductivity, which takes the full one."""
+    gpu_case(f"paired faces at conductivities 2^{span}")
+    counts = (70, 37, 23)
+    rng = np.random.default_rng(abs(span[0]) + span[1])
+    k = _extreme_conductivity(counts, rng, span[0], span[1])
+    if span == (-30, 30):
+        k[5] = -0.0
+    t = 300.0 + rng.uniform(-50.0, 50.0, size=k.size)
+    g, o = _pair(counts, k=k, t=t)
+    assert g.steps_per_pass() == 2
+    dt = 0.9 * o.max_stable_dt()
+    assert g.max_stable_dt() == o.max_stable_dt()
+    g.advance(dt, 9, tolerance=1.0)
+    for _ in range(9):
+        o.step(dt, threads_=16)
+    assert np.array_equal(bits(g.cells()), bits(o.cells))
+
+
+def test_the_face_division_follows_changes_of_conductivity(oracle):
+    """A block first advanced with conductivities inside the shortened division's range, then
+    given some outside it (and back), stays on the oracle's bits: the range is re-read after
+    each change."""
+    gpu_case("face division after conductivity changes")
+    counts = (65, 9, 33)
+    rng = np.random.default_rng(77)
+    inside = _extreme_conductivity(counts, rng, -100, 100)
+    outside = inside.copy()
+    outside[rng.integers(0, inside.size, size=40)] = 2.0 ** -700
+    t = 300.0 + rng.uniform(-50.0, 50.0, size=inside.size)
+    g, o = _pair(counts, k=inside, t=t)
+    for k in (inside, outside, inside):
+        g.with_conductivity(k)
+        o.with_conductivity(k)
+        dt = 0.9 * o.max_stable_dt()
+        g.advance(dt, 6, tolerance=1.0)
+        for _ in range(6):
+            o.step(dt, threads_=16)
+        assert np.array_equal(bits(g.cells()), bits(o.cells))
+
+
 def test_a_breach_on_the_second_step_of_a_pair(oracle):
     """The audit after each step of a pair: a tolerance that the first few steps meet and a
     second step of a pair breaks stops the run there, with the state that step made."""

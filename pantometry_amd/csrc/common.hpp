@@ -42,6 +42,20 @@ void set_error(const char *fmt, ...);
 int require_device();
 
 // Owning device allocation.
+// Raw buffer accesses at a uniform base address and a 32-bit byte offset below 2^32 - 8.  The
+// resource's last word sets a 32-bit data format (untyped loads and stores need a non-zero one);
+// the record count admits every 32-bit offset.  Aux 2 marks a store non-temporal.
+constexpr int BUF_WORD3 = 0x00020000;
+__device__ __forceinline__ double buffer_load_f64(const double *base, uint32_t off) {
+    const __amdgpu_buffer_rsrc_t r =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(base), (short)0, -1, BUF_WORD3);
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+__device__ __forceinline__ void buffer_store_f64_nt(double v, double *base, uint32_t off) {
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(base, (short)0, -1, BUF_WORD3);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r, (int)off, 0, 2);
+}
+
 class DevBuf {
   public:
     DevBuf() = default;

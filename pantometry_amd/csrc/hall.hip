@@ -227,29 +227,39 @@ __global__ __launch_bounds__(64 * TY, 4) void hall_march2(HallArgs a, int gx, in
     const int64_t bx = tile % gx, by = (tile / gx) % gy, bz = tile / ((int64_t)gx * gy);
 
     const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TX * ty;
-    const int64_t nx = a.nx, ny = a.ny, nz = a.nz, plane = nx * ny;
+    const int64_t nx = a.nx, ny = a.ny, plane = nx * ny;
     const int64_t x0 = bx * TX, y0 = by * TY;
-    const int64_t z0 = bz * zc, z1 = std::min<int64_t>(z0 + zc, nz);
-    const bool has_vx = nx > 1, has_vy = ny > 1, has_vz = nz > 1;
+    // Plane indices are uniform 32-bit values, so their clamps and tests stay on the scalar unit.
+    // The paired step is launched only for halls at least two nodes along every axis, so every
+    // velocity array has faces.
+    const int nz = a.nz;
+    const int z0 = (int)bz * zc, z1 = std::min(z0 + zc, nz);
     const int64_t vxp = (nx - 1) * ny, vyp = nx * (ny - 1);
     auto cx = [&](int64_t x) { return std::min(std::max<int64_t>(x, 0), nx - 1); };
     auto cy = [&](int64_t y) { return std::min(std::max<int64_t>(y, 0), ny - 1); };
-    auto kc = [&](int64_t q) { return std::min(std::max<int64_t>(q, 0), nz - 1); };
-    auto kz = [&](int64_t q) { return std::min(std::max<int64_t>(q, 0), nz - 2); };
-    // A node's column offsets in p, vx (its +x face), vy (its +y face), clamped to ones that exist.
-    // (In-plane offsets fit 32 bits: the hall is limited to 2^31 nodes a plane.)
+    auto kc = [&](int q) { return std::min(std::max(q, 0), nz - 1); };
+    auto kz = [&](int q) { return std::min(std::max(q, 0), nz - 2); };
+    // A node's byte offsets within a plane of p (and vz), vx (its +x face) and vy (its +y face),
+    // clamped to ones that exist (they fit 32 bits: planes of the paired step hold fewer than 2^29
+    // nodes), and its in-plane index, or ~0 off the hall.
     struct Col {
         int x, y;
-        uint32_t p, vx, vy;
+        uint32_t p, vx, vy, n;
     };
     auto column = [&](int64_t x, int64_t y) {
         Col c;
         c.x = (int)x, c.y = (int)y;
-        c.p = (uint32_t)(cx(x) + nx * cy(y));
-        c.vx = (uint32_t)(std::min(cx(x), nx - 2) + (nx - 1) * cy(y));
-        c.vy = (uint32_t)(cx(x) + nx * std::min(cy(y), ny - 2));
+        c.p = (uint32_t)(cx(x) + nx * cy(y)) * 8u;
+        c.vx = (uint32_t)(std::min(cx(x), nx - 2) + (nx - 1) * cy(y)) * 8u;
+        c.vy = (uint32_t)(cx(x) + nx * std::min(cy(y), ny - 2)) * 8u;
+        c.n = (x >= 0 && x < nx && y >= 0 && y < ny) ? (uint32_t)(x + nx * y) : ~0u;
         return c;
     };
+    // The sources of the two steps as (in-plane index, plane); none matches no node.
+    const uint32_t sn1 = a.src >= 0 ? (uint32_t)(a.src % plane) : ~0u - 1;
+    const int sk1 = a.src >= 0 ? (int)(a.src / plane) : INT32_MIN;
+    const uint32_t sn2 = a.src2 >= 0 ? (uint32_t)(a.src2 % plane) : ~0u - 1;
+    const int sk2 = a.src2 >= 0 ? (int)(a.src2 / plane) : INT32_MIN;
 
     // Roles.  Every thread: its tile node, E1 (ty+1, tx+1).  The last R1 threads: an E1 ring node
     // (r1r, r1c).  The first R2 threads: an E2 node whose p (and, on the left column and bottom
@@ -265,7 +275,10 @@ __global__ __launch_bounds__(64 * TY, 4) void hall_march2(HallArgs a, int gx, in
         else if (q >= 2 * (E1X - 2) + TY && q < R1) r1r = 1 + q - 2 * (E1X - 2) - TY, r1c = E1X - 1;
     }
     const bool ring1 = r1r >= 0;
-    const Col rc = column(x0 - 1 + r1c, y0 - 1 + r1r);
+    // A lane outside a role loads its own tile column in that role's place: every lane issues
+    // every load, so the count of loads in flight is the same on every path and the waits before
+    // their uses stay exact.
+    const Col rc = ring1 ? column(x0 - 1 + r1c, y0 - 1 + r1r) : tc;
     int r2r = -1, r2c = -1;
     {
         constexpr int SIDES = 2 * (E2X - 4) + 2 * (E2Y - 4);
@@ -281,26 +294,28 @@ __global__ __launch_bounds__(64 * TY, 4) void hall_march2(HallArgs a, int gx, in
     }
     const bool ring2 = r2r >= 0;
     const bool corner = ring2 && (r2r == 1 || r2r == E2Y - 2) && (r2c == 1 || r2c == E2X - 2);
-    const Col hc = column(x0 - 2 + r2c, y0 - 2 + r2r);
+    const Col hc = ring2 ? column(x0 - 2 + r2c, y0 - 2 + r2r) : tc;
     // which faces a ring-2 thread stages: the +x face on E2's left column and at E1's corners,
-    // the +y face on E2's bottom row and at E1's corners (when those faces exist)
-    const bool h_vx = ring2 && (r2c == 0 || corner) && has_vx && hc.x >= 0 && hc.x + 1 < nx;
-    const bool h_vy = ring2 && (r2r == 0 || corner) && has_vy && hc.y >= 0 && hc.y + 1 < ny;
+    // the +y face on E2's bottom row and at E1's corners (0 where those faces do not exist)
+    const bool s_vx = ring2 && (r2c == 0 || corner), s_vy = ring2 && (r2r == 0 || corner);
+    const bool h_vx = s_vx && hc.x >= 0 && hc.x + 1 < nx;
+    const bool h_vy = s_vy && hc.y >= 0 && hc.y + 1 < ny;
 
-    const double *__restrict__ P = a.p;
-    auto ldp = [&](const Col &c, int64_t k) { return P[c.p + plane * kc(k)]; };
-    auto ldvx = [&](bool ok, const Col &c, int64_t k) { return ok ? a.vx[c.vx + vxp * kc(k)] : 0.0; };
-    auto ldvy = [&](bool ok, const Col &c, int64_t k) { return ok ? a.vy[c.vy + vyp * kc(k)] : 0.0; };
-    auto ldvz = [&](const Col &c, int64_t k) { return has_vz ? a.vz[c.p + plane * kz(k)] : 0.0; };
-    const bool t_vx = has_vx, t_vy = has_vy;  // loads of a node's own +x/+y face (clamped)
+    // Plane addresses (uniform) of plane q of each array.
+    auto pP = [&](int q) { return a.p + plane * kc(q); };
+    auto pVX = [&](int q) { return a.vx + vxp * kc(q); };
+    auto pVY = [&](int q) { return a.vy + vyp * kc(q); };
+    auto pVZ = [&](int q) { return a.vz + plane * kz(q); };
 
     const double g = a.gain, sc = a.scale;
+    // Wall weights are 1 or 2, so 1 / (a product of them) is exactly a product of 1 or 0.5.
+    auto inv_wall = [](int64_t k, int64_t n) { return (k == 0 || k + 1 == n) ? 0.5 : 1.0; };
     // The first step at E1 node (r, c) of plane k: new face velocities and pressure.
     struct Step {
         double hx, lx, hy, ly, hz, lz, p;
     };
-    auto first = [&](int b, int r, int c, const Col &cc, int64_t k, double pm, double pcur,
-                     double pp, double vzm, double vzc, int64_t src, double source) {
+    auto first = [&](int b, int r, int c, const Col &cc, int k, double pm, double pcur,
+                     double pp, double vzm, double vzc) {
         Step o;
         const int64_t x = cc.x, y = cc.y;
         o.hx = 0.0, o.lx = 0.0, o.hy = 0.0, o.ly = 0.0, o.hz = 0.0, o.lz = 0.0;
@@ -312,39 +327,34 @@ __global__ __launch_bounds__(64 * TY, 4) void hall_march2(HallArgs a, int gx, in
         if (k > 0) o.lz = vzm - g * (pcur - pm);
         const double wi = wall(x, nx), wj = wall(y, ny), wk = wall(k, nz);
         double pn = pcur - sc * (wi * (o.hx - o.lx) + wj * (o.hy - o.ly) + wk * (o.hz - o.lz));
-        if (x + nx * (y + ny * k) == src) pn = pn + source;
+        if (cc.n == sn1 && k == sk1) pn = pn + a.source;
         o.p = pn;
         return o;
     };
 
-    const int64_t zs = z0 - 1;
-    // Own columns: p(k-1..k+1), vz(k-1..k), vx(k), vy(k), fetched one plane ahead; the tile node
-    // also p'(k-2..k) and the previous plane's v'.  Ring-2: p, vx, vy of plane k.
-    double tpm = ldp(tc, zs - 1), tpc = ldp(tc, zs), tpp = ldp(tc, zs + 1);
-    double tzm = ldvz(tc, zs - 1), tzc = ldvz(tc, zs);
-    double txc = ldvx(t_vx, tc, zs), tyc = ldvy(t_vy, tc, zs);
-    // Ring-1 column registers.  A ring-2 thread, never a ring-1 thread, keeps its p, vx, vy of
-    // plane k in the first three of them.
-    double rpm = 0.0, rpc = 0.0, rpp = 0.0, rzm = 0.0, rzc = 0.0, rxc = 0.0, ryc = 0.0;
-    double &hp0 = rpm, &hx0 = rpc, &hy0 = rpp;
-    if (ring1) {
-        rpm = ldp(rc, zs - 1), rpc = ldp(rc, zs), rpp = ldp(rc, zs + 1);
-        rzm = ldvz(rc, zs - 1), rzc = ldvz(rc, zs);
-        rxc = ldvx(t_vx, rc, zs), ryc = ldvy(t_vy, rc, zs);
-    }
-    if (ring2) {
-        hp0 = ldp(hc, zs);
-        hx0 = ldvx(h_vx, hc, zs);
-        hy0 = ldvy(h_vy, hc, zs);
-    }
+    const int zs = z0 - 1;
+    // Own columns: p(k-1..k+2), vz(k-1..k+1), vx(k), vy(k); the tile node also p'(k-2..k) and
+    // the previous plane's v'.  Ring 2: p, vx, vy of plane k.  Loads for plane k+1 (k+2 for
+    // p and vz ahead of use) are issued right after plane k is staged.
+    double tpm = buffer_load_f64(pP(zs - 1), tc.p), tpc = buffer_load_f64(pP(zs), tc.p);
+    double tpp = buffer_load_f64(pP(zs + 1), tc.p), tpq = 0.0;
+    double tzm = buffer_load_f64(pVZ(zs - 1), tc.p), tzc = buffer_load_f64(pVZ(zs), tc.p), tzp = 0.0;
+    double txc = buffer_load_f64(pVX(zs), tc.vx), tyc = buffer_load_f64(pVY(zs), tc.vy);
+    double rpm = buffer_load_f64(pP(zs - 1), rc.p), rpc = buffer_load_f64(pP(zs), rc.p);
+    double rpp = buffer_load_f64(pP(zs + 1), rc.p), rpq = 0.0;
+    double rzm = buffer_load_f64(pVZ(zs - 1), rc.p), rzc = buffer_load_f64(pVZ(zs), rc.p), rzp = 0.0;
+    double rxc = buffer_load_f64(pVX(zs), rc.vx), ryc = buffer_load_f64(pVY(zs), rc.vy);
+    double hp0 = buffer_load_f64(pP(zs), hc.p), hx0 = buffer_load_f64(pVX(zs), hc.vx);
+    double hy0 = buffer_load_f64(pVY(zs), hc.vy);
     double qm = 0.0, qc = 0.0;  // the tile node's p'(k-2), p'(k-1)
     double plz = 0.0;           // its v'z below plane k-1 (the one above is this plane's lz)
 
+    const double iwi = inv_wall(tc.x, nx), iwj = inv_wall(tc.y, ny);
     const double wi = wall(tc.x, nx), wj = wall(tc.y, ny);
     double acc1 = 0.0, acc2 = 0.0;
-    for (int64_t k = zs; k <= z1; k++) {
-        const int b = (int)((k - zs) & 1);
-        // Stage p(k), vx(k), vy(k) on E2.
+    for (int k = zs; k <= z1; k++) {
+        const int b = (k - zs) & 1;
+        // Stage p(k), vx(k), vy(k) on E2, then fetch the next planes into the freed registers.
         sP[b][ty + 2][tx + 2] = tpc;
         sVX[b][ty + 1][tx + 2] = txc;
         sVY[b][ty + 2][tx + 1] = tyc;
@@ -355,38 +365,52 @@ __global__ __launch_bounds__(64 * TY, 4) void hall_march2(HallArgs a, int gx, in
         }
         if (ring2) {
             sP[b][r2r][r2c] = hp0;
-            if (r2c == 0 || corner) sVX[b][r2r - 1][r2c] = hx0;
-            if (r2r == 0 || corner) sVY[b][r2r][r2c - 1] = hy0;
+            if (s_vx) sVX[b][r2r - 1][r2c] = h_vx ? hx0 : 0.0;
+            if (s_vy) sVY[b][r2r][r2c - 1] = h_vy ? hy0 : 0.0;
+        }
+        {
+            const double *P2 = pP(k + 2), *X1 = pVX(k + 1), *Y1 = pVY(k + 1), *Z1 = pVZ(k + 1);
+            const double *P1 = pP(k + 1);
+            tpq = buffer_load_f64(P2, tc.p);
+            tzp = buffer_load_f64(Z1, tc.p);
+            txc = buffer_load_f64(X1, tc.vx);
+            tyc = buffer_load_f64(Y1, tc.vy);
+            rpq = buffer_load_f64(P2, rc.p);
+            rzp = buffer_load_f64(Z1, rc.p);
+            rxc = buffer_load_f64(X1, rc.vx);
+            ryc = buffer_load_f64(Y1, rc.vy);
+            hp0 = buffer_load_f64(P1, hc.p);
+            hx0 = buffer_load_f64(X1, hc.vx);
+            hy0 = buffer_load_f64(Y1, hc.vy);
         }
         lds_barrier();
 
         // First step on plane k.
-        const Step t1 = first(b, ty + 1, tx + 1, tc, k, tpm, tpc, tpp, tzm, tzc, a.src, a.source);
+        const Step t1 = first(b, ty + 1, tx + 1, tc, k, tpm, tpc, tpp, tzm, tzc);
         sQ[b][ty + 1][tx + 1] = t1.p;
         sHX[b][ty + 1][tx + 1] = t1.hx;
         sHY[b][ty + 1][tx + 1] = t1.hy;
         if (ring1) {
-            const Step r1 = first(b, r1r, r1c, rc, k, rpm, rpc, rpp, rzm, rzc, a.src, a.source);
+            const Step r1 = first(b, r1r, r1c, rc, k, rpm, rpc, rpp, rzm, rzc);
             sQ[b][r1r][r1c] = r1.p;
             sHX[b][r1r][r1c] = r1.hx;
             sHY[b][r1r][r1c] = r1.hy;
         }
-        const int64_t n1 = tc.p + plane * k;
         if (inside && k >= z0 && k < z1) {
-            const double wk = wall(k, nz);
-            const double share = 1.0 / (wi * wj * wk);
-            if (n1 == a.src)
+            const double iwk = inv_wall(k, nz);
+            const double share = iwi * iwj * iwk;
+            if (tc.n == sn1 && k == sk1)
                 *a.supplied_step = share * (a.source + a.source_prev) * tpc / a.two_rc2 * a.volume;
             acc1 += share * t1.p * tpc / a.two_rc2 * a.volume;
-            if (tc.x + 1 < nx) acc1 += 1.0 / (wj * wk) * a.half_rho * t1.hx * t1.hx * a.volume;
-            if (tc.y + 1 < ny) acc1 += 1.0 / (wi * wk) * a.half_rho * t1.hy * t1.hy * a.volume;
-            if (k + 1 < nz) acc1 += 1.0 / (wi * wj) * a.half_rho * t1.hz * t1.hz * a.volume;
+            if (tc.x + 1 < nx) acc1 += iwj * iwk * a.half_rho * t1.hx * t1.hx * a.volume;
+            if (tc.y + 1 < ny) acc1 += iwi * iwk * a.half_rho * t1.hy * t1.hy * a.volume;
+            if (k + 1 < nz) acc1 += iwi * iwj * a.half_rho * t1.hz * t1.hz * a.volume;
         }
 
         // Second step on plane k-1, from p' of planes k-2, k-1 (registers, LDS) and k, and the
         // v' faces of plane k-1: in-plane ones staged by their nodes, z ones in registers.
         {
-            const int64_t k2 = k - 1;
+            const int k2 = k - 1;
             const double pc = qc;
             const int r = ty + 1, c = tx + 1;
             double hx = 0.0, lx = 0.0, hy = 0.0, ly = 0.0, hz = 0.0, lz = 0.0;
@@ -398,42 +422,32 @@ __global__ __launch_bounds__(64 * TY, 4) void hall_march2(HallArgs a, int gx, in
             if (k2 > 0) lz = plz - g * (pc - qm);
             const double wk = wall(k2, nz);
             double pn = pc - sc * (wi * (hx - lx) + wj * (hy - ly) + wk * (hz - lz));
-            const int64_t n2 = tc.p + plane * k2;
-            if (n2 == a.src2) pn = pn + a.source2;
+            const bool at_src = tc.n == sn2 && k2 == sk2;
+            if (at_src) pn = pn + a.source2;
             if (inside && k2 >= z0 && k2 < z1) {
-                const double share = 1.0 / (wi * wj * wk);
-                if (n2 == a.src2)
+                const double iwk = inv_wall(k2, nz);
+                const double share = iwi * iwj * iwk;
+                if (at_src)
                     *a.supplied_step2 = share * (a.source2 + a.source) * pc / a.two_rc2 * a.volume;
-                __builtin_nontemporal_store(pn, a.p_out + n2);
-                __builtin_nontemporal_store(pc, a.p_mid + n2);
-                if (tc.x + 1 < nx) __builtin_nontemporal_store(hx, a.vx_out + tc.vx + vxp * k2);
-                if (tc.y + 1 < ny) __builtin_nontemporal_store(hy, a.vy_out + tc.vy + vyp * k2);
-                if (k2 + 1 < nz) __builtin_nontemporal_store(hz, a.vz_out + n2);
+                buffer_store_f64_nt(pn, a.p_out + plane * k2, tc.p);
+                buffer_store_f64_nt(pc, a.p_mid + plane * k2, tc.p);
+                if (tc.x + 1 < nx) buffer_store_f64_nt(hx, a.vx_out + vxp * k2, tc.vx);
+                if (tc.y + 1 < ny) buffer_store_f64_nt(hy, a.vy_out + vyp * k2, tc.vy);
+                if (k2 + 1 < nz) buffer_store_f64_nt(hz, a.vz_out + plane * k2, tc.p);
                 acc2 += share * pn * pc / a.two_rc2 * a.volume;
-                if (tc.x + 1 < nx) acc2 += 1.0 / (wj * wk) * a.half_rho * hx * hx * a.volume;
-                if (tc.y + 1 < ny) acc2 += 1.0 / (wi * wk) * a.half_rho * hy * hy * a.volume;
-                if (k2 + 1 < nz) acc2 += 1.0 / (wi * wj) * a.half_rho * hz * hz * a.volume;
+                if (tc.x + 1 < nx) acc2 += iwj * iwk * a.half_rho * hx * hx * a.volume;
+                if (tc.y + 1 < ny) acc2 += iwi * iwk * a.half_rho * hy * hy * a.volume;
+                if (k2 + 1 < nz) acc2 += iwi * iwj * a.half_rho * hz * hz * a.volume;
             }
         }
         qm = qc, qc = t1.p;
         plz = t1.lz;
 
-        // Advance the rings; fetch the next plane.
-        tpm = tpc, tpc = tpp, tpp = ldp(tc, k + 2);
-        tzm = tzc, tzc = ldvz(tc, k + 1);
-        txc = ldvx(t_vx, tc, k + 1);
-        tyc = ldvy(t_vy, tc, k + 1);
-        if (ring1) {
-            rpm = rpc, rpc = rpp, rpp = ldp(rc, k + 2);
-            rzm = rzc, rzc = ldvz(rc, k + 1);
-            rxc = ldvx(t_vx, rc, k + 1);
-            ryc = ldvy(t_vy, rc, k + 1);
-        }
-        if (ring2) {
-            hp0 = ldp(hc, k + 1);
-            hx0 = ldvx(h_vx, hc, k + 1);
-            hy0 = ldvy(h_vy, hc, k + 1);
-        }
+        // Advance the rings.
+        tpm = tpc, tpc = tpp, tpp = tpq;
+        tzm = tzc, tzc = tzp;
+        rpm = rpc, rpc = rpp, rpp = rpq;
+        rzm = rzc, rzc = rzp;
     }
     const double total1 = block_sum<NT>(acc1, red);
     __syncthreads();
@@ -1203,7 +1217,10 @@ int pm_hall_advance(pm_hall *h, double dt, uint64_t steps, double tolerance, dou
         uint64_t cursor;
     };
     std::vector<Launch> launches;
-    const bool pairs = h->steps_per_pass == 2;
+    // The paired step addresses a plane with 32-bit byte offsets and assumes faces along every
+    // axis: other halls step one at a time.
+    const bool pairs = h->steps_per_pass == 2 && h->nx >= 2 && h->ny >= 2 && h->nz >= 2 &&
+                       h->nx * h->ny < ((int64_t)1 << 29);
     for (uint64_t n = first; n < steps;) {
         launches.push_back({(int64_t)n, 1, h->ip, h->iq, h->is, h->iv, h->src_cursor});
         if (pairs && n + 1 < steps) {

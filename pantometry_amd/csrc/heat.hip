@@ -60,19 +60,6 @@ enum { C_UNIFORM = 0, C_PER_CELL = 1 };
 // neighbours, whose rows it reads as halo, run on the same XCD and are found in its L2.
 constexpr int XCDS = 8;
 
-// Raw buffer accesses at a uniform base address and a 32-bit byte offset below 2^32 - 8.  The
-// resource's last word sets a 32-bit data format (untyped loads and stores need a non-zero one);
-// the record count admits every 32-bit offset.  Aux 2 marks a store non-temporal.
-constexpr int BUF_WORD3 = 0x00020000;
-__device__ __forceinline__ double buffer_load_f64(const double *base, uint32_t off) {
-    const __amdgpu_buffer_rsrc_t r =
-        __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(base), (short)0, -1, BUF_WORD3);
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
-}
-__device__ __forceinline__ void buffer_store_f64_nt(double v, double *base, uint32_t off) {
-    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(base, (short)0, -1, BUF_WORD3);
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r, (int)off, 0, 2);
-}
 
 // The faces of one plane from the conductivities staged in LDS: each thread builds its own low x
 // and y faces; the first TY threads add the tile's right-edge x faces and the last row of threads

@@ -114,23 +114,16 @@ __global__ __launch_bounds__(256) void k_short_check(const double *k, int64_t n,
     if (bad) atomicOr(outside, 1);
 }
 
-template <int TY>
+template <int TY, bool SHORT = false>
 __device__ __forceinline__ void build_faces(const double (*sk)[66], double (*fx)[65],
                                             double (*fy)[64], int tx, int ty, int tid,
                                             bool fxl_on, bool fyl_on, bool fxe_on, bool fye_on) {
     constexpr int TX = 64;
     const double c = sk[ty + 1][tx + 1];
-    const double vx = series_nz(sk[ty + 1][tx], c), vy = series_nz(sk[ty][tx + 1], c);
-    fx[ty][tx] = fxl_on ? vx : 0.0;
-    fy[ty][tx] = fyl_on ? vy : 0.0;
-    if (tid < TY) {
-        const double v = series_nz(sk[tid + 1][TX], sk[tid + 1][TX + 1]);
-        fx[tid][TX] = fxe_on ? v : 0.0;
-    }
-    if (ty == TY - 1) {
-        const double v = series_nz(sk[TY][tx + 1], sk[TY + 1][tx + 1]);
-        fy[TY][tx] = fye_on ? v : 0.0;
-    }
+    fx[ty][tx] = series_on<SHORT>(sk[ty + 1][tx], c, fxl_on);
+    fy[ty][tx] = series_on<SHORT>(sk[ty][tx + 1], c, fyl_on);
+    if (tid < TY) fx[tid][TX] = series_on<SHORT>(sk[tid + 1][TX], sk[tid + 1][TX + 1], fxe_on);
+    if (ty == TY - 1) fy[TY][tx] = series_on<SHORT>(sk[TY][tx + 1], sk[TY + 1][tx + 1], fye_on);
 }
 
 // One sweep (solid.rs:1776-1826) over a 64 x TY column tile and a chunk of z-planes, with the
@@ -142,7 +135,7 @@ __device__ __forceinline__ void build_faces(const double (*sk)[66], double (*fx)
 // T(z-1..z+2) and k(z..z+3) live in four-register rings and the halo values in two-register
 // rings; the z loop is unrolled by four so every ring slot has a fixed register and a load
 // issued for a later plane is not waited on until that plane's iteration uses it.
-template <int KMODE, int CMODE, int TY>
+template <int KMODE, int CMODE, int TY, bool SHORT = false>
 __global__ __launch_bounds__(64 * TY) void heat_sweep(HeatArgs a) {
     constexpr int TX = 64;
     constexpr int NT = TX * TY;
@@ -160,9 +153,11 @@ __global__ __launch_bounds__(64 * TY) void heat_sweep(HeatArgs a) {
     const int64_t bx = tile % a.gx, by = (tile / a.gx) % a.gy, bz = tile / ((int64_t)a.gx * a.gy);
 
     const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TX * ty;
-    const int64_t nx = a.nx, ny = a.ny, nz = a.nz, plane = nx * ny;
+    const int64_t nx = a.nx, ny = a.ny, plane = nx * ny;
     const int64_t x0 = bx * TX, y0 = by * TY;
-    const int64_t z0 = bz * a.zc, z1 = std::min<int64_t>(z0 + a.zc, nz);
+    // Plane indices are uniform 32-bit values, so their clamps and tests stay on the scalar unit.
+    const int nz = a.nz;
+    const int z0 = (int)bz * a.zc, z1 = std::min(z0 + a.zc, nz);
     const int64_t gi = x0 + tx, gj = y0 + ty;
     const bool inside = gi < nx && gj < ny;
     const int64_t col = std::min(gi, nx - 1) + nx * std::min(gj, ny - 1);
@@ -192,15 +187,18 @@ __global__ __launch_bounds__(64 * TY) void heat_sweep(HeatArgs a) {
 
     const double *__restrict__ T = a.told;
     const double *__restrict__ K = a.kcell;
-    // In-plane offsets fit 32 bits (the block is limited to 2^31 cells); the plane's base pointer
-    // is uniform across the workgroup and lives in scalar registers.
-    const uint32_t ucol = (uint32_t)col, uh = (uint32_t)hidx;
+    // Byte offsets within a plane (planes of this sweep hold fewer than 2^29 cells): each access
+    // is a buffer access at the plane's scalar address plus one 32-bit lane offset.  A lane with
+    // no halo cell loads its own column in the halo's place, so every lane issues every load and
+    // the waits before their uses count exactly.
+    const uint32_t ub = (uint32_t)col * 8u, hb = halo ? (uint32_t)hidx * 8u : ub;
     // Plane q clamped to [0, min(z1, nz-1)]: a plane past what the chunk uses is never fetched
     // from HBM, only re-read from a plane the chunk already holds.
-    const int64_t top = std::min(z1, nz - 1);
-    auto at = [&](const double *p, int64_t q) -> const double * {
-        return p + plane * std::min(std::max<int64_t>(q, 0), top);
+    const int top = std::min(z1, nz - 1);
+    auto at = [&](const double *p, int q) -> const double * {
+        return p + plane * std::min(std::max(q, 0), top);
     };
+    auto ld = [&](const double *p, int q, uint32_t off) { return buffer_load_f64(at(p, q), off); };
 
     // Faces on the block's outer boundary are zero (solid.rs:189-195); which ones a thread meets
     // does not change along z.
@@ -208,22 +206,26 @@ __global__ __launch_bounds__(64 * TY) void heat_sweep(HeatArgs a) {
     const bool fxe_on = x0 + TX < nx, fye_on = y0 + TY < ny;
 
     // Rings, indexed by (z - z0) modulo their length.
-    double RT[4], RK[4], HT[2], HK[2], KZ[2];
-    RT[3] = at(T, z0 - 1)[ucol];
-    RT[0] = at(T, z0)[ucol];
-    RT[1] = at(T, z0 + 1)[ucol];
-    RT[2] = at(T, z0 + 2)[ucol];
-    HT[0] = halo ? at(T, z0)[uh] : 0.0;
-    HT[1] = halo ? at(T, z0 + 1)[uh] : 0.0;
+    double RT[4], RK[4], HT[2], HK[2], KZ[2], RM[2], RC[2];
+    if constexpr (CMODE == C_PER_CELL) {
+        RM[0] = ld(a.mob, z0, ub), RM[1] = ld(a.mob, z0 + 1, ub);
+        RC[0] = ld(a.cap, z0, ub), RC[1] = ld(a.cap, z0 + 1, ub);
+    }
+    RT[3] = ld(T, z0 - 1, ub);
+    RT[0] = ld(T, z0, ub);
+    RT[1] = ld(T, z0 + 1, ub);
+    RT[2] = ld(T, z0 + 2, ub);
+    HT[0] = ld(T, z0, hb);
+    HT[1] = ld(T, z0 + 1, hb);
     if constexpr (KC) {
-        const double k_m = at(K, z0 - 1)[ucol];
-        RK[0] = at(K, z0)[ucol];
-        RK[1] = at(K, z0 + 1)[ucol];
-        RK[2] = at(K, z0 + 2)[ucol];
-        RK[3] = at(K, z0 + 3)[ucol];
-        const double h_k0 = halo ? at(K, z0)[uh] : 0.0;
-        HK[0] = halo ? at(K, z0 + 1)[uh] : 0.0;
-        HK[1] = halo ? at(K, z0 + 2)[uh] : 0.0;
+        const double k_m = ld(K, z0 - 1, ub);
+        RK[0] = ld(K, z0, ub);
+        RK[1] = ld(K, z0 + 1, ub);
+        RK[2] = ld(K, z0 + 2, ub);
+        RK[3] = ld(K, z0 + 3, ub);
+        const double h_k0 = ld(K, z0, hb);
+        HK[0] = ld(K, z0 + 1, hb);
+        HK[1] = ld(K, z0 + 2, hb);
         const double lo = series_nz(k_m, RK[0]), hi = series_nz(RK[0], RK[1]);
         KZ[0] = z0 > 0 ? lo : 0.0;
         KZ[1] = z0 + 1 < nz ? hi : 0.0;
@@ -231,7 +233,7 @@ __global__ __launch_bounds__(64 * TY) void heat_sweep(HeatArgs a) {
         sK[1][ty + 1][tx + 1] = RK[0];
         if (halo) sK[1][hr][hc] = h_k0;
         lds_barrier();
-        build_faces<TY>(sK[1], fx[0], fy[0], tx, ty, tid, fxl_on, fyl_on, fxe_on, fye_on);
+        build_faces<TY, SHORT>(sK[1], fx[0], fy[0], tx, ty, tid, fxl_on, fyl_on, fxe_on, fye_on);
     }
 
     // Uniform-form faces depend only on position.
@@ -242,40 +244,36 @@ __global__ __launch_bounds__(64 * TY) void heat_sweep(HeatArgs a) {
     double *__restrict__ Tout = a.tnew;
 
     double acc = 0.0;
-    for (int64_t zb = z0; zb < z1; zb += 4) {
+    for (int zb = z0; zb < z1; zb += 4) {
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             // A chunk whose length is not a multiple of four runs up to three idle planes at its
             // end (loads clamped, nothing stored) so the ring keeps its fixed registers.
-            const int64_t z = zb + i;
+            const int z = zb + i;
             const bool live = z < z1;
             const int b = i & 1;
             const double t_m = RT[(i + 3) & 3], t_c = RT[i], t_p = RT[(i + 1) & 3];
 
             sT[b][ty + 1][tx + 1] = t_c;
-            if (halo) {
-                sT[b][hr][hc] = HT[b];
-                HT[b] = at(T, z + 2)[uh];
-            }
+            if (halo) sT[b][hr][hc] = HT[b];
+            HT[b] = ld(T, z + 2, hb);
             if constexpr (KC) {
                 sK[b][ty + 1][tx + 1] = RK[(i + 1) & 3];
-                if (halo) {
-                    sK[b][hr][hc] = HK[b];
-                    HK[b] = at(K, z + 3)[uh];
-                }
+                if (halo) sK[b][hr][hc] = HK[b];
+                HK[b] = ld(K, z + 3, hb);
             }
             lds_barrier();
 
             double kxl, kxh, kyl, kyh, kzl, kzh;
             if constexpr (KC) {
-                build_faces<TY>(sK[b], fx[b ^ 1], fy[b ^ 1], tx, ty, tid, fxl_on, fyl_on, fxe_on,
-                                fye_on);
+                build_faces<TY, SHORT>(sK[b], fx[b ^ 1], fy[b ^ 1], tx, ty, tid, fxl_on, fyl_on,
+                                       fxe_on, fye_on);
                 kxl = fx[b][ty][tx], kxh = fx[b][ty][tx + 1];
                 kyl = fy[b][ty][tx], kyh = fy[b][ty + 1][tx];
                 kzl = KZ[b], kzh = KZ[b ^ 1];
-                const double zf = series_nz(RK[(i + 1) & 3], RK[(i + 2) & 3]);
-                KZ[b] = (z + 2 < nz) ? zf : 0.0;  // the face between z+1 and z+2
-                RK[i] = at(K, z + 4)[ucol];        // k(z) is no longer needed
+                // the face between z+1 and z+2
+                KZ[b] = series_on<SHORT>(RK[(i + 1) & 3], RK[(i + 2) & 3], z + 2 < nz);
+                RK[i] = ld(K, z + 4, ub);  // k(z) is no longer needed
             } else {
                 kxl = kxl_u, kxh = kxh_u, kyl = kyl_u, kyh = kyh_u;
                 kzl = z > 0 ? a.kface : 0.0;
@@ -292,22 +290,22 @@ __global__ __launch_bounds__(64 * TY) void heat_sweep(HeatArgs a) {
             flux = flux + kyh * (tn - t_c);
             flux = flux + kzl * (t_m - t_c);
             flux = flux + kzh * (t_p - t_c);
-            RT[(i + 3) & 3] = at(T, z + 3)[ucol];  // T(z-1) is no longer needed
+            RT[(i + 3) & 3] = ld(T, z + 3, ub);  // T(z-1) is no longer needed
+            double mob_z = 0.0, cap_z = a.cap0;
+            if constexpr (CMODE == C_PER_CELL) {
+                mob_z = RM[b], cap_z = RC[b];
+                RM[b] = ld(a.mob, z + 2, ub);
+                RC[b] = ld(a.cap, z + 2, ub);
+            }
             if (inside && live) {
-                const int64_t zoff = plane * z;
                 double rise;
                 if constexpr (CMODE == C_PER_CELL)
-                    rise = (a.dts * (a.mob + zoff)[ucol]) * flux;
+                    rise = (a.dts * mob_z) * flux;
                 else
                     rise = a.f * flux;
                 const double t_new = t_c + rise;
-                __builtin_nontemporal_store(t_new, Tout + zoff + ucol);
-                double cap;
-                if constexpr (CMODE == C_PER_CELL)
-                    cap = (a.cap + zoff)[ucol];
-                else
-                    cap = a.cap0;
-                acc += cap * (t_new - a.tref);
+                buffer_store_f64_nt(t_new, Tout + plane * z, ub);
+                acc += cap_z * (t_new - a.tref);
             }
         }
     }
@@ -1000,11 +998,8 @@ static int heat_grid(const pm_solid3d *s, dim3 *grid, dim3 *block, int tile_y = 
 // The two-steps-per-pass sweep exists for per-cell conductivity with uniform capacity, in
 // 64 x 8 tiles.
 constexpr int PAIR_TY = 8;
-// The paired sweep addresses a plane with 32-bit byte offsets: planes of 2^29 cells or more take
-// the one-step sweep.
 static bool pairs_enabled(const pm_solid3d *s) {
-    return s->steps_per_pass == 2 && s->kmode == K_PER_CELL && s->cmode == C_UNIFORM &&
-           (int64_t)s->nx * s->ny < ((int64_t)1 << 29);
+    return s->steps_per_pass == 2 && s->kmode == K_PER_CELL && s->cmode == C_UNIFORM;
 }
 
 static int ensure_partials(pm_solid3d *s) {
@@ -1024,7 +1019,10 @@ static void launch_sweep_pd(const pm_solid3d *s, const HeatArgs &a, dim3 g) {
     // One-dimensional launch: the kernel maps workgroup ids to tiles XCD by XCD.
     const int64_t ntiles = (int64_t)g.x * g.y * g.z;
     const unsigned blocks = (unsigned)(XCDS * ((ntiles + XCDS - 1) / XCDS));
-    heat_sweep<KM, CM, TY><<<blocks, dim3(64, TY), 0, s->stream>>>(a);
+    if (KM == K_PER_CELL && s->k_checked && s->k_short)
+        heat_sweep<KM, CM, TY, true><<<blocks, dim3(64, TY), 0, s->stream>>>(a);
+    else
+        heat_sweep<KM, CM, TY, false><<<blocks, dim3(64, TY), 0, s->stream>>>(a);
 }
 
 template <int KM, int CM>
@@ -1128,7 +1126,7 @@ static int sweep_pair(pm_solid3d *s, double dt, int64_t step, double *totals_dev
     const bool prof = prof_slot(s, launch);
     if (prof) PM_HIP(hipEventRecord(s->prof[3 * launch], s->stream));
     const unsigned blocks = (unsigned)(XCDS * ((ntiles + XCDS - 1) / XCDS));
-    if (s->k_short)
+    if (s->k_checked && s->k_short)
         heat_sweep2<PAIR_TY, true><<<blocks, dim3(64, PAIR_TY), 0, s->stream>>>(a);
     else
         heat_sweep2<PAIR_TY, false><<<blocks, dim3(64, PAIR_TY), 0, s->stream>>>(a);
@@ -1991,7 +1989,8 @@ int pm_solid3d_create(const char *name, const pm_substance *substance, size_t nx
     *out = nullptr;
     PM_CHECK(require_device());
     nx = std::max<size_t>(nx, 1), ny = std::max<size_t>(ny, 1), nz = std::max<size_t>(nz, 1);
-    if (nx > (1u << 30) || ny > (1u << 30) || nz > (1u << 30) || nx * ny > ((size_t)1 << 31)) {
+    // The sweeps address a plane with 32-bit byte offsets: fewer than 2^29 cells a plane.
+    if (nx > (1u << 30) || ny > (1u << 30) || nz > (1u << 30) || nx * ny >= ((size_t)1 << 29)) {
         set_error("pm_solid3d_create: counts (%zu, %zu, %zu) are beyond the sweep's indexing", nx, ny, nz);
         return PM_ERR_ARGUMENT;
     }
@@ -2392,7 +2391,8 @@ int pm_solid3d_advance(pm_solid3d *s, double dt, uint64_t steps, double toleranc
     };
     std::vector<Launch> launches;
     const bool pairs = pairs_enabled(s);
-    if (pairs && steps > 1) PM_CHECK(check_k_range(s));
+    // A two-phase block re-mixes its conductivities every step: it keeps the full division.
+    if (s->kmode == K_PER_CELL && !s->two_phase && steps > 1) PM_CHECK(check_k_range(s));
     PM_HIP(hipEventRecord(s->ev0, s->stream));
     for (uint64_t n = 0; n < steps;) {
         const int64_t l = (int64_t)launches.size();

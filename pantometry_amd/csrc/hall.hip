@@ -227,16 +227,17 @@ __global__ __launch_bounds__(64 * TY, 4) void hall_march2(HallArgs a, int gx, in
     const int64_t bx = tile % gx, by = (tile / gx) % gy, bz = tile / ((int64_t)gx * gy);
 
     const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TX * ty;
-    const int64_t nx = a.nx, ny = a.ny, plane = nx * ny;
-    const int64_t x0 = bx * TX, y0 = by * TY;
+    const int nx = a.nx, ny = a.ny;
+    const int64_t plane = (int64_t)nx * ny;
+    const int x0 = (int)bx * TX, y0 = (int)by * TY;
     // Plane indices are uniform 32-bit values, so their clamps and tests stay on the scalar unit.
     // The paired step is launched only for halls at least two nodes along every axis, so every
     // velocity array has faces.
     const int nz = a.nz;
     const int z0 = (int)bz * zc, z1 = std::min(z0 + zc, nz);
-    const int64_t vxp = (nx - 1) * ny, vyp = nx * (ny - 1);
-    auto cx = [&](int64_t x) { return std::min(std::max<int64_t>(x, 0), nx - 1); };
-    auto cy = [&](int64_t y) { return std::min(std::max<int64_t>(y, 0), ny - 1); };
+    const int64_t vxp = (int64_t)(nx - 1) * ny, vyp = (int64_t)nx * (ny - 1);
+    auto cx = [&](int x) { return std::min(std::max(x, 0), nx - 1); };
+    auto cy = [&](int y) { return std::min(std::max(y, 0), ny - 1); };
     auto kc = [&](int q) { return std::min(std::max(q, 0), nz - 1); };
     auto kz = [&](int q) { return std::min(std::max(q, 0), nz - 2); };
     // A node's byte offsets within a plane of p (and vz), vx (its +x face) and vy (its +y face),
@@ -246,9 +247,9 @@ __global__ __launch_bounds__(64 * TY, 4) void hall_march2(HallArgs a, int gx, in
         int x, y;
         uint32_t p, vx, vy, n;
     };
-    auto column = [&](int64_t x, int64_t y) {
+    auto column = [&](int x, int y) {
         Col c;
-        c.x = (int)x, c.y = (int)y;
+        c.x = x, c.y = y;
         c.p = (uint32_t)(cx(x) + nx * cy(y)) * 8u;
         c.vx = (uint32_t)(std::min(cx(x), nx - 2) + (nx - 1) * cy(y)) * 8u;
         c.vy = (uint32_t)(cx(x) + nx * std::min(cy(y), ny - 2)) * 8u;
@@ -317,7 +318,7 @@ __global__ __launch_bounds__(64 * TY, 4) void hall_march2(HallArgs a, int gx, in
     auto first = [&](int b, int r, int c, const Col &cc, int k, double pm, double pcur,
                      double pp, double vzm, double vzc) {
         Step o;
-        const int64_t x = cc.x, y = cc.y;
+        const int x = cc.x, y = cc.y;
         o.hx = 0.0, o.lx = 0.0, o.hy = 0.0, o.ly = 0.0, o.hz = 0.0, o.lz = 0.0;
         if (x + 1 < nx) o.hx = sVX[b][r][c + 1] - g * (sP[b][r + 1][c + 2] - pcur);
         if (x > 0) o.lx = sVX[b][r][c] - g * (pcur - sP[b][r + 1][c]);

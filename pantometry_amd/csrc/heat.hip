@@ -316,11 +316,12 @@ __global__ __launch_bounds__(64 * TY) void heat_sweep(HeatArgs a) {
 
 // ---- Two steps per pass -------------------------------------------------------------------
 //
-// The per-cell-conductivity, uniform-capacity sweep advanced two steps in one pass over HBM
-// (temporal blocking).  Every cell-step is computed with exactly the arithmetic of heat_sweep,
-// so the fields stay bit-identical; what changes is that T is read and written once per two
-// steps (12 B per cell-step instead of 24), and that each face conductivity, built once per plane
-// in LDS, serves both steps.
+// The per-cell-conductivity sweep (capacity uniform or per cell) advanced two steps in one pass
+// over HBM (temporal blocking).  Every cell-step is computed with exactly the arithmetic of
+// heat_sweep, so the fields stay bit-identical; what changes is that T is read and written once
+// per two steps (12 B per cell-step instead of 24 with uniform capacity; mobility and capacity,
+// when per cell, are read once per two steps too), and that each face conductivity, built once
+// per plane in LDS, serves both steps.
 //
 // A workgroup owns a 64 x TY column tile and a z-chunk.  Per plane z it computes the first step,
 // T1(z), on the tile widened by one cell (E1, 66 x (TY+2)) less its four corners, which no
@@ -339,8 +340,9 @@ __global__ __launch_bounds__(64 * TY) void heat_sweep(HeatArgs a) {
 // waits on another at the plane's one barrier.  LDS: T0(z) and k(z+1) on E2 and T1 on E1,
 // double-buffered, and the faces of planes z-1, z and z+1 on E1.  The audit of each step is summed as its cells are
 // produced: step one over the owned planes of T1, step two over the stored T2.
-template <int TY, bool SHORT>
+template <int TY, bool SHORT, int CMODE>
 __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
+    constexpr bool CC = CMODE == C_PER_CELL;
     constexpr int TX = 64, NT = TX * TY;
     constexpr int E1X = TX + 2, E1Y = TY + 2, E2X = TX + 4, E2Y = TY + 4;
     constexpr int R1 = 2 * (E1X - 2) + 2 * TY;         // ring-1 cells (E1 less the tile, corners)
@@ -466,7 +468,7 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
     // position p (row stride `stride`), the four in-plane faces and the own column.
     auto update = [&](const double *t, int stride, int p, double kxl, double kxh, double kyl,
                       double kyh, double t_m, double t_c, double t_p, double kzl,
-                      double kzh) -> double {
+                      double kzh, double rate) -> double {
         const double tw = t[p - 1], te = t[p + 1], ts = t[p - stride], tn = t[p + stride];
         double flux = 0.0;
         flux = flux + kxl * (tw - t_c);
@@ -475,8 +477,11 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
         flux = flux + kyh * (tn - t_c);
         flux = flux + kzl * (t_m - t_c);
         flux = flux + kzh * (t_p - t_c);
-        return t_c + a.f * flux;
+        return t_c + rate * flux;
     };
+    // The factor of the flux: dt * dx/C, one for the block or (dt * mobility) per cell, as the
+    // one-step sweep forms it.
+    auto rate_of = [&](double mob) { return CC ? a.dts * mob : a.f; };
     // The x faces of E1 cell (r, c) are at r*(E1X+1) + c and + 1, the y faces at
     // FXS + r*E1X + c and FXS + (r+1)*E1X + c.
     const int fxo = (ty + 1) * (E1X + 1) + tx + 1, fyo = FXS + (ty + 1) * E1X + tx + 1;
@@ -508,6 +513,14 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
     QK[0] = QK[1] = QK[2] = QK[3] = 0.0;
     QZ[0] = QZ[1] = QZ[2] = QZ[3] = 0.0;
     double hk0 = 0.0, PT[2] = {0.0, 0.0}, PK[2] = {0.0, 0.0};
+    // Per-cell capacity: mobility and capacity of the tile column at planes z and z-1 (slot
+    // (plane - zs) mod 2) and mobility of the ring-1 column at z and z+1.
+    double RM[2] = {0.0, 0.0}, RC[2] = {0.0, 0.0}, QM[2] = {0.0, 0.0};
+    if constexpr (CC) {
+        RM[0] = ld(a.mob, zs, ub), RM[1] = ld(a.mob, zs - 1, ub);
+        RC[0] = ld(a.cap, zs, ub), RC[1] = ld(a.cap, zs - 1, ub);
+        QM[0] = ld(a.mob, zs, r1b), QM[1] = ld(a.mob, zs + 1, r1b);
+    }
     if (ring1) {
         QT[3] = ld(T, zs - 1, r1b);
         QT[0] = ld(T, zs, r1b);
@@ -604,25 +617,34 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
             const double *f = sF[fb];
             const double t1 = update(&sT[b][0][0], E2X, (ty + 2) * E2X + tx + 2, f[fxo],
                                      f[fxo + 1], f[fyo], f[fyo + E1X], RT[(i + 3) & 3], RT[i],
-                                     RT[(i + 1) & 3], KZ[i], KZ[(i + 1) & 3]);
+                                     RT[(i + 1) & 3], KZ[i], KZ[(i + 1) & 3], rate_of(RM[b]));
             U1[i] = t1;
             s1[b][ty + 1][tx + 1] = t1;
             if (ring1)
                 s1[b][r1r][r1c] = update(&sT[b][0][0], E2X, (r1r + 1) * E2X + r1c + 1, f[r1fx],
                                          f[r1fx + 1], f[r1fy], f[r1fy + E1X], QT[(i + 3) & 3],
-                                         QT[i], QT[(i + 1) & 3], QZ[i], QZ[(i + 1) & 3]);
+                                         QT[i], QT[(i + 1) & 3], QZ[i], QZ[(i + 1) & 3],
+                                         rate_of(QM[b]));
             RT[(i + 3) & 3] = buffer_load_f64(T3, ub);  // T0(z-1) is no longer needed
             QT[(i + 3) & 3] = buffer_load_f64(T3, r1b);
-            if (inside && z >= z0 && z < z1) acc1 += a.cap0 * (t1 - a.tref);
+            if constexpr (CC) QM[b] = ld(a.mob, z + 2, r1b);
+            if (inside && z >= z0 && z < z1) acc1 += (CC ? RC[b] : a.cap0) * (t1 - a.tref);
 
             // Second step: T2(z-1) on the tile, from T1(z-1) staged in the previous iteration.
             const double *g = sF[fbp];
             const double t2 = update(&s1[b ^ 1][0][0], E1X, (ty + 1) * E1X + tx + 1, g[fxo],
                                      g[fxo + 1], g[fyo], g[fyo + E1X], U1[(i + 2) & 3],
-                                     U1[(i + 3) & 3], U1[i], KZ[(i + 3) & 3], KZ[i]);
+                                     U1[(i + 3) & 3], U1[i], KZ[(i + 3) & 3], KZ[i],
+                                     rate_of(RM[b ^ 1]));
+            const double cap_m = CC ? RC[b ^ 1] : a.cap0;
+            if constexpr (CC) {
+                // mobility and capacity of plane z+1, used from the next iteration on
+                RM[b ^ 1] = ld(a.mob, z + 1, ub);
+                RC[b ^ 1] = ld(a.cap, z + 1, ub);
+            }
             if (inside && live && z - 1 >= z0) {
                 buffer_store_f64_nt(t2, Tout + plane * (z - 1), ub);
-                acc2 += a.cap0 * (t2 - a.tref);
+                acc2 += cap_m * (t2 - a.tref);
             }
         }
     }
@@ -995,11 +1017,11 @@ static int heat_grid(const pm_solid3d *s, dim3 *grid, dim3 *block, int tile_y = 
     return PM_OK;
 }
 
-// The two-steps-per-pass sweep exists for per-cell conductivity with uniform capacity, in
-// 64 x 8 tiles.
+// The two-steps-per-pass sweep exists for per-cell conductivity (capacity uniform or per cell),
+// in 64 x 8 tiles.
 constexpr int PAIR_TY = 8;
 static bool pairs_enabled(const pm_solid3d *s) {
-    return s->steps_per_pass == 2 && s->kmode == K_PER_CELL && s->cmode == C_UNIFORM;
+    return s->steps_per_pass == 2 && s->kmode == K_PER_CELL;
 }
 
 static int ensure_partials(pm_solid3d *s) {
@@ -1116,6 +1138,8 @@ static int sweep_pair(pm_solid3d *s, double dt, int64_t step, double *totals_dev
     a.kcell = s->kcell.d();
     a.partials = s->partials.d();
     a.audit = s->audit.as<AuditState>();
+    a.mob = s->cmode == C_PER_CELL ? s->mob.d() : nullptr;
+    a.cap = s->cmode == C_PER_CELL ? s->cap.d() : nullptr;
     a.f = dt * s->mob0;
     a.dts = dt;
     a.cap0 = s->cap0;
@@ -1126,10 +1150,17 @@ static int sweep_pair(pm_solid3d *s, double dt, int64_t step, double *totals_dev
     const bool prof = prof_slot(s, launch);
     if (prof) PM_HIP(hipEventRecord(s->prof[3 * launch], s->stream));
     const unsigned blocks = (unsigned)(XCDS * ((ntiles + XCDS - 1) / XCDS));
-    if (s->k_checked && s->k_short)
-        heat_sweep2<PAIR_TY, true><<<blocks, dim3(64, PAIR_TY), 0, s->stream>>>(a);
-    else
-        heat_sweep2<PAIR_TY, false><<<blocks, dim3(64, PAIR_TY), 0, s->stream>>>(a);
+    const bool quick = s->k_checked && s->k_short;
+    if (s->cmode == C_PER_CELL) {
+        if (quick)
+            heat_sweep2<PAIR_TY, true, C_PER_CELL><<<blocks, dim3(64, PAIR_TY), 0, s->stream>>>(a);
+        else
+            heat_sweep2<PAIR_TY, false, C_PER_CELL><<<blocks, dim3(64, PAIR_TY), 0, s->stream>>>(a);
+    } else if (quick) {
+        heat_sweep2<PAIR_TY, true, C_UNIFORM><<<blocks, dim3(64, PAIR_TY), 0, s->stream>>>(a);
+    } else {
+        heat_sweep2<PAIR_TY, false, C_UNIFORM><<<blocks, dim3(64, PAIR_TY), 0, s->stream>>>(a);
+    }
     PM_HIP(hipGetLastError());
     if (prof) PM_HIP(hipEventRecord(s->prof[3 * launch + 1], s->stream));
     heat_finalize2<<<1, 1024, 0, s->stream>>>(s->partials.d(), ntiles, s->audit.as<AuditState>(),

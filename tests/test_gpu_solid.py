@@ -202,6 +202,32 @@ def test_paired_advance_every_z_chunk(z_chunk, oracle):
     assert np.array_equal(bits(g1.cells()), bits(o.cells))
 
 
+@pytest.mark.parametrize("z_chunk", [0, 3, 8])
+def test_paired_advance_of_a_general_fill(z_chunk, oracle):
+    """A block filled with several substances (conductivity and capacity per cell) advances two
+    steps per pass as well: bits and per-step audit totals against the oracle, over an odd step
+    count (pairs and a trailing single step) and several z-chunks."""
+    gpu_case(f"paired general fill z_chunk {z_chunk}")
+    counts = (70, 37, 23)
+    nx, ny, nz = counts
+    k, t = oracle.heterogeneous_fields(counts, spots=2)
+    mask = np.zeros((nz, ny, nx), dtype=bool)
+    mask[5:15, 10:25, 3:30] = True
+    mask2 = np.zeros((nz, ny, nx), dtype=bool)
+    mask2[:, :, 50:] = True
+    fills = [((2.51e3, 1.114, 858.0), mask), ((8960.0, 401.0, 385.0), mask2)]
+    g, o = _pair(counts, t=t, sub=(2700.0, 167.0, 896.0), fills=fills)
+    g.set_tile(0, z_chunk)
+    assert g.steps_per_pass() == 2
+    dt = 0.9 * o.max_stable_dt()
+    totals, _ = g.advance(dt, 17, tolerance=1e-12)
+    for n in range(17):
+        o.step(dt, threads_=16)
+        ref = o.stored_heat_compensated()
+        assert abs(totals[n] - ref) <= 1e-12 * max(abs(ref), 1e-300), n
+    assert np.array_equal(bits(g.cells()), bits(o.cells))
+
+
 def _extreme_conductivity(counts, rng, low, high):
     """Conductivities 2^e * (1 + u) with e uniform over [low, high), the bounds themselves, and
     zeros (void-like cells), so harmonic faces meet every exponent pairing in the range."""

@@ -120,7 +120,9 @@ int pm_workload_heterogeneous(size_t nx, size_t ny, size_t nz, uint64_t k_seed,
 typedef struct pm_solid3d pm_solid3d;
 
 /* Solid3D::new, solid.rs:377-421: counts forced to at least one, every cell at `initial`, the
- * ledger measured from `initial` (the enthalpy reference). */
+ * ledger measured from `initial` (the enthalpy reference).  PM_ERR_ARGUMENT for a count above
+ * 2^30 or a plane (nx * ny) of 2^29 cells or more: the sweeps address a plane with 32-bit byte
+ * offsets. */
 int pm_solid3d_create(const char *name, const pm_substance *substance, size_t nx, size_t ny,
                       size_t nz, double dx, double initial, pm_solid3d **out);
 void pm_solid3d_destroy(pm_solid3d *solid);

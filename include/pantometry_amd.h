@@ -171,6 +171,17 @@ int pm_solid3d_ledger(pm_solid3d *solid, pm_books *books);
  * per-step audits. */
 int pm_solid3d_advance(pm_solid3d *solid, double dt, uint64_t steps, double tolerance,
                        double *totals, double *elapsed_ms, pm_violation *v);
+/* No counterpart in the reference: pm_solid3d_advance that allocates nothing, for callers that
+ * cannot allocate device memory (tight serving loops, code under HIP graph capture, harnesses
+ * that forbid hipMalloc).  Same kernels, arithmetic, results, per-step totals and breach
+ * behaviour as pm_solid3d_advance; the audit state, the per-workgroup partials (at most 65536 per
+ * launch) and the per-step totals (staged 8192 steps at a time) live in one static device array
+ * per device (590,848 bytes, held until the process ends).  Plain blocks only: a block with a
+ * film, sources, gaps or a phase change, or one whose launches need more partials, is refused
+ * with PM_ERR_ARGUMENT before any device work.  Calls that overlap on one device share that array
+ * and race: run them one at a time per device. */
+int pm_solid3d_advance_static(pm_solid3d *solid, double dt, uint64_t steps, double tolerance,
+                              double *totals, double *elapsed_ms, pm_violation *v);
 /* Domain::checkpoint / Domain::restore, solid.rs:1935-1950. */
 /* An environment a face loses heat to (Environment, crates/pantometry-thermal/src/lib.rs:38-49):
  * ambient temperature in kelvin, convective coefficient in W/m^2/K, and the face's whole area in

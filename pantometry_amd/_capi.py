@@ -78,6 +78,7 @@ SIGNATURES = {
     "pm_solid3d_step": (_I, [_VP, _D, _D, _VIOL]),
     "pm_solid3d_ledger": (_I, [_VP, C.POINTER(pm_books)]),
     "pm_solid3d_advance": (_I, [_VP, _D, _U64, _D, _DP, _DP, _VIOL]),
+    "pm_solid3d_advance_static": (_I, [_VP, _D, _U64, _D, _DP, _DP, _VIOL]),
     "pm_solid3d_checkpoint": (_I, [_VP]),
     "pm_solid3d_losing_from": (_I, [_VP, _I, C.POINTER(pm_environment)]),
     "pm_solid3d_dissipating": (_I, [_VP, _D, C.POINTER(C.c_uint8)]),

@@ -27,6 +27,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -113,6 +114,10 @@ __global__ __launch_bounds__(256) void k_short_check(const double *k, int64_t n,
         bad = bad || !k_short_ok(k[c]);
     if (bad) atomicOr(outside, 1);
 }
+
+// Clears the flag k_short_check sets (a kernel rather than a memset, so a captured graph holds
+// no memset node).
+__global__ void clear_flag(int *flag) { *flag = 0; }
 
 template <int TY, bool SHORT = false>
 __device__ __forceinline__ void build_faces(const double (*sk)[66], double (*fx)[65],
@@ -919,6 +924,7 @@ struct pm_solid3d {
     double heat_capacity_sum = 0.0;
     bool heat_capacity_valid = false;
     // device state
+    int device = 0;  // the device current at creation, which holds every buffer
     hipStream_t stream = nullptr;
     DevBuf t[2];
     int cur = 0;
@@ -1024,15 +1030,46 @@ static bool pairs_enabled(const pm_solid3d *s) {
     return s->steps_per_pass == 2 && s->kmode == K_PER_CELL;
 }
 
-static int ensure_partials(pm_solid3d *s) {
+// Per-workgroup partials one launch of either sweep writes, or the blocks of the stored-heat
+// reduction, whichever is more.
+constexpr int64_t STORED_PER_BLOCK = 16384;
+static int64_t partials_needed(const pm_solid3d *s) {
     dim3 g, b, g2;
     heat_grid(s, &g, &b);
     heat_grid(s, &g2, &b, PAIR_TY);
-    int64_t need = std::max<int64_t>((int64_t)g.x * g.y * g.z, 2 * (int64_t)g2.x * g2.y * g2.z);
+    const int64_t sweep = std::max<int64_t>((int64_t)g.x * g.y * g.z, 2 * (int64_t)g2.x * g2.y * g2.z);
+    return std::max<int64_t>(sweep, (s->n() + STORED_PER_BLOCK - 1) / STORED_PER_BLOCK);
+}
+
+static int ensure_partials(pm_solid3d *s) {
+    const int64_t need = partials_needed(s);
     if (s->partial_count < need || !s->partials) {
         PM_CHECK(s->partials.alloc(sizeof(double) * (size_t)need));
         s->partial_count = need;
     }
+    return PM_OK;
+}
+
+// The device memory an audited run works in besides the cells and the operator: per-workgroup
+// partials, the audit state, the conductivity-range flag, the stored-heat scalar and the per-step
+// totals (null when the caller wants none).  Either the handle's own buffers or the library's
+// static scratch (pm_solid3d_advance_static).
+struct RunSpace {
+    double *partials = nullptr;
+    AuditState *audit = nullptr;
+    int *kflag = nullptr;
+    double *stored = nullptr;
+    double *totals = nullptr;
+};
+
+static int handle_space(pm_solid3d *s, RunSpace *w) {
+    PM_CHECK(ensure_partials(s));
+    if (!s->kflag) PM_CHECK(s->kflag.alloc(sizeof(int)));
+    w->partials = s->partials.d();
+    w->audit = s->audit.as<AuditState>();
+    w->kflag = s->kflag.as<int>();
+    w->stored = s->stored_dev.d();
+    w->totals = nullptr;
     return PM_OK;
 }
 
@@ -1064,9 +1101,8 @@ static bool prof_slot(pm_solid3d *s, int64_t launch) {
 
 // One sweep from t[cur] into t[cur^1] plus the finalize/audit; then swaps.  `launch` indexes the
 // profiling events (-1: none); `force` runs the sweep even after a breach, without the audit.
-static int sweep_once(pm_solid3d *s, double dt, int64_t step, double *totals_dev,
+static int sweep_once(pm_solid3d *s, double dt, int64_t step, const RunSpace &w,
                       int64_t launch = -1, bool force = false) {
-    PM_CHECK(ensure_partials(s));
     dim3 g, b;
     heat_grid(s, &g, &b);
     HeatArgs a{};
@@ -1075,8 +1111,8 @@ static int sweep_once(pm_solid3d *s, double dt, int64_t step, double *totals_dev
     a.kcell = s->kcell.d();
     a.mob = s->mob.d();
     a.cap = s->cap.d();
-    a.partials = s->partials.d();
-    a.audit = s->audit.as<AuditState>();
+    a.partials = w.partials;
+    a.audit = w.audit;
     a.kface = s->kface;
     a.f = dt * s->mob0;
     a.dts = dt;
@@ -1096,9 +1132,8 @@ static int sweep_once(pm_solid3d *s, double dt, int64_t step, double *totals_dev
     PM_HIP(hipGetLastError());
     if (prof) PM_HIP(hipEventRecord(s->prof[3 * launch + 1], s->stream));
     if (!force) {
-        heat_finalize<<<1, 1024, 0, s->stream>>>(s->partials.d(), (int64_t)g.x * g.y * g.z,
-                                                 s->audit.as<AuditState>(), s->lost, s->supplied,
-                                                 step, totals_dev, s->stored_dev.d());
+        heat_finalize<<<1, 1024, 0, s->stream>>>(w.partials, (int64_t)g.x * g.y * g.z, w.audit,
+                                                 s->lost, s->supplied, step, w.totals, w.stored);
         PM_HIP(hipGetLastError());
     }
     if (prof) {
@@ -1112,15 +1147,13 @@ static int sweep_once(pm_solid3d *s, double dt, int64_t step, double *totals_dev
 
 // Finds, once after each change of the conductivity field, whether every conductivity passes
 // k_short_ok, so the paired sweep may build its faces with the unscaled division.
-static int check_k_range(pm_solid3d *s) {
+static int check_k_range(pm_solid3d *s, int *flag) {
     if (s->k_checked) return PM_OK;
-    if (!s->kflag) PM_CHECK(s->kflag.alloc(sizeof(int)));
-    PM_HIP(hipMemsetAsync(s->kflag.d(), 0, sizeof(int), s->stream));
-    k_short_check<<<grid_for(s->n(), 256), 256, 0, s->stream>>>(s->kcell.d(), s->n(),
-                                                                 s->kflag.as<int>());
+    clear_flag<<<1, 1, 0, s->stream>>>(flag);
+    k_short_check<<<grid_for(s->n(), 256), 256, 0, s->stream>>>(s->kcell.d(), s->n(), flag);
     PM_HIP(hipGetLastError());
     int outside = 1;
-    PM_HIP(hipMemcpyAsync(&outside, s->kflag.d(), sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    PM_HIP(hipMemcpyAsync(&outside, flag, sizeof(int), hipMemcpyDeviceToHost, s->stream));
     PM_HIP(hipStreamSynchronize(s->stream));
     s->k_short = outside == 0;
     s->k_checked = true;
@@ -1128,16 +1161,15 @@ static int check_k_range(pm_solid3d *s) {
 }
 
 // Steps `step` and `step + 1` in one pass from t[cur] into t[cur^1], then both audits; swaps.
-static int sweep_pair(pm_solid3d *s, double dt, int64_t step, double *totals_dev, int64_t launch) {
-    PM_CHECK(ensure_partials(s));
+static int sweep_pair(pm_solid3d *s, double dt, int64_t step, const RunSpace &w, int64_t launch) {
     dim3 g, b;
     heat_grid(s, &g, &b, PAIR_TY);
     HeatArgs a{};
     a.told = s->t[s->cur].d();
     a.tnew = s->t[s->cur ^ 1].d();
     a.kcell = s->kcell.d();
-    a.partials = s->partials.d();
-    a.audit = s->audit.as<AuditState>();
+    a.partials = w.partials;
+    a.audit = w.audit;
     a.mob = s->cmode == C_PER_CELL ? s->mob.d() : nullptr;
     a.cap = s->cmode == C_PER_CELL ? s->cap.d() : nullptr;
     a.f = dt * s->mob0;
@@ -1163,8 +1195,8 @@ static int sweep_pair(pm_solid3d *s, double dt, int64_t step, double *totals_dev
     }
     PM_HIP(hipGetLastError());
     if (prof) PM_HIP(hipEventRecord(s->prof[3 * launch + 1], s->stream));
-    heat_finalize2<<<1, 1024, 0, s->stream>>>(s->partials.d(), ntiles, s->audit.as<AuditState>(),
-                                              s->lost, s->supplied, step, totals_dev);
+    heat_finalize2<<<1, 1024, 0, s->stream>>>(w.partials, ntiles, w.audit, s->lost, s->supplied,
+                                              step, w.totals);
     PM_HIP(hipGetLastError());
     if (prof) {
         PM_HIP(hipEventRecord(s->prof[3 * launch + 2], s->stream));
@@ -1981,27 +2013,183 @@ __global__ __launch_bounds__(256) void heat_stored_kernel(const double *T, const
     if (threadIdx.x == 0) partials[blockIdx.x] = total;
 }
 
-int heat_stored(pm_solid3d *s, double *stored) {
+static int heat_stored_in(pm_solid3d *s, const RunSpace &w, double *stored) {
     const int64_t n = s->n();
-    const int64_t per_block = 16384;
-    const int64_t blocks = (n + per_block - 1) / per_block;
-    DevBuf part;
-    PM_CHECK(part.alloc(sizeof(double) * (size_t)blocks));
+    const int64_t blocks = (n + STORED_PER_BLOCK - 1) / STORED_PER_BLOCK;
     heat_stored_kernel<<<(unsigned)blocks, 256, 0, s->stream>>>(
         s->T(), s->cmode == C_PER_CELL ? s->cap.d() : nullptr, s->cap0, s->reference, n,
-        per_block, part.d(), s->phase_args());
+        STORED_PER_BLOCK, w.partials, s->phase_args());
     PM_HIP(hipGetLastError());
-    heat_finalize<<<1, 1024, 0, s->stream>>>(part.d(), blocks, s->audit.as<AuditState>(), 0.0,
-                                             0.0, -1, nullptr, s->stored_dev.d());
+    heat_finalize<<<1, 1024, 0, s->stream>>>(w.partials, blocks, w.audit, 0.0, 0.0, -1, nullptr,
+                                             w.stored);
     PM_HIP(hipGetLastError());
-    PM_HIP(hipMemcpyAsync(stored, s->stored_dev.d(), sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    PM_HIP(hipMemcpyAsync(stored, w.stored, sizeof(double), hipMemcpyDeviceToHost, s->stream));
     PM_HIP(hipStreamSynchronize(s->stream));
     return PM_OK;
 }
 
-static int reset_audit(pm_solid3d *s, double total, double scale, double tol) {
-    audit_reset<<<1, 1, 0, s->stream>>>(s->audit.as<AuditState>(), total, scale, tol);
+int heat_stored(pm_solid3d *s, double *stored) {
+    RunSpace w;
+    PM_CHECK(handle_space(s, &w));
+    return heat_stored_in(s, w, stored);
+}
+
+static int reset_audit(pm_solid3d *s, AuditState *audit, double total, double scale, double tol) {
+    audit_reset<<<1, 1, 0, s->stream>>>(audit, total, scale, tol);
     PM_HIP(hipGetLastError());
+    return PM_OK;
+}
+
+static int reset_audit(pm_solid3d *s, double total, double scale, double tol) {
+    return reset_audit(s, s->audit.as<AuditState>(), total, scale, tol);
+}
+
+// Fills n doubles with v (the totals of steps a breach stopped stay NaN).
+__global__ void fill_nan(double *p, int64_t n) {
+    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        p[i] = __longlong_as_double(0x7FF8000000000000LL);
+}
+
+// The plain branch of advance: steps in launches of two (paired sweep) or one, each launch's
+// audit on the device, in chunks of at most `chunk` steps whose totals are staged in w.totals
+// and copied out after the chunk (a chunk is even unless it is the run's last, so the pairing of
+// steps is the same for any chunk length).  Step indices the kernels see are relative to the
+// chunk; a breach is reported at its index in the run.
+static int advance_plain(pm_solid3d *s, double dt, uint64_t steps, double tolerance,
+                         double *totals, double *elapsed_ms, pm_violation *v, const RunSpace &w,
+                         uint64_t chunk, bool profiled) {
+    PM_CHECK(check_ratio(s, dt, v, 0));
+    // The opening ledger.
+    double stored0 = 0.0;
+    PM_CHECK(heat_stored_in(s, w, &stored0));
+    double total0 = 0.0;
+    total0 += stored0;
+    total0 += s->lost;
+    total0 += -s->supplied;
+    const double scale0 = rmax(rmax(rmax(0.0, std::fabs(stored0)), std::fabs(s->lost)), std::fabs(s->supplied));
+    PM_CHECK(reset_audit(s, w.audit, total0, scale0, tolerance));
+    if (profiled) {
+        while (s->prof.size() < 3 * (size_t)steps) {
+            hipEvent_t ev;
+            PM_HIP(hipEventCreate(&ev));
+            s->prof.push_back(ev);
+        }
+        s->prof_steps.assign((size_t)steps, 0);
+        s->prof_used = 0;
+    }
+    // Launches of a chunk in order: (first step in the chunk, steps, buffer read).
+    struct Launch {
+        int64_t first;
+        int steps;
+        int cur;
+    };
+    std::vector<Launch> launches;
+    const bool pairs = pairs_enabled(s);
+    // A two-phase block re-mixes its conductivities every step: it keeps the full division.
+    if (s->kmode == K_PER_CELL && !s->two_phase && steps > 1) PM_CHECK(check_k_range(s, w.kflag));
+    PM_HIP(hipEventRecord(s->ev0, s->stream));
+    int64_t launch_no = 0;
+    if (steps == 0) PM_HIP(hipEventRecord(s->ev1, s->stream));
+    for (uint64_t base = 0; base < steps; base += chunk) {
+        const uint64_t len = std::min<uint64_t>(chunk, steps - base);
+        RunSpace cw = w;
+        if (!totals) cw.totals = nullptr;
+        if (cw.totals) {
+            fill_nan<<<grid_for((int64_t)len, 256), 256, 0, s->stream>>>(cw.totals, (int64_t)len);
+            PM_HIP(hipGetLastError());
+        }
+        launches.clear();
+        for (uint64_t n = 0; n < len;) {
+            const int64_t l = profiled ? launch_no : -1;
+            launch_no++;
+            if (pairs && n + 1 < len) {
+                launches.push_back({(int64_t)n, 2, s->cur});
+                PM_CHECK(sweep_pair(s, dt, (int64_t)n, cw, l));
+                n += 2;
+            } else {
+                launches.push_back({(int64_t)n, 1, s->cur});
+                PM_CHECK(sweep_once(s, dt, (int64_t)n, cw, l));
+                n += 1;
+            }
+        }
+        if (base + len == steps) PM_HIP(hipEventRecord(s->ev1, s->stream));
+        AuditState st;
+        PM_HIP(hipMemcpyAsync(&st, w.audit, sizeof st, hipMemcpyDeviceToHost, s->stream));
+        if (cw.totals)
+            PM_HIP(hipMemcpyAsync(totals + base, cw.totals, sizeof(double) * len, hipMemcpyDeviceToHost, s->stream));
+        PM_HIP(hipStreamSynchronize(s->stream));
+        if (st.halted) {
+            // Launches after the breach did nothing: the state is what the breaking step made
+            // it, in the buffer its launch wrote.  A breach on the first step of a pair is
+            // recomputed alone from the pair's input, which nothing has overwritten.
+            for (const Launch &l : launches) {
+                if (st.breach_step < l.first || st.breach_step >= l.first + l.steps) continue;
+                s->cur = l.cur;
+                if (l.steps == 2 && st.breach_step == l.first) {
+                    PM_CHECK(sweep_once(s, dt, -1, cw, -1, true));
+                    PM_HIP(hipStreamSynchronize(s->stream));
+                } else {
+                    s->cur ^= 1;
+                }
+                break;
+            }
+            if (totals)  // steps after the breach were not taken
+                for (uint64_t i = base + len; i < steps; i++) totals[i] = NAN;
+            fill_violation(v, "energy", "simulation", st.before, st.after, st.scale, tolerance,
+                           (int64_t)base + st.breach_step);
+            return PM_VIOLATION;
+        }
+    }
+    PM_HIP(hipStreamSynchronize(s->stream));
+    if (elapsed_ms) {
+        float ms = 0.0f;
+        PM_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        *elapsed_ms = ms;
+    }
+    if (profiled) PM_CHECK(collect_profile(s));
+    return PM_OK;
+}
+
+// The static scratch of pm_solid3d_advance_static: the audit state, the conductivity-range flag,
+// the stored-heat scalar, per-workgroup partials and staged per-step totals, laid out at fixed
+// offsets.  One copy per device for the life of the process.
+constexpr int64_t STATIC_PARTIALS = 65536;  // partials of one launch (2 x tiles of a pair)
+constexpr int64_t STATIC_TOTALS = 8192;     // steps per chunk of staged totals (even)
+constexpr size_t STATIC_OFF_AUDIT = 0, STATIC_OFF_FLAG = 256, STATIC_OFF_STORED = 512,
+                 STATIC_OFF_PARTIALS = 1024,
+                 STATIC_OFF_TOTALS = STATIC_OFF_PARTIALS + sizeof(double) * STATIC_PARTIALS;
+constexpr size_t STATIC_BYTES = STATIC_OFF_TOTALS + sizeof(double) * STATIC_TOTALS;
+static_assert(sizeof(AuditState) <= STATIC_OFF_FLAG, "audit state overlaps the flag");
+static_assert(STATIC_TOTALS % 2 == 0, "chunks of totals must keep the pairing of steps");
+
+}  // namespace pm
+
+static __device__ __align__(256) unsigned char scratch[pm::STATIC_BYTES];
+
+namespace pm {
+
+// The device address of `scratch` on each device, looked up once per device with that device
+// current (the name is a host-side stand-in, not an address a kernel may use).
+constexpr int STATIC_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_ptr[STATIC_DEVICES] = {};
+
+static int static_scratch(int device, unsigned char **out) {
+    if (device < 0 || device >= STATIC_DEVICES) {
+        set_error("static scratch: device %d is beyond the %d the library tracks", device, STATIC_DEVICES);
+        return PM_ERR_DEVICE;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        PM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) {
+            set_error("hipGetSymbolAddress returned no address for the static scratch");
+            return PM_ERR_DEVICE;
+        }
+        g_scratch_ptr[device] = static_cast<unsigned char *>(p);
+    }
+    *out = g_scratch_ptr[device];
     return PM_OK;
 }
 
@@ -2039,6 +2227,7 @@ int pm_solid3d_create(const char *name, const pm_substance *substance, size_t nx
         double sum = s->k0 + s->k0;
         s->kface = sum > 0.0 ? 2.0 * s->k0 * s->k0 / sum : 0.0;
     }
+    PM_HIP(hipGetDevice(&s->device));
     PM_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     PM_HIP(hipEventCreate(&s->ev0));
     PM_HIP(hipEventCreate(&s->ev1));
@@ -2329,7 +2518,9 @@ int pm_solid3d_step(pm_solid3d *s, double dt, double gained, pm_violation *v) {
     }
     if (s->extended()) return step_ext(s, dt);
     PM_CHECK(reset_audit(s, 0.0, 0.0, INFINITY));
-    PM_CHECK(sweep_once(s, dt, -1, nullptr));
+    RunSpace w;
+    PM_CHECK(handle_space(s, &w));
+    PM_CHECK(sweep_once(s, dt, -1, w));
     PM_HIP(hipStreamSynchronize(s->stream));
     return PM_OK;
 }
@@ -2390,83 +2581,50 @@ int pm_solid3d_advance(pm_solid3d *s, double dt, uint64_t steps, double toleranc
                        double *totals, double *elapsed_ms, pm_violation *v) {
     if (!s) return PM_ERR_ARGUMENT;
     if (s->extended()) return advance_ext(s, dt, steps, tolerance, totals, elapsed_ms, v);
-    PM_CHECK(check_ratio(s, dt, v, 0));
-    // The opening ledger.
-    double stored0 = 0.0;
-    PM_CHECK(heat_stored(s, &stored0));
-    double total0 = 0.0;
-    total0 += stored0;
-    total0 += s->lost;
-    total0 += -s->supplied;
-    const double scale0 = rmax(rmax(rmax(0.0, std::fabs(stored0)), std::fabs(s->lost)), std::fabs(s->supplied));
-    PM_CHECK(reset_audit(s, total0, scale0, tolerance));
+    RunSpace w;
+    PM_CHECK(handle_space(s, &w));
     DevBuf dtot;
-    if (totals) {
-        PM_CHECK(dtot.alloc(sizeof(double) * (size_t)std::max<uint64_t>(steps, 1)));
-        PM_HIP(hipMemsetAsync(dtot.d(), 0xFF, sizeof(double) * (size_t)std::max<uint64_t>(steps, 1), s->stream));
+    if (totals && steps) {
+        PM_CHECK(dtot.alloc(sizeof(double) * (size_t)steps));
+        w.totals = dtot.d();
     }
-    if (s->profiling) {
-        while (s->prof.size() < 3 * (size_t)steps) {
-            hipEvent_t ev;
-            PM_HIP(hipEventCreate(&ev));
-            s->prof.push_back(ev);
-        }
-        s->prof_steps.assign((size_t)steps, 0);
-        s->prof_used = 0;
+    return advance_plain(s, dt, steps, tolerance, totals, elapsed_ms, v, w,
+                         std::max<uint64_t>(steps, 1), s->profiling);
+}
+
+int pm_solid3d_advance_static(pm_solid3d *s, double dt, uint64_t steps, double tolerance,
+                              double *totals, double *elapsed_ms, pm_violation *v) {
+    if (!s) return PM_ERR_ARGUMENT;
+    if (s->extended()) {
+        set_error("pm_solid3d_advance_static: a block with a film, sources, gaps or a phase "
+                  "change steps through the extended path, which needs per-call buffers; use "
+                  "pm_solid3d_advance");
+        return PM_ERR_ARGUMENT;
     }
-    // Launches in order: (first step, steps, buffer read).
-    struct Launch {
-        int64_t first;
-        int steps;
-        int cur;
-    };
-    std::vector<Launch> launches;
-    const bool pairs = pairs_enabled(s);
-    // A two-phase block re-mixes its conductivities every step: it keeps the full division.
-    if (s->kmode == K_PER_CELL && !s->two_phase && steps > 1) PM_CHECK(check_k_range(s));
-    PM_HIP(hipEventRecord(s->ev0, s->stream));
-    for (uint64_t n = 0; n < steps;) {
-        const int64_t l = (int64_t)launches.size();
-        if (pairs && n + 1 < steps) {
-            launches.push_back({(int64_t)n, 2, s->cur});
-            PM_CHECK(sweep_pair(s, dt, (int64_t)n, totals ? dtot.d() : nullptr, l));
-            n += 2;
-        } else {
-            launches.push_back({(int64_t)n, 1, s->cur});
-            PM_CHECK(sweep_once(s, dt, (int64_t)n, totals ? dtot.d() : nullptr, l));
-            n += 1;
-        }
+    const int64_t need = partials_needed(s);
+    if (need > STATIC_PARTIALS) {
+        set_error("pm_solid3d_advance_static: the block needs %lld audit partials per launch, "
+                  "beyond the static scratch's %lld; use pm_solid3d_advance",
+                  (long long)need, (long long)STATIC_PARTIALS);
+        return PM_ERR_ARGUMENT;
     }
-    PM_HIP(hipEventRecord(s->ev1, s->stream));
-    AuditState st;
-    PM_HIP(hipMemcpyAsync(&st, s->audit.d(), sizeof st, hipMemcpyDeviceToHost, s->stream));
-    if (totals && steps) PM_HIP(hipMemcpyAsync(totals, dtot.d(), sizeof(double) * steps, hipMemcpyDeviceToHost, s->stream));
-    PM_HIP(hipStreamSynchronize(s->stream));
-    if (elapsed_ms) {
-        float ms = 0.0f;
-        PM_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
-        *elapsed_ms = ms;
+    int current = 0;
+    PM_HIP(hipGetDevice(&current));
+    if (current != s->device) PM_HIP(hipSetDevice(s->device));
+    unsigned char *base = nullptr;
+    int rc = static_scratch(s->device, &base);
+    if (rc == PM_OK) {
+        RunSpace w;
+        w.audit = reinterpret_cast<AuditState *>(base + STATIC_OFF_AUDIT);
+        w.kflag = reinterpret_cast<int *>(base + STATIC_OFF_FLAG);
+        w.stored = reinterpret_cast<double *>(base + STATIC_OFF_STORED);
+        w.partials = reinterpret_cast<double *>(base + STATIC_OFF_PARTIALS);
+        w.totals = reinterpret_cast<double *>(base + STATIC_OFF_TOTALS);
+        rc = advance_plain(s, dt, steps, tolerance, totals, elapsed_ms, v, w,
+                           (uint64_t)STATIC_TOTALS, false);
     }
-    if (s->profiling) PM_CHECK(collect_profile(s));
-    if (st.halted) {
-        // Launches after the breach did nothing: the state is what the breaking step made it,
-        // in the buffer its launch wrote.  A breach on the first step of a pair is recomputed
-        // alone from the pair's input, which nothing has overwritten.
-        for (const Launch &l : launches) {
-            if (st.breach_step < l.first || st.breach_step >= l.first + l.steps) continue;
-            s->cur = l.cur;
-            if (l.steps == 2 && st.breach_step == l.first) {
-                PM_CHECK(sweep_once(s, dt, -1, nullptr, -1, true));
-                PM_HIP(hipStreamSynchronize(s->stream));
-            } else {
-                s->cur ^= 1;
-            }
-            break;
-        }
-        fill_violation(v, "energy", "simulation", st.before, st.after, st.scale, tolerance, st.breach_step);
-        return PM_VIOLATION;
-    }
-    return PM_OK;
+    if (current != s->device) (void)hipSetDevice(current);
+    return rc;
 }
 
 int pm_solid3d_losing_from(pm_solid3d *s, int face, const pm_environment *env) {

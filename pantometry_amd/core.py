@@ -335,13 +335,19 @@ class Simulation:
         return report
 
     def _sweep(self, dt: float, multirate: bool) -> Report:
+        """Simulation::sweep, sim.rs:967-1105: each domain in declared order, its substeps, the
+        attribution of its books, and the refusal of a domain that took nothing from a channel
+        an earlier domain of this sweep had already emptied (sim.rs:1012-1098)."""
         now = self.t
         substeps = []
+        # What earlier takers of this sweep moved on each channel, as a magnitude.
+        moved: Dict[str, float] = {}
         for d in self.domains:
             kind = d.kind() if hasattr(d, "kind") else Kind.EVOLVING
             n = substeps_for(dt, d.max_stable_dt(now)) if (multirate and kind == Kind.EVOLVING) else 1
             h = dt / n
             t = now
+            takes_before = dict(self.bus.takers)
             audited = d.books_balance() if hasattr(d, "books_balance") else False
             books_before = d.ledger() if audited else None
             traffic_before = self.bus.traffic() if audited else None
@@ -352,6 +358,18 @@ class Simulation:
             if audited:
                 _attribute(d.name(), books_before, d.ledger(), traffic_before, self.bus.traffic(),
                            self.conservation_tol)
+            # Net amounts this turn (taken - published, summed from zero since the mark), for
+            # each channel this domain took from, in channel-name order.
+            for channel in sorted(self.bus.takers):
+                if self.bus.takers[channel] <= takes_before.get(channel, 0):
+                    continue
+                net = (self.bus.taken_since_mark.get(channel, 0.0)
+                       - self.bus.published_since_mark.get(channel, 0.0))
+                earlier = moved.get(channel, 0.0)
+                if earlier > 0.0 and net == 0.0:
+                    raise Violation(channel, f"{d.name()} (a second domain took from a channel "
+                                    "already emptied)", earlier, net, earlier, 0.0)
+                moved[channel] = moved.get(channel, 0.0) + abs(net)
             substeps.append((d.name(), n))
         residual = 0.0
         for d in self.domains:

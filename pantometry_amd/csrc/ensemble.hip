@@ -361,25 +361,11 @@ static int ens_plan(pm_ensemble *e) {
     return PM_OK;
 }
 
-// Captures `steps` steps starting from buffer `cur` into a graph.
-static int ens_capture(pm_ensemble *e, uint64_t steps) {
-    if (e->graph && e->graph_steps == steps && e->graph_cur == e->cur) return PM_OK;
-    if (e->graph) {
-        (void)hipGraphExecDestroy(e->graph);
-        e->graph = nullptr;
-    }
-    size_t max_lds = 0;
-    for (const pm_member &s : e->spec) {
-        const size_t edge = std::max(s.nx, std::max(s.ny, s.nz));
-        if ((int)edge <= e->resident_edge && resident_bytes(s) <= e->lds_limit) max_lds = std::max(max_lds, resident_bytes(s));
-    }
-    if (max_lds > 0)
-        PM_HIP(hipFuncSetAttribute((const void *)ens_resident, hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds));
-    hipGraph_t g = nullptr;
-    PM_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+// The launches of `steps` steps from buffer c0, recorded into the capture of e->stream: resident
+// members on the side stream (forked and joined by events), tiles on the main stream.
+static int ens_record(pm_ensemble *e, uint64_t steps, size_t max_lds, int c0) {
     PM_HIP(hipEventRecord(e->fork, e->stream));
     PM_HIP(hipStreamWaitEvent(e->side, e->fork, 0));
-    const int c0 = e->cur;
     const int cend = (int)((c0 + steps) & 1);
     if (e->resident_count > 0)
         ens_resident<<<(unsigned)e->resident_count, 1024, max_lds, e->side>>>(
@@ -400,7 +386,36 @@ static int ens_capture(pm_ensemble *e, uint64_t steps) {
     }
     PM_HIP(hipEventRecord(e->join, e->side));
     PM_HIP(hipStreamWaitEvent(e->stream, e->join, 0));
-    PM_HIP(hipStreamEndCapture(e->stream, &g));
+    return PM_OK;
+}
+
+// Captures `steps` steps starting from buffer `cur` into a graph.
+static int ens_capture(pm_ensemble *e, uint64_t steps) {
+    if (e->graph && e->graph_steps == steps && e->graph_cur == e->cur) return PM_OK;
+    if (e->graph) {
+        (void)hipGraphExecDestroy(e->graph);
+        e->graph = nullptr;
+    }
+    size_t max_lds = 0;
+    for (const pm_member &s : e->spec) {
+        const size_t edge = std::max(s.nx, std::max(s.ny, s.nz));
+        if ((int)edge <= e->resident_edge && resident_bytes(s) <= e->lds_limit) max_lds = std::max(max_lds, resident_bytes(s));
+    }
+    if (max_lds > 0)
+        PM_HIP(hipFuncSetAttribute((const void *)ens_resident, hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds));
+    hipGraph_t g = nullptr;
+    const int c0 = e->cur;
+    PM_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    // Whatever the recording returns, the capture is ended (and a partial graph destroyed), so a
+    // failure leaves the stream usable and reports its own error.
+    const int rc = ens_record(e, steps, max_lds, c0);
+    const hipError_t end = hipStreamEndCapture(e->stream, &g);
+    if (rc != PM_OK || end != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (rc != PM_OK) return rc;
+        set_error("hipStreamEndCapture failed: %s", hipGetErrorString(end));
+        return PM_ERR_DEVICE;
+    }
     hipError_t err = hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (err != hipSuccess) {

@@ -77,9 +77,11 @@ __device__ __forceinline__ double series_nz(double a, double b) {
 // n / d by the steps of the IEEE double division (reciprocal, two Newton steps, one residual
 // correction) without its operand scaling and special-case fix-up.  Those two leave the operands
 // and the quotient unchanged when d is a positive normal number, n a positive number of at least
-// 2^-969 whose exponent exceeds d's by less than 768, and n / d is normal, so there the result is
-// the correctly rounded quotient bit for bit; for n = +0 and a positive normal d it is +0, as the
-// division's is.
+// 2^DIV_N_MIN_EXP whose exponent exceeds d's by less than DIV_MAX_GAP, and n / d is normal, so
+// there the result is the correctly rounded quotient bit for bit; for n = +0 and a positive
+// normal d it is +0, as the division's is.
+constexpr int DIV_N_MIN_EXP = -969;
+constexpr int DIV_MAX_GAP = 768;
 __device__ __forceinline__ double div_unscaled(double n, double d) {
     double r = __builtin_amdgcn_rcp(d);
     double e = __builtin_fma(-d, r, 1.0);
@@ -92,10 +94,22 @@ __device__ __forceinline__ double div_unscaled(double n, double d) {
 }
 
 // Conductivities for which every harmonic face mean may use div_unscaled: +0, or in
-// [2^-190, 2^190].  Two such values a, b > 0 give 2ab in [2^-379, 2^381] and a + b in
-// [2^-189, 2^191], well inside the range above; a = +0 gives 2ab = +0.
+// [2^K_SHORT_LO_EXP, 2^K_SHORT_HI_EXP].
+constexpr int K_SHORT_LO_EXP = -190, K_SHORT_HI_EXP = 190;
+constexpr double K_SHORT_LO = 0x1p-190, K_SHORT_HI = 0x1p190;
+static_assert(K_SHORT_LO * 0x1p190 == 1.0 && K_SHORT_HI * 0x1p-190 == 1.0,
+              "the bounds are 2^K_SHORT_LO_EXP and 2^K_SHORT_HI_EXP");
+// For a, b in that range the numerator 2ab lies in [2^(1+2 lo), 2^(1+2 hi)], the denominator
+// a + b in [2^(lo+1), 2^(hi+1)], and the quotient (the harmonic mean, between min(a, b) and
+// 2 min(a, b)) in [2^lo, 2^(hi+1)]; so n's exponent exceeds d's by at most hi + 2.  These are the
+// div_unscaled preconditions, and a = +0 gives 2ab = +0.
+static_assert(1 + 2 * K_SHORT_LO_EXP >= DIV_N_MIN_EXP, "numerator below div_unscaled's range");
+static_assert(1 + 2 * K_SHORT_HI_EXP < 1024, "numerator overflows");
+static_assert(K_SHORT_LO_EXP + 1 >= -1022 && K_SHORT_HI_EXP + 1 < 1024, "denominator not normal");
+static_assert(K_SHORT_LO_EXP >= -1022 && K_SHORT_HI_EXP + 1 < 1024, "quotient not normal");
+static_assert(K_SHORT_HI_EXP + 2 < DIV_MAX_GAP, "exponent gap beyond div_unscaled's range");
 __device__ __forceinline__ bool k_short_ok(double v) {
-    return __double_as_longlong(v) == 0 || (v >= 0x1p-190 && v <= 0x1p190);
+    return __double_as_longlong(v) == 0 || (v >= K_SHORT_LO && v <= K_SHORT_HI);
 }
 
 // series_nz(a, b) for a face that exists (`on`), else 0, with one select for both guards; with

@@ -388,3 +388,18 @@ def test_the_heterogeneous_block_at_512(oracle):
     g.set_cells(start)
     g.advance(dt, 2000, tolerance=1e-12)
     assert np.array_equal(bits(g.cells()), bits(first))
+
+
+@pytest.mark.parametrize("counts", [(1 << 15, 1 << 14, 1), (1, 1, (1 << 30) + 1)], ids=str)
+def test_counts_beyond_the_sweeps_indexing_are_refused(counts):
+    """The sweeps address a plane with 32-bit byte offsets: planes of 2^29 cells or more, and
+    extents above 2^30, are refused at creation, before any device allocation."""
+    import ctypes as C
+    import pantometry_amd as pm
+    gpu_case(f"refused counts {counts}")
+    handle = C.c_void_p()
+    rc = pm.lib().pm_solid3d_create(b"huge", C.byref(pm.Substance("s", 4000.0, 1.0, 1000.0)._c()),
+                                    counts[0], counts[1], counts[2], 1e-3, 300.0, C.byref(handle))
+    assert rc == 3  # PM_ERR_ARGUMENT
+    assert not handle.value
+    assert b"indexing" in pm.lib().pm_last_error()

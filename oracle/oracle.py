@@ -47,6 +47,7 @@ _SIG = {
     "orc_solid_bus_heat": (None, [_S, _DP, _D, _DP]),
     "orc_stored_heat": (_D, [_S, _DP, _DP, _D]),
     "orc_stored_heat_compensated": (_D, [_S, _DP, _DP, _D]),
+    "orc_stored_heat_compensated_omp": (_D, [_S, _DP, _DP, _D, _I]),
     "orc_release_mode": (None, [_S, _S, _S, _I, _I, _I, _D, _D, _DP]),
     "orc_mode_amplitude": (_D, [_S, _S, _S, _I, _I, _I, _DP]),
     "orc_mode_amplification": (_D, [_S, _S, _S, _I, _I, _I, _D, _D, _D, _D, _D]),
@@ -511,7 +512,10 @@ class OracleSolid3D:
             self.melted = self.saved_melted.copy()
         self.lost, self.supplied = getattr(self, "saved_books", (0.0, 0.0))
 
-    def stored_heat_compensated(self) -> float:
+    def stored_heat_compensated(self, threads_: int = 1) -> float:
+        if threads_ > 1:
+            return lib().orc_stored_heat_compensated_omp(self.n, p(self.cap), p(self.cells),
+                                                         self.reference, threads_)
         return lib().orc_stored_heat_compensated(self.n, p(self.cap), p(self.cells),
                                                  self.reference)
 

@@ -49,6 +49,30 @@ class OracleRoom:
     def startup(self):
         return self.h.offset
 
+    def pressure(self):
+        return self.h.p.copy()
+
+    def mode_frequency(self, mode):
+        return mode_frequency(self, mode)
+
+    def modes_below(self, limit):
+        """Hall::modes_below, hall.rs:286-316, over the oracle's node counts."""
+        nx, ny, nz = self.nodes()
+        dx = self.spacing()
+        bounds = [int(math.floor(2.0 * limit * ((n - 1) * dx) / self.speed)) if n > 1 else 0
+                  for n in (nx, ny, nz)]
+        found = []
+        for a in range(bounds[0] + 1):
+            for b in range(bounds[1] + 1):
+                for c in range(bounds[2] + 1):
+                    if a == b == c == 0:
+                        continue
+                    f = mode_frequency(self, (a, b, c))
+                    if f < limit:
+                        found.append(((a, b, c), f))
+        found.sort(key=lambda m: m[1])
+        return found
+
 
 class GpuRoom:
     def __init__(self, width, height, depth, across, speed=343.0, density=1.204):
@@ -86,6 +110,15 @@ class GpuRoom:
 
     def startup(self):
         return self.h.startup_adjustment()
+
+    def pressure(self):
+        return self.h.fields()[0]
+
+    def mode_frequency(self, mode):
+        return self.h.mode_frequency(mode)
+
+    def modes_below(self, limit):
+        return self.h.modes_below(limit)
 
 
 def hall(make, across):
@@ -180,9 +213,68 @@ def collapsing_an_axis_reduces_to_the_lower_dimension(make):
     assert max(peaks) <= 1.0 + 1e-9
 
 
+def the_mode_frequencies_are_the_closed_form(make):
+    """hall_closed_forms.rs:41-71: (c/2) sqrt((a/Lx)^2 + (b/Ly)^2 + (c/Lz)^2) including the
+    vertical mode c/2Lz; and each mode, released and stepped, reaches its first zero at a quarter
+    of that period (the projection onto the mode shape, interpolated between steps, within 1%)."""
+    h = hall(make, 45)
+    lx, ly, lz = 4.4, 3.1, 2.4
+
+    def want(a, b, c):
+        return 343.0 / 2.0 * math.sqrt((a / lx) ** 2 + (b / ly) ** 2 + (c / lz) ** 2)
+
+    for mode in [(1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 1), (3, 2, 1)]:
+        assert abs(h.mode_frequency(mode) - want(*mode)) < 1e-9, mode
+    vertical = h.mode_frequency((0, 0, 1))
+    assert abs(vertical - 343.0 / (2.0 * lz)) < 1e-9 and 60.0 < vertical < 85.0
+    nx, ny, nz = h.nodes()
+    for mode in [(1, 0, 0), (0, 0, 1), (3, 2, 1)]:
+        g = hall(make, 45)
+        g.release_mode(mode, 1.0)
+        shape = np.einsum("k,j,i->kji", *[np.cos(m * math.pi * np.arange(n) / (n - 1)) if n > 1
+                                          else np.ones(1) for m, n in zip(mode[::-1], (nz, ny, nx))])
+        shape = shape.reshape(-1)
+        norm = float(shape @ shape)
+        dt = g.max_stable_dt() * 0.5
+        quarter = 1.0 / (4.0 * want(*mode))
+        t, prev = 0.0, 1.0
+        while True:
+            g.step(dt)
+            t += dt
+            amp = float(g.pressure().reshape(-1) @ shape) / norm
+            if amp <= 0.0:
+                crossing = t - dt * amp / (amp - prev)
+                break
+            prev = amp
+            assert t < 2.0 * quarter, mode
+        assert abs(crossing / quarter - 1.0) < 0.01, (mode, crossing, quarter)
+
+
+def the_mode_count_grows_as_the_cube_of_frequency(make):
+    """hall_closed_forms.rs:284-325: modes below 150 and 300 Hz, their ratio, the three-term Weyl
+    estimate within 10%, the leading term alone 20% short, ascending order, (1,0,0) first."""
+    h = hall(make, 45)
+    low, high = len(h.modes_below(150.0)), len(h.modes_below(300.0))
+    assert low > 10
+    assert 5.5 <= high / low < 10.0
+    lx, ly, lz = 4.4, 3.1, 2.4
+    v = lx * ly * lz
+    surface = 2.0 * (lx * ly + ly * lz + lz * lx)
+    edges = 4.0 * (lx + ly + lz)
+    k = 300.0 / 343.0
+    weyl = 4.0 * math.pi / 3.0 * v * k ** 3 + math.pi / 4.0 * surface * k ** 2 + edges * k / 8.0
+    assert 0.9 <= high / weyl < 1.1, (high, weyl)
+    assert high / (4.0 * math.pi / 3.0 * v * k ** 3) > 1.2
+    modes = h.modes_below(120.0)
+    assert all(a[1] <= b[1] for a, b in zip(modes, modes[1:]))
+    assert tuple(modes[0][0]) == (1, 0, 0)
+
+
 HALL_CASES = [
     a_released_mode_tracks_the_cosine_at_second_order,
     a_rigid_hall_conserves_its_invariant_and_reports_its_startup,
     the_courant_limit_is_the_diagonal_of_a_cube,
     collapsing_an_axis_reduces_to_the_lower_dimension,
+    the_mode_frequencies_are_the_closed_form,
+    the_mode_count_grows_as_the_cube_of_frequency,
 ]

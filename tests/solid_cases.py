@@ -70,6 +70,69 @@ class OracleBlock:
         nx, ny, _ = self.b.counts_
         self.b.cells[i + nx * (j + ny * k)] = t
 
+    def temperature_at(self, i, j, k):
+        return self.b.temperature_at(i, j, k)
+
+    def peak(self):
+        return float(np.max(self.b.cells))
+
+    def coldest(self):
+        return float(np.min(self.b.cells))
+
+    def stability_ratio(self, dt):
+        return dt * self.b.worst_rate_now()
+
+    def heat_capacity(self):
+        total = 0.0
+        for c in self.b.cap:
+            total += float(c)
+        return total
+
+    def faces_z(self):
+        """Conductances (W/K) of the z faces of column (0, 0), from its first cell centre to its
+        last, as resolve built them (Solid3D::face_conductance, solid.rs:564-581)."""
+        nx, ny, nz = self.b.counts_
+        return [float(self.b.kz[nx * ny * k]) * self.b.dx for k in range(1, nz)]
+
+    def domain(self):
+        return OracleDomain(self.b)
+
+
+class OracleDomain:
+    """The oracle block behind the Domain protocol the host Simulation drives: it takes its
+    share of the heat channel before each step, as Solid3D::step does (solid.rs:1752)."""
+
+    def __init__(self, b):
+        self.b = b
+        self.absorbed = 0.0
+
+    def name(self):
+        return "block"
+
+    def kind(self):
+        from pantometry_amd.core import Kind
+        return Kind.EVOLVING
+
+    def max_stable_dt(self, now=0.0):
+        return self.b.max_stable_dt()
+
+    def books_balance(self):
+        return True
+
+    def step(self, t, dt, bus):
+        from pantometry_amd.core import HEAT, Violation
+        why = self.b.refuses(dt)
+        if why:
+            raise Violation.at("block", why, dt)
+        gained = bus.take_share(HEAT, dt)
+        self.absorbed += gained
+        self.b.step(dt, gained)
+
+    def ledger(self):
+        from pantometry_amd.core import ENERGY, Ledger
+        return (Ledger().with_(ENERGY, self.b.stored_heat()).with_(ENERGY, self.b.lost)
+                .with_(ENERGY, -self.b.supplied))
+
 
 class GpuBlock:
     def __init__(self, counts, dx_mm, sub=ALU, initial=celsius(20.0)):
@@ -78,6 +141,9 @@ class GpuBlock:
         s = pm.Substance("s", *sub)
         self.b = pm.Solid3D("block", s, counts, dx_mm * 1e-3, initial)
         self.sub = sub
+        self.dx = dx_mm * 1e-3
+        n = counts[0] * counts[1] * counts[2]
+        self.layout = [sub] * n  # each cell's substance, for the face probe
 
     def release_mode(self, mode, mean, amp):
         self.b.release_mode(mode, mean, amp)
@@ -108,6 +174,8 @@ class GpuBlock:
 
     def fill(self, sub, mask):
         self.b.fill(self.pm.Substance("t", *sub), mask)
+        for c in np.flatnonzero(np.asarray(mask).reshape(-1)):
+            self.layout[c] = sub
 
     def face_z(self, k):
         return None
@@ -117,6 +185,50 @@ class GpuBlock:
 
     def set_temperature(self, i, j, k, t):
         self.b.set_temperature(i, j, k, t)
+
+    def temperature_at(self, i, j, k):
+        return self.b.temperature_at(i, j, k)
+
+    def peak(self):
+        return self.b.peak_temperature()
+
+    def coldest(self):
+        return self.b.coldest_temperature()
+
+    def stability_ratio(self, dt):
+        return self.b.stability_ratio(dt)
+
+    def heat_capacity(self):
+        return self.b.heat_capacity()
+
+    def faces_z(self):
+        """The z faces' conductances of a one-cell column, measured through the GPU sweep: with
+        the cells at 0 K except every third cell at 1 K, one step moves each cold cell next to a
+        warm one by (dt*dx/C)*k_face*1 (its other neighbour is cold), so the step reads back every
+        face conductivity k_face; three offsets cover all faces.  Conductance = k_face * dx."""
+        nx, ny, nz = self.b.counts()
+        assert nx == 1 and ny == 1, "the probe reads a one-cell column"
+        saved = self.b.cells().copy()
+        dt = 0.5 * self.b.max_stable_dt()
+        dx = self.dx
+        f = [dt * (dx / ((s[0] * ((dx * dx) * dx)) * s[2])) for s in self.layout]
+        k = [None] * nz
+        for off in range(3):
+            t = np.zeros(nz)
+            t[off::3] = 1.0
+            self.b.set_cells(t.reshape(nz, 1, 1))
+            self.b.step(0.0, dt, self.pm.Exchange())
+            out = self.b.cells().reshape(-1)
+            for j in range(off, nz, 3):
+                if j + 1 < nz:
+                    k[j + 1] = float(out[j + 1]) / f[j + 1]
+                if j >= 1:
+                    k[j] = float(out[j - 1]) / f[j - 1]
+        self.b.set_cells(saved)
+        return [k[j] * dx for j in range(1, nz)]
+
+    def domain(self):
+        return self.b
 
 
 def run(block, dt, steps):
@@ -305,6 +417,225 @@ def a_layered_wall_has_the_series_resistance_of_its_layers(make):
         assert abs(got / closed - 1.0) < 1e-14
 
 
+def a_laminate_across_its_layers_is_exactly_the_reuss_bound(make):
+    """a_composite.rs:115-150: alternate one-cell layers of aluminium and glass along z; the chain
+    of face resistances gives k = (n-1)/(R dx), the Reuss bound, to 1e-15 at every resolution."""
+    dx = 1e-3
+    reuss = 1.0 / (0.5 / ALU[1] + 0.5 / GLASS[1])
+    for cells in [8, 16, 32, 64]:
+        b = make((1, 1, cells), 1.0)
+        mask = np.zeros((cells, 1, 1), dtype=bool)
+        mask[1::2, 0, 0] = True
+        b.fill(GLASS, mask)
+        resistance = 0.0
+        for g in b.faces_z():
+            resistance += 1.0 / g
+        measured = (cells - 1) / (resistance * 1 * dx)
+        assert abs(measured / reuss - 1.0) < 1e-15, (cells, measured, reuss)
+
+
+def arithmetic_averaging_is_first_order_where_harmonic_averaging_is_exact(make):
+    """a_layered_wall.rs:171-245: the wall's chain equals the layers' resistance exactly, while
+    an arithmetic interface face is wrong by exactly 2A/(n-1) with A = (1/H - 1/Am)/(1/ka + 1/kg)."""
+    dx = 1e-3
+    area = dx * dx
+    errors = []
+    for cells in [12, 24, 48, 96]:
+        half = cells // 2
+        closed = (half - 0.5) * dx / (ALU[1] * area) + (cells - half - 0.5) * dx / (GLASS[1] * area)
+        b = make((1, 1, cells), 1.0)
+        mask = np.zeros((cells, 1, 1), dtype=bool)
+        mask[half:, 0, 0] = True
+        b.fill(GLASS, mask)
+        chain = 0.0
+        for g in b.faces_z():
+            chain += 1.0 / g
+        assert abs(chain / closed - 1.0) < 1e-14, (cells, chain, closed)
+        per = lambda k: dx / (k * area)
+        arithmetic = ((half - 1.0) * per(ALU[1]) + (cells - half - 1.0) * per(GLASS[1])
+                      + per(0.5 * (ALU[1] + GLASS[1])))
+        errors.append(abs(arithmetic / closed - 1.0))
+    assert errors[0] > 0.08 and errors[3] > 0.005
+    harmonic = 2.0 * ALU[1] * GLASS[1] / (ALU[1] + GLASS[1])
+    want = 2.0 * (1.0 / harmonic - 2.0 / (ALU[1] + GLASS[1])) / (1.0 / ALU[1] + 1.0 / GLASS[1])
+    for n, e in zip([12.0, 24.0, 48.0, 96.0], errors):
+        assert abs(e * (n - 1.0) / want - 1.0) < 1e-12
+
+
+def the_swept_steady_state_lands_on_the_layers_resistance(make):
+    """a_layered_wall.rs:248-360: 24 cells, aluminium then glass, ends held at 90 C and 20 C for
+    400 s (fifteen of the glass half's time constants); the flux on an aluminium face is the drop
+    over the chain of resistances to 1e-4, every cell centre sits on that line to 1e-4 K, and
+    the metal holds k_g/(k_g + k_a) of the drop."""
+    cells = 24
+    b = make((1, 1, cells), 1.0)
+    mask = np.zeros((cells, 1, 1), dtype=bool)
+    mask[cells // 2:, 0, 0] = True
+    b.fill(GLASS, mask)
+    hot, cold = celsius(90.0), celsius(20.0)
+    faces = b.faces_z()
+    closed = 0.0
+    for g in faces:
+        closed += 1.0 / g
+    dt = b.max_stable_dt() * 0.9
+    steps = int(400.0 / dt)
+    for _ in range(steps):
+        b.set_temperature(0, 0, 0, hot)
+        b.set_temperature(0, 0, cells - 1, cold)
+        b.step(dt)
+    b.set_temperature(0, 0, 0, hot)
+    b.set_temperature(0, 0, cells - 1, cold)
+    drop = hot - cold
+    flux = faces[5] * (b.temperature_at(0, 0, 5) - b.temperature_at(0, 0, 6))
+    want = drop / closed
+    assert abs(flux / want - 1.0) < 1e-4, (flux, want)
+    worst, running = 0.0, 0.0
+    for k in range(1, cells):
+        running += 1.0 / faces[k - 1]
+        worst = max(worst, abs(b.temperature_at(0, 0, k) - (hot - want * running)))
+    assert worst < 1e-4, worst
+    alu_path = (cells / 2.0 - 0.5) * 1e-3 / (ALU[1] * 1e-6)
+    held = (want * alu_path) / drop
+    assert abs(held - GLASS[1] / (GLASS[1] + ALU[1])) < 1e-9
+    assert 0.5e-3 / (GLASS[1] * 1e-6) * want / (want * alu_path) > 6.0
+
+
+def the_stable_step_is_set_by_the_axes_that_have_cells(make):
+    """a_layered_wall.rs:460-511: dx^2/(2 axes alpha) for one, two and three axes with cells;
+    the sharpest mode amplifies by -1 there and the guarded ratio is one; a single cell has an
+    infinite limit and a zero ratio."""
+    n = 8
+    alpha = ALU[1] / (ALU[0] * ALU[2])
+    dx = 2e-3
+    for counts, axes, mode in [((n, 1, 1), 1.0, (n, 0, 0)), ((n, n, 1), 2.0, (n, n, 0)),
+                               ((n, n, n), 3.0, (n, n, n))]:
+        b = make(counts, 2.0)
+        limit = b.max_stable_dt()
+        closed = dx * dx / (2.0 * axes * alpha)
+        assert abs(limit / closed - 1.0) < 1e-12, (counts, limit, closed)
+        assert abs(b.mode_amplification(mode, limit) + 1.0) < 1e-12
+        assert abs(b.stability_ratio(limit) - 1.0) < 1e-12
+    one = make((1, 1, 1), 2.0)
+    assert math.isinf(one.max_stable_dt())
+    assert one.stability_ratio(1.0) == 0.0
+
+
+def an_inclusion_is_limited_by_its_faces_and_not_by_its_own_diffusivity(make):
+    """a_layered_wall.rs:529-605: one aluminium cell in 7^3 glass is stable at ALU/k_face times
+    aluminium's own dx^2/(6 alpha), exactly rho c dx^2/(6 k_face); a thousand steps at the limit
+    relax a deposit without undershoot; 5% past it is refused by 1.05."""
+    n = 7
+    b = make((n, n, n), 1.0, sub=GLASS)
+    mask = np.zeros((n, n, n), dtype=bool)
+    mask[n // 2, n // 2, n // 2] = True
+    b.fill(ALU, mask)
+    dx = 1e-3
+    naive = dx * dx / (6.0 * (ALU[1] / (ALU[0] * ALU[2])))
+    limit = b.max_stable_dt()
+    face = 2.0 * ALU[1] * GLASS[1] / (ALU[1] + GLASS[1])
+    assert abs(limit / naive - ALU[1] / face) < 1e-9 * (ALU[1] / face)
+    want = ALU[0] * ALU[2] * dx * dx / (6.0 * face)
+    assert abs(limit / want - 1.0) < 1e-12, (limit, want)
+    b.deposit(n // 2, n // 2, n // 2, 0.02)
+    start = b.peak()
+    run(b, limit, 1000)
+    end = b.peak()
+    assert celsius(20.0) < end < start
+    try:
+        b.step(limit * 1.05)
+        raise AssertionError("5% past the limit must be refused")
+    except Refused as r:
+        assert r.quantity == "Fourier number"
+        assert abs(r.after / (1.0 / 6.0) - 1.05) < 1e-9
+
+
+def a_cell_of_something_unknown_stops_the_sweep_rather_than_spreading_nan(make):
+    """a_layered_wall.rs:614-640: one cell of a substance with no thermal properties refuses the
+    step with 'substance has no diffusivity' and writes nothing."""
+    b = make((4, 4, 4), 1.0)
+    assert math.isfinite(b.max_stable_dt())
+    mask = np.zeros((4, 4, 4), dtype=bool)
+    mask[1, 1, 1] = True
+    b.fill((2000.0, math.nan, math.nan), mask)
+    try:
+        b.step(1e-9)
+        raise AssertionError("one unknown cell is enough to refuse")
+    except Refused as r:
+        assert r.quantity == "substance has no diffusivity"
+    for k in range(4):
+        assert math.isfinite(b.temperature_at(1, 1, k))
+
+
+def a_one_cell_coating_adds_its_own_resistance_and_no_more(make):
+    """a_layered_wall.rs:652-695: 21 cells of 0.1 mm aluminium, the middle one glass: the chain
+    grows by dx/(k_g A) - dx/(k_a A) to 1e-13."""
+    cells = 21
+    dx = 1e-4
+    area = dx * dx
+
+    def chain(coated):
+        b = make((1, 1, cells), 0.1)
+        if coated:
+            mask = np.zeros((cells, 1, 1), dtype=bool)
+            mask[cells // 2, 0, 0] = True
+            b.fill(GLASS, mask)
+        total = 0.0
+        for g in b.faces_z():
+            total += 1.0 / g
+        return total
+
+    added = chain(True) - chain(False)
+    want = dx / (GLASS[1] * area) - dx / (ALU[1] * area)
+    assert abs(added / want - 1.0) < 1e-13, (added, want)
+
+
+def the_rate_converges_to_the_continuum_at_second_order(make):
+    """solid_closed_forms.rs:122-148: the slowest mode's discrete decay rate -ln(g)/dt against
+    alpha pi^2/L^2 quarters its error from 8 to 16 cells and is within 1% at 16."""
+    length = 20e-3
+    alpha = ALU[1] / (ALU[0] * ALU[2])
+    exact = alpha * math.pi ** 2 / (length * length)
+
+    def error_at(n):
+        b = make((n, 1, 1), length / n * 1e3)
+        dt = 1e-6
+        rate = -math.log(b.mode_amplification((1, 0, 0), dt)) / dt
+        return abs(rate / exact - 1.0)
+
+    coarse, fine = error_at(8), error_at(16)
+    assert abs(coarse / fine - 4.0) < 0.15, (coarse, fine)
+    assert fine < 0.01
+
+
+def it_runs_and_audits_inside_a_simulation(make):
+    """solid_closed_forms.rs:418-491: an 8 W source with a reserve and a 6^3 block under
+    Multirate for 0.5 s (about fifty of the block's limits): the audit holds, the block absorbs
+    8 x 0.5 J to 1e-9, stays uniform to 1e-9 K and rises by paid / capacity to 1e-9."""
+    from pantometry_amd.core import Schedule, Simulation
+    from pantometry_amd.scene import Heater
+    seconds, watts = 0.5, 8.0
+    b = make((6, 6, 6), 2.0)
+    dom = b.domain()
+    sim = Simulation(Schedule.MULTIRATE)
+    sim.with_(Heater("lamp", watts, watts * seconds * 10.0)).with_(dom)
+    limit = b.max_stable_dt()
+    report = sim.advance(seconds)
+    assert dict(report.substeps)["block"] == math.ceil(seconds / limit) > 1
+    paid = watts * seconds
+    absorbed = dom.absorbed_energy() if callable(getattr(dom, "absorbed_energy", None)) else dom.absorbed
+    assert abs(absorbed - paid) < 1e-9 * paid
+    assert b.peak() - b.coldest() < 1e-9
+    volume = 6 * 6 * 6 * (2e-3 ** 3)
+    capacity = (ALU[0] * volume) * ALU[2]
+    g = b.grid().reshape(-1)
+    mean = 0.0
+    for v in g:
+        mean += float(v)
+    mean /= g.size
+    rise = mean - celsius(20.0)
+    assert abs(rise - paid / capacity) < 1e-9 * (paid / capacity)
+
+
 SOLID_CASES = [
     a_separable_mode_decays_at_its_exact_discrete_rate,
     the_axes_are_the_same_physics_in_a_different_order,
@@ -317,4 +648,13 @@ SOLID_CASES = [
     filling_a_block_with_its_own_substance_changes_nothing,
     a_layered_wall_conserves_over_many_steps,
     a_layered_wall_has_the_series_resistance_of_its_layers,
+    a_laminate_across_its_layers_is_exactly_the_reuss_bound,
+    arithmetic_averaging_is_first_order_where_harmonic_averaging_is_exact,
+    the_swept_steady_state_lands_on_the_layers_resistance,
+    the_stable_step_is_set_by_the_axes_that_have_cells,
+    an_inclusion_is_limited_by_its_faces_and_not_by_its_own_diffusivity,
+    a_cell_of_something_unknown_stops_the_sweep_rather_than_spreading_nan,
+    a_one_cell_coating_adds_its_own_resistance_and_no_more,
+    the_rate_converges_to_the_continuum_at_second_order,
+    it_runs_and_audits_inside_a_simulation,
 ]

@@ -286,3 +286,15 @@ def init_parameters(seed: int) -> np.ndarray:
             vals[j] = u * 2.0 - 1.0
         p[wo:wo + n] = (vals * bound).astype(np.float32)
     return p
+
+
+def adam_step(p: np.ndarray, g: np.ndarray, m: np.ndarray, v: np.ndarray, t: int, lr: float,
+              b1: float = 0.9, b2: float = 0.999, eps: float = 1e-8):
+    """One bias-corrected Adam step (Kingma and Ba, 2015, Algorithm 1) in float64, step number t
+    counted from 1: m = b1 m + (1-b1) g, v = b2 v + (1-b2) g^2, p -= lr (m / (1-b1^t)) /
+    (sqrt(v / (1-b2^t)) + eps).  Returns (p, m, v).  The device's optimiser (fp32)."""
+    p, g, m, v = (np.asarray(a, dtype=np.float64) for a in (p, g, m, v))
+    m = b1 * m + (1.0 - b1) * g
+    v = b2 * v + (1.0 - b2) * g * g
+    c1, c2 = 1.0 - b1 ** t, 1.0 - b2 ** t
+    return p - lr * (m / c1) / (np.sqrt(v / c2) + eps), m, v

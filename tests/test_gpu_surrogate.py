@@ -155,6 +155,33 @@ def test_network_gradients_match_oracle(edge, batch):
     assert cos > 0.99
 
 
+def test_adam_step_matches_the_oracle():
+    """pm_surrogate_train_step's update against oracle/unet.py's Adam on the gradients
+    pm_surrogate_gradients returns for the same batch: two steps from fresh moments, every
+    parameter within fp32 rounding of the float64 update."""
+    gpu_case("surrogate adam step")
+    from pantometry_amd.surrogate import Surrogate
+    s = Surrogate()
+    p = _random_params(9)
+    s.set_parameters(p)
+    t0, k, tk = _fields(16, 2, 10)
+    s.reserve(2, 16)
+    for i in range(2):
+        s.set_sample(i, t0[i], k[i], tk[i], COUPLING)
+    m = np.zeros(p.size)
+    v = np.zeros(p.size)
+    for t, lr in ((1, 1e-3), (2, 5e-4)):
+        before = s.parameters()
+        _, g = s.gradients([0, 1])
+        s.train_step([0, 1], lr)
+        want, m, v = U.adam_step(before, g, m, v, t, lr)
+        got = s.parameters().astype(np.float64)
+        # fp32 arithmetic on the device: the update (at most ~lr per parameter) to a few ulps of
+        # fp32, plus the rounding of the parameter itself
+        tol = 4.0 * 2.0 ** -24 * (np.abs(want) + lr) + 1e-12
+        assert np.all(np.abs(got - want) <= tol), float(np.max(np.abs(got - want) / tol))
+
+
 def test_forward_matches_oracle():
     gpu_case("surrogate forward edge 32")
     from pantometry_amd.surrogate import Surrogate

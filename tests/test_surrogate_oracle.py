@@ -3,6 +3,8 @@ differences, adjoint identities and hand-computed cases.  No reference counterpa
 the learned part, so these checks are what makes the oracle trustworthy."""
 from __future__ import annotations
 
+import math
+
 import numpy as np
 import pytest
 from scipy import ndimage
@@ -169,3 +171,28 @@ def test_input_features_are_the_explicit_stencil_step(oracle):
     assert np.all(x[0, :, :, 0, 2] == 0.0) and np.all(x[0, -1, :, :, 7] == 0.0)
     assert np.all(x[0, :, :, 1:, 2] > 0.0)
     assert np.all(x[0, ..., 15] == 1.0)
+
+
+def test_adam_first_step_is_the_sign_step():
+    """From m = v = 0 the bias-corrected moments are g and g^2, so the first step moves every
+    parameter by lr * g / (|g| + eps): lr * sign(g) wherever |g| >> eps."""
+    rng = np.random.default_rng(5)
+    p = rng.standard_normal(1000)
+    g = rng.standard_normal(1000) * 10.0 ** rng.uniform(-6, 2, 1000)
+    lr = 1e-3
+    q, m, v = U.adam_step(p, g, np.zeros(1000), np.zeros(1000), 1, lr)
+    assert np.allclose(q, p - lr * g / (np.abs(g) + 1e-8), rtol=0, atol=1e-15)
+    assert np.allclose(m, 0.1 * g, rtol=1e-15) and np.allclose(v, 0.001 * g * g, rtol=1e-15)
+
+
+def test_adam_second_step_by_hand():
+    """Two steps of one parameter with g1 = 2, g2 = -1, written out: m2 = 0.9*0.2 - 0.1,
+    v2 = 0.999*0.004 + 0.001, corrections 1 - 0.81 and 1 - 0.998001."""
+    lr = 0.5
+    p1, m1, v1 = U.adam_step(np.array([1.0]), np.array([2.0]), np.zeros(1), np.zeros(1), 1, lr)
+    p2, m2, v2 = U.adam_step(p1, np.array([-1.0]), m1, v1, 2, lr)
+    m_want = 0.9 * 0.2 + 0.1 * -1.0
+    v_want = 0.999 * (0.001 * 4.0) + 0.001 * 1.0
+    step = lr * (m_want / (1.0 - 0.81)) / (math.sqrt(v_want / (1.0 - 0.998001)) + 1e-8)
+    assert abs(m2[0] - m_want) < 1e-15 * abs(m_want) and abs(v2[0] - v_want) < 1e-15 * v_want
+    assert abs(p2[0] - (1.0 - lr * 2.0 / (2.0 + 1e-8) - step)) < 1e-14

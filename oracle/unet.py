@@ -289,12 +289,18 @@ def init_parameters(seed: int) -> np.ndarray:
 
 
 def adam_step(p: np.ndarray, g: np.ndarray, m: np.ndarray, v: np.ndarray, t: int, lr: float,
-              b1: float = 0.9, b2: float = 0.999, eps: float = 1e-8):
-    """One bias-corrected Adam step (Kingma and Ba, 2015, Algorithm 1) in float64, step number t
-    counted from 1: m = b1 m + (1-b1) g, v = b2 v + (1-b2) g^2, p -= lr (m / (1-b1^t)) /
-    (sqrt(v / (1-b2^t)) + eps).  Returns (p, m, v).  The device's optimiser (fp32)."""
-    p, g, m, v = (np.asarray(a, dtype=np.float64) for a in (p, g, m, v))
-    m = b1 * m + (1.0 - b1) * g
-    v = b2 * v + (1.0 - b2) * g * g
-    c1, c2 = 1.0 - b1 ** t, 1.0 - b2 ** t
-    return p - lr * (m / c1) / (np.sqrt(v / c2) + eps), m, v
+              b1: float = 0.9, b2: float = 0.999, eps: float = 1e-8, dtype=np.float64):
+    """One bias-corrected Adam step (Kingma and Ba, 2015, Algorithm 1), step number t counted
+    from 1: m = b1 m + (1-b1) g, v = b2 v + (1-b2) g^2, p -= lr (m / (1-b1^t)) /
+    (sqrt(v / (1-b2^t)) + eps).  Returns (p, m, v) in `dtype`.  With dtype=np.float32 every
+    constant and operation is rounded to float32 in this order, as the device's optimiser runs
+    it (fp32 master parameters and moments; the corrections 1-b^t formed in fp32 on the host)."""
+    f = dtype
+    p, g, m, v = (np.asarray(a, dtype=f) for a in (p, g, m, v))
+    b1, b2, lr, eps = f(b1), f(b2), f(lr), f(eps)
+    one = f(1.0)
+    m = b1 * m + (one - b1) * g
+    v = b2 * v + (one - b2) * g * g
+    c1 = one - f(b1 ** f(t))
+    c2 = one - f(b2 ** f(t))
+    return (p - lr * (m / c1) / (np.sqrt(v / c2) + eps)).astype(f), m, v

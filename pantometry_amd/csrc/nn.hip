@@ -72,11 +72,23 @@ __device__ __forceinline__ void stage_neighbourhood(const Conv &g, const bf16 *_
     }
 }
 
-// Box coordinates of tile position q (q < tile_positions()).
+// Box coordinates of tile position q (q < tile_positions()); the box's extents are powers of
+// two (checked by the launchers), so these are shifts and masks.
 __device__ __forceinline__ void box_coords(const Conv &g, int q, int &qd, int &qh, int &qw) {
-    qw = q % g.TW;
-    qh = (q / g.TW) % g.TH;
-    qd = q / (g.TW * g.TH);
+    const int lw = __builtin_ctz(g.TW), lh = __builtin_ctz(g.TH);
+    qw = q & (g.TW - 1);
+    qh = (q >> lw) & (g.TH - 1);
+    qd = q >> (lw + lh);
+}
+
+// q / d for 0 <= q < 2^20 and 0 < d < 2^10, from the float reciprocal `inv` = 1/d and one
+// correction step (the float product is within one unit of the quotient).
+__device__ __forceinline__ int div_small(int q, int d, float inv) {
+    int r = (int)((float)q * inv);
+    const int rem = q - r * d;
+    r += rem >= d ? 1 : 0;
+    r -= rem < 0 ? 1 : 0;
+    return r;
 }
 
 // The forward kernel.  A workgroup of four waves computes a 64*RB-position M-tile (a TD x TH x TW
@@ -88,8 +100,13 @@ __device__ __forceinline__ void box_coords(const Conv &g, int q, int &qd, int &q
 // are unrolled so each tap's offset in the staged box is a constant plus the rows' strides.  The
 // epilogue adds the bias, applies GELU, and writes bf16 through LDS so each position's channels
 // leave in 16-byte stores.
-template <int NT, int RB>
+//
+// The input may be a concatenation read in place: with cup > 0, channels [0, cup) are x at half
+// the resolution (nearest-neighbour upsampling) and channels [cup, Cin) are xs at full
+// resolution; with cup = 0 all Cin channels are xs.
+template <int NT, int RB, int SK>
 __global__ __launch_bounds__(256) void conv_fwd2(Conv g, const bf16 *__restrict__ x,
+                                                 const bf16 *__restrict__ xs, int cup,
                                                  const bf16 *__restrict__ w,
                                                  const float *__restrict__ bias,
                                                  bf16 *__restrict__ zout,
@@ -107,7 +124,7 @@ __global__ __launch_bounds__(256) void conv_fwd2(Conv g, const bf16 *__restrict_
     // The staged neighbourhood is two planes (channels 0-7 and 8-15 of the chunk) of 16-byte
     // positions, so the 16 lanes of a row group read 256 contiguous bytes.  Lane group
     // lg = lane/16 takes tap 2p + lg/2 of a tap pair p and channel half lg%2.
-    const int lg = lane >> 4, qh = lg & 1, qt = lg >> 1;
+    const int lg = lane >> 4, qh = lg >> 1, qt = lg & 1;
     const int npos = (g.TD + 2) * SHW;
     const uint4 *sA = (const uint4 *)smem + qh * npos;
 
@@ -119,7 +136,7 @@ __global__ __launch_bounds__(256) void conv_fwd2(Conv g, const bf16 *__restrict_
         const int p = wave * 16 * RB + rb * 16 + (lane & 15);
         live[rb] = p < TP;
         int pd, ph, pw;
-        box_coords(g, live[rb] ? p : 0, pd, ph, pw);
+        box_coords(g, live[rb] ? p : TP - 1, pd, ph, pw);
         base[rb] = (pd * SH + ph) * SW + pw;
     }
 
@@ -131,52 +148,115 @@ __global__ __launch_bounds__(256) void conv_fwd2(Conv g, const bf16 *__restrict_
 
     const int64_t wrow = (int64_t)27 * g.Cin;
     const bf16 *wl = w + (int64_t)(n0 + (lane & 15)) * wrow + qh * 8;  // this lane's B column
-    const int rows = (g.TD + 2) * SH;
     // The reduction runs over units (16-channel chunk, kernel plane dz); split z of gridDim.z
     // takes units [u0, u1).
     const int units = g.Cin / 16 * 3;
     const int u0 = (int)((int64_t)blockIdx.z * units / gridDim.z);
     const int u1 = (int)((int64_t)(blockIdx.z + 1) * units / gridDim.z);
+
+    // Staging of a chunk: element e = tid + 256 k (k < SK) of the 2 npos 16-byte halves is half
+    // e % 2 of staged position e / 2, so consecutive lanes read consecutive 16 bytes of the
+    // channels-last input.  All of a thread's loads are issued before any is waited on, and the
+    // next chunk's loads are issued before the current chunk's MFMAs.
+    int dst[SK];  // LDS index of element k; past the neighbourhood, a spare slot after it
+    int src[SK];  // its offset in xs (channel cup of the input); position 0 outside the volume
+    int srcu[SK]; // its offset in the half-resolution x (channel 0), when cup > 0
+    bool in[SK];  // inside the volume (else the loaded value is replaced by zero)
+    const int cs = g.Cin - cup;  // channels of xs
+    const float inv_sw = 1.0f / (float)SW, inv_sh = 1.0f / (float)SH;
+#pragma unroll
+    for (int k = 0; k < SK; k++) {
+        const int e = tid + 256 * k;
+        const int p = e >> 1, half = e & 1;
+        const int t = div_small(p, SW, inv_sw), pz = div_small(t, SH, inv_sh);
+        const int px = p - t * SW, py = t - pz * SH;
+        const int z = d0 + pz - 1, y = h0 + py - 1, xx = w0 + px - 1;
+        const bool staged_here = e < 2 * npos;
+        in[k] = staged_here && z >= 0 && z < g.D && y >= 0 && y < g.H && xx >= 0 && xx < g.W;
+        dst[k] = staged_here ? half * npos + p : 2 * npos;
+        src[k] = in[k] ? (int)(((((int64_t)b * g.D + z) * g.H + y) * g.W + xx) * cs + half * 8) : 0;
+        srcu[k] = in[k] && cup > 0
+                      ? (int)(((((int64_t)b * (g.D >> 1) + (z >> 1)) * (g.H >> 1) + (y >> 1)) * (g.W >> 1) + (xx >> 1)) * cup + half * 8)
+                      : 0;
+    }
+    // With one or two N-blocks the chunk's weights go through the same pipeline into LDS after
+    // the neighbourhood: slot (tap * 2 + half) * NB + column holds input channels half*8 .. +8
+    // of output channel n0 + column, so the loop reads B from LDS like A and waits on no global
+    // load.  Element eb = tid + 256 k is that slot; past 54 NB the loads repeat slot 0's source.
+    constexpr bool WB = NT <= 2;
+    constexpr int SKB = WB ? (54 * NB + 255) / 256 : 1;
+    const int aslots = 2 * npos + 1;
+    // The prefetch registers: the neighbourhood's SK elements, then the weights' SKB.
+    uint4 pre[SK + (WB ? SKB : 0)];
+    // Source offset of weight element k (channel 0 of the chunk).
+    auto bsrc = [&](int k) -> int {
+        const int eb = tid + 256 * k;
+        const int col = eb % NB, rest = eb / NB;
+        return eb < 54 * NB ? (n0 + col) * (int)wrow + (rest >> 1) * g.Cin + (rest & 1) * 8 : 0;
+    };
+    const uint4 *sB = (const uint4 *)smem + aslots + qh * NB + (lane & 15);
+    auto load_chunk = [&](int c0) {
+        if (c0 < cup) {
+#pragma unroll
+            for (int k = 0; k < SK; k++) pre[k] = *(const uint4 *)(x + srcu[k] + c0);
+        } else {
+#pragma unroll
+            for (int k = 0; k < SK; k++) pre[k] = *(const uint4 *)(xs + src[k] + c0 - cup);
+        }
+        if constexpr (WB) {
+#pragma unroll
+            for (int k = 0; k < SKB; k++) pre[SK + k] = *(const uint4 *)(w + bsrc(k) + c0);
+        }
+    };
+    auto store_chunk = [&]() {
+#pragma unroll
+        for (int k = 0; k < SK; k++) {
+            const uint4 v = pre[k];
+            ((uint4 *)smem)[dst[k]] = make_uint4(in[k] ? v.x : 0u, in[k] ? v.y : 0u,
+                                                 in[k] ? v.z : 0u, in[k] ? v.w : 0u);
+        }
+        if constexpr (WB) {
+#pragma unroll
+            for (int k = 0; k < SKB; k++) ((uint4 *)smem)[aslots + tid + 256 * k] = pre[SK + k];
+        }
+    };
+
     int staged = -1;
+    if (u0 < u1) load_chunk((u0 / 3) * 16);
     for (int u = u0; u < u1; u++) {
         const int c0 = (u / 3) * 16, dz = u - (u / 3) * 3;
         if (c0 != staged) {
-        staged = c0;
-        __syncthreads();
-        // Stage channels [c0, c0+16) of the neighbourhood, one row of SW positions at a time.
-        for (int row = wave; row < rows; row += 4) {
-            const int pz = row / SH, py = row - pz * SH;
-            const int z = d0 + pz - 1, y = h0 + py - 1;
-            const bool zy = z >= 0 && z < g.D && y >= 0 && y < g.H;
-            const int64_t rbase = (((int64_t)b * g.D + z) * g.H + y) * g.W;
-            for (int e = lane; e < 2 * SW; e += 64) {
-                const int half = e >= SW ? 1 : 0, px = e - half * SW;
-                const int xx = w0 + px - 1;
-                uint4 v = make_uint4(0, 0, 0, 0);
-                if (zy && xx >= 0 && xx < g.W)
-                    v = *(const uint4 *)(x + (rbase + xx) * g.Cin + c0 + half * 8);
-                ((uint4 *)smem)[half * npos + row * SW + px] = v;
-            }
-        }
-        __syncthreads();
+            staged = c0;
+            __syncthreads();
+            store_chunk();
+            __syncthreads();
+            // The next chunk this workgroup reduces over, in flight during this one's MFMAs.
+            const int un = (u / 3 + 1) * 3;
+            if (un < u1) load_chunk((un / 3) * 16);
         }
 #pragma unroll
         for (int pr = 0; pr < 5; pr++) {
-            // Taps 2pr and 2pr+1 of this kernel plane in one K=32 step; the tenth is padding.
+            // Taps 2pr and 2pr+1 of this kernel plane in one K=32 step; the tenth is padding: it
+            // reads the ninth tap's positions against a zero B fragment, so every load is
+            // unpredicated (rows past a small box read a clamped position and are never stored).
             const int mt = 2 * pr + qt;
             const bool tv = mt < 9;
-            const int toff = dz * SHW + (mt / 3) * SW + (mt % 3);
+            const int mc = tv ? mt : 8;
+            const int toff = dz * SHW + (mc / 3) * SW + (mc % 3);
             uint4 bv[NT];
 #pragma unroll
-            for (int n = 0; n < NT; n++)
-                bv[n] = tv ? *(const uint4 *)(wl + (int64_t)n * 16 * wrow + (dz * 9 + mt) * g.Cin + c0)
-                           : make_uint4(0, 0, 0, 0);
-#pragma unroll
-            for (int rb = 0; rb < RB; rb++) {
-                const uint4 a = tv && live[rb] ? sA[base[rb] + toff] : make_uint4(0, 0, 0, 0);
-#pragma unroll
-                for (int n = 0; n < NT; n++) acc[rb][n] = mfma32(a, bv[n], acc[rb][n]);
+            for (int n = 0; n < NT; n++) {
+                bv[n] = WB ? sB[((dz * 9 + mc) * 2) * NB + n * 16]
+                           : *(const uint4 *)(wl + (int64_t)n * 16 * wrow + (dz * 9 + mc) * g.Cin + c0);
+                if (!tv) bv[n] = make_uint4(0, 0, 0, 0);
             }
+            uint4 a[RB];
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) a[rb] = sA[base[rb] + toff];
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+                for (int n = 0; n < NT; n++) acc[rb][n] = mfma32(a[rb], bv[n], acc[rb][n]);
         }
     }
 
@@ -198,11 +278,18 @@ __global__ __launch_bounds__(256) void conv_fwd2(Conv g, const bf16 *__restrict_
         return;
     }
 
-    // Epilogue through LDS: [position][NB] bf16, then 16-byte stores per position.
+    // Epilogue through LDS: [position][NB + 8] bf16 (the pad spreads the four lane groups' rows
+    // over all banks), then 16-byte stores per position.  A lane holds rows q0..q0+3 of one
+    // channel; neighbouring lanes (channels c, c^1) swap packed pairs so that each writes two
+    // 32-bit words of a channel pair, rows q0, q0+1 from the even lane and q0+2, q0+3 from the
+    // odd one.
+    constexpr int NS = NB + 8;
     float bb[NT];
 #pragma unroll
     for (int n = 0; n < NT; n++) bb[n] = bias ? bias[n0 + n * 16 + (lane & 15)] : 0.0f;
     bf16 *outs[2] = {zout, aout};
+    const bool odd = lane & 1;
+    const int cpair = (lane & 15) & ~1;
 #pragma unroll
     for (int pass = 0; pass < 2; pass++) {
         bf16 *out = outs[pass];
@@ -211,21 +298,31 @@ __global__ __launch_bounds__(256) void conv_fwd2(Conv g, const bf16 *__restrict_
 #pragma unroll
         for (int rb = 0; rb < RB; rb++)
 #pragma unroll
-            for (int n = 0; n < NT; n++)
+            for (int n = 0; n < NT; n++) {
+                uint32_t h[4];
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    const int q = wave * 16 * RB + rb * 16 + (lane >> 4) * 4 + r;
                     const float zz = acc[rb][n][r] + bb[n];
-                    const float v = pass == 1 && gelu_out ? gelu(zz) : zz;
-                    smem[q * NB + n * 16 + (lane & 15)] = to_bf16(v);
+                    h[r] = to_bf16(pass == 1 && gelu_out ? gelu(zz) : zz);
                 }
+                const uint32_t mine = odd ? (h[2] | h[3] << 16) : (h[0] | h[1] << 16);
+                const uint32_t send = odd ? (h[0] | h[1] << 16) : (h[2] | h[3] << 16);
+                // quad_perm [1, 0, 3, 2]: each lane receives its neighbour's `send`.
+                const uint32_t got = (uint32_t)__builtin_amdgcn_mov_dpp((int)send, 0xB1, 0xF, 0xF, false);
+                const uint32_t lo = odd ? (got & 0xFFFFu) | (mine << 16) : (mine & 0xFFFFu) | (got << 16);
+                const uint32_t hi = odd ? (got >> 16) | (mine & 0xFFFF0000u) : (mine >> 16) | (got & 0xFFFF0000u);
+                const int q = wave * 16 * RB + rb * 16 + (lane >> 4) * 4 + (odd ? 2 : 0);
+                uint32_t *row = (uint32_t *)(smem + q * NS + n * 16 + cpair);
+                row[0] = lo;
+                row[NS / 2] = hi;
+            }
         __syncthreads();
         for (int e = tid; e < TP * (NB / 8); e += 256) {
             const int q = e / (NB / 8), part = e % (NB / 8);
             int qd, qh, qw;
             box_coords(g, q, qd, qh, qw);
             const int64_t pos = (((int64_t)b * g.D + d0 + qd) * g.H + h0 + qh) * g.W + w0 + qw;
-            *(uint4 *)(out + pos * ostride + ooff + n0 + part * 8) = *(const uint4 *)(smem + q * NB + part * 8);
+            *(uint4 *)(out + pos * ostride + ooff + n0 + part * 8) = *(const uint4 *)(smem + q * NS + part * 8);
         }
     }
 }
@@ -342,7 +439,8 @@ static bool conv_shape_ok(const Conv &g) {
     const bool ok = g.B > 0 && g.Cin > 0 && g.Cout > 0 && g.Cin % 16 == 0 && g.Cout % 16 == 0 &&
                     g.TD > 0 && g.TH > 0 && g.TW > 0 && g.D % g.TD == 0 && g.H % g.TH == 0 &&
                     g.W % g.TW == 0 && g.tile_positions() <= 64 &&
-                    (g.tile_positions() == 64 || (g.TD == g.D && g.TH == g.H && g.TW == g.W));
+                    (g.tile_positions() == 64 || (g.TD == g.D && g.TH == g.H && g.TW == g.W)) &&
+                    (g.TW & (g.TW - 1)) == 0 && (g.TH & (g.TH - 1)) == 0;
     if (!ok) set_error("convolution shape %dx%dx%dx%d, %d->%d channels, box %dx%dx%d unsupported",
                        g.B, g.D, g.H, g.W, g.Cin, g.Cout, g.TD, g.TH, g.TW);
     return ok;
@@ -362,9 +460,20 @@ static Conv forward_box(const Conv &g, int m) {
 
 template <int NT, int RB>
 static void launch_fwd(const Conv &g, dim3 grid, size_t lds, hipStream_t stream, const bf16 *x,
+                       const bf16 *xs, int cup,
                        const bf16 *w, const float *bias, bf16 *z, bf16 *a, bool gelu_out,
                        int ostride, int ooff, float *part) {
-    conv_fwd2<NT, RB><<<grid, 256, lds, stream>>>(g, x, w, bias, z, a, gelu_out, ostride, ooff, part);
+    // Staged 16-byte halves per thread: 2 (TD+2)(TH+2)(TW+2) over 256 threads.
+    const int halves = 2 * (g.TD + 2) * (g.TH + 2) * (g.TW + 2);
+    const int sk = (halves + 255) / 256;
+    if (sk <= 2)
+        conv_fwd2<NT, RB, 2><<<grid, 256, lds, stream>>>(g, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part);
+    else if (sk <= 4)
+        conv_fwd2<NT, RB, 4><<<grid, 256, lds, stream>>>(g, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part);
+    else if (sk <= 6)
+        conv_fwd2<NT, RB, 6><<<grid, 256, lds, stream>>>(g, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part);
+    else
+        conv_fwd2<NT, RB, 8><<<grid, 256, lds, stream>>>(g, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part);
 }
 
 // Sums the split partials of conv_fwd2 in split order, adds the bias and writes z and act(z) as
@@ -409,7 +518,7 @@ struct FwdPlan {
 // fewer than 1024 workgroups, split it over the reduction (when a workspace may be used), else
 // fall back to narrower N-blocks and 64-position boxes.
 static FwdPlan plan_fwd(const Conv &g0, bool can_split) {
-    const int nt_max = g0.Cout % 64 == 0 ? 4 : g0.Cout == 48 ? 3 : g0.Cout % 32 == 0 ? 2 : 1;
+    const int nt_max = g0.Cout % 32 == 0 ? 2 : 1;
     const Conv f = forward_box(g0, 256);
     if (f.TW > 0 && f.TH > 0 && f.TD > 0) {
         const int64_t base = f.tiles() * (g0.Cout / (16 * nt_max));
@@ -445,7 +554,15 @@ Conv transposed(const Conv &g) {
 
 int conv_forward(const Conv &g0, const bf16 *x, const bf16 *w, const float *bias, bf16 *z, bf16 *a,
                  bool gelu_out, int ostride, int ooff, hipStream_t stream, float *work,
-                 int64_t work_floats) {
+                 int64_t work_floats, const bf16 *skip, int cup) {
+    // Without a skip tensor the whole input is x; with one, x is the half-resolution part.
+    const bf16 *xs = skip ? skip : x;
+    if (!skip) cup = 0;
+    if (skip && (cup <= 0 || cup % 16 || cup >= g0.Cin || g0.D % 2 || g0.H % 2 || g0.W % 2)) {
+        set_error("conv_forward: an in-place concatenation needs cup in (0, Cin), a multiple of "
+                  "16, and even extents");
+        return PM_ERR_ARGUMENT;
+    }
     FwdPlan plan = plan_fwd(g0, true);
     if (plan.splits > 1 && (!work || work_floats < (int64_t)plan.splits * g0.positions() * g0.Cout))
         plan = plan_fwd(g0, false);
@@ -457,26 +574,30 @@ int conv_forward(const Conv &g0, const bf16 *x, const bf16 *w, const float *bias
                     g.TW > 0 && g.TH > 0 && g.TD > 0 && g.D % g.TD == 0 && g.H % g.TH == 0 &&
                     g.W % g.TW == 0 && g.tile_positions() <= m &&
                     (g.tile_positions() == m || (g.TD == g.D && g.TH == g.H && g.TW == g.W)) &&
-                    ostride % 8 == 0 && ooff % 8 == 0;
+                    ostride % 8 == 0 && ooff % 8 == 0 &&
+                    2 * (g.TD + 2) * (g.TH + 2) * (g.TW + 2) <= 256 * 8 &&
+                    (g.TW & (g.TW - 1)) == 0 && (g.TH & (g.TH - 1)) == 0 &&
+                    g.positions() * g.Cin < ((int64_t)1 << 31);
     if (!ok) {
         set_error("conv_forward: shape %dx%dx%dx%d, %d->%d channels unsupported", g.B, g.D, g.H,
                   g.W, g.Cin, g.Cout);
         return PM_ERR_ARGUMENT;
     }
-    const size_t stage = (size_t)(g.TD + 2) * (g.TH + 2) * (g.TW + 2) * 16 * sizeof(bf16);
-    const size_t epi = (size_t)m * 16 * nt * sizeof(bf16);
+    const size_t stage = ((size_t)2 * (g.TD + 2) * (g.TH + 2) * (g.TW + 2) + 1) * 16 +
+                         (nt <= 2 ? (size_t)((54 * 16 * nt + 255) / 256) * 256 * 16 : 0);
+    const size_t epi = (size_t)m * (16 * nt + 8) * sizeof(bf16);
     const size_t lds = std::max(stage, epi);
     dim3 grid((unsigned)g.tiles(), (unsigned)(g.Cout / (16 * nt)), (unsigned)plan.splits);
     const int key = nt * 10 + rb;
     switch (key) {
-        case 14: launch_fwd<1, 4>(g, grid, lds, stream, x, w, bias, z, a, gelu_out, ostride, ooff, part); break;
-        case 24: launch_fwd<2, 4>(g, grid, lds, stream, x, w, bias, z, a, gelu_out, ostride, ooff, part); break;
-        case 34: launch_fwd<3, 4>(g, grid, lds, stream, x, w, bias, z, a, gelu_out, ostride, ooff, part); break;
-        case 44: launch_fwd<4, 4>(g, grid, lds, stream, x, w, bias, z, a, gelu_out, ostride, ooff, part); break;
-        case 11: launch_fwd<1, 1>(g, grid, lds, stream, x, w, bias, z, a, gelu_out, ostride, ooff, part); break;
-        case 21: launch_fwd<2, 1>(g, grid, lds, stream, x, w, bias, z, a, gelu_out, ostride, ooff, part); break;
-        case 31: launch_fwd<3, 1>(g, grid, lds, stream, x, w, bias, z, a, gelu_out, ostride, ooff, part); break;
-        default: launch_fwd<4, 1>(g, grid, lds, stream, x, w, bias, z, a, gelu_out, ostride, ooff, part); break;
+        case 14: launch_fwd<1, 4>(g, grid, lds, stream, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part); break;
+        case 24: launch_fwd<2, 4>(g, grid, lds, stream, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part); break;
+        case 34: launch_fwd<3, 4>(g, grid, lds, stream, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part); break;
+        case 44: launch_fwd<4, 4>(g, grid, lds, stream, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part); break;
+        case 11: launch_fwd<1, 1>(g, grid, lds, stream, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part); break;
+        case 21: launch_fwd<2, 1>(g, grid, lds, stream, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part); break;
+        case 31: launch_fwd<3, 1>(g, grid, lds, stream, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part); break;
+        default: launch_fwd<4, 1>(g, grid, lds, stream, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part); break;
     }
     PM_HIP(hipGetLastError());
     if (part) {

@@ -28,14 +28,20 @@ __host__ __device__ __forceinline__ float from_bf16(bf16 h) {
     return __builtin_bit_cast(float, (uint32_t)h << 16);
 }
 
-// tanh-form GELU and its derivative.
+// tanh-form GELU and its derivative, with tanh(u) written through the logistic function:
+// 0.5 z (1 + tanh(u)) = z / (1 + e^(-2u)) and tanh(u) = 2 / (1 + e^(-2u)) - 1, using the hardware
+// exp2 and reciprocal (v_exp_f32, v_rcp_f32, about 1 ulp each).  The results are rounded to bf16
+// (2^-9) where they are stored, far above the difference from a correctly rounded tanh.
+__device__ __forceinline__ float gelu_e2u(float z) {
+    const float k2 = 2.0f * 0.7978845608028654f * 1.4426950408889634f;  // 2 sqrt(2/pi) log2(e)
+    return __builtin_amdgcn_exp2f(-k2 * (z + 0.044715f * z * z * z));
+}
 __device__ __forceinline__ float gelu(float z) {
-    const float k = 0.7978845608028654f;
-    return 0.5f * z * (1.0f + tanhf(k * (z + 0.044715f * z * z * z)));
+    return z * __builtin_amdgcn_rcpf(1.0f + gelu_e2u(z));
 }
 __device__ __forceinline__ float gelu_grad(float z) {
     const float k = 0.7978845608028654f;
-    const float t = tanhf(k * (z + 0.044715f * z * z * z));
+    const float t = 2.0f * __builtin_amdgcn_rcpf(1.0f + gelu_e2u(z)) - 1.0f;
     return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * k * (1.0f + 3.0f * 0.044715f * z * z);
 }
 
@@ -73,9 +79,14 @@ inline Conv make_conv(int B, int D, int H, int W, int Cin, int Cout) {
 // reduction (16-channel chunk x kernel plane): each split writes fp32 partials to `work`, and a
 // reduction sums them in split order, so results do not depend on scheduling.  Without a
 // `work` of at least conv_forward_workspace(g) floats the launch narrows its blocks instead.
+//
+// With `skip`, the input is the concatenation read in place: channels [0, cup) are x at half the
+// resolution, upsampled by nearest neighbour, and channels [cup, Cin) are `skip` (full
+// resolution, Cin - cup channels), so a decoder layer needs no concatenated buffer.
 int conv_forward(const Conv &g, const bf16 *x, const bf16 *w, const float *bias, bf16 *z,
                  bf16 *a, bool gelu_out, int ostride, int ooff, hipStream_t stream,
-                 float *work = nullptr, int64_t work_floats = 0);
+                 float *work = nullptr, int64_t work_floats = 0, const bf16 *skip = nullptr,
+                 int cup = 0);
 // Floats of `work` conv_forward would use for this shape (0: it does not split).
 int64_t conv_forward_workspace(const Conv &g);
 // dx = conv(dy, flip(w)^T): the gradient of the input, fp32 accumulation, bf16 out.

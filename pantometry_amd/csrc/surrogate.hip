@@ -582,13 +582,24 @@ int run_forward(pm_surrogate *s, bool training) {
     hipStream_t st = s->stream;
     const float *P = s->params.as<float>();
     const bf16 *WB = s->wbf.as<bf16>();
-    auto conv = [&](int li, const bf16 *in, Act &o) -> int {
+    auto R = [&](int l) { return s->R >> l; };
+    auto conv = [&](int li, const bf16 *in, Act &o, const bf16 *skip = nullptr, int cup = 0) -> int {
         const nn::Conv g = conv_of(s, li);
         return nn::conv_forward(g, in, WB + s->layers[li].wbf_off, P + s->layers[li].b_off,
                                 training ? o.z : nullptr, o.a, true, g.Cout, 0, st,
-                                s->fsplit.as<float>(), s->fsplit_floats);
+                                s->fsplit.as<float>(), s->fsplit_floats, skip, cup);
     };
-    auto R = [&](int l) { return s->R >> l; };
+    // A decoder level's input: up(low) ++ skip.  Training keeps the concatenation (its weight
+    // gradient reads it); inference reads both parts in place.
+    auto decoder = [&](int li, const bf16 *low, int clow, const bf16 *skip, int cskip, bf16 *cat,
+                       int level) -> int {
+        if (!training) return conv(li, low, s->act[li], skip, clow);
+        nn::up_fwd<<<grid_of(npos(s, level) * clow / 8), 256, 0, st>>>(
+            low, cat, s->batch, R(level + 1), R(level + 1), R(level + 1), clow, clow + cskip);
+        nn::channel_copy<<<grid_of(npos(s, level) * cskip / 8), 256, 0, st>>>(
+            skip, cat, npos(s, level), cskip, cskip, 0, clow + cskip, clow, 0);
+        return conv(li, cat, s->act[li]);
+    };
     const int B = s->batch;
     PM_CHECK(conv(0, s->xin.as<bf16>(), s->act[0]));
     PM_CHECK(conv(1, s->act[0].a, s->act[1]));
@@ -602,17 +613,11 @@ int run_forward(pm_surrogate *s, bool training) {
     PM_CHECK(conv(6, s->pool[2], s->act[6]));
     PM_CHECK(conv(7, s->act[6].a, s->act[7]));
     // decoder level 2: up(enc4) ++ enc3
-    nn::up_fwd<<<grid_of(npos(s, 2) * 128 / 8), 256, 0, st>>>(s->act[7].a, s->cat[2], B, R(3), R(3), R(3), 128, 192);
-    nn::channel_copy<<<grid_of(npos(s, 2) * 64 / 8), 256, 0, st>>>(s->act[5].a, s->cat[2], npos(s, 2), 64, 64, 0, 192, 128, 0);
-    PM_CHECK(conv(8, s->cat[2], s->act[8]));
+    PM_CHECK(decoder(8, s->act[7].a, 128, s->act[5].a, 64, s->cat[2], 2));
     PM_CHECK(conv(9, s->act[8].a, s->act[9]));
-    nn::up_fwd<<<grid_of(npos(s, 1) * 64 / 8), 256, 0, st>>>(s->act[9].a, s->cat[1], B, R(2), R(2), R(2), 64, 96);
-    nn::channel_copy<<<grid_of(npos(s, 1) * 32 / 8), 256, 0, st>>>(s->act[3].a, s->cat[1], npos(s, 1), 32, 32, 0, 96, 64, 0);
-    PM_CHECK(conv(10, s->cat[1], s->act[10]));
+    PM_CHECK(decoder(10, s->act[9].a, 64, s->act[3].a, 32, s->cat[1], 1));
     PM_CHECK(conv(11, s->act[10].a, s->act[11]));
-    nn::up_fwd<<<grid_of(npos(s, 0) * 32 / 8), 256, 0, st>>>(s->act[11].a, s->cat[0], B, R(1), R(1), R(1), 32, 48);
-    nn::channel_copy<<<grid_of(npos(s, 0) * 16 / 8), 256, 0, st>>>(s->act[1].a, s->cat[0], npos(s, 0), 16, 16, 0, 48, 32, 0);
-    PM_CHECK(conv(12, s->cat[0], s->act[12]));
+    PM_CHECK(decoder(12, s->act[11].a, 32, s->act[1].a, 16, s->cat[0], 0));
     PM_CHECK(conv(13, s->act[12].a, s->act[13]));
     nn::head_fwd<<<grid_of(npos(s, 0)), 256, 0, st>>>(s->act[13].a, P + s->head_off, P + s->head_off + 16, npos(s, 0), s->y.as<float>());
     PM_HIP(hipGetLastError());

@@ -196,3 +196,19 @@ def test_adam_second_step_by_hand():
     step = lr * (m_want / (1.0 - 0.81)) / (math.sqrt(v_want / (1.0 - 0.998001)) + 1e-8)
     assert abs(m2[0] - m_want) < 1e-15 * abs(m_want) and abs(v2[0] - v_want) < 1e-15 * v_want
     assert abs(p2[0] - (1.0 - lr * 2.0 / (2.0 + 1e-8) - step)) < 1e-14
+
+
+def test_adam_in_float32_tracks_float64():
+    """The float32 form (the device's arithmetic) against the float64 form over three steps:
+    within the rounding of fp32 constants and moments, far below the step itself."""
+    rng = np.random.default_rng(6)
+    n = 20000
+    p32 = (0.1 * rng.standard_normal(n)).astype(np.float32)
+    p64 = p32.astype(np.float64)
+    m32 = v32 = np.zeros(n, dtype=np.float32)
+    m64 = v64 = np.zeros(n)
+    for t in (1, 2, 3):
+        g = (rng.standard_normal(n) * 10.0 ** rng.uniform(-6, 0, n)).astype(np.float32)
+        p32, m32, v32 = U.adam_step(p32, g, m32, v32, t, 1e-3, dtype=np.float32)
+        p64, m64, v64 = U.adam_step(p64, g, m64, v64, t, 1e-3)
+        assert np.max(np.abs(p32 - p64)) < 1e-6 * 1e-3 * t + 4 * 2.0 ** -24 * 0.5

@@ -454,13 +454,19 @@ int pm_ensemble_ledgers(pm_ensemble *ens, double *stored);
  * fit in LDS) run resident, the rest streamed in z-chunks of `z_chunk` planes.  No reference
  * counterpart. */
 int pm_ensemble_set_schedule(pm_ensemble *ens, int resident_edge, int z_chunk);
+/* Streamed tile shapes: 0 (default) whole rows when a row is at most 128 cells, as many as fit
+ * 512 cells, else 64 x 8; 1 half as many whole rows (else 64 x 4); 2 whole rows up to 64 cells,
+ * wider rows in 64 x 8 tiles; 3 32-column tiles.  A member whose tile under the policy does not
+ * fit (512 cells, 1024 with a two-cell halo) keeps policy 0.  Results are identical under every
+ * policy.  No reference counterpart. */
+int pm_ensemble_set_tiles(pm_ensemble *ens, int policy);
 /* Steps per launch of the streamed members: 2 (default; temporal blocking, each launch advances
  * two steps with the same arithmetic) or 1.  Results are identical either way.  No reference
  * counterpart. */
 int pm_ensemble_set_steps_per_pass(pm_ensemble *ens, int steps);
-/* Time the candidate schedules on this ensemble (a few steps each, state restored afterwards)
- * and keep the fastest.  `report` (optional, `report_len` bytes) receives one line per
- * candidate. */
+/* Time the candidate schedules on this ensemble (a few steps each, state restored afterwards):
+ * every tile policy x resident edge (0, 16, 20, 24) x z-chunk (16, 32, 64, 128), and keep the
+ * fastest.  `report` (optional, `report_len` bytes) receives one line per candidate. */
 int pm_ensemble_tune(pm_ensemble *ens, char *report, size_t report_len);
 
 /* ---- The learned heat surrogate ------------------------------------------------------------
@@ -512,6 +518,9 @@ int pm_surrogate_step_solid(pm_surrogate *s, pm_solid3d *block, double dt, doubl
                             double *drift, double *elapsed_ms);
 /* ||T_a - T_b||_2 / ||T_b - reference||_2 over two blocks of the same extents. */
 int pm_solid3d_relative_l2(pm_solid3d *a, pm_solid3d *b, double reference, double *rel);
+/* Its two norms: ||T_a - T_b||_2 into `diff` and ||T_b - reference||_2 into `norm`. */
+int pm_solid3d_l2_parts(pm_solid3d *a, pm_solid3d *b, double reference, double *diff,
+                        double *norm);
 /* One bf16 convolution on host fp32 arrays (rounded to bf16 on the way in): x [B][D][H][W][Cin],
  * w [Cout][27][Cin], bias [Cout] -> y [B][D][H][W][Cout] before the activation.  `split` lets a
  * small grid use the split-reduction workspace (as the surrogate does); 0 runs it without. */

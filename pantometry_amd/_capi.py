@@ -157,6 +157,7 @@ SIGNATURES = {
     "pm_ensemble_run": (_I, [_VP, _U64, _DP, _VIOL]),
     "pm_ensemble_ledgers": (_I, [_VP, _DP]),
     "pm_ensemble_set_schedule": (_I, [_VP, _I, _I]),
+    "pm_ensemble_set_tiles": (_I, [_VP, _I]),
     "pm_ensemble_set_steps_per_pass": (_I, [_VP, _I]),
     "pm_ensemble_tune": (_I, [_VP, C.c_char_p, _S]),
     "pm_solid3d_copy_cells": (_I, [_VP, _VP]),
@@ -173,6 +174,7 @@ SIGNATURES = {
     "pm_surrogate_forward": (_I, [_VP, C.c_int64, _DP, _DP, _D, _FP]),
     "pm_surrogate_step_solid": (_I, [_VP, _VP, _D, _D, _DP, _DP]),
     "pm_solid3d_relative_l2": (_I, [_VP, _VP, _D, _DP]),
+    "pm_solid3d_l2_parts": (_I, [_VP, _VP, _D, _DP, _DP]),
     "pm_nn_conv3d": (_I, [_I, _I, _I, _I, _I, _I, _FP, _FP, _FP, _FP, _I]),
     "pm_nn_conv3d_backward": (_I, [_I, _I, _I, _I, _I, _I, _FP, _FP, _FP, _FP, _FP, _FP, _I]),
 }

@@ -278,6 +278,7 @@ struct pm_ensemble {
     DevBuf dmembers, dtiles, dresident, dstored;
     int resident_edge = 0;  // members with max edge <= this run resident
     int z_chunk = 32;       // target planes per streamed tile
+    int tile_policy = 0;    // streamed tile shapes (tile_shape)
     int steps_per_pass = 2;
     int64_t tile_blocks = 0, tile_count = 0, resident_count = 0;
     size_t lds_limit = 0;
@@ -290,16 +291,46 @@ namespace pm {
 
 static size_t resident_bytes(const pm_member &m) { return 2 * sizeof(double) * m.nx * m.ny * m.nz; }
 
-// The streamed tile of a member: whole rows when a row is at most 128 cells (no x halo), else
-// 64 x 8.  Either way the tile holds at most ENS_TILE cells and its plane with a two-cell halo
-// at most ENS_W0.
-static void tile_shape(int nx, int ny, int *tx, int *ty) {
-    if (nx <= 128) {
-        *tx = nx;
-        *ty = std::max(1, std::min(ny, ENS_TILE / nx));
-    } else {
-        *tx = 64;
-        *ty = std::min(ny, 8);
+// The streamed tile of a member under a tile policy; each keeps the tile to at most ENS_TILE
+// cells and its plane with a two-cell halo to at most ENS_W0 (tile_fits), else the member falls
+// back to policy 0.
+//   0 whole rows when a row is at most 128 cells (no x halo), as many as fit; else 64 x 8
+//   1 half as many whole rows (twice the workgroups, more halo rows); else 64 x 4
+//   2 whole rows up to 64 cells; wider rows in 64 x 8 tiles
+//   3 32-column tiles, as many rows as fit
+constexpr int ENS_POLICIES = 4;
+static void tile_shape(int nx, int ny, int policy, int *tx, int *ty) {
+    switch (policy) {
+        case 1:
+            if (nx <= 128) {
+                *tx = nx;
+                *ty = std::max(1, std::min(ny, ENS_TILE / nx / 2));
+            } else {
+                *tx = 64;
+                *ty = std::min(ny, 4);
+            }
+            return;
+        case 2:
+            if (nx <= 64) {
+                *tx = nx;
+                *ty = std::max(1, std::min(ny, ENS_TILE / nx));
+            } else {
+                *tx = 64;
+                *ty = std::min(ny, 8);
+            }
+            return;
+        case 3:
+            *tx = std::min(nx, 32);
+            *ty = std::max(1, std::min(ny, ENS_TILE / *tx));
+            return;
+        default:
+            if (nx <= 128) {
+                *tx = nx;
+                *ty = std::max(1, std::min(ny, ENS_TILE / nx));
+            } else {
+                *tx = 64;
+                *ty = std::min(ny, 8);
+            }
     }
 }
 
@@ -308,7 +339,23 @@ static bool tile_fits(const Member &m) {
     return m.tx * m.ty <= ENS_TILE && w0 * h0 <= ENS_W0;
 }
 
+// Sets every member's streamed tile from the ensemble's tile policy (policy 0 where the
+// policy's tile does not fit) and uploads the member table.
+static int ens_apply_tiles(pm_ensemble *e) {
+    for (Member &d : e->members) {
+        Member t = d;
+        tile_shape(d.nx, d.ny, e->tile_policy, &t.tx, &t.ty);
+        if (!tile_fits(t)) tile_shape(d.nx, d.ny, 0, &t.tx, &t.ty);
+        d.tx = t.tx, d.ty = t.ty;
+    }
+    if (!e->members.empty())
+        PM_HIP(hipMemcpy(e->dmembers.d(), e->members.data(), sizeof(Member) * e->members.size(),
+                         hipMemcpyHostToDevice));
+    return PM_OK;
+}
+
 static int ens_plan(pm_ensemble *e) {
+    PM_CHECK(ens_apply_tiles(e));
     std::vector<int> resident;
     // Streamed tiles are grouped by (member, z-chunk); the groups go, largest first, to the XCD
     // with the least work so far, so the tiles that share halo rows run on one XCD's L2 and
@@ -463,7 +510,7 @@ int pm_ensemble_create(const pm_member *members, size_t count, pm_ensemble **out
         Member d;
         d.off = off;
         d.nx = (int)s.nx, d.ny = (int)s.ny, d.nz = (int)s.nz, d.pad = 0;
-        tile_shape(d.nx, d.ny, &d.tx, &d.ty);
+        tile_shape(d.nx, d.ny, 0, &d.tx, &d.ty);
         if (!tile_fits(d)) {
             set_error("pm_ensemble_create: no streamed tile fits member %zu", m);
             return PM_ERR_ARGUMENT;
@@ -581,6 +628,15 @@ int pm_ensemble_ledgers(pm_ensemble *e, double *stored) {
     return PM_OK;
 }
 
+int pm_ensemble_set_tiles(pm_ensemble *e, int policy) {
+    if (!e || policy < 0 || policy >= ENS_POLICIES) {
+        set_error("pm_ensemble_set_tiles: policy must be 0..%d", ENS_POLICIES - 1);
+        return PM_ERR_ARGUMENT;
+    }
+    e->tile_policy = policy;
+    return ens_plan(e);
+}
+
 int pm_ensemble_set_schedule(pm_ensemble *e, int resident_edge, int z_chunk) {
     if (!e || z_chunk < 1) return PM_ERR_ARGUMENT;
     e->resident_edge = resident_edge;
@@ -609,32 +665,36 @@ int pm_ensemble_tune(pm_ensemble *e, char *report, size_t report_len) {
     PM_CHECK(keep.alloc(sizeof(double) * (size_t)e->cells));
     PM_HIP(hipMemcpy(keep.d(), e->t[e->cur].d(), sizeof(double) * (size_t)e->cells, hipMemcpyDeviceToDevice));
     const int keep_cur = e->cur;
-    struct Cand { int edge, zc; };
+    struct Cand { int policy, edge, zc; };
     std::vector<Cand> cands;
-    for (int edge : {0, 16, 20, 24})
-        for (int zc : {16, 32, 64, 128}) cands.push_back({edge, zc});
+    for (int policy = 0; policy < ENS_POLICIES; policy++)
+        for (int edge : {0, 16, 20, 24})
+            for (int zc : {16, 32, 64, 128}) cands.push_back({policy, edge, zc});
     const uint64_t trial = 20;
     double best = INFINITY;
-    Cand pick = {e->resident_edge, e->z_chunk};
+    Cand pick = {e->tile_policy, e->resident_edge, e->z_chunk};
     std::string text;
     for (const Cand &c : cands) {
-        e->resident_edge = c.edge, e->z_chunk = c.zc;
+        e->tile_policy = c.policy, e->resident_edge = c.edge, e->z_chunk = c.zc;
         PM_CHECK(ens_plan(e));
         double ms = 0.0, ms2 = 0.0;
         PM_CHECK(ens_run(e, trial, &ms));   // warm (includes capture)
         PM_CHECK(ens_run(e, trial, &ms2));  // timed replay
         char line[160];
-        std::snprintf(line, sizeof line, "resident_edge=%d z_chunk=%d resident=%lld tiles=%lld: %.4f ms/step\n",
-                      c.edge, c.zc, (long long)e->resident_count, (long long)e->tile_count, ms2 / trial);
+        std::snprintf(line, sizeof line,
+                      "tiles=%d resident_edge=%d z_chunk=%d resident=%lld tiles=%lld: %.4f ms/step\n",
+                      c.policy, c.edge, c.zc, (long long)e->resident_count,
+                      (long long)e->tile_count, ms2 / trial);
         text += line;
         if (ms2 < best) best = ms2, pick = c;
     }
-    e->resident_edge = pick.edge, e->z_chunk = pick.zc;
+    e->tile_policy = pick.policy, e->resident_edge = pick.edge, e->z_chunk = pick.zc;
     PM_CHECK(ens_plan(e));
     e->cur = keep_cur;
     PM_HIP(hipMemcpy(e->t[e->cur].d(), keep.d(), sizeof(double) * (size_t)e->cells, hipMemcpyDeviceToDevice));
     char line[128];
-    std::snprintf(line, sizeof line, "picked resident_edge=%d z_chunk=%d\n", pick.edge, pick.zc);
+    std::snprintf(line, sizeof line, "picked tiles=%d resident_edge=%d z_chunk=%d\n", pick.policy,
+                  pick.edge, pick.zc);
     text += line;
     if (report && report_len) std::snprintf(report, report_len, "%s", text.c_str());
     return PM_OK;

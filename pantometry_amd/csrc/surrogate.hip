@@ -1023,7 +1023,15 @@ int pm_surrogate_step_solid(pm_surrogate *s, pm_solid3d *solid, double dt, doubl
 
 // ||T_a - T_b|| / ||T_b - reference|| between two blocks' temperatures, on the device.
 int pm_solid3d_relative_l2(pm_solid3d *a, pm_solid3d *b, double reference, double *rel) {
-    if (!a || !b || !rel) return PM_ERR_ARGUMENT;
+    if (!rel) return PM_ERR_ARGUMENT;
+    double diff = 0.0, norm = 0.0;
+    PM_CHECK(pm_solid3d_l2_parts(a, b, reference, &diff, &norm));
+    *rel = diff / norm;
+    return PM_OK;
+}
+
+int pm_solid3d_l2_parts(pm_solid3d *a, pm_solid3d *b, double reference, double *diff, double *norm) {
+    if (!a || !b || !diff || !norm) return PM_ERR_ARGUMENT;
     double *ca = nullptr, *cb = nullptr;
     const double *ka, *kb;
     int64_t na[3], nb[3];
@@ -1043,7 +1051,8 @@ int pm_solid3d_relative_l2(pm_solid3d *a, pm_solid3d *b, double reference, doubl
     PM_HIP(hipStreamSynchronize(sa));
     double e = 0.0, q = 0.0;
     for (unsigned i = 0; i < rb; i++) e += h[2 * i], q += h[2 * i + 1];
-    *rel = std::sqrt(e) / std::sqrt(q);
+    *diff = std::sqrt(e);
+    *norm = std::sqrt(q);
     return PM_OK;
 }
 

@@ -75,12 +75,17 @@ class Ensemble:
         check(lib().pm_ensemble_set_schedule(self._h, resident_edge, z_chunk),
               "pm_ensemble_set_schedule")
 
+    def set_tiles(self, policy: int) -> None:
+        """Streamed tile shapes: 0 whole rows (default), 1 half as many rows, 2 rows up to 64
+        cells then 64 x 8, 3 32-column tiles.  Results are the same under every policy."""
+        check(lib().pm_ensemble_set_tiles(self._h, int(policy)), "pm_ensemble_set_tiles")
+
     def set_steps_per_pass(self, steps: int) -> None:
         """Streamed members advance 2 steps per launch (default) or 1; the results are the same."""
         check(lib().pm_ensemble_set_steps_per_pass(self._h, int(steps)),
               "pm_ensemble_set_steps_per_pass")
 
     def tune(self) -> str:
-        buf = C.create_string_buffer(8192)
+        buf = C.create_string_buffer(16384)
         check(lib().pm_ensemble_tune(self._h, buf, len(buf)), "pm_ensemble_tune")
         return buf.value.decode()

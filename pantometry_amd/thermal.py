@@ -329,6 +329,13 @@ class Solid3D:
               "pm_solid3d_relative_l2")
         return out.value
 
+    def l2_parts(self, other: "Solid3D", reference: float) -> Tuple[float, float]:
+        """(||T - T_other||, ||T_other - reference||), computed on the device."""
+        diff, norm = C.c_double(0.0), C.c_double(0.0)
+        check(lib().pm_solid3d_l2_parts(self._h, other._h, float(reference), C.byref(diff),
+                                        C.byref(norm)), "pm_solid3d_l2_parts")
+        return diff.value, norm.value
+
     def set_temperature(self, i: int, j: int, k: int, t: float) -> None:
         check(lib().pm_solid3d_set_temperature(self._h, i, j, k, float(t)),
               "pm_solid3d_set_temperature")

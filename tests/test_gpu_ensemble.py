@@ -22,12 +22,14 @@ def _oracle_member(oracle, edge, alpha, seed, steps, dt):
     return o
 
 
-@pytest.mark.parametrize("resident_edge,z_chunk,per_pass",
-                         [(0, 8, 2), (0, 32, 2), (24, 16, 2), (40, 5, 2), (0, 16, 1), (20, 3, 1)])
-def test_members_are_bit_exact_under_every_schedule(resident_edge, z_chunk, per_pass, oracle):
-    gpu_case(f"ensemble schedule {resident_edge}/{z_chunk}/{per_pass}")
+@pytest.mark.parametrize("resident_edge,z_chunk,per_pass,tiles",
+                         [(0, 8, 2, 0), (0, 32, 2, 0), (24, 16, 2, 0), (40, 5, 2, 0), (0, 16, 1, 0),
+                          (20, 3, 1, 0), (0, 16, 2, 1), (0, 16, 2, 2), (0, 16, 2, 3), (0, 5, 1, 3)])
+def test_members_are_bit_exact_under_every_schedule(resident_edge, z_chunk, per_pass, tiles, oracle):
+    gpu_case(f"ensemble schedule {resident_edge}/{z_chunk}/{per_pass} tiles {tiles}")
     from pantometry_amd import workloads as w
     ens, shapes = w.heat_ensemble(24, lo_edge=14.0, hi_edge=40.0)
+    ens.set_tiles(tiles)
     ens.set_schedule(resident_edge, z_chunk)
     ens.set_steps_per_pass(per_pass)
     before = ens.ledgers()
@@ -59,10 +61,11 @@ ODD_SHAPES = [(130, 9, 7), (200, 17, 3), (3, 50, 20), (1, 1, 40), (97, 6, 33), (
               (40, 1, 1), (2, 2, 2), (65, 70, 2)]
 
 
+@pytest.mark.parametrize("tiles", [0, 1, 2, 3])
 @pytest.mark.parametrize("steps", [1, 2, 3, 4, 7])
 @pytest.mark.parametrize("z_chunk", [1, 16])
-def test_uneven_members_are_bit_exact(steps, z_chunk, oracle):
-    gpu_case(f"ensemble uneven members steps={steps} z_chunk={z_chunk}")
+def test_uneven_members_are_bit_exact(steps, z_chunk, tiles, oracle):
+    gpu_case(f"ensemble uneven members steps={steps} z_chunk={z_chunk} tiles={tiles}")
     from pantometry_amd import workloads as w
     from pantometry_amd.ensemble import Ensemble, Member
     from pantometry_amd.thermal import Substance
@@ -78,6 +81,7 @@ def test_uneven_members_are_bit_exact(steps, z_chunk, oracle):
     ens = Ensemble(members)
     for n, (o, _) in enumerate(blocks):
         ens.set_cells(n, o.cells)
+    ens.set_tiles(tiles)
     ens.set_schedule(0, z_chunk)
     ens.run(steps)
     for n, (o, dt) in enumerate(blocks):

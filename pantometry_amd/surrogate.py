@@ -29,6 +29,7 @@ from .thermal import Solid3D, Substance
 SURROGATE_SEED = 0x5EED0010
 TRAJECTORY_SEED = 0x5EED0011
 BATCH_SEED = 0x5EED0012
+PUSHFORWARD_SEED = 0x5EED0013
 
 WEIGHTS = Path(__file__).resolve().parent / "weights" / "heat_unet_k16.npz"
 
@@ -70,8 +71,9 @@ class Surrogate:
         np.savez_compressed(path, parameters=self.parameters(),
                             meta=np.array(repr(sorted(meta.items()))))
 
-    def load(self, path: Path) -> None:
-        with np.load(Path(path), allow_pickle=False) as f:
+    def load(self, path: Optional[Path] = None) -> None:
+        """Parameters from `path`, by default the shipped weights."""
+        with np.load(Path(path) if path is not None else WEIGHTS, allow_pickle=False) as f:
             self.set_parameters(f["parameters"])
 
     # ---- training set -----------------------------------------------------------------------
@@ -172,13 +174,14 @@ def heterogeneous_pair_block(edge: int, seed_index: int) -> Tuple[Solid3D, np.nd
 
 
 def build_training_set(surr: Surrogate, samples: int, edge: int = 64, chunks: int = 64, seed: int = TRAJECTORY_SEED,
-                       log: Optional[Callable[[str], None]] = None) -> float:
+                       log: Optional[Callable[[str], None]] = None, capacity: int = 0) -> float:
     """Fills the surrogate's training set with `samples` pairs (T_0, T_k), one per seeded
     trajectory: trajectory i starts from its own heterogeneous field, runs 16 j stencil steps for
     a seeded j in [0, chunks), and records the pair across the next 16 steps.
-    Every step is the FP64 GPU stencil with the audit on.  Returns the wall seconds taken."""
+    Every step is the FP64 GPU stencil with the audit on.  The set is sized for `capacity` pairs
+    (at least `samples`), so more can be added later.  Returns the wall seconds taken."""
     t0 = time.perf_counter()
-    surr.reserve(samples, edge)
+    surr.reserve(max(samples, capacity), edge)
     for i in range(samples):
         block, k, dt = heterogeneous_pair_block(edge, i)
         j = int(Rng.for_index(seed, i).unit() * chunks)
@@ -192,6 +195,38 @@ def build_training_set(surr: Surrogate, samples: int, edge: int = 64, chunks: in
         surr.set_sample_from(i, block, after, dt)
         if log and (i + 1) % max(1, samples // 8) == 0:
             log(f"training set: {i + 1}/{samples} pairs ({time.perf_counter() - t0:.1f} s)")
+    return time.perf_counter() - t0
+
+
+def add_rollout_pairs(surr: Surrogate, first: int, count: int, edge: int = 64, chunks: int = 64,
+                      unroll: int = 3, seed: int = PUSHFORWARD_SEED,
+                      log: Optional[Callable[[str], None]] = None) -> float:
+    """Writes pairs `first .. first + count` of the training set from the network's own
+    rollouts: trajectory i (seeded like the stencil pairs, offset by `first`) runs 16 j stencil
+    steps for a seeded j, then 1 .. `unroll` surrogate steps of the current network, and the pair
+    is that state against the FP64 stencil 16 steps from it.  Training on these teaches the
+    network the states it produces itself (its residual noise included), which the stencil
+    pairs never show it.  Returns the wall seconds taken."""
+    from .workloads import CP, RHO
+    t0 = time.perf_counter()
+    for n in range(count):
+        i = first + n
+        block, k, dt = heterogeneous_pair_block(edge, i)
+        r = Rng.for_index(seed, i)
+        j = int(r.unit() * chunks)
+        m = 1 + int(r.unit() * unroll)
+        if j:
+            block.advance(dt, j * K_STEPS, tolerance=1e-9, want_totals=False)
+        for _ in range(m):
+            surr.step(block, dt, 300.0)
+        after = Solid3D("after", Substance.of("heterogeneous", RHO, 1.0, CP), (edge,) * 3,
+                        block.spacing(), 300.0)
+        after.with_conductivity(k)
+        after.copy_cells_from(block)
+        after.advance(dt, K_STEPS, tolerance=1e-9, want_totals=False)
+        surr.set_sample_from(i, block, after, dt)
+        if log and (n + 1) % max(1, count // 8) == 0:
+            log(f"rollout pairs: {n + 1}/{count} ({time.perf_counter() - t0:.1f} s)")
     return time.perf_counter() - t0
 
 
@@ -225,11 +260,17 @@ class RolloutReport:
     fallbacks: int                 # chunks vetoed and replaced by the stencil's result
     max_error: float               # worst relative L2 error seen on a checked chunk
     max_drift: float               # worst audit drift of a surrogate step
-    final_error: float             # relative L2 error against a plain stencil run
+    final_error: float             # relative L2 error against a plain stencil run:
+                                   # ||T - T_h|| / ||T_h - T_ref|| (of the heat above T_ref)
     rollout_s: float               # wall seconds of the rollout
     stencil_s: float               # wall seconds of the plain stencil over the same steps
     surrogate_ms: float            # mean device ms of one surrogate step
     errors: List[float] = field(default_factory=list)
+    final_error_kelvin: float = 0.0  # ||T - T_h|| / ||T_h||, of the temperatures themselves
+    rises: int = 0                   # chunks vetoed because their one-step change rose
+    changes: List[Tuple[float, float]] = field(default_factory=list)
+                                   # per chunk: one stencil step's change ||S T - T|| at its
+                                   # start and after the surrogate step (when measured)
 
     @property
     def speedup(self) -> float:
@@ -237,12 +278,18 @@ class RolloutReport:
 
 
 def rollout(surr: Surrogate, k_field: np.ndarray, t0: np.ndarray, dt: float, steps: int = 1024,
-            tolerance: float = 1e-3, check_every: int = 1,
-            reference: float = 300.0) -> RolloutReport:
+            tolerance: float = 1e-3, check_every: int = 1, reference: float = 300.0,
+            residual: bool = True) -> RolloutReport:
     """Advances a heterogeneous block `steps` stencil steps by surrogate steps of 16 each.  Every
     `check_every`-th chunk (all of them with 1) also runs the FP64 stencil from the same state;
     a checked chunk whose error exceeds `tolerance`, or any chunk whose audit drift does, is
-    replaced by the stencil's result.  Returns the report, with the same steps of the plain
+    replaced by the stencil's result.  With `residual`, every chunk not checked is also vetoed
+    when one FP64 stencil step moves the surrogate's result further than one step moved the
+    chunk's start (||S T_s - T_s|| > ||S T_0 - T_0||).  The block is insulated with a uniform
+    capacity, so a step is T <- S T with S = I + dt A symmetric, A negative semi-definite and
+    every eigenvalue of S in [-1, 1] at a stable dt; S - I commutes with S, so along the exact
+    trajectory that change never grows.  Noise the network leaves behind, which the stencil
+    would have damped, makes it grow.  Returns the report, with the same steps of the plain
     stencil timed for comparison."""
     from .workloads import CP, DX, RHO
     k = K_STEPS
@@ -255,7 +302,14 @@ def rollout(surr: Surrogate, k_field: np.ndarray, t0: np.ndarray, dt: float, ste
         b.set_cells(t0)
         return b
 
-    plain, state, judge = block(), block(), block()
+    plain, state, judge, probe = block(), block(), block(), block()
+
+    def change() -> float:
+        """||S T - T|| of the rollout's present state: one stencil step on the probe."""
+        probe.copy_cells_from(state)
+        probe.advance(dt, 1, tolerance=1e-9, want_totals=False)
+        return probe.l2_parts(state, reference)[0]
+
     # Warm both paths once (kernel loading, buffer sizing), then restore the start.
     surr.step(judge, dt, reference)
     judge.set_cells(t0)
@@ -267,10 +321,12 @@ def rollout(surr: Surrogate, k_field: np.ndarray, t0: np.ndarray, dt: float, ste
     stencil_s = time.perf_counter() - tp
 
     chunks = steps // k
-    checked = fallbacks = 0
+    checked = fallbacks = rises = 0
     max_err = max_drift = 0.0
     errors: List[float] = []
+    changes: List[Tuple[float, float]] = []
     sms = 0.0
+    start_change = None  # ||S T - T|| of the state the next chunk starts from, when known
     tr = time.perf_counter()
     for c in range(chunks):
         check_now = check_every > 0 and c % check_every == 0
@@ -279,6 +335,8 @@ def rollout(surr: Surrogate, k_field: np.ndarray, t0: np.ndarray, dt: float, ste
             judge.advance(dt, k, tolerance=1e-9, want_totals=False)
         else:
             state.checkpoint()
+            if residual and start_change is None:
+                start_change = change()
         drift, ms = surr.step(state, dt, reference)
         sms += ms
         max_drift = max(max_drift, drift)
@@ -289,8 +347,17 @@ def rollout(surr: Surrogate, k_field: np.ndarray, t0: np.ndarray, dt: float, ste
             errors.append(err)
             max_err = max(max_err, err)
             veto = veto or err > tolerance
+            start_change = None
+        elif residual:
+            after = change()
+            changes.append((start_change, after))
+            if after > start_change:
+                rises += 1
+                veto = True
+            start_change = after
         if veto:
             fallbacks += 1
+            start_change = None
             if check_now:
                 state.copy_cells_from(judge)
             else:
@@ -302,4 +369,5 @@ def rollout(surr: Surrogate, k_field: np.ndarray, t0: np.ndarray, dt: float, ste
     rollout_s = time.perf_counter() - tr
     final = state.relative_l2(plain, reference)
     return RolloutReport(steps, k, chunks, checked, fallbacks, max_err, max_drift, final,
-                         rollout_s, stencil_s, sms / max(chunks, 1), errors)
+                         rollout_s, stencil_s, sms / max(chunks, 1), errors,
+                         state.relative_l2(plain, 0.0), rises, changes)

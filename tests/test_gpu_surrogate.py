@@ -227,7 +227,7 @@ def test_rollout_veto_falls_back_to_the_exact_stencil():
     rep = rollout(s, k, t0, dt, steps=64, tolerance=0.0)
     assert rep.chunks == 4 and rep.checked == 4 and rep.fallbacks == 4
     assert rep.final_error == 0.0        # every chunk replaced: bit-exact to the plain stencil
-    rep = rollout(s, k, t0, dt, steps=64, tolerance=math.inf, check_every=2)
+    rep = rollout(s, k, t0, dt, steps=64, tolerance=math.inf, check_every=2, residual=False)
     assert rep.fallbacks == 0 and rep.checked == 2
     assert rep.max_drift <= 1e-12
     # the untrained network predicts no change: its error is the persistence error
@@ -262,10 +262,11 @@ def test_training_on_stencil_data_beats_persistence():
 
 
 def test_committed_weights_predict_held_out_chunks():
-    """The shipped network (trained by pantometry_amd.train_surrogate on 4096 GPU-stencil pairs)
-    on trajectories it never saw: one 16-step chunk, eight chunks in, within 3e-3 relative L2 of
-    the stencil (||T_s - T_h|| / ||T_h - 300 K||) and at least five times closer than
-    persistence."""
+    """The shipped network (trained by pantometry_amd.train_surrogate on 4096 GPU-stencil pairs
+    and pairs from its own rollouts) on trajectories it never saw: one 16-step chunk, eight
+    chunks in, within 1e-3 of the stencil in the error of the heat (||T_s - T_h|| /
+    ||T_h - 300 K||; 1.3-1.9e-4 observed) and at least twenty times closer than persistence
+    (56-66 observed)."""
     gpu_case("surrogate committed weights edge 64")
     from pantometry_amd.surrogate import (WEIGHTS, Surrogate, coupling, heterogeneous_pair_block)
     s = Surrogate()
@@ -280,4 +281,4 @@ def test_committed_weights_predict_held_out_chunks():
         err = np.linalg.norm(pred - tk) / np.linalg.norm(tk - 300.0)
         gain = np.linalg.norm(t0 - tk) / np.linalg.norm(pred - tk)
         print(f"[surrogate] held-out {i}: error {err:.2e}, persistence/network {gain:.1f}", flush=True)
-        assert err <= 3e-3 and gain >= 5.0
+        assert err <= 1e-3 and gain >= 20.0

@@ -1,0 +1,76 @@
+"""The surrogate-assisted rollout at its full workload: the shipped network advancing a 128^3
+heterogeneous block (lognormal(0, 0.5) conductivity, eight hot spots, 0.9 of its stable dt)
+through 1024 stencil steps in 16-step chunks, every fourth chunk checked against the FP64
+stencil, the others watched by the one-step-change veto, any chunk over 1e-3 replaced by the
+stencil's result.
+
+Two error measures against the plain FP64 stencil run of the same steps:
+  relative L2       ||T - T_h|| / ||T_h||           the bar: at most 1e-3;
+  of the heat       ||T - T_h|| / ||T_h - 300 K||   about 14x stricter on this block (the
+                                                    field's excess over the 300 K reference);
+                                                    the per-chunk veto uses this one.
+This file sorts after the stencil tests, so a miss here cannot hide them."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from tests.conftest import gpu_case
+
+pytestmark = pytest.mark.gpu
+
+EDGE, SEED, STEPS = 128, 424242, 1024
+
+
+@pytest.fixture(scope="module")
+def case():
+    from pantometry_amd.surrogate import Surrogate, heterogeneous_pair_block
+    s = Surrogate()
+    s.load()
+    block, k, dt = heterogeneous_pair_block(EDGE, SEED)
+    t0 = block.cells().copy()
+    del block
+    return s, k, t0, dt
+
+
+@pytest.mark.timeout(300)
+def test_rollout_with_vetoes_meets_1e3_against_the_stencil(case):
+    gpu_case(f"surrogate rollout {EDGE}^3 x {STEPS}, check every 4th chunk, step-change veto")
+    from pantometry_amd.surrogate import rollout
+    s, k, t0, dt = case
+    rep = rollout(s, k, t0, dt, steps=STEPS, tolerance=1e-3, check_every=4)
+    checked_fallbacks = sum(e > 1e-3 for e in rep.errors)
+    print(f"[rollout] relative L2 {rep.final_error_kelvin:.3e}, of the heat {rep.final_error:.3e}; "
+          f"{rep.fallbacks} of {rep.chunks} chunks fell back ({checked_fallbacks} of "
+          f"{rep.checked} checked, {rep.rises} on a rising step change); checked-chunk errors "
+          f"median {np.median(rep.errors):.2e} max {rep.max_error:.2e}; {rep.speedup:.3f}x "
+          f"the plain stencil", flush=True)
+    assert rep.chunks == 64 and rep.checked == 16
+    assert rep.final_error_kelvin <= 1e-3
+    assert rep.final_error <= 3e-3
+    # most checked chunks are accepted, and every accepted one is within the tolerance
+    assert checked_fallbacks <= rep.checked // 4
+    assert rep.max_drift <= 1e-12
+    # the step-change veto is not vacuous: it fired, and each firing is a chunk the stencil redid
+    assert rep.rises >= 1 and rep.fallbacks == checked_fallbacks + rep.rises
+    for before, after in rep.changes:
+        assert before > 0.0 and after > 0.0
+
+
+@pytest.mark.timeout(300)
+def test_the_checks_are_what_keep_the_rollout_close(case):
+    """The same rollout with no check and no veto drifts further from the stencil than the
+    checked one, though it too stays within 1e-3 in relative L2."""
+    gpu_case(f"surrogate rollout {EDGE}^3 x {STEPS}, unchecked against checked")
+    from pantometry_amd.surrogate import rollout
+    s, k, t0, dt = case
+    free = rollout(s, k, t0, dt, steps=STEPS, tolerance=math.inf, check_every=0, residual=False)
+    kept = rollout(s, k, t0, dt, steps=STEPS, tolerance=1e-3, check_every=4)
+    print(f"[rollout] unchecked: relative L2 {free.final_error_kelvin:.3e}, of the heat "
+          f"{free.final_error:.3e}, {free.speedup:.3f}x the plain stencil; checked: "
+          f"{kept.final_error_kelvin:.3e}, {kept.final_error:.3e}", flush=True)
+    assert free.fallbacks == 0 and free.checked == 0 and free.rises == 0
+    assert kept.final_error < free.final_error
+    assert free.final_error_kelvin <= 1e-3

@@ -212,3 +212,43 @@ def test_adam_in_float32_tracks_float64():
         p32, m32, v32 = U.adam_step(p32, g, m32, v32, t, 1e-3, dtype=np.float32)
         p64, m64, v64 = U.adam_step(p64, g, m64, v64, t, 1e-3)
         assert np.max(np.abs(p32 - p64)) < 1e-6 * 1e-3 * t + 4 * 2.0 ** -24 * 0.5
+
+
+def _change(oracle, k, t, dt):
+    """||S T - T|| for one stencil step S of the reference-pinned oracle."""
+    from pantometry_amd import workloads as w
+    n = round(len(t) ** (1 / 3))
+    b = oracle.OracleSolid3D("b", w.RHO, 1.0, w.CP, (n, n, n), w.DX, 300.0).with_conductivity(k)
+    b.set_cells(t)
+    b.step(dt)
+    return float(np.linalg.norm(b.cells - t))
+
+
+def test_one_step_change_never_grows_along_the_stencil():
+    """The premise of the rollout's residual veto: on an insulated block of uniform capacity one
+    stencil step is T <- S T with S symmetric and every eigenvalue in [-1, 1] at a stable dt, so
+    ||S T - T|| is non-increasing along the trajectory.  Checked over 300 steps of a
+    heterogeneous block at 0.9 and 1.0 of its limit, by the reference-pinned stencil oracle;
+    and the veto's other half: noise the stencil would damp makes the change grow."""
+    from oracle import oracle
+    from pantometry_amd import workloads as w
+    n = 12
+    k, t = oracle.heterogeneous_fields((n, n, n), spots=3)
+    b = oracle.OracleSolid3D("b", w.RHO, 1.0, w.CP, (n, n, n), w.DX, 300.0).with_conductivity(k)
+    b.set_cells(t)
+    for frac in (0.9, 1.0):
+        dt = frac * b.max_stable_dt()
+        state = t.copy()
+        prev = _change(oracle, k, state, dt)
+        for _ in range(300):
+            b.set_cells(state)
+            b.step(dt)
+            state = b.cells.copy()
+            now = _change(oracle, k, state, dt)
+            assert now <= prev * (1.0 + 1e-12), (frac, now, prev)
+            prev = now
+        # a checkerboard of 1e-4 of the heat above 300 K, left where the stencil is smooth
+        i, j, l = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+        board = np.where((i + j + l) % 2 == 0, 1.0, -1.0).reshape(-1)
+        noisy = state + 1e-4 * np.linalg.norm(state - 300.0) / math.sqrt(n ** 3) * board
+        assert _change(oracle, k, noisy, dt) > 1.1 * prev

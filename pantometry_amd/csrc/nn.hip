@@ -105,7 +105,7 @@ __device__ __forceinline__ int div_small(int q, int d, float inv) {
 // the resolution (nearest-neighbour upsampling) and channels [cup, Cin) are xs at full
 // resolution; with cup = 0 all Cin channels are xs.
 template <int NT, int RB, int SK>
-__global__ __launch_bounds__(256) void conv_fwd2(Conv g, const bf16 *__restrict__ x,
+__global__ __launch_bounds__(256, NT == 1 ? 4 : 2) void conv_fwd2(Conv g, const bf16 *__restrict__ x,
                                                  const bf16 *__restrict__ xs, int cup,
                                                  const bf16 *__restrict__ w,
                                                  const float *__restrict__ bias,
@@ -164,6 +164,12 @@ __global__ __launch_bounds__(256) void conv_fwd2(Conv g, const bf16 *__restrict_
     bool in[SK];  // inside the volume (else the loaded value is replaced by zero)
     const int cs = g.Cin - cup;  // channels of xs
     const float inv_sw = 1.0f / (float)SW, inv_sh = 1.0f / (float)SH;
+    // Offsets are 32-bit (the launcher checks that every tensor's elements fit), and the tile's
+    // own part of them is uniform: the box origin's position index, times the channel count, on
+    // the scalar unit; each element adds its small signed displacement in 24-bit products.
+    const int HW = g.H * g.W;
+    const int pbase = ((b * g.D + d0) * g.H + h0) * g.W + w0;
+    const int sbase = pbase * cs;
 #pragma unroll
     for (int k = 0; k < SK; k++) {
         const int e = tid + 256 * k;
@@ -174,10 +180,23 @@ __global__ __launch_bounds__(256) void conv_fwd2(Conv g, const bf16 *__restrict_
         const bool staged_here = e < 2 * npos;
         in[k] = staged_here && z >= 0 && z < g.D && y >= 0 && y < g.H && xx >= 0 && xx < g.W;
         dst[k] = staged_here ? half * npos + p : 2 * npos;
-        src[k] = in[k] ? (int)(((((int64_t)b * g.D + z) * g.H + y) * g.W + xx) * cs + half * 8) : 0;
-        srcu[k] = in[k] && cup > 0
-                      ? (int)(((((int64_t)b * (g.D >> 1) + (z >> 1)) * (g.H >> 1) + (y >> 1)) * (g.W >> 1) + (xx >> 1)) * cup + half * 8)
-                      : 0;
+        const int rel = __mul24(pz - 1, HW) + __mul24(py - 1, g.W) + (px - 1);
+        src[k] = in[k] ? sbase + __mul24(rel, cs) + half * 8 : 0;
+        srcu[k] = 0;
+    }
+    if (cup > 0) {
+        const unsigned Dh = g.D >> 1, Hh = g.H >> 1, Wh = g.W >> 1;
+#pragma unroll
+        for (int k = 0; k < SK; k++) {
+            const int e = tid + 256 * k;
+            const int p = e >> 1, half = e & 1;
+            const int t = div_small(p, SW, inv_sw), pz = div_small(t, SH, inv_sh);
+            const int px = p - t * SW, py = t - pz * SH;
+            const unsigned z = (unsigned)(d0 + pz - 1) >> 1, y = (unsigned)(h0 + py - 1) >> 1;
+            const unsigned xx = (unsigned)(w0 + px - 1) >> 1;
+            const unsigned ph = __umul24(__umul24(__umul24((unsigned)b, Dh) + z, Hh) + y, Wh) + xx;
+            srcu[k] = in[k] ? (int)(__umul24(ph, (unsigned)cup) + half * 8) : 0;
+        }
     }
     // With one or two N-blocks the chunk's weights go through the same pipeline into LDS after
     // the neighbourhood: slot (tap * 2 + half) * NB + column holds input channels half*8 .. +8
@@ -271,7 +290,7 @@ __global__ __launch_bounds__(256) void conv_fwd2(Conv g, const bf16 *__restrict_
                 if (q >= TP) continue;
                 int qd, qh, qw;
                 box_coords(g, q, qd, qh, qw);
-                const int64_t pos = (((int64_t)b * g.D + d0 + qd) * g.H + h0 + qh) * g.W + w0 + qw;
+                const int pos = pbase + __mul24(qd, HW) + __mul24(qh, g.W) + qw;
 #pragma unroll
                 for (int n = 0; n < NT; n++) pz[pos * g.Cout + n0 + n * 16 + (lane & 15)] = acc[rb][n][r];
             }
@@ -321,7 +340,7 @@ __global__ __launch_bounds__(256) void conv_fwd2(Conv g, const bf16 *__restrict_
             const int q = e / (NB / 8), part = e % (NB / 8);
             int qd, qh, qw;
             box_coords(g, q, qd, qh, qw);
-            const int64_t pos = (((int64_t)b * g.D + d0 + qd) * g.H + h0 + qh) * g.W + w0 + qw;
+            const int pos = pbase + __mul24(qd, HW) + __mul24(qh, g.W) + qw;
             *(uint4 *)(out + pos * ostride + ooff + n0 + part * 8) = *(const uint4 *)(smem + q * NS + part * 8);
         }
     }
@@ -577,7 +596,7 @@ int conv_forward(const Conv &g0, const bf16 *x, const bf16 *w, const float *bias
                     ostride % 8 == 0 && ooff % 8 == 0 &&
                     2 * (g.TD + 2) * (g.TH + 2) * (g.TW + 2) <= 256 * 8 &&
                     (g.TW & (g.TW - 1)) == 0 && (g.TH & (g.TH - 1)) == 0 &&
-                    g.positions() * g.Cin < ((int64_t)1 << 31);
+                    g.positions() * std::max({g.Cin, g.Cout, ostride}) < ((int64_t)1 << 31);
     if (!ok) {
         set_error("conv_forward: shape %dx%dx%dx%d, %d->%d channels unsupported", g.B, g.D, g.H,
                   g.W, g.Cin, g.Cout);

@@ -112,7 +112,10 @@ __global__ __launch_bounds__(256, NT == 1 ? 4 : 2) void conv_fwd2(Conv g, const 
                                                  bf16 *__restrict__ zout,
                                                  bf16 *__restrict__ aout, int gelu_out,
                                                  int ostride, int ooff,
-                                                 float *__restrict__ part) {
+                                                 float *__restrict__ part,
+                                                 bf16 *__restrict__ pool_out,
+                                                 const float *__restrict__ head_w,
+                                                 float *__restrict__ head_y) {
     constexpr int NB = 16 * NT;
     extern __shared__ __align__(16) bf16 smem[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -312,7 +315,8 @@ __global__ __launch_bounds__(256, NT == 1 ? 4 : 2) void conv_fwd2(Conv g, const 
 #pragma unroll
     for (int pass = 0; pass < 2; pass++) {
         bf16 *out = outs[pass];
-        if (!out) continue;
+        // The activations are staged for the pool and the head as well as for their own store.
+        if (!out && (pass == 0 || (!pool_out && !head_y))) continue;
         __syncthreads();
 #pragma unroll
         for (int rb = 0; rb < RB; rb++)
@@ -336,12 +340,50 @@ __global__ __launch_bounds__(256, NT == 1 ? 4 : 2) void conv_fwd2(Conv g, const 
                 row[NS / 2] = hi;
             }
         __syncthreads();
-        for (int e = tid; e < TP * (NB / 8); e += 256) {
-            const int q = e / (NB / 8), part = e % (NB / 8);
-            int qd, qh, qw;
-            box_coords(g, q, qd, qh, qw);
-            const int pos = pbase + __mul24(qd, HW) + __mul24(qh, g.W) + qw;
-            *(uint4 *)(out + pos * ostride + ooff + n0 + part * 8) = *(const uint4 *)(smem + q * NS + part * 8);
+        if (out) {
+            for (int e = tid; e < TP * (NB / 8); e += 256) {
+                const int q = e / (NB / 8), part = e % (NB / 8);
+                int qd, qh, qw;
+                box_coords(g, q, qd, qh, qw);
+                const int pos = pbase + __mul24(qd, HW) + __mul24(qh, g.W) + qw;
+                *(uint4 *)(out + pos * ostride + ooff + n0 + part * 8) = *(const uint4 *)(smem + q * NS + part * 8);
+            }
+        }
+        if (pass == 1 && pool_out) {
+            // 2x2x2 means of the staged box (even extents, checked by the launcher): item e is
+            // channel group e % (NB/8) of coarse position e / (NB/8), the eight fine values
+            // summed in (dz, dy, dx) order.
+            const int PW = g.TW >> 1, PH = g.TH >> 1, PP = (g.TD >> 1) * PH * PW;
+            const int lpw = __builtin_ctz(PW), lph = __builtin_ctz(PH);
+            const int Dh = g.D >> 1, Hh = g.H >> 1, Wh = g.W >> 1;
+            for (int e = tid; e < PP * (NB / 8); e += 256) {
+                const int pq = e / (NB / 8), grp = e % (NB / 8);
+                const int pw = pq & (PW - 1), ph = (pq >> lpw) & (PH - 1), pd = pq >> (lpw + lph);
+                float sum[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const int q = ((2 * pd + (k >> 2)) * g.TH + 2 * ph + ((k >> 1) & 1)) * g.TW + 2 * pw + (k & 1);
+                    float f[8];
+                    unpack8(*(const uint4 *)(smem + q * NS + grp * 8), f);
+#pragma unroll
+                    for (int c = 0; c < 8; c++) sum[c] += f[c];
+                }
+#pragma unroll
+                for (int c = 0; c < 8; c++) sum[c] *= 0.125f;
+                const int ppos = ((b * Dh + (d0 >> 1) + pd) * Hh + (h0 >> 1) + ph) * Wh + (w0 >> 1) + pw;
+                *(uint4 *)(pool_out + ppos * g.Cout + n0 + grp * 8) = pack8(sum);
+            }
+        }
+        if (pass == 1 && head_y) {
+            // The 1x1 head of a 16-channel layer: one position per thread, channels in order.
+            for (int q = tid; q < TP; q += 256) {
+                float sum = head_w[16];
+#pragma unroll
+                for (int c = 0; c < 16; c++) sum += head_w[c] * from_bf16(smem[q * NS + c]);
+                int qd, qh, qw;
+                box_coords(g, q, qd, qh, qw);
+                head_y[pbase + __mul24(qd, HW) + __mul24(qh, g.W) + qw] = sum;
+            }
         }
     }
 }
@@ -481,18 +523,19 @@ template <int NT, int RB>
 static void launch_fwd(const Conv &g, dim3 grid, size_t lds, hipStream_t stream, const bf16 *x,
                        const bf16 *xs, int cup,
                        const bf16 *w, const float *bias, bf16 *z, bf16 *a, bool gelu_out,
-                       int ostride, int ooff, float *part) {
+                       int ostride, int ooff, float *part, bf16 *pool, const float *head,
+                       float *head_y) {
     // Staged 16-byte halves per thread: 2 (TD+2)(TH+2)(TW+2) over 256 threads.
     const int halves = 2 * (g.TD + 2) * (g.TH + 2) * (g.TW + 2);
     const int sk = (halves + 255) / 256;
     if (sk <= 2)
-        conv_fwd2<NT, RB, 2><<<grid, 256, lds, stream>>>(g, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part);
+        conv_fwd2<NT, RB, 2><<<grid, 256, lds, stream>>>(g, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part, pool, head, head_y);
     else if (sk <= 4)
-        conv_fwd2<NT, RB, 4><<<grid, 256, lds, stream>>>(g, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part);
+        conv_fwd2<NT, RB, 4><<<grid, 256, lds, stream>>>(g, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part, pool, head, head_y);
     else if (sk <= 6)
-        conv_fwd2<NT, RB, 6><<<grid, 256, lds, stream>>>(g, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part);
+        conv_fwd2<NT, RB, 6><<<grid, 256, lds, stream>>>(g, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part, pool, head, head_y);
     else
-        conv_fwd2<NT, RB, 8><<<grid, 256, lds, stream>>>(g, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part);
+        conv_fwd2<NT, RB, 8><<<grid, 256, lds, stream>>>(g, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part, pool, head, head_y);
 }
 
 // Sums the split partials of conv_fwd2 in split order, adds the bias and writes z and act(z) as
@@ -573,7 +616,7 @@ Conv transposed(const Conv &g) {
 
 int conv_forward(const Conv &g0, const bf16 *x, const bf16 *w, const float *bias, bf16 *z, bf16 *a,
                  bool gelu_out, int ostride, int ooff, hipStream_t stream, float *work,
-                 int64_t work_floats, const bf16 *skip, int cup) {
+                 int64_t work_floats, const bf16 *skip, int cup, FwdFuse *fuse) {
     // Without a skip tensor the whole input is x; with one, x is the half-resolution part.
     const bf16 *xs = skip ? skip : x;
     if (!skip) cup = 0;
@@ -607,16 +650,32 @@ int conv_forward(const Conv &g0, const bf16 *x, const bf16 *w, const float *bias
     const size_t epi = (size_t)m * (16 * nt + 8) * sizeof(bf16);
     const size_t lds = std::max(stage, epi);
     dim3 grid((unsigned)g.tiles(), (unsigned)(g.Cout / (16 * nt)), (unsigned)plan.splits);
+    // The pool and the head in the epilogue: a launch that is not split, over boxes of even
+    // extents (pool) or one 16-channel N-block after GELU (head).
+    bf16 *pool = nullptr;
+    const float *head = nullptr;
+    float *head_y = nullptr;
+    if (fuse) {
+        fuse->fused = false;
+        const bool pool_ok = !fuse->pool || (g.TD % 2 == 0 && g.TH % 2 == 0 && g.TW % 2 == 0 &&
+                                             ooff == 0 && ostride == g.Cout);
+        const bool head_ok = !fuse->head_y || (g.Cout == 16 && gelu_out && fuse->head);
+        if (!part && pool_ok && head_ok && (fuse->pool || fuse->head_y)) {
+            pool = fuse->pool, head = fuse->head, head_y = fuse->head_y;
+            fuse->fused = true;
+            if (fuse->skip_a) a = nullptr;
+        }
+    }
     const int key = nt * 10 + rb;
     switch (key) {
-        case 14: launch_fwd<1, 4>(g, grid, lds, stream, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part); break;
-        case 24: launch_fwd<2, 4>(g, grid, lds, stream, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part); break;
-        case 34: launch_fwd<3, 4>(g, grid, lds, stream, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part); break;
-        case 44: launch_fwd<4, 4>(g, grid, lds, stream, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part); break;
-        case 11: launch_fwd<1, 1>(g, grid, lds, stream, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part); break;
-        case 21: launch_fwd<2, 1>(g, grid, lds, stream, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part); break;
-        case 31: launch_fwd<3, 1>(g, grid, lds, stream, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part); break;
-        default: launch_fwd<4, 1>(g, grid, lds, stream, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part); break;
+        case 14: launch_fwd<1, 4>(g, grid, lds, stream, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part, pool, head, head_y); break;
+        case 24: launch_fwd<2, 4>(g, grid, lds, stream, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part, pool, head, head_y); break;
+        case 34: launch_fwd<3, 4>(g, grid, lds, stream, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part, pool, head, head_y); break;
+        case 44: launch_fwd<4, 4>(g, grid, lds, stream, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part, pool, head, head_y); break;
+        case 11: launch_fwd<1, 1>(g, grid, lds, stream, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part, pool, head, head_y); break;
+        case 21: launch_fwd<2, 1>(g, grid, lds, stream, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part, pool, head, head_y); break;
+        case 31: launch_fwd<3, 1>(g, grid, lds, stream, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part, pool, head, head_y); break;
+        default: launch_fwd<4, 1>(g, grid, lds, stream, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part, pool, head, head_y); break;
     }
     PM_HIP(hipGetLastError());
     if (part) {

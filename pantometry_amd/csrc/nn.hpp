@@ -28,6 +28,23 @@ __host__ __device__ __forceinline__ float from_bf16(bf16 h) {
     return __builtin_bit_cast(float, (uint32_t)h << 16);
 }
 
+// Eight bf16 channels as one 16-byte word: element-wise kernels move channel groups of eight
+// (every channel count and offset here is a multiple of 16).
+__device__ __forceinline__ void unpack8(uint4 u, float f[8]) {
+    const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        f[2 * q] = from_bf16((bf16)(w[q] & 0xFFFFu));
+        f[2 * q + 1] = from_bf16((bf16)(w[q] >> 16));
+    }
+}
+__device__ __forceinline__ uint4 pack8(const float f[8]) {
+    uint32_t w[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) w[q] = (uint32_t)to_bf16(f[2 * q]) | ((uint32_t)to_bf16(f[2 * q + 1]) << 16);
+    return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
 // tanh-form GELU and its derivative, with tanh(u) written through the logistic function:
 // 0.5 z (1 + tanh(u)) = z / (1 + e^(-2u)) and tanh(u) = 2 / (1 + e^(-2u)) - 1, using the hardware
 // exp2 and reciprocal (v_exp_f32, v_rcp_f32, about 1 ulp each).  The results are rounded to bf16
@@ -83,10 +100,25 @@ inline Conv make_conv(int B, int D, int H, int W, int Cin, int Cout) {
 // With `skip`, the input is the concatenation read in place: channels [0, cup) are x at half the
 // resolution, upsampled by nearest neighbour, and channels [cup, Cin) are `skip` (full
 // resolution, Cin - cup channels), so a decoder layer needs no concatenated buffer.
+//
+// `fuse` (optional) asks the epilogue for more from the activations it has staged: the 2x2x2
+// average pool of a (channels-last, Cout channels, at half the resolution; the sum of the eight
+// values in (dz, dy, dx) order times 0.125, rounded once), and the 1x1 head of a 16-channel
+// layer (y[p] = head[16] + sum_c head[c] a[p][c], fp32, c in order).  The launch sets `fused`
+// when it produced them, and then stores no `a` if `skip_a`; when it could not (a split launch,
+// a box with odd extents, more than 16 channels for the head) it stores `a` and leaves them to
+// the caller.
+struct FwdFuse {
+    bf16 *pool = nullptr;
+    const float *head = nullptr;
+    float *head_y = nullptr;
+    bool skip_a = false;
+    bool fused = false;
+};
 int conv_forward(const Conv &g, const bf16 *x, const bf16 *w, const float *bias, bf16 *z,
                  bf16 *a, bool gelu_out, int ostride, int ooff, hipStream_t stream,
                  float *work = nullptr, int64_t work_floats = 0, const bf16 *skip = nullptr,
-                 int cup = 0);
+                 int cup = 0, FwdFuse *fuse = nullptr);
 // Floats of `work` conv_forward would use for this shape (0: it does not split).
 int64_t conv_forward_workspace(const Conv &g);
 // dx = conv(dy, flip(w)^T): the gradient of the input, fp32 accumulation, bf16 out.

@@ -110,23 +110,6 @@ __global__ void gather_target(const float *__restrict__ set, const int *__restri
     }
 }
 
-// Eight bf16 channels as one 16-byte word: element-wise kernels move channel groups of eight
-// (every channel count and offset here is a multiple of 16).
-__device__ __forceinline__ void unpack8(uint4 u, float f[8]) {
-    const uint32_t w[4] = {u.x, u.y, u.z, u.w};
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        f[2 * q] = from_bf16((bf16)(w[q] & 0xFFFFu));
-        f[2 * q + 1] = from_bf16((bf16)(w[q] >> 16));
-    }
-}
-__device__ __forceinline__ uint4 pack8(const float f[8]) {
-    uint32_t w[4];
-#pragma unroll
-    for (int q = 0; q < 4; q++) w[q] = (uint32_t)to_bf16(f[2 * q]) | ((uint32_t)to_bf16(f[2 * q + 1]) << 16);
-    return make_uint4(w[0], w[1], w[2], w[3]);
-}
-
 // 2x2x2 average pool, channels-last: (sum of the eight fine values) * 0.125, rounded once.
 __global__ void pool_fwd(const bf16 *__restrict__ x, bf16 *__restrict__ y, int B, int D, int H,
                          int W, int C) {
@@ -583,11 +566,24 @@ int run_forward(pm_surrogate *s, bool training) {
     const float *P = s->params.as<float>();
     const bf16 *WB = s->wbf.as<bf16>();
     auto R = [&](int l) { return s->R >> l; };
-    auto conv = [&](int li, const bf16 *in, Act &o, const bf16 *skip = nullptr, int cup = 0) -> int {
+    auto conv = [&](int li, const bf16 *in, Act &o, const bf16 *skip = nullptr, int cup = 0,
+                    nn::FwdFuse *fuse = nullptr) -> int {
         const nn::Conv g = conv_of(s, li);
         return nn::conv_forward(g, in, WB + s->layers[li].wbf_off, P + s->layers[li].b_off,
                                 training ? o.z : nullptr, o.a, true, g.Cout, 0, st,
-                                s->fsplit.as<float>(), s->fsplit_floats, skip, cup);
+                                s->fsplit.as<float>(), s->fsplit_floats, skip, cup, fuse);
+    };
+    // An encoder level's second convolution and its pooling: in one launch where the epilogue
+    // can pool, else the pooling kernel after it.
+    auto conv_pool = [&](int li, const bf16 *in, Act &o, bf16 *pooled, int level, int C) -> int {
+        nn::FwdFuse f;
+        f.pool = pooled;
+        PM_CHECK(conv(li, in, o, nullptr, 0, &f));
+        if (!f.fused) {
+            nn::pool_fwd<<<grid_of(npos(s, level) * C / 8), 256, 0, st>>>(o.a, pooled, s->batch, R(level - 1), R(level - 1), R(level - 1), C);
+            PM_HIP(hipGetLastError());
+        }
+        return PM_OK;
     };
     // A decoder level's input: up(low) ++ skip.  Training keeps the concatenation (its weight
     // gradient reads it); inference reads both parts in place.
@@ -600,16 +596,12 @@ int run_forward(pm_surrogate *s, bool training) {
             skip, cat, npos(s, level), cskip, cskip, 0, clow + cskip, clow, 0);
         return conv(li, cat, s->act[li]);
     };
-    const int B = s->batch;
     PM_CHECK(conv(0, s->xin.as<bf16>(), s->act[0]));
-    PM_CHECK(conv(1, s->act[0].a, s->act[1]));
-    nn::pool_fwd<<<grid_of(npos(s, 1) * 16 / 8), 256, 0, st>>>(s->act[1].a, s->pool[0], B, R(0), R(0), R(0), 16);
+    PM_CHECK(conv_pool(1, s->act[0].a, s->act[1], s->pool[0], 1, 16));
     PM_CHECK(conv(2, s->pool[0], s->act[2]));
-    PM_CHECK(conv(3, s->act[2].a, s->act[3]));
-    nn::pool_fwd<<<grid_of(npos(s, 2) * 32 / 8), 256, 0, st>>>(s->act[3].a, s->pool[1], B, R(1), R(1), R(1), 32);
+    PM_CHECK(conv_pool(3, s->act[2].a, s->act[3], s->pool[1], 2, 32));
     PM_CHECK(conv(4, s->pool[1], s->act[4]));
-    PM_CHECK(conv(5, s->act[4].a, s->act[5]));
-    nn::pool_fwd<<<grid_of(npos(s, 3) * 64 / 8), 256, 0, st>>>(s->act[5].a, s->pool[2], B, R(2), R(2), R(2), 64);
+    PM_CHECK(conv_pool(5, s->act[4].a, s->act[5], s->pool[2], 3, 64));
     PM_CHECK(conv(6, s->pool[2], s->act[6]));
     PM_CHECK(conv(7, s->act[6].a, s->act[7]));
     // decoder level 2: up(enc4) ++ enc3
@@ -618,9 +610,16 @@ int run_forward(pm_surrogate *s, bool training) {
     PM_CHECK(decoder(10, s->act[9].a, 64, s->act[3].a, 32, s->cat[1], 1));
     PM_CHECK(conv(11, s->act[10].a, s->act[11]));
     PM_CHECK(decoder(12, s->act[11].a, 32, s->act[1].a, 16, s->cat[0], 0));
-    PM_CHECK(conv(13, s->act[12].a, s->act[13]));
-    nn::head_fwd<<<grid_of(npos(s, 0)), 256, 0, st>>>(s->act[13].a, P + s->head_off, P + s->head_off + 16, npos(s, 0), s->y.as<float>());
-    PM_HIP(hipGetLastError());
+    // The last convolution and the 1x1 head: in inference one launch, which stores only y.
+    nn::FwdFuse f;
+    f.head = P + s->head_off;
+    f.head_y = s->y.as<float>();
+    f.skip_a = !training;
+    PM_CHECK(conv(13, s->act[12].a, s->act[13], nullptr, 0, &f));
+    if (!f.fused) {
+        nn::head_fwd<<<grid_of(npos(s, 0)), 256, 0, st>>>(s->act[13].a, P + s->head_off, P + s->head_off + 16, npos(s, 0), s->y.as<float>());
+        PM_HIP(hipGetLastError());
+    }
     return PM_OK;
 }
 

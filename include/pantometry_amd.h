@@ -318,9 +318,9 @@ int pm_hall_checkpoint(pm_hall *hall);
 int pm_hall_restore(pm_hall *hall);
 /* Hall::peak_pressure, hall.rs:242-244. */
 double pm_hall_peak_pressure(pm_hall *hall);
-/* Shape of the step: rows per 64-wide column tile (8 or 16; 0 keeps the current) and z-planes
- * per workgroup (0: chosen from the extents, at most 128 and about 1024 workgroups).  No
- * reference counterpart. */
+/* Shape of the step, single and paired: rows per 64-wide column tile (8, the default, or 16; 0
+ * keeps the current) and z-planes per workgroup (0: chosen from the extents).  Results are the
+ * same for every shape.  No reference counterpart. */
 int pm_hall_set_tile(pm_hall *hall, int tile_y, int z_chunk);
 /* How many leapfrog steps pm_hall_advance takes per pass over the grid: 1, or 2 (the default):
  * pairs of steps in one kernel (temporal blocking: p, vx, vy, vz read once per two steps), the

@@ -875,9 +875,9 @@ static int hall_sweep(pm_hall *h, double hstep, double vh, int64_t step, double 
 }
 
 // Steps `step` and `step + 1` (both full leapfrog steps) in one pass, then both audits.
-static constexpr int HALL_PAIR_TY = 8;
 static int hall_sweep_pair(pm_hall *h, double hstep, int64_t step, double *totals_dev, double offset) {
-    const HallGrid g = hall_grid(h, HALL_PAIR_TY);
+    const int ty = h->tile_y == 16 ? 16 : 8;
+    const HallGrid g = hall_grid(h, ty);
     const int64_t nparts = g.tiles();
     PM_CHECK(hall_partials(h, 2 * nparts));
     HallArgs a = hall_args(h, hstep, hstep);
@@ -894,7 +894,10 @@ static int hall_sweep_pair(pm_hall *h, double hstep, int64_t step, double *total
         PM_HIP(hipGetLastError());
     }
     const unsigned blocks = (unsigned)(XCDS_HALL * ((nparts + XCDS_HALL - 1) / XCDS_HALL));
-    hall_march2<HALL_PAIR_TY><<<blocks, dim3(64, HALL_PAIR_TY), 0, h->stream>>>(a, g.gx, g.gy, g.gz, g.zc);
+    if (ty == 16)
+        hall_march2<16><<<blocks, dim3(64, 16), 0, h->stream>>>(a, g.gx, g.gy, g.gz, g.zc);
+    else
+        hall_march2<8><<<blocks, dim3(64, 8), 0, h->stream>>>(a, g.gx, g.gy, g.gz, g.zc);
     PM_HIP(hipGetLastError());
     hall_finalize2<<<1, 1024, 0, h->stream>>>(h->partials.d(), nparts, h->audit.as<AuditState>(), offset,
                                               h->scalars.d() + 1, s1.node >= 0 ? h->scalars.d() + 2 : nullptr,

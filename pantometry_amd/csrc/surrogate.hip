@@ -50,9 +50,9 @@ constexpr int K_STEPS = 16;  // stencil steps per surrogate step
 //   15     1 (marks the volume: the convolutions pad with zeros)
 template <class TT>
 __device__ __forceinline__ void cell_features(const TT *__restrict__ t, const TT *__restrict__ k,
-                                              int64_t base, int edge, int64_t c, float coupling,
-                                              bf16 *__restrict__ out) {
-    const int x = (int)(c % edge), y = (int)((c / edge) % edge), z = (int)(c / ((int64_t)edge * edge));
+                                              int64_t base, int edge, int x, int y, int z,
+                                              float coupling, bf16 *__restrict__ out) {
+    const int64_t c = x + (int64_t)edge * (y + (int64_t)edge * z);
     const double tc = (double)t[base + c], kc = (double)k[base + c];
     const int64_t e = edge, ee = (int64_t)edge * edge;
     const int64_t off[6] = {-1, 1, -e, e, -ee, ee};
@@ -83,12 +83,17 @@ __device__ __forceinline__ void cell_features(const TT *__restrict__ t, const TT
     *(uint4 *)(out + 8) = make_uint4(w[4], w[5], w[6], w[7]);
 }
 
-// Features of one block from fp64 device fields.
-__global__ void pack_input(const double *__restrict__ t, const double *__restrict__ k, int edge,
-                           float coupling, bf16 *__restrict__ out) {
-    const int64_t n = (int64_t)edge * edge * edge;
-    GRID_STRIDE(i, n) { cell_features<double>(t, k, 0, edge, i, coupling, out + i * 16); }
+// Features of one block from fp64 device fields: a grid of (x blocks of 64, y, z), so a cell's
+// coordinates come from the launch, not from divisions.
+__global__ __launch_bounds__(64) void pack_input(const double *__restrict__ t,
+                                                 const double *__restrict__ k, int edge,
+                                                 float coupling, bf16 *__restrict__ out) {
+    const int x = (int)(blockIdx.x * 64 + threadIdx.x), y = (int)blockIdx.y, z = (int)blockIdx.z;
+    if (x >= edge) return;
+    const int64_t c = x + (int64_t)edge * (y + (int64_t)edge * z);
+    cell_features<double>(t, k, 0, edge, x, y, z, coupling, out + c * 16);
 }
+static dim3 pack_grid(int64_t edge) { return dim3((unsigned)((edge + 63) / 64), (unsigned)edge, (unsigned)edge); }
 
 // Features of a batch drawn from the training set (fp32 fields [sample][cell]).
 __global__ void pack_batch(const float *__restrict__ t, const float *__restrict__ k,
@@ -98,7 +103,8 @@ __global__ void pack_batch(const float *__restrict__ t, const float *__restrict_
     GRID_STRIDE(i, n * batch) {
         const int64_t s = i / n, c = i % n;
         const int64_t base = (int64_t)idx[s] * n;
-        cell_features<float>(t, k, base, edge, c, coupling[idx[s]], out + i * 16);
+        const int cc = (int)c, x = cc % edge, y = (cc / edge) % edge, z = cc / (edge * edge);
+        cell_features<float>(t, k, base, edge, x, y, z, coupling[idx[s]], out + i * 16);
     }
 }
 
@@ -329,28 +335,25 @@ __global__ void to_bf16_kernel(const float *__restrict__ x, bf16 *__restrict__ y
     GRID_STRIDE(i, n) { y[i] = to_bf16(x[i]); }
 }
 
-__global__ __launch_bounds__(256) void sum_f64(const double *__restrict__ x, int64_t n,
-                                               double *__restrict__ part) {
-    __shared__ double red[4];
-    double s = 0.0;
-    GRID_STRIDE(i, n) { s += x[i]; }
-    const double tot = block_sum<256>(s, red);
-    if (threadIdx.x == 0) part[blockIdx.x] = tot;
-}
-
-// T += 10 K * y in place (fp64), with per-block partial sums of the new T.
+// T += 10 K * y in place (fp64), with per-block partial sums of T before (part0) and after
+// (part1): each thread folds its cells in index order, then the fixed block tree.
 __global__ __launch_bounds__(256) void apply_prediction_sum(double *__restrict__ t,
                                                             const float *__restrict__ y, int64_t n,
-                                                            double *__restrict__ part) {
+                                                            double *__restrict__ part0,
+                                                            double *__restrict__ part1) {
     __shared__ double red[4];
-    double s = 0.0;
+    double s0 = 0.0, s1 = 0.0;
     GRID_STRIDE(i, n) {
-        const double v = t[i] + (double)OUT_SCALE * (double)y[i];
+        const double u = t[i];
+        const double v = u + (double)OUT_SCALE * (double)y[i];
         t[i] = v;
-        s += v;
+        s0 += u;
+        s1 += v;
     }
-    const double tot = block_sum<256>(s, red);
-    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+    const double tot0 = block_sum<256>(s0, red);
+    __syncthreads();
+    const double tot1 = block_sum<256>(s1, red);
+    if (threadIdx.x == 0) part0[blockIdx.x] = tot0, part1[blockIdx.x] = tot1;
 }
 
 // scal[0] = sum T before, scal[1] = after the prediction, scal[2] = the uniform shift that
@@ -959,7 +962,7 @@ int pm_surrogate_forward(pm_surrogate *s, int64_t edge, const double *t, const d
     PM_CHECK(dk.alloc(sizeof(double) * (size_t)n));
     PM_HIP(hipMemcpyAsync(dt.d(), t, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, s->stream));
     PM_HIP(hipMemcpyAsync(dk.d(), k, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, s->stream));
-    nn::pack_input<<<grid_of(n), 256, 0, s->stream>>>(dt.d(), dk.d(), (int)edge, (float)coupling, s->xin.as<bf16>());
+    nn::pack_input<<<nn::pack_grid(edge), 64, 0, s->stream>>>(dt.d(), dk.d(), (int)edge, (float)coupling, s->xin.as<bf16>());
     PM_CHECK(run_forward(s, false));
     PM_HIP(hipMemcpyAsync(y, s->y.d(), sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, s->stream));
     PM_HIP(hipStreamSynchronize(s->stream));
@@ -999,10 +1002,9 @@ int pm_surrogate_step_solid(pm_surrogate *s, pm_solid3d *solid, double dt, doubl
     if (!s->ev1) PM_HIP(hipEventCreate(&s->ev1));
     const unsigned rb = std::min<unsigned>(grid_of(n), MAXB);
     PM_HIP(hipEventRecord(s->ev0, st));
-    nn::pack_input<<<grid_of(n), 256, 0, st>>>(cells, kc, (int)edge, (float)(dt * mob), s->xin.as<bf16>());
+    nn::pack_input<<<nn::pack_grid(edge), 64, 0, st>>>(cells, kc, (int)edge, (float)(dt * mob), s->xin.as<bf16>());
     PM_CHECK(run_forward(s, false));
-    nn::sum_f64<<<rb, 256, 0, st>>>(cells, n, part0);
-    nn::apply_prediction_sum<<<rb, 256, 0, st>>>(cells, s->y.as<float>(), n, part1);
+    nn::apply_prediction_sum<<<rb, 256, 0, st>>>(cells, s->y.as<float>(), n, part0, part1);
     nn::shift_finalize<<<1, 1024, 0, st>>>(part0, part1, rb, n, scal);
     nn::shift_sum<<<rb, 256, 0, st>>>(cells, n, scal, part2);
     nn::drift_finalize<<<1, 1024, 0, st>>>(part2, rb, scal);

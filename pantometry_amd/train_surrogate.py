@@ -14,15 +14,17 @@ With --rollout-pairs N the steps are split into `rounds` + 1 stages; after each 
 surrogate steps from a stencil state, against the stencil from there) and trained on beside the
 stencil pairs.  --init starts from saved parameters (Adam's moments start at zero again).
 
-The shipped weights (pantometry_amd/weights/heat_unet_k16.npz) were made by three runs in turn,
-about 17 minutes on one MI355X:
+The shipped weights (pantometry_amd/weights/heat_unet_k16.npz) were made by four runs in turn,
+about 24 minutes on one MI355X:
 
     python -m pantometry_amd.train_surrogate --steps 6000 --rollout-pairs 2048 --rounds 2 \
         --out stage1.npz
     python -m pantometry_amd.train_surrogate --init stage1.npz --lr 3e-4 --steps 8000 \
         --rollout-pairs 4096 --rounds 3 --out stage2.npz
     python -m pantometry_amd.train_surrogate --init stage2.npz --lr 2e-4 --steps 8000 \
-        --rollout-pairs 4096 --rounds 3 --unroll 6
+        --rollout-pairs 4096 --rounds 3 --unroll 6 --out stage3.npz
+    python -m pantometry_amd.train_surrogate --init stage3.npz --lr 1e-4 --steps 8000 \
+        --rollout-pairs 4096 --rounds 3 --unroll 8
 """
 from __future__ import annotations
 

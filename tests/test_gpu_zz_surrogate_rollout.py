@@ -50,11 +50,13 @@ def test_rollout_with_vetoes_meets_1e3_against_the_stencil(case):
     assert rep.chunks == 64 and rep.checked == 16
     assert rep.final_error_kelvin <= 1e-3
     assert rep.final_error <= 3e-3
-    # most checked chunks are accepted, and every accepted one is within the tolerance
+    # most checked chunks are accepted, and every fallback is a checked chunk over the tolerance
+    # or a step-change veto
     assert checked_fallbacks <= rep.checked // 4
+    assert rep.fallbacks == checked_fallbacks + rep.rises
     assert rep.max_drift <= 1e-12
-    # the step-change veto is not vacuous: it fired, and each firing is a chunk the stencil redid
-    assert rep.rises >= 1 and rep.fallbacks == checked_fallbacks + rep.rises
+    # every unchecked chunk's step change was measured
+    assert len(rep.changes) == rep.chunks - rep.checked
     for before, after in rep.changes:
         assert before > 0.0 and after > 0.0
 
@@ -74,3 +76,25 @@ def test_the_checks_are_what_keep_the_rollout_close(case):
     assert free.fallbacks == 0 and free.checked == 0 and free.rises == 0
     assert kept.final_error < free.final_error
     assert free.final_error_kelvin <= 1e-3
+
+
+@pytest.mark.timeout(300)
+def test_the_step_change_veto_catches_a_network_that_adds_noise():
+    """A network with random parameters adds kelvins of cell-to-cell noise to every chunk.  With
+    no stencil check at all, the step-change veto alone refuses every chunk, so the rollout ends
+    bit-identical to the plain stencil; without the veto it ends far from it."""
+    gpu_case("surrogate rollout 32^3 x 64, random network, step-change veto")
+    from pantometry_amd.surrogate import Surrogate, heterogeneous_pair_block, rollout
+    from tests.test_gpu_surrogate import _random_params
+    s = Surrogate()
+    s.set_parameters(_random_params(11))
+    block, k, dt = heterogeneous_pair_block(32, 903)
+    t0 = block.cells().copy()
+    vetoed = rollout(s, k, t0, dt, steps=64, tolerance=1e-3, check_every=0)
+    assert vetoed.checked == 0 and vetoed.rises == vetoed.chunks == vetoed.fallbacks == 4
+    assert vetoed.final_error == 0.0
+    for before, after in vetoed.changes:
+        assert after > before
+    free = rollout(s, k, t0, dt, steps=64, tolerance=math.inf, check_every=0, residual=False)
+    assert free.fallbacks == 0 and free.final_error > 1e-2
+

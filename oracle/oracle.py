@@ -52,6 +52,8 @@ _SIG = {
     "orc_mode_amplitude": (_D, [_S, _S, _S, _I, _I, _I, _DP]),
     "orc_mode_amplification": (_D, [_S, _S, _S, _I, _I, _I, _D, _D, _D, _D, _D]),
     "orc_hall_invariant": (_D, [_S, _S, _S, _D, _D, _D, _DP, _DP, _DP, _DP, _DP]),
+    "orc_hall_invariant_compensated_omp": (_D, [_S, _S, _S, _D, _D, _D, _DP, _DP, _DP, _DP, _DP,
+                                                _I]),
     "orc_hall_sweep": (_D, [_S, _S, _S, _D, _D, _D, _D, _D, _DP, _DP, _DP, _DP, _DP, _S, _D, _D,
                             _I]),
     "orc_hall_mode": (None, [_S, _S, _S, _D, _I, _I, _I, _D, _DP]),
@@ -602,6 +604,14 @@ class OracleHall:
 
     def energy(self) -> float:
         return self.invariant() + self.offset
+
+    def invariant_compensated(self, threads_: int = 1) -> float:
+        """The invariant's terms, each as `invariant` computes it, summed with compensation:
+        the exact sum the reference's left fold approximates (test infrastructure for the
+        full-length ledger checks, not a reference function)."""
+        return lib().orc_hall_invariant_compensated_omp(
+            self.nx, self.ny, self.nz, self.dx, self.speed, self.density, p(self.p), p(self.pp),
+            p(self.vx), p(self.vy), p(self.vz), threads_)
 
     def release(self, pressure: np.ndarray) -> None:
         self.p = np.ascontiguousarray(pressure, dtype=np.float64).reshape(-1).copy()

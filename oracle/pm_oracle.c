@@ -628,6 +628,78 @@ double orc_hall_invariant(size_t nx, size_t ny, size_t nz, double dx, double spe
     return total;
 }
 
+/* The same terms as orc_hall_invariant, each evaluated as there, summed with Neumaier's
+ * compensation instead of a left fold: per (group, plane) on `threads` threads, the partial sums
+ * and compensations then combined in group and plane order.  Within ~1e-16 relative of the exact
+ * sum of the terms, where the reference's left fold over tens of millions of nodes drifts by
+ * ~1e-10.  Deterministic for any thread count. */
+static void orc_neumaier(double *sum, double *comp, double x) {
+    double t = *sum + x;
+    if (fabs(*sum) >= fabs(x))
+        *comp += (*sum - t) + x;
+    else
+        *comp += (x - t) + *sum;
+    *sum = t;
+}
+
+double orc_hall_invariant_compensated_omp(size_t nx, size_t ny, size_t nz, double dx, double speed,
+                                          double density, const double *p, const double *pp,
+                                          const double *vx, const double *vy, const double *vz,
+                                          int threads) {
+    double volume = dx * dx * dx;
+    double rc2 = density * speed * speed;
+    if (rc2 <= 0.0) return 0.0;
+    if (threads < 1) threads = 1;
+    double half_rho = 0.5 * density;
+    double *part = (double *)calloc(8 * nz, sizeof(double));  /* [group][plane][sum, comp] */
+    if (!part) return NAN;
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (long long kk = 0; kk < (long long)nz; kk++) {
+        size_t k = (size_t)kk;
+        double s, c;
+        s = 0.0, c = 0.0;
+        for (size_t j = 0; j < ny; j++)
+            for (size_t i = 0; i < nx; i++) {
+                double share = 1.0 / (orc_wall(i, nx) * orc_wall(j, ny) * orc_wall(k, nz));
+                size_t idx = i + nx * (j + ny * k);
+                orc_neumaier(&s, &c, share * p[idx] * pp[idx] / (2.0 * rc2) * volume);
+            }
+        part[(0 * nz + k) * 2] = s, part[(0 * nz + k) * 2 + 1] = c;
+        s = 0.0, c = 0.0;
+        for (size_t j = 0; j < ny; j++)
+            for (size_t i = 0; i + 1 < nx; i++) {
+                double v = vx[i + (nx - 1) * (j + ny * k)];
+                double share = 1.0 / (orc_wall(j, ny) * orc_wall(k, nz));
+                orc_neumaier(&s, &c, share * half_rho * v * v * volume);
+            }
+        part[(1 * nz + k) * 2] = s, part[(1 * nz + k) * 2 + 1] = c;
+        s = 0.0, c = 0.0;
+        for (size_t j = 0; j + 1 < ny; j++)
+            for (size_t i = 0; i < nx; i++) {
+                double v = vy[i + nx * (j + (ny - 1) * k)];
+                double share = 1.0 / (orc_wall(i, nx) * orc_wall(k, nz));
+                orc_neumaier(&s, &c, share * half_rho * v * v * volume);
+            }
+        part[(2 * nz + k) * 2] = s, part[(2 * nz + k) * 2 + 1] = c;
+        s = 0.0, c = 0.0;
+        if (k + 1 < nz)
+            for (size_t j = 0; j < ny; j++)
+                for (size_t i = 0; i < nx; i++) {
+                    double v = vz[i + nx * (j + ny * k)];
+                    double share = 1.0 / (orc_wall(i, nx) * orc_wall(j, ny));
+                    orc_neumaier(&s, &c, share * half_rho * v * v * volume);
+                }
+        part[(3 * nz + k) * 2] = s, part[(3 * nz + k) * 2 + 1] = c;
+    }
+    double sum = 0.0, comp = 0.0;
+    for (size_t q = 0; q < 4 * nz; q++) {
+        orc_neumaier(&sum, &comp, part[2 * q]);
+        comp += part[2 * q + 1];
+    }
+    free(part);
+    return sum + comp;
+}
+
 /* The sweeps of Hall::step, hall.rs:430-505, after the Courant guard and the stagger choice
  * (the caller passes vh = h/2 on the first step, h afterwards).  pressure_prev receives the
  * pre-update pressure (hall.rs:463).  A point source adds `source` Pa at node `src` after the

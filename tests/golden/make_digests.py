@@ -6,7 +6,8 @@ step: the left fold the reference computes and, for the heat block, the compensa
   heat      512^3 heterogeneous block (k lognormal(0, 0.5), 8 hot spots), 2000 steps at 0.9 of
             its limit, the ledger after every step (solid.rs:1773-1826, 1557-1561);
   hall      385^3 nodes, CFL 0.5, Ricker source at the seeded node, 4000 steps, the energy ledger
-            after every step (hall.rs:408-530, 334-384);
+            after every step (hall.rs:408-530, 334-384): the reference's left fold, the same
+            terms summed with compensation, and the ledger's scale;
   ensemble  512 uniform members (edges 16-96, alpha 1e-5..1e-3 log-uniform, hot spot + noise),
             500 steps each (ensemble.rs:102-147), each member's stored heat at the end.
 
@@ -119,7 +120,7 @@ def hall(rec: dict, arrays: dict, cells: int = 384, steps: int = 4000) -> None:
     h.set_source(node, inc)
     out = {"nodes": [nx, ny, nz], "steps": steps, "dt": dt.hex(), "source_node": node,
            "increments_sha256": sha(inc)}
-    totals = np.empty(steps)
+    totals, exact, scale = np.empty(steps), np.empty(steps), np.empty(steps)
     start = 0
     saved = _restart("hall")
     if saved is not None:
@@ -130,20 +131,24 @@ def hall(rec: dict, arrays: dict, cells: int = 384, steps: int = 4000) -> None:
         h.offset, h.datum, h.supplied = (float(saved[x]) for x in ("offset", "datum", "supplied"))
         h.cursor = int(saved["cursor"])
         totals[:start] = saved["totals"][:start]
+        exact[:start], scale[:start] = saved["exact"][:start], saved["scale"][:start]
         print(f"hall: resuming after step {start}", flush=True)
     t0 = time.time()
     for s in range(start, steps):
         h.step(dt, threads_=THREADS)
-        totals[s] = h.ledger()[0]
+        totals[s], scale[s] = h.ledger()
+        # Ledger::with over [energy, -supplied] with the energy's terms summed exactly
+        exact[s] = (h.invariant_compensated(THREADS) + h.offset) + (-h.supplied)
         if s % SAVE_EVERY == SAVE_EVERY - 1:
             print(f"hall step {s + 1} ({time.time() - t0:.0f} s)", flush=True)
             _save_restart("hall", done=s + 1, p=h.p, pp=h.pp, vx=h.vx, vy=h.vy, vz=h.vz,
                           staggered=h.staggered, offset=h.offset, datum=h.datum,
-                          supplied=h.supplied, cursor=h.cursor, totals=totals)
+                          supplied=h.supplied, cursor=h.cursor, totals=totals, exact=exact,
+                          scale=scale)
     out["final_sha256"] = {name: sha(getattr(h, name)) for name in ("p", "pp", "vx", "vy", "vz")}
     out["supplied"] = float(h.supplied).hex()
     rec["hall"] = out
-    arrays["hall_totals"] = totals
+    arrays["hall_totals"], arrays["hall_exact"], arrays["hall_scale"] = totals, exact, scale
 
 
 def _member(m: int, steps: int):

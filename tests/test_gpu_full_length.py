@@ -9,7 +9,10 @@ bit and every step's ledger total:
 
   heat      512^3 x 2000 steps: every total within 1e-12 of the compensated sum and no farther
             from the reference's left fold than that fold's own rounding (plus 1e-12);
-  hall      385^3 x 4000 steps with the Ricker source: every total within 1e-12 of the oracle's;
+  hall      385^3 x 4000 steps with the Ricker source: every total within 1e-12 (of that
+            step's ledger scale) of the compensated sum, and no farther from the reference's
+            left fold than that fold's own rounding (plus 1e-12); the fold over 57 M nodes
+            drifts by up to ~1e-10 of the scale;
   ensemble  512 members x 500 steps in one graph launch: every member's cells bit for bit and its
             stored heat within 1e-12.
 """
@@ -39,17 +42,17 @@ def digests():
     return rec, arrays
 
 
-def _record(digests, name):
+def _record(digests, name, keys):
     rec, arrays = digests
-    if name not in rec:
-        pytest.skip(f"no {name} digest in tests/golden/full_length.json "
+    if name not in rec or any(k not in arrays for k in keys):
+        pytest.skip(f"no {name} digest in tests/golden/ "
                     f"(python tests/golden/make_digests.py {name})")
     return rec[name], arrays
 
 
 @pytest.mark.timeout(600)
 def test_heat_512_for_2000_steps(digests):
-    d, arrays = _record(digests, "heat")
+    d, arrays = _record(digests, "heat", ("heat_fold", "heat_exact"))
     gpu_case("heat 512^3 x 2000 against the oracle's digests")
     import pantometry_amd as pm
     from pantometry_amd import workloads as w
@@ -74,7 +77,7 @@ def test_heat_512_for_2000_steps(digests):
 
 @pytest.mark.timeout(600)
 def test_hall_385_for_4000_steps(digests):
-    d, arrays = _record(digests, "hall")
+    d, arrays = _record(digests, "hall", ("hall_totals", "hall_exact", "hall_scale"))
     gpu_case("hall 385^3 x 4000 against the oracle's digests")
     from pantometry_amd import workloads as w
     case = w.acoustic_hall(d["nodes"][0] - 1, d["steps"])
@@ -85,14 +88,17 @@ def test_hall_385_for_4000_steps(digests):
     for name, field in zip(("p", "pp", "vx", "vy", "vz"), h.fields()):
         assert sha(field) == d["final_sha256"][name], name
     assert h.supplied() == float.fromhex(d["supplied"])
-    ref = arrays["hall_totals"]
-    scale = max(abs(h.energy()), abs(h.supplied()))
-    assert np.all(np.abs(totals - ref) <= 1e-12 * scale)
+    fold, exact, scale = arrays["hall_totals"], arrays["hall_exact"], arrays["hall_scale"]
+    assert np.all(np.abs(totals - exact) <= 1e-12 * scale)
+    assert np.all(np.abs(totals - fold) <= np.abs(fold - exact) + 1e-12 * scale)
+    print(f"largest |gpu - exact| / scale {np.max(np.abs(totals - exact) / np.maximum(scale, 1e-300)):.3e}, "
+          f"largest |fold - exact| / scale {np.max(np.abs(fold - exact) / np.maximum(scale, 1e-300)):.3e}",
+          flush=True)
 
 
 @pytest.mark.timeout(600)
 def test_ensemble_of_512_for_500_steps(digests):
-    d, arrays = _record(digests, "ensemble")
+    d, arrays = _record(digests, "ensemble", ("ensemble_stored",))
     gpu_case("ensemble 512 x 500 against the oracle's digests")
     from pantometry_amd import workloads as w
     ens, shapes = w.heat_ensemble(d["count"])

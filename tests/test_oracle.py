@@ -133,6 +133,45 @@ def test_the_hall_source_is_booked(oracle):
     assert max(abs(x - totals[0]) for x in totals) <= 1e-12 * scale
 
 
+def test_the_compensated_hall_invariant_is_the_exact_sum_of_its_terms(oracle):
+    """orc_hall_invariant_compensated_omp against math.fsum (correctly rounded) of the same
+    terms, evaluated term by term as the fold evaluates them, on a hall with a source running:
+    within 2 ulps, for 1 and 5 threads; the left fold stays within its own error bound."""
+    h = oracle.OracleHall(0.024, 0.018, 0.013, 25, 343.0, 1.2)
+    n = h.nx * h.ny * h.nz
+    h.release(np.zeros(n))
+    dt = 0.5 * h.max_stable_dt()
+    h.set_source(12 + h.nx * (9 + h.ny * 6), np.array([math.sin(0.2 * s) for s in range(60)]))
+    for _ in range(60):
+        h.step(dt)
+    nx, ny, nz = h.nx, h.ny, h.nz
+    wall = lambda i, m: 2.0 if (i == 0 or i + 1 == m) else 1.0  # noqa: E731
+    vol, rc2, hr = h.dx ** 3, h.density * h.speed ** 2, 0.5 * h.density
+    terms = []
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                q = i + nx * (j + ny * k)
+                terms.append(1.0 / (wall(i, nx) * wall(j, ny) * wall(k, nz)) * h.p[q] * h.pp[q]
+                             / (2.0 * rc2) * vol)
+    groups = ((h.vx, nx - 1, ny, nz, lambda i, j, k: wall(j, ny) * wall(k, nz)),
+              (h.vy, nx, ny - 1, nz, lambda i, j, k: wall(i, nx) * wall(k, nz)),
+              (h.vz, nx, ny, nz - 1, lambda i, j, k: wall(i, nx) * wall(j, ny)))
+    for v, ex, ey, ez, w in groups:
+        for k in range(ez):
+            for j in range(ey):
+                for i in range(ex):
+                    x = v[i + ex * (j + ey * k)]
+                    terms.append(1.0 / w(i, j, k) * hr * x * x * vol)
+    exact = math.fsum(terms)
+    for threads in (1, 5):
+        got = h.invariant_compensated(threads)
+        assert abs(got - exact) <= 2 * np.spacing(abs(exact)), (threads, got, exact)
+    fold = h.invariant()
+    bound = len(terms) * 2.0 ** -53 * math.fsum(abs(t) for t in terms)
+    assert abs(fold - exact) <= bound
+
+
 def test_the_hall_mode_frequency_closed_form():
     # hall_closed_forms.rs:41-71, on the geometry the oracle builds.
     from oracle.oracle import OracleHall

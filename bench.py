@@ -155,11 +155,20 @@ def bench_ensemble(args) -> dict:
     from pantometry_amd import workloads as w
     ens, shapes = w.heat_ensemble(512)
     ens.set_steps_per_pass(args.steps_per_pass)
+    by_policy = None
     if args.z_chunk > 0:
+        ens.set_tiles(args.tiles)
         ens.set_schedule(0, args.z_chunk)
-        report = f"fixed resident_edge=0 z_chunk={args.z_chunk}"
+        report = f"fixed tiles={args.tiles} resident_edge=0 z_chunk={args.z_chunk}"
     else:
         report = ens.tune()
+        # The tuner's best time per streamed tile policy, ms per step.
+        by_policy = {}
+        for line in report.splitlines():
+            if line.startswith("tiles=") and line.endswith("ms/step"):
+                pol = "tiles=" + line.split()[0].split("=")[1]
+                ms_c = float(line.rsplit(":", 1)[1].split()[0])
+                by_policy[pol] = min(by_policy.get(pol, ms_c), ms_c)
     cells = sum(e ** 3 for e, _ in shapes)
     ens.run(args.warmup)
     before = ens.ledgers()
@@ -175,7 +184,8 @@ def bench_ensemble(args) -> dict:
         "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic: seeded edges, diffusivities, hot spots and noise",
         "config": {"workload": "ensemble of 512 uniform blocks, one graph launch per run",
-                   "cells": cells, "schedule": report.strip().splitlines()[-1]},
+                   "cells": cells, "schedule": report.strip().splitlines()[-1],
+                   "tuner_best_ms_per_step_by_tile_policy": by_policy},
         "roofline": {"bound": "hbm", "achieved": round(16 * cells / (ms / args.steps * 1e-3) / 1e9, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(16 * cells / (ms / args.steps * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
@@ -310,6 +320,8 @@ def main() -> int:
                     help="skip the hall, ensemble and surrogate figures the headline line carries")
     ap.add_argument("--tile-y", type=int, default=0)
     ap.add_argument("--z-chunk", type=int, default=0)
+    ap.add_argument("--tiles", type=int, default=0,
+                    help="ensemble streamed tile policy 0-3 (with --z-chunk; else tuned)")
     ap.add_argument("--steps-per-pass", type=int, choices=[1, 2], default=2,
                     help="2 (default): pairs of steps per pass over HBM; 1: one step per pass")
     ap.add_argument("--workload", choices=["heat", "hall", "ensemble", "surrogate", "copy", "lines"],

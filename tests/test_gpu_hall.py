@@ -120,18 +120,22 @@ def test_a_breach_inside_a_pair_leaves_the_state_of_the_breaking_step():
 
     # A tolerance met by every earlier step and broken by a second step of a pair (step 2, 4,
     # ...).  A hall released in a mode with a little seeded noise, whose ledger (its energy alone)
-    # moves by rounding every step, gives such a step.
-    g, o = _pair(4.4, 3.1, 2.4, 31)
-    nx, ny, nz = g.nodes()
-    p0 = o.mode((2, 1, 1), 1.0) + 1e-3 * np.random.default_rng(7).standard_normal(nx * ny * nz)
-    g.released_from(p0.reshape(nz, ny, nx))
-    dt = 0.9 * g.max_stable_dt()
-    g.checkpoint()
-    totals, _ = g.advance(dt, 80, tolerance=1.0)
-    prev = np.concatenate([[totals[0]], totals[:-1]])
-    scale = np.maximum(np.abs(totals), np.abs(prev))
-    drift = np.where(scale > 0, np.abs(totals - prev) / np.where(scale > 0, scale, 1.0), 0.0)
-    js = [j for j in range(2, 80, 2) if drift[j] > drift[1:j].max()]
+    # moves by rounding every step, gives such a step for some seed: the first seed whose run has
+    # an even step drifting more than every step before it.
+    for seed in range(7, 47):
+        g, o = _pair(4.4, 3.1, 2.4, 31)
+        nx, ny, nz = g.nodes()
+        p0 = o.mode((2, 1, 1), 1.0) + 1e-3 * np.random.default_rng(seed).standard_normal(nx * ny * nz)
+        g.released_from(p0.reshape(nz, ny, nx))
+        dt = 0.9 * g.max_stable_dt()
+        g.checkpoint()
+        totals, _ = g.advance(dt, 80, tolerance=1.0)
+        prev = np.concatenate([[totals[0]], totals[:-1]])
+        scale = np.maximum(np.abs(totals), np.abs(prev))
+        drift = np.where(scale > 0, np.abs(totals - prev) / np.where(scale > 0, scale, 1.0), 0.0)
+        js = [j for j in range(2, 80, 2) if drift[j] > drift[1:j].max()]
+        if js:
+            break
     assert js, drift
     j = js[0]
     tol = 0.5 * (drift[j] + drift[1:j].max())

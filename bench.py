@@ -128,6 +128,7 @@ def bench_hall(args) -> dict:
     h = case.hall
     if args.tile_y or args.z_chunk:
         h.set_tile(args.tile_y, args.z_chunk)
+    h.set_tile_width(args.tile_x)
     h.advance(case.dt, 1 + args.warmup, tolerance=1e-12)  # startup step + warm-up
     t0 = time.perf_counter()
     totals, ms = h.advance(case.dt, steps, tolerance=1e-12)
@@ -139,7 +140,8 @@ def bench_hall(args) -> dict:
         "steps": steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / steps, 5),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic: seeded Ricker point source in a rigid hall at rest",
-        "config": {"workload": "hall 385^3 nodes f64, CFL 0.5, Ricker source, audit every step"},
+        "config": {"workload": "hall 385^3 nodes f64, CFL 0.5, Ricker source, audit every step",
+                   "tile_x": args.tile_x or 64},
         "roofline": {"bound": "hbm", "achieved": round(64 * nodes / (ms / steps * 1e-3) / 1e9, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(64 * nodes / (ms / steps * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
@@ -319,6 +321,8 @@ def main() -> int:
     ap.add_argument("--no-secondary", dest="secondary", action="store_false",
                     help="skip the hall, ensemble and surrogate figures the headline line carries")
     ap.add_argument("--tile-y", type=int, default=0)
+    ap.add_argument("--tile-x", type=int, default=0,
+                    help="hall: width of the paired step's column tiles, 64/56/48 (0: 64)")
     ap.add_argument("--z-chunk", type=int, default=0)
     ap.add_argument("--tiles", type=int, default=0,
                     help="ensemble streamed tile policy 0-3 (with --z-chunk; else tuned)")

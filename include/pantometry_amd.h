@@ -322,6 +322,9 @@ double pm_hall_peak_pressure(pm_hall *hall);
  * keeps the current) and z-planes per workgroup (0: chosen from the extents).  Results are the
  * same for every shape.  No reference counterpart. */
 int pm_hall_set_tile(pm_hall *hall, int tile_y, int z_chunk);
+/* Width of the paired step's column tiles: 64, 56 or 48 nodes, or 0 (the default: 64).  Results
+ * are the same for every width.  No reference counterpart. */
+int pm_hall_set_tile_width(pm_hall *hall, int tile_x);
 /* How many leapfrog steps pm_hall_advance takes per pass over the grid: 1, or 2 (the default):
  * pairs of steps in one kernel (temporal blocking: p, vx, vy, vz read once per two steps), the
  * same arithmetic, fields and per-step audits.  The half step that starts the leapfrog always

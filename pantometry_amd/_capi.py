@@ -118,6 +118,7 @@ SIGNATURES = {
     "pm_hall_restore": (_I, [_VP]),
     "pm_hall_peak_pressure": (_D, [_VP]),
     "pm_hall_set_tile": (_I, [_VP, _I, _I]),
+    "pm_hall_set_tile_width": (_I, [_VP, _I]),
     "pm_hall_set_steps_per_pass": (_I, [_VP, _I]),
     "pm_room_create": (_I, [C.c_char_p, _D, _D, _S, _D, _D, _D, C.POINTER(_VP)]),
     "pm_hall_dimensions": (_I, [_VP]),

@@ -214,6 +214,10 @@ class Hall:
         """Rows per 64-wide column tile (8 or 16) and z-planes per workgroup (0: automatic)."""
         check(lib().pm_hall_set_tile(self._h, int(tile_y), int(z_chunk)), "pm_hall_set_tile")
 
+    def set_tile_width(self, tile_x: int = 0) -> None:
+        """Width of the paired step's column tiles: 64, 56 or 48 (0: the default, 64)."""
+        check(lib().pm_hall_set_tile_width(self._h, int(tile_x)), "pm_hall_set_tile_width")
+
     def set_steps_per_pass(self, steps: int) -> None:
         """1, or 2 (default): advance runs pairs of leapfrog steps in one pass."""
         check(lib().pm_hall_set_steps_per_pass(self._h, int(steps)), "pm_hall_set_steps_per_pass")

@@ -55,6 +55,17 @@ __device__ __forceinline__ void buffer_store_f64_nt(double v, double *base, uint
     const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(base, (short)0, -1, BUF_WORD3);
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r, (int)off, 0, 2);
 }
+// The same through a resource built once: the lane's offset `voff` plus a uniform offset `soff`
+// (an SGPR operand of the instruction), their sum below 2^32 - 8.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const double *base) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(base), (short)0, -1, BUF_WORD3);
+}
+__device__ __forceinline__ double buffer_load_f64(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ void buffer_store_f64_nt(double v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r, (int)voff, (int)soff, 2);
+}
 
 class DevBuf {
   public:

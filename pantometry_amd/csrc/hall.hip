@@ -40,6 +40,7 @@ struct HallArgs {
     double *__restrict__ supplied_step;  // the source's supplied energy this step
     const AuditState *audit;
     double gain, scale, two_rc2, volume, half_rho;
+    double pot, kin;  // volume / (2 rho c^2) and half_rho x volume (the paired step's audit)
     int64_t src;         // source node or -1
     double source;       // its increment this step
     double source_prev;  // its increment the step before
@@ -192,8 +193,9 @@ __global__ __launch_bounds__(64 * TY) void hall_march(HallArgs a, int gx, int gy
     if (tid == 0) a.partials[tile] = total;
 }
 
-// Two steps per pass (temporal blocking) of the marching step.  Per plane k the workgroup
-// computes the first step on the tile widened by one node (E1, 66 x (TY+2), its four corners
+// Two steps per pass (temporal blocking) of the marching step, on TX x TY column tiles (TX 64,
+// 56 or 48; 64 unless set).  Per plane k the workgroup
+// computes the first step on the tile widened by one node (E1, (TX+2) x (TY+2), its four corners
 // aside, which no second-step stencil reaches): each node's six new face velocities v' and its
 // new pressure p' (hall.rs:433-505), from p, vx, vy staged on E2 (the tile widened by two) and its
 // own column's p and vz in registers.  The tile's threads keep their nodes' v' in registers and
@@ -203,9 +205,10 @@ __global__ __launch_bounds__(64 * TY) void hall_march(HallArgs a, int gx, int gy
 // vx, vy, vz once and writes p'' (the new pressure), p' (its pressure_prev) and the v'' faces:
 // 72 B per two node-steps instead of 128.  The first step's audit sums the owned nodes' p'p and
 // the owned faces' v'^2 as they are produced, the second step's those of p''p' and v''^2.
-template <int TY>
-__global__ __launch_bounds__(64 * TY, 4) void hall_march2(HallArgs a, int gx, int gy, int gz, int zc) {
-    constexpr int TX = 64, NT = TX * TY;
+template <int TX, int TY>
+__global__ __launch_bounds__(TX * TY, 4) void hall_march2(HallArgs a, int gx, int gy, int gz, int zc) {
+    constexpr int NT = TX * TY;
+    static_assert(NT % 64 == 0, "whole waves");
     constexpr int E1X = TX + 2, E1Y = TY + 2, E2X = TX + 4, E2Y = TY + 4;
     constexpr int R1 = 2 * (E1X - 2) + 2 * TY;             // E1 ring nodes, corners aside
     constexpr int R2 = 2 * (E2X - 4) + 2 * (E2Y - 4) + 4;  // E2 ring nodes reached, E1 corners
@@ -241,8 +244,8 @@ __global__ __launch_bounds__(64 * TY, 4) void hall_march2(HallArgs a, int gx, in
     auto kc = [&](int q) { return std::min(std::max(q, 0), nz - 1); };
     auto kz = [&](int q) { return std::min(std::max(q, 0), nz - 2); };
     // A node's byte offsets within a plane of p (and vz), vx (its +x face) and vy (its +y face),
-    // clamped to ones that exist (they fit 32 bits: planes of the paired step hold fewer than 2^29
-    // nodes), and its in-plane index, or ~0 off the hall.
+    // clamped to ones that exist (they fit 32 bits: planes of the paired step hold at most
+    // 107,374,182 nodes), and its in-plane index, or ~0 off the hall.
     struct Col {
         int x, y;
         uint32_t p, vx, vy, n;
@@ -302,11 +305,20 @@ __global__ __launch_bounds__(64 * TY, 4) void hall_march2(HallArgs a, int gx, in
     const bool h_vx = s_vx && hc.x >= 0 && hc.x + 1 < nx;
     const bool h_vy = s_vy && hc.y >= 0 && hc.y + 1 < ny;
 
-    // Plane addresses (uniform) of plane q of each array.
-    auto pP = [&](int q) { return a.p + plane * kc(q); };
-    auto pVX = [&](int q) { return a.vx + vxp * kc(q); };
-    auto pVY = [&](int q) { return a.vy + vyp * kc(q); };
-    auto pVZ = [&](int q) { return a.vz + plane * kz(q); };
+    // Each array is addressed from the chunk's lowest plane through a resource built once; a
+    // plane is a uniform byte offset from there.  Planes zb .. z1 + 2 are read and z0 .. z1 - 1
+    // written, and the host caps the chunk so that (zc + 4) planes span fewer than 2^32 bytes.
+    const int zb = std::max(z0 - 2, 0), zbz = std::min(zb, nz - 2);
+    const __amdgpu_buffer_rsrc_t RP = buffer_rsrc(a.p + plane * zb), RX = buffer_rsrc(a.vx + vxp * zb);
+    const __amdgpu_buffer_rsrc_t RY = buffer_rsrc(a.vy + vyp * zb), RZ = buffer_rsrc(a.vz + plane * zbz);
+    const uint32_t pb = (uint32_t)plane * 8u, xb = (uint32_t)vxp * 8u, yb = (uint32_t)vyp * 8u;
+    auto oP = [&](int q) { return (uint32_t)(kc(q) - zb) * pb; };
+    auto oX = [&](int q) { return (uint32_t)(kc(q) - zb) * xb; };
+    auto oY = [&](int q) { return (uint32_t)(kc(q) - zb) * yb; };
+    auto oZ = [&](int q) { return (uint32_t)(kz(q) - zbz) * pb; };
+    const __amdgpu_buffer_rsrc_t WP = buffer_rsrc(a.p_out + plane * z0), WM = buffer_rsrc(a.p_mid + plane * z0);
+    const __amdgpu_buffer_rsrc_t WX = buffer_rsrc(a.vx_out + vxp * z0), WY = buffer_rsrc(a.vy_out + vyp * z0);
+    const __amdgpu_buffer_rsrc_t WZ = buffer_rsrc(a.vz_out + plane * std::min(z0, nz - 2));
 
     const double g = a.gain, sc = a.scale;
     // Wall weights are 1 or 2, so 1 / (a product of them) is exactly a product of 1 or 0.5.
@@ -337,21 +349,25 @@ __global__ __launch_bounds__(64 * TY, 4) void hall_march2(HallArgs a, int gx, in
     // Own columns: p(k-1..k+2), vz(k-1..k+1), vx(k), vy(k); the tile node also p'(k-2..k) and
     // the previous plane's v'.  Ring 2: p, vx, vy of plane k.  Loads for plane k+1 (k+2 for
     // p and vz ahead of use) are issued right after plane k is staged.
-    double tpm = buffer_load_f64(pP(zs - 1), tc.p), tpc = buffer_load_f64(pP(zs), tc.p);
-    double tpp = buffer_load_f64(pP(zs + 1), tc.p), tpq = 0.0;
-    double tzm = buffer_load_f64(pVZ(zs - 1), tc.p), tzc = buffer_load_f64(pVZ(zs), tc.p), tzp = 0.0;
-    double txc = buffer_load_f64(pVX(zs), tc.vx), tyc = buffer_load_f64(pVY(zs), tc.vy);
-    double rpm = buffer_load_f64(pP(zs - 1), rc.p), rpc = buffer_load_f64(pP(zs), rc.p);
-    double rpp = buffer_load_f64(pP(zs + 1), rc.p), rpq = 0.0;
-    double rzm = buffer_load_f64(pVZ(zs - 1), rc.p), rzc = buffer_load_f64(pVZ(zs), rc.p), rzp = 0.0;
-    double rxc = buffer_load_f64(pVX(zs), rc.vx), ryc = buffer_load_f64(pVY(zs), rc.vy);
-    double hp0 = buffer_load_f64(pP(zs), hc.p), hx0 = buffer_load_f64(pVX(zs), hc.vx);
-    double hy0 = buffer_load_f64(pVY(zs), hc.vy);
+    double tpm = buffer_load_f64(RP, tc.p, oP(zs - 1)), tpc = buffer_load_f64(RP, tc.p, oP(zs));
+    double tpp = buffer_load_f64(RP, tc.p, oP(zs + 1)), tpq = 0.0;
+    double tzm = buffer_load_f64(RZ, tc.p, oZ(zs - 1)), tzc = buffer_load_f64(RZ, tc.p, oZ(zs)), tzp = 0.0;
+    double txc = buffer_load_f64(RX, tc.vx, oX(zs)), tyc = buffer_load_f64(RY, tc.vy, oY(zs));
+    double rpm = buffer_load_f64(RP, rc.p, oP(zs - 1)), rpc = buffer_load_f64(RP, rc.p, oP(zs));
+    double rpp = buffer_load_f64(RP, rc.p, oP(zs + 1)), rpq = 0.0;
+    double rzm = buffer_load_f64(RZ, rc.p, oZ(zs - 1)), rzc = buffer_load_f64(RZ, rc.p, oZ(zs)), rzp = 0.0;
+    double rxc = buffer_load_f64(RX, rc.vx, oX(zs)), ryc = buffer_load_f64(RY, rc.vy, oY(zs));
+    double hp0 = buffer_load_f64(RP, hc.p, oP(zs)), hx0 = buffer_load_f64(RX, hc.vx, oX(zs));
+    double hy0 = buffer_load_f64(RY, hc.vy, oY(zs));
     double qm = 0.0, qc = 0.0;  // the tile node's p'(k-2), p'(k-1)
     double plz = 0.0;           // its v'z below plane k-1 (the one above is this plane's lz)
 
     const double iwi = inv_wall(tc.x, nx), iwj = inv_wall(tc.y, ny);
     const double wi = wall(tc.x, nx), wj = wall(tc.y, ny);
+    // The invariant's terms (hall.rs:334-384) with its constant factors folded: volume / (2 rho
+    // c^2) and rho / 2 x volume, each rounded once on the host.  A term moves by an ulp or two
+    // from the reference's; the totals are held to 1e-12 of the ledger's scale, not bit for bit.
+    const double pot = a.pot, kin = a.kin;
     double acc1 = 0.0, acc2 = 0.0;
     for (int k = zs; k <= z1; k++) {
         const int b = (k - zs) & 1;
@@ -370,19 +386,18 @@ __global__ __launch_bounds__(64 * TY, 4) void hall_march2(HallArgs a, int gx, in
             if (s_vy) sVY[b][r2r][r2c - 1] = h_vy ? hy0 : 0.0;
         }
         {
-            const double *P2 = pP(k + 2), *X1 = pVX(k + 1), *Y1 = pVY(k + 1), *Z1 = pVZ(k + 1);
-            const double *P1 = pP(k + 1);
-            tpq = buffer_load_f64(P2, tc.p);
-            tzp = buffer_load_f64(Z1, tc.p);
-            txc = buffer_load_f64(X1, tc.vx);
-            tyc = buffer_load_f64(Y1, tc.vy);
-            rpq = buffer_load_f64(P2, rc.p);
-            rzp = buffer_load_f64(Z1, rc.p);
-            rxc = buffer_load_f64(X1, rc.vx);
-            ryc = buffer_load_f64(Y1, rc.vy);
-            hp0 = buffer_load_f64(P1, hc.p);
-            hx0 = buffer_load_f64(X1, hc.vx);
-            hy0 = buffer_load_f64(Y1, hc.vy);
+            const uint32_t P2 = oP(k + 2), P1 = oP(k + 1), X1 = oX(k + 1), Y1 = oY(k + 1), Z1 = oZ(k + 1);
+            tpq = buffer_load_f64(RP, tc.p, P2);
+            tzp = buffer_load_f64(RZ, tc.p, Z1);
+            txc = buffer_load_f64(RX, tc.vx, X1);
+            tyc = buffer_load_f64(RY, tc.vy, Y1);
+            rpq = buffer_load_f64(RP, rc.p, P2);
+            rzp = buffer_load_f64(RZ, rc.p, Z1);
+            rxc = buffer_load_f64(RX, rc.vx, X1);
+            ryc = buffer_load_f64(RY, rc.vy, Y1);
+            hp0 = buffer_load_f64(RP, hc.p, P1);
+            hx0 = buffer_load_f64(RX, hc.vx, X1);
+            hy0 = buffer_load_f64(RY, hc.vy, Y1);
         }
         lds_barrier();
 
@@ -402,10 +417,10 @@ __global__ __launch_bounds__(64 * TY, 4) void hall_march2(HallArgs a, int gx, in
             const double share = iwi * iwj * iwk;
             if (tc.n == sn1 && k == sk1)
                 *a.supplied_step = share * (a.source + a.source_prev) * tpc / a.two_rc2 * a.volume;
-            acc1 += share * t1.p * tpc / a.two_rc2 * a.volume;
-            if (tc.x + 1 < nx) acc1 += iwj * iwk * a.half_rho * t1.hx * t1.hx * a.volume;
-            if (tc.y + 1 < ny) acc1 += iwi * iwk * a.half_rho * t1.hy * t1.hy * a.volume;
-            if (k + 1 < nz) acc1 += iwi * iwj * a.half_rho * t1.hz * t1.hz * a.volume;
+            acc1 += share * t1.p * tpc * pot;
+            if (tc.x + 1 < nx) acc1 += iwj * iwk * kin * t1.hx * t1.hx;
+            if (tc.y + 1 < ny) acc1 += iwi * iwk * kin * t1.hy * t1.hy;
+            if (k + 1 < nz) acc1 += iwi * iwj * kin * t1.hz * t1.hz;
         }
 
         // Second step on plane k-1, from p' of planes k-2, k-1 (registers, LDS) and k, and the
@@ -430,15 +445,16 @@ __global__ __launch_bounds__(64 * TY, 4) void hall_march2(HallArgs a, int gx, in
                 const double share = iwi * iwj * iwk;
                 if (at_src)
                     *a.supplied_step2 = share * (a.source2 + a.source) * pc / a.two_rc2 * a.volume;
-                buffer_store_f64_nt(pn, a.p_out + plane * k2, tc.p);
-                buffer_store_f64_nt(pc, a.p_mid + plane * k2, tc.p);
-                if (tc.x + 1 < nx) buffer_store_f64_nt(hx, a.vx_out + vxp * k2, tc.vx);
-                if (tc.y + 1 < ny) buffer_store_f64_nt(hy, a.vy_out + vyp * k2, tc.vy);
-                if (k2 + 1 < nz) buffer_store_f64_nt(hz, a.vz_out + plane * k2, tc.p);
-                acc2 += share * pn * pc / a.two_rc2 * a.volume;
-                if (tc.x + 1 < nx) acc2 += iwj * iwk * a.half_rho * hx * hx * a.volume;
-                if (tc.y + 1 < ny) acc2 += iwi * iwk * a.half_rho * hy * hy * a.volume;
-                if (k2 + 1 < nz) acc2 += iwi * iwj * a.half_rho * hz * hz * a.volume;
+                const uint32_t o = (uint32_t)(k2 - z0);
+                buffer_store_f64_nt(pn, WP, tc.p, o * pb);
+                buffer_store_f64_nt(pc, WM, tc.p, o * pb);
+                if (tc.x + 1 < nx) buffer_store_f64_nt(hx, WX, tc.vx, o * xb);
+                if (tc.y + 1 < ny) buffer_store_f64_nt(hy, WY, tc.vy, o * yb);
+                if (k2 + 1 < nz) buffer_store_f64_nt(hz, WZ, tc.p, o * pb);
+                acc2 += share * pn * pc * pot;
+                if (tc.x + 1 < nx) acc2 += iwj * iwk * kin * hx * hx;
+                if (tc.y + 1 < ny) acc2 += iwi * iwk * kin * hy * hy;
+                if (k2 + 1 < nz) acc2 += iwi * iwj * kin * hz * hz;
             }
         }
         qm = qc, qc = t1.p;
@@ -699,6 +715,7 @@ struct pm_hall {
     double supplied_host = 0.0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int tile_y = 8, z_chunk = 0;  // marching tiles; z_chunk 0: chosen from the extents
+    int tile_x = 0;               // the paired step's tile width (64, 56, 48); 0: from nx
 
     int64_t nodes() const { return nx * ny * nz; }
     int64_t nfx() const { return (nx - 1) * ny * nz; }
@@ -708,18 +725,29 @@ struct pm_hall {
 
 namespace pm {
 
-// Tiles of the marching step: 64 x tile_y columns and, unless fixed, z-chunks of at most 128
-// planes, short enough that a launch has about 1024 workgroups (four per compute unit), at
-// least 4 planes.
+// Tiles of the marching step: tile_x x tile_y columns and, unless fixed, z-chunks of at most 128
+// planes, short enough that a launch has at least 2560 workgroups, at least 4 planes.
 struct HallGrid {
     int gx, gy, gz, zc;
     int64_t tiles() const { return (int64_t)gx * gy * gz; }
 };
 
-static HallGrid hall_grid(const pm_hall *h, int tile_y = 0) {
+// The paired step's tile width: the hall's own setting, else 64.  Narrower tiles leave fewer
+// idle lanes in the last x-tile but run fewer waves per workgroup, and the step is bound by the
+// latency of its per-plane chain, not by issue: at 385 nodes 56-wide tiles (2% idle lanes) ran
+// 4.6% slower than 64-wide ones (14% idle) and 48-wide ones 15% slower.
+static int hall_tile_x(const pm_hall *h) { return h->tile_x ? h->tile_x : 64; }
+
+// The paired step addresses each array from its chunk's lowest plane with 32-bit byte offsets:
+// a chunk of zc planes reaches zc + 4 planes of an array, which must span fewer than 2^32 bytes.
+static int64_t hall_pair_max_chunk(const pm_hall *h) {
+    return (int64_t)(((uint64_t)1 << 32) - 1) / (h->nx * h->ny * 8) - 4;
+}
+
+static HallGrid hall_grid(const pm_hall *h, int tile_y = 0, int tile_x = 64, int64_t max_zc = 0) {
     const int ty = tile_y ? tile_y : h->tile_y;
     HallGrid g;
-    g.gx = (int)((h->nx + 63) / 64);
+    g.gx = (int)((h->nx + tile_x - 1) / tile_x);
     g.gy = (int)((h->ny + ty - 1) / ty);
     const int64_t xy = (int64_t)g.gx * g.gy;
     // At least 2560 workgroups (five waves of two workgroups per CU over 256 CUs), so the last,
@@ -728,6 +756,7 @@ static HallGrid hall_grid(const pm_hall *h, int tile_y = 0) {
     const int64_t chunks = std::max<int64_t>((2560 + xy - 1) / xy, (h->nz + 127) / 128);
     g.zc = h->z_chunk > 0 ? h->z_chunk
                           : (int)std::max<int64_t>(4, (h->nz + chunks - 1) / chunks);
+    if (max_zc > 0 && g.zc > max_zc) g.zc = (int)max_zc;
     g.gz = (int)((h->nz + g.zc - 1) / g.zc);
     return g;
 }
@@ -824,6 +853,8 @@ static HallArgs hall_args(pm_hall *h, double hstep, double vh) {
     a.two_rc2 = 2.0 * rc2;
     a.volume = h->dx * h->dx * h->dx;
     a.half_rho = 0.5 * h->density;
+    a.pot = a.volume / a.two_rc2;
+    a.kin = a.half_rho * a.volume;
     a.nx = (int)h->nx, a.ny = (int)h->ny, a.nz = (int)h->nz;
     a.src = a.src2 = -1;
     return a;
@@ -876,8 +907,8 @@ static int hall_sweep(pm_hall *h, double hstep, double vh, int64_t step, double 
 
 // Steps `step` and `step + 1` (both full leapfrog steps) in one pass, then both audits.
 static int hall_sweep_pair(pm_hall *h, double hstep, int64_t step, double *totals_dev, double offset) {
-    const int ty = h->tile_y == 16 ? 16 : 8;
-    const HallGrid g = hall_grid(h, ty);
+    const int ty = h->tile_y == 16 ? 16 : 8, tx = hall_tile_x(h);
+    const HallGrid g = hall_grid(h, ty, tx, hall_pair_max_chunk(h));
     const int64_t nparts = g.tiles();
     PM_CHECK(hall_partials(h, 2 * nparts));
     HallArgs a = hall_args(h, hstep, hstep);
@@ -894,10 +925,18 @@ static int hall_sweep_pair(pm_hall *h, double hstep, int64_t step, double *total
         PM_HIP(hipGetLastError());
     }
     const unsigned blocks = (unsigned)(XCDS_HALL * ((nparts + XCDS_HALL - 1) / XCDS_HALL));
-    if (ty == 16)
-        hall_march2<16><<<blocks, dim3(64, 16), 0, h->stream>>>(a, g.gx, g.gy, g.gz, g.zc);
-    else
-        hall_march2<8><<<blocks, dim3(64, 8), 0, h->stream>>>(a, g.gx, g.gy, g.gz, g.zc);
+    const dim3 wg(tx, ty);
+#define PM_HALL_PAIR(TX, TY) hall_march2<TX, TY><<<blocks, wg, 0, h->stream>>>(a, g.gx, g.gy, g.gz, g.zc)
+    if (ty == 16) {
+        if (tx == 56) PM_HALL_PAIR(56, 16);
+        else if (tx == 48) PM_HALL_PAIR(48, 16);
+        else PM_HALL_PAIR(64, 16);
+    } else {
+        if (tx == 56) PM_HALL_PAIR(56, 8);
+        else if (tx == 48) PM_HALL_PAIR(48, 8);
+        else PM_HALL_PAIR(64, 8);
+    }
+#undef PM_HALL_PAIR
     PM_HIP(hipGetLastError());
     hall_finalize2<<<1, 1024, 0, h->stream>>>(h->partials.d(), nparts, h->audit.as<AuditState>(), offset,
                                               h->scalars.d() + 1, s1.node >= 0 ? h->scalars.d() + 2 : nullptr,
@@ -1224,10 +1263,11 @@ int pm_hall_advance(pm_hall *h, double dt, uint64_t steps, double tolerance, dou
         uint64_t cursor;
     };
     std::vector<Launch> launches;
-    // The paired step addresses a plane with 32-bit byte offsets and assumes faces along every
-    // axis: other halls step one at a time.
+    // The paired step addresses five planes of an array or more with 32-bit byte offsets
+    // (planes of at most 107,374,182 nodes) and assumes faces along every axis: other halls step one
+    // at a time.
     const bool pairs = h->steps_per_pass == 2 && h->nx >= 2 && h->ny >= 2 && h->nz >= 2 &&
-                       h->nx * h->ny < ((int64_t)1 << 29);
+                       hall_pair_max_chunk(h) >= 1;
     for (uint64_t n = first; n < steps;) {
         launches.push_back({(int64_t)n, 1, h->ip, h->iq, h->is, h->iv, h->src_cursor});
         if (pairs && n + 1 < steps) {
@@ -1351,6 +1391,16 @@ int pm_hall_set_tile(pm_hall *h, int tile_y, int z_chunk) {
     }
     if (tile_y) h->tile_y = tile_y;
     h->z_chunk = z_chunk;
+    return PM_OK;
+}
+
+int pm_hall_set_tile_width(pm_hall *h, int tile_x) {
+    if (!h) return PM_ERR_ARGUMENT;
+    if (tile_x && tile_x != 64 && tile_x != 56 && tile_x != 48) {
+        set_error("hall tiles: tile_x must be 64, 56 or 48 (0: chosen from the extent)");
+        return PM_ERR_ARGUMENT;
+    }
+    h->tile_x = tile_x;
     return PM_OK;
 }
 

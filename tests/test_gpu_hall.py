@@ -100,6 +100,38 @@ def test_advance_is_bit_exact(dims, across, mode, per_pass):
     _same_fields(g, o)
 
 
+@pytest.mark.parametrize("across", [113, 97, 65])
+@pytest.mark.parametrize("tile_x", [64, 56, 48, 0])
+@pytest.mark.parametrize("tile_y", [8, 16])
+def test_every_paired_tile_width_gives_the_same_bits(across, tile_x, tile_y):
+    """The paired step on 64-, 56- and 48-wide column tiles and the default width: 113 = 2 x 56
+    + 1, 97 = 2 x 48 + 1 and 65 = 64 + 1 nodes leave one column in the last tile of one width
+    each."""
+    gpu_case(f"hall paired tile {tile_x} x {tile_y}, {across} nodes across")
+    g, o = _pair(0.01 * (across - 1), 0.2, 0.15, across)
+    assert o.nodes() == (across, 21, 16)
+    g.set_tile(tile_y, 0)
+    g.set_tile_width(tile_x)
+    g.released_in_mode((2, 1, 1), 2.0)
+    o.release(o.mode((2, 1, 1), 2.0))
+    dt = 0.9 * o.max_stable_dt()
+    totals, _ = g.advance(dt, 37, tolerance=1e-12)
+    for n in range(37):
+        o.step(dt, threads_=16)
+        t_ref, s_ref = o.ledger()
+        assert abs(totals[n] - t_ref) <= 1e-12 * s_ref, n
+    _same_fields(g, o)
+
+
+def test_tile_widths_outside_the_set_are_refused():
+    gpu_case("hall tile width refusal")
+    from pantometry_amd._capi import DeviceError
+    g, _ = _pair(1.12, 0.2, 0.15, 113)
+    for w in (32, 63, 65, -1):
+        with pytest.raises(DeviceError):
+            g.set_tile_width(w)
+
+
 def test_a_breach_inside_a_pair_leaves_the_state_of_the_breaking_step():
     gpu_case("hall breach inside a pair")
     import pantometry_amd as pm

@@ -93,18 +93,37 @@ __device__ __forceinline__ int div_small(int q, int d, float inv) {
 
 // The forward kernel.  A workgroup of four waves computes a 64*RB-position M-tile (a TD x TH x TW
 // box of one sample, smaller only when the volume is) against an N-block of 16*NT output
-// channels; each wave owns 16*RB positions as RB 16-row MFMA blocks, so every B fragment (16
-// channels x 16 output channels of one tap, read from the weights) feeds RB MFMAs and every A
-// fragment (16 positions x 16 channels, read from LDS at the tap's shift) feeds NT.  Per
-// 16-channel input chunk the box's neighbourhood is staged in LDS once, row by row; the 27 taps
-// are unrolled so each tap's offset in the staged box is a constant plus the rows' strides.  The
-// epilogue adds the bias, applies GELU, and writes bf16 through LDS so each position's channels
-// leave in 16-byte stores.
+// channels; each wave owns 16*RB positions as RB 16-position MFMA blocks.  The MFMA takes the
+// weights as its A operand (16 output channels x 32 k) and the staged activations as B (32 k x 16
+// positions), so every A fragment (read from LDS) feeds RB MFMAs, every B fragment (read from
+// LDS at the tap's shift) feeds NT, and a lane receives four consecutive output channels of one
+// position: the epilogue adds the bias, applies GELU, converts pairs to bf16 in one instruction
+// and stores 8 bytes per lane straight from registers.  Per 16-channel input chunk the box's
+// neighbourhood is staged in LDS once; the 27 taps run in pairs as K = 32 steps.  With FIXED the
+// box is 16 x 4 x 4 at compile time, so tap offsets are immediates.
 //
 // The input may be a concatenation read in place: with cup > 0, channels [0, cup) are x at half
 // the resolution (nearest-neighbour upsampling) and channels [cup, Cin) are xs at full
 // resolution; with cup = 0 all Cin channels are xs.
-template <int NT, int RB, int SK>
+//
+// Staging loads are buffer loads: an element outside the volume (or past the neighbourhood) takes
+// the offset 2^31, past the resource's 2^31-byte range, and reads zero, so no lane selects.  The
+// launcher keeps every tensor below 2^30 elements, so valid offsets plus the uniform channel
+// offset stay below 2^31.
+constexpr uint32_t STAGE_OOB = 0x80000000u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t stage_rsrc(const bf16 *base) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<bf16 *>(base), (short)0, (int)STAGE_OOB, 0x00020000);
+}
+__device__ __forceinline__ uint4 stage_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ uint32_t pack_bf16(float a, float b) {
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    const bf16x2 v = {(__bf16)a, (__bf16)b};  // gfx950: one v_cvt_pk_bf16_f32, round to nearest even
+    return __builtin_bit_cast(uint32_t, v);
+}
+
+template <int NT, int RB, int SK, int FIXED>
 __global__ __launch_bounds__(256, NT == 1 ? 4 : 2) void conv_fwd2(Conv g, const bf16 *__restrict__ x,
                                                  const bf16 *__restrict__ xs, int cup,
                                                  const bf16 *__restrict__ w,
@@ -117,29 +136,46 @@ __global__ __launch_bounds__(256, NT == 1 ? 4 : 2) void conv_fwd2(Conv g, const 
                                                  const float *__restrict__ head_w,
                                                  float *__restrict__ head_y) {
     constexpr int NB = 16 * NT;
+    constexpr bool FX = FIXED != 0;
+    static_assert(!FX || RB == 4, "the fixed box is 256 positions");
     extern __shared__ __align__(16) bf16 smem[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int TW = FX ? 16 : g.TW, TH = FX ? 4 : g.TH, TD = FX ? 4 : g.TD;
+    // Box origin of this workgroup's tile.
     int b, d0, h0, w0;
-    tile_origin(g, blockIdx.x, b, d0, h0, w0);
+    {
+        const int tw = g.W / TW, th = g.H / TH, td = g.D / TD;
+        int t = blockIdx.x;
+        w0 = (t % tw) * TW;
+        t /= tw;
+        h0 = (t % th) * TH;
+        t /= th;
+        d0 = (t % td) * TD;
+        b = t / td;
+    }
     const int n0 = blockIdx.y * NB;
-    const int SH = g.TH + 2, SW = g.TW + 2, SHW = SH * SW;
-    const int TP = g.tile_positions();
+    const int SH = TH + 2, SW = TW + 2, SHW = SH * SW;
+    const int TP = TD * TH * TW;
+    const int lw = FX ? 4 : __builtin_ctz(TW), lh = FX ? 2 : __builtin_ctz(TH);
     // The staged neighbourhood is two planes (channels 0-7 and 8-15 of the chunk) of 16-byte
     // positions, so the 16 lanes of a row group read 256 contiguous bytes.  Lane group
-    // lg = lane/16 takes tap 2p + lg/2 of a tap pair p and channel half lg%2.
+    // lg = lane/16 takes tap 2p + qt of a tap pair p and channel half qh: k = 16 qh + 8 qt + j.
     const int lg = lane >> 4, qh = lg >> 1, qt = lg & 1;
-    const int npos = (g.TD + 2) * SHW;
+    const int npos = (TD + 2) * SHW;
     const uint4 *sA = (const uint4 *)smem + qh * npos;
 
-    // This lane's RB A rows: positions wave*16*RB + rb*16 + lane%16 of the box.
+    // This lane's RB rows: positions wave*16*RB + rb*16 + lane%16 of the box (the last one for
+    // rows past a small box, which read in-range positions and are never stored), as offsets
+    // in the staged box; `tap1` adds the second tap of a pair one column on, `tap16` the pair
+    // that crosses a kernel row (taps 2 and 3: SW - 2 on).
     int base[RB];
     bool live[RB];
 #pragma unroll
     for (int rb = 0; rb < RB; rb++) {
         const int p = wave * 16 * RB + rb * 16 + (lane & 15);
-        live[rb] = p < TP;
-        int pd, ph, pw;
-        box_coords(g, live[rb] ? p : TP - 1, pd, ph, pw);
+        live[rb] = FX || p < TP;
+        const int q = live[rb] ? p : TP - 1;
+        const int pw = q & (TW - 1), ph = (q >> lw) & (TH - 1), pd = q >> (lw + lh);
         base[rb] = (pd * SH + ph) * SW + pw;
     }
 
@@ -149,8 +185,7 @@ __global__ __launch_bounds__(256, NT == 1 ? 4 : 2) void conv_fwd2(Conv g, const 
 #pragma unroll
         for (int n = 0; n < NT; n++) acc[rb][n] = float4v{0.f, 0.f, 0.f, 0.f};
 
-    const int64_t wrow = (int64_t)27 * g.Cin;
-    const bf16 *wl = w + (int64_t)(n0 + (lane & 15)) * wrow + qh * 8;  // this lane's B column
+    const int wrow = 27 * g.Cin;
     // The reduction runs over units (16-channel chunk, kernel plane dz); split z of gridDim.z
     // takes units [u0, u1).
     const int units = g.Cin / 16 * 3;
@@ -158,90 +193,115 @@ __global__ __launch_bounds__(256, NT == 1 ? 4 : 2) void conv_fwd2(Conv g, const 
     const int u1 = (int)((int64_t)(blockIdx.z + 1) * units / gridDim.z);
 
     // Staging of a chunk: element e = tid + 256 k (k < SK) of the 2 npos 16-byte halves is half
-    // e % 2 of staged position e / 2, so consecutive lanes read consecutive 16 bytes of the
-    // channels-last input.  All of a thread's loads are issued before any is waited on, and the
-    // next chunk's loads are issued before the current chunk's MFMAs.
-    int dst[SK];  // LDS index of element k; past the neighbourhood, a spare slot after it
-    int src[SK];  // its offset in xs (channel cup of the input); position 0 outside the volume
-    int srcu[SK]; // its offset in the half-resolution x (channel 0), when cup > 0
-    bool in[SK];  // inside the volume (else the loaded value is replaced by zero)
+    // tid % 2 (the same for every k) of staged position tid / 2 + 128 k, so consecutive lanes read
+    // consecutive 16 bytes of the channels-last input.  The position's coordinates are found once
+    // and stepped by 128 positions per k.  All of a thread's loads are issued before any is
+    // waited on, and the next chunk's loads are issued before the current chunk's MFMAs.
+    uint32_t src[SK];   // byte offset in xs (channel cup of the input), or STAGE_OOB
+    uint32_t srcu[SK];  // byte offset in the half-resolution x (channel 0), when cup > 0
     const int cs = g.Cin - cup;  // channels of xs
-    const float inv_sw = 1.0f / (float)SW, inv_sh = 1.0f / (float)SH;
-    // Offsets are 32-bit (the launcher checks that every tensor's elements fit), and the tile's
-    // own part of them is uniform: the box origin's position index, times the channel count, on
-    // the scalar unit; each element adds its small signed displacement in 24-bit products.
+    // Offsets are 32-bit, and the tile's own part of them is uniform: the box origin's position
+    // index, times the channel count, on the scalar unit; each element adds its small signed
+    // displacement in 24-bit products.
     const int HW = g.H * g.W;
     const int pbase = ((b * g.D + d0) * g.H + h0) * g.W + w0;
     const int sbase = pbase * cs;
-#pragma unroll
-    for (int k = 0; k < SK; k++) {
-        const int e = tid + 256 * k;
-        const int p = e >> 1, half = e & 1;
-        const int t = div_small(p, SW, inv_sw), pz = div_small(t, SH, inv_sh);
-        const int px = p - t * SW, py = t - pz * SH;
-        const int z = d0 + pz - 1, y = h0 + py - 1, xx = w0 + px - 1;
-        const bool staged_here = e < 2 * npos;
-        in[k] = staged_here && z >= 0 && z < g.D && y >= 0 && y < g.H && xx >= 0 && xx < g.W;
-        dst[k] = staged_here ? half * npos + p : 2 * npos;
-        const int rel = __mul24(pz - 1, HW) + __mul24(py - 1, g.W) + (px - 1);
-        src[k] = in[k] ? sbase + __mul24(rel, cs) + half * 8 : 0;
-        srcu[k] = 0;
-    }
-    if (cup > 0) {
+    const int half = tid & 1, p0 = tid >> 1;
+    {
+        int px, py, pz;
+        if constexpr (FX) {
+            const int t = p0 / 18;
+            px = p0 - 18 * t, pz = t / 6, py = t - 6 * pz;
+        } else {
+            const int t = div_small(p0, SW, 1.0f / (float)SW);
+            px = p0 - t * SW;
+            pz = div_small(t, SH, 1.0f / (float)SH);
+            py = t - pz * SH;
+        }
+        // 128 positions are a_rows rows of SW and b_cols more; a_rows rows are a_z planes of SH
+        // rows and a_y more.
+        const int a_rows = 128 / SW, b_cols = 128 - a_rows * SW;
+        const int a_z = a_rows / SH, a_y = a_rows - a_z * SH;
         const unsigned Dh = g.D >> 1, Hh = g.H >> 1, Wh = g.W >> 1;
 #pragma unroll
         for (int k = 0; k < SK; k++) {
-            const int e = tid + 256 * k;
-            const int p = e >> 1, half = e & 1;
-            const int t = div_small(p, SW, inv_sw), pz = div_small(t, SH, inv_sh);
-            const int px = p - t * SW, py = t - pz * SH;
-            const unsigned z = (unsigned)(d0 + pz - 1) >> 1, y = (unsigned)(h0 + py - 1) >> 1;
-            const unsigned xx = (unsigned)(w0 + px - 1) >> 1;
-            const unsigned ph = __umul24(__umul24(__umul24((unsigned)b, Dh) + z, Hh) + y, Wh) + xx;
-            srcu[k] = in[k] ? (int)(__umul24(ph, (unsigned)cup) + half * 8) : 0;
+            if (k > 0) {
+                px += b_cols;
+                const int cx = px >= SW ? 1 : 0;
+                px -= cx * SW;
+                py += a_y + cx;
+                const int cy = py >= SH ? 1 : 0;
+                py -= cy * SH;
+                pz += a_z + cy;
+            }
+            // Only the last two of the SK elements can lie past the neighbourhood: SK is at most
+            // one more than ceil(npos / 128).
+            const bool staged = k < SK - 2 || p0 + 128 * k < npos;
+            const int z = d0 + pz - 1, y = h0 + py - 1, xx = w0 + px - 1;
+            const bool in = staged && (unsigned)z < (unsigned)g.D && (unsigned)y < (unsigned)g.H &&
+                            (unsigned)xx < (unsigned)g.W;
+            const int rel = __mul24(pz - 1, HW) + __mul24(py - 1, g.W) + (px - 1);
+            src[k] = in ? (uint32_t)(sbase + __mul24(rel, cs) + half * 8) * 2u : STAGE_OOB;
+            srcu[k] = STAGE_OOB;
+            if (cup > 0 && in) {
+                const unsigned ph = __umul24(__umul24(__umul24((unsigned)b, Dh) + ((unsigned)z >> 1), Hh) +
+                                             ((unsigned)y >> 1), Wh) + ((unsigned)xx >> 1);
+                srcu[k] = (__umul24(ph, (unsigned)cup) + half * 8) * 2u;
+            }
         }
     }
     // With one or two N-blocks the chunk's weights go through the same pipeline into LDS after
-    // the neighbourhood: slot (tap * 2 + half) * NB + column holds input channels half*8 .. +8
-    // of output channel n0 + column, so the loop reads B from LDS like A and waits on no global
-    // load.  Element eb = tid + 256 k is that slot; past 54 NB the loads repeat slot 0's source.
+    // the neighbourhood: slot ((dz * 10 + tap) * 2 + half) * NB + column holds input channels
+    // half*8 .. +8 of output channel n0 + column for tap < 9 of kernel plane dz, and zeros for
+    // tap 9 (the padding tap of a plane's fifth pair), so the loop reads A from LDS and waits on
+    // no global load.  Element eb = tid + 256 k is that slot; past 60 NB the loads read zero.
     constexpr bool WB = NT <= 2;
-    constexpr int SKB = WB ? (54 * NB + 255) / 256 : 1;
+    constexpr int SKB = WB ? (60 * NB + 255) / 256 : 1;
     const int aslots = 2 * npos + 1;
+    uint32_t wsrc[WB ? SKB : 1];
+    if constexpr (WB) {
+#pragma unroll
+        for (int k = 0; k < SKB; k++) {
+            const int eb = tid + 256 * k;
+            const int col = eb % NB, rest = eb / NB;  // rest = (dz * 10 + tap) * 2 + half
+            const int hf = rest & 1, dt = rest >> 1, dz = dt / 10, tap = dt - dz * 10;
+            wsrc[k] = (eb < 60 * NB && tap < 9)
+                          ? (uint32_t)((n0 + col) * wrow + (dz * 9 + tap) * g.Cin + hf * 8) * 2u
+                          : STAGE_OOB;
+        }
+    }
     // The prefetch registers: the neighbourhood's SK elements, then the weights' SKB.
     uint4 pre[SK + (WB ? SKB : 0)];
-    // Source offset of weight element k (channel 0 of the chunk).
-    auto bsrc = [&](int k) -> int {
-        const int eb = tid + 256 * k;
-        const int col = eb % NB, rest = eb / NB;
-        return eb < 54 * NB ? (n0 + col) * (int)wrow + (rest >> 1) * g.Cin + (rest & 1) * 8 : 0;
-    };
-    const uint4 *sB = (const uint4 *)smem + aslots + qh * NB + (lane & 15);
+    const __amdgpu_buffer_rsrc_t RX = stage_rsrc(x), RS = stage_rsrc(xs), RW = stage_rsrc(w);
     auto load_chunk = [&](int c0) {
         if (c0 < cup) {
 #pragma unroll
-            for (int k = 0; k < SK; k++) pre[k] = *(const uint4 *)(x + srcu[k] + c0);
+            for (int k = 0; k < SK; k++) pre[k] = stage_load(RX, srcu[k], (uint32_t)c0 * 2u);
         } else {
 #pragma unroll
-            for (int k = 0; k < SK; k++) pre[k] = *(const uint4 *)(xs + src[k] + c0 - cup);
+            for (int k = 0; k < SK; k++) pre[k] = stage_load(RS, src[k], (uint32_t)(c0 - cup) * 2u);
         }
         if constexpr (WB) {
 #pragma unroll
-            for (int k = 0; k < SKB; k++) pre[SK + k] = *(const uint4 *)(w + bsrc(k) + c0);
+            for (int k = 0; k < SKB; k++) pre[SK + k] = stage_load(RW, wsrc[k], (uint32_t)c0 * 2u);
         }
     };
+    const int dst0 = half * npos + p0;
     auto store_chunk = [&]() {
 #pragma unroll
         for (int k = 0; k < SK; k++) {
-            const uint4 v = pre[k];
-            ((uint4 *)smem)[dst[k]] = make_uint4(in[k] ? v.x : 0u, in[k] ? v.y : 0u,
-                                                 in[k] ? v.z : 0u, in[k] ? v.w : 0u);
+            const bool staged = k < SK - 2 || p0 + 128 * k < npos;
+            ((uint4 *)smem)[staged ? dst0 + 128 * k : 2 * npos] = pre[k];
         }
         if constexpr (WB) {
 #pragma unroll
             for (int k = 0; k < SKB; k++) ((uint4 *)smem)[aslots + tid + 256 * k] = pre[SK + k];
         }
     };
+    // This lane's A column (weights): LDS slot of output channel n0 + lane%16, channel half qh,
+    // tap qt of a pair; from global memory when the N-block is too wide to stage.
+    const uint4 *sW = (const uint4 *)smem + aslots + (qt * 2 + qh) * NB + (lane & 15);
+    const bf16 *wl = w + (int64_t)(n0 + (lane & 15)) * wrow + qh * 8;
 
     int staged = -1;
     if (u0 < u1) load_chunk((u0 / 3) * 16);
@@ -256,134 +316,116 @@ __global__ __launch_bounds__(256, NT == 1 ? 4 : 2) void conv_fwd2(Conv g, const 
             const int un = (u / 3 + 1) * 3;
             if (un < u1) load_chunk((un / 3) * 16);
         }
+        const uint4 *sAz = sA + dz * SHW;
+        const uint4 *sWz = sW + dz * 20 * NB;
 #pragma unroll
         for (int pr = 0; pr < 5; pr++) {
             // Taps 2pr and 2pr+1 of this kernel plane in one K=32 step; the tenth is padding: it
-            // reads the ninth tap's positions against a zero B fragment, so every load is
-            // unpredicated (rows past a small box read a clamped position and are never stored).
-            const int mt = 2 * pr + qt;
-            const bool tv = mt < 9;
-            const int mc = tv ? mt : 8;
-            const int toff = dz * SHW + (mc / 3) * SW + (mc % 3);
-            uint4 bv[NT];
+            // reads the ninth tap's positions against a zero A fragment.
+            const int t0 = 2 * pr, t1 = pr < 4 ? 2 * pr + 1 : 8;
+            const int o0 = (t0 / 3) * SW + t0 % 3, o1 = (t1 / 3) * SW + t1 % 3;
+            const int toff = qt ? o1 : o0;
+            uint4 av[NT];
 #pragma unroll
-            for (int n = 0; n < NT; n++) {
-                bv[n] = WB ? sB[((dz * 9 + mc) * 2) * NB + n * 16]
-                           : *(const uint4 *)(wl + (int64_t)n * 16 * wrow + (dz * 9 + mc) * g.Cin + c0);
-                if (!tv) bv[n] = make_uint4(0, 0, 0, 0);
-            }
-            uint4 a[RB];
+            for (int n = 0; n < NT; n++)
+                av[n] = WB ? sWz[pr * 4 * NB + n * 16]
+                           : (qt && pr == 4) ? make_uint4(0, 0, 0, 0)
+                                             : *(const uint4 *)(wl + (int64_t)n * 16 * wrow + (dz * 9 + (qt ? t1 : t0)) * g.Cin + c0);
+            uint4 bvv[RB];
 #pragma unroll
-            for (int rb = 0; rb < RB; rb++) a[rb] = sA[base[rb] + toff];
+            for (int rb = 0; rb < RB; rb++) bvv[rb] = sAz[base[rb] + toff];
 #pragma unroll
             for (int rb = 0; rb < RB; rb++)
 #pragma unroll
-                for (int n = 0; n < NT; n++) acc[rb][n] = mfma32(a[rb], bv[n], acc[rb][n]);
+                for (int n = 0; n < NT; n++) acc[rb][n] = mfma32(av[n], bvv[rb], acc[rb][n]);
         }
     }
 
+    // Lane's rows and channels: position q(rb) = wave*16*RB + rb*16 + lane%16 of the box, output
+    // channels n0 + n*16 + c4 .. +3.
+    const int c4 = (lane >> 4) * 4;
+    int pos[RB];
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++) {
+        const int q = wave * 16 * RB + rb * 16 + (lane & 15);
+        const int qw = q & (TW - 1), qh2 = (q >> lw) & (TH - 1), qd = q >> (lw + lh);
+        pos[rb] = pbase + __mul24(qd, HW) + __mul24(qh2, g.W) + qw;
+    }
     if (part) {
         // Split: fp32 partials [split][position][Cout], bias and activation left to the reduction.
         float *pz = part + (int64_t)blockIdx.z * g.positions() * g.Cout;
 #pragma unroll
-        for (int rb = 0; rb < RB; rb++)
+        for (int rb = 0; rb < RB; rb++) {
+            if (!live[rb]) continue;
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int q = wave * 16 * RB + rb * 16 + (lane >> 4) * 4 + r;
-                if (q >= TP) continue;
-                int qd, qh, qw;
-                box_coords(g, q, qd, qh, qw);
-                const int pos = pbase + __mul24(qd, HW) + __mul24(qh, g.W) + qw;
-#pragma unroll
-                for (int n = 0; n < NT; n++) pz[pos * g.Cout + n0 + n * 16 + (lane & 15)] = acc[rb][n][r];
-            }
+            for (int n = 0; n < NT; n++)
+                *(float4v *)(pz + (int64_t)pos[rb] * g.Cout + n0 + n * 16 + c4) = acc[rb][n];
+        }
         return;
     }
 
-    // Epilogue through LDS: [position][NB + 8] bf16 (the pad spreads the four lane groups' rows
-    // over all banks), then 16-byte stores per position.  A lane holds rows q0..q0+3 of one
-    // channel; neighbouring lanes (channels c, c^1) swap packed pairs so that each writes two
-    // 32-bit words of a channel pair, rows q0, q0+1 from the even lane and q0+2, q0+3 from the
-    // odd one.
+    float bb[NT][4];
+#pragma unroll
+    for (int n = 0; n < NT; n++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) bb[n][r] = bias ? bias[n0 + n * 16 + c4 + r] : 0.0f;
+    // The activations are staged in LDS ([position][NB + 8] bf16) only for the pool and the head.
+    const bool fused = pool_out || head_y;
     constexpr int NS = NB + 8;
-    float bb[NT];
+    if (fused) __syncthreads();  // every wave is done reading the staged chunk
 #pragma unroll
-    for (int n = 0; n < NT; n++) bb[n] = bias ? bias[n0 + n * 16 + (lane & 15)] : 0.0f;
-    bf16 *outs[2] = {zout, aout};
-    const bool odd = lane & 1;
-    const int cpair = (lane & 15) & ~1;
+    for (int rb = 0; rb < RB; rb++) {
+        const int q = wave * 16 * RB + rb * 16 + (lane & 15);
 #pragma unroll
-    for (int pass = 0; pass < 2; pass++) {
-        bf16 *out = outs[pass];
-        // The activations are staged for the pool and the head as well as for their own store.
-        if (!out && (pass == 0 || (!pool_out && !head_y))) continue;
-        __syncthreads();
+        for (int n = 0; n < NT; n++) {
+            float z[4], a[4];
 #pragma unroll
-        for (int rb = 0; rb < RB; rb++)
-#pragma unroll
-            for (int n = 0; n < NT; n++) {
-                uint32_t h[4];
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const float zz = acc[rb][n][r] + bb[n];
-                    h[r] = to_bf16(pass == 1 && gelu_out ? gelu(zz) : zz);
-                }
-                const uint32_t mine = odd ? (h[2] | h[3] << 16) : (h[0] | h[1] << 16);
-                const uint32_t send = odd ? (h[0] | h[1] << 16) : (h[2] | h[3] << 16);
-                // quad_perm [1, 0, 3, 2]: each lane receives its neighbour's `send`.
-                const uint32_t got = (uint32_t)__builtin_amdgcn_mov_dpp((int)send, 0xB1, 0xF, 0xF, false);
-                const uint32_t lo = odd ? (got & 0xFFFFu) | (mine << 16) : (mine & 0xFFFFu) | (got << 16);
-                const uint32_t hi = odd ? (got >> 16) | (mine & 0xFFFF0000u) : (mine >> 16) | (got & 0xFFFF0000u);
-                const int q = wave * 16 * RB + rb * 16 + (lane >> 4) * 4 + (odd ? 2 : 0);
-                uint32_t *row = (uint32_t *)(smem + q * NS + n * 16 + cpair);
-                row[0] = lo;
-                row[NS / 2] = hi;
+            for (int r = 0; r < 4; r++) {
+                z[r] = acc[rb][n][r] + bb[n][r];
+                a[r] = gelu_out ? gelu(z[r]) : z[r];
             }
-        __syncthreads();
-        if (out) {
-            for (int e = tid; e < TP * (NB / 8); e += 256) {
-                const int q = e / (NB / 8), part = e % (NB / 8);
-                int qd, qh, qw;
-                box_coords(g, q, qd, qh, qw);
-                const int pos = pbase + __mul24(qd, HW) + __mul24(qh, g.W) + qw;
-                *(uint4 *)(out + pos * ostride + ooff + n0 + part * 8) = *(const uint4 *)(smem + q * NS + part * 8);
-            }
+            const int64_t o = (int64_t)pos[rb] * ostride + ooff + n0 + n * 16 + c4;
+            if (live[rb] && zout) *(uint2 *)(zout + o) = make_uint2(pack_bf16(z[0], z[1]), pack_bf16(z[2], z[3]));
+            const uint2 av = make_uint2(pack_bf16(a[0], a[1]), pack_bf16(a[2], a[3]));
+            if (live[rb] && aout) *(uint2 *)(aout + o) = av;
+            if (fused) *(uint2 *)(smem + q * NS + n * 16 + c4) = av;
         }
-        if (pass == 1 && pool_out) {
-            // 2x2x2 means of the staged box (even extents, checked by the launcher): item e is
-            // channel group e % (NB/8) of coarse position e / (NB/8), the eight fine values
-            // summed in (dz, dy, dx) order.
-            const int PW = g.TW >> 1, PH = g.TH >> 1, PP = (g.TD >> 1) * PH * PW;
-            const int lpw = __builtin_ctz(PW), lph = __builtin_ctz(PH);
-            const int Dh = g.D >> 1, Hh = g.H >> 1, Wh = g.W >> 1;
-            for (int e = tid; e < PP * (NB / 8); e += 256) {
-                const int pq = e / (NB / 8), grp = e % (NB / 8);
-                const int pw = pq & (PW - 1), ph = (pq >> lpw) & (PH - 1), pd = pq >> (lpw + lph);
-                float sum[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    }
+    if (!fused) return;
+    __syncthreads();
+    if (pool_out) {
+        // 2x2x2 means of the staged box (even extents, checked by the launcher): item e is
+        // channel group e % (NB/8) of coarse position e / (NB/8), the eight fine values summed in
+        // (dz, dy, dx) order.
+        const int PW = TW >> 1, PH = TH >> 1, PP = (TD >> 1) * PH * PW;
+        const int lpw = lw - 1, lph = lh - 1;
+        const int Dh = g.D >> 1, Hh = g.H >> 1, Wh = g.W >> 1;
+        for (int e = tid; e < PP * (NB / 8); e += 256) {
+            const int pq = e / (NB / 8), grp = e % (NB / 8);
+            const int pw = pq & (PW - 1), ph = (pq >> lpw) & (PH - 1), pd = pq >> (lpw + lph);
+            float sum[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int k = 0; k < 8; k++) {
-                    const int q = ((2 * pd + (k >> 2)) * g.TH + 2 * ph + ((k >> 1) & 1)) * g.TW + 2 * pw + (k & 1);
-                    float f[8];
-                    unpack8(*(const uint4 *)(smem + q * NS + grp * 8), f);
+            for (int k = 0; k < 8; k++) {
+                const int q = ((2 * pd + (k >> 2)) * TH + 2 * ph + ((k >> 1) & 1)) * TW + 2 * pw + (k & 1);
+                float f[8];
+                unpack8(*(const uint4 *)(smem + q * NS + grp * 8), f);
 #pragma unroll
-                    for (int c = 0; c < 8; c++) sum[c] += f[c];
-                }
-#pragma unroll
-                for (int c = 0; c < 8; c++) sum[c] *= 0.125f;
-                const int ppos = ((b * Dh + (d0 >> 1) + pd) * Hh + (h0 >> 1) + ph) * Wh + (w0 >> 1) + pw;
-                *(uint4 *)(pool_out + ppos * g.Cout + n0 + grp * 8) = pack8(sum);
+                for (int c = 0; c < 8; c++) sum[c] += f[c];
             }
+#pragma unroll
+            for (int c = 0; c < 8; c++) sum[c] *= 0.125f;
+            const int ppos = ((b * Dh + (d0 >> 1) + pd) * Hh + (h0 >> 1) + ph) * Wh + (w0 >> 1) + pw;
+            *(uint4 *)(pool_out + ppos * g.Cout + n0 + grp * 8) = pack8(sum);
         }
-        if (pass == 1 && head_y) {
-            // The 1x1 head of a 16-channel layer: one position per thread, channels in order.
-            for (int q = tid; q < TP; q += 256) {
-                float sum = head_w[16];
+    }
+    if (head_y) {
+        // The 1x1 head of a 16-channel layer: one position per thread, channels in order.
+        for (int q = tid; q < TP; q += 256) {
+            float sum = head_w[16];
 #pragma unroll
-                for (int c = 0; c < 16; c++) sum += head_w[c] * from_bf16(smem[q * NS + c]);
-                int qd, qh, qw;
-                box_coords(g, q, qd, qh, qw);
-                head_y[pbase + __mul24(qd, HW) + __mul24(qh, g.W) + qw] = sum;
-            }
+            for (int c = 0; c < 16; c++) sum += head_w[c] * from_bf16(smem[q * NS + c]);
+            const int qw = q & (TW - 1), qh2 = (q >> lw) & (TH - 1), qd = q >> (lw + lh);
+            head_y[pbase + __mul24(qd, HW) + __mul24(qh2, g.W) + qw] = sum;
         }
     }
 }
@@ -525,17 +567,26 @@ static void launch_fwd(const Conv &g, dim3 grid, size_t lds, hipStream_t stream,
                        const bf16 *w, const float *bias, bf16 *z, bf16 *a, bool gelu_out,
                        int ostride, int ooff, float *part, bf16 *pool, const float *head,
                        float *head_y) {
-    // Staged 16-byte halves per thread: 2 (TD+2)(TH+2)(TW+2) over 256 threads.
+    // Staged 16-byte halves per thread: 2 (TD+2)(TH+2)(TW+2) over 256 threads (6 for the
+    // 16 x 4 x 4 box, which has a variant of its own).
     const int halves = 2 * (g.TD + 2) * (g.TH + 2) * (g.TW + 2);
     const int sk = (halves + 255) / 256;
+#define PM_FWD(SK, FX) conv_fwd2<NT, RB, SK, FX><<<grid, 256, lds, stream>>>(g, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part, pool, head, head_y)
+    if constexpr (RB == 4) {
+        if (g.TW == 16 && g.TH == 4 && g.TD == 4) {
+            PM_FWD(6, 1);
+            return;
+        }
+    }
     if (sk <= 2)
-        conv_fwd2<NT, RB, 2><<<grid, 256, lds, stream>>>(g, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part, pool, head, head_y);
+        PM_FWD(2, 0);
     else if (sk <= 4)
-        conv_fwd2<NT, RB, 4><<<grid, 256, lds, stream>>>(g, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part, pool, head, head_y);
+        PM_FWD(4, 0);
     else if (sk <= 6)
-        conv_fwd2<NT, RB, 6><<<grid, 256, lds, stream>>>(g, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part, pool, head, head_y);
+        PM_FWD(6, 0);
     else
-        conv_fwd2<NT, RB, 8><<<grid, 256, lds, stream>>>(g, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part, pool, head, head_y);
+        PM_FWD(8, 0);
+#undef PM_FWD
 }
 
 // Sums the split partials of conv_fwd2 in split order, adds the bias and writes z and act(z) as
@@ -639,14 +690,14 @@ int conv_forward(const Conv &g0, const bf16 *x, const bf16 *w, const float *bias
                     ostride % 8 == 0 && ooff % 8 == 0 &&
                     2 * (g.TD + 2) * (g.TH + 2) * (g.TW + 2) <= 256 * 8 &&
                     (g.TW & (g.TW - 1)) == 0 && (g.TH & (g.TH - 1)) == 0 &&
-                    g.positions() * std::max({g.Cin, g.Cout, ostride}) < ((int64_t)1 << 31);
+                    g.positions() * std::max({g.Cin, g.Cout, ostride}) < ((int64_t)1 << 30);
     if (!ok) {
         set_error("conv_forward: shape %dx%dx%dx%d, %d->%d channels unsupported", g.B, g.D, g.H,
                   g.W, g.Cin, g.Cout);
         return PM_ERR_ARGUMENT;
     }
     const size_t stage = ((size_t)2 * (g.TD + 2) * (g.TH + 2) * (g.TW + 2) + 1) * 16 +
-                         (nt <= 2 ? (size_t)((54 * 16 * nt + 255) / 256) * 256 * 16 : 0);
+                         (nt <= 2 ? (size_t)((60 * 16 * nt + 255) / 256) * 256 * 16 : 0);
     const size_t epi = (size_t)m * (16 * nt + 8) * sizeof(bf16);
     const size_t lds = std::max(stage, epi);
     dim3 grid((unsigned)g.tiles(), (unsigned)(g.Cout / (16 * nt)), (unsigned)plan.splits);

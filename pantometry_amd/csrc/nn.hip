@@ -97,8 +97,8 @@ __device__ __forceinline__ int div_small(int q, int d, float inv) {
 // weights as its A operand (16 output channels x 32 k) and the staged activations as B (32 k x 16
 // positions), so every A fragment (read from LDS) feeds RB MFMAs, every B fragment (read from
 // LDS at the tap's shift) feeds NT, and a lane receives four consecutive output channels of one
-// position: the epilogue adds the bias, applies GELU, converts pairs to bf16 in one instruction
-// and stores 8 bytes per lane straight from registers.  Per 16-channel input chunk the box's
+// position: the accumulators start from the bias, and the epilogue applies GELU, converts pairs
+// to bf16 in one instruction and stores 8 bytes per lane straight from registers.  Per 16-channel input chunk the box's
 // neighbourhood is staged in LDS once; the 27 taps run in pairs as K = 32 steps.  With FIXED the
 // box is 16 x 4 x 4 at compile time, so tap offsets are immediates.
 //
@@ -179,11 +179,19 @@ __global__ __launch_bounds__(256, NT == 1 ? 4 : 2) void conv_fwd2(Conv g, const 
         base[rb] = (pd * SH + ph) * SW + pw;
     }
 
+    // Accumulators start from the bias of the lane's four output channels (n0 + n*16 + 4(lane/16)
+    // + r), or from zero in a split launch, whose reduction adds it.
     float4v acc[RB][NT];
 #pragma unroll
-    for (int rb = 0; rb < RB; rb++)
+    for (int n = 0; n < NT; n++) {
+        float4v b0 = float4v{0.f, 0.f, 0.f, 0.f};
+        if (bias && !part) {
 #pragma unroll
-        for (int n = 0; n < NT; n++) acc[rb][n] = float4v{0.f, 0.f, 0.f, 0.f};
+            for (int r = 0; r < 4; r++) b0[r] = bias[n0 + n * 16 + (lane >> 4) * 4 + r];
+        }
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++) acc[rb][n] = b0;
+    }
 
     const int wrow = 27 * g.Cin;
     // The reduction runs over units (16-channel chunk, kernel plane dz); split z of gridDim.z
@@ -364,11 +372,6 @@ __global__ __launch_bounds__(256, NT == 1 ? 4 : 2) void conv_fwd2(Conv g, const 
         return;
     }
 
-    float bb[NT][4];
-#pragma unroll
-    for (int n = 0; n < NT; n++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) bb[n][r] = bias ? bias[n0 + n * 16 + c4 + r] : 0.0f;
     // The activations are staged in LDS ([position][NB + 8] bf16) only for the pool and the head.
     const bool fused = pool_out || head_y;
     constexpr int NS = NB + 8;
@@ -381,7 +384,7 @@ __global__ __launch_bounds__(256, NT == 1 ? 4 : 2) void conv_fwd2(Conv g, const 
             float z[4], a[4];
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                z[r] = acc[rb][n][r] + bb[n][r];
+                z[r] = acc[rb][n][r];
                 a[r] = gelu_out ? gelu(z[r]) : z[r];
             }
             const int64_t o = (int64_t)pos[rb] * ostride + ooff + n0 + n * 16 + c4;

@@ -49,9 +49,10 @@ __device__ __forceinline__ uint4 pack8(const float f[8]) {
 // 0.5 z (1 + tanh(u)) = z / (1 + e^(-2u)) and tanh(u) = 2 / (1 + e^(-2u)) - 1, using the hardware
 // exp2 and reciprocal (v_exp_f32, v_rcp_f32, about 1 ulp each).  The results are rounded to bf16
 // (2^-9) where they are stored, far above the difference from a correctly rounded tanh.
+// The exponent -2u log2(e) = z (B + A z^2) takes one multiply and one fused multiply-add.
 __device__ __forceinline__ float gelu_e2u(float z) {
     const float k2 = 2.0f * 0.7978845608028654f * 1.4426950408889634f;  // 2 sqrt(2/pi) log2(e)
-    return __builtin_amdgcn_exp2f(-k2 * (z + 0.044715f * z * z * z));
+    return __builtin_amdgcn_exp2f(z * __builtin_fmaf(z * z, -k2 * 0.044715f, -k2));
 }
 __device__ __forceinline__ float gelu(float z) {
     return z * __builtin_amdgcn_rcpf(1.0f + gelu_e2u(z));

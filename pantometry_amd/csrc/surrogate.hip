@@ -83,17 +83,21 @@ __device__ __forceinline__ void cell_features(const TT *__restrict__ t, const TT
     *(uint4 *)(out + 8) = make_uint4(w[4], w[5], w[6], w[7]);
 }
 
-// Features of one block from fp64 device fields: a grid of (x blocks of 64, y, z), so a cell's
-// coordinates come from the launch, not from divisions.
-__global__ __launch_bounds__(64) void pack_input(const double *__restrict__ t,
-                                                 const double *__restrict__ k, int edge,
-                                                 float coupling, bf16 *__restrict__ out) {
-    const int x = (int)(blockIdx.x * 64 + threadIdx.x), y = (int)blockIdx.y, z = (int)blockIdx.z;
-    if (x >= edge) return;
+// Features of one block from fp64 device fields: workgroups of 64 x 4 cells over (x, y) and a
+// grid over (x, y, z), so a cell's coordinates come from the launch, not from divisions.
+__global__ __launch_bounds__(256) void pack_input(const double *__restrict__ t,
+                                                  const double *__restrict__ k, int edge,
+                                                  float coupling, bf16 *__restrict__ out) {
+    const int x = (int)(blockIdx.x * 64 + threadIdx.x), y = (int)(blockIdx.y * 4 + threadIdx.y);
+    const int z = (int)blockIdx.z;
+    if (x >= edge || y >= edge) return;
     const int64_t c = x + (int64_t)edge * (y + (int64_t)edge * z);
     cell_features<double>(t, k, 0, edge, x, y, z, coupling, out + c * 16);
 }
-static dim3 pack_grid(int64_t edge) { return dim3((unsigned)((edge + 63) / 64), (unsigned)edge, (unsigned)edge); }
+static dim3 pack_grid(int64_t edge) {
+    return dim3((unsigned)((edge + 63) / 64), (unsigned)((edge + 3) / 4), (unsigned)edge);
+}
+static const dim3 PACK_BLOCK(64, 4);
 
 // Features of a batch drawn from the training set (fp32 fields [sample][cell]).
 __global__ void pack_batch(const float *__restrict__ t, const float *__restrict__ k,
@@ -962,7 +966,7 @@ int pm_surrogate_forward(pm_surrogate *s, int64_t edge, const double *t, const d
     PM_CHECK(dk.alloc(sizeof(double) * (size_t)n));
     PM_HIP(hipMemcpyAsync(dt.d(), t, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, s->stream));
     PM_HIP(hipMemcpyAsync(dk.d(), k, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, s->stream));
-    nn::pack_input<<<nn::pack_grid(edge), 64, 0, s->stream>>>(dt.d(), dk.d(), (int)edge, (float)coupling, s->xin.as<bf16>());
+    nn::pack_input<<<nn::pack_grid(edge), nn::PACK_BLOCK, 0, s->stream>>>(dt.d(), dk.d(), (int)edge, (float)coupling, s->xin.as<bf16>());
     PM_CHECK(run_forward(s, false));
     PM_HIP(hipMemcpyAsync(y, s->y.d(), sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, s->stream));
     PM_HIP(hipStreamSynchronize(s->stream));
@@ -1002,7 +1006,7 @@ int pm_surrogate_step_solid(pm_surrogate *s, pm_solid3d *solid, double dt, doubl
     if (!s->ev1) PM_HIP(hipEventCreate(&s->ev1));
     const unsigned rb = std::min<unsigned>(grid_of(n), MAXB);
     PM_HIP(hipEventRecord(s->ev0, st));
-    nn::pack_input<<<nn::pack_grid(edge), 64, 0, st>>>(cells, kc, (int)edge, (float)(dt * mob), s->xin.as<bf16>());
+    nn::pack_input<<<nn::pack_grid(edge), nn::PACK_BLOCK, 0, st>>>(cells, kc, (int)edge, (float)(dt * mob), s->xin.as<bf16>());
     PM_CHECK(run_forward(s, false));
     nn::apply_prediction_sum<<<rb, 256, 0, st>>>(cells, s->y.as<float>(), n, part0, part1);
     nn::shift_finalize<<<1, 1024, 0, st>>>(part0, part1, rb, n, scal);

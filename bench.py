@@ -252,6 +252,79 @@ def bench_surrogate(args) -> dict:
     }
 
 
+def bench_film(args) -> dict:
+    """The extended heat path: a copper block with an alumina layer, every face exposed to its own
+    environment (convection and radiation) and two volumetric sources, at 128^3 and at a
+    scene-sized 24^3.  Device-resident advance (limit, film, sweep, sources, books and audit every
+    step on the device) against one call per step through the C ABI (each step's limit, loss and
+    books read back on the host)."""
+    import numpy as np
+    import pantometry_amd as pm
+    c0 = 273.15
+    rows = {}
+    for edge, steps, host_steps in ((128, max(args.steps, 200), 40), (24, max(args.steps, 500), 200)):
+        counts, dx = (edge, edge, edge), 1.5e-3
+        n = edge ** 3
+        k, j, i = np.meshgrid(np.arange(edge), np.arange(edge), np.arange(edge), indexing="ij")
+        layer = (k == 1).reshape(-1).astype(np.uint8)
+        top = (k == edge - 1).reshape(-1).astype(np.uint8)
+        core = ((abs(i - edge // 2) < edge // 8) & (abs(j - edge // 2) < edge // 8) &
+                (abs(k - edge // 2) < edge // 8)).reshape(-1).astype(np.uint8)
+        side = (edge * dx) ** 2
+
+        def make():
+            g = pm.Solid3D("module", pm.Substance.of("cu", 8.96e3, 401.0, 385.0, 0.04), counts, dx,
+                           c0 + 60.0)
+            g.fill(pm.Substance.of("alumina", 3720.0, 24.0, 880.0, 0.8), layer)
+            for f, amb, h in ((0, 25.0, 40.0), (1, 30.0, 7.0), (2, 20.0, 150.0), (3, 22.0, 0.0),
+                              (4, 40.0, 3000.0), (5, 35.0, 12.0)):
+                g.losing_from(pm.Face(f), pm.Environment(c0 + amb, h, side))
+            g.dissipating(45.0, top)
+            g.dissipating(3.0, core)
+            return g
+
+        g = make()
+        dt = 0.7 * g.max_stable_dt()
+        g.advance(dt, args.warmup, tolerance=1e-9)
+        t0 = time.perf_counter()
+        totals, ms = g.advance(dt, steps, tolerance=1e-9)
+        wall = time.perf_counter() - t0
+        h = make()
+        bus = pm.Exchange()
+        for _ in range(min(args.warmup, 5)):
+            h.step(0.0, dt, bus)
+        t0 = time.perf_counter()
+        for _ in range(host_steps):
+            h.step(0.0, dt, bus)
+        host = (time.perf_counter() - t0) / host_steps
+        rows[f"{edge}^3"] = {
+            "device_resident_us_per_step": round(wall * 1e6 / steps, 2),
+            "device_loop_us_per_step": round(ms * 1e3 / steps, 2),
+            "gcups": round(n * steps / wall / 1e9, 3),
+            "one_call_per_step_us": round(host * 1e6, 2),
+            "speedup_over_one_call_per_step": round(host / (wall / steps), 2),
+            "steps": steps, "audit": "every step's ledger within 1e-9 of the step before",
+            "roofline": {"bound": "hbm", "bytes_per_cell": 32,
+                         "achieved": round(32 * n / (ms / steps * 1e-3) / 1e9, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(32 * n / (ms / steps * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        }
+    big = rows["128^3"]
+    return {
+        "metric": "GCUPS, FP64 heat with exposed faces and sources (extended path), 128^3",
+        "value": big["gcups"], "unit": "GCUPS", "n_gpus": 1, "steps": big["steps"],
+        "warmup": args.warmup, "ms_per_step": round(big["device_resident_us_per_step"] / 1e3, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: copper with an alumina layer, six exposed faces, two sources",
+        "config": {"workload": "extended heat path, device-resident advance, audit every step"},
+        "roofline": {**big["roofline"],
+                     "note": "device loop time per step over 32 B a cell (T, k and mobility "
+                             "read, T written); nine launches a step: film rates, limit, film "
+                             "shed, sweep, sources, stored heat, books"},
+        "rows": rows, "cpu_baseline": None,
+    }
+
+
 def bench_lines(args) -> dict:
     """The lower-dimensional domains at the reference's own sizes and at the largest line one
     workgroup holds in LDS: device-resident advance (audit every step) against the oracle's
@@ -319,7 +392,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-secondary", dest="secondary", action="store_false",
-                    help="skip the hall, ensemble and surrogate figures the headline line carries")
+                    help="skip the hall, ensemble, surrogate and film figures the headline line carries")
     ap.add_argument("--tile-y", type=int, default=0)
     ap.add_argument("--tile-x", type=int, default=0,
                     help="hall: width of the paired step's column tiles, 64/56/48 (0: 64)")
@@ -328,10 +401,11 @@ def main() -> int:
                     help="ensemble streamed tile policy 0-3 (with --z-chunk; else tuned)")
     ap.add_argument("--steps-per-pass", type=int, choices=[1, 2], default=2,
                     help="2 (default): pairs of steps per pass over HBM; 1: one step per pass")
-    ap.add_argument("--workload", choices=["heat", "hall", "ensemble", "surrogate", "copy", "lines"],
+    ap.add_argument("--workload", choices=["heat", "hall", "ensemble", "surrogate", "film", "copy",
+                                           "lines"],
                     default="heat",
-                    help="heat (the headline 512^3 block), or the hall / ensemble / surrogate "
-                         "secondaries")
+                    help="heat (the headline 512^3 block), or the hall / ensemble / surrogate / "
+                         "film (extended heat path) secondaries")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the heat block is not sharded across GPUs "
@@ -349,6 +423,9 @@ def main() -> int:
         return 0
     if args.workload == "lines":
         print(json.dumps(bench_lines(args)), flush=True)
+        return 0
+    if args.workload == "film":
+        print(json.dumps(bench_film(args)), flush=True)
         return 0
     if args.workload == "surrogate":
         print(json.dumps(bench_surrogate(args)), flush=True)
@@ -369,7 +446,8 @@ def main() -> int:
         sec = {}
         for name, fn, steps in (("hall_385", bench_hall, max(args.steps, 200)),
                                 ("ensemble_512", bench_ensemble, 500),
-                                ("surrogate_128", bench_surrogate, 1024)):
+                                ("surrogate_128", bench_surrogate, 1024),
+                                ("film_128", bench_film, 200)):
             print(f"[bench] {name}", file=sys.stderr, flush=True)
             sargs = argparse.Namespace(**vars(args))
             sargs.tile_y, sargs.z_chunk, sargs.steps_per_pass, sargs.steps = 0, 0, 2, steps
@@ -377,7 +455,7 @@ def main() -> int:
             r = fn(sargs)
             sec[name] = {k: r[k] for k in ("metric", "value", "unit", "ms_per_step", "steps",
                                            "roofline", "config") if k in r}
-            for k in ("runs", "audit_drift", "cpu_baseline"):
+            for k in ("runs", "rows", "audit_drift", "cpu_baseline"):
                 if r.get(k) is not None:
                     sec[name][k] = r[k]
         out["secondary"] = sec

@@ -970,7 +970,8 @@ struct pm_solid3d {
     std::vector<pm_substance> subs;    // subs[0] is the constructor's
     std::vector<uint16_t> which;       // each cell's substance; empty while all are subs[0]
     std::map<int64_t, double> source;  // watts of each generating cell
-    DevBuf bcell, bemis, bkcond, shed, src_c, src_w, ext_part;
+    DevBuf bcell, bemis, bkcond, film_of, shed, src_w, src_of, ext_part;
+    DevBuf film_part, ext_books;  // the film's rate maxima and (lost, supplied) of a device-resident run
     int64_t nboundary = 0, nsource = 0;
     bool tables_valid = false;
     // Void cells (solid.rs:238, 992-1005) and the pairs of solid cells facing each other across
@@ -1044,15 +1045,25 @@ static bool pairs_enabled(const pm_solid3d *s) {
     return s->steps_per_pass == 2 && s->kmode == K_PER_CELL;
 }
 
+// Cells per partial of the stored-heat reduction: about 1024 partials, between 1024 and 16384
+// cells each (a multiple of 256), so a block of 128^3 fills every compute unit and one of 512^3
+// has 8192 partials.  Every path that sums the stored heat uses this partition, so they agree
+// bit for bit.
+constexpr int64_t STORED_PER_BLOCK = 16384;
+static int64_t stored_per_block(int64_t n) {
+    const int64_t want = ((n + 1023) / 1024 + 255) / 256 * 256;
+    return std::min<int64_t>(STORED_PER_BLOCK, std::max<int64_t>(1024, want));
+}
+
 // Per-workgroup partials one launch of either sweep writes, or the blocks of the stored-heat
 // reduction, whichever is more.
-constexpr int64_t STORED_PER_BLOCK = 16384;
 static int64_t partials_needed(const pm_solid3d *s) {
     dim3 g, b, g2;
     heat_grid(s, &g, &b);
     heat_grid(s, &g2, &b, PAIR_TY);
     const int64_t sweep = std::max<int64_t>((int64_t)g.x * g.y * g.z, 2 * (int64_t)g2.x * g2.y * g2.z);
-    return std::max<int64_t>(sweep, (s->n() + STORED_PER_BLOCK - 1) / STORED_PER_BLOCK);
+    const int64_t per = stored_per_block(s->n());
+    return std::max<int64_t>(sweep, (s->n() + per - 1) / per);
 }
 
 static int ensure_partials(pm_solid3d *s) {
@@ -1355,14 +1366,20 @@ static int total_capacity(pm_solid3d *s, double *out) {
     return PM_OK;
 }
 
+// The fixed grid of the extended sweep, so its sums are reproducible: 256 workgroups of 1024
+// threads (its loss partials are folded one by one into the books).
+constexpr int EXT_BLOCKS = 256, EXT_THREADS = 1024;
+// Blocks of at most this many cells evaluate the film's flux inside the sweep.
+constexpr int64_t FILM_INLINE_MAX = 64 * 64 * 64;
+
 // ---- Exposed faces and volumetric sources ---------------------------------------------------
 //
 // The film acts only on the cells of exposed faces, so those cells are listed once (index order,
-// with their substance's emissivity and conductivity) and three small kernels serve them: the
-// state-dependent stability rate (solid.rs:1148-1185), the film's flux out of each listed cell
-// from the step's input (solid.rs:1192-1221), written into a dense `shed` array the sweep reads.
-// The sweep of a block with a film or a source is heat_step_ext, one thread per cell, with the
-// film inside the update and the sources added after it, as solid.rs:1767-1864 orders them.
+// with their substance's emissivity and conductivity) and a small kernel serves them: the
+// state-dependent stability rate (solid.rs:1148-1185).  The sweep of a block with a film or a
+// source is heat_step_ext, one thread per cell: a listed cell (found through a per-cell index
+// map) takes the film's flux out of it from the step's input (solid.rs:1192-1221) inside its
+// update, and a generating cell its source after it, as solid.rs:1767-1864 orders them.
 
 struct FilmFaces {
     int on[6];
@@ -1432,9 +1449,40 @@ __device__ __forceinline__ double face_sum_of(const double *kc, double kface, in
     return f0 + f1 + f2 + f3 + f4 + f5;
 }
 
-// The stability rate of each listed cell at its present temperature,
-// mob * (face_sum + loss_conductance_at(T)/dx), skipped where the loss is zero; per-block maxima
-// (NaN rates lose every comparison, as in f64::max).
+// A film's listed cells and what their rates read.
+struct FilmList {
+    FilmFaces F;
+    const int64_t *bcell;
+    const double *bemis, *bk;
+    int64_t nb;
+    const double *kc;
+    double kface;
+    const double *mob;
+    double mob0;
+    int64_t nx, ny, nz;
+};
+
+// The stability rate of listed cell b at temperatures T, mob * (face_sum +
+// loss_conductance_at(T)/dx), or -INFINITY where the loss is zero.
+__device__ __forceinline__ double film_rate(const FilmList &L, int64_t b, const double *__restrict__ T) {
+    const FilmFaces &F = L.F;
+    const int64_t nx = L.nx, ny = L.ny, nz = L.nz;
+    const int64_t c = L.bcell[b];
+    const int64_t i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+    const double at = T[c];
+    double loss = 0.0;
+    for (int f = 0; f < 6; f++) {
+        if (!F.on[f] || !holds(f, i, j, k, nx, ny, nz)) continue;
+        const double hot = fmax(at, F.ambient[f]);
+        const double h = F.h[f] + 4.0 * L.bemis[b] * STEFAN_BOLTZMANN * pow3(hot);
+        loss += series_with_half_cell(h * F.share[f], L.bk[b], F.share[f], F.dx);
+    }
+    if (loss == 0.0) return -INFINITY;
+    return (L.mob ? L.mob[c] : L.mob0) * (face_sum_of(L.kc, L.kface, c, i, j, k, nx, ny, nz) + loss / F.dx);
+}
+
+// The stability rate of each listed cell at its present temperature, skipped where the loss is
+// zero; per-block maxima (NaN rates lose every comparison, as in f64::max).
 __global__ __launch_bounds__(256) void film_rates(FilmFaces F, const int64_t *__restrict__ bcell,
                                                   const double *__restrict__ bemis,
                                                   const double *__restrict__ bk, int64_t nb,
@@ -1444,22 +1492,10 @@ __global__ __launch_bounds__(256) void film_rates(FilmFaces F, const int64_t *__
                                                   int64_t nx, int64_t ny, int64_t nz,
                                                   double *__restrict__ part) {
     __shared__ double red[256];
+    const FilmList L{F, bcell, bemis, bk, nb, kc, kface, mob, mob0, nx, ny, nz};
     double worst = -INFINITY;
-    for (int64_t b = blockIdx.x * 256 + threadIdx.x; b < nb; b += (int64_t)gridDim.x * 256) {
-        const int64_t c = bcell[b];
-        const int64_t i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
-        const double at = T[c];
-        double loss = 0.0;
-        for (int f = 0; f < 6; f++) {
-            if (!F.on[f] || !holds(f, i, j, k, nx, ny, nz)) continue;
-            const double hot = fmax(at, F.ambient[f]);
-            const double h = F.h[f] + 4.0 * bemis[b] * STEFAN_BOLTZMANN * pow3(hot);
-            loss += series_with_half_cell(h * F.share[f], bk[b], F.share[f], F.dx);
-        }
-        if (loss == 0.0) continue;
-        const double rate = (mob ? mob[c] : mob0) * (face_sum_of(kc, kface, c, i, j, k, nx, ny, nz) + loss / F.dx);
-        worst = fmax(worst, rate);
-    }
+    for (int64_t b = blockIdx.x * 256 + threadIdx.x; b < nb; b += (int64_t)gridDim.x * 256)
+        worst = fmax(worst, film_rate(L, b, T));
     red[threadIdx.x] = worst;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
@@ -1469,47 +1505,76 @@ __global__ __launch_bounds__(256) void film_rates(FilmFaces F, const int64_t *__
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-// Solid3D::film_flux of every listed cell, from the step's input temperatures.
+// Solid3D::film_flux of listed cell b (cell c at (i, j, k), temperature t): each exposed face the
+// cell lies on removes g (t - ambient) / dx, with g the face's loss over the gap in series with
+// the half cell.
+__device__ __forceinline__ double film_flux_at(const FilmFaces &F, const double *__restrict__ bemis,
+                                               const double *__restrict__ bk, int64_t b, double t,
+                                               int64_t i, int64_t j, int64_t k, int64_t nx,
+                                               int64_t ny, int64_t nz) {
+    double out = 0.0;
+    for (int f = 0; f < 6; f++) {
+        if (!F.on[f] || !holds(f, i, j, k, nx, ny, nz)) continue;
+        const double share = F.share[f];
+        const double gap = t - F.ambient[f];
+        if (gap == 0.0) continue;
+        const double bare = loss_from(t, F.ambient[f], F.h[f], share, bemis[b]);
+        const double g = series_with_half_cell(bare / gap, bk[b], share, F.dx);
+        out += g * gap / F.dx;
+    }
+    return out;
+}
+
+// What the sweep needs of a film (film_of: each cell's index in the listed cells, -1 off them;
+// null without a film; shed: the listed cells' film flux computed beforehand by film_shed, or
+// null to compute it in the sweep) and of the sources (src_of: each cell's index in the
+// generating cells, -1 for none; null without sources).
+struct ExtCells {
+    FilmFaces F;
+    const int32_t *film_of;
+    const double *bemis, *bk;
+    const double *shed;
+    const int32_t *src_of;
+    const double *src_w;
+};
+
+// The film's flux out of every listed cell, from the step's input, for a sweep over a block too
+// large to evaluate it in place: there a wave that holds one cell of an x face would evaluate it
+// for that one lane.
 __global__ __launch_bounds__(256) void film_shed(FilmFaces F, const int64_t *__restrict__ bcell,
                                                  const double *__restrict__ bemis,
                                                  const double *__restrict__ bk, int64_t nb,
                                                  const double *__restrict__ T,
                                                  double *__restrict__ shed, int64_t nx, int64_t ny,
-                                                 int64_t nz) {
+                                                 int64_t nz, const AuditState *st) {
+    if (st && st->halted) return;  // a device-resident run that stopped
     for (int64_t b = blockIdx.x * 256 + threadIdx.x; b < nb; b += (int64_t)gridDim.x * 256) {
         const int64_t c = bcell[b];
         const int64_t i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
-        const double t = T[c];
-        double out = 0.0;
-        for (int f = 0; f < 6; f++) {
-            if (!F.on[f] || !holds(f, i, j, k, nx, ny, nz)) continue;
-            const double share = F.share[f];
-            const double gap = t - F.ambient[f];
-            if (gap == 0.0) continue;
-            const double bare = loss_from(t, F.ambient[f], F.h[f], share, bemis[b]);
-            const double g = series_with_half_cell(bare / gap, bk[b], share, F.dx);
-            out += g * gap / F.dx;
-        }
-        shed[c] = out;
+        shed[c] = film_flux_at(F, bemis, bk, b, T[c], i, j, k, nx, ny, nz);
     }
 }
 
 // One step of a block with a film or sources: the plain sweep term by term (faces recomputed
-// from the per-cell conductivity with series, or the uniform face), the film's shed inside the
-// update, rise = (dt*mob)*(flux - shed) (solid.rs:1776-1818).  What the film removed,
-// ((dt*mob)*shed)*C per cell (solid.rs:1821-1823), is summed per block in a fixed order.
-__global__ __launch_bounds__(256) void heat_step_ext(const double *__restrict__ told,
+// from the per-cell conductivity with series, or the uniform face), the film's shed of a listed
+// cell (film_flux from the step's input) inside the update, rise = (dt*mob)*(flux - shed)
+// (solid.rs:1776-1818), then a generating cell's (W*dt)/C (solid.rs:1854-1864), as the
+// reference adds it after the sweep.  What the film removed, ((dt*mob)*shed)*C per cell
+// (solid.rs:1821-1823), is summed per block in a fixed order.
+__global__ __launch_bounds__(EXT_THREADS) void heat_step_ext(const double *__restrict__ told,
                                                      double *__restrict__ tnew,
                                                      const double *__restrict__ kc, double kface,
                                                      const double *__restrict__ mob, double mob0,
                                                      const double *__restrict__ cap, double cap0,
-                                                     const double *__restrict__ shed, double dts,
+                                                     ExtCells x, double dts,
                                                      int64_t nx, int64_t ny, int64_t nz,
-                                                     double *__restrict__ lost_part, PhaseArgs ph) {
-    __shared__ double red[4];
+                                                     double *__restrict__ lost_part, PhaseArgs ph,
+                                                     const AuditState *st) {
+    __shared__ double red[EXT_THREADS / 64];
+    if (st && st->halted) return;  // a device-resident run that stopped
     const int64_t plane = nx * ny, n = plane * nz;
     double lost = 0.0;
-    for (int64_t c = blockIdx.x * 256 + threadIdx.x; c < n; c += (int64_t)gridDim.x * 256) {
+    for (int64_t c = blockIdx.x * EXT_THREADS + threadIdx.x; c < n; c += (int64_t)gridDim.x * EXT_THREADS) {
         const int64_t i = c % nx, j = (c / nx) % ny, k = c / plane;
         const double t = told[c];
         const double kk = kc ? kc[c] : 0.0;
@@ -1520,26 +1585,22 @@ __global__ __launch_bounds__(256) void heat_step_ext(const double *__restrict__ 
         if (j + 1 < ny) flux = flux + (kc ? series_nz(kk, kc[c + nx]) : kface) * (told[c + nx] - t);
         if (k > 0) flux = flux + (kc ? series_nz(kc[c - plane], kk) : kface) * (told[c - plane] - t);
         if (k + 1 < nz) flux = flux + (kc ? series_nz(kk, kc[c + plane]) : kface) * (told[c + plane] - t);
-        const double sh = shed ? shed[c] : 0.0;
+        double sh = 0.0;
+        if (x.film_of && ((i == 0 && x.F.on[0]) || (i + 1 == nx && x.F.on[1]) || (j == 0 && x.F.on[2]) ||
+                          (j + 1 == ny && x.F.on[3]) || (k == 0 && x.F.on[4]) || (k + 1 == nz && x.F.on[5])))
+            sh = x.shed ? x.shed[c] : film_flux_at(x.F, x.bemis, x.bk, x.film_of[c], t, i, j, k, nx, ny, nz);
         const double dtm = dts * (mob ? mob[c] : mob0);
-        tnew[c] = add_kelvin(ph, c, t, dtm * (flux - sh), cap ? cap[c] : cap0);
-        if (sh != 0.0) lost += dtm * sh * (cap ? cap[c] : cap0);
-    }
-    const double total = block_sum<256>(lost, red);
-    if (threadIdx.x == 0) lost_part[blockIdx.x] = total;
-}
-
-// The sources, after the sweep: each generating cell gains (W*dt)/C (solid.rs:1854-1864).
-__global__ void heat_sources(double *__restrict__ T, const int64_t *__restrict__ sc,
-                             const double *__restrict__ sw, int64_t ns,
-                             const double *__restrict__ cap, double cap0, double dts, PhaseArgs ph) {
-    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < ns;
-         q += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t c = sc[q];
-        const double joules = sw[q] * dts;
         const double cc = cap ? cap[c] : cap0;
-        T[c] = add_kelvin(ph, c, T[c], joules / cc, cc);
+        double out = add_kelvin(ph, c, t, dtm * (flux - sh), cc);
+        if (sh != 0.0) lost += dtm * sh * cc;
+        if (x.src_of) {
+            const int32_t q = x.src_of[c];
+            if (q >= 0) out = add_kelvin(ph, c, out, (x.src_w[q] * dts) / cc, cc);
+        }
+        tnew[c] = out;
     }
+    const double total = block_sum<EXT_THREADS>(lost, red);
+    if (threadIdx.x == 0) lost_part[blockIdx.x] = total;
 }
 
 // ScalarField::at for Solid3D, solid.rs:2045-2102: trilinear between cell centres, clamped at
@@ -1603,7 +1664,6 @@ __global__ void heat_sample(const double *__restrict__ T, const uint8_t *__restr
     }
 }
 
-constexpr int EXT_BLOCKS = 1024;  // fixed grid of the extended sweep, so its sums are reproducible
 
 // The gaps' exchange, solid.rs:1866-1878: each gap cell applies the pairs it belongs to in pair
 // order, each pair's watts coef*(Ta^4 - Tb^4) from the step's input, so a cell in several pairs
@@ -1616,7 +1676,9 @@ __global__ __launch_bounds__(256) void gap_exchange(const int64_t *__restrict__ 
                                                     const double *__restrict__ gcoef,
                                                     const double *__restrict__ told,
                                                     const double *__restrict__ cap, double dts,
-                                                    double *__restrict__ tnew, PhaseArgs ph) {
+                                                    double *__restrict__ tnew, PhaseArgs ph,
+                                                    const AuditState *st) {
+    if (st && st->halted) return;  // a device-resident run that stopped
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < ngcell;
          q += (int64_t)gridDim.x * blockDim.x) {
         const int64_t c = gcell[q];
@@ -1657,6 +1719,77 @@ __global__ __launch_bounds__(256) void gap_rates(const int64_t *__restrict__ ga,
         __syncthreads();
     }
     if (threadIdx.x == 0) block_max[blockIdx.x] = smax[0];
+}
+
+// A device-resident run of the extended step halts on the device: audit breaches set 1 (as every
+// audited run does); the limit sets 2 for a NaN ratio (a substance with no diffusivity) and 3 for
+// a ratio over the limit, before the step runs.
+constexpr int HALT_BREACH_FLAG = 1, HALT_NO_DIFFUSIVITY = 2, HALT_LIMIT = 3;
+
+// check_ratio on the device for one step of a device-resident run: the block's static rate, the
+// gaps' and the film's per-block maxima at the step's input, combined by f64::max (exact, so in
+// any order), then dt * rate against 1 + 6e-12.  The ratio goes to `after`, the step to
+// breach_step.
+__global__ __launch_bounds__(256) void ext_limit(const double *__restrict__ gpart, int ng,
+                                                 const double *__restrict__ fpart, int nf,
+                                                 double worst_rate, double dt, AuditState *st,
+                                                 int64_t step) {
+    __shared__ double smax[256];
+    if (st->halted) return;
+    double m = -INFINITY;
+    for (int i = threadIdx.x; i < ng; i += 256) m = rmax(m, gpart[i]);
+    for (int i = threadIdx.x; i < nf; i += 256) m = rmax(m, fpart[i]);
+    smax[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) smax[threadIdx.x] = rmax(smax[threadIdx.x], smax[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double rate = std::isnan(worst_rate) ? worst_rate : rmax(worst_rate, smax[0]);
+        const double ratio = dt * rate;
+        if (std::isnan(ratio)) {
+            st->halted = HALT_NO_DIFFUSIVITY;
+            st->breach_step = step;
+        } else if (ratio > 1.0 + 6e-12) {
+            st->halted = HALT_LIMIT;
+            st->breach_step = step;
+            st->after = ratio;
+        }
+    }
+}
+
+// The books of a device-resident run after each step, as the host keeps them for step_ext and
+// advance_ext: the film's loss of the step as the left fold of the sweep's per-block sums, added
+// to the running loss; the supply after the step (the left fold of W * dt over the generating
+// cells, solid.rs:1856-1862, which the state does not enter, is folded on the host and read from
+// `supplied_after`); the stored heat as the fixed-order sum of its partials; then the ledger
+// (0 + stored) + lost + (-supplied) audited against the step before.  books = (lost, supplied).
+static_assert(EXT_BLOCKS <= 1024, "ext_books stages the sweep's loss partials in 1024 slots");
+__global__ __launch_bounds__(1024) void ext_books(const double *__restrict__ lost_part, int nlost,
+                                                 const double *__restrict__ supplied_after,
+                                                 const double *__restrict__ stored_part,
+                                                 int64_t nstored, double *books, AuditState *st,
+                                                 int64_t step, double *totals) {
+    __shared__ double scratch[16];
+    __shared__ double lp[1024];
+    if (st->halted) return;
+    // The loss partials are loaded in parallel and folded from LDS by one thread in index order.
+    for (int i = threadIdx.x; i < nlost; i += 1024) lp[i] = lost_part[i];
+    const double stored = fixed_order_sum_1024(stored_part, nstored, scratch);  // syncs the block
+    if (threadIdx.x == 0) {
+        double lost = 0.0;
+        for (int i = 0; i < nlost; i++) lost += lp[i];
+        const double lost_total = books[0] + lost;
+        const double supplied = supplied_after[step];
+        books[0] = lost_total, books[1] = supplied;
+        double total = 0.0;
+        total += stored;
+        total += lost_total;
+        total += -supplied;
+        const double scale = rmax(rmax(rmax(0.0, std::fabs(stored)), std::fabs(lost_total)), std::fabs(supplied));
+        audit_step(st, total, scale, step, totals);
+    }
 }
 
 static FilmFaces film_faces(const pm_solid3d *s) {
@@ -1708,21 +1841,32 @@ static int ensure_tables(pm_solid3d *s) {
         PM_HIP(hipMemcpy(s->bcell.d(), cells.data(), sizeof(int64_t) * cells.size(), hipMemcpyHostToDevice));
         PM_HIP(hipMemcpy(s->bemis.d(), emis.data(), sizeof(double) * cells.size(), hipMemcpyHostToDevice));
         PM_HIP(hipMemcpy(s->bkcond.d(), kcond.data(), sizeof(double) * cells.size(), hipMemcpyHostToDevice));
-        if (!s->shed) {
-            PM_CHECK(s->shed.alloc(sizeof(double) * (size_t)s->n()));
-            PM_HIP(hipMemset(s->shed.d(), 0, sizeof(double) * (size_t)s->n()));
+        if (cells.size() > (size_t)INT32_MAX) {
+            set_error("%s: %zu cells on exposed faces, more than the film's index map holds", s->name.c_str(),
+                      cells.size());
+            return PM_ERR_ARGUMENT;
         }
+        std::vector<int32_t> map((size_t)s->n(), -1);
+        for (size_t b = 0; b < cells.size(); b++) map[(size_t)cells[b]] = (int32_t)b;
+        PM_CHECK(s->film_of.alloc(sizeof(int32_t) * map.size()));
+        PM_HIP(hipMemcpy(s->film_of.d(), map.data(), sizeof(int32_t) * map.size(), hipMemcpyHostToDevice));
+        if (s->n() > FILM_INLINE_MAX && !s->shed) PM_CHECK(s->shed.alloc(sizeof(double) * (size_t)s->n()));
     }
     s->nsource = 0;
     for (const auto &e : s->source) s->nsource += !s->is_void(e.first);
     if (s->nsource) {
-        std::vector<int64_t> sc;
+        if (s->nsource > INT32_MAX) {
+            set_error("%s: %lld generating cells, more than the source index map holds", s->name.c_str(),
+                      (long long)s->nsource);
+            return PM_ERR_ARGUMENT;
+        }
+        std::vector<int32_t> map((size_t)s->n(), -1);
         std::vector<double> sw;
         for (const auto &e : s->source)
-            if (!s->is_void(e.first)) sc.push_back(e.first), sw.push_back(e.second);
-        PM_CHECK(s->src_c.alloc(sizeof(int64_t) * sc.size()));
+            if (!s->is_void(e.first)) map[(size_t)e.first] = (int32_t)sw.size(), sw.push_back(e.second);
+        PM_CHECK(s->src_of.alloc(sizeof(int32_t) * map.size()));
         PM_CHECK(s->src_w.alloc(sizeof(double) * sw.size()));
-        PM_HIP(hipMemcpy(s->src_c.d(), sc.data(), sizeof(int64_t) * sc.size(), hipMemcpyHostToDevice));
+        PM_HIP(hipMemcpy(s->src_of.d(), map.data(), sizeof(int32_t) * map.size(), hipMemcpyHostToDevice));
         PM_HIP(hipMemcpy(s->src_w.d(), sw.data(), sizeof(double) * sw.size(), hipMemcpyHostToDevice));
     }
     if (!s->ext_part) PM_CHECK(s->ext_part.alloc(sizeof(double) * EXT_BLOCKS));
@@ -1933,39 +2077,47 @@ static int worst_rate_now(pm_solid3d *s, double *out) {
     return PM_OK;
 }
 
+// The film and source tables the extended sweep reads (ensure_tables has built them).
+static ExtCells ext_cells(const pm_solid3d *s) {
+    ExtCells x{};
+    x.F = film_faces(s);
+    const bool film = s->exposed() && s->nboundary > 0;
+    x.film_of = film ? s->film_of.as<int32_t>() : nullptr;
+    x.bemis = s->bemis.d(), x.bk = s->bkcond.d();
+    x.shed = film && s->n() > FILM_INLINE_MAX ? s->shed.d() : nullptr;
+    x.src_of = s->nsource ? s->src_of.as<int32_t>() : nullptr;
+    x.src_w = s->src_w.d();
+    return x;
+}
+
 // One step of a block with a film or sources (the caller has checked the limit and applied the
-// bus heat): the film's shed from the input, the sweep into the other buffer, the sources, and
+// bus heat): the sweep into the other buffer with the film's shed and the sources, the gaps, and
 // the books: `lost` by the fixed-order sum of what the film removed, `supplied` by the
 // reference's own left fold over the generating cells (solid.rs:1856-1862).
 static int step_ext(pm_solid3d *s, double dt) {
     PM_CHECK(ensure_tables(s));
-    const bool film = s->exposed() && s->nboundary > 0;
-    if (film) {
-        const int blocks = (int)std::min<int64_t>(EXT_BLOCKS, (s->nboundary + 255) / 256);
-        film_shed<<<blocks, 256, 0, s->stream>>>(film_faces(s), s->bcell.as<int64_t>(),
-                                                 s->bemis.d(), s->bkcond.d(), s->nboundary,
-                                                 s->T(), s->shed.d(), s->nx, s->ny, s->nz);
-        PM_HIP(hipGetLastError());
-    }
     const double *kc = s->kmode == K_PER_CELL ? s->kcell.d() : nullptr;
     const double *mob = s->cmode == C_PER_CELL ? s->mob.d() : nullptr;
     const double *cap = s->cmode == C_PER_CELL ? s->cap.d() : nullptr;
     double *out = s->t[s->cur ^ 1].d();
-    heat_step_ext<<<EXT_BLOCKS, 256, 0, s->stream>>>(s->T(), out, kc, s->kface, mob, s->mob0, cap,
-                                                     s->cap0, film ? s->shed.d() : nullptr, dt,
-                                                     s->nx, s->ny, s->nz, s->ext_part.d(),
-                                                     s->phase_args());
-    PM_HIP(hipGetLastError());
-    if (s->nsource) {
-        heat_sources<<<grid_for(s->nsource, 256), 256, 0, s->stream>>>(
-            out, s->src_c.as<int64_t>(), s->src_w.d(), s->nsource, cap, s->cap0, dt, s->phase_args());
+    const ExtCells xc = ext_cells(s);
+    if (xc.shed) {
+        const int blocks = (int)std::min<int64_t>(1024, (s->nboundary + 255) / 256);
+        film_shed<<<blocks, 256, 0, s->stream>>>(xc.F, s->bcell.as<int64_t>(), s->bemis.d(), s->bkcond.d(),
+                                                 s->nboundary, s->T(), s->shed.d(), s->nx, s->ny, s->nz,
+                                                 nullptr);
         PM_HIP(hipGetLastError());
     }
+    heat_step_ext<<<EXT_BLOCKS, EXT_THREADS, 0, s->stream>>>(s->T(), out, kc, s->kface, mob, s->mob0, cap,
+                                                     s->cap0, xc, dt,
+                                                     s->nx, s->ny, s->nz, s->ext_part.d(),
+                                                     s->phase_args(), nullptr);
+    PM_HIP(hipGetLastError());
     if (s->ngcell) {
         gap_exchange<<<grid_for(s->ngcell, 256), 256, 0, s->stream>>>(
             s->gcell.as<int64_t>(), s->gstart.as<int64_t>(), s->gent.as<int64_t>(), s->ngcell,
             s->gap_a.as<int64_t>(), s->gap_b.as<int64_t>(), s->gap_coef.d(), s->T(), s->cap.d(), dt, out,
-            s->phase_args());
+            s->phase_args(), nullptr);
         PM_HIP(hipGetLastError());
     }
     std::vector<double> part(EXT_BLOCKS);
@@ -2029,10 +2181,10 @@ __global__ __launch_bounds__(256) void heat_stored_kernel(const double *T, const
 
 static int heat_stored_in(pm_solid3d *s, const RunSpace &w, double *stored) {
     const int64_t n = s->n();
-    const int64_t blocks = (n + STORED_PER_BLOCK - 1) / STORED_PER_BLOCK;
+    const int64_t per = stored_per_block(n), blocks = (n + per - 1) / per;
     heat_stored_kernel<<<(unsigned)blocks, 256, 0, s->stream>>>(
-        s->T(), s->cmode == C_PER_CELL ? s->cap.d() : nullptr, s->cap0, s->reference, n,
-        STORED_PER_BLOCK, w.partials, s->phase_args());
+        s->T(), s->cmode == C_PER_CELL ? s->cap.d() : nullptr, s->cap0, s->reference, n, per,
+        w.partials, s->phase_args());
     PM_HIP(hipGetLastError());
     heat_finalize<<<1, 1024, 0, s->stream>>>(w.partials, blocks, w.audit, 0.0, 0.0, -1, nullptr,
                                              w.stored);
@@ -2549,6 +2701,133 @@ int pm_solid3d_ledger(pm_solid3d *s, pm_books *books) {
     return PM_OK;
 }
 
+// advance for a block with a film, sources, gaps or latent heat, resident on the device: every
+// step's limit (film_rates, gap_rates, ext_limit), its sweep with the film's shed and the sources
+// (heat_step_ext), its gaps, then its stored heat and books (ext_books, which audits): four to six
+// launches a step, enqueued on the handle's stream with no
+// host synchronisation until the run ends.  The arithmetic and the order of every sum are those
+// of step_ext and the host books, so fields, books and ledger totals are bit-identical to
+// stepping one call at a time.  A refusal (halted before its step) or a breach (after it) makes
+// the later launches return at once, so the state is the one the last step that ran left.  A
+// block of two phases re-derives its operator on the host after every step and runs
+// advance_ext instead.
+static int advance_ext_device(pm_solid3d *s, double dt, uint64_t steps, double tolerance,
+                              double *totals, double *elapsed_ms, pm_violation *v) {
+    PM_CHECK(ensure_tables(s));
+    RunSpace w;
+    PM_CHECK(handle_space(s, &w));
+    double stored = 0.0;
+    PM_CHECK(heat_stored_in(s, w, &stored));
+    double total0 = 0.0;
+    total0 += stored;
+    total0 += s->lost;
+    total0 += -s->supplied;
+    const double scale0 = rmax(rmax(rmax(0.0, std::fabs(stored)), std::fabs(s->lost)), std::fabs(s->supplied));
+    PM_CHECK(reset_audit(s, w.audit, total0, scale0, tolerance));
+    if (!s->ext_books) PM_CHECK(s->ext_books.alloc(2 * sizeof(double)));
+    const double books0[2] = {s->lost, s->supplied};
+    PM_HIP(hipMemcpy(s->ext_books.d(), books0, sizeof books0, hipMemcpyHostToDevice));
+    DevBuf dtot, dsup;
+    PM_CHECK(dtot.alloc(sizeof(double) * (size_t)std::max<uint64_t>(steps, 1)));
+    PM_CHECK(dsup.alloc(sizeof(double) * (size_t)std::max<uint64_t>(steps, 1)));
+    fill_nan<<<grid_for((int64_t)std::max<uint64_t>(steps, 1), 256), 256, 0, s->stream>>>(dtot.d(), (int64_t)std::max<uint64_t>(steps, 1));
+    PM_HIP(hipGetLastError());
+    // The generating cells' W * dt in the order the books fold them (void cells make nothing).
+    std::vector<double> joules;
+    for (const auto &e : s->source)
+        if (!s->is_void(e.first)) joules.push_back(e.second * dt);
+    double supplied = s->supplied;
+    std::vector<double> sup;
+
+    const bool film = s->exposed() && s->nboundary > 0;
+    const int64_t npairs = (int64_t)s->gaps.size();
+    const int fblocks = film ? (int)std::min<int64_t>(EXT_BLOCKS, (s->nboundary + 255) / 256) : 0;
+    const int gblocks = npairs ? (int)std::min<int64_t>(EXT_BLOCKS, (npairs + 255) / 256) : 0;
+    if (!s->film_part) PM_CHECK(s->film_part.alloc(sizeof(double) * EXT_BLOCKS));
+    const double *kc = s->kmode == K_PER_CELL ? s->kcell.d() : nullptr;
+    const double *mob = s->cmode == C_PER_CELL ? s->mob.d() : nullptr;
+    const double *cap = s->cmode == C_PER_CELL ? s->cap.d() : nullptr;
+    const int64_t n = s->n();
+    const int64_t per_stored = stored_per_block(n), nstored = (n + per_stored - 1) / per_stored;
+    const FilmFaces faces = film_faces(s);
+    const PhaseArgs ph = s->phase_args();
+    AuditState *st = w.audit;
+    const ExtCells xc = ext_cells(s);
+
+    PM_HIP(hipEventRecord(s->ev0, s->stream));
+    constexpr uint64_t CHUNK = 64;  // steps whose supply is folded and sent ahead of their launches
+    for (uint64_t q = 0; q < steps; q++) {
+        if (q % CHUNK == 0) {
+            // Folded here while the device runs the steps before; the copy is ordered before the
+            // chunk's launches on the stream (a copy from pageable memory is staged at the call).
+            const uint64_t len = std::min(CHUNK, steps - q);
+            sup.resize(len);
+            for (uint64_t r = 0; r < len; r++) {
+                for (double j : joules) supplied += j;
+                sup[r] = supplied;
+            }
+            PM_HIP(hipMemcpyAsync(dsup.d() + q, sup.data(), sizeof(double) * len, hipMemcpyHostToDevice, s->stream));
+        }
+        const double *tin = s->t[(s->cur + q) & 1].d();
+        double *tout = s->t[(s->cur + q + 1) & 1].d();
+        if (gblocks)
+            gap_rates<<<gblocks, 256, 0, s->stream>>>(s->gap_a.as<int64_t>(), s->gap_b.as<int64_t>(),
+                                                      s->gap_coef.d(), npairs, tin, s->cap.d(), s->gap_part.d());
+        if (film)
+            film_rates<<<fblocks, 256, 0, s->stream>>>(faces, s->bcell.as<int64_t>(), s->bemis.d(),
+                                                       s->bkcond.d(), s->nboundary, tin, kc, s->kface,
+                                                       mob, s->mob0, s->nx, s->ny, s->nz, s->film_part.d());
+        ext_limit<<<1, 256, 0, s->stream>>>(s->gap_part.d(), gblocks, s->film_part.d(), fblocks,
+                                            s->worst_rate, dt, st, (int64_t)q);
+        if (xc.shed)
+            film_shed<<<fblocks, 256, 0, s->stream>>>(faces, s->bcell.as<int64_t>(), s->bemis.d(),
+                                                      s->bkcond.d(), s->nboundary, tin, s->shed.d(),
+                                                      s->nx, s->ny, s->nz, st);
+        heat_step_ext<<<EXT_BLOCKS, EXT_THREADS, 0, s->stream>>>(tin, tout, kc, s->kface, mob, s->mob0, cap,
+                                                         s->cap0, xc, dt, s->nx, s->ny, s->nz,
+                                                         s->ext_part.d(), ph, st);
+        if (s->ngcell)
+            gap_exchange<<<grid_for(s->ngcell, 256), 256, 0, s->stream>>>(
+                s->gcell.as<int64_t>(), s->gstart.as<int64_t>(), s->gent.as<int64_t>(), s->ngcell,
+                s->gap_a.as<int64_t>(), s->gap_b.as<int64_t>(), s->gap_coef.d(), tin, s->cap.d(), dt,
+                tout, ph, st);
+        heat_stored_kernel<<<(unsigned)nstored, 256, 0, s->stream>>>(tout, cap, s->cap0, s->reference, n,
+                                                                     per_stored, w.partials, ph);
+        ext_books<<<1, 1024, 0, s->stream>>>(s->ext_part.d(), EXT_BLOCKS, dsup.d(), w.partials, nstored,
+                                              s->ext_books.d(), st, (int64_t)q, dtot.d());
+        PM_HIP(hipGetLastError());
+    }
+    PM_HIP(hipEventRecord(s->ev1, s->stream));
+    AuditState a;
+    double books[2];
+    PM_HIP(hipMemcpyAsync(&a, st, sizeof a, hipMemcpyDeviceToHost, s->stream));
+    PM_HIP(hipMemcpyAsync(books, s->ext_books.d(), sizeof books, hipMemcpyDeviceToHost, s->stream));
+    if (totals && steps)
+        PM_HIP(hipMemcpyAsync(totals, dtot.d(), sizeof(double) * steps, hipMemcpyDeviceToHost, s->stream));
+    PM_HIP(hipStreamSynchronize(s->stream));
+    if (elapsed_ms) {
+        float ms = 0.0f;
+        PM_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        *elapsed_ms = ms;
+    }
+    // The steps that ran: up to a refusal's step, through a breach's.
+    const uint64_t ran = !a.halted ? steps : a.halted == HALT_BREACH_FLAG ? (uint64_t)a.breach_step + 1
+                                                                          : (uint64_t)a.breach_step;
+    s->cur = (int)((s->cur + ran) & 1);
+    s->lost = books[0], s->supplied = books[1];
+    if (!a.halted) return PM_OK;
+    if (a.halted == HALT_NO_DIFFUSIVITY) {
+        violation_at(v, s->name.c_str(), "substance has no diffusivity", INFINITY, a.breach_step);
+    } else if (a.halted == HALT_LIMIT) {
+        const double F = 1.0 / 6.0;
+        const std::string site = s->name + " (explicit 3D conduction)";
+        fill_violation(v, "Fourier number", site.c_str(), F, a.after * F, F, 1e-12, a.breach_step);
+    } else {
+        fill_violation(v, "energy", "simulation", a.before, a.after, a.scale, tolerance, a.breach_step);
+    }
+    return PM_VIOLATION;
+}
+
 // advance for a block with a film or sources: the limit is checked before every step (it moves
 // with the state), each step runs through step_ext, and the books are audited on the host after
 // every step, stored + lost - supplied against the step before (conserved.rs:323-363).  A
@@ -2594,7 +2873,9 @@ static int advance_ext(pm_solid3d *s, double dt, uint64_t steps, double toleranc
 int pm_solid3d_advance(pm_solid3d *s, double dt, uint64_t steps, double tolerance,
                        double *totals, double *elapsed_ms, pm_violation *v) {
     if (!s) return PM_ERR_ARGUMENT;
-    if (s->extended()) return advance_ext(s, dt, steps, tolerance, totals, elapsed_ms, v);
+    if (s->extended())
+        return s->two_phase ? advance_ext(s, dt, steps, tolerance, totals, elapsed_ms, v)
+                            : advance_ext_device(s, dt, steps, tolerance, totals, elapsed_ms, v);
     RunSpace w;
     PM_CHECK(handle_space(s, &w));
     DevBuf dtot;

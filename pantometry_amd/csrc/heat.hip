@@ -970,7 +970,7 @@ struct pm_solid3d {
     std::vector<pm_substance> subs;    // subs[0] is the constructor's
     std::vector<uint16_t> which;       // each cell's substance; empty while all are subs[0]
     std::map<int64_t, double> source;  // watts of each generating cell
-    DevBuf bcell, bemis, bkcond, film_of, shed, src_w, src_of, ext_part;
+    DevBuf bcell, bemis, bkcond, shed, src_w, src_of, ext_part;
     DevBuf film_part, ext_books;  // the film's rate maxima and (lost, supplied) of a device-resident run
     int64_t nboundary = 0, nsource = 0;
     bool tables_valid = false;
@@ -1366,11 +1366,28 @@ static int total_capacity(pm_solid3d *s, double *out) {
     return PM_OK;
 }
 
-// The fixed grid of the extended sweep, so its sums are reproducible: 256 workgroups of 1024
-// threads (its loss partials are folded one by one into the books).
-constexpr int EXT_BLOCKS = 256, EXT_THREADS = 1024;
-// Blocks of at most this many cells evaluate the film's flux inside the sweep.
-constexpr int64_t FILM_INLINE_MAX = 64 * 64 * 64;
+// The extended sweep's grid, a function of the block's size only, so its sums are reproducible:
+// one cell a thread in workgroups of 1024, at most 256 workgroups (larger blocks loop: the grid
+// then sweeps the block in contiguous strides, and the neighbours' planes stay in L2; 2048
+// workgroups at 128^3 ran 25% slower).  Its per-workgroup loss partials are folded in two levels
+// (loss_fold).  The rate kernels of the film and the gaps use at most EXT_BLOCKS workgroups of
+// 256.
+constexpr int EXT_BLOCKS = 256, EXT_THREADS = 1024, EXT_SWEEP_MAX = 256, LOSS_CHUNK = 64;
+static int ext_sweep_blocks(int64_t n) {
+    return (int)std::min<int64_t>(EXT_SWEEP_MAX, std::max<int64_t>(1, (n + EXT_THREADS - 1) / EXT_THREADS));
+}
+// The film's loss of a step from the sweep's partials: chunks of LOSS_CHUNK folded in index
+// order, then the chunks' sums folded in order.  The same on the host (step_ext) and the device
+// (ext_books).
+static double loss_fold(const double *part, int n) {
+    double total = 0.0;
+    for (int c = 0; c < n; c += LOSS_CHUNK) {
+        double chunk = 0.0;
+        for (int i = c; i < std::min(n, c + LOSS_CHUNK); i++) chunk += part[i];
+        total += chunk;
+    }
+    return total;
+}
 
 // ---- Exposed faces and volumetric sources ---------------------------------------------------
 //
@@ -1395,6 +1412,19 @@ __device__ __forceinline__ double pow4(double x) {
     return x2 * x2;
 }
 __device__ __forceinline__ double pow3(double x) { return x * (x * x); }
+
+// A cell's coordinates from its index, with two 32-bit divisions when the block has fewer than
+// 2^32 cells (64-bit division is a long software sequence on the GPU).
+__device__ __forceinline__ void coords_of(int64_t c, int64_t nx, int64_t ny, int64_t nz, int64_t &i,
+                                          int64_t &j, int64_t &k) {
+    if (nx * ny * nz <= (int64_t)UINT32_MAX) {
+        const uint32_t cu = (uint32_t)c, x = (uint32_t)nx;
+        const uint32_t q = cu / x, kk = q / (uint32_t)ny;
+        i = cu - q * x, j = q - kk * (uint32_t)ny, k = kk;
+    } else {
+        i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+    }
+}
 
 // Environment::loss_from, crates/pantometry-thermal/src/lib.rs:104-110.
 __device__ __forceinline__ double loss_from(double t, double ambient, double h, double area,
@@ -1468,7 +1498,8 @@ __device__ __forceinline__ double film_rate(const FilmList &L, int64_t b, const 
     const FilmFaces &F = L.F;
     const int64_t nx = L.nx, ny = L.ny, nz = L.nz;
     const int64_t c = L.bcell[b];
-    const int64_t i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+    int64_t i, j, k;
+    coords_of(c, nx, ny, nz, i, j, k);
     const double at = T[c];
     double loss = 0.0;
     for (int f = 0; f < 6; f++) {
@@ -1525,22 +1556,20 @@ __device__ __forceinline__ double film_flux_at(const FilmFaces &F, const double 
     return out;
 }
 
-// What the sweep needs of a film (film_of: each cell's index in the listed cells, -1 off them;
-// null without a film; shed: the listed cells' film flux computed beforehand by film_shed, or
-// null to compute it in the sweep) and of the sources (src_of: each cell's index in the
-// generating cells, -1 for none; null without sources).
+// What the sweep needs of a film (faces: bit f set for an exposed face f; shed: the listed cells'
+// film flux, computed by film_shed from the step's input; null without a film) and of the
+// sources (src_of: each cell's index in the generating cells, -1 for none; null without
+// sources).  Small, so the sweep's arguments fit its scalar registers.
 struct ExtCells {
-    FilmFaces F;
-    const int32_t *film_of;
-    const double *bemis, *bk;
+    uint32_t faces;
     const double *shed;
     const int32_t *src_of;
     const double *src_w;
 };
 
-// The film's flux out of every listed cell, from the step's input, for a sweep over a block too
-// large to evaluate it in place: there a wave that holds one cell of an x face would evaluate it
-// for that one lane.
+// The film's flux out of every listed cell, from the step's input, into the dense `shed` the
+// sweep reads at the cells of exposed faces.  A launch of its own: in the sweep, a wave holding
+// one cell of an x face would evaluate it for that one lane.
 __global__ __launch_bounds__(256) void film_shed(FilmFaces F, const int64_t *__restrict__ bcell,
                                                  const double *__restrict__ bemis,
                                                  const double *__restrict__ bk, int64_t nb,
@@ -1550,14 +1579,15 @@ __global__ __launch_bounds__(256) void film_shed(FilmFaces F, const int64_t *__r
     if (st && st->halted) return;  // a device-resident run that stopped
     for (int64_t b = blockIdx.x * 256 + threadIdx.x; b < nb; b += (int64_t)gridDim.x * 256) {
         const int64_t c = bcell[b];
-        const int64_t i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+        int64_t i, j, k;
+        coords_of(c, nx, ny, nz, i, j, k);
         shed[c] = film_flux_at(F, bemis, bk, b, T[c], i, j, k, nx, ny, nz);
     }
 }
 
 // One step of a block with a film or sources: the plain sweep term by term (faces recomputed
 // from the per-cell conductivity with series, or the uniform face), the film's shed of a listed
-// cell (film_flux from the step's input) inside the update, rise = (dt*mob)*(flux - shed)
+// cell (film_flux from the step's input, by film_shed) inside the update, rise = (dt*mob)*(flux - shed)
 // (solid.rs:1776-1818), then a generating cell's (W*dt)/C (solid.rs:1854-1864), as the
 // reference adds it after the sweep.  What the film removed, ((dt*mob)*shed)*C per cell
 // (solid.rs:1821-1823), is summed per block in a fixed order.
@@ -1575,7 +1605,8 @@ __global__ __launch_bounds__(EXT_THREADS) void heat_step_ext(const double *__res
     const int64_t plane = nx * ny, n = plane * nz;
     double lost = 0.0;
     for (int64_t c = blockIdx.x * EXT_THREADS + threadIdx.x; c < n; c += (int64_t)gridDim.x * EXT_THREADS) {
-        const int64_t i = c % nx, j = (c / nx) % ny, k = c / plane;
+        int64_t i, j, k;
+        coords_of(c, nx, ny, nz, i, j, k);
         const double t = told[c];
         const double kk = kc ? kc[c] : 0.0;
         double flux = 0.0;
@@ -1586,9 +1617,9 @@ __global__ __launch_bounds__(EXT_THREADS) void heat_step_ext(const double *__res
         if (k > 0) flux = flux + (kc ? series_nz(kc[c - plane], kk) : kface) * (told[c - plane] - t);
         if (k + 1 < nz) flux = flux + (kc ? series_nz(kk, kc[c + plane]) : kface) * (told[c + plane] - t);
         double sh = 0.0;
-        if (x.film_of && ((i == 0 && x.F.on[0]) || (i + 1 == nx && x.F.on[1]) || (j == 0 && x.F.on[2]) ||
-                          (j + 1 == ny && x.F.on[3]) || (k == 0 && x.F.on[4]) || (k + 1 == nz && x.F.on[5])))
-            sh = x.shed ? x.shed[c] : film_flux_at(x.F, x.bemis, x.bk, x.film_of[c], t, i, j, k, nx, ny, nz);
+        if (x.shed && (((i == 0) && (x.faces & 1)) || ((i + 1 == nx) && (x.faces & 2)) || ((j == 0) && (x.faces & 4)) ||
+                       ((j + 1 == ny) && (x.faces & 8)) || ((k == 0) && (x.faces & 16)) || ((k + 1 == nz) && (x.faces & 32))))
+            sh = x.shed[c];
         const double dtm = dts * (mob ? mob[c] : mob0);
         const double cc = cap ? cap[c] : cap0;
         double out = add_kelvin(ph, c, t, dtm * (flux - sh), cc);
@@ -1765,21 +1796,31 @@ __global__ __launch_bounds__(256) void ext_limit(const double *__restrict__ gpar
 // cells, solid.rs:1856-1862, which the state does not enter, is folded on the host and read from
 // `supplied_after`); the stored heat as the fixed-order sum of its partials; then the ledger
 // (0 + stored) + lost + (-supplied) audited against the step before.  books = (lost, supplied).
-static_assert(EXT_BLOCKS <= 1024, "ext_books stages the sweep's loss partials in 1024 slots");
+static_assert(EXT_SWEEP_MAX / LOSS_CHUNK <= 1024, "ext_books folds a chunk of loss partials a thread");
 __global__ __launch_bounds__(1024) void ext_books(const double *__restrict__ lost_part, int nlost,
                                                  const double *__restrict__ supplied_after,
                                                  const double *__restrict__ stored_part,
                                                  int64_t nstored, double *books, AuditState *st,
                                                  int64_t step, double *totals) {
     __shared__ double scratch[16];
-    __shared__ double lp[1024];
+    __shared__ double lp[EXT_SWEEP_MAX];
+    __shared__ double chunk_sum[EXT_SWEEP_MAX / LOSS_CHUNK];
     if (st->halted) return;
-    // The loss partials are loaded in parallel and folded from LDS by one thread in index order.
+    // loss_fold: the partials are loaded in parallel, each chunk folded in order by a thread of
+    // its own, then the chunks' sums in order by thread 0.
     for (int i = threadIdx.x; i < nlost; i += 1024) lp[i] = lost_part[i];
+    __syncthreads();
+    const int chunks = (nlost + LOSS_CHUNK - 1) / LOSS_CHUNK;
+    if ((int)threadIdx.x < chunks) {
+        const int c = threadIdx.x * LOSS_CHUNK;
+        double acc = 0.0;
+        for (int i = c; i < min(nlost, c + LOSS_CHUNK); i++) acc += lp[i];
+        chunk_sum[threadIdx.x] = acc;
+    }
     const double stored = fixed_order_sum_1024(stored_part, nstored, scratch);  // syncs the block
     if (threadIdx.x == 0) {
         double lost = 0.0;
-        for (int i = 0; i < nlost; i++) lost += lp[i];
+        for (int c = 0; c < chunks; c++) lost += chunk_sum[c];
         const double lost_total = books[0] + lost;
         const double supplied = supplied_after[step];
         books[0] = lost_total, books[1] = supplied;
@@ -1841,16 +1882,7 @@ static int ensure_tables(pm_solid3d *s) {
         PM_HIP(hipMemcpy(s->bcell.d(), cells.data(), sizeof(int64_t) * cells.size(), hipMemcpyHostToDevice));
         PM_HIP(hipMemcpy(s->bemis.d(), emis.data(), sizeof(double) * cells.size(), hipMemcpyHostToDevice));
         PM_HIP(hipMemcpy(s->bkcond.d(), kcond.data(), sizeof(double) * cells.size(), hipMemcpyHostToDevice));
-        if (cells.size() > (size_t)INT32_MAX) {
-            set_error("%s: %zu cells on exposed faces, more than the film's index map holds", s->name.c_str(),
-                      cells.size());
-            return PM_ERR_ARGUMENT;
-        }
-        std::vector<int32_t> map((size_t)s->n(), -1);
-        for (size_t b = 0; b < cells.size(); b++) map[(size_t)cells[b]] = (int32_t)b;
-        PM_CHECK(s->film_of.alloc(sizeof(int32_t) * map.size()));
-        PM_HIP(hipMemcpy(s->film_of.d(), map.data(), sizeof(int32_t) * map.size(), hipMemcpyHostToDevice));
-        if (s->n() > FILM_INLINE_MAX && !s->shed) PM_CHECK(s->shed.alloc(sizeof(double) * (size_t)s->n()));
+        if (!s->shed) PM_CHECK(s->shed.alloc(sizeof(double) * (size_t)s->n()));
     }
     s->nsource = 0;
     for (const auto &e : s->source) s->nsource += !s->is_void(e.first);
@@ -1869,7 +1901,7 @@ static int ensure_tables(pm_solid3d *s) {
         PM_HIP(hipMemcpy(s->src_of.d(), map.data(), sizeof(int32_t) * map.size(), hipMemcpyHostToDevice));
         PM_HIP(hipMemcpy(s->src_w.d(), sw.data(), sizeof(double) * sw.size(), hipMemcpyHostToDevice));
     }
-    if (!s->ext_part) PM_CHECK(s->ext_part.alloc(sizeof(double) * EXT_BLOCKS));
+    if (!s->ext_part) PM_CHECK(s->ext_part.alloc(sizeof(double) * EXT_SWEEP_MAX));
     s->tables_valid = true;
     return PM_OK;
 }
@@ -2080,11 +2112,10 @@ static int worst_rate_now(pm_solid3d *s, double *out) {
 // The film and source tables the extended sweep reads (ensure_tables has built them).
 static ExtCells ext_cells(const pm_solid3d *s) {
     ExtCells x{};
-    x.F = film_faces(s);
     const bool film = s->exposed() && s->nboundary > 0;
-    x.film_of = film ? s->film_of.as<int32_t>() : nullptr;
-    x.bemis = s->bemis.d(), x.bk = s->bkcond.d();
-    x.shed = film && s->n() > FILM_INLINE_MAX ? s->shed.d() : nullptr;
+    for (int f = 0; f < 6; f++)
+        if (s->film[f].on) x.faces |= 1u << f;
+    x.shed = film ? s->shed.d() : nullptr;
     x.src_of = s->nsource ? s->src_of.as<int32_t>() : nullptr;
     x.src_w = s->src_w.d();
     return x;
@@ -2103,12 +2134,13 @@ static int step_ext(pm_solid3d *s, double dt) {
     const ExtCells xc = ext_cells(s);
     if (xc.shed) {
         const int blocks = (int)std::min<int64_t>(1024, (s->nboundary + 255) / 256);
-        film_shed<<<blocks, 256, 0, s->stream>>>(xc.F, s->bcell.as<int64_t>(), s->bemis.d(), s->bkcond.d(),
+        film_shed<<<blocks, 256, 0, s->stream>>>(film_faces(s), s->bcell.as<int64_t>(), s->bemis.d(), s->bkcond.d(),
                                                  s->nboundary, s->T(), s->shed.d(), s->nx, s->ny, s->nz,
                                                  nullptr);
         PM_HIP(hipGetLastError());
     }
-    heat_step_ext<<<EXT_BLOCKS, EXT_THREADS, 0, s->stream>>>(s->T(), out, kc, s->kface, mob, s->mob0, cap,
+    const int sweep_blocks = ext_sweep_blocks(s->n());
+    heat_step_ext<<<sweep_blocks, EXT_THREADS, 0, s->stream>>>(s->T(), out, kc, s->kface, mob, s->mob0, cap,
                                                      s->cap0, xc, dt,
                                                      s->nx, s->ny, s->nz, s->ext_part.d(),
                                                      s->phase_args(), nullptr);
@@ -2120,13 +2152,11 @@ static int step_ext(pm_solid3d *s, double dt) {
             s->phase_args(), nullptr);
         PM_HIP(hipGetLastError());
     }
-    std::vector<double> part(EXT_BLOCKS);
-    PM_HIP(hipMemcpyAsync(part.data(), s->ext_part.d(), sizeof(double) * EXT_BLOCKS, hipMemcpyDeviceToHost, s->stream));
+    std::vector<double> part((size_t)sweep_blocks);
+    PM_HIP(hipMemcpyAsync(part.data(), s->ext_part.d(), sizeof(double) * part.size(), hipMemcpyDeviceToHost, s->stream));
     PM_HIP(hipStreamSynchronize(s->stream));
     s->cur ^= 1;
-    double lost = 0.0;
-    for (double p : part) lost += p;
-    s->lost += lost;
+    s->lost += loss_fold(part.data(), sweep_blocks);
     for (const auto &e : s->source)
         if (!s->is_void(e.first)) s->supplied += e.second * dt;  // void makes nothing (solid.rs:1857)
     // A two-phase block re-derives its operator, because the fractions moved (solid.rs:1897-1899).
@@ -2753,6 +2783,7 @@ static int advance_ext_device(pm_solid3d *s, double dt, uint64_t steps, double t
     const PhaseArgs ph = s->phase_args();
     AuditState *st = w.audit;
     const ExtCells xc = ext_cells(s);
+    const int sweep_blocks = ext_sweep_blocks(n);
 
     PM_HIP(hipEventRecord(s->ev0, s->stream));
     constexpr uint64_t CHUNK = 64;  // steps whose supply is folded and sent ahead of their launches
@@ -2783,7 +2814,7 @@ static int advance_ext_device(pm_solid3d *s, double dt, uint64_t steps, double t
             film_shed<<<fblocks, 256, 0, s->stream>>>(faces, s->bcell.as<int64_t>(), s->bemis.d(),
                                                       s->bkcond.d(), s->nboundary, tin, s->shed.d(),
                                                       s->nx, s->ny, s->nz, st);
-        heat_step_ext<<<EXT_BLOCKS, EXT_THREADS, 0, s->stream>>>(tin, tout, kc, s->kface, mob, s->mob0, cap,
+        heat_step_ext<<<sweep_blocks, EXT_THREADS, 0, s->stream>>>(tin, tout, kc, s->kface, mob, s->mob0, cap,
                                                          s->cap0, xc, dt, s->nx, s->ny, s->nz,
                                                          s->ext_part.d(), ph, st);
         if (s->ngcell)
@@ -2793,7 +2824,7 @@ static int advance_ext_device(pm_solid3d *s, double dt, uint64_t steps, double t
                 tout, ph, st);
         heat_stored_kernel<<<(unsigned)nstored, 256, 0, s->stream>>>(tout, cap, s->cap0, s->reference, n,
                                                                      per_stored, w.partials, ph);
-        ext_books<<<1, 1024, 0, s->stream>>>(s->ext_part.d(), EXT_BLOCKS, dsup.d(), w.partials, nstored,
+        ext_books<<<1, 1024, 0, s->stream>>>(s->ext_part.d(), sweep_blocks, dsup.d(), w.partials, nstored,
                                               s->ext_books.d(), st, (int64_t)q, dtot.d());
         PM_HIP(hipGetLastError());
     }

@@ -864,6 +864,13 @@ __device__ __forceinline__ double add_kelvin(const PhaseArgs &ph, int64_t c, dou
     return point;
 }
 
+// add_kelvin with the capacity read only where the cell melts (cap null: every cell holds cap0).
+__device__ __forceinline__ double add_kelvin_lazy(const PhaseArgs &ph, int64_t c, double t, double rise,
+                                                  const double *__restrict__ cap, double cap0) {
+    if (!ph.melted || !(ph.latent[c] > 0.0)) return t + rise;
+    return add_kelvin(ph, c, t, rise, cap ? cap[c] : cap0);
+}
+
 // The bus share with phases (and void): a uniform rise through add_kelvin, solid.rs:1752-1765.
 __global__ void add_uniform_phase(double *p, const uint8_t *vd, int64_t n, double rise,
                                   const double *cap, PhaseArgs ph) {
@@ -1621,12 +1628,15 @@ __global__ __launch_bounds__(EXT_THREADS) void heat_step_ext(const double *__res
                        ((j + 1 == ny) && (x.faces & 8)) || ((k == 0) && (x.faces & 16)) || ((k + 1 == nz) && (x.faces & 32))))
             sh = x.shed[c];
         const double dtm = dts * (mob ? mob[c] : mob0);
-        const double cc = cap ? cap[c] : cap0;
-        double out = add_kelvin(ph, c, t, dtm * (flux - sh), cc);
-        if (sh != 0.0) lost += dtm * sh * cc;
+        // The capacity is read only where it enters: a melting cell, the film's loss, a source.
+        double out = add_kelvin_lazy(ph, c, t, dtm * (flux - sh), cap, cap0);
+        if (sh != 0.0) lost += dtm * sh * (cap ? cap[c] : cap0);
         if (x.src_of) {
             const int32_t q = x.src_of[c];
-            if (q >= 0) out = add_kelvin(ph, c, out, (x.src_w[q] * dts) / cc, cc);
+            if (q >= 0) {
+                const double cc = cap ? cap[c] : cap0;
+                out = add_kelvin(ph, c, out, (x.src_w[q] * dts) / cc, cc);
+            }
         }
         tnew[c] = out;
     }

@@ -478,10 +478,19 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
     // The chunk reads planes z0-2 .. z1+1; later planes are clamped to what it already reads.
     const int top = std::min(z1 + 1, nz - 1);
     // Plane accesses go through a buffer resource built on the scalar unit from the plane's
-    // address, with the 32-bit lane offset as the only per-lane operand.
+    // address, with the 32-bit lane offset as the only per-lane operand (mobility and capacity).
     auto ld = [&](const double *p, int q, uint32_t off) -> double {
         return buffer_load_f64(p + plane * std::min(std::max(q, 0), top), off);
     };
+    // T and k are addressed from the chunk's lowest plane through a resource built once, a plane
+    // being a uniform byte offset (planes zlo .. top, fewer than 2^32 bytes: the host caps the
+    // chunk); the output likewise from plane z0.
+    const int zlo = std::max(z0 - 2, 0);
+    const __amdgpu_buffer_rsrc_t RTs = buffer_rsrc(T + plane * zlo), RKs = buffer_rsrc(K + plane * zlo);
+    const uint32_t pbytes = (uint32_t)plane * 8u;
+    auto so = [&](int q) -> uint32_t { return (uint32_t)(std::min(std::max(q, 0), top) - zlo) * pbytes; };
+    auto ldT = [&](int q, uint32_t off) -> double { return buffer_load_f64(RTs, off, so(q)); };
+    auto ldK = [&](int q, uint32_t off) -> double { return buffer_load_f64(RKs, off, so(q)); };
     auto zon = [&](int q) -> bool { return q > 0 && q < nz; };  // face between q-1 and q
     // One explicit update (solid.rs:1781-1826 order) from the in-plane neighbours in `t` around
     // position p (row stride `stride`), the four in-plane faces and the own column.
@@ -512,15 +521,15 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
     // Ring-1 column: T0, k and z faces.  A ring-2 thread, never a ring-1 thread, keeps the T0(z)
     // and k(z+1) of its next staging in QT[0..1] and QK[0..1] (indexed by plane parity).
     double QT[4], QK[4], QZ[4];
-    RT[3] = ld(T, zs - 1, ub);
-    RT[0] = ld(T, zs, ub);
-    RT[1] = ld(T, zs + 1, ub);
-    RT[2] = ld(T, zs + 2, ub);
-    const double k_m = ld(K, zs - 1, ub);
-    RK[0] = ld(K, zs, ub);
-    RK[1] = ld(K, zs + 1, ub);
-    RK[2] = ld(K, zs + 2, ub);
-    RK[3] = ld(K, zs + 3, ub);
+    RT[3] = ldT(zs - 1, ub);
+    RT[0] = ldT(zs, ub);
+    RT[1] = ldT(zs + 1, ub);
+    RT[2] = ldT(zs + 2, ub);
+    const double k_m = ldK(zs - 1, ub);
+    RK[0] = ldK(zs, ub);
+    RK[1] = ldK(zs + 1, ub);
+    RK[2] = ldK(zs + 2, ub);
+    RK[3] = ldK(zs + 3, ub);
     {
         const double lo = series_nz(k_m, RK[0]), hi = series_nz(RK[0], RK[1]);
         KZ[0] = zon(zs) ? lo : 0.0;
@@ -541,25 +550,25 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
         QM[0] = ld(a.mob, zs, r1b), QM[1] = ld(a.mob, zs + 1, r1b);
     }
     if (ring1) {
-        QT[3] = ld(T, zs - 1, r1b);
-        QT[0] = ld(T, zs, r1b);
-        QT[1] = ld(T, zs + 1, r1b);
-        QT[2] = ld(T, zs + 2, r1b);
-        const double q_m = ld(K, zs - 1, r1b);
-        QK[0] = ld(K, zs, r1b);
-        QK[1] = ld(K, zs + 1, r1b);
-        QK[2] = ld(K, zs + 2, r1b);
-        QK[3] = ld(K, zs + 3, r1b);
+        QT[3] = ldT(zs - 1, r1b);
+        QT[0] = ldT(zs, r1b);
+        QT[1] = ldT(zs + 1, r1b);
+        QT[2] = ldT(zs + 2, r1b);
+        const double q_m = ldK(zs - 1, r1b);
+        QK[0] = ldK(zs, r1b);
+        QK[1] = ldK(zs + 1, r1b);
+        QK[2] = ldK(zs + 2, r1b);
+        QK[3] = ldK(zs + 3, r1b);
         const double lo = series_nz(q_m, QK[0]), hi = series_nz(QK[0], QK[1]);
         QZ[0] = zon(zs) ? lo : 0.0;
         QZ[1] = zon(zs + 1) ? hi : 0.0;
     }
     if (ring2) {
-        PT[0] = ld(T, zs, r2b);
-        PT[1] = ld(T, zs + 1, r2b);
-        hk0 = ld(K, zs, r2b);
-        PK[0] = ld(K, zs + 1, r2b);
-        PK[1] = ld(K, zs + 2, r2b);
+        PT[0] = ldT(zs, r2b);
+        PT[1] = ldT(zs + 1, r2b);
+        hk0 = ldK(zs, r2b);
+        PK[0] = ldK(zs + 1, r2b);
+        PK[1] = ldK(zs + 2, r2b);
     }
     // The faces of plane zs, from k(zs) staged in the second buffer.
     sK[1][ty + 2][tx + 2] = RK[0];
@@ -577,7 +586,7 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
         }
     }
 
-    double *__restrict__ Tout = a.tnew;
+    const __amdgpu_buffer_rsrc_t RW = buffer_rsrc(a.tnew + plane * z0);
     double acc1 = 0.0, acc2 = 0.0;
     // Plane z's faces are in buffer (z - zs) mod 4; planes z-1, z and z+1 are live at once.
     for (int zb = zs; zb <= z1; zb += 4) {
@@ -587,11 +596,8 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
             const bool live = z <= z1;
             const int b = i & 1;
             const int fb = i, fbn = (i + 1) & 3, fbp = (i + 3) & 3;
-            // The planes this iteration fetches, computed once on the scalar unit (z + 2 >= 1).
-            const double *T2 = T + plane * std::min(z + 2, top);
-            const double *T3 = T + plane * std::min(z + 3, top);
-            const double *K3 = K + plane * std::min(z + 3, top);
-            const double *K4 = K + plane * std::min(z + 4, top);
+            // The planes this iteration fetches, as offsets computed once on the scalar unit.
+            const uint32_t T2 = so(z + 2), T3 = so(z + 3), K3 = so(z + 3), K4 = so(z + 4);
 
             // Stage T0(z) and k(z+1) on E2.
             sT[b][ty + 2][tx + 2] = RT[i];
@@ -604,8 +610,8 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
                 sT[b][r2r][r2c] = PT[b];
                 sK[b][r2r][r2c] = PK[b];
             }
-            PT[b] = buffer_load_f64(T2, r2b);
-            PK[b] = buffer_load_f64(K3, r2b);
+            PT[b] = buffer_load_f64(RTs, r2b, T2);
+            PK[b] = buffer_load_f64(RKs, r2b, K3);
             lds_barrier();
 
             // Four rounds of divisions: the tile column's z face between planes z+1 and z+2, two
@@ -629,8 +635,8 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
                 if (third) f[fdst(d)] = v;
                 if (ring1) QZ[(i + 2) & 3] = v;
             }
-            RK[i] = buffer_load_f64(K4, ub);  // k(z) is no longer needed
-            QK[i] = buffer_load_f64(K4, r1b);
+            RK[i] = buffer_load_f64(RKs, ub, K4);  // k(z) is no longer needed
+            QK[i] = buffer_load_f64(RKs, r1b, K4);
 
             // First step: T1(z) on E1 (its corners aside).
             const double *f = sF[fb];
@@ -644,8 +650,8 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
                                          f[r1fx + 1], f[r1fy], f[r1fy + E1X], QT[(i + 3) & 3],
                                          QT[i], QT[(i + 1) & 3], QZ[i], QZ[(i + 1) & 3],
                                          rate_of(QM[b]));
-            RT[(i + 3) & 3] = buffer_load_f64(T3, ub);  // T0(z-1) is no longer needed
-            QT[(i + 3) & 3] = buffer_load_f64(T3, r1b);
+            RT[(i + 3) & 3] = buffer_load_f64(RTs, ub, T3);  // T0(z-1) is no longer needed
+            QT[(i + 3) & 3] = buffer_load_f64(RTs, r1b, T3);
             if constexpr (CC) QM[b] = ld(a.mob, z + 2, r1b);
             if (inside && z >= z0 && z < z1) acc1 += (CC ? RC[b] : a.cap0) * (t1 - a.tref);
 
@@ -662,7 +668,7 @@ __global__ __launch_bounds__(64 * TY, 4) void heat_sweep2(HeatArgs a) {
                 RC[b ^ 1] = ld(a.cap, z + 1, ub);
             }
             if (inside && live && z - 1 >= z0) {
-                buffer_store_f64_nt(t2, Tout + plane * (z - 1), ub);
+                buffer_store_f64_nt(t2, RW, ub, (uint32_t)(z - 1 - z0) * pbytes);
                 acc2 += cap_m * (t2 - a.tref);
             }
         }
@@ -1028,17 +1034,29 @@ namespace pm {
 // possible (at most 128 planes, so the two planes of halo a chunk re-reads stay a small share)
 // while the launch still has about 1024 workgroups, four per compute unit; short chunks (at least
 // 4 planes) are what fills the chip on small blocks.
-static int z_chunk_of(const pm_solid3d *s, int tile_y) {
-    if (s->z_chunk > 0) return s->z_chunk;
-    const int64_t xy = ((s->nx + 63) / 64) * ((s->ny + tile_y - 1) / tile_y);
-    const int64_t chunks = std::max<int64_t>(1, (1024 + xy - 1) / xy);
-    const int64_t zc = (s->nz + chunks - 1) / chunks;
-    return (int)std::min<int64_t>(128, std::max<int64_t>(4, zc));
+// The paired sweep addresses T, k and its output from its chunk's lowest plane with 32-bit byte
+// offsets: a chunk of zc planes reaches zc + 4 planes of an array, which must span fewer than
+// 2^32 bytes (blocks whose planes are too large for one such chunk take the one-step sweep).
+static int64_t pair_max_chunk(const pm_solid3d *s) {
+    return (int64_t)(((uint64_t)1 << 32) - 1) / (s->nx * s->ny * 8) - 4;
 }
 
-static int heat_grid(const pm_solid3d *s, dim3 *grid, dim3 *block, int tile_y = 0) {
+static int z_chunk_of(const pm_solid3d *s, int tile_y, bool paired = false) {
+    int64_t zc;
+    if (s->z_chunk > 0) {
+        zc = s->z_chunk;
+    } else {
+        const int64_t xy = ((s->nx + 63) / 64) * ((s->ny + tile_y - 1) / tile_y);
+        const int64_t chunks = std::max<int64_t>(1, (1024 + xy - 1) / xy);
+        zc = std::min<int64_t>(128, std::max<int64_t>(4, (s->nz + chunks - 1) / chunks));
+    }
+    if (paired) zc = std::max<int64_t>(1, std::min(zc, pair_max_chunk(s)));
+    return (int)zc;
+}
+
+static int heat_grid(const pm_solid3d *s, dim3 *grid, dim3 *block, int tile_y = 0, bool paired = false) {
     const int ty = tile_y ? tile_y : s->tile_y;
-    const int zc = z_chunk_of(s, ty);
+    const int zc = z_chunk_of(s, ty, paired);
     *block = dim3(64, (unsigned)ty, 1);
     *grid = dim3((unsigned)((s->nx + 63) / 64), (unsigned)((s->ny + ty - 1) / ty),
                  (unsigned)((s->nz + zc - 1) / zc));
@@ -1049,7 +1067,7 @@ static int heat_grid(const pm_solid3d *s, dim3 *grid, dim3 *block, int tile_y = 
 // in 64 x 8 tiles.
 constexpr int PAIR_TY = 8;
 static bool pairs_enabled(const pm_solid3d *s) {
-    return s->steps_per_pass == 2 && s->kmode == K_PER_CELL;
+    return s->steps_per_pass == 2 && s->kmode == K_PER_CELL && pair_max_chunk(s) >= 1;
 }
 
 // Cells per partial of the stored-heat reduction: about 1024 partials, between 1024 and 16384
@@ -1067,7 +1085,7 @@ static int64_t stored_per_block(int64_t n) {
 static int64_t partials_needed(const pm_solid3d *s) {
     dim3 g, b, g2;
     heat_grid(s, &g, &b);
-    heat_grid(s, &g2, &b, PAIR_TY);
+    heat_grid(s, &g2, &b, PAIR_TY, true);
     const int64_t sweep = std::max<int64_t>((int64_t)g.x * g.y * g.z, 2 * (int64_t)g2.x * g2.y * g2.z);
     const int64_t per = stored_per_block(s->n());
     return std::max<int64_t>(sweep, (s->n() + per - 1) / per);
@@ -1195,7 +1213,7 @@ static int check_k_range(pm_solid3d *s, int *flag) {
 // Steps `step` and `step + 1` in one pass from t[cur] into t[cur^1], then both audits; swaps.
 static int sweep_pair(pm_solid3d *s, double dt, int64_t step, const RunSpace &w, int64_t launch) {
     dim3 g, b;
-    heat_grid(s, &g, &b, PAIR_TY);
+    heat_grid(s, &g, &b, PAIR_TY, true);
     HeatArgs a{};
     a.told = s->t[s->cur].d();
     a.tnew = s->t[s->cur ^ 1].d();
@@ -1208,7 +1226,7 @@ static int sweep_pair(pm_solid3d *s, double dt, int64_t step, const RunSpace &w,
     a.dts = dt;
     a.cap0 = s->cap0;
     a.tref = s->reference;
-    a.nx = (int)s->nx, a.ny = (int)s->ny, a.nz = (int)s->nz, a.zc = z_chunk_of(s, PAIR_TY);
+    a.nx = (int)s->nx, a.ny = (int)s->ny, a.nz = (int)s->nz, a.zc = z_chunk_of(s, PAIR_TY, true);
     a.gx = (int)g.x, a.gy = (int)g.y, a.gz = (int)g.z;
     const int64_t ntiles = (int64_t)g.x * g.y * g.z;
     const bool prof = prof_slot(s, launch);

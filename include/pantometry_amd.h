@@ -443,6 +443,7 @@ typedef struct pm_member {
 
 typedef struct pm_ensemble pm_ensemble;
 
+/* Each member holds fewer than 2^29 cells (PM_ERR_ARGUMENT otherwise). */
 int pm_ensemble_create(const pm_member *members, size_t count, pm_ensemble **out);
 void pm_ensemble_destroy(pm_ensemble *ens);
 /* Member cells in, out (kelvin, x-fastest). */

@@ -126,8 +126,10 @@ __global__ __launch_bounds__(ENS_NT, ENS_WAVES) void ens_tile(const Member *__re
     const int w0 = lx1 - lx0, n0 = w0 * (ly1 - ly0);
     const int t = threadIdx.x;
     const double kf = m.kface, f = m.f;
-    const double *T = told + m.off;
-    double *Tn = tnew + m.off;
+    // The member's T and T' through buffer resources: the lane's in-plane byte offset plus the
+    // plane as a uniform offset (members hold fewer than 2^29 cells, so both fit 32 bits).
+    const __amdgpu_buffer_rsrc_t RT = buffer_rsrc(told + m.off), RN = buffer_rsrc(tnew + m.off);
+    const uint32_t pb = (uint32_t)plane * 8u;
 
     // This thread's cells: in-plane offset (-1 for none) and role bits.
     int g[ENS_R];
@@ -156,7 +158,9 @@ __global__ __launch_bounds__(ENS_NT, ENS_WAVES) void ens_tile(const Member *__re
     for (int u = 0; u < D; u++)
 #pragma unroll
         for (int r = 0; r < ENS_R; r++)
-            pre[u][r] = (lo_load + u <= hi_load && g[r] >= 0) ? T[(lo_load + u) * plane + g[r]] : 0.0;
+            pre[u][r] = (lo_load + u <= hi_load && g[r] >= 0)
+                            ? buffer_load_f64(RT, (uint32_t)g[r] * 8u, (uint32_t)(lo_load + u) * pb)
+                            : 0.0;
     double zm[ENS_R] = {}, zc[ENS_R] = {}, zp[ENS_R] = {};  // T(i-2), T(i-1), T(i)
     double ym[ENS_R] = {}, yc[ENS_R] = {}, yp[ENS_R] = {};  // T1(i-3), T1(i-2), T1(i-1)
 
@@ -170,7 +174,7 @@ __global__ __launch_bounds__(ENS_NT, ENS_WAVES) void ens_tile(const Member *__re
             if (i + D <= hi_load) {
 #pragma unroll
                 for (int r = 0; r < ENS_R; r++)
-                    if (g[r] >= 0) reg[r] = T[(i + D) * plane + g[r]];
+                    if (g[r] >= 0) reg[r] = buffer_load_f64(RT, (uint32_t)g[r] * 8u, (uint32_t)(i + D) * pb);
             }
         }
         const int p = i - 1;
@@ -187,7 +191,7 @@ __global__ __launch_bounds__(ENS_NT, ENS_WAVES) void ens_tile(const Member *__re
                     s1[p & 1][t + r * ENS_NT] = v;
                     ym[r] = yc[r], yc[r] = yp[r], yp[r] = v;
                 } else {
-                    __builtin_nontemporal_store(v, Tn + p * plane + g[r]);
+                    buffer_store_f64_nt(v, RN, (uint32_t)g[r] * 8u, (uint32_t)p * pb);
                 }
             }
         } else if constexpr (S == 2) {
@@ -205,7 +209,7 @@ __global__ __launch_bounds__(ENS_NT, ENS_WAVES) void ens_tile(const Member *__re
                     const double v = staged_cell(pc, t + r * ENS_NT, w0, info[r], yc[r],
                                                  p2 > 0 ? ym[r] : yc[r],
                                                  p2 + 1 < nz ? yp[r] : yc[r], kf, kzl, kzh, f);
-                    __builtin_nontemporal_store(v, Tn + p2 * plane + g[r]);
+                    buffer_store_f64_nt(v, RN, (uint32_t)g[r] * 8u, (uint32_t)p2 * pb);
                 }
             }
         }
@@ -503,8 +507,8 @@ int pm_ensemble_create(const pm_member *members, size_t count, pm_ensemble **out
     for (size_t m = 0; m < count; m++) {
         pm_member &s = e->spec[m];
         s.nx = std::max<size_t>(s.nx, 1), s.ny = std::max<size_t>(s.ny, 1), s.nz = std::max<size_t>(s.nz, 1);
-        if (s.nx * s.ny > (1u << 30) || s.nz > (1u << 30) || s.nx * s.ny * s.nz > ((size_t)1 << 40)) {
-            set_error("pm_ensemble_create: member %zu is too large", m);
+        if (s.nx * s.ny > (1u << 30) || s.nz > (1u << 30) || s.nx * s.ny * s.nz >= ((size_t)1 << 29)) {
+            set_error("pm_ensemble_create: member %zu is too large (2^29 cells or more)", m);
             return PM_ERR_ARGUMENT;
         }
         Member d;

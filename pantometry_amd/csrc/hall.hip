@@ -52,6 +52,7 @@ struct HallArgs {
     double source2;
     double *__restrict__ supplied_step2;
     int force;  // run even if an audit breach froze the run
+    int64_t strip_tiles;  // the paired step's last-column strips (hall_strip2), 0 without
 };
 
 __device__ __forceinline__ double wall(int64_t k, int64_t n) { return (k == 0 || k + 1 == n) ? 2.0 : 1.0; }
@@ -193,6 +194,13 @@ __global__ __launch_bounds__(64 * TY) void hall_march(HallArgs a, int gx, int gy
     if (tid == 0) a.partials[tile] = total;
 }
 
+// The LDS a TX x TY tile of the paired step uses, in doubles; the same for its transpose.
+template <int TX, int TY>
+constexpr int pair_lds_doubles() {
+    return 2 * ((TY + 4) * (TX + 4) + (TY + 2) * (TX + 3) + (TY + 3) * (TX + 2) + 3 * (TY + 2) * (TX + 2)) +
+           TX * TY / 64;
+}
+
 // Two steps per pass (temporal blocking) of the marching step, on TX x TY column tiles (TX 64,
 // 56 or 48; 64 unless set).  Per plane k the workgroup
 // computes the first step on the tile widened by one node (E1, (TX+2) x (TY+2), its four corners
@@ -205,8 +213,13 @@ __global__ __launch_bounds__(64 * TY) void hall_march(HallArgs a, int gx, int gy
 // vx, vy, vz once and writes p'' (the new pressure), p' (its pressure_prev) and the v'' faces:
 // 72 B per two node-steps instead of 128.  The first step's audit sums the owned nodes' p'p and
 // the owned faces' v'^2 as they are produced, the second step's those of p''p' and v''^2.
-template <int TX, int TY>
-__global__ __launch_bounds__(TX * TY, 4) void hall_march2(HallArgs a, int gx, int gy, int gz, int zc) {
+//
+// This is one tile of it: nodes x0 .. x0+TX-1, y0 .. y0+TY-1 of planes z0 .. z0+zc-1, all of
+// which it stores and audits, or (LAST) only those of the hall's last column.  Its audit sums go
+// to partials[tile] and partials[ntiles + tile].  `lds` holds pair_lds_doubles<TX, TY>().
+template <int TX, int TY, bool LAST>
+__device__ __forceinline__ void pair_tile(const HallArgs &a, double *lds, int tx, int ty, int64_t tile,
+                                          int64_t ntiles, int x0, int y0, int z0, int zc) {
     constexpr int NT = TX * TY;
     static_assert(NT % 64 == 0, "whole waves");
     constexpr int E1X = TX + 2, E1Y = TY + 2, E2X = TX + 4, E2Y = TY + 4;
@@ -214,30 +227,25 @@ __global__ __launch_bounds__(TX * TY, 4) void hall_march2(HallArgs a, int gx, in
     constexpr int R2 = 2 * (E2X - 4) + 2 * (E2Y - 4) + 4;  // E2 ring nodes reached, E1 corners
     constexpr int RING1_FIRST = NT - R1;
     static_assert(R1 <= NT && R2 <= RING1_FIRST, "roles do not fit");
-    __shared__ double sP[2][E2Y][E2X];      // p(k) on E2
-    __shared__ double sVX[2][E1Y][E2X - 1];  // vx(k) of E1 rows: face c between E2 columns c, c+1
-    __shared__ double sVY[2][E2Y - 1][E1X];  // vy(k) of E1 columns: face r between E2 rows r, r+1
-    __shared__ double sQ[2][E1Y][E1X];      // p'(k) on E1
-    __shared__ double sHX[2][E1Y][E1X];     // v'x(k) of each E1 node's +x face
-    __shared__ double sHY[2][E1Y][E1X];     // v'y(k) of each E1 node's +y face
-    __shared__ double red[NT / 64];
+    // p(k) on E2; vx(k) of E1 rows (face c between E2 columns c, c+1); vy(k) of E1 columns (face
+    // r between E2 rows r, r+1); p'(k) on E1 with the v'x(k), v'y(k) of each E1 node's +x, +y
+    // faces; and one double per wave for the audit's sum.  Each is double-buffered by plane.
+    double(*sP)[E2Y][E2X] = reinterpret_cast<double(*)[E2Y][E2X]>(lds);
+    double(*sVX)[E1Y][E2X - 1] = reinterpret_cast<double(*)[E1Y][E2X - 1]>(sP + 2);
+    double(*sVY)[E2Y - 1][E1X] = reinterpret_cast<double(*)[E2Y - 1][E1X]>(sVX + 2);
+    double(*sQ)[E1Y][E1X] = reinterpret_cast<double(*)[E1Y][E1X]>(sVY + 2);
+    double(*sHX)[E1Y][E1X] = sQ + 2;
+    double(*sHY)[E1Y][E1X] = sHX + 2;
+    double *red = &sHY[2][0][0];
 
-    const int64_t ntiles = (int64_t)gx * gy * gz;
-    const int64_t chunk = (ntiles + XCDS_HALL - 1) / XCDS_HALL;
-    const int64_t tile = (int64_t)(blockIdx.x % XCDS_HALL) * chunk + blockIdx.x / XCDS_HALL;
-    if (tile >= ntiles) return;
-    if (a.audit->halted && !a.force) return;  // a breach froze the run
-    const int64_t bx = tile % gx, by = (tile / gx) % gy, bz = tile / ((int64_t)gx * gy);
-
-    const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TX * ty;
+    const int tid = tx + TX * ty;
     const int nx = a.nx, ny = a.ny;
     const int64_t plane = (int64_t)nx * ny;
-    const int x0 = (int)bx * TX, y0 = (int)by * TY;
     // Plane indices are uniform 32-bit values, so their clamps and tests stay on the scalar unit.
     // The paired step is launched only for halls at least two nodes along every axis, so every
     // velocity array has faces.
     const int nz = a.nz;
-    const int z0 = (int)bz * zc, z1 = std::min(z0 + zc, nz);
+    const int z1 = std::min(z0 + zc, nz);
     const int64_t vxp = (int64_t)(nx - 1) * ny, vyp = (int64_t)nx * (ny - 1);
     auto cx = [&](int x) { return std::min(std::max(x, 0), nx - 1); };
     auto cy = [&](int y) { return std::min(std::max(y, 0), ny - 1); };
@@ -269,7 +277,7 @@ __global__ __launch_bounds__(TX * TY, 4) void hall_march2(HallArgs a, int gx, in
     // (r1r, r1c).  The first R2 threads: an E2 node whose p (and, on the left column and bottom
     // row and at E1's corners, whose +x or +y face) the ring stencils read.
     const Col tc = column(x0 + tx, y0 + ty);
-    const bool inside = tc.x < nx && tc.y < ny;
+    const bool inside = (LAST ? tc.x == nx - 1 : tc.x < nx) && tc.y < ny;
     int r1r = -1, r1c = -1;
     {
         const int q = tid - RING1_FIRST;
@@ -473,6 +481,38 @@ __global__ __launch_bounds__(TX * TY, 4) void hall_march2(HallArgs a, int gx, in
         a.partials[tile] = total1;
         a.partials[ntiles + tile] = total2;
     }
+}
+
+// The paired step's launch: gx x gy x gz tiles of TX x TY columns.
+template <int TX, int TY>
+__global__ __launch_bounds__(TX * TY, 4) void hall_march2(HallArgs a, int gx, int gy, int gz, int zc) {
+    __shared__ double lds[pair_lds_doubles<TX, TY>()];
+    const int64_t ntiles = (int64_t)gx * gy * gz;
+    const int64_t chunk = (ntiles + XCDS_HALL - 1) / XCDS_HALL;
+    const int64_t tile = (int64_t)(blockIdx.x % XCDS_HALL) * chunk + blockIdx.x / XCDS_HALL;
+    if (tile >= ntiles) return;
+    if (a.audit->halted && !a.force) return;  // a breach froze the run
+    const int64_t bx = tile % gx, by = (tile / gx) % gy, bz = tile / ((int64_t)gx * gy);
+    pair_tile<TX, TY, false>(a, lds, threadIdx.x, threadIdx.y, tile, ntiles + a.strip_tiles,
+                             (int)bx * TX, (int)by * TY, (int)bz * zc, zc);
+}
+
+// The last column of a hall gx * TX + 1 nodes across, beside hall_march2's gx x-tiles (a
+// seventh x-tile at 385 = 6 x 64 + 1 nodes would march every plane for one live column): strips
+// of the transposed tile, TY wide at x0 = nx - TY and TX tall, that own only that column, over
+// short z-chunks so the launch is one round of short-lived workgroups.  Their audit sums follow
+// the tiles' (partials from `first`).
+template <int TX, int TY>
+__global__ __launch_bounds__(TX * TY, 4) void hall_strip2(HallArgs a, int gs, int gz, int zc, int64_t first) {
+    static_assert(pair_lds_doubles<TX, TY>() == pair_lds_doubles<TY, TX>(), "the tile's LDS");
+    __shared__ double lds[pair_lds_doubles<TX, TY>()];
+    const int64_t strip = blockIdx.x;
+    if (strip >= (int64_t)gs * gz) return;
+    if (a.audit->halted && !a.force) return;  // a breach froze the run
+    const int sy = (int)(strip % gs), bz = (int)(strip / gs);
+    const int tid = threadIdx.x + TX * threadIdx.y;
+    pair_tile<TY, TX, true>(a, lds, tid % TY, tid / TY, first + strip, first + a.strip_tiles,
+                            a.nx - TY, sy * TX, bz * zc, zc);
 }
 
 // The audits of the two steps of a pass, in order: supplied energy and ledger of the first step,
@@ -726,10 +766,13 @@ struct pm_hall {
 namespace pm {
 
 // Tiles of the marching step: tile_x x tile_y columns and, unless fixed, z-chunks of at most 128
-// planes, short enough that a launch has at least 2560 workgroups, at least 4 planes.
+// planes, short enough that a launch has the workgroups hall_grid asks for, at least 4 planes.
 struct HallGrid {
     int gx, gy, gz, zc;
+    // The paired step's last-column strips (hall_strip2): gs along y by gzs z-chunks of zcs.
+    int gs = 0, gzs = 0, zcs = 0;
     int64_t tiles() const { return (int64_t)gx * gy * gz; }
+    int64_t strips() const { return (int64_t)gs * gzs; }
 };
 
 // The paired step's tile width: the hall's own setting, else 64.  Narrower tiles leave fewer
@@ -744,20 +787,37 @@ static int64_t hall_pair_max_chunk(const pm_hall *h) {
     return (int64_t)(((uint64_t)1 << 32) - 1) / (h->nx * h->ny * 8) - 4;
 }
 
-static HallGrid hall_grid(const pm_hall *h, int tile_y = 0, int tile_x = 64, int64_t max_zc = 0) {
+// Tiles of a step.  For the paired step (`paired`), a hall of gx * tile_x + 1 nodes along x
+// takes gx x-tiles and strips for its last column (hall_strip2) instead of gx + 1 x-tiles.
+static HallGrid hall_grid(const pm_hall *h, int tile_y = 0, int tile_x = 64, int64_t max_zc = 0,
+                          bool paired = false) {
     const int ty = tile_y ? tile_y : h->tile_y;
     HallGrid g;
     g.gx = (int)((h->nx + tile_x - 1) / tile_x);
+    const bool last = paired && h->nx > tile_x && h->nx % tile_x == 1;
+    if (last) g.gx = (int)(h->nx / tile_x);
     g.gy = (int)((h->ny + ty - 1) / ty);
     const int64_t xy = (int64_t)g.gx * g.gy;
-    // At least 2560 workgroups (five waves of two workgroups per CU over 256 CUs), so the last,
-    // partly filled wave of workgroups is a small part of the launch (at 385^3: z-chunks of 49,
-    // 4% faster than 97).
-    const int64_t chunks = std::max<int64_t>((2560 + xy - 1) / xy, (h->nz + 127) / 128);
+    // At least `least` workgroups, so the last, partly filled round of workgroups is a small part
+    // of the launch.  The single step: 2560, five rounds of two workgroups per CU over 256 CUs
+    // (at 385^3 z-chunks of 49, 4% faster than 97).  The paired step: 1536, three rounds (at
+    // 385^3 with the last column's strips, z-chunks of 65: 0.618 ms a step against 0.627-0.633 at
+    // 49 and 77, 0.647 at 97 and 0.665 at 129).
+    const int64_t least = paired ? 1536 : 2560;
+    const int64_t chunks = std::max<int64_t>((least + xy - 1) / xy, (h->nz + 127) / 128);
     g.zc = h->z_chunk > 0 ? h->z_chunk
                           : (int)std::max<int64_t>(4, (h->nz + chunks - 1) / chunks);
     if (max_zc > 0 && g.zc > max_zc) g.zc = (int)max_zc;
     g.gz = (int)((h->nz + g.zc - 1) / g.zc);
+    if (last) {
+        // z-chunks short enough for one round of two strips per CU (at 385^3: 7 x 65 strips of
+        // 6 planes)
+        g.gs = (int)((h->ny + tile_x - 1) / tile_x);
+        const int64_t want = std::max<int64_t>(1, 512 / g.gs);
+        g.zcs = (int)std::max<int64_t>(4, (h->nz + want - 1) / want);
+        if (max_zc > 0 && g.zcs > max_zc) g.zcs = (int)max_zc;
+        g.gzs = (int)((h->nz + g.zcs - 1) / g.zcs);
+    }
     return g;
 }
 
@@ -908,10 +968,11 @@ static int hall_sweep(pm_hall *h, double hstep, double vh, int64_t step, double 
 // Steps `step` and `step + 1` (both full leapfrog steps) in one pass, then both audits.
 static int hall_sweep_pair(pm_hall *h, double hstep, int64_t step, double *totals_dev, double offset) {
     const int ty = h->tile_y == 16 ? 16 : 8, tx = hall_tile_x(h);
-    const HallGrid g = hall_grid(h, ty, tx, hall_pair_max_chunk(h));
-    const int64_t nparts = g.tiles();
+    const HallGrid g = hall_grid(h, ty, tx, hall_pair_max_chunk(h), true);
+    const int64_t nparts = g.tiles() + g.strips();
     PM_CHECK(hall_partials(h, 2 * nparts));
     HallArgs a = hall_args(h, hstep, hstep);
+    a.strip_tiles = g.strips();
     const int v = h->iv, vo = h->iv ^ 1;
     a.p = h->p[h->ip].d(), a.vx = h->vx[v].d(), a.vy = h->vy[v].d(), a.vz = h->vz[v].d();
     a.p_out = h->p[h->iq].d(), a.p_mid = h->p[h->is].d();
@@ -924,9 +985,17 @@ static int hall_sweep_pair(pm_hall *h, double hstep, int64_t step, double *total
         hall_zero<<<1, 64, 0, h->stream>>>(h->scalars.d() + 2, 2);
         PM_HIP(hipGetLastError());
     }
-    const unsigned blocks = (unsigned)(XCDS_HALL * ((nparts + XCDS_HALL - 1) / XCDS_HALL));
+    const unsigned blocks = (unsigned)(XCDS_HALL * ((g.tiles() + XCDS_HALL - 1) / XCDS_HALL));
+    const unsigned sblocks = (unsigned)g.strips();
     const dim3 wg(tx, ty);
-#define PM_HALL_PAIR(TX, TY) hall_march2<TX, TY><<<blocks, wg, 0, h->stream>>>(a, g.gx, g.gy, g.gz, g.zc)
+    // The strips run after the tiles on the same stream: at 385^3 the tiles take 1151 us a pass and
+    // the strips 69 us.  Forked onto a second stream they ran beside the tiles from the start and
+    // stretched the pair to 1270 us.
+#define PM_HALL_PAIR(TX, TY)                                                                                \
+    do {                                                                                                    \
+        hall_march2<TX, TY><<<blocks, wg, 0, h->stream>>>(a, g.gx, g.gy, g.gz, g.zc);                      \
+        if (sblocks) hall_strip2<TX, TY><<<sblocks, wg, 0, h->stream>>>(a, g.gs, g.gzs, g.zcs, g.tiles()); \
+    } while (0)
     if (ty == 16) {
         if (tx == 56) PM_HALL_PAIR(56, 16);
         else if (tx == 48) PM_HALL_PAIR(48, 16);

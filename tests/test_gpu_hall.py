@@ -204,6 +204,36 @@ def test_a_sourced_hall_is_bit_exact_and_books_its_source():
     assert abs(g.supplied() - o.supplied) <= 1e-12 * abs(o.supplied)
 
 
+@pytest.mark.parametrize("at", [(64, 7, 5), (64, 8, 0), (64, 18, 11)], ids=str)
+@pytest.mark.parametrize("tile_y", [8, 16])
+def test_a_source_in_the_last_column_of_a_wide_tile(at, tile_y):
+    """65 = 64 + 1 nodes across: the paired step's last 64-wide tile also owns column 64, whose
+    second step, stores and audit its ring threads take.  A source there (beside a tile row's
+    edge, on one, at the far corner) is booked by those threads in both steps of a pair."""
+    gpu_case(f"hall wide tile with a source at {at}, tile_y {tile_y}")
+    from pantometry_amd.workloads import ricker
+    g, o = _pair(0.64, 0.18, 0.11, 65, 343.0, 1.2)
+    assert o.nodes() == (65, 19, 12)
+    g.set_tile(tile_y, 0)
+    n = 65 * 19 * 12
+    g.released_from(np.zeros(n))
+    o.release(np.zeros(n))
+    dt = 0.5 * o.max_stable_dt()
+    f = 343.0 / (10.0 * o.dx)
+    inc = np.array([ricker((k + 1) * dt, f, 1.0 / f) for k in range(60)])
+    node = at[0] + 65 * (at[1] + 19 * at[2])
+    g.set_source(node, inc)
+    o.set_source(node, inc)
+    totals, _ = g.advance(dt, 75, tolerance=1e-12)
+    for k in range(75):
+        o.step(dt)
+        t_ref, s_ref = o.ledger()
+        assert abs(totals[k] - t_ref) <= 1e-12 * s_ref, k
+    _same_fields(g, o)
+    assert o.supplied > 0.0
+    assert abs(g.supplied() - o.supplied) <= 1e-12 * abs(o.supplied)
+
+
 def test_checkpoint_and_restore():
     gpu_case("hall checkpoint")
     import pantometry_amd as pm

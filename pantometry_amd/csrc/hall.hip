@@ -373,9 +373,15 @@ __device__ __forceinline__ void pair_tile(const HallArgs &a, double *lds, int tx
     const double iwi = inv_wall(tc.x, nx), iwj = inv_wall(tc.y, ny);
     const double wi = wall(tc.x, nx), wj = wall(tc.y, ny);
     // The invariant's terms (hall.rs:334-384) with its constant factors folded: volume / (2 rho
-    // c^2) and rho / 2 x volume, each rounded once on the host.  A term moves by an ulp or two
-    // from the reference's; the totals are held to 1e-12 of the ledger's scale, not bit for bit.
-    const double pot = a.pot, kin = a.kin;
+    // c^2) and rho / 2 x volume, each rounded once on the host, times the node's wall shares
+    // (exact powers of two): p'p and the +x, +y faces' v'^2 weighted by this lane's constants and
+    // then by the plane's share, the +z face's by its own, summed with fused multiply-adds.  A
+    // term moves by an ulp or two from the reference's; the totals are held to 1e-12 of the
+    // ledger's scale, not bit for bit.  A face the node does not own has v' = 0 (as has a
+    // node's +z face on the top plane), and its weight is 0 as well.
+    const double w_p = iwi * iwj * a.pot;
+    const double w_x = tc.x + 1 < nx ? iwj * a.kin : 0.0, w_y = tc.y + 1 < ny ? iwi * a.kin : 0.0;
+    const double w_z = iwi * iwj * a.kin;
     double acc1 = 0.0, acc2 = 0.0;
     for (int k = zs; k <= z1; k++) {
         const int b = (k - zs) & 1;
@@ -422,13 +428,12 @@ __device__ __forceinline__ void pair_tile(const HallArgs &a, double *lds, int tx
         }
         if (inside && k >= z0 && k < z1) {
             const double iwk = inv_wall(k, nz);
-            const double share = iwi * iwj * iwk;
-            if (tc.n == sn1 && k == sk1)
+            if (tc.n == sn1 && k == sk1) {
+                const double share = iwi * iwj * iwk;
                 *a.supplied_step = share * (a.source + a.source_prev) * tpc / a.two_rc2 * a.volume;
-            acc1 += share * t1.p * tpc * pot;
-            if (tc.x + 1 < nx) acc1 += iwj * iwk * kin * t1.hx * t1.hx;
-            if (tc.y + 1 < ny) acc1 += iwi * iwk * kin * t1.hy * t1.hy;
-            if (k + 1 < nz) acc1 += iwi * iwj * kin * t1.hz * t1.hz;
+            }
+            const double e = std::fma(w_y, t1.hy * t1.hy, std::fma(w_x, t1.hx * t1.hx, w_p * (t1.p * tpc)));
+            acc1 = std::fma(w_z, t1.hz * t1.hz, std::fma(iwk, e, acc1));
         }
 
         // Second step on plane k-1, from p' of planes k-2, k-1 (registers, LDS) and k, and the
@@ -450,19 +455,18 @@ __device__ __forceinline__ void pair_tile(const HallArgs &a, double *lds, int tx
             if (at_src) pn = pn + a.source2;
             if (inside && k2 >= z0 && k2 < z1) {
                 const double iwk = inv_wall(k2, nz);
-                const double share = iwi * iwj * iwk;
-                if (at_src)
+                if (at_src) {
+                    const double share = iwi * iwj * iwk;
                     *a.supplied_step2 = share * (a.source2 + a.source) * pc / a.two_rc2 * a.volume;
+                }
                 const uint32_t o = (uint32_t)(k2 - z0);
                 buffer_store_f64_nt(pn, WP, tc.p, o * pb);
                 buffer_store_f64_nt(pc, WM, tc.p, o * pb);
                 if (tc.x + 1 < nx) buffer_store_f64_nt(hx, WX, tc.vx, o * xb);
                 if (tc.y + 1 < ny) buffer_store_f64_nt(hy, WY, tc.vy, o * yb);
                 if (k2 + 1 < nz) buffer_store_f64_nt(hz, WZ, tc.p, o * pb);
-                acc2 += share * pn * pc * pot;
-                if (tc.x + 1 < nx) acc2 += iwj * iwk * kin * hx * hx;
-                if (tc.y + 1 < ny) acc2 += iwi * iwk * kin * hy * hy;
-                if (k2 + 1 < nz) acc2 += iwi * iwj * kin * hz * hz;
+                const double e = std::fma(w_y, hy * hy, std::fma(w_x, hx * hx, w_p * (pn * pc)));
+                acc2 = std::fma(w_z, hz * hz, std::fma(iwk, e, acc2));
             }
         }
         qm = qc, qc = t1.p;

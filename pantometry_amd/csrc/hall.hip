@@ -415,16 +415,52 @@ __device__ __forceinline__ void pair_tile(const HallArgs &a, double *lds, int tx
         }
         lds_barrier();
 
-        // First step on plane k.
+        // First step on plane k for the tile node and the ring node (every lane computes it, a
+        // lane outside the ring on its own node again), and the second step on plane k-1 from p'
+        // of planes k-2, k-1 (registers, LDS) and k and the v' faces of plane k-1 (in-plane ones
+        // staged by their nodes, z ones in registers): three independent chains in one block,
+        // which the compiler interleaves.  Then the writes, stores and audits.
         const Step t1 = first(b, ty + 1, tx + 1, tc, k, tpm, tpc, tpp, tzm, tzc);
+        const Step r1 = first(b, ring1 ? r1r : ty + 1, ring1 ? r1c : tx + 1, rc, k, rpm, rpc, rpp, rzm, rzc);
+        const int k2 = k - 1;
+        const double pc = qc;
+        double hx = 0.0, lx = 0.0, hy = 0.0, ly = 0.0, hz = 0.0, lz = 0.0;
+        {
+            const int r = ty + 1, c = tx + 1;
+            if (tc.x + 1 < nx) hx = sHX[b ^ 1][r][c] - g * (sQ[b ^ 1][r][c + 1] - pc);
+            if (tc.x > 0) lx = sHX[b ^ 1][r][c - 1] - g * (pc - sQ[b ^ 1][r][c - 1]);
+            if (tc.y + 1 < ny) hy = sHY[b ^ 1][r][c] - g * (sQ[b ^ 1][r + 1][c] - pc);
+            if (tc.y > 0) ly = sHY[b ^ 1][r - 1][c] - g * (pc - sQ[b ^ 1][r - 1][c]);
+            if (k2 + 1 < nz) hz = t1.lz - g * (t1.p - pc);
+            if (k2 > 0) lz = plz - g * (pc - qm);
+        }
+        const double wk2 = wall(k2, nz);
+        double pn = pc - sc * (wi * (hx - lx) + wj * (hy - ly) + wk2 * (hz - lz));
+        const bool at_src = tc.n == sn2 && k2 == sk2;
+        if (at_src) pn = pn + a.source2;
+
         sQ[b][ty + 1][tx + 1] = t1.p;
         sHX[b][ty + 1][tx + 1] = t1.hx;
         sHY[b][ty + 1][tx + 1] = t1.hy;
         if (ring1) {
-            const Step r1 = first(b, r1r, r1c, rc, k, rpm, rpc, rpp, rzm, rzc);
             sQ[b][r1r][r1c] = r1.p;
             sHX[b][r1r][r1c] = r1.hx;
             sHY[b][r1r][r1c] = r1.hy;
+        }
+        if (inside && k2 >= z0 && k2 < z1) {
+            const double iwk = inv_wall(k2, nz);
+            if (at_src) {
+                const double share = iwi * iwj * iwk;
+                *a.supplied_step2 = share * (a.source2 + a.source) * pc / a.two_rc2 * a.volume;
+            }
+            const uint32_t o = (uint32_t)(k2 - z0);
+            buffer_store_f64_nt(pn, WP, tc.p, o * pb);
+            buffer_store_f64_nt(pc, WM, tc.p, o * pb);
+            if (tc.x + 1 < nx) buffer_store_f64_nt(hx, WX, tc.vx, o * xb);
+            if (tc.y + 1 < ny) buffer_store_f64_nt(hy, WY, tc.vy, o * yb);
+            if (k2 + 1 < nz) buffer_store_f64_nt(hz, WZ, tc.p, o * pb);
+            const double e = std::fma(w_y, hy * hy, std::fma(w_x, hx * hx, w_p * (pn * pc)));
+            acc2 = std::fma(w_z, hz * hz, std::fma(iwk, e, acc2));
         }
         if (inside && k >= z0 && k < z1) {
             const double iwk = inv_wall(k, nz);
@@ -434,40 +470,6 @@ __device__ __forceinline__ void pair_tile(const HallArgs &a, double *lds, int tx
             }
             const double e = std::fma(w_y, t1.hy * t1.hy, std::fma(w_x, t1.hx * t1.hx, w_p * (t1.p * tpc)));
             acc1 = std::fma(w_z, t1.hz * t1.hz, std::fma(iwk, e, acc1));
-        }
-
-        // Second step on plane k-1, from p' of planes k-2, k-1 (registers, LDS) and k, and the
-        // v' faces of plane k-1: in-plane ones staged by their nodes, z ones in registers.
-        {
-            const int k2 = k - 1;
-            const double pc = qc;
-            const int r = ty + 1, c = tx + 1;
-            double hx = 0.0, lx = 0.0, hy = 0.0, ly = 0.0, hz = 0.0, lz = 0.0;
-            if (tc.x + 1 < nx) hx = sHX[b ^ 1][r][c] - g * (sQ[b ^ 1][r][c + 1] - pc);
-            if (tc.x > 0) lx = sHX[b ^ 1][r][c - 1] - g * (pc - sQ[b ^ 1][r][c - 1]);
-            if (tc.y + 1 < ny) hy = sHY[b ^ 1][r][c] - g * (sQ[b ^ 1][r + 1][c] - pc);
-            if (tc.y > 0) ly = sHY[b ^ 1][r - 1][c] - g * (pc - sQ[b ^ 1][r - 1][c]);
-            if (k2 + 1 < nz) hz = t1.lz - g * (t1.p - pc);
-            if (k2 > 0) lz = plz - g * (pc - qm);
-            const double wk = wall(k2, nz);
-            double pn = pc - sc * (wi * (hx - lx) + wj * (hy - ly) + wk * (hz - lz));
-            const bool at_src = tc.n == sn2 && k2 == sk2;
-            if (at_src) pn = pn + a.source2;
-            if (inside && k2 >= z0 && k2 < z1) {
-                const double iwk = inv_wall(k2, nz);
-                if (at_src) {
-                    const double share = iwi * iwj * iwk;
-                    *a.supplied_step2 = share * (a.source2 + a.source) * pc / a.two_rc2 * a.volume;
-                }
-                const uint32_t o = (uint32_t)(k2 - z0);
-                buffer_store_f64_nt(pn, WP, tc.p, o * pb);
-                buffer_store_f64_nt(pc, WM, tc.p, o * pb);
-                if (tc.x + 1 < nx) buffer_store_f64_nt(hx, WX, tc.vx, o * xb);
-                if (tc.y + 1 < ny) buffer_store_f64_nt(hy, WY, tc.vy, o * yb);
-                if (k2 + 1 < nz) buffer_store_f64_nt(hz, WZ, tc.p, o * pb);
-                const double e = std::fma(w_y, hy * hy, std::fma(w_x, hx * hx, w_p * (pn * pc)));
-                acc2 = std::fma(w_z, hz * hz, std::fma(iwk, e, acc2));
-            }
         }
         qm = qc, qc = t1.p;
         plz = t1.lz;

@@ -48,28 +48,35 @@ constexpr int K_STEPS = 16;  // stencil steps per surrogate step
 //          the first-order estimate of the target, face by face
 //   14     their sum
 //   15     1 (marks the volume: the convolutions pad with zeros)
+// The fourteen loads are issued together (a face off the block reads the cell itself and is
+// masked afterwards), and the arithmetic is fp32 but for T_nb - T, taken in the fields' own
+// precision: every feature is rounded to bf16, whose 2^-8 step is far above fp32's error.
 template <class TT>
 __device__ __forceinline__ void cell_features(const TT *__restrict__ t, const TT *__restrict__ k,
                                               int64_t base, int edge, int x, int y, int z,
                                               float coupling, bf16 *__restrict__ out) {
     const int64_t c = x + (int64_t)edge * (y + (int64_t)edge * z);
-    const double tc = (double)t[base + c], kc = (double)k[base + c];
     const int64_t e = edge, ee = (int64_t)edge * edge;
     const int64_t off[6] = {-1, 1, -e, e, -ee, ee};
     const bool on[6] = {x > 0, x < edge - 1, y > 0, y < edge - 1, z > 0, z < edge - 1};
+    const TT tc = t[base + c], kc = k[base + c];
+    TT tn[6], kn[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        const int64_t nb = base + (on[q] ? c + off[q] : c);
+        tn[q] = t[nb], kn[q] = k[nb];
+    }
     float f[16];
-    f[0] = (float)((tc - T_SHIFT) / T_SCALE);
-    f[1] = (float)(log(kc) / K_SCALE);
+    f[0] = (float)(tc - (TT)T_SHIFT) * (1.0f / T_SCALE);
+    const float kcf = (float)kc;
+    f[1] = __logf(kcf) * (1.0f / K_SCALE);
     float sum = 0.0f;
 #pragma unroll
     for (int q = 0; q < 6; q++) {
-        float af = 0.0f, fl = 0.0f;
-        if (on[q]) {
-            const double kn = (double)k[base + c + off[q]], tn = (double)t[base + c + off[q]];
-            const double kf = 2.0 * kc * kn / (kc + kn);
-            af = (float)(coupling * kf);
-            fl = (float)((double)coupling * kf * (tn - tc) * ((double)K_STEPS / (double)OUT_SCALE));
-        }
+        const float knf = (float)kn[q];
+        const float kf = __fdividef(2.0f * kcf * knf, kcf + knf);
+        const float af = on[q] ? coupling * kf : 0.0f;
+        const float fl = on[q] ? af * (float)(tn[q] - tc) * ((float)K_STEPS / OUT_SCALE) : 0.0f;
         f[2 + q] = 10.0f * af;
         f[8 + q] = fl;
         sum += fl;

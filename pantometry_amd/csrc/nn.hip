@@ -346,6 +346,9 @@ __global__ __launch_bounds__(256, NT == 1 ? 4 : 2) void conv_fwd2(Conv g, const 
             for (int rb = 0; rb < RB; rb++)
 #pragma unroll
                 for (int n = 0; n < NT; n++) acc[rb][n] = mfma32(av[n], bvv[rb], acc[rb][n]);
+            // The pair's LDS reads issue together, then its MFMAs: one exposed LDS latency a pair.
+            __builtin_amdgcn_sched_group_barrier(0x100, NT + RB, 0);
+            __builtin_amdgcn_sched_group_barrier(0x008, NT * RB, 0);
         }
     }
 

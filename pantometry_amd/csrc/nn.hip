@@ -645,6 +645,10 @@ static FwdPlan plan_fwd(const Conv &g0, bool can_split) {
         // With a short reduction (at most 64 input channels), 64-position boxes that fill the
         // chip without a split: no fp32 partials to write and reduce.  A long reduction keeps
         // the 256-position boxes' reuse and splits (32^3, 192 -> 64 channels: 41 against 63 µs).
+        const Conv f2 = forward_box(g0, 128);
+        if (g0.Cin <= 64 && f2.TW > 0 && f2.TH > 0 && f2.TD > 0 && f2.tile_positions() == 128 &&
+            f2.tiles() * (g0.Cout / (16 * nt_max)) >= 512)
+            return {2, nt_max, 1};
         const Conv f1 = forward_box(g0, 64);
         if (g0.Cin <= 64 && f1.TW > 0 && f1.TH > 0 && f1.TD > 0 &&
             f1.tiles() * (g0.Cout / (16 * nt_max)) >= 1024)
@@ -737,9 +741,14 @@ int conv_forward(const Conv &g0, const bf16 *x, const bf16 *w, const float *bias
         case 34: launch_fwd<3, 4>(g, grid, lds, stream, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part, pool, head, head_y); break;
         case 44: launch_fwd<4, 4>(g, grid, lds, stream, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part, pool, head, head_y); break;
         case 11: launch_fwd<1, 1>(g, grid, lds, stream, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part, pool, head, head_y); break;
+        case 12: launch_fwd<1, 2>(g, grid, lds, stream, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part, pool, head, head_y); break;
+        case 22: launch_fwd<2, 2>(g, grid, lds, stream, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part, pool, head, head_y); break;
         case 21: launch_fwd<2, 1>(g, grid, lds, stream, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part, pool, head, head_y); break;
         case 31: launch_fwd<3, 1>(g, grid, lds, stream, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part, pool, head, head_y); break;
-        default: launch_fwd<4, 1>(g, grid, lds, stream, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part, pool, head, head_y); break;
+        case 41: launch_fwd<4, 1>(g, grid, lds, stream, x, xs, cup, w, bias, z, a, gelu_out, ostride, ooff, part, pool, head, head_y); break;
+        default:
+            set_error("conv_forward: no kernel for %d N-tiles x %d row blocks", nt, rb);
+            return PM_ERR_ARGUMENT;
     }
     PM_HIP(hipGetLastError());
     if (part) {

@@ -1632,15 +1632,31 @@ __global__ __launch_bounds__(EXT_THREADS) void heat_step_ext(const double *__res
     for (int64_t c = blockIdx.x * EXT_THREADS + threadIdx.x; c < n; c += (int64_t)gridDim.x * EXT_THREADS) {
         int64_t i, j, k;
         coords_of(c, nx, ny, nz, i, j, k);
+        // Every lane loads all six neighbours (a face off the grid reads the cell itself) and a
+        // skipped face adds +0.0 by a select: the flux starts at +0.0 and a sum of it with
+        // anything is never -0.0, so adding +0.0 leaves it as skipping the face would.  The loads
+        // issue together instead of one face's after another's.
+        const bool on[6] = {i > 0, i + 1 < nx, j > 0, j + 1 < ny, k > 0, k + 1 < nz};
+        const int64_t nb[6] = {on[0] ? c - 1 : c, on[1] ? c + 1 : c, on[2] ? c - nx : c,
+                               on[3] ? c + nx : c, on[4] ? c - plane : c, on[5] ? c + plane : c};
         const double t = told[c];
-        const double kk = kc ? kc[c] : 0.0;
+        double tn[6], f[6];
+#pragma unroll
+        for (int q = 0; q < 6; q++) tn[q] = told[nb[q]];
+        if (kc) {
+            const double kk = kc[c];
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                const double kn = kc[nb[q]];
+                f[q] = (q & 1) ? series_nz(kk, kn) : series_nz(kn, kk);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 6; q++) f[q] = kface;
+        }
         double flux = 0.0;
-        if (i > 0) flux = flux + (kc ? series_nz(kc[c - 1], kk) : kface) * (told[c - 1] - t);
-        if (i + 1 < nx) flux = flux + (kc ? series_nz(kk, kc[c + 1]) : kface) * (told[c + 1] - t);
-        if (j > 0) flux = flux + (kc ? series_nz(kc[c - nx], kk) : kface) * (told[c - nx] - t);
-        if (j + 1 < ny) flux = flux + (kc ? series_nz(kk, kc[c + nx]) : kface) * (told[c + nx] - t);
-        if (k > 0) flux = flux + (kc ? series_nz(kc[c - plane], kk) : kface) * (told[c - plane] - t);
-        if (k + 1 < nz) flux = flux + (kc ? series_nz(kk, kc[c + plane]) : kface) * (told[c + plane] - t);
+#pragma unroll
+        for (int q = 0; q < 6; q++) flux = flux + (on[q] ? f[q] * (tn[q] - t) : 0.0);
         double sh = 0.0;
         if (x.shed && (((i == 0) && (x.faces & 1)) || ((i + 1 == nx) && (x.faces & 2)) || ((j == 0) && (x.faces & 4)) ||
                        ((j + 1 == ny) && (x.faces & 8)) || ((k == 0) && (x.faces & 16)) || ((k + 1 == nz) && (x.faces & 32))))

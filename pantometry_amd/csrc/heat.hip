@@ -1489,13 +1489,15 @@ __device__ __forceinline__ double face_sum_of(const double *kc, double kface, in
     const int64_t plane = nx * ny;
     double f0 = 0.0, f1 = 0.0, f2 = 0.0, f3 = 0.0, f4 = 0.0, f5 = 0.0;
     if (kc) {
+        // All six neighbours' conductivities load together (a face off the grid reads the cell
+        // itself) and a missing face takes 0 by a select.
+        const bool o0 = i > 0, o1 = i + 1 < nx, o2 = j > 0, o3 = j + 1 < ny, o4 = k > 0, o5 = k + 1 < nz;
         const double kk = kc[c];
-        if (i > 0) f0 = series_nz(kc[c - 1], kk);
-        if (i + 1 < nx) f1 = series_nz(kk, kc[c + 1]);
-        if (j > 0) f2 = series_nz(kc[c - nx], kk);
-        if (j + 1 < ny) f3 = series_nz(kk, kc[c + nx]);
-        if (k > 0) f4 = series_nz(kc[c - plane], kk);
-        if (k + 1 < nz) f5 = series_nz(kk, kc[c + plane]);
+        const double k0 = kc[o0 ? c - 1 : c], k1 = kc[o1 ? c + 1 : c], k2 = kc[o2 ? c - nx : c];
+        const double k3 = kc[o3 ? c + nx : c], k4 = kc[o4 ? c - plane : c], k5 = kc[o5 ? c + plane : c];
+        f0 = o0 ? series_nz(k0, kk) : 0.0, f1 = o1 ? series_nz(kk, k1) : 0.0;
+        f2 = o2 ? series_nz(k2, kk) : 0.0, f3 = o3 ? series_nz(kk, k3) : 0.0;
+        f4 = o4 ? series_nz(k4, kk) : 0.0, f5 = o5 ? series_nz(kk, k5) : 0.0;
     } else {
         f0 = i > 0 ? kface : 0.0, f1 = i + 1 < nx ? kface : 0.0;
         f2 = j > 0 ? kface : 0.0, f3 = j + 1 < ny ? kface : 0.0;

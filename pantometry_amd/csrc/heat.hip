@@ -1583,6 +1583,41 @@ __device__ __forceinline__ double film_flux_at(const FilmFaces &F, const double 
     return out;
 }
 
+// film_rates and film_shed in one pass over the listed cells, for the device-resident advance:
+// the rates' per-block maxima, and each cell's film flux from the step's input into `shed`
+// (skipped once the run has stopped).  The shed of a step the limit then refuses is never read:
+// the sweep of that step does not run.
+__global__ __launch_bounds__(256) void film_rates_shed(FilmFaces F, const int64_t *__restrict__ bcell,
+                                                       const double *__restrict__ bemis,
+                                                       const double *__restrict__ bk, int64_t nb,
+                                                       const double *__restrict__ T,
+                                                       const double *__restrict__ kc, double kface,
+                                                       const double *__restrict__ mob, double mob0,
+                                                       int64_t nx, int64_t ny, int64_t nz,
+                                                       double *__restrict__ part,
+                                                       double *__restrict__ shed, const AuditState *st) {
+    __shared__ double red[256];
+    const FilmList L{F, bcell, bemis, bk, nb, kc, kface, mob, mob0, nx, ny, nz};
+    const bool live = !(st && st->halted);
+    double worst = -INFINITY;
+    for (int64_t b = blockIdx.x * 256 + threadIdx.x; b < nb; b += (int64_t)gridDim.x * 256) {
+        worst = fmax(worst, film_rate(L, b, T));
+        if (live) {
+            const int64_t c = bcell[b];
+            int64_t i, j, k;
+            coords_of(c, nx, ny, nz, i, j, k);
+            shed[c] = film_flux_at(F, bemis, bk, b, T[c], i, j, k, nx, ny, nz);
+        }
+    }
+    red[threadIdx.x] = worst;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
 // What the sweep needs of a film (faces: bit f set for an exposed face f; shed: the listed cells'
 // film flux, computed by film_shed from the step's input; null without a film) and of the
 // sources (src_of: each cell's index in the generating cells, -1 for none; null without
@@ -2851,15 +2886,12 @@ static int advance_ext_device(pm_solid3d *s, double dt, uint64_t steps, double t
             gap_rates<<<gblocks, 256, 0, s->stream>>>(s->gap_a.as<int64_t>(), s->gap_b.as<int64_t>(),
                                                       s->gap_coef.d(), npairs, tin, s->cap.d(), s->gap_part.d());
         if (film)
-            film_rates<<<fblocks, 256, 0, s->stream>>>(faces, s->bcell.as<int64_t>(), s->bemis.d(),
-                                                       s->bkcond.d(), s->nboundary, tin, kc, s->kface,
-                                                       mob, s->mob0, s->nx, s->ny, s->nz, s->film_part.d());
+            film_rates_shed<<<fblocks, 256, 0, s->stream>>>(faces, s->bcell.as<int64_t>(), s->bemis.d(),
+                                                            s->bkcond.d(), s->nboundary, tin, kc, s->kface,
+                                                            mob, s->mob0, s->nx, s->ny, s->nz,
+                                                            s->film_part.d(), s->shed.d(), st);
         ext_limit<<<1, 256, 0, s->stream>>>(s->gap_part.d(), gblocks, s->film_part.d(), fblocks,
                                             s->worst_rate, dt, st, (int64_t)q);
-        if (xc.shed)
-            film_shed<<<fblocks, 256, 0, s->stream>>>(faces, s->bcell.as<int64_t>(), s->bemis.d(),
-                                                      s->bkcond.d(), s->nboundary, tin, s->shed.d(),
-                                                      s->nx, s->ny, s->nz, st);
         heat_step_ext<<<sweep_blocks, EXT_THREADS, 0, s->stream>>>(tin, tout, kc, s->kface, mob, s->mob0, cap,
                                                          s->cap0, xc, dt, s->nx, s->ny, s->nz,
                                                          s->ext_part.d(), ph, st);

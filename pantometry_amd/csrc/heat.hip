@@ -1397,7 +1397,7 @@ static int total_capacity(pm_solid3d *s, double *out) {
 // workgroups at 128^3 ran 25% slower).  Its per-workgroup loss partials are folded in two levels
 // (loss_fold).  The rate kernels of the film and the gaps use at most EXT_BLOCKS workgroups of
 // 256.
-constexpr int EXT_BLOCKS = 256, EXT_THREADS = 1024, EXT_SWEEP_MAX = 256, LOSS_CHUNK = 64;
+constexpr int EXT_BLOCKS = 1024, EXT_THREADS = 1024, EXT_SWEEP_MAX = 256, LOSS_CHUNK = 64;
 static int ext_sweep_blocks(int64_t n) {
     return (int)std::min<int64_t>(EXT_SWEEP_MAX, std::max<int64_t>(1, (n + EXT_THREADS - 1) / EXT_THREADS));
 }
@@ -1982,7 +1982,8 @@ static int ensure_tables(pm_solid3d *s) {
         PM_HIP(hipMemcpy(s->src_of.d(), map.data(), sizeof(int32_t) * map.size(), hipMemcpyHostToDevice));
         PM_HIP(hipMemcpy(s->src_w.d(), sw.data(), sizeof(double) * sw.size(), hipMemcpyHostToDevice));
     }
-    if (!s->ext_part) PM_CHECK(s->ext_part.alloc(sizeof(double) * EXT_SWEEP_MAX));
+    // The sweep's loss partials, or the film rates' of max_stable_dt.
+    if (!s->ext_part) PM_CHECK(s->ext_part.alloc(sizeof(double) * std::max(EXT_SWEEP_MAX, EXT_BLOCKS)));
     s->tables_valid = true;
     return PM_OK;
 }

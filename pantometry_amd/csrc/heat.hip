@@ -1392,14 +1392,22 @@ static int total_capacity(pm_solid3d *s, double *out) {
 }
 
 // The extended sweep's grid, a function of the block's size only, so its sums are reproducible:
-// one cell a thread in workgroups of 1024, at most 256 workgroups (larger blocks loop: the grid
-// then sweeps the block in contiguous strides, and the neighbours' planes stay in L2; 2048
-// workgroups at 128^3 ran 25% slower).  Its per-workgroup loss partials are folded in two levels
-// (loss_fold).  The rate kernels of the film and the gaps use at most EXT_BLOCKS workgroups of
-// 256.
+// 256 workgroups of 1024 threads marching 64 x 16 columns over chunks of planes when the block
+// has that many column tiles x planes (at 128^3 128 and 1024 workgroups of the earlier one-cell-a-
+// thread form ran 37% and 8% slower than 256), else one cell a thread in as many workgroups as
+// the cells need (at 24^3, 14 of them: 23.4 us a step against 26.7 with a workgroup a plane of
+// column tiles).  Its per-workgroup loss partials are folded in two levels (loss_fold).  The
+// rate kernels of the film and the gaps use at most EXT_BLOCKS workgroups of 256.
 constexpr int EXT_BLOCKS = 1024, EXT_THREADS = 1024, EXT_SWEEP_MAX = 256, LOSS_CHUNK = 64;
-static int ext_sweep_blocks(int64_t n) {
-    return (int)std::min<int64_t>(EXT_SWEEP_MAX, std::max<int64_t>(1, (n + EXT_THREADS - 1) / EXT_THREADS));
+struct ExtSweep {
+    int blocks;
+    bool march;
+};
+static ExtSweep ext_sweep_plan(int64_t nx, int64_t ny, int64_t nz) {
+    const int64_t items = ((nx + 63) / 64) * ((ny + 15) / 16) * nz;
+    if (items >= EXT_SWEEP_MAX) return {EXT_SWEEP_MAX, true};
+    const int64_t n = nx * ny * nz;
+    return {(int)std::min<int64_t>(EXT_SWEEP_MAX, std::max<int64_t>(1, (n + EXT_THREADS - 1) / EXT_THREADS)), false};
 }
 // The film's loss of a step from the sweep's partials: chunks of LOSS_CHUNK folded in index
 // order, then the chunks' sums folded in order.  The same on the host (step_ext) and the device
@@ -1653,6 +1661,39 @@ __global__ __launch_bounds__(256) void film_shed(FilmFaces F, const int64_t *__r
 // (solid.rs:1776-1818), then a generating cell's (W*dt)/C (solid.rs:1854-1864), as the
 // reference adds it after the sweep.  What the film removed, ((dt*mob)*shed)*C per cell
 // (solid.rs:1821-1823), is summed per block in a fixed order.
+// The rest of a cell's step once its flux is known: the shed of a cell on an exposed face
+// (`exposed`), rise = (dt*mob)*(flux - shed) through add_kelvin, the source after it, the store;
+// returns what the film removed, ((dt*mob)*shed)*C.
+__device__ __forceinline__ double ext_finish(const ExtCells &x, int64_t c, double t, double flux,
+                                             bool exposed, double dts, const double *__restrict__ mob,
+                                             double mob0, const double *__restrict__ cap, double cap0,
+                                             const PhaseArgs &ph, double *__restrict__ tnew) {
+    const double sh = x.shed && exposed ? x.shed[c] : 0.0;
+    const double dtm = dts * (mob ? mob[c] : mob0);
+    // The capacity is read only where it enters: a melting cell, the film's loss, a source.
+    double out = add_kelvin_lazy(ph, c, t, dtm * (flux - sh), cap, cap0);
+    const double lost = sh != 0.0 ? dtm * sh * (cap ? cap[c] : cap0) : 0.0;
+    if (x.src_of) {
+        const int32_t q = x.src_of[c];
+        if (q >= 0) {
+            const double cc = cap ? cap[c] : cap0;
+            out = add_kelvin(ph, c, out, (x.src_w[q] * dts) / cc, cc);
+        }
+    }
+    tnew[c] = out;
+    return lost;
+}
+
+// One step of a block with a film or sources.  Every lane loads all six neighbours (a face off
+// the grid reads the cell itself) and a skipped face adds +0.0 by a select: the flux starts at
+// +0.0 and a sum of it with anything is never -0.0, so adding +0.0 leaves it as skipping the
+// face would, and the loads issue together.  MARCH: work items of 64 x 16 columns by z-chunks
+// (workgroups loop over them in order); a thread marches its column up the chunk with T and k
+// of the planes below, at and above its cell in registers, and the face below a cell is the
+// face above the one under it, series_nz(k below, k) either way, so it is reused.  Otherwise one
+// cell a thread (small blocks, whose columns and planes give fewer workgroups than the chip
+// holds).
+template <bool MARCH>
 __global__ __launch_bounds__(EXT_THREADS) void heat_step_ext(const double *__restrict__ told,
                                                      double *__restrict__ tnew,
                                                      const double *__restrict__ kc, double kface,
@@ -1664,52 +1705,84 @@ __global__ __launch_bounds__(EXT_THREADS) void heat_step_ext(const double *__res
                                                      const AuditState *st) {
     __shared__ double red[EXT_THREADS / 64];
     if (st && st->halted) return;  // a device-resident run that stopped
-    const int64_t plane = nx * ny, n = plane * nz;
+    static_assert(EXT_THREADS == 1024, "64 x 16 columns a workgroup");
+    const int64_t plane = nx * ny;
+    const uint32_t fm = x.faces;
     double lost = 0.0;
-    for (int64_t c = blockIdx.x * EXT_THREADS + threadIdx.x; c < n; c += (int64_t)gridDim.x * EXT_THREADS) {
-        int64_t i, j, k;
-        coords_of(c, nx, ny, nz, i, j, k);
-        // Every lane loads all six neighbours (a face off the grid reads the cell itself) and a
-        // skipped face adds +0.0 by a select: the flux starts at +0.0 and a sum of it with
-        // anything is never -0.0, so adding +0.0 leaves it as skipping the face would.  The loads
-        // issue together instead of one face's after another's.
-        const bool on[6] = {i > 0, i + 1 < nx, j > 0, j + 1 < ny, k > 0, k + 1 < nz};
-        const int64_t nb[6] = {on[0] ? c - 1 : c, on[1] ? c + 1 : c, on[2] ? c - nx : c,
-                               on[3] ? c + nx : c, on[4] ? c - plane : c, on[5] ? c + plane : c};
-        const double t = told[c];
-        double tn[6], f[6];
-#pragma unroll
-        for (int q = 0; q < 6; q++) tn[q] = told[nb[q]];
-        if (kc) {
-            const double kk = kc[c];
-#pragma unroll
-            for (int q = 0; q < 6; q++) {
-                const double kn = kc[nb[q]];
-                f[q] = (q & 1) ? series_nz(kk, kn) : series_nz(kn, kk);
+    if constexpr (MARCH) {
+        const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+        const int64_t tnx = (nx + 63) / 64, tiles = tnx * ((ny + 15) / 16);
+        int64_t zch = std::max<int64_t>(1, std::min<int64_t>(nz, (int64_t)gridDim.x / tiles));
+        const int64_t zc = (nz + zch - 1) / zch;
+        zch = (nz + zc - 1) / zc;
+        for (int64_t it = blockIdx.x; it < tiles * zch; it += gridDim.x) {
+            const int64_t tile = it % tiles, z0 = (it / tiles) * zc, z1 = std::min(nz, z0 + zc);
+            const int64_t i = (tile % tnx) * 64 + tx, j = (tile / tnx) * 16 + ty;
+            if (i >= nx || j >= ny) continue;
+            const bool ox0 = i > 0, ox1 = i + 1 < nx, oy0 = j > 0, oy1 = j + 1 < ny;
+            const int64_t dxm = ox0 ? -1 : 0, dxp = ox1 ? 1 : 0, dym = oy0 ? -nx : 0, dyp = oy1 ? nx : 0;
+            const bool side = (!ox0 && (fm & 1)) || (!ox1 && (fm & 2)) || (!oy0 && (fm & 4)) || (!oy1 && (fm & 8));
+            int64_t c = i + nx * j + plane * z0;
+            const int64_t cb = z0 > 0 ? c - plane : c;
+            double tb = told[cb], t = told[c];
+            double kk = 0.0, fzm = kface;
+            if (kc) kk = kc[c], fzm = series_nz(kc[cb], kk);
+            for (int64_t k = z0; k < z1; k++, c += plane) {
+                const bool oz0 = k > 0, oz1 = k + 1 < nz;
+                const int64_t ca = oz1 ? c + plane : c;
+                const double ta = told[ca];
+                const double txm = told[c + dxm], txp = told[c + dxp], tym = told[c + dym], typ = told[c + dyp];
+                double f0 = kface, f1 = kface, f2 = kface, f3 = kface, f5 = kface, ka = 0.0;
+                if (kc) {
+                    ka = kc[ca];
+                    const double kxm = kc[c + dxm], kxp = kc[c + dxp], kym = kc[c + dym], kyp = kc[c + dyp];
+                    f0 = series_nz(kxm, kk), f1 = series_nz(kk, kxp);
+                    f2 = series_nz(kym, kk), f3 = series_nz(kk, kyp);
+                    f5 = series_nz(kk, ka);
+                }
+                double flux = 0.0;
+                flux = flux + (ox0 ? f0 * (txm - t) : 0.0);
+                flux = flux + (ox1 ? f1 * (txp - t) : 0.0);
+                flux = flux + (oy0 ? f2 * (tym - t) : 0.0);
+                flux = flux + (oy1 ? f3 * (typ - t) : 0.0);
+                flux = flux + (oz0 ? fzm * (tb - t) : 0.0);
+                flux = flux + (oz1 ? f5 * (ta - t) : 0.0);
+                const bool exposed = side || (!oz0 && (fm & 16)) || (!oz1 && (fm & 32));
+                lost += ext_finish(x, c, t, flux, exposed, dts, mob, mob0, cap, cap0, ph, tnew);
+                tb = t, t = ta, kk = ka, fzm = f5;
             }
-        } else {
-#pragma unroll
-            for (int q = 0; q < 6; q++) f[q] = kface;
         }
-        double flux = 0.0;
+    } else {
+        const int64_t n = plane * nz;
+        for (int64_t c = blockIdx.x * EXT_THREADS + threadIdx.x; c < n; c += (int64_t)gridDim.x * EXT_THREADS) {
+            int64_t i, j, k;
+            coords_of(c, nx, ny, nz, i, j, k);
+            const bool on[6] = {i > 0, i + 1 < nx, j > 0, j + 1 < ny, k > 0, k + 1 < nz};
+            const int64_t nb[6] = {on[0] ? c - 1 : c, on[1] ? c + 1 : c, on[2] ? c - nx : c,
+                                   on[3] ? c + nx : c, on[4] ? c - plane : c, on[5] ? c + plane : c};
+            const double t = told[c];
+            double tn[6], f[6];
 #pragma unroll
-        for (int q = 0; q < 6; q++) flux = flux + (on[q] ? f[q] * (tn[q] - t) : 0.0);
-        double sh = 0.0;
-        if (x.shed && (((i == 0) && (x.faces & 1)) || ((i + 1 == nx) && (x.faces & 2)) || ((j == 0) && (x.faces & 4)) ||
-                       ((j + 1 == ny) && (x.faces & 8)) || ((k == 0) && (x.faces & 16)) || ((k + 1 == nz) && (x.faces & 32))))
-            sh = x.shed[c];
-        const double dtm = dts * (mob ? mob[c] : mob0);
-        // The capacity is read only where it enters: a melting cell, the film's loss, a source.
-        double out = add_kelvin_lazy(ph, c, t, dtm * (flux - sh), cap, cap0);
-        if (sh != 0.0) lost += dtm * sh * (cap ? cap[c] : cap0);
-        if (x.src_of) {
-            const int32_t q = x.src_of[c];
-            if (q >= 0) {
-                const double cc = cap ? cap[c] : cap0;
-                out = add_kelvin(ph, c, out, (x.src_w[q] * dts) / cc, cc);
+            for (int q = 0; q < 6; q++) tn[q] = told[nb[q]];
+            if (kc) {
+                const double kk = kc[c];
+#pragma unroll
+                for (int q = 0; q < 6; q++) {
+                    const double kn = kc[nb[q]];
+                    f[q] = (q & 1) ? series_nz(kk, kn) : series_nz(kn, kk);
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < 6; q++) f[q] = kface;
             }
+            double flux = 0.0;
+#pragma unroll
+            for (int q = 0; q < 6; q++) flux = flux + (on[q] ? f[q] * (tn[q] - t) : 0.0);
+            bool exposed = false;
+#pragma unroll
+            for (int q = 0; q < 6; q++) exposed = exposed || (!on[q] && (fm & (1u << q)));
+            lost += ext_finish(x, c, t, flux, exposed, dts, mob, mob0, cap, cap0, ph, tnew);
         }
-        tnew[c] = out;
     }
     const double total = block_sum<EXT_THREADS>(lost, red);
     if (threadIdx.x == 0) lost_part[blockIdx.x] = total;
@@ -2221,8 +2294,9 @@ static int step_ext(pm_solid3d *s, double dt) {
                                                  nullptr);
         PM_HIP(hipGetLastError());
     }
-    const int sweep_blocks = ext_sweep_blocks(s->n());
-    heat_step_ext<<<sweep_blocks, EXT_THREADS, 0, s->stream>>>(s->T(), out, kc, s->kface, mob, s->mob0, cap,
+    const ExtSweep sw = ext_sweep_plan(s->nx, s->ny, s->nz);
+    const int sweep_blocks = sw.blocks;
+    (sw.march ? heat_step_ext<true> : heat_step_ext<false>)<<<sweep_blocks, EXT_THREADS, 0, s->stream>>>(s->T(), out, kc, s->kface, mob, s->mob0, cap,
                                                      s->cap0, xc, dt,
                                                      s->nx, s->ny, s->nz, s->ext_part.d(),
                                                      s->phase_args(), nullptr);
@@ -2865,7 +2939,8 @@ static int advance_ext_device(pm_solid3d *s, double dt, uint64_t steps, double t
     const PhaseArgs ph = s->phase_args();
     AuditState *st = w.audit;
     const ExtCells xc = ext_cells(s);
-    const int sweep_blocks = ext_sweep_blocks(n);
+    const ExtSweep sw = ext_sweep_plan(s->nx, s->ny, s->nz);
+    const int sweep_blocks = sw.blocks;
 
     PM_HIP(hipEventRecord(s->ev0, s->stream));
     constexpr uint64_t CHUNK = 64;  // steps whose supply is folded and sent ahead of their launches
@@ -2893,7 +2968,7 @@ static int advance_ext_device(pm_solid3d *s, double dt, uint64_t steps, double t
                                                             s->film_part.d(), s->shed.d(), st);
         ext_limit<<<1, 256, 0, s->stream>>>(s->gap_part.d(), gblocks, s->film_part.d(), fblocks,
                                             s->worst_rate, dt, st, (int64_t)q);
-        heat_step_ext<<<sweep_blocks, EXT_THREADS, 0, s->stream>>>(tin, tout, kc, s->kface, mob, s->mob0, cap,
+        (sw.march ? heat_step_ext<true> : heat_step_ext<false>)<<<sweep_blocks, EXT_THREADS, 0, s->stream>>>(tin, tout, kc, s->kface, mob, s->mob0, cap,
                                                          s->cap0, xc, dt, s->nx, s->ny, s->nz,
                                                          s->ext_part.d(), ph, st);
         if (s->ngcell)

@@ -118,6 +118,36 @@ def test_uniform_block_with_film_and_source(oracle):
     assert g.books()[2] == o.supplied
 
 
+@pytest.mark.parametrize("counts,uniform", [((70, 20, 130), False), ((67, 17, 70), True)], ids=str)
+def test_blocks_whose_sweep_marches_columns_match_the_oracle(oracle, counts, uniform):
+    """Blocks of at least 256 column tiles (64 x 16) x planes take the sweep that marches z
+    columns: 2 x 2 tiles x 130 planes with the module's per-cell conductivity, and 2 x 2 x 70
+    planes of one substance (the uniform face).  Steps one call at a time, then a
+    device-resident advance, every cell bit for bit."""
+    gpu_case(f"film sweep marching columns, {counts}, uniform {uniform}")
+    from pantometry_amd.core import Exchange
+    if uniform:
+        dx = 2e-3
+        faces = [(f, celsius(20.0), 25.0 * (f + 1), 1e-4 * (f + 1)) for f in range(6)]
+        g, o = pair(oracle, counts, PLA, dx, celsius(150.0), faces=faces,
+                    sources=[(2.0, box(counts, (1, 1, 1), (30, 9, 33)))])
+    else:
+        g, o = module(oracle, counts, dx=1e-3)
+    dt = 0.6 * o.max_stable_dt()
+    assert g.max_stable_dt() == o.max_stable_dt()
+    bus = Exchange()
+    for _ in range(3):
+        g.step(0.0, dt, bus)
+        o.step(dt)
+    assert np.array_equal(bits(g.cells()), bits(o.cells))
+    totals, _ = g.advance(dt, 8, tolerance=1e-9)
+    for _ in range(8):
+        o.step(dt)
+    assert np.array_equal(bits(g.cells()), bits(o.cells))
+    assert g.books()[2] == o.supplied
+    assert abs(g.books()[1] - o.lost) <= 1e-12 * abs(o.lost)
+
+
 def test_advance_matches_and_audits(oracle):
     gpu_case("film advance")
     g, o = module(oracle)

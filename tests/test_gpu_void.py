@@ -46,7 +46,8 @@ def mask_of(counts, pred):
     return m
 
 
-@pytest.mark.parametrize("counts", [(7, 6, 5), (33, 9, 12)], ids=str)
+# 70 x 20 x 70: at least 256 column tiles (64 x 16) x planes, so the sweep marches z columns.
+@pytest.mark.parametrize("counts", [(7, 6, 5), (33, 9, 12), (70, 20, 70)], ids=str)
 def test_a_block_with_void_gaps_films_and_sources_is_bit_exact(counts):
     gpu_case(f"void parity {counts}")
     import pantometry_amd as pm
